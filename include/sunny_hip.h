@@ -1,0 +1,137 @@
+/*
+ * libsunnyhip — MI355X (gfx950) spin-dynamics hot path of Sunny.jl behind a
+ * plain C ABI (no torch or HIP types in any signature).
+ *
+ * The reference (Sunny.jl 0.9.2) has no FFI layer: its extension point is
+ * Julia multiple dispatch on `step!` (src/Integrators.jl:309).  Each entry
+ * point below replaces one reference function for a device-resident copy of
+ * a `System` (or a batch of replicas of it); the Julia-side binding is a
+ * `ccall` per entry point (see the `SunnyHIP` module shipped beside the
+ * library).
+ *
+ * Conventions
+ *  - Return value: SNH_OK (0) or an SNH_ERR_* code; never throws.  The text
+ *    of the last failure is available from snh_last_error().
+ *  - Host arrays use Julia's column-major memory order for
+ *    Array{Vec3,4}(L1,L2,L3,na): site = c1 + L1*(c2 + L2*(c3 + L3*i)),
+ *    0-based, xyz contiguous per site (src/System/Types.jl:131-136).
+ *    Batches concatenate replicas in order.  Coherent states are N
+ *    consecutive (re, im) pairs per site.
+ *  - Host pointers are borrowed for the duration of the call only.
+ *  - One handle = one HIP stream; a handle is not internally synchronised
+ *    (the same contract as a Sunny `System`, src/System/System.jl:138-174).
+ */
+#ifndef SUNNY_HIP_H
+#define SUNNY_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SNH_OK 0
+#define SNH_ERR_INVALID 1      /* invalid argument or state */
+#define SNH_ERR_HIP 2          /* HIP runtime failure (no device, out of memory, ...) */
+#define SNH_ERR_NOCONVERGE 3   /* implicit midpoint did not converge within max_iters */
+#define SNH_ERR_UNSUPPORTED 4  /* configuration outside what this build supports */
+
+typedef struct snh_system snh_system;
+
+/* Description of a homogeneous System (and of R replicas of it).
+ * Mirrors the data Sunny keeps in System{N} (src/System/Types.jl:106-140):
+ * per-sublattice kappa / g / field, per-sublattice onsite couplings, and the
+ * pair couplings of every sublattice (src/System/Types.jl:46-74). */
+typedef struct {
+    int32_t na;                 /* atoms per chemical cell (Bravais sublattices) */
+    int32_t N;                  /* 0 = :dipole; 2..7 = :SUN with N = 2s+1 */
+    int32_t precision;          /* 64 (plain FP64) or 32 (FP32 kernels, FP64 norms/residual/energy) */
+    const double *kappa;        /* [na] */
+    const double *g;            /* [na][3][3] row-major */
+    const double *field;        /* [na][3] external field (energy units) */
+    const double *stevens;      /* [na][28] c0|c2[5]|c4[9]|c6[13], q descending; :dipole only */
+    const int32_t *kmax;        /* [na] 0, 2, 4 or 6; :dipole only */
+    const double *Lambda;       /* [na][N][N][2] onsite matrix, row-major (re, im); :SUN only */
+    int32_t nbonds;             /* directed bonds, both directions of every pair */
+    const int32_t *bond;        /* [nbonds][6] = i, j, n1, n2, n3, isculled (0-based atoms) */
+    const double *bilin;        /* [nbonds][9] row-major J of bond i -> j (reverse bond holds J') */
+    const double *biquad;       /* [nbonds][25] row-major coupling of Stevens quadrupoles */
+    const int32_t *has_biquad;  /* [nbonds] */
+    const double *scalar;       /* [nbonds] constant shift */
+    int32_t ewald;              /* 1: long-range dipole-dipole (enable_dipole_dipole!) */
+    double mu0_muB2;            /* strength, e.g. 0.6745817653324668 meV*A^3 (src/Units.jl:91) */
+    double demag[9];            /* demagnetization tensor, row-major */
+    double latvecs[9];          /* lattice vectors as columns, row-major */
+    const double *positions;    /* [na][3] fractional positions (Ewald only) */
+    int32_t nreplicas;          /* R >= 1 */
+    const int32_t *dims;        /* [R][3] supercell dims per replica (may differ without Ewald) */
+} snh_spec;
+
+/* Builds the device copy: replaces System construction plus the couplings set
+ * by set_exchange!/set_onsite_coupling!/enable_dipole_dipole!
+ * (src/System/System.jl:33-93, src/System/Interactions.jl:182-189).  With
+ * Ewald, the real-space tensor A and its transform are computed on the GPU
+ * (src/System/Ewald.jl:2-22, 61-152). */
+int snh_create(const snh_spec *spec, snh_system **out);
+int snh_destroy(snh_system *sys);
+
+/* Text of the most recent failure on this handle (or of the last failed
+ * snh_create when sys is NULL).  Valid until the next call on the handle. */
+const char *snh_last_error(const snh_system *sys);
+
+/* Total sites over all replicas, and sites of replica r. */
+int snh_num_sites(const snh_system *sys, int64_t *total);
+int snh_replica_sites(const snh_system *sys, int32_t r, int64_t *nsites);
+
+/* sys.dipoles / sys.coherents of all replicas (src/System/Types.jl:133-134).
+ * Setting coherents also sets dipoles = expected_spin(Z). */
+int snh_set_dipoles(snh_system *sys, const double *dipoles);
+int snh_get_dipoles(snh_system *sys, double *dipoles);
+int snh_set_coherents(snh_system *sys, const double *coherents);
+int snh_get_coherents(snh_system *sys, double *coherents);
+
+/* step!(sys, Langevin(dt; damping, kT)) repeated nsteps times
+ * (src/Integrators.jl:313-357).  kT: [R] per-replica temperatures.
+ * noise: NULL for device Philox4x32-10 noise keyed by (seed, replica) and
+ * counted by (site, step0 + k); otherwise the raw draws of randn!(rng, xi)
+ * for ONE step (nsteps must be 1), laid out like the state:
+ * 3 standard normals per site (:dipole) or N complex normals with variance
+ * 1/2 per component (:SUN).  Scaling by sqrt(2 dt damping kT) happens here,
+ * as in fill_noise! (src/Integrators.jl:288-298). */
+int snh_step_langevin(snh_system *sys, double dt, double damping, const double *kT, const double *noise,
+                      uint64_t seed, uint64_t step0, int32_t nsteps);
+
+/* step!(sys, ImplicitMidpoint(dt; damping, tol); max_iters) repeated nsteps
+ * times (src/Integrators.jl:379-452).  iters_out: NULL or [R], the
+ * fixed-point iterations of the last step.  A replica that does not converge
+ * leaves its state at the last completed step and the call returns
+ * SNH_ERR_NOCONVERGE with the reference's message. */
+int snh_step_midpoint(snh_system *sys, double dt, double damping, double tol, int32_t max_iters, int32_t nsteps,
+                      int32_t *iters_out);
+
+/* energy(sys) for every replica, FP64 sum (src/System/Interactions.jl:401-508,
+ * src/System/Ewald.jl:155-181).  E: [R]. */
+int snh_energy(snh_system *sys, double *E);
+
+/* set_energy_grad_dipoles! (:dipole; out [sites][3]) or
+ * set_energy_grad_coherents! (:SUN; out [sites][N][2]) at the current state
+ * (src/System/Interactions.jl:516-669). */
+int snh_energy_grad(snh_system *sys, double *out);
+
+/* Blocks until all work queued on the handle's stream has finished. */
+int snh_synchronize(snh_system *sys);
+
+/* Diagnostics with no counterpart in the reference: per-kernel timing by HIP
+ * events recorded on the handle's stream around every launch of a kernel
+ * family ("dipole_predict", "dipole_correct", "sun_predict", "sun_correct",
+ * "midpoint_iter", "ewald_field").  enable = 0 stops recording and clears
+ * the totals.  snh_kernel_time synchronises and reports the summed
+ * milliseconds and the number of launches recorded since enabling. */
+int snh_enable_timing(snh_system *sys, int32_t enable);
+int snh_kernel_time(snh_system *sys, const char *family, double *ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SUNNY_HIP_H */

@@ -1,0 +1,1069 @@
+/*
+ * FP64 CPU oracle for Sunny.jl's spin-dynamics hot path.  TEST INFRASTRUCTURE
+ * ONLY: it is the checker, never the product.  Each function cites the
+ * reference lines it restates (paths relative to the Sunny.jl 0.9.2 tree).
+ */
+#include "sunny_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef double complex cplx;
+
+/* ------------------------------------------------------------------------ */
+/* Indexing: System.jl:204-212 (bonded_site with periodic wrap), Types.jl:131 */
+
+static inline long site_index(const int L[3], int c1, int c2, int c3, int i)
+{
+    return (long)c1 + (long)L[0] * ((long)c2 + (long)L[1] * ((long)c3 + (long)L[2] * i));
+}
+
+static inline int wrap(int x, int L)
+{
+    int r = x % L;
+    return r < 0 ? r + L : r;
+}
+
+static inline long ncells(const so_spec *sp) { return (long)sp->L[0] * sp->L[1] * sp->L[2]; }
+static inline long nsites(const so_spec *sp) { return ncells(sp) * sp->na; }
+
+/* ------------------------------------------------------------------------ */
+/* Stevens anisotropy, OnsiteCoupling.jl:316-425.  c = c0 | c2[5] | c4[9] | c6[13],
+   each block in descending q = k..-k.  1-based Julia indices ck[m] map to
+   ck[m-1] here.                                                              */
+
+void so_stevens_energy_grad(const double S[3], const double c[28], int kmax,
+                            double *Eout, double grad[3])
+{
+    const double *c2 = c + 1, *c4 = c + 6, *c6 = c + 15;
+    double E = c[0];
+    double dE_dz = 0.0;
+    cplx dE_dJp = 0.0;
+
+    if (kmax == 0) goto done;
+
+    double X = S[0] * S[0] + S[1] * S[1] + S[2] * S[2];
+    cplx Jp1 = S[0] + I * S[1];
+    double Jz1 = S[2];
+    cplx Jp2 = Jp1 * Jp1;
+    double Jz2 = Jz1 * Jz1;
+    {
+        double A1 = 3 * Jz2 - X, A2 = Jz1, A3 = 1;
+        double dA1 = 6 * Jz1, dA2 = 1;
+        E += (c2[0] * creal(Jp2) + c2[4] * cimag(Jp2)) * A3 +
+             (c2[1] * creal(Jp1) + c2[3] * cimag(Jp1)) * A2 + c2[2] * A1;
+        dE_dz += (c2[1] * creal(Jp1) + c2[3] * cimag(Jp1)) * dA2 + c2[2] * dA1;
+        dE_dJp += (2.0 / 2) * (c2[0] * Jp1 - I * c2[4] * Jp1) * A3 +
+                  (1.0 / 2) * (c2[1] - I * c2[3]) * A2;
+    }
+    if (kmax == 2) goto done;
+
+    double X2 = X * X;
+    cplx Jp3 = Jp2 * Jp1;
+    double Jz3 = Jz2 * Jz1;
+    cplx Jp4 = Jp2 * Jp2;
+    double Jz4 = Jz2 * Jz2;
+    {
+        double A1 = 35 * Jz4 - (30 * X) * Jz2 + (3 * X2);
+        double A2 = 7 * Jz3 - (3 * X) * Jz1;
+        double A3 = 7 * Jz2 - X;
+        double A4 = Jz1;
+        double A5 = 1;
+        double dA1 = 140 * Jz3 - (60 * X) * Jz1;
+        double dA2 = 21 * Jz2 - 3 * X;
+        double dA3 = 14 * Jz1;
+        double dA4 = 1;
+        E += (c4[0] * creal(Jp4) + c4[8] * cimag(Jp4)) * A5 +
+             (c4[1] * creal(Jp3) + c4[7] * cimag(Jp3)) * A4 +
+             (c4[2] * creal(Jp2) + c4[6] * cimag(Jp2)) * A3 +
+             (c4[3] * creal(Jp1) + c4[5] * cimag(Jp1)) * A2 + c4[4] * A1;
+        dE_dz += (c4[1] * creal(Jp3) + c4[7] * cimag(Jp3)) * dA4 +
+                 (c4[2] * creal(Jp2) + c4[6] * cimag(Jp2)) * dA3 +
+                 (c4[3] * creal(Jp1) + c4[5] * cimag(Jp1)) * dA2 + c4[4] * dA1;
+        dE_dJp += (4.0 / 2) * (c4[0] * Jp3 - I * c4[8] * Jp3) * A5 +
+                  (3.0 / 2) * (c4[1] * Jp2 - I * c4[7] * Jp2) * A4 +
+                  (2.0 / 2) * (c4[2] * Jp1 - I * c4[6] * Jp1) * A3 +
+                  (1.0 / 2) * (c4[3] - I * c4[5]) * A2;
+    }
+    if (kmax == 4) goto done;
+
+    double X3 = X2 * X;
+    cplx Jp5 = Jp4 * Jp1;
+    double Jz5 = Jz4 * Jz1;
+    cplx Jp6 = Jp3 * Jp3;
+    double Jz6 = Jz3 * Jz3;
+    {
+        double A1 = 231 * Jz6 - (315 * X) * Jz4 + (105 * X2) * Jz2 - (5 * X3);
+        double A2 = 33 * Jz5 - (30 * X) * Jz3 + (5 * X2) * Jz1;
+        double A3 = 33 * Jz4 - (18 * X) * Jz2 + X2;
+        double A4 = 11 * Jz3 - (3 * X) * Jz1;
+        double A5 = 11 * Jz2 - X;
+        double A6 = Jz1;
+        double A7 = 1;
+        double dA1 = 1386 * Jz5 - (1260 * X) * Jz3 + (210 * X2) * Jz1;
+        double dA2 = 165 * Jz4 - (90 * X) * Jz2 + 5 * X2;
+        double dA3 = 132 * Jz3 - (36 * X) * Jz1;
+        double dA4 = 33 * Jz2 - 3 * X;
+        double dA5 = 22 * Jz1;
+        double dA6 = 1;
+        E += (c6[0] * creal(Jp6) + c6[12] * cimag(Jp6)) * A7 +
+             (c6[1] * creal(Jp5) + c6[11] * cimag(Jp5)) * A6 +
+             (c6[2] * creal(Jp4) + c6[10] * cimag(Jp4)) * A5 +
+             (c6[3] * creal(Jp3) + c6[9] * cimag(Jp3)) * A4 +
+             (c6[4] * creal(Jp2) + c6[8] * cimag(Jp2)) * A3 +
+             (c6[5] * creal(Jp1) + c6[7] * cimag(Jp1)) * A2 + c6[6] * A1;
+        dE_dz += (c6[1] * creal(Jp5) + c6[11] * cimag(Jp5)) * dA6 +
+                 (c6[2] * creal(Jp4) + c6[10] * cimag(Jp4)) * dA5 +
+                 (c6[3] * creal(Jp3) + c6[9] * cimag(Jp3)) * dA4 +
+                 (c6[4] * creal(Jp2) + c6[8] * cimag(Jp2)) * dA3 +
+                 (c6[5] * creal(Jp1) + c6[7] * cimag(Jp1)) * dA2 + c6[6] * dA1;
+        dE_dJp += (6.0 / 2) * (c6[0] * Jp5 - I * c6[12] * Jp5) * A7 +
+                  (5.0 / 2) * (c6[1] * Jp4 - I * c6[11] * Jp4) * A6 +
+                  (4.0 / 2) * (c6[2] * Jp3 - I * c6[10] * Jp3) * A5 +
+                  (3.0 / 2) * (c6[3] * Jp2 - I * c6[9] * Jp2) * A4 +
+                  (2.0 / 2) * (c6[4] * Jp1 - I * c6[8] * Jp1) * A3 +
+                  (1.0 / 2) * (c6[5] - I * c6[7]) * A2;
+    }
+
+done:
+    *Eout = E;
+    grad[0] = +2 * creal(dE_dJp);
+    grad[1] = -2 * cimag(dE_dJp);
+    grad[2] = dE_dz;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Spin matrices, Spin.jl:1-14; Stevens quadrupoles from Stevens.jl:12-86.  */
+
+#define NMAX 16
+
+void so_spin_matrices(int N, double *Sx, double *Sy, double *Sz)
+{
+    memset(Sx, 0, sizeof(double) * 2 * N * N);
+    memset(Sy, 0, sizeof(double) * 2 * N * N);
+    memset(Sz, 0, sizeof(double) * 2 * N * N);
+    double s = (N - 1) / 2.0;
+    for (int j = 1; j <= N - 1; j++) {
+        double off = sqrt(2 * (s + 1) * j - j * (j + 1)) / 2;
+        /* element (j-1, j) and (j, j-1), 0-based */
+        int a = j - 1, b = j;
+        Sx[2 * (a * N + b)] = off;
+        Sx[2 * (b * N + a)] = off;
+        Sy[2 * (a * N + b) + 1] = -off; /* -i*off above the diagonal */
+        Sy[2 * (b * N + a) + 1] = +off;
+    }
+    for (int k = 0; k < N; k++) Sz[2 * (k * N + k)] = s - k;
+}
+
+static void cmat_mul(int N, const cplx *A, const cplx *B, cplx *C)
+{
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            cplx acc = 0;
+            for (int k = 0; k < N; k++) acc += A[i * N + k] * B[k * N + j];
+            C[i * N + j] = acc;
+        }
+}
+
+void so_stevens_quadrupoles(int N, double *Qout)
+{
+    double sx[2 * NMAX * NMAX], sy[2 * NMAX * NMAX], sz[2 * NMAX * NMAX];
+    so_spin_matrices(N, sx, sy, sz);
+    cplx Jx[NMAX * NMAX], Jy[NMAX * NMAX], Jz[NMAX * NMAX];
+    for (int k = 0; k < N * N; k++) {
+        Jx[k] = sx[2 * k] + I * sx[2 * k + 1];
+        Jy[k] = sy[2 * k] + I * sy[2 * k + 1];
+        Jz[k] = sz[2 * k] + I * sz[2 * k + 1];
+    }
+    cplx XX[NMAX * NMAX], YY[NMAX * NMAX], ZZ[NMAX * NMAX], XZ[NMAX * NMAX], ZX[NMAX * NMAX],
+        YZ[NMAX * NMAX], ZY[NMAX * NMAX], XY[NMAX * NMAX], YX[NMAX * NMAX];
+    cmat_mul(N, Jx, Jx, XX);
+    cmat_mul(N, Jy, Jy, YY);
+    cmat_mul(N, Jz, Jz, ZZ);
+    cmat_mul(N, Jx, Jz, XZ);
+    cmat_mul(N, Jz, Jx, ZX);
+    cmat_mul(N, Jy, Jz, YZ);
+    cmat_mul(N, Jz, Jy, ZY);
+    cmat_mul(N, Jx, Jy, XY);
+    cmat_mul(N, Jy, Jx, YX);
+    /* k = 2: O22 = (Jp^2+Jm^2)/2 = Jx^2-Jy^2; O21 = {Jx,Jz}/2; O20 = 3Jz^2 - X;
+       O2,-1 = {Jy,Jz}/2; O2,-2 = (Jp^2-Jm^2)/(2i) = {Jx,Jy}. */
+    for (int k = 0; k < N * N; k++) {
+        cplx q[5];
+        q[0] = XX[k] - YY[k];
+        q[1] = (XZ[k] + ZX[k]) / 2;
+        q[2] = 3 * ZZ[k] - (XX[k] + YY[k] + ZZ[k]);
+        q[3] = (YZ[k] + ZY[k]) / 2;
+        q[4] = XY[k] + YX[k];
+        for (int a = 0; a < 5; a++) {
+            Qout[2 * (a * N * N + k)] = creal(q[a]);
+            Qout[2 * (a * N * N + k) + 1] = cimag(q[a]);
+        }
+    }
+}
+
+/* quadrupole / grad_quadrupole of a classical dipole, Spin.jl:48-69 */
+static void quadrupole(const double *S, double Q[5])
+{
+    double x = S[0], y = S[1], z = S[2];
+    Q[0] = x * x - y * y;
+    Q[1] = z * x;
+    Q[2] = -x * x - y * y + 2 * z * z;
+    Q[3] = z * y;
+    Q[4] = 2 * y * x;
+}
+
+static void grad_quadrupole(const double *S, double G[5][3])
+{
+    double x = S[0], y = S[1], z = S[2];
+    G[0][0] = 2 * x;  G[0][1] = -2 * y; G[0][2] = 0;
+    G[1][0] = z;      G[1][1] = 0;      G[1][2] = x;
+    G[2][0] = -2 * x; G[2][1] = -2 * y; G[2][2] = 4 * z;
+    G[3][0] = 0;      G[3][1] = z;      G[3][2] = y;
+    G[4][0] = 2 * y;  G[4][1] = 2 * x;  G[4][2] = 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SU(N) helpers: Spin.jl:72-104, 131-167.                                   */
+
+typedef struct {
+    int N;
+    double off[NMAX];   /* Sx superdiagonal */
+    double m[NMAX];     /* Sz diagonal */
+    cplx S[3][NMAX * NMAX];
+    cplx Q[5][NMAX * NMAX];
+} sun_ops;
+
+static void sun_ops_init(sun_ops *o, int N)
+{
+    o->N = N;
+    double s = (N - 1) / 2.0;
+    for (int j = 1; j <= N - 1; j++) o->off[j - 1] = sqrt(2 * (s + 1) * j - j * (j + 1)) / 2;
+    for (int k = 0; k < N; k++) o->m[k] = s - k;
+    double sx[2 * NMAX * NMAX], sy[2 * NMAX * NMAX], sz[2 * NMAX * NMAX], q[5 * 2 * NMAX * NMAX];
+    so_spin_matrices(N, sx, sy, sz);
+    so_stevens_quadrupoles(N, q);
+    for (int k = 0; k < N * N; k++) {
+        o->S[0][k] = sx[2 * k] + I * sx[2 * k + 1];
+        o->S[1][k] = sy[2 * k] + I * sy[2 * k + 1];
+        o->S[2][k] = sz[2 * k] + I * sz[2 * k + 1];
+        for (int a = 0; a < 5; a++) o->Q[a][k] = q[2 * (a * N * N + k)] + I * q[2 * (a * N * N + k) + 1];
+    }
+}
+
+static void expected_spin1(const sun_ops *o, const cplx *Z, double S[3])
+{
+    int N = o->N;
+    cplx c = 0;
+    for (int k = 0; k < N - 1; k++) c += conj(Z[k]) * (o->off[k] * Z[k + 1]);
+    double nz = 0;
+    for (int k = 0; k < N; k++) nz += creal(conj(Z[k]) * (o->m[k] * Z[k]));
+    S[0] = 2 * creal(c);
+    S[1] = 2 * cimag(c);
+    S[2] = nz;
+}
+
+static void expected_quadrupole1(const sun_ops *o, const cplx *Z, double Q[5])
+{
+    int N = o->N;
+    for (int a = 0; a < 5; a++) {
+        cplx acc = 0;
+        for (int j = 0; j < N; j++)
+            for (int i = 0; i < N; i++) acc += conj(Z[i]) * o->Q[a][i * N + j] * Z[j];
+        Q[a] = creal(acc);
+    }
+}
+
+void so_expected_spin(int N, long ns, const double *Zr, double *S)
+{
+    sun_ops o;
+    sun_ops_init(&o, N);
+    const cplx *Z = (const cplx *)Zr;
+    for (long s = 0; s < ns; s++) expected_spin1(&o, Z + s * N, S + 3 * s);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Ewald helpers (Ewald.jl:185-209): a separable complex DFT over cells.     */
+
+/* In-place DFT of a complex field f[c3][c2][c1] along each axis.
+   sign = -1 forward (FFTW convention), +1 backward (no scaling). */
+static void dft3(cplx *f, const int L[3], int sign)
+{
+    int maxL = L[0];
+    if (L[1] > maxL) maxL = L[1];
+    if (L[2] > maxL) maxL = L[2];
+    cplx *line = malloc(sizeof(cplx) * maxL);
+    cplx *out = malloc(sizeof(cplx) * maxL);
+    long stride[3] = {1, L[0], (long)L[0] * L[1]};
+    for (int ax = 0; ax < 3; ax++) {
+        int n = L[ax];
+        cplx *tw = malloc(sizeof(cplx) * n);
+        for (int k = 0; k < n; k++) tw[k] = cexp(sign * 2 * M_PI * I * k / n);
+        int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+        for (int a = 0; a < L[o1]; a++)
+            for (int b = 0; b < L[o2]; b++) {
+                long base = a * stride[o1] + b * stride[o2];
+                for (int x = 0; x < n; x++) line[x] = f[base + x * stride[ax]];
+                for (int k = 0; k < n; k++) {
+                    cplx acc = 0;
+                    for (int x = 0; x < n; x++) acc += line[x] * tw[(long)k * x % n];
+                    out[k] = acc;
+                }
+                for (int x = 0; x < n; x++) f[base + x * stride[ax]] = out[x];
+            }
+        free(tw);
+    }
+    free(line);
+    free(out);
+}
+
+/* F[A] for all (alpha, beta, i, j): FA[j][i][a][b][k3][k2][k1] complex, full spectrum. */
+static cplx *ewald_FA(const so_spec *sp)
+{
+    long nc = ncells(sp);
+    int na = sp->na;
+    cplx *FA = malloc(sizeof(cplx) * nc * 9 * na * na);
+    for (int j = 0; j < na; j++)
+        for (int i = 0; i < na; i++)
+            for (int ab = 0; ab < 9; ab++) {
+                cplx *f = FA + (((long)j * na + i) * 9 + ab) * nc;
+                const double *A = sp->ewald_A + ((long)j * na + i) * nc * 9;
+                for (long c = 0; c < nc; c++) f[c] = A[c * 9 + ab];
+                dft3(f, sp->L, -1);
+            }
+    return FA;
+}
+
+/* F[mu] with mu = -g S: Fmu[j][b][k3][k2][k1] */
+static cplx *ewald_Fmu(const so_spec *sp, const double *S)
+{
+    long nc = ncells(sp);
+    int na = sp->na;
+    cplx *Fmu = malloc(sizeof(cplx) * nc * 3 * na);
+    for (int j = 0; j < na; j++) {
+        const double *g = sp->g + 9 * j;
+        for (int b = 0; b < 3; b++) {
+            cplx *f = Fmu + ((long)j * 3 + b) * nc;
+            for (long c = 0; c < nc; c++) {
+                const double *s = S + 3 * (c + nc * j);
+                f[c] = -(g[3 * b + 0] * s[0] + g[3 * b + 1] * s[1] + g[3 * b + 2] * s[2]);
+            }
+            dft3(f, sp->L, -1);
+        }
+    }
+    return Fmu;
+}
+
+/* accum_ewald_grad!, Ewald.jl:185-209 */
+static void ewald_accum_grad(const so_spec *sp, const double *S, double *dE)
+{
+    long nc = ncells(sp);
+    int na = sp->na;
+    cplx *FA = ewald_FA(sp);
+    cplx *Fmu = ewald_Fmu(sp, S);
+    cplx *Fphi = calloc(nc * 3 * na, sizeof(cplx));
+    /* Fphi[a,m,i] += conj(FA[a,b,m,i,j]) * Fmu[b,m,j] (loop order of :198) */
+    for (int j = 0; j < na; j++)
+        for (int i = 0; i < na; i++)
+            for (long m = 0; m < nc; m++)
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++)
+                        Fphi[((long)i * 3 + a) * nc + m] +=
+                            conj(FA[(((long)j * na + i) * 9 + 3 * a + b) * nc + m]) *
+                            Fmu[((long)j * 3 + b) * nc + m];
+    for (int i = 0; i < na; i++)
+        for (int a = 0; a < 3; a++) dft3(Fphi + ((long)i * 3 + a) * nc, sp->L, +1);
+    /* dE -= g' phi, with phi = real(ifft)/Ncells */
+    for (int i = 0; i < na; i++) {
+        const double *g = sp->g + 9 * i;
+        for (long c = 0; c < nc; c++) {
+            double phi[3];
+            for (int a = 0; a < 3; a++) phi[a] = creal(Fphi[((long)i * 3 + a) * nc + c]) / nc;
+            double *d = dE + 3 * (c + nc * i);
+            for (int b = 0; b < 3; b++)
+                d[b] -= g[0 * 3 + b] * phi[0] + g[1 * 3 + b] * phi[1] + g[2 * 3 + b] * phi[2];
+        }
+    }
+    free(FA);
+    free(Fmu);
+    free(Fphi);
+}
+
+/* ewald_energy, Ewald.jl:155-181: half spectrum m1 in 0..L1/2 with sqrt(2)
+   weights on the planes whose conjugate partners are missing. */
+double so_ewald_energy(const so_spec *sp, const double *S)
+{
+    long nc = ncells(sp);
+    int na = sp->na;
+    int L1 = sp->L[0], L2 = sp->L[1], L3 = sp->L[2];
+    int m1max = L1 / 2; /* rfft keeps m1 = 0..floor(L1/2) */
+    int even = (L1 % 2 == 0);
+    cplx *FA = ewald_FA(sp);
+    cplx *Fmu = ewald_Fmu(sp, S);
+    /* scale Fmu on m1 in [1, end-1] (even) or [1, end] (odd) by sqrt(2) */
+    for (int j = 0; j < na; j++)
+        for (int b = 0; b < 3; b++)
+            for (int m3 = 0; m3 < L3; m3++)
+                for (int m2 = 0; m2 < L2; m2++)
+                    for (int m1 = 1; m1 <= m1max; m1++) {
+                        if (even && m1 == m1max) continue;
+                        Fmu[((long)j * 3 + b) * nc + m1 + (long)L1 * (m2 + (long)L2 * m3)] *= sqrt(2.0);
+                    }
+    double E = 0;
+    for (int j = 0; j < na; j++)
+        for (int i = 0; i < na; i++)
+            for (int m3 = 0; m3 < L3; m3++)
+                for (int m2 = 0; m2 < L2; m2++)
+                    for (int m1 = 0; m1 <= m1max; m1++) {
+                        long m = m1 + (long)L1 * (m2 + (long)L2 * m3);
+                        for (int a = 0; a < 3; a++)
+                            for (int b = 0; b < 3; b++)
+                                E += 0.5 * creal(conj(Fmu[((long)i * 3 + a) * nc + m]) *
+                                                 conj(FA[(((long)j * na + i) * 9 + 3 * a + b) * nc + m]) *
+                                                 Fmu[((long)j * 3 + b) * nc + m]);
+                    }
+    free(FA);
+    free(Fmu);
+    return E / nc;
+}
+
+/* ewald_grad_at / ewald_pairwise_grad_at, Ewald.jl:211-230 */
+void so_ewald_grad_bruteforce(const so_spec *sp, const double *S, double *dE)
+{
+    long nc = ncells(sp);
+    int na = sp->na;
+    const int *L = sp->L;
+    for (int i1 = 0; i1 < na; i1++)
+        for (int a3 = 0; a3 < L[2]; a3++)
+            for (int a2 = 0; a2 < L[1]; a2++)
+                for (int a1 = 0; a1 < L[0]; a1++) {
+                    double acc[3] = {0, 0, 0};
+                    const double *g1 = sp->g + 9 * i1;
+                    for (int i2 = 0; i2 < na; i2++) {
+                        const double *g2 = sp->g + 9 * i2;
+                        for (int b3 = 0; b3 < L[2]; b3++)
+                            for (int b2 = 0; b2 < L[1]; b2++)
+                                for (int b1 = 0; b1 < L[0]; b1++) {
+                                    long off = site_index(L, wrap(b1 - a1, L[0]), wrap(b2 - a2, L[1]),
+                                                          wrap(b3 - a3, L[2]), 0);
+                                    const double *A = sp->ewald_A + (((long)i2 * na + i1) * nc + off) * 9;
+                                    const double *s2 = S + 3 * site_index(L, b1, b2, b3, i2);
+                                    double gs[3], Ags[3];
+                                    for (int r = 0; r < 3; r++)
+                                        gs[r] = g2[3 * r] * s2[0] + g2[3 * r + 1] * s2[1] + g2[3 * r + 2] * s2[2];
+                                    for (int r = 0; r < 3; r++)
+                                        Ags[r] = A[3 * r] * gs[0] + A[3 * r + 1] * gs[1] + A[3 * r + 2] * gs[2];
+                                    for (int r = 0; r < 3; r++)
+                                        acc[r] += g1[0 * 3 + r] * Ags[0] + g1[1 * 3 + r] * Ags[1] + g1[2 * 3 + r] * Ags[2];
+                                }
+                    }
+                    double *d = dE + 3 * site_index(L, a1, a2, a3, i1);
+                    d[0] = acc[0];
+                    d[1] = acc[1];
+                    d[2] = acc[2];
+                }
+}
+
+/* precompute_dipole_ewald_aux at q = 0, Ewald.jl:61-152 */
+void so_ewald_precompute(const double lv[9], int na, const double *pos, const int L[3],
+                         const double demag[9], double mu0_muB2, double *Aout)
+{
+    /* superlattice vectors (columns) and reciprocal vectors (columns) */
+    double SL[9], RL[9];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) SL[3 * r + c] = lv[3 * r + c] * L[c];
+    /* recipvecs = 2pi inv(latvecs)' */
+    double a = SL[0], b = SL[1], c = SL[2], d = SL[3], e = SL[4], f = SL[5], g = SL[6], h = SL[7], k = SL[8];
+    double det = a * (e * k - f * h) - b * (d * k - f * g) + c * (d * h - e * g);
+    double inv[9] = {(e * k - f * h) / det, -(b * k - c * h) / det, (b * f - c * e) / det,
+                     -(d * k - f * g) / det, (a * k - c * g) / det, -(a * f - c * d) / det,
+                     (d * h - e * g) / det, -(a * h - b * g) / det, (a * e - b * d) / det};
+    for (int r = 0; r < 3; r++)
+        for (int cc = 0; cc < 3; cc++) RL[3 * r + cc] = 2 * M_PI * inv[3 * cc + r];
+    double V = det;
+    double Lc = cbrt(V);
+    double sigma = Lc / 3, s2 = sigma * sigma, s3 = sigma * sigma * sigma;
+    double rmax = 6 * sqrt(2.0) * sigma, kmax = 6 * sqrt(2.0) / sigma;
+    int nmax[3], mmax[3];
+    for (int ax = 0; ax < 3; ax++) {
+        double av[3] = {SL[ax], SL[3 + ax], SL[6 + ax]};
+        double bv[3] = {RL[ax], RL[3 + ax], RL[6 + ax]};
+        double bn = sqrt(bv[0] * bv[0] + bv[1] * bv[1] + bv[2] * bv[2]);
+        double an = sqrt(av[0] * av[0] + av[1] * av[1] + av[2] * av[2]);
+        double adb = (av[0] * bv[0] + av[1] * bv[1] + av[2] * bv[2]);
+        nmax[ax] = (int)nearbyint(rmax / (adb / bn) + 1e-6) + 1; /* Julia round: ties to even */
+        mmax[ax] = (int)nearbyint(kmax / (adb / an) + 1e-6);
+    }
+    long nc = (long)L[0] * L[1] * L[2];
+#pragma omp parallel for collapse(2) schedule(dynamic)
+    for (int j = 0; j < na; j++)
+        for (long cell = 0; cell < nc; cell++) {
+            int c1 = cell % L[0], c2 = (cell / L[0]) % L[1], c3 = cell / ((long)L[0] * L[1]);
+            for (int i = 0; i < na; i++) {
+                double acc[9] = {0};
+                double fr[3] = {c1 + pos[3 * j] - pos[3 * i], c2 + pos[3 * j + 1] - pos[3 * i + 1],
+                                c3 + pos[3 * j + 2] - pos[3 * i + 2]};
+                double dr[3];
+                for (int r = 0; r < 3; r++) dr[r] = lv[3 * r] * fr[0] + lv[3 * r + 1] * fr[1] + lv[3 * r + 2] * fr[2];
+                for (int n1 = -nmax[0]; n1 <= nmax[0]; n1++)
+                    for (int n2 = -nmax[1]; n2 <= nmax[1]; n2++)
+                        for (int n3 = -nmax[2]; n3 <= nmax[2]; n3++) {
+                            double rv[3];
+                            for (int r = 0; r < 3; r++)
+                                rv[r] = dr[r] + SL[3 * r] * n1 + SL[3 * r + 1] * n2 + SL[3 * r + 2] * n3;
+                            double r2 = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
+                            if (0 < r2 && r2 <= rmax * rmax) {
+                                double rr = sqrt(r2), r3 = r2 * rr;
+                                double rh[3] = {rv[0] / rr, rv[1] / rr, rv[2] / rr};
+                                double erfc0 = erfc(rr / (sqrt(2.0) * sigma));
+                                double gauss0 = sqrt(2 / M_PI) * (rr / sigma) * exp(-r2 / (2 * s2));
+                                for (int p = 0; p < 3; p++)
+                                    for (int q = 0; q < 3; q++)
+                                        acc[3 * p + q] += (1 / (4 * M_PI)) *
+                                            ((p == q ? 1.0 : 0.0) / r3 * (erfc0 + gauss0) -
+                                             (3 * rh[p] * rh[q] / r3) * (erfc0 + (1 + r2 / (3 * s2)) * gauss0));
+                            }
+                        }
+                for (int m1 = -mmax[0]; m1 <= mmax[0]; m1++)
+                    for (int m2 = -mmax[1]; m2 <= mmax[1]; m2++)
+                        for (int m3 = -mmax[2]; m3 <= mmax[2]; m3++) {
+                            double kv[3];
+                            for (int r = 0; r < 3; r++)
+                                kv[r] = RL[3 * r] * m1 + RL[3 * r + 1] * m2 + RL[3 * r + 2] * m3;
+                            double k2 = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
+                            double eps2 = 1e-16;
+                            if (k2 <= eps2) {
+                                for (int p = 0; p < 9; p++) acc[p] += demag[p] / V;
+                            } else if (k2 <= kmax * kmax) {
+                                double phase = cos(-(kv[0] * dr[0] + kv[1] * dr[1] + kv[2] * dr[2]));
+                                double w = phase * (1 / V) * (exp(-s2 * k2 / 2) / k2);
+                                for (int p = 0; p < 3; p++)
+                                    for (int q = 0; q < 3; q++) acc[3 * p + q] += w * kv[p] * kv[q];
+                            }
+                        }
+                if (dr[0] == 0 && dr[1] == 0 && dr[2] == 0)
+                    for (int p = 0; p < 3; p++) acc[4 * p] += -1 / (3 * pow(2 * M_PI, 1.5) * s3);
+                double *A = Aout + (((long)j * na + i) * nc + cell) * 9;
+                for (int p = 0; p < 9; p++) A[p] = acc[p] * mu0_muB2;
+            }
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* set_energy_grad_dipoles!, Interactions.jl:516-592                         */
+
+static void grad_dipoles_impl(const so_spec *sp, const double *S, double *dE, int sun_mode)
+{
+    long nc = ncells(sp), ns = nsites(sp);
+    const int *L = sp->L;
+    memset(dE, 0, sizeof(double) * 3 * ns);
+    /* Zeeman, :520-522: dE += g' B */
+    for (long s = 0; s < ns; s++) {
+        int i = s / nc;
+        const double *g = sp->g + 9 * i, *B = sp->field + 3 * i;
+        for (int b = 0; b < 3; b++) dE[3 * s + b] += g[0 * 3 + b] * B[0] + g[1 * 3 + b] * B[1] + g[2 * 3 + b] * B[2];
+    }
+    for (int i = 0; i < sp->na; i++) {
+        /* anisotropy, :546-552 (dipole modes only) */
+        if (!sun_mode && sp->kmax[i] >= 0) {
+            for (long c = 0; c < nc; c++) {
+                long s = c + nc * i;
+                double E, gr[3];
+                so_stevens_energy_grad(S + 3 * s, sp->stevens + 28 * i, sp->kmax[i], &E, gr);
+                dE[3 * s] += gr[0];
+                dE[3 * s + 1] += gr[1];
+                dE[3 * s + 2] += gr[2];
+            }
+        }
+        /* bonds, :554-591: non-culled bonds of sublattice i, scatter both ends */
+        for (int b = 0; b < sp->nb; b++) {
+            const int *bd = sp->bond + 6 * b;
+            if (bd[0] != i || bd[5]) continue;
+            const double *J = sp->bilin + 9 * b;
+            for (int c3 = 0; c3 < L[2]; c3++)
+                for (int c2 = 0; c2 < L[1]; c2++)
+                    for (int c1 = 0; c1 < L[0]; c1++) {
+                        long si = site_index(L, c1, c2, c3, i);
+                        long sj = site_index(L, wrap(c1 + bd[2], L[0]), wrap(c2 + bd[3], L[1]),
+                                             wrap(c3 + bd[4], L[2]), bd[1]);
+                        const double *Si = S + 3 * si, *Sj = S + 3 * sj;
+                        double gi[3], gj[3];
+                        for (int r = 0; r < 3; r++) {
+                            gi[r] = J[3 * r] * Sj[0] + J[3 * r + 1] * Sj[1] + J[3 * r + 2] * Sj[2];
+                            gj[r] = J[r] * Si[0] + J[3 + r] * Si[1] + J[6 + r] * Si[2];
+                        }
+                        for (int r = 0; r < 3; r++) {
+                            dE[3 * si + r] += gi[r];
+                            dE[3 * sj + r] += gj[r];
+                        }
+                        if (!sun_mode && sp->has_biquad[b]) {
+                            const double *Bq = sp->biquad + 25 * b;
+                            double Qi[5], Qj[5], Gi[5][3], Gj[5][3];
+                            quadrupole(Si, Qi);
+                            quadrupole(Sj, Qj);
+                            grad_quadrupole(Si, Gi);
+                            grad_quadrupole(Sj, Gj);
+                            /* dE_i += (B Qj)' dQi ; dE_j += (B' Qi)' dQj */
+                            double BQj[5], BtQi[5];
+                            for (int p = 0; p < 5; p++) {
+                                BQj[p] = 0;
+                                BtQi[p] = 0;
+                                for (int q = 0; q < 5; q++) {
+                                    BQj[p] += Bq[5 * p + q] * Qj[q];
+                                    BtQi[p] += Bq[5 * q + p] * Qi[q];
+                                }
+                            }
+                            for (int r = 0; r < 3; r++) {
+                                double ai = 0, aj = 0;
+                                for (int p = 0; p < 5; p++) {
+                                    ai += BQj[p] * Gi[p][r];
+                                    aj += BtQi[p] * Gj[p][r];
+                                }
+                                dE[3 * si + r] += ai;
+                                dE[3 * sj + r] += aj;
+                            }
+                        }
+                    }
+        }
+    }
+    if (sp->ewald_A) ewald_accum_grad(sp, S, dE);
+}
+
+void so_grad_dipoles(const so_spec *sp, const double *S, double *dE)
+{
+    grad_dipoles_impl(sp, S, dE, sp->N > 0);
+}
+
+/* set_energy_grad_coherents!, Interactions.jl:596-669 */
+void so_grad_coherents(const so_spec *sp, const double *Zr, double *HZr)
+{
+    int N = sp->N;
+    long nc = ncells(sp), ns = nsites(sp);
+    const int *L = sp->L;
+    sun_ops o;
+    sun_ops_init(&o, N);
+    const cplx *Z = (const cplx *)Zr;
+    cplx *HZ = (cplx *)HZr;
+    memset(HZ, 0, sizeof(cplx) * N * ns);
+    double *dip = malloc(sizeof(double) * 3 * ns);
+    double *dE = malloc(sizeof(double) * 3 * ns);
+    for (long s = 0; s < ns; s++) expected_spin1(&o, Z + N * s, dip + 3 * s);
+    grad_dipoles_impl(sp, dip, dE, 1);
+    for (int i = 0; i < sp->na; i++) {
+        const double *Lam = sp->Lambda + (long)2 * N * N * i;
+        for (long c = 0; c < nc; c++) {
+            long s = c + nc * i;
+            /* mul_spin_matrices: (Lambda + dE.S) Z */
+            for (int a = 0; a < N; a++) {
+                cplx acc = 0;
+                for (int b = 0; b < N; b++) {
+                    cplx h = Lam[2 * (a * N + b)] + I * Lam[2 * (a * N + b) + 1];
+                    for (int al = 0; al < 3; al++) h += dE[3 * s + al] * o.S[al][a * N + b];
+                    acc += h * Z[N * s + b];
+                }
+                HZ[N * s + a] += acc;
+            }
+        }
+        for (int b = 0; b < sp->nb; b++) {
+            const int *bd = sp->bond + 6 * b;
+            if (bd[0] != i || bd[5] || !sp->has_biquad[b]) continue;
+            const double *Bq = sp->biquad + 25 * b;
+            for (int c3 = 0; c3 < L[2]; c3++)
+                for (int c2 = 0; c2 < L[1]; c2++)
+                    for (int c1 = 0; c1 < L[0]; c1++) {
+                        long si = site_index(L, c1, c2, c3, i);
+                        long sj = site_index(L, wrap(c1 + bd[2], L[0]), wrap(c2 + bd[3], L[1]),
+                                             wrap(c3 + bd[4], L[2]), bd[1]);
+                        double Qi[5], Qj[5], dQi[5], dQj[5];
+                        expected_quadrupole1(&o, Z + N * si, Qi);
+                        expected_quadrupole1(&o, Z + N * sj, Qj);
+                        for (int p = 0; p < 5; p++) {
+                            dQi[p] = 0;
+                            dQj[p] = 0;
+                            for (int q = 0; q < 5; q++) {
+                                dQi[p] += Bq[5 * p + q] * Qj[q];
+                                dQj[p] += Bq[5 * q + p] * Qi[q];
+                            }
+                        }
+                        /* mul_quadrupole_matrices: (dE/dQ . Q) Z, Spin.jl:151-167 */
+                        for (int a = 0; a < N; a++) {
+                            cplx ai = 0, aj = 0;
+                            for (int bb = 0; bb < N; bb++) {
+                                cplx hi = 0, hj = 0;
+                                for (int p = 0; p < 5; p++) {
+                                    hi += dQi[p] * o.Q[p][a * N + bb];
+                                    hj += dQj[p] * o.Q[p][a * N + bb];
+                                }
+                                ai += hi * Z[N * si + bb];
+                                aj += hj * Z[N * sj + bb];
+                            }
+                            HZ[N * si + a] += ai;
+                            HZ[N * sj + a] += aj;
+                        }
+                    }
+        }
+    }
+    free(dip);
+    free(dE);
+}
+
+/* energy, Interactions.jl:401-508 */
+double so_energy(const so_spec *sp, const double *S, const double *Zr)
+{
+    long nc = ncells(sp), ns = nsites(sp);
+    const int *L = sp->L;
+    int N = sp->N;
+    const cplx *Z = (const cplx *)Zr;
+    sun_ops o;
+    if (N > 0) sun_ops_init(&o, N);
+    double E = 0;
+    for (long s = 0; s < ns; s++) {
+        int i = s / nc;
+        const double *g = sp->g + 9 * i, *B = sp->field + 3 * i, *d = S + 3 * s;
+        for (int r = 0; r < 3; r++) E += B[r] * (g[3 * r] * d[0] + g[3 * r + 1] * d[1] + g[3 * r + 2] * d[2]);
+    }
+    for (int i = 0; i < sp->na; i++) {
+        double Ei = 0;
+        if (N == 0) {
+            for (long c = 0; c < nc; c++) {
+                double e, gr[3];
+                so_stevens_energy_grad(S + 3 * (c + nc * i), sp->stevens + 28 * i, sp->kmax[i], &e, gr);
+                Ei += e;
+            }
+        } else {
+            const double *Lam = sp->Lambda + (long)2 * N * N * i;
+            for (long c = 0; c < nc; c++) {
+                const cplx *z = Z + N * (c + nc * i);
+                cplx acc = 0;
+                for (int a = 0; a < N; a++)
+                    for (int b = 0; b < N; b++)
+                        acc += conj(z[a]) * (Lam[2 * (a * N + b)] + I * Lam[2 * (a * N + b) + 1]) * z[b];
+                Ei += creal(acc);
+            }
+        }
+        for (int b = 0; b < sp->nb; b++) {
+            const int *bd = sp->bond + 6 * b;
+            if (bd[0] != i || bd[5]) continue;
+            const double *J = sp->bilin + 9 * b;
+            for (int c3 = 0; c3 < L[2]; c3++)
+                for (int c2 = 0; c2 < L[1]; c2++)
+                    for (int c1 = 0; c1 < L[0]; c1++) {
+                        long si = site_index(L, c1, c2, c3, i);
+                        long sj = site_index(L, wrap(c1 + bd[2], L[0]), wrap(c2 + bd[3], L[1]),
+                                             wrap(c3 + bd[4], L[2]), bd[1]);
+                        const double *Si = S + 3 * si, *Sj = S + 3 * sj;
+                        if (N > 0)
+                            Ei += sp->scalar[b] * sp->kappa[i] * sp->kappa[bd[1]];
+                        else
+                            Ei += sp->scalar[b];
+                        for (int r = 0; r < 3; r++)
+                            Ei += Si[r] * (J[3 * r] * Sj[0] + J[3 * r + 1] * Sj[1] + J[3 * r + 2] * Sj[2]);
+                        if (sp->has_biquad[b]) {
+                            double Qi[5], Qj[5];
+                            if (N == 0) {
+                                quadrupole(Si, Qi);
+                                quadrupole(Sj, Qj);
+                            } else {
+                                expected_quadrupole1(&o, Z + N * si, Qi);
+                                expected_quadrupole1(&o, Z + N * sj, Qj);
+                            }
+                            const double *Bq = sp->biquad + 25 * b;
+                            for (int p = 0; p < 5; p++)
+                                for (int q = 0; q < 5; q++) Ei += Qi[p] * Bq[5 * p + q] * Qj[q];
+                        }
+                    }
+        }
+        E += Ei;
+    }
+    if (sp->ewald_A) E += so_ewald_energy(sp, S);
+    return E;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Integrators.jl:266-452                                                    */
+
+static inline void cross(const double *a, const double *b, double *c)
+{
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* rhs_dipole!, :266-275 */
+static void rhs_dipole(long ns, double *dS, const double *S, const double *xi, const double *dE,
+                       double dt, double lambda)
+{
+    for (long s = 0; s < ns; s++) {
+        const double *Sv = S + 3 * s, *g = dE + 3 * s;
+        double v[3];
+        if (lambda == 0) {
+            for (int r = 0; r < 3; r++) v[r] = dt * g[r];
+        } else {
+            double sxg[3];
+            cross(Sv, g, sxg);
+            for (int r = 0; r < 3; r++) v[r] = xi[3 * s + r] + dt * g[r] - dt * lambda * sxg[r];
+        }
+        double o[3];
+        cross(Sv, v, o);
+        for (int r = 0; r < 3; r++) dS[3 * s + r] = -o[r];
+    }
+}
+
+/* normalize_dipole, System.jl:421-423 */
+static inline void normalize_dipole(double *s, double kappa)
+{
+    if (kappa == 0) {
+        s[0] = s[1] = s[2] = 0;
+        return;
+    }
+    double n = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    for (int r = 0; r < 3; r++) s[r] = kappa * (s[r] / n);
+}
+
+/* normalize_ket, System.jl:416-418 */
+static inline void normalize_ket(int N, cplx *z, double kappa)
+{
+    if (kappa == 0) {
+        for (int a = 0; a < N; a++) z[a] = 0;
+        return;
+    }
+    double n2 = 0;
+    for (int a = 0; a < N; a++) n2 += creal(conj(z[a]) * z[a]);
+    double d = sqrt(n2 / kappa);
+    for (int a = 0; a < N; a++) z[a] = z[a] / d;
+}
+
+/* fill_noise!, :288-298 */
+static void fill_noise(long n, double *xi, const double *raw, double dt, double lambda, double kT)
+{
+    if (lambda == 0 || kT == 0 || !raw) {
+        memset(xi, 0, sizeof(double) * n);
+        return;
+    }
+    double sc = sqrt(2 * dt * lambda * kT);
+    for (long k = 0; k < n; k++) xi[k] = raw[k] * sc;
+}
+
+/* step!(::System{0}, ::Langevin), :313-332 */
+void so_step_langevin_dipole(const so_spec *sp, double *S, const double *noise, double dt,
+                             double damping, double kT)
+{
+    long ns = nsites(sp), nc = ncells(sp);
+    double *Sp = malloc(sizeof(double) * 3 * ns), *dS1 = malloc(sizeof(double) * 3 * ns),
+           *dS2 = malloc(sizeof(double) * 3 * ns), *xi = malloc(sizeof(double) * 3 * ns),
+           *dE = malloc(sizeof(double) * 3 * ns);
+    fill_noise(3 * ns, xi, noise, dt, damping, kT);
+    so_grad_dipoles(sp, S, dE);
+    rhs_dipole(ns, dS1, S, xi, dE, dt, damping);
+    for (long s = 0; s < ns; s++) {
+        for (int r = 0; r < 3; r++) Sp[3 * s + r] = S[3 * s + r] + dS1[3 * s + r];
+        normalize_dipole(Sp + 3 * s, sp->kappa[s / nc]);
+    }
+    so_grad_dipoles(sp, Sp, dE);
+    rhs_dipole(ns, dS2, Sp, xi, dE, dt, damping);
+    for (long s = 0; s < ns; s++) {
+        for (int r = 0; r < 3; r++) S[3 * s + r] = S[3 * s + r] + (dS1[3 * s + r] + dS2[3 * s + r]) / 2;
+        normalize_dipole(S + 3 * s, sp->kappa[s / nc]);
+    }
+    free(Sp);
+    free(dS1);
+    free(dS2);
+    free(xi);
+    free(dE);
+}
+
+/* rhs_sun!, :277-286, with proj from MathBasics.jl:62 */
+static void rhs_sun(int N, long ns, cplx *dZ, const cplx *Z, const cplx *zeta, const cplx *HZ,
+                    double dt, double lambda)
+{
+    for (long s = 0; s < ns; s++) {
+        const cplx *z = Z + N * s, *h = HZ + N * s;
+        if (lambda == 0) {
+            for (int a = 0; a < N; a++) dZ[N * s + a] = -I * dt * h[a];
+        } else {
+            cplx v[NMAX];
+            for (int a = 0; a < N; a++) v[a] = zeta[N * s + a] + dt * (I + lambda) * h[a];
+            cplx nv = 0;
+            double nn = 0;
+            for (int a = 0; a < N; a++) {
+                nv += conj(z[a]) * v[a];
+                nn += creal(conj(z[a]) * z[a]);
+            }
+            for (int a = 0; a < N; a++) dZ[N * s + a] = -(v[a] - z[a] * (nv / nn));
+        }
+    }
+}
+
+/* step!(::System{N}, ::Langevin), :335-357 */
+void so_step_langevin_sun(const so_spec *sp, double *Zr, double *S, const double *noise, double dt,
+                          double damping, double kT)
+{
+    int N = sp->N;
+    long ns = nsites(sp), nc = ncells(sp);
+    cplx *Z = (cplx *)Zr;
+    cplx *Zp = malloc(sizeof(cplx) * N * ns), *dZ1 = malloc(sizeof(cplx) * N * ns),
+         *dZ2 = malloc(sizeof(cplx) * N * ns), *zeta = malloc(sizeof(cplx) * N * ns),
+         *HZ = malloc(sizeof(cplx) * N * ns);
+    fill_noise(2 * N * ns, (double *)zeta, noise, dt, damping, kT);
+    so_grad_coherents(sp, (double *)Z, (double *)HZ);
+    rhs_sun(N, ns, dZ1, Z, zeta, HZ, dt, damping);
+    for (long s = 0; s < ns; s++) {
+        for (int a = 0; a < N; a++) Zp[N * s + a] = Z[N * s + a] + dZ1[N * s + a];
+        normalize_ket(N, Zp + N * s, sp->kappa[s / nc]);
+    }
+    so_grad_coherents(sp, (double *)Zp, (double *)HZ);
+    rhs_sun(N, ns, dZ2, Zp, zeta, HZ, dt, damping);
+    for (long s = 0; s < ns; s++) {
+        for (int a = 0; a < N; a++) Z[N * s + a] = Z[N * s + a] + (dZ1[N * s + a] + dZ2[N * s + a]) / 2;
+        normalize_ket(N, Z + N * s, sp->kappa[s / nc]);
+    }
+    so_expected_spin(N, ns, Zr, S);
+    free(Zp);
+    free(dZ1);
+    free(dZ2);
+    free(zeta);
+    free(HZ);
+}
+
+/* fast_isapprox, :361-371 */
+static int fast_isapprox(long n, const double *x, const double *y, long comps, double atol)
+{
+    double acc = 0;
+    for (long s = 0; s < n; s++) {
+        double d2 = 0;
+        for (long r = 0; r < comps; r++) {
+            double d = x[comps * s + r] - y[comps * s + r];
+            d2 += d * d;
+        }
+        acc += d2;
+        if (acc > atol * atol) return 0;
+    }
+    return !isnan(acc);
+}
+
+/* step!(::System{0}, ::ImplicitMidpoint), :379-413 */
+int so_step_midpoint_dipole(const so_spec *sp, double *S, double dt, double damping, double tol,
+                            int max_iters)
+{
+    long ns = nsites(sp), nc = ncells(sp);
+    double atol = tol * sqrt((double)ns);
+    double *dS = malloc(sizeof(double) * 3 * ns), *Sh = malloc(sizeof(double) * 3 * ns),
+           *S1 = malloc(sizeof(double) * 3 * ns), *S2 = malloc(sizeof(double) * 3 * ns),
+           *dE = malloc(sizeof(double) * 3 * ns);
+    double *xi = calloc(3 * ns, sizeof(double)); /* kT == 0 is enforced by the constructor */
+    memcpy(S1, S, sizeof(double) * 3 * ns);
+    memcpy(S2, S, sizeof(double) * 3 * ns);
+    int ret = -1;
+    for (int it = 1; it <= max_iters; it++) {
+        for (long s = 0; s < ns; s++) {
+            for (int r = 0; r < 3; r++) Sh[3 * s + r] = (S[3 * s + r] + S1[3 * s + r]) / 2;
+            normalize_dipole(Sh + 3 * s, sp->kappa[s / nc]);
+        }
+        so_grad_dipoles(sp, Sh, dE);
+        rhs_dipole(ns, dS, Sh, xi, dE, dt, damping);
+        for (long k = 0; k < 3 * ns; k++) S2[k] = S[k] + dS[k];
+        if (fast_isapprox(ns, S1, S2, 3, atol)) {
+            for (long s = 0; s < ns; s++) {
+                for (int r = 0; r < 3; r++) S[3 * s + r] = S2[3 * s + r];
+                normalize_dipole(S + 3 * s, sp->kappa[s / nc]);
+            }
+            ret = it;
+            break;
+        }
+        double *t = S1;
+        S1 = S2;
+        S2 = t;
+    }
+    free(dS);
+    free(Sh);
+    free(S1);
+    free(S2);
+    free(dE);
+    free(xi);
+    return ret;
+}
+
+/* step!(::System{N}, ::ImplicitMidpoint), :422-452 */
+int so_step_midpoint_sun(const so_spec *sp, double *Zr, double *S, double dt, double damping,
+                         double tol, int max_iters)
+{
+    int N = sp->N;
+    long ns = nsites(sp), nc = ncells(sp);
+    double atol = tol * sqrt((double)ns);
+    cplx *Z = (cplx *)Zr;
+    cplx *dZ = malloc(sizeof(cplx) * N * ns), *Zb = malloc(sizeof(cplx) * N * ns),
+         *Z1 = malloc(sizeof(cplx) * N * ns), *Z2 = malloc(sizeof(cplx) * N * ns),
+         *HZ = malloc(sizeof(cplx) * N * ns);
+    cplx *zeta = calloc(N * ns, sizeof(cplx));
+    memcpy(Z1, Z, sizeof(cplx) * N * ns);
+    memcpy(Z2, Z, sizeof(cplx) * N * ns);
+    int ret = -1;
+    for (int it = 1; it <= max_iters; it++) {
+        for (long k = 0; k < N * ns; k++) Zb[k] = (Z[k] + Z1[k]) / 2;
+        so_grad_coherents(sp, (double *)Zb, (double *)HZ);
+        rhs_sun(N, ns, dZ, Zb, zeta, HZ, dt, damping);
+        for (long k = 0; k < N * ns; k++) Z2[k] = Z[k] + dZ[k];
+        if (fast_isapprox(ns, (double *)Z1, (double *)Z2, 2 * N, atol)) {
+            for (long s = 0; s < ns; s++) {
+                for (int a = 0; a < N; a++) Z[N * s + a] = Z2[N * s + a];
+                normalize_ket(N, Z + N * s, sp->kappa[s / nc]);
+            }
+            so_expected_spin(N, ns, Zr, S);
+            ret = it;
+            break;
+        }
+        cplx *t = Z1;
+        Z1 = Z2;
+        Z2 = t;
+    }
+    free(dZ);
+    free(Zb);
+    free(Z1);
+    free(Z2);
+    free(HZ);
+    free(zeta);
+    return ret;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw, SC'11)
+   and the Box-Muller map used for device-side noise.                       */
+
+void so_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+void so_philox_normals(uint64_t seed, uint64_t step, long ns, int per_site, double *out)
+{
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    int nblk = (per_site + 3) / 4;
+    for (long s = 0; s < ns; s++)
+        for (int b = 0; b < nblk; b++) {
+            uint32_t ctr[4] = {(uint32_t)s, (uint32_t)b, (uint32_t)step, (uint32_t)(step >> 32)}, u[4];
+            so_philox4x32(ctr, key, u);
+            for (int p = 0; p < 2; p++) {
+                double ua = ((double)u[2 * p] + 1.0) * (1.0 / 4294967296.0);
+                double ub = (double)u[2 * p + 1] * (1.0 / 4294967296.0);
+                double r = sqrt(-2.0 * log(ua));
+                double th = 2.0 * M_PI * ub;
+                int k0 = 4 * b + 2 * p;
+                if (k0 < per_site) out[(long)per_site * s + k0] = r * cos(th);
+                if (k0 + 1 < per_site) out[(long)per_site * s + k0 + 1] = r * sin(th);
+            }
+        }
+}

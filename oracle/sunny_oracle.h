@@ -1,0 +1,105 @@
+/*
+ * FP64 CPU oracle for Sunny.jl's spin-dynamics hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker against which the
+ * HIP implementation is compared.  It may be loaded only by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; the product
+ * library never links or calls it.
+ *
+ * It restates, loop for loop, the reference algorithm (Sunny.jl 0.9.2):
+ *   src/Integrators.jl:266-452          rhs_dipole!/rhs_sun!/fill_noise!, step!
+ *   src/System/Interactions.jl:401-669  energy, set_energy_grad_dipoles!,
+ *                                       set_energy_grad_coherents!
+ *   src/System/OnsiteCoupling.jl:316-425 classical Stevens anisotropy
+ *   src/Operators/Spin.jl:1-167         spin matrices, expected_spin, ...
+ *   src/System/Ewald.jl:1-241           A precompute, FFT convolution, energy
+ *   src/System/System.jl:204-212,416-423 bonded_site, normalize_*
+ *
+ * Layout (identical to Julia's column-major Array{Vec3,4}(L1,L2,L3,na)):
+ *   site = c1 + L1*(c2 + L2*(c3 + L3*i)), all 0-based
+ *   dipoles:   double[nsites][3]
+ *   coherents: double[nsites][N][2]   (re, im)
+ */
+#ifndef SUNNY_ORACLE_H
+#define SUNNY_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int L[3];                 /* supercell dims L1, L2, L3 */
+    int na;                   /* atoms (Bravais sublattices) per cell */
+    int N;                    /* 0 = :dipole mode; N >= 2 = :SUN mode */
+    const double *kappa;      /* [na]  |S| = kappa (dipole) or |Z|^2 = kappa (SU(N)) */
+    const double *g;          /* [na][3][3] row-major g tensor */
+    const double *field;      /* [na][3] external field B (energy units) */
+    const double *stevens;    /* [na][28] c0 | c2[5] | c4[9] | c6[13], q descending */
+    const int *kmax;          /* [na] 0,2,4,6 */
+    const double *Lambda;     /* [na][N][N][2] onsite Hermitian matrix (SU(N)) */
+    int nb;                   /* number of directed bonds (both directions) */
+    const int *bond;          /* [nb][6] = i, j, n1, n2, n3, isculled */
+    const double *bilin;      /* [nb][9] row-major J (bond i -> j) */
+    const double *biquad;     /* [nb][25] row-major 5x5 coupling of Stevens quadrupoles */
+    const int *has_biquad;    /* [nb] */
+    const double *scalar;     /* [nb] constant shift */
+    const double *ewald_A;    /* NULL or [na(j)][na(i)][L3][L2][L1][3][3] (A[off,i,j]) */
+} so_spec;
+
+/* Stevens classical anisotropy, OnsiteCoupling.jl:316-425. c = [c0|c2|c4|c6]. */
+void so_stevens_energy_grad(const double S[3], const double c[28], int kmax,
+                            double *E, double grad[3]);
+
+/* Spin matrices in the N-dim irrep, Spin.jl:1-14. Each out is [N][N][2]. */
+void so_spin_matrices(int N, double *Sx, double *Sy, double *Sz);
+/* Stevens quadrupoles O[2,q], q = 2..-2, Stevens.jl:12-86. out [5][N][N][2]. */
+void so_stevens_quadrupoles(int N, double *Q);
+
+/* expected_spin (Spin.jl:72-87) for all sites; Z [ns][N][2] -> S [ns][3]. */
+void so_expected_spin(int N, long ns, const double *Z, double *S);
+
+/* set_energy_grad_dipoles!, Interactions.jl:516-592, with Ewald (Ewald.jl:185-209). */
+void so_grad_dipoles(const so_spec *sp, const double *S, double *dE);
+/* set_energy_grad_coherents!, Interactions.jl:596-669. */
+void so_grad_coherents(const so_spec *sp, const double *Z, double *HZ);
+/* energy, Interactions.jl:401-508 (+ ewald_energy Ewald.jl:155-181).
+   For SU(N), dipoles must equal expected_spin(Z). */
+double so_energy(const so_spec *sp, const double *S, const double *Z);
+
+/* One step!. noise is the raw draw of randn!(rng, xi) in Julia memory order:
+   dipole: [ns][3] standard normals; SU(N): [ns][N][2], complex normals with
+   variance 1/2 per component.  NULL means zeros. Integrators.jl:313-357. */
+void so_step_langevin_dipole(const so_spec *sp, double *S, const double *noise,
+                             double dt, double damping, double kT);
+void so_step_langevin_sun(const so_spec *sp, double *Z, double *S, const double *noise,
+                          double dt, double damping, double kT);
+/* Integrators.jl:379-452. Returns iterations used (>0), or -1 when the
+   fixed point fails to converge within max_iters (the reference errors). */
+int so_step_midpoint_dipole(const so_spec *sp, double *S, double dt, double damping,
+                            double tol, int max_iters);
+int so_step_midpoint_sun(const so_spec *sp, double *Z, double *S, double dt,
+                         double damping, double tol, int max_iters);
+
+/* Ewald.jl:61-152 at q = 0: A[j][i][c3][c2][c1][3][3] (row-major 3x3), already
+   scaled by mu0_muB2.  latvecs row-major: latvecs[r*3+c] = component r of a_c.
+   positions [na][3] fractional. demag row-major 3x3. */
+void so_ewald_precompute(const double latvecs[9], int na, const double *positions,
+                         const int L[3], const double demag[9], double mu0_muB2,
+                         double *A);
+/* Ewald energy, Ewald.jl:155-181 (half-spectrum rfft with sqrt(2) weights). */
+double so_ewald_energy(const so_spec *sp, const double *S);
+/* Field at every site by the brute-force pairwise sum, Ewald.jl:211-230. */
+void so_ewald_grad_bruteforce(const so_spec *sp, const double *S, double *dE);
+
+/* Philox4x32-10 (Salmon et al., SC'11) and the Box-Muller map used by the
+   device noise generator. Writes n standard normals for (site, step) with the
+   layout normal[k] = k-th normal of that site. */
+void so_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void so_philox_normals(uint64_t seed, uint64_t step, long nsites, int per_site,
+                       double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

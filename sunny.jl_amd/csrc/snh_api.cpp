@@ -1,0 +1,878 @@
+// C-ABI of libsunnyhip: device-resident System (and replica batches), the
+// step!/energy entry points, and their orchestration on one HIP stream.
+#include "../../include/sunny_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "snh_internal.h"
+
+using namespace snh;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Failure {
+    int code;
+    std::string msg;
+};
+
+#define SNH_HIP(call)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            throw Failure{SNH_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)};             \
+    } while (0)
+
+#define SNH_REQUIRE(cond, code, msg)                                                                    \
+    do {                                                                                                \
+        if (!(cond)) throw Failure{code, msg};                                                          \
+    } while (0)
+
+template <typename X>
+X* dalloc(size_t n)
+{
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    SNH_HIP(hipMalloc(&p, n * sizeof(X)));
+    return static_cast<X*>(p);
+}
+
+template <typename X>
+X* dupload(const std::vector<X>& v)
+{
+    X* p = dalloc<X>(v.size());
+    if (!v.empty()) SNH_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice));
+    return p;
+}
+
+// Stevens quadrupoles O[2,q], q = 2..-2, in the N-dim spin irrep, as
+// polynomials of spin matrices (src/Operators/Stevens.jl:12-86 at k = 2).
+std::vector<double> quadrupole_ops(int N)
+{
+    using cd = std::complex<double>;
+    double s = (N - 1) / 2.0;
+    std::vector<cd> Sx(N * N), Sy(N * N), Sz(N * N);
+    for (int j = 1; j < N; j++) {
+        double off = std::sqrt(2 * (s + 1) * j - (double)j * (j + 1)) / 2;
+        Sx[(j - 1) * N + j] = off;
+        Sx[j * N + j - 1] = off;
+        Sy[(j - 1) * N + j] = cd(0, -off);
+        Sy[j * N + j - 1] = cd(0, off);
+    }
+    for (int k = 0; k < N; k++) Sz[k * N + k] = s - k;
+    auto mul = [N](const std::vector<cd>& A, const std::vector<cd>& B) {
+        std::vector<cd> C(N * N);
+        for (int a = 0; a < N; a++)
+            for (int b = 0; b < N; b++) {
+                cd acc = 0;
+                for (int k = 0; k < N; k++) acc += A[a * N + k] * B[k * N + b];
+                C[a * N + b] = acc;
+            }
+        return C;
+    };
+    auto xx = mul(Sx, Sx), yy = mul(Sy, Sy), zz = mul(Sz, Sz), xz = mul(Sx, Sz), zx = mul(Sz, Sx),
+         yz = mul(Sy, Sz), zy = mul(Sz, Sy), xy = mul(Sx, Sy), yx = mul(Sy, Sx);
+    std::vector<double> out(5 * N * N * 2);
+    for (int k = 0; k < N * N; k++) {
+        cd q[5] = {xx[k] - yy[k], (xz[k] + zx[k]) / 2.0, 2.0 * zz[k] - xx[k] - yy[k], (yz[k] + zy[k]) / 2.0,
+                   xy[k] + yx[k]};
+        for (int p = 0; p < 5; p++) {
+            out[2 * (p * N * N + k)] = q[p].real();
+            out[2 * (p * N * N + k) + 1] = q[p].imag();
+        }
+    }
+    return out;
+}
+
+struct KernelTimer {
+    bool enabled = false;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> events;
+    std::vector<hipEvent_t> pool;
+
+    hipEvent_t get()
+    {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        SNH_HIP(hipEventCreate(&e));
+        return e;
+    }
+    void clear()
+    {
+        for (auto& p : events) {
+            pool.push_back(p.second.first);
+            pool.push_back(p.second.second);
+        }
+        events.clear();
+    }
+    ~KernelTimer()
+    {
+        clear();
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+struct snh_system {
+    std::string err;
+    virtual ~snh_system() = default;
+    virtual int64_t total_sites() const = 0;
+    virtual int replicas() const = 0;
+    virtual int64_t replica_sites(int r) const = 0;
+    virtual void set_state(const double* x, bool coherent) = 0;
+    virtual void get_state(double* x, bool coherent) = 0;
+    virtual void step_langevin(double dt, double damping, const double* kT, const double* noise, uint64_t seed,
+                               uint64_t step0, int nsteps) = 0;
+    virtual void step_midpoint(double dt, double damping, double tol, int max_iters, int nsteps, int* iters) = 0;
+    virtual void energy(double* E) = 0;
+    virtual void energy_grad(double* out) = 0;
+    virtual void synchronize() = 0;
+    virtual void enable_timing(bool on) = 0;
+    virtual void kernel_time(const char* fam, double* ms, int64_t* n) = 0;
+};
+
+namespace {
+
+template <typename T>
+struct SystemImpl final : snh_system {
+    hipStream_t st = nullptr;
+    int R = 0, na = 0, N = 0, nb = 0, nblocks = 0;
+    long long total = 0;
+    bool ewald = false;
+    std::vector<ReplicaInfo> reps;
+    std::vector<int> rblk;
+    // device tables
+    std::vector<void*> owned;
+    Couplings<T> C{};
+    int* d_rblk = nullptr;
+    // state
+    T* X = nullptr;     // current state (3 or 2N planes)
+    T* Xp = nullptr;    // Heun stage
+    T* Xo = nullptr;    // ping-pong output
+    T* B1 = nullptr;    // midpoint buffers (lazy)
+    T* B2 = nullptr;
+    T* Bh = nullptr;    // materialised midpoint state (Ewald only)
+    T* Sd = nullptr;    // expected dipoles of a coherent state (SU(N) + Ewald)
+    double* partial = nullptr;
+    double* d_out = nullptr;   // [R]
+    int* conv_iter = nullptr;  // [R]
+    double* atol2 = nullptr;   // [R]
+    T* nscale = nullptr;       // [R]
+    double* host_noise = nullptr;
+    double* pinned = nullptr;  // [R] host readback
+    int* pinned_i = nullptr;   // [R]
+    int planes = 3;
+    int last_iters = 4;
+    // Ewald
+    EwaldGeom G{};
+    void* M = nullptr;
+    void* Fmu = nullptr;
+    void* Fphi = nullptr;
+    T* phi1 = nullptr;
+    T* phi2 = nullptr;
+    KernelTimer timer;
+
+    template <typename X_>
+    X_* own(X_* p)
+    {
+        owned.push_back(p);
+        return p;
+    }
+
+    ~SystemImpl() override
+    {
+        if (st) (void)hipStreamSynchronize(st);
+        timer.clear();
+        for (void* p : owned) (void)hipFree(p);
+        if (pinned) (void)hipHostFree(pinned);
+        if (pinned_i) (void)hipHostFree(pinned_i);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    void build(const snh_spec& sp)
+    {
+        R = sp.nreplicas;
+        na = sp.na;
+        N = sp.N;
+        nb = sp.nbonds;
+        ewald = sp.ewald != 0;
+        planes = (N == 0) ? 3 : 2 * N;
+        SNH_HIP(hipStreamCreate(&st));
+
+        // replicas and dims classes
+        std::map<std::vector<int>, int> dcls_of;
+        std::vector<std::vector<int>> dcls_dims;
+        std::vector<int> rdcls(R);
+        long long off = 0;
+        for (int r = 0; r < R; r++) {
+            std::vector<int> d = {sp.dims[3 * r], sp.dims[3 * r + 1], sp.dims[3 * r + 2]};
+            for (int a = 0; a < 3; a++)
+                SNH_REQUIRE(d[a] >= 1, SNH_ERR_INVALID, "replica dims must be positive");
+            auto it = dcls_of.find(d);
+            if (it == dcls_of.end()) {
+                it = dcls_of.emplace(d, (int)dcls_dims.size()).first;
+                dcls_dims.push_back(d);
+            }
+            rdcls[r] = it->second;
+            ReplicaInfo ri;
+            ri.L1 = d[0];
+            ri.L2 = d[1];
+            ri.L3 = d[2];
+            long long nc = (long long)d[0] * d[1] * d[2];
+            SNH_REQUIRE(nc * na < (1ll << 31), SNH_ERR_UNSUPPORTED, "a replica must have fewer than 2^31 sites");
+            ri.ncell = (int)nc;
+            ri.off = off;
+            off += nc * na;
+            reps.push_back(ri);
+        }
+        total = off;
+        // blocks: replica-major, then sublattice, then chunks of kBlock cells
+        std::vector<BlockInfo> blocks;
+        rblk.assign(R + 1, 0);
+        for (int r = 0; r < R; r++) {
+            rblk[r] = (int)blocks.size();
+            for (int i = 0; i < na; i++)
+                for (int c0 = 0; c0 < reps[r].ncell; c0 += kBlock) blocks.push_back({r, i, c0, rdcls[r]});
+        }
+        rblk[R] = (int)blocks.size();
+        nblocks = (int)blocks.size();
+
+        // bonds: group by source sublattice (gather lists)
+        std::vector<int> order(nb);
+        for (int b = 0; b < nb; b++) {
+            order[b] = b;
+            const int* bd = sp.bond + 6 * b;
+            SNH_REQUIRE(bd[0] >= 0 && bd[0] < na && bd[1] >= 0 && bd[1] < na, SNH_ERR_INVALID,
+                        "bond atom index out of range");
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sp.bond[6 * a] < sp.bond[6 * b]; });
+        std::vector<int> bstart(na + 1, 0), bj(nb), bflag(nb);
+        std::vector<T> bJ(9 * (size_t)nb), bQ(25 * (size_t)nb), bsc(nb);
+        for (int k = 0; k < nb; k++) {
+            int b = order[k];
+            const int* bd = sp.bond + 6 * b;
+            bstart[bd[0] + 1]++;
+            bj[k] = bd[1];
+            const double* J = sp.bilin + 9 * b;
+            bool iso = J[1] == 0 && J[2] == 0 && J[3] == 0 && J[5] == 0 && J[6] == 0 && J[7] == 0 && J[0] == J[4] &&
+                       J[0] == J[8];
+            bool bq = sp.has_biquad && sp.has_biquad[b];
+            bflag[k] = (iso ? kBondIso : 0) | (bq ? kBondBiquad : 0) | (bd[5] ? kBondCulled : 0);
+            for (int p = 0; p < 9; p++) bJ[9 * (size_t)k + p] = (T)J[p];
+            for (int p = 0; p < 25; p++) bQ[25 * (size_t)k + p] = bq ? (T)sp.biquad[25 * b + p] : (T)0;
+            bsc[k] = sp.scalar ? (T)sp.scalar[b] : (T)0;
+        }
+        for (int i = 0; i < na; i++) bstart[i + 1] += bstart[i];
+        std::vector<int> bnred((size_t)dcls_dims.size() * nb * 3);
+        for (size_t d = 0; d < dcls_dims.size(); d++)
+            for (int k = 0; k < nb; k++)
+                for (int a = 0; a < 3; a++) {
+                    int L = dcls_dims[d][a];
+                    int n = sp.bond[6 * order[k] + 2 + a];
+                    bnred[(d * nb + k) * 3 + a] = ((n % L) + L) % L;
+                }
+        // per-sublattice tables
+        std::vector<T> kap(na), zee(3 * na), stev(28 * na), gm(9 * na), fld(3 * na);
+        std::vector<int> kmax(na, 0);
+        for (int i = 0; i < na; i++) {
+            kap[i] = (T)sp.kappa[i];
+            for (int p = 0; p < 9; p++) gm[9 * i + p] = (T)sp.g[9 * i + p];
+            for (int p = 0; p < 3; p++) fld[3 * i + p] = (T)sp.field[3 * i + p];
+            for (int b = 0; b < 3; b++) {
+                double z = 0;
+                for (int a = 0; a < 3; a++) z += sp.g[9 * i + 3 * a + b] * sp.field[3 * i + a];
+                zee[3 * i + b] = (T)z;
+            }
+            if (N == 0) {
+                SNH_REQUIRE(sp.stevens && sp.kmax, SNH_ERR_INVALID, ":dipole mode needs Stevens coefficients");
+                int km = sp.kmax[i];
+                SNH_REQUIRE(km == 0 || km == 2 || km == 4 || km == 6, SNH_ERR_INVALID, "kmax must be 0, 2, 4 or 6");
+                kmax[i] = km;
+                for (int p = 0; p < 28; p++) stev[28 * i + p] = (T)sp.stevens[28 * i + p];
+            }
+        }
+        std::vector<T> lam, qops;
+        if (N > 0) {
+            SNH_REQUIRE(sp.Lambda, SNH_ERR_INVALID, ":SUN mode needs the onsite matrices Lambda");
+            lam.resize((size_t)na * N * N * 2);
+            for (size_t k = 0; k < lam.size(); k++) lam[k] = (T)sp.Lambda[k];
+            auto q = quadrupole_ops(N);
+            qops.assign(q.begin(), q.end());
+        } else {
+            lam.assign(2, (T)0);
+            qops.assign(2, (T)0);
+        }
+
+        C.blocks = own(dupload(blocks));
+        C.reps = own(dupload(reps));
+        C.bstart = own(dupload(bstart));
+        C.bj = own(dupload(bj));
+        C.bflag = own(dupload(bflag));
+        C.bnred = own(dupload(bnred));
+        C.bJ = own(dupload(bJ));
+        C.bQ = own(dupload(bQ));
+        C.bscalar = own(dupload(bsc));
+        C.kappa = own(dupload(kap));
+        C.zee = own(dupload(zee));
+        C.stev = own(dupload(stev));
+        C.kmax = own(dupload(kmax));
+        C.gmat = own(dupload(gm));
+        C.field = own(dupload(fld));
+        C.lam = own(dupload(lam));
+        C.spin_ops = nullptr;
+        C.quad_ops = own(dupload(qops));
+        C.na = na;
+        C.nb = nb;
+        C.N = N;
+        C.total = total;
+        d_rblk = own(dupload(rblk));
+
+        X = own(dalloc<T>((size_t)planes * total));
+        Xp = own(dalloc<T>((size_t)planes * total));
+        Xo = own(dalloc<T>((size_t)planes * total));
+        SNH_HIP(hipMemset(X, 0, sizeof(T) * planes * total));
+        partial = own(dalloc<double>(nblocks));
+        d_out = own(dalloc<double>(R));
+        conv_iter = own(dalloc<int>(R));
+        atol2 = own(dalloc<double>(R));
+        nscale = own(dalloc<T>(R));
+        SNH_HIP(hipHostMalloc((void**)&pinned, sizeof(double) * R, hipHostMallocDefault));
+        SNH_HIP(hipHostMalloc((void**)&pinned_i, sizeof(int) * R, hipHostMallocDefault));
+        if (ewald) build_ewald(sp);
+    }
+
+    void build_ewald(const snh_spec& sp)
+    {
+        for (int r = 1; r < R; r++)
+            SNH_REQUIRE(reps[r].L1 == reps[0].L1 && reps[r].L2 == reps[0].L2 && reps[r].L3 == reps[0].L3,
+                        SNH_ERR_UNSUPPORTED, "Ewald requires all replicas to share the same dims");
+        SNH_REQUIRE(sp.positions, SNH_ERR_INVALID, "Ewald needs atom positions");
+        const int L[3] = {reps[0].L1, reps[0].L2, reps[0].L3};
+        for (int a = 0; a < 3; a++)
+            SNH_REQUIRE(L[a] <= 32, SNH_ERR_UNSUPPORTED, "Ewald FFT supports at most 32 cells per axis");
+        G.L1 = L[0];
+        G.L2 = L[1];
+        G.L3 = L[2];
+        G.H1 = L[0] / 2 + 1;
+        G.na = na;
+        G.ncell = (long long)L[0] * L[1] * L[2];
+        G.nk = (long long)G.H1 * L[1] * L[2];
+        SNH_REQUIRE(ewald_fft_lds_bytes(G.nk, 16) <= 65536, SNH_ERR_UNSUPPORTED,
+                    "Ewald FFT grid too large for one workgroup's LDS");
+        SNH_REQUIRE((size_t)R * 3 * na * sizeof(T) * 2 <= 65536, SNH_ERR_UNSUPPORTED,
+                    "too many replicas x sublattices for the Ewald contraction");
+
+        // A precompute parameters, Ewald.jl:61-91
+        EwaldAParams P{};
+        for (int p = 0; p < 9; p++) P.lv[p] = sp.latvecs[p];
+        for (int rr = 0; rr < 3; rr++)
+            for (int c = 0; c < 3; c++) P.SL[3 * rr + c] = sp.latvecs[3 * rr + c] * L[c];
+        const double* S = P.SL;
+        double det = S[0] * (S[4] * S[8] - S[5] * S[7]) - S[1] * (S[3] * S[8] - S[5] * S[6]) +
+                     S[2] * (S[3] * S[7] - S[4] * S[6]);
+        SNH_REQUIRE(std::fabs(det) > 0, SNH_ERR_INVALID, "singular lattice vectors");
+        double inv[9] = {(S[4] * S[8] - S[5] * S[7]) / det,  -(S[1] * S[8] - S[2] * S[7]) / det,
+                         (S[1] * S[5] - S[2] * S[4]) / det,  -(S[3] * S[8] - S[5] * S[6]) / det,
+                         (S[0] * S[8] - S[2] * S[6]) / det,  -(S[0] * S[5] - S[2] * S[3]) / det,
+                         (S[3] * S[7] - S[4] * S[6]) / det,  -(S[0] * S[7] - S[1] * S[6]) / det,
+                         (S[0] * S[4] - S[1] * S[3]) / det};
+        for (int rr = 0; rr < 3; rr++)
+            for (int c = 0; c < 3; c++) P.RL[3 * rr + c] = 2 * M_PI * inv[3 * c + rr];
+        for (int p = 0; p < 9; p++) P.demag[p] = sp.demag[p];
+        P.V = det;
+        P.sigma = std::cbrt(det) / 3;
+        P.rmax = 6 * std::sqrt(2.0) * P.sigma;
+        P.kmax = 6 * std::sqrt(2.0) / P.sigma;
+        for (int ax = 0; ax < 3; ax++) {
+            double av[3] = {P.SL[ax], P.SL[3 + ax], P.SL[6 + ax]};
+            double bv[3] = {P.RL[ax], P.RL[3 + ax], P.RL[6 + ax]};
+            double an = std::sqrt(av[0] * av[0] + av[1] * av[1] + av[2] * av[2]);
+            double bn = std::sqrt(bv[0] * bv[0] + bv[1] * bv[1] + bv[2] * bv[2]);
+            double adb = av[0] * bv[0] + av[1] * bv[1] + av[2] * bv[2];
+            P.nmax[ax] = (int)std::nearbyint(P.rmax / (adb / bn) + 1e-6) + 1;  // round(): ties to even
+            P.mmax[ax] = (int)std::nearbyint(P.kmax / (adb / an) + 1e-6);
+        }
+        P.mu0_muB2 = sp.mu0_muB2;
+        P.L1 = L[0];
+        P.L2 = L[1];
+        P.L3 = L[2];
+        P.na = na;
+        std::vector<double> pos(sp.positions, sp.positions + 3 * na);
+        double* d_pos = dupload(pos);
+        double* d_A = dalloc<double>((size_t)na * na * G.ncell * 9);
+        const int n3 = 3 * na;
+        M = own(dalloc<char>((size_t)G.nk * n3 * n3 * 2 * sizeof(T)));
+        launch_ewald_A(P, d_pos, d_A, st);
+        launch_ewald_fwdA<T>(G, d_A, M, st);
+        SNH_HIP(hipGetLastError());
+        SNH_HIP(hipStreamSynchronize(st));
+        SNH_HIP(hipFree(d_A));
+        SNH_HIP(hipFree(d_pos));
+        Fmu = own(dalloc<char>((size_t)R * n3 * G.nk * 2 * sizeof(T)));
+        Fphi = own(dalloc<char>((size_t)R * n3 * G.nk * 2 * sizeof(T)));
+        phi1 = own(dalloc<T>(3 * (size_t)total));
+        phi2 = own(dalloc<T>(3 * (size_t)total));
+        if (N > 0) Sd = own(dalloc<T>(3 * (size_t)total));
+    }
+
+    // --- timing helpers
+    hipEvent_t tbegin()
+    {
+        if (!timer.enabled) return nullptr;
+        hipEvent_t e = timer.get();
+        SNH_HIP(hipEventRecord(e, st));
+        return e;
+    }
+    void tend(const char* fam, hipEvent_t e0)
+    {
+        if (!e0) return;
+        hipEvent_t e1 = timer.get();
+        SNH_HIP(hipEventRecord(e1, st));
+        timer.events.push_back({fam, {e0, e1}});
+    }
+
+    // Ewald potential phi of state planes Xs (dipoles; expected dipoles for SU(N)).
+    void ewald_field(const T* Xs, T* phi)
+    {
+        const T* S = Xs;
+        if (N > 0) {
+            launch_sun_expected_spin<T>(C, nblocks, Xs, Sd, st);
+            S = Sd;
+        }
+        hipEvent_t e = tbegin();
+        launch_ewald_field<T>(G, R, C.reps, total, C.gmat, S, M, Fmu, Fphi, phi, st);
+        tend("ewald_field", e);
+    }
+
+    int64_t total_sites() const override { return total; }
+    int replicas() const override { return R; }
+    int64_t replica_sites(int r) const override { return (int64_t)reps[r].ncell * na; }
+
+    void set_state(const double* x, bool coherent) override
+    {
+        SNH_REQUIRE(coherent == (N > 0), SNH_ERR_INVALID,
+                    coherent ? "coherent states require :SUN mode" : "in :SUN mode set coherent states");
+        std::vector<T> h((size_t)planes * total);
+        for (long long g = 0; g < total; g++)
+            for (int p = 0; p < planes; p++) h[(size_t)p * total + g] = (T)x[(size_t)planes * g + p];
+        SNH_HIP(hipMemcpyAsync(X, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
+        SNH_HIP(hipStreamSynchronize(st));
+    }
+
+    void get_state(double* x, bool coherent) override
+    {
+        std::vector<T> h((size_t)planes * total);
+        if (coherent || N == 0) {
+            SNH_REQUIRE(!coherent || N > 0, SNH_ERR_INVALID, "coherent states require :SUN mode");
+            SNH_HIP(hipMemcpyAsync(h.data(), X, sizeof(T) * h.size(), hipMemcpyDeviceToHost, st));
+            SNH_HIP(hipStreamSynchronize(st));
+            for (long long g = 0; g < total; g++)
+                for (int p = 0; p < planes; p++) x[(size_t)planes * g + p] = (double)h[(size_t)p * total + g];
+        } else {
+            // dipoles of a coherent state: expected_spin(Z) (Integrators.jl:354)
+            T* tmp = Xp;  // stage buffer is free between steps
+            launch_sun_expected_spin<T>(C, nblocks, X, tmp, st);
+            SNH_HIP(hipGetLastError());
+            std::vector<T> d(3 * (size_t)total);
+            SNH_HIP(hipMemcpyAsync(d.data(), tmp, sizeof(T) * d.size(), hipMemcpyDeviceToHost, st));
+            SNH_HIP(hipStreamSynchronize(st));
+            for (long long g = 0; g < total; g++)
+                for (int p = 0; p < 3; p++) x[3 * g + p] = (double)d[(size_t)p * total + g];
+        }
+    }
+
+    void step_langevin(double dt, double damping, const double* kT, const double* noise, uint64_t seed,
+                       uint64_t step0, int nsteps) override
+    {
+        SNH_REQUIRE(std::isfinite(dt) && dt > 0, SNH_ERR_INVALID, "Select positive dt");
+        SNH_REQUIRE(damping >= 0, SNH_ERR_INVALID, "Select positive damping");
+        SNH_REQUIRE(kT, SNH_ERR_INVALID, "kT array required");
+        SNH_REQUIRE(nsteps >= 0, SNH_ERR_INVALID, "nsteps must be non-negative");
+        SNH_REQUIRE(!noise || nsteps == 1, SNH_ERR_INVALID, "host-supplied noise covers exactly one step");
+        std::vector<T> sc(R);
+        bool anynoise = false;
+        for (int r = 0; r < R; r++) {
+            SNH_REQUIRE(kT[r] >= 0, SNH_ERR_INVALID, "Select nonnegative kT");
+            double v = (damping == 0 || kT[r] == 0) ? 0.0 : std::sqrt(2 * dt * damping * kT[r]);
+            sc[r] = (T)v;
+            anynoise |= v != 0;
+        }
+        SNH_HIP(hipMemcpyAsync(nscale, sc.data(), sizeof(T) * R, hipMemcpyHostToDevice, st));
+        NoiseArgs<T> nz{};
+        nz.scale = nscale;
+        nz.seed = seed;
+        nz.per_site = planes;
+        nz.mode = anynoise ? (noise ? 1 : 2) : 0;
+        if (nz.mode == 1) {
+            if (!host_noise) host_noise = own(dalloc<double>((size_t)planes * total));
+            SNH_HIP(hipMemcpyAsync(host_noise, noise, sizeof(double) * planes * total, hipMemcpyHostToDevice, st));
+            nz.host = host_noise;
+        }
+        SNH_HIP(hipStreamSynchronize(st));  // host vectors above are pageable
+        const T tdt = (T)dt, tl = (T)damping;
+        for (int s = 0; s < nsteps; s++) {
+            nz.step = step0 + (uint64_t)s;
+            const T* ewS = nullptr;
+            const T* ewSp = nullptr;
+            if (ewald) {
+                ewald_field(X, phi1);
+                ewS = phi1;
+            }
+            hipEvent_t e = tbegin();
+            if (N == 0)
+                launch_dipole_predict<T>(C, nblocks, X, Xp, ewS, nz, tdt, tl, st);
+            else
+                launch_sun_predict<T>(C, nblocks, X, Xp, ewS, nz, tdt, tl, st);
+            tend(N == 0 ? "dipole_predict" : "sun_predict", e);
+            if (ewald) {
+                ewald_field(Xp, phi2);
+                ewSp = phi2;
+            }
+            e = tbegin();
+            if (N == 0)
+                launch_dipole_correct<T>(C, nblocks, X, Xp, Xo, ewS, ewSp, nz, tdt, tl, st);
+            else
+                launch_sun_correct<T>(C, nblocks, X, Xp, Xo, ewS, ewSp, nz, tdt, tl, st);
+            tend(N == 0 ? "dipole_correct" : "sun_correct", e);
+            std::swap(X, Xo);
+        }
+        SNH_HIP(hipGetLastError());
+    }
+
+    void step_midpoint(double dt, double damping, double tol, int max_iters, int nsteps, int* iters) override
+    {
+        SNH_REQUIRE(std::isfinite(dt) && dt > 0, SNH_ERR_INVALID, "Select positive dt");
+        SNH_REQUIRE(damping >= 0, SNH_ERR_INVALID, "Select nonnegative damping");
+        SNH_REQUIRE(max_iters >= 1, SNH_ERR_INVALID, "max_iters must be at least 1");
+        if (!B1) {
+            B1 = own(dalloc<T>((size_t)planes * total));
+            B2 = own(dalloc<T>((size_t)planes * total));
+            if (ewald) Bh = own(dalloc<T>((size_t)planes * total));
+        }
+        // atol = tol * sqrt(length(S)), length counting sites (Integrators.jl:383)
+        std::vector<double> a2(R);
+        for (int r = 0; r < R; r++) {
+            double at = tol * std::sqrt((double)replica_sites(r));
+            a2[r] = at * at;
+        }
+        SNH_HIP(hipMemcpyAsync(atol2, a2.data(), sizeof(double) * R, hipMemcpyHostToDevice, st));
+        SNH_HIP(hipStreamSynchronize(st));
+        const T tdt = (T)dt, tl = (T)damping;
+        T* bufs[2] = {B2, B1};  // S'' of iteration m lives in bufs[m % 2]: m odd -> B1
+        for (int s = 0; s < nsteps; s++) {
+            SNH_HIP(hipMemsetAsync(conv_iter, 0, sizeof(int) * R, st));
+            SNH_HIP(hipMemcpyAsync(B2, X, sizeof(T) * planes * total, hipMemcpyDeviceToDevice, st));  // S' = S
+            int it = 0;
+            bool done = false;
+            int chunk = std::max(1, std::min(last_iters, max_iters));
+            while (!done) {
+                int upto = std::min(max_iters, it + chunk);
+                for (; it < upto;) {
+                    ++it;
+                    T* S1 = bufs[(it + 1) % 2];
+                    T* S2 = bufs[it % 2];
+                    const T* phi = nullptr;
+                    const T* hat = nullptr;
+                    if (ewald) {
+                        if (N == 0) {
+                            launch_dipole_midpoint_hat<T>(C, nblocks, X, S1, Bh, conv_iter, st);
+                            hat = Bh;
+                        } else {
+                            launch_sun_midpoint_bar<T>(C, nblocks, X, S1, Bh, conv_iter, st);
+                        }
+                        ewald_field(Bh, phi1);
+                        phi = phi1;
+                    }
+                    hipEvent_t e = tbegin();
+                    if (N == 0)
+                        launch_dipole_midpoint_iter<T>(C, nblocks, X, S1, S2, hat, phi, tdt, tl, partial, conv_iter,
+                                                       it, st);
+                    else
+                        launch_sun_midpoint_iter<T>(C, nblocks, X, S1, S2, phi, tdt, tl, partial, conv_iter, it, st);
+                    tend("midpoint_iter", e);
+                    launch_midpoint_check(partial, d_rblk, R, atol2, conv_iter, it, st);
+                    if (N == 0)
+                        launch_dipole_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, it, st);
+                    else
+                        launch_sun_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, it, st);
+                }
+                SNH_HIP(hipGetLastError());
+                SNH_HIP(hipMemcpyAsync(pinned_i, conv_iter, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+                SNH_HIP(hipStreamSynchronize(st));
+                bool all = true;
+                for (int r = 0; r < R; r++) all &= pinned_i[r] != 0;
+                if (all) {
+                    done = true;
+                } else if (it >= max_iters) {
+                    char msg[256];
+                    if (N == 0)
+                        snprintf(msg, sizeof msg,
+                                 "Spherical midpoint method failed to converge to tolerance %g after %d iterations.",
+                                 tol, max_iters);
+                    else
+                        snprintf(msg, sizeof msg, "Schrödinger midpoint method failed to converge in %d iterations.",
+                                 max_iters);
+                    throw Failure{SNH_ERR_NOCONVERGE, msg};
+                } else {
+                    chunk = 1;
+                }
+            }
+            int mx = 0;
+            for (int r = 0; r < R; r++) mx = std::max(mx, pinned_i[r]);
+            last_iters = mx;
+            if (iters)
+                for (int r = 0; r < R; r++) iters[r] = pinned_i[r];
+        }
+    }
+
+    void energy(double* E) override
+    {
+        const T* phi = nullptr;
+        if (ewald) {
+            ewald_field(X, phi1);
+            phi = phi1;
+        }
+        if (N == 0)
+            launch_dipole_energy<T>(C, nblocks, X, phi, partial, st);
+        else
+            launch_sun_energy<T>(C, nblocks, X, phi, partial, st);
+        launch_reduce_partials(partial, d_rblk, R, d_out, st);
+        SNH_HIP(hipGetLastError());
+        SNH_HIP(hipMemcpyAsync(pinned, d_out, sizeof(double) * R, hipMemcpyDeviceToHost, st));
+        SNH_HIP(hipStreamSynchronize(st));
+        for (int r = 0; r < R; r++) E[r] = pinned[r];
+    }
+
+    void energy_grad(double* out) override
+    {
+        const T* phi = nullptr;
+        if (ewald) {
+            ewald_field(X, phi1);
+            phi = phi1;
+        }
+        if (N == 0)
+            launch_dipole_grad<T>(C, nblocks, X, phi, Xp, st);
+        else
+            launch_sun_grad<T>(C, nblocks, X, phi, Xp, st);
+        SNH_HIP(hipGetLastError());
+        std::vector<T> h((size_t)planes * total);
+        SNH_HIP(hipMemcpyAsync(h.data(), Xp, sizeof(T) * h.size(), hipMemcpyDeviceToHost, st));
+        SNH_HIP(hipStreamSynchronize(st));
+        for (long long g = 0; g < total; g++)
+            for (int p = 0; p < planes; p++) out[(size_t)planes * g + p] = (double)h[(size_t)p * total + g];
+    }
+
+    void synchronize() override { SNH_HIP(hipStreamSynchronize(st)); }
+
+    void enable_timing(bool on) override
+    {
+        SNH_HIP(hipStreamSynchronize(st));
+        timer.clear();
+        timer.enabled = on;
+    }
+
+    void kernel_time(const char* fam, double* ms, int64_t* n) override
+    {
+        SNH_HIP(hipStreamSynchronize(st));
+        double acc = 0;
+        int64_t cnt = 0;
+        for (auto& ev : timer.events) {
+            if (ev.first != fam) continue;
+            float t = 0;
+            SNH_HIP(hipEventElapsedTime(&t, ev.second.first, ev.second.second));
+            acc += t;
+            cnt++;
+        }
+        *ms = acc;
+        *n = cnt;
+    }
+};
+
+int fail(snh_system* sys, const Failure& f)
+{
+    if (sys) sys->err = f.msg;
+    return f.code;
+}
+
+template <typename F>
+int guarded(snh_system* sys, F&& fn)
+{
+    if (!sys) return SNH_ERR_INVALID;
+    try {
+        fn();
+        return SNH_OK;
+    } catch (const Failure& f) {
+        return fail(sys, f);
+    } catch (const std::exception& e) {
+        return fail(sys, Failure{SNH_ERR_INVALID, e.what()});
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// extern "C" entry points
+
+extern "C" int snh_create(const snh_spec* spec, snh_system** out)
+{
+    if (!spec || !out) {
+        g_create_error = "null argument";
+        return SNH_ERR_INVALID;
+    }
+    *out = nullptr;
+    snh_system* sys = nullptr;
+    try {
+        SNH_REQUIRE(spec->na >= 1, SNH_ERR_INVALID, "na must be at least 1");
+        SNH_REQUIRE(spec->N == 0 || (spec->N >= 2 && spec->N <= 7), SNH_ERR_UNSUPPORTED,
+                    "supported modes: :dipole (N = 0) and :SUN with 2 <= N <= 7");
+        SNH_REQUIRE(spec->precision == 32 || spec->precision == 64, SNH_ERR_INVALID, "precision must be 32 or 64");
+        SNH_REQUIRE(spec->nreplicas >= 1 && spec->dims, SNH_ERR_INVALID, "need at least one replica");
+        SNH_REQUIRE(spec->kappa && spec->g && spec->field, SNH_ERR_INVALID, "kappa, g and field are required");
+        SNH_REQUIRE(spec->nbonds == 0 || (spec->bond && spec->bilin), SNH_ERR_INVALID, "bond tables missing");
+        int ndev = 0;
+        SNH_HIP(hipGetDeviceCount(&ndev));
+        SNH_REQUIRE(ndev > 0, SNH_ERR_HIP, "no HIP device");
+        if (spec->precision == 64) {
+            auto* s = new SystemImpl<double>();
+            sys = s;
+            s->build(*spec);
+        } else {
+            auto* s = new SystemImpl<float>();
+            sys = s;
+            s->build(*spec);
+        }
+        *out = sys;
+        return SNH_OK;
+    } catch (const Failure& f) {
+        g_create_error = f.msg;
+        delete sys;
+        return f.code;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        delete sys;
+        return SNH_ERR_INVALID;
+    }
+}
+
+extern "C" int snh_destroy(snh_system* sys)
+{
+    delete sys;
+    return SNH_OK;
+}
+
+extern "C" const char* snh_last_error(const snh_system* sys)
+{
+    return sys ? sys->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" int snh_num_sites(const snh_system* sys, int64_t* total)
+{
+    if (!sys || !total) return SNH_ERR_INVALID;
+    *total = sys->total_sites();
+    return SNH_OK;
+}
+
+extern "C" int snh_replica_sites(const snh_system* sys, int32_t r, int64_t* n)
+{
+    if (!sys || !n) return SNH_ERR_INVALID;
+    auto* s = const_cast<snh_system*>(sys);
+    return guarded(s, [&] {
+        SNH_REQUIRE(r >= 0 && r < sys->replicas(), SNH_ERR_INVALID, "replica index out of range");
+        *n = sys->replica_sites(r);
+    });
+}
+
+extern "C" int snh_set_dipoles(snh_system* sys, const double* x)
+{
+    return guarded(sys, [&] {
+        SNH_REQUIRE(x, SNH_ERR_INVALID, "null state");
+        sys->set_state(x, false);
+    });
+}
+
+extern "C" int snh_get_dipoles(snh_system* sys, double* x)
+{
+    return guarded(sys, [&] {
+        SNH_REQUIRE(x, SNH_ERR_INVALID, "null state");
+        sys->get_state(x, false);
+    });
+}
+
+extern "C" int snh_set_coherents(snh_system* sys, const double* x)
+{
+    return guarded(sys, [&] {
+        SNH_REQUIRE(x, SNH_ERR_INVALID, "null state");
+        sys->set_state(x, true);
+    });
+}
+
+extern "C" int snh_get_coherents(snh_system* sys, double* x)
+{
+    return guarded(sys, [&] {
+        SNH_REQUIRE(x, SNH_ERR_INVALID, "null state");
+        sys->get_state(x, true);
+    });
+}
+
+extern "C" int snh_step_langevin(snh_system* sys, double dt, double damping, const double* kT, const double* noise,
+                                 uint64_t seed, uint64_t step0, int32_t nsteps)
+{
+    return guarded(sys, [&] { sys->step_langevin(dt, damping, kT, noise, seed, step0, nsteps); });
+}
+
+extern "C" int snh_step_midpoint(snh_system* sys, double dt, double damping, double tol, int32_t max_iters,
+                                 int32_t nsteps, int32_t* iters_out)
+{
+    return guarded(sys, [&] { sys->step_midpoint(dt, damping, tol, max_iters, nsteps, iters_out); });
+}
+
+extern "C" int snh_energy(snh_system* sys, double* E)
+{
+    return guarded(sys, [&] {
+        SNH_REQUIRE(E, SNH_ERR_INVALID, "null output");
+        sys->energy(E);
+    });
+}
+
+extern "C" int snh_energy_grad(snh_system* sys, double* out)
+{
+    return guarded(sys, [&] {
+        SNH_REQUIRE(out, SNH_ERR_INVALID, "null output");
+        sys->energy_grad(out);
+    });
+}
+
+extern "C" int snh_synchronize(snh_system* sys)
+{
+    return guarded(sys, [&] { sys->synchronize(); });
+}
+
+extern "C" int snh_enable_timing(snh_system* sys, int32_t enable)
+{
+    return guarded(sys, [&] { sys->enable_timing(enable != 0); });
+}
+
+extern "C" int snh_kernel_time(snh_system* sys, const char* fam, double* ms, int64_t* n)
+{
+    return guarded(sys, [&] {
+        SNH_REQUIRE(fam && ms && n, SNH_ERR_INVALID, "null argument");
+        sys->kernel_time(fam, ms, n);
+    });
+}

@@ -1,0 +1,18 @@
+"""sunnyhip: MI355X-native spin-dynamics hot path of Sunny.jl.
+
+Host-side mirror of Sunny's `System` / `Langevin` / `ImplicitMidpoint` /
+`step!` API over the C ABI of libsunnyhip.so (include/sunny_hip.h).
+"""
+from .crystal import (Bond, Crystal, bond_parity, cubic_crystal, diamond_crystal, lattice_vectors,
+                      pyrochlore_crystal, pyrochlore_local_axes, triangular_crystal)
+from .integrators import (ConvergenceError, ImplicitMidpoint, Langevin, ReplicaBatch, energy, energy_grad_coherents,
+                          energy_grad_dipoles, energy_per_site, step_)
+from .operators import (SCALAR_BIQUAD_METRIC, StevensExpansion, matrix_to_stevens_coefficients, rcs_factors,
+                        spin_matrices, stevens_matrices)
+from .system import (VACUUM_PERMEABILITY, Moment, System, enable_dipole_dipole_, expected_spin_host,
+                     ket_from_dipole, polarize_spins_, randomize_spins_, set_coherent_, set_dipole_, set_exchange_,
+                     set_field_, set_onsite_coupling_, set_pair_biquad_, set_spin_rescaling_,
+                     set_stevens_expansion_)
+from . import workloads  # noqa: F401
+
+__all__ = [n for n in dir() if not n.startswith("_")]

@@ -1,0 +1,144 @@
+"""ctypes binding of libsunnyhip.so (declarations: include/sunny_hip.h).
+
+The library is the only compute path.  Loading fails loudly when it has not
+been built; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsunnyhip.so")
+
+SNH_OK = 0
+SNH_ERR_INVALID = 1
+SNH_ERR_HIP = 2
+SNH_ERR_NOCONVERGE = 3
+SNH_ERR_UNSUPPORTED = 4
+
+EXPORTS = [
+    "snh_create", "snh_destroy", "snh_last_error", "snh_num_sites", "snh_replica_sites",
+    "snh_set_dipoles", "snh_get_dipoles", "snh_set_coherents", "snh_get_coherents",
+    "snh_step_langevin", "snh_step_midpoint", "snh_energy", "snh_energy_grad", "snh_synchronize",
+    "snh_enable_timing", "snh_kernel_time",
+]
+
+
+class SnhSpec(C.Structure):
+    _fields_ = [
+        ("na", C.c_int32), ("N", C.c_int32), ("precision", C.c_int32),
+        ("kappa", C.POINTER(C.c_double)), ("g", C.POINTER(C.c_double)), ("field", C.POINTER(C.c_double)),
+        ("stevens", C.POINTER(C.c_double)), ("kmax", C.POINTER(C.c_int32)), ("Lambda", C.POINTER(C.c_double)),
+        ("nbonds", C.c_int32), ("bond", C.POINTER(C.c_int32)), ("bilin", C.POINTER(C.c_double)),
+        ("biquad", C.POINTER(C.c_double)), ("has_biquad", C.POINTER(C.c_int32)), ("scalar", C.POINTER(C.c_double)),
+        ("ewald", C.c_int32), ("mu0_muB2", C.c_double), ("demag", C.c_double * 9), ("latvecs", C.c_double * 9),
+        ("positions", C.POINTER(C.c_double)), ("nreplicas", C.c_int32), ("dims", C.POINTER(C.c_int32)),
+    ]
+
+
+class HipError(RuntimeError):
+    pass
+
+
+class ConvergenceError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libsunnyhip.so once; raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"libsunnyhip.so not found at {LIB_PATH}; build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        L.snh_create.argtypes = [C.POINTER(SnhSpec), C.POINTER(vp)]
+        L.snh_destroy.argtypes = [vp]
+        L.snh_last_error.argtypes = [vp]
+        L.snh_last_error.restype = C.c_char_p
+        L.snh_num_sites.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.snh_replica_sites.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
+        for f in ("snh_set_dipoles", "snh_get_dipoles", "snh_set_coherents", "snh_get_coherents",
+                  "snh_energy", "snh_energy_grad"):
+            getattr(L, f).argtypes = [vp, dp]
+        L.snh_step_langevin.argtypes = [vp, C.c_double, C.c_double, dp, dp, C.c_uint64, C.c_uint64, C.c_int32]
+        L.snh_step_midpoint.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                        C.POINTER(C.c_int32)]
+        L.snh_synchronize.argtypes = [vp]
+        L.snh_enable_timing.argtypes = [vp, C.c_int32]
+        L.snh_kernel_time.argtypes = [vp, C.c_char_p, dp, C.POINTER(C.c_int64)]
+        for f in EXPORTS:
+            if f != "snh_last_error":
+                getattr(L, f).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def dptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def check(rc, handle=None):
+    if rc == SNH_OK:
+        return
+    msg = lib().snh_last_error(handle).decode()
+    if rc == SNH_ERR_NOCONVERGE:
+        raise ConvergenceError(msg)
+    if rc == SNH_ERR_HIP:
+        raise HipError(msg)
+    raise ValueError(msg)
+
+
+class Handle:
+    """Owns one snh_system*; keeps the spec arrays alive for its lifetime."""
+
+    def __init__(self, spec: dict):
+        L = lib()
+        self._keep = spec
+        s = SnhSpec()
+        s.na, s.N, s.precision = spec["na"], spec["N"], spec["precision"]
+        s.kappa, s.g, s.field = dptr(spec["kappa"]), dptr(spec["g"]), dptr(spec["field"])
+        s.stevens, s.kmax, s.Lambda = dptr(spec["stevens"]), iptr(spec["kmax"]), dptr(spec["Lambda"])
+        s.nbonds = spec["nbonds"]
+        s.bond, s.bilin, s.biquad = iptr(spec["bond"]), dptr(spec["bilin"]), dptr(spec["biquad"])
+        s.has_biquad, s.scalar = iptr(spec["has_biquad"]), dptr(spec["scalar"])
+        s.ewald = spec["ewald"]
+        s.mu0_muB2 = spec["mu0_muB2"]
+        s.demag[:] = list(spec["demag"])
+        s.latvecs[:] = list(spec["latvecs"])
+        s.positions = dptr(spec["positions"])
+        s.nreplicas = spec["nreplicas"]
+        s.dims = iptr(spec["dims"])
+        h = C.c_void_p()
+        rc = L.snh_create(C.byref(s), C.byref(h))
+        if rc != SNH_OK:
+            check(rc, None)
+        self.h = h
+        n = C.c_int64()
+        check(L.snh_num_sites(self.h, C.byref(n)), self.h)
+        self.total = n.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.snh_destroy(self.h)
+            self.h = None
+
+    def call(self, name, *args):
+        check(getattr(lib(), name)(self.h, *args), self.h)

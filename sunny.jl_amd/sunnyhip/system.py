@@ -1,0 +1,403 @@
+"""Host-side mirror of Sunny's `System` and its coupling setters.
+
+Names follow Sunny (src/System/System.jl, src/System/PairExchange.jl,
+src/System/OnsiteCoupling.jl, src/System/Interactions.jl) with Julia's `!`
+spelled `_`: `set_exchange!` -> `set_exchange_`, `step!` -> `step_`.
+Couplings are assembled here on the host (setup); every energy, gradient and
+time step runs in libsunnyhip on the GPU through `sunnyhip._lib`.
+
+Spin state layout matches Julia's column-major `Array{Vec3,4}(L1,L2,L3,na)`:
+the flat buffer is `[site][xyz]` with site = c1 + L1*(c2 + L2*(c3 + L3*i)),
+and `sys.dipoles` is a view indexed `[c1, c2, c3, i, :]` (0-based).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from .crystal import Bond, Crystal, bond_parity
+from .operators import (SCALAR_BIQUAD_METRIC, StevensExpansion, matrix_to_stevens_coefficients, rcs_factors,
+                        spin_matrices)
+
+VACUUM_PERMEABILITY = 0.6745817653324668  # mu0 muB^2 in meV*A^3, src/Units.jl:91
+
+
+class Moment:
+    """Spin label s and g-tensor of one atom (src/System/Moment.jl)."""
+
+    def __init__(self, s, g):
+        self.s = float(s)
+        g = np.asarray(g, dtype=float)
+        self.g = g * np.eye(3) if g.ndim == 0 else g.reshape(3, 3)
+
+
+def _julia_view(flat, dims, na, width):
+    L1, L2, L3 = dims
+    return flat.reshape(na, L3, L2, L1, width).transpose(3, 2, 1, 0, 4)
+
+
+class System:
+    """A supercell of `crystal` with per-sublattice moments and couplings.
+
+    mode: "dipole" or "SUN" (src/System/System.jl:33-93).  precision selects
+    the device kernels: 64 (plain FP64) or 32 (FP32 arithmetic with FP64
+    renormalisation, residual and energy sums).
+    """
+
+    def __init__(self, crystal: Crystal, moments, mode: str, dims=(1, 1, 1), seed=None, precision=64):
+        if mode not in ("dipole", "SUN"):
+            raise ValueError("Mode must be `:SUN`, `:dipole`")
+        self.crystal = crystal
+        self.mode = mode
+        self.dims = tuple(int(d) for d in dims)
+        na = crystal.natoms
+        self.na = na
+        mom = dict(moments)
+        if sorted(mom) != list(range(na)):
+            raise ValueError("a Moment is required for every atom (0-based)")
+        self.spins = np.array([mom[i].s for i in range(na)])
+        self.Ns_atom = np.rint(2 * self.spins + 1).astype(int)
+        self.gs = np.array([mom[i].g for i in range(na)])
+        if mode == "SUN":
+            if len(set(self.Ns_atom)) != 1:
+                raise ValueError("Currently all spins S must be equal in SU(N) mode")
+            self.N = int(self.Ns_atom[0])
+            self.kappas = np.ones(na)
+        else:
+            self.N = 0
+            self.kappas = self.spins.copy()
+        self.extfield = np.zeros((na, 3))
+        self.onsite = [StevensExpansion.zero() if self.N == 0 else np.zeros((self.N, self.N), complex)
+                       for _ in range(na)]
+        self.pairs = {}  # Bond -> [scalar, bilin 3x3, biquad 5x5 or None]
+        self.ewald = None
+        self.precision = int(precision)
+        self.rng = np.random.default_rng(seed)
+        self.noise_seed = int(self.rng.integers(0, 2**63))
+        self.step_counter = 0
+        ns = self.nsites
+        self._dip = np.zeros((ns, 3))
+        self._coh = np.zeros((ns, max(self.N, 1)), complex)
+        self._handle = None
+        self._host_dirty = True   # host state newer than the device copy
+        self._dev_dirty = False   # device state newer than the host copy
+        polarize_spins_(self, (0, 0, 1))
+
+    # ---- sizes and state views
+    @property
+    def ncells(self):
+        return int(np.prod(self.dims))
+
+    @property
+    def nsites(self):
+        return self.ncells * self.na
+
+    def _pull(self):
+        if self._dev_dirty:
+            h = self._handle
+            d = np.empty((h.total, 3))
+            h.call("snh_get_dipoles", _lib.dptr(d))
+            self._dip[:] = d
+            if self.N > 0:
+                z = np.empty((h.total, self.N, 2))
+                h.call("snh_get_coherents", _lib.dptr(z))
+                self._coh[:] = z[..., 0] + 1j * z[..., 1]
+            self._dev_dirty = False
+
+    @property
+    def dipoles(self):
+        """View [c1, c2, c3, i, xyz]; reading may be followed by writes, so the
+        next device call re-uploads the host state."""
+        self._pull()
+        self._host_dirty = True
+        return _julia_view(self._dip, self.dims, self.na, 3)
+
+    @property
+    def coherents(self):
+        self._pull()
+        self._host_dirty = True
+        return _julia_view(self._coh, self.dims, self.na, self.N)
+
+    def dipoles_flat(self):
+        self._pull()
+        return self._dip.copy()
+
+    def coherents_flat(self):
+        self._pull()
+        return self._coh.copy()
+
+    def set_state_flat(self, dip=None, coh=None):
+        self._pull()
+        if coh is not None:
+            self._coh[:] = np.asarray(coh).reshape(self.nsites, self.N)
+            self._dip[:] = expected_spin_host(self._coh, self.N)
+        elif dip is not None:
+            self._dip[:] = np.asarray(dip).reshape(self.nsites, 3)
+        self._host_dirty = True
+
+    def site_index(self, site):
+        c1, c2, c3, i = site
+        L1, L2, L3 = self.dims
+        return c1 + L1 * (c2 + L2 * (c3 + L3 * i))
+
+    def _invalidate(self):
+        if self._handle is not None:
+            self._pull()
+            self._handle = None
+            self._host_dirty = True
+
+    # ---- coupling tables (the C-ABI spec)
+    def spec(self, precision=None, dims_list=None):
+        """Arrays describing the couplings, as consumed by snh_create."""
+        na, N = self.na, self.N
+        stev = np.zeros((na, 28))
+        kmax = np.zeros(na, np.int32)
+        lam = np.zeros((na, max(N, 1), max(N, 1), 2))
+        for i in range(na):
+            if N == 0:
+                stev[i] = self.onsite[i].packed()
+                kmax[i] = self.onsite[i].kmax
+            else:
+                lam[i, :, :, 0] = self.onsite[i].real
+                lam[i, :, :, 1] = self.onsite[i].imag
+        bonds = sorted(self.pairs.items(), key=lambda kv: (kv[0].i, bond_parity(kv[0]), kv[0].j, kv[0].n))
+        nb = len(bonds)
+        bond = np.zeros((max(nb, 1), 6), np.int32)
+        bilin = np.zeros((max(nb, 1), 9))
+        biquad = np.zeros((max(nb, 1), 25))
+        has_bq = np.zeros(max(nb, 1), np.int32)
+        scalar = np.zeros(max(nb, 1))
+        for k, (b, (sc, J, B)) in enumerate(bonds):
+            bond[k] = (b.i, b.j, *b.n, int(bond_parity(b)))
+            bilin[k] = np.asarray(J, float).reshape(9)
+            if B is not None:
+                biquad[k] = np.asarray(B, float).reshape(25)
+                has_bq[k] = 1
+            scalar[k] = sc
+        dims_list = [self.dims] if dims_list is None else dims_list
+        ew = self.ewald
+        return {
+            "na": na, "N": N, "precision": int(precision or self.precision),
+            "kappa": np.ascontiguousarray(self.kappas, float), "g": np.ascontiguousarray(self.gs.reshape(na, 9)),
+            "field": np.ascontiguousarray(self.extfield), "stevens": stev, "kmax": kmax,
+            "Lambda": np.ascontiguousarray(lam), "nbonds": nb, "bond": bond, "bilin": bilin, "biquad": biquad,
+            "has_biquad": has_bq, "scalar": scalar, "ewald": int(ew is not None),
+            "mu0_muB2": float(ew[0]) if ew else 0.0,
+            "demag": np.asarray(ew[1] if ew else np.zeros((3, 3)), float).reshape(9),
+            "latvecs": self.crystal.latvecs.reshape(9).copy(),
+            "positions": np.ascontiguousarray(self.crystal.positions, float),
+            "nreplicas": len(dims_list), "dims": np.ascontiguousarray(np.array(dims_list, np.int32).reshape(-1, 3)),
+        }
+
+    # ---- device copy
+    def _device(self):
+        if self._handle is None:
+            self._handle = _lib.Handle(self.spec())
+            self._host_dirty = True
+        if self._host_dirty:
+            h = self._handle
+            if self.N == 0:
+                h.call("snh_set_dipoles", _lib.dptr(np.ascontiguousarray(self._dip)))
+            else:
+                z = np.ascontiguousarray(np.stack([self._coh.real, self._coh.imag], -1))
+                h.call("snh_set_coherents", _lib.dptr(z))
+            self._host_dirty = False
+        return self._handle
+
+    def __repr__(self):
+        mode = "[Dipole mode]" if self.N == 0 else f"[SU({self.N})]"
+        return f"System({mode}, Supercell ({'x'.join(map(str, self.dims))})x{self.na})"
+
+
+# ---------------------------------------------------------------------------
+# Host-side SU(N) helpers for setting states (src/Operators/Spin.jl:72-116)
+
+def expected_spin_host(Z, N):
+    Sx, Sy, Sz = spin_matrices((N - 1) / 2)
+    Z = np.asarray(Z)
+    return np.stack([np.einsum("si,ij,sj->s", Z.conj(), S, Z).real for S in (Sx, Sy, Sz)], -1)
+
+
+def ket_from_dipole(d, N):
+    Sx, Sy, Sz = spin_matrices((N - 1) / 2)
+    w, v = np.linalg.eigh(d[0] * Sx + d[1] * Sy + d[2] * Sz)
+    return v[:, np.argmax(w)]
+
+
+def _normalize_ket(Z, kappa):
+    return np.zeros_like(Z) if kappa == 0 else Z / np.sqrt(np.vdot(Z, Z).real / kappa)
+
+
+def _normalize_dipole(s, kappa):
+    return np.zeros(3) if kappa == 0 else kappa * s / np.linalg.norm(s)
+
+
+# ---------------------------------------------------------------------------
+# State setters (src/System/System.jl:425-570)
+
+def set_dipole_(sys: System, direction, site):
+    sys._pull()
+    k = sys.site_index(site)
+    kappa = sys.kappas[site[3]]
+    if sys.N == 0:
+        sys._dip[k] = _normalize_dipole(np.asarray(direction, float), kappa)
+    else:
+        Z = _normalize_ket(ket_from_dipole(np.asarray(direction, float), sys.N), kappa)
+        sys._coh[k] = Z
+        sys._dip[k] = expected_spin_host(Z[None], sys.N)[0]
+    sys._host_dirty = True
+
+
+def set_coherent_(sys: System, Z, site):
+    if sys.N == 0:
+        raise ValueError("Cannot set zero-length coherent state")
+    sys._pull()
+    k = sys.site_index(site)
+    Z = _normalize_ket(np.asarray(Z, complex), sys.kappas[site[3]])
+    sys._coh[k] = Z
+    sys._dip[k] = expected_spin_host(Z[None], sys.N)[0]
+    sys._host_dirty = True
+
+
+def polarize_spins_(sys: System, direction):
+    sys._pull()
+    d = np.asarray(direction, float)
+    kap = np.repeat(sys.kappas, sys.ncells)
+    if sys.N == 0:
+        sys._dip[:] = kap[:, None] * (d / np.linalg.norm(d))
+    else:
+        Z = ket_from_dipole(d, sys.N)
+        sys._coh[:] = Z[None, :] / np.sqrt(np.vdot(Z, Z).real / kap[:, None])
+        sys._dip[:] = expected_spin_host(sys._coh, sys.N)
+    sys._host_dirty = True
+
+
+def randomize_spins_(sys: System):
+    """Spins uniform on the sphere (dipoles) or normalised complex Gaussian
+    kets (SU(N)), drawn from the system's own generator (System.jl:444-452)."""
+    sys._pull()
+    kap = np.repeat(sys.kappas, sys.ncells)
+    if sys.N == 0:
+        v = sys.rng.standard_normal((sys.nsites, 3))
+        sys._dip[:] = kap[:, None] * v / np.linalg.norm(v, axis=1, keepdims=True)
+    else:
+        z = (sys.rng.standard_normal((sys.nsites, sys.N)) + 1j * sys.rng.standard_normal((sys.nsites, sys.N))) / np.sqrt(2)
+        sys._coh[:] = z / np.sqrt((np.abs(z) ** 2).sum(1, keepdims=True) / kap[:, None])
+        sys._dip[:] = expected_spin_host(sys._coh, sys.N)
+    sys._host_dirty = True
+
+
+# ---------------------------------------------------------------------------
+# Coupling setters
+
+def set_field_(sys: System, B):
+    """External field for every site (src/System/Interactions.jl:191-212)."""
+    sys.extfield[:] = np.asarray(B, float)
+    sys._invalidate()
+
+
+def _to_float_or_mat3(J):
+    J = np.asarray(J, float)
+    if J.ndim == 0:
+        return float(J)
+    J = J.reshape(3, 3)
+    if np.allclose(J, J[0, 0] * np.eye(3), rtol=1e-12, atol=0):
+        return float(J[0, 0])
+    return J
+
+
+def set_exchange_(sys: System, J, bond: Bond, biquad=0.0, propagate=None):
+    """Exchange S_i.J S_j (+ biquad (S_i.S_j)^2) on `bond` and its equivalents
+    (src/System/PairExchange.jl:379-385, 322-340, 185-260).
+
+    Scalar J propagates to every bond of the same distance shell (exact for
+    Heisenberg couplings).  Matrix J applies to `bond` in every cell (pass
+    `propagate=[bonds...]` with pre-rotated matrices for symmetry partners)."""
+    if bond.i == bond.j and tuple(bond.n) == (0, 0, 0):
+        raise ValueError("Bond must connect different atoms.")
+    Jv = _to_float_or_mat3(J)
+    scalar = 0.0
+    biquad = float(biquad)
+    if biquad != 0.0:
+        # adapt_for_biquad: (Si.Sj)^2 = Qi.g Qj - Si.Sj/2 + s1(s1+1)s2(s2+1)/3
+        s1, s2 = sys.spins[bond.i], sys.spins[bond.j]
+        Jv = Jv - biquad / 2 if isinstance(Jv, float) else Jv - (biquad / 2) * np.eye(3)
+        scalar += biquad * s1 * (s1 + 1) * s2 * (s2 + 1) / 3
+    if isinstance(Jv, float):
+        targets = sys.crystal.equivalent_bonds(bond) if propagate is None else list(propagate)
+    else:
+        targets = [bond] if propagate is None else list(propagate)
+    for b in targets:
+        Jm = Jv * np.eye(3) if isinstance(Jv, float) else np.asarray(Jv)
+        bq = None
+        if biquad != 0.0:
+            f = biquad
+            if sys.mode == "dipole":
+                f *= rcs_factors(sys.spins[b.i])[2] * rcs_factors(sys.spins[b.j])[2]
+            bq = f * np.diag(SCALAR_BIQUAD_METRIC)
+        sys.pairs[b] = [scalar, Jm, bq]
+        sys.pairs[b.reverse()] = [scalar, Jm.T.copy(), None if bq is None else bq.T.copy()]
+    sys._invalidate()
+
+
+def set_pair_biquad_(sys: System, B, bond: Bond):
+    """Coupling Q_i . B Q_j of Stevens quadrupoles (5x5, q descending) on one
+    bond in every cell; the reverse bond carries B' (PairExchange.jl:447-468).
+    In :dipole mode B is renormalised by rcs_2(s_i) rcs_2(s_j)."""
+    B = np.asarray(B, float).reshape(5, 5)
+    if sys.mode == "dipole":
+        B = B * rcs_factors(sys.spins[bond.i])[2] * rcs_factors(sys.spins[bond.j])[2]
+    for b, M in ((bond, B), (bond.reverse(), B.T)):
+        ent = sys.pairs.get(b, [0.0, np.zeros((3, 3)), None])
+        ent[2] = M.copy() if ent[2] is None else ent[2] + M
+        sys.pairs[b] = ent
+    sys._invalidate()
+
+
+def set_onsite_coupling_(sys: System, op, i: int):
+    """Onsite anisotropy on atom i (src/System/OnsiteCoupling.jl:1-23, 159-205).
+    `op` is an N x N matrix in the spin-s irrep, or a function of the spin
+    matrices.  No symmetry propagation: call once per atom (with the operator
+    rotated as required)."""
+    s = sys.spins[i]
+    if callable(op):
+        op = op(spin_matrices(s))
+    A = np.asarray(op, complex)
+    N = int(round(2 * s + 1))
+    if A.shape != (N, N):
+        raise ValueError("Invalid matrix size.")
+    if not np.allclose(A, A.conj().T):
+        raise ValueError("Operator is not Hermitian")
+    if N == 2:
+        raise ValueError("Onsite coupling is trivial for quantum spin 1/2.")
+    if sys.mode == "SUN":
+        sys.onsite[i] = A.copy()
+    else:
+        c = matrix_to_stevens_coefficients((A + A.conj().T) / 2)
+        lam = rcs_factors(s)
+        sys.onsite[i] = StevensExpansion.from_coefficients({k: lam[k] * c[k] for k in range(7)})
+    sys._invalidate()
+
+
+def set_stevens_expansion_(sys: System, stvexp: StevensExpansion, i: int):
+    """Sets already-renormalised classical Stevens coefficients (:dipole)."""
+    if sys.mode != "dipole":
+        raise ValueError("Stevens expansions apply to :dipole mode")
+    sys.onsite[i] = stvexp
+    sys._invalidate()
+
+
+def enable_dipole_dipole_(sys: System, mu0_muB2=VACUUM_PERMEABILITY, demag=1 / 3):
+    """Long-range dipole-dipole by Ewald summation (Interactions.jl:182-189)."""
+    demag = np.asarray(demag, float)
+    if demag.ndim == 0:
+        demag = demag * np.eye(3)
+    sys.ewald = (float(mu0_muB2), demag.reshape(3, 3).copy())
+    sys._invalidate()
+
+
+def set_spin_rescaling_(sys: System, kappas):
+    """Per-sublattice |S| (dipole) or |Z|^2 (SU(N)) rescaling (System.jl:573-)."""
+    sys._pull()
+    sys.kappas = np.asarray(kappas, float).reshape(sys.na).copy()
+    sys._invalidate()

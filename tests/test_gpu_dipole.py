@@ -1,0 +1,149 @@
+"""GPU parity of the :dipole path against the FP64 oracle.
+
+Bars: FP64 <= 1e-11 relative (per step and over short trajectories); FP32
+<= 2e-5 relative on fields and states, energy within 1e-6 of the energy
+scale.  All calls go through libsunnyhip's C ABI.
+"""
+import numpy as np
+import pytest
+
+import sunnyhip as sh
+from oracle import oracle as O
+
+from helpers import announce, energy_scale, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = {64: 1e-11, 32: 2e-5}
+
+
+def models(precision):
+    yield "cubic4", sh.workloads.cubic_model(L=4, seed=3, precision=precision)
+    yield "cubic_ragged", sh.workloads.cubic_model(dims=(3, 5, 2), seed=4, precision=precision)
+    yield "pyro2", sh.workloads.pyrochlore_model(L=2, seed=5, precision=precision)
+    s = sh.workloads.pyrochlore_model(dims=(1, 2, 3), seed=6, precision=precision)
+    yield "pyro_ragged", s
+
+
+def aniso_model(precision):
+    """Diamond, s = 3: bilinear exchange matrix, field, k = 2, 4, 6 Stevens
+    terms and a quadrupole coupling (the shared.jl interaction set)."""
+    cryst = sh.diamond_crystal()
+    s = sh.System(cryst, [(i, sh.Moment(s=3, g=2)) for i in range(8)], "dipole", dims=(2, 2, 2), seed=7,
+                  precision=precision)
+    sh.set_field_(s, [0.0, 0.2, 0.3])
+    J = np.array([[0.5, 0.2, -0.4], [0.2, 0.5, -0.4], [0.4, 0.4, 1.5]])
+    sh.set_exchange_(s, J, sh.Bond(0, 1, (0, 0, 0)))
+    O_ = sh.stevens_matrices(3)
+    for i in range(8):
+        sh.set_onsite_coupling_(s, 0.01 * ((1 / 20) * O_[4][4] + (1 / 4) * O_[4][0]) + 2e-4 * O_[6][6]
+                                + 0.05 * O_[2][2], i)
+    B = np.array([[1.2, 0, 0, 0, 0], [0, 1, 0, 0, 0], [0, 0, 1.1, 0, -1.4], [0, 0, 0, 1, 0], [0, 0, 1.4, 0, 1.3]])
+    sh.set_pair_biquad_(s, 0.1 * B, sh.Bond(0, 0, (0, 0, 1)))
+    sh.set_exchange_(s, 0.3, sh.Bond(0, 2, (0, 0, 0)), biquad=0.2)
+    sh.randomize_spins_(s)
+    return s
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_grad_and_energy(precision):
+    cases = list(models(precision)) + [("diamond_aniso", aniso_model(precision))]
+    for name, s in cases:
+        announce(f"grad/energy {name} fp{precision}")
+        S = s.dipoles_flat()
+        o = O.Oracle(s.spec())
+        g_dev = sh.energy_grad_dipoles(s)
+        g_ref = o.grad(S)
+        assert rel_err(g_dev, g_ref) <= TOL[precision], name
+        E_dev, E_ref = sh.energy(s), o.energy(S)
+        assert abs(E_dev - E_ref) <= TOL[precision] * 0.1 * energy_scale(s) + 1e-13, (name, E_dev, E_ref)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_langevin_host_noise(precision):
+    for name, s in list(models(precision)) + [("diamond_aniso", aniso_model(precision))]:
+        announce(f"langevin host-noise {name} fp{precision}")
+        o = O.Oracle(s.spec())
+        rng = np.random.default_rng(11)
+        integ = sh.Langevin(0.02, damping=0.1, kT=0.5)
+        S = s.dipoles_flat()
+        for _ in range(3):
+            noise = rng.standard_normal((s.nsites, 3))
+            S = o.step_langevin(S=S, noise=noise, dt=integ.dt, damping=integ.damping, kT=integ.kT)
+            sh.step_(s, integ, noise=noise)
+        S_dev = s.dipoles_flat()
+        kap = np.repeat(s.kappas, s.ncells)
+        assert rel_err(S_dev, S, scale=kap.max()) <= 3 * TOL[precision], name
+        assert np.max(np.abs(np.linalg.norm(S_dev, axis=1) - kap)) <= 1e-12 * kap.max() + (
+            1e-6 * kap.max() if precision == 32 else 0)
+
+
+def test_langevin_device_philox_trajectory():
+    """Device Philox noise reproduced on the CPU: 20 FP64 steps agree."""
+    s = sh.workloads.pyrochlore_model(L=2, seed=12, precision=64)
+    announce("langevin philox pyro2 fp64 x20")
+    o = O.Oracle(s.spec())
+    integ = sh.Langevin(0.025, damping=0.2, kT=0.3)
+    S = s.dipoles_flat()
+    key = O.replica_key(s.noise_seed, 0)
+    for k in range(20):
+        noise = O.philox_normals(key, s.step_counter + k, s.nsites, 3)
+        S = o.step_langevin(S=S, noise=noise, dt=integ.dt, damping=integ.damping, kT=integ.kT)
+    sh.step_(s, integ, nsteps=20)
+    assert rel_err(s.dipoles_flat(), S, scale=1.5) <= 1e-10
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_midpoint(precision):
+    for name, s in list(models(precision)) + [("diamond_aniso", aniso_model(precision))]:
+        announce(f"midpoint {name} fp{precision}")
+        o = O.Oracle(s.spec())
+        damping = 0.1 if name.startswith("cubic") else 0.0
+        tol = 1e-12 if precision == 64 else 1e-6
+        integ = sh.ImplicitMidpoint(0.02, damping=damping, tol=tol)
+        S = s.dipoles_flat()
+        its_ref = []
+        for _ in range(3):
+            S, it = o.step_midpoint(S=S, dt=integ.dt, damping=damping, tol=tol)
+            its_ref.append(it)
+        its_dev = [int(sh.step_(s, integ)[0]) for _ in range(3)]
+        kap = np.repeat(s.kappas, s.ncells)
+        err = rel_err(s.dipoles_flat(), S, scale=kap.max())
+        if precision == 64:
+            assert err <= 1e-11, name
+            assert all(abs(a - b) <= 1 for a, b in zip(its_dev, its_ref)), (its_dev, its_ref)
+        else:
+            assert err <= 5e-6, (name, err)
+
+
+def test_midpoint_nonconvergence_message():
+    s = sh.workloads.cubic_model(L=4, seed=2, precision=64)
+    announce("midpoint nonconvergence cubic4")
+    S0 = s.dipoles_flat()
+    with pytest.raises(sh.ConvergenceError, match="Spherical midpoint method failed to converge"):
+        sh.step_(s, sh.ImplicitMidpoint(0.05), max_iters=1)
+    np.testing.assert_array_equal(s.dipoles_flat(), S0)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_replica_batch_uneven(precision):
+    """Replicas of different dims in one launch, each matching its own oracle."""
+    dims = [(4, 4, 4), (3, 2, 5), (6, 1, 2)]
+    systems = [sh.workloads.cubic_model(dims=d, seed=20 + r, precision=precision) for r, d in enumerate(dims)]
+    announce(f"replica batch cubic {dims} fp{precision}")
+    batch = sh.ReplicaBatch(systems, precision=precision)
+    kTs = np.array([0.1, 0.5, 1.0])
+    integ = sh.Langevin(0.03, damping=0.1, kT=0.0)
+    refs = [s.dipoles_flat() for s in systems]
+    sh.step_(batch, integ, nsteps=5, kTs=kTs)
+    E_dev = sh.energy(batch)
+    batch.sync_to_systems()
+    for r, s in enumerate(systems):
+        o = O.Oracle(s.spec())
+        S = refs[r]
+        key = O.replica_key(batch.noise_seed, r)
+        for k in range(5):
+            noise = O.philox_normals(key, k, s.nsites, 3)
+            S = o.step_langevin(S=S, noise=noise, dt=integ.dt, damping=integ.damping, kT=kTs[r])
+        assert rel_err(s.dipoles_flat(), S, scale=1.0) <= (1e-10 if precision == 64 else 1e-4)
+        assert abs(E_dev[r] - o.energy(S)) <= (1e-10 if precision == 64 else 1e-4) * energy_scale(s)

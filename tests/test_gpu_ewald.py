@@ -1,0 +1,89 @@
+"""GPU Ewald dipole-dipole path: known answers and parity with the oracle."""
+import numpy as np
+import pytest
+
+import sunnyhip as sh
+from oracle import oracle as O
+
+from helpers import announce, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def single_dipole(dims, precision=64):
+    cryst = sh.cubic_crystal(1.0)
+    s = sh.System(cryst, [(0, sh.Moment(s=1, g=1))], "dipole", dims=dims, precision=precision)
+    sh.enable_dipole_dipole_(s, 1.0, demag=0.0)
+    return s
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 3, 4)])
+def test_single_dipole_cubic_box(dims):
+    """E per site = -1/6 for a lattice of parallel dipoles, demag = 0
+    (test/test_ewald.jl:16-31)."""
+    announce(f"ewald single dipole {dims}")
+    s = single_dipole(dims)
+    assert abs(sh.energy_per_site(s) - (-1 / 6)) < 1e-13
+
+
+def random_triclinic(precision):
+    """Three atoms in a skewed cell with random g tensors and demag
+    (test/test_ewald.jl:33-57)."""
+    rng = np.random.default_rng(0)
+    cryst = sh.Crystal(sh.lattice_vectors(1.1, 0.9, 0.8, 92, 85, 95),
+                       [[0, 0.01, 0.01], [0.1, 0.01, 0.01], [0.6, 0.4, 0.5]])
+    mom = [(0, sh.Moment(s=1, g=rng.random((3, 3)))), (1, sh.Moment(s=1.5, g=rng.random((3, 3)))),
+           (2, sh.Moment(s=2, g=rng.random((3, 3))))]
+    s = sh.System(cryst, mom, "dipole", dims=(2, 3, 1), seed=1, precision=precision)
+    R = rng.standard_normal((3, 3))
+    sh.enable_dipole_dipole_(s, 1.0, demag=R.T @ R)
+    sh.randomize_spins_(s)
+    return s
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_ewald_grad_energy_parity(precision):
+    tol = 1e-11 if precision == 64 else 3e-5
+    for name, s in [("triclinic", random_triclinic(precision)),
+                    ("pyro2_dipolar", sh.workloads.pyrochlore_model(L=2, J1=0.05, J2=0, A=0, s=3.5, seed=2,
+                                                                    dipolar=True, precision=precision))]:
+        announce(f"ewald parity {name} fp{precision}")
+        S = s.dipoles_flat()
+        o = O.Oracle(s.spec())
+        g_ref = o.grad(S)
+        assert rel_err(sh.energy_grad_dipoles(s), g_ref) <= tol, name
+        E_ref = o.energy(S)
+        assert abs(sh.energy(s) - E_ref) <= tol * max(1.0, np.abs(g_ref).max() * np.abs(S).max() * s.nsites), name
+
+
+def test_ewald_langevin_and_midpoint_fp64():
+    s = sh.workloads.pyrochlore_model(L=2, J1=0.05, J2=0, A=0, s=3.5, seed=8, dipolar=True, precision=64)
+    announce("ewald langevin+midpoint pyro2 fp64")
+    o = O.Oracle(s.spec())
+    rng = np.random.default_rng(1)
+    S = s.dipoles_flat()
+    integ = sh.Langevin(0.01, damping=0.1, kT=0.2)
+    for _ in range(2):
+        noise = rng.standard_normal((s.nsites, 3))
+        S = o.step_langevin(S=S, noise=noise, dt=integ.dt, damping=integ.damping, kT=integ.kT)
+        sh.step_(s, integ, noise=noise)
+    assert rel_err(s.dipoles_flat(), S, scale=3.5) <= 1e-11
+    mp = sh.ImplicitMidpoint(0.01)
+    S, it = o.step_midpoint(S=S, dt=0.01)
+    it_dev = sh.step_(s, mp)[0]
+    assert rel_err(s.dipoles_flat(), S, scale=3.5) <= 1e-11
+    assert abs(int(it_dev) - it) <= 1
+
+
+def test_ewald_sun():
+    """Ewald acting on expected dipoles of SU(3) coherent states."""
+    cryst = sh.cubic_crystal(1.0)
+    s = sh.System(cryst, [(0, sh.Moment(s=1, g=2))], "SUN", dims=(2, 2, 3), seed=3, precision=64)
+    sh.set_exchange_(s, 0.3, sh.Bond(0, 0, (1, 0, 0)))
+    sh.enable_dipole_dipole_(s, 0.5)
+    sh.randomize_spins_(s)
+    announce("ewald sun cubic 2x2x3 fp64")
+    Z = s.coherents_flat()
+    o = O.Oracle(s.spec())
+    assert rel_err(sh.energy_grad_coherents(s), o.grad_coherents(Z)) <= 1e-11
+    assert abs(sh.energy(s) - o.energy(Z=Z)) <= 1e-11
