@@ -1,0 +1,208 @@
+"""Throughput of the Langevin (stochastic Heun) step on one MI355X.
+
+Workload: pyrochlore lattice, Fd-3m conventional cell a = 10 A with 16 sites
+per cell, 32x32x32 cells = 524,288 spins per replica, :dipole mode with
+s = 3/2, g = 2; Heisenberg J1 = 1.0 and J2 = 0.1 meV, easy-axis
+-0.05 (S.n_i)^2 along the local <111> axes; Langevin dt = 0.025,
+damping = 0.2, kT = 0.3; random spins from the seed; device Philox noise.
+
+Reported value: spin-updates/s of the FP32 batched step (R replicas advanced
+by the same launches, inputs resident in HBM).  Beside it: the plain FP64
+step on one replica, the dominant kernel's roofline from HIP events recorded
+on the library's stream, the FP32-vs-FP64 energy deviation, and the CPU
+baseline (the FP64 CPU port stepping independent replicas, one per thread).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--replicas R]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "sunny.jl_amd")]
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = ("pyrochlore Fd-3m 16c a=10A, 32^3 cells x 16 = 524288 spins/replica, dipole s=3/2, "
+            "J1=1.0 J2=0.1 meV, -0.05(S.n_111)^2, Langevin dt=0.025 damping=0.2 kT=0.3")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_time(handle, family):
+    from sunnyhip import _lib
+    ms = ctypes.c_double()
+    n = ctypes.c_int64()
+    handle.call("snh_kernel_time", family.encode(), ctypes.byref(ms), ctypes.byref(n))
+    return ms.value, n.value
+
+
+def timed_steps(obj, integ, steps, warmup):
+    """Warm up, then time exactly `steps` steps bracketed by synchronisation."""
+    import sunnyhip as sh
+    h = obj._device()
+    if warmup:
+        sh.step_(obj, integ, nsteps=warmup)
+    h.call("snh_synchronize")
+    h.call("snh_enable_timing", 1)
+    t0 = time.perf_counter()
+    sh.step_(obj, integ, nsteps=steps)
+    h.call("snh_synchronize")
+    t1 = time.perf_counter()
+    fams = {f: kernel_time(h, f) for f in ("dipole_predict", "dipole_correct")}
+    h.call("snh_enable_timing", 0)
+    return (t1 - t0), fams
+
+
+def cpu_baseline(base_sys, seconds=15.0):
+    """FP64 CPU port (the oracle) on the same 524,288-spin replica: one
+    independent replica per thread, as Threads.@threads over clones."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle as O
+    cores = max(1, min(16, os.cpu_count() or 1))
+    spec = base_sys.spec()
+    rng = np.random.default_rng(5)
+    states = []
+    for _ in range(cores):
+        v = rng.standard_normal((base_sys.nsites, 3))
+        states.append(1.5 * v / np.linalg.norm(v, axis=1, keepdims=True))
+    oracles = [O.Oracle(spec) for _ in range(cores)]
+
+    def one(k, nsteps):
+        S = states[k]
+        nz = np.random.default_rng(k).standard_normal((base_sys.nsites, 3))
+        for _ in range(nsteps):
+            S = oracles[k].step_langevin(S=S, noise=nz, dt=0.025, damping=0.2, kT=0.3)
+        return nsteps
+
+    t0 = time.perf_counter()
+    one(0, 1)  # calibrate one step on one core
+    t1 = time.perf_counter()
+    per_step = t1 - t0
+    nsteps = max(1, int(seconds / per_step))
+    with ThreadPoolExecutor(cores) as ex:
+        t0 = time.perf_counter()
+        done = sum(ex.map(lambda k: one(k, nsteps), range(cores)))
+        t1 = time.perf_counter()
+    value = done * base_sys.nsites / (t1 - t0)
+    return {"value": value, "unit": "spin-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} replicas x {nsteps} Langevin steps of the 524288-spin pyrochlore model, "
+                      f"FP64 CPU port (oracle), one replica per thread, {t1 - t0:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--replicas", type=int, default=64)
+    ap.add_argument("--L", type=int, default=32)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (replicas shard across GPUs "
+              "as independent processes; see the design notes). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import sunnyhip as sh
+    t0 = time.perf_counter()
+    base = sh.workloads.pyrochlore_model(L=args.L, seed=0, precision=32)
+    log(f"model built in {time.perf_counter() - t0:.1f}s: {base.nsites} spins, {base.spec()['nbonds']} directed bonds")
+    integ = sh.Langevin(0.025, damping=0.2, kT=0.3)
+    R = args.replicas
+
+    # FP32 batched
+    reps = [base] + [sh.clone_system(base, seed=100 + r) for r in range(1, R)]
+    for r in range(1, R):
+        sh.randomize_spins_(reps[r])
+    batch = sh.ReplicaBatch(reps, precision=32)
+    log(f"FP32 batch: {R} replicas, {batch.nsites} spins")
+    dt32, fams32 = timed_steps(batch, integ, args.steps, args.warmup)
+    ups32 = batch.nsites * args.steps / dt32
+    log(f"FP32 batched: {dt32 * 1e3 / args.steps:.3f} ms/step, {ups32:.3e} spin-updates/s")
+
+    # FP64 plain (one replica, same initial state as replica 0)
+    plain = sh.clone_system(base)
+    plain.precision = 64
+    steps64 = max(10, args.steps // 4)
+    dt64, fams64 = timed_steps(plain, integ, steps64, args.warmup)
+    ups64 = plain.nsites * steps64 / dt64
+    log(f"FP64 plain: {dt64 * 1e3 / steps64:.3f} ms/step, {ups64:.3e} spin-updates/s")
+
+    # FP32 vs FP64 on the same start and noise: energy deviation after k steps
+    a = sh.clone_system(base)
+    b = sh.clone_system(base)
+    a.precision, b.precision = 32, 64
+    sh.step_(a, integ, nsteps=10)
+    sh.step_(b, integ, nsteps=10)
+    Ea, Eb = sh.energy(a), sh.energy(b)
+    dS = float(np.max(np.abs(a.dipoles_flat() - b.dipoles_flat())))
+    log(f"FP32 vs FP64 after 10 steps: rel dE = {abs(Ea - Eb) / abs(Eb):.2e}, max|dS| = {dS:.2e}")
+
+    # plain copy of the batch's state planes: calibrates traffic counters
+    # (profiles/) and gives the achievable streaming bandwidth
+    hb = batch._device()
+    hb.call("snh_enable_timing", 1)
+    nbytes = ctypes.c_int64()
+    hb.call("snh_copy_calibration", 20, ctypes.byref(nbytes))
+    cms, cn = kernel_time(hb, "copy_planes")
+    hb.call("snh_enable_timing", 0)
+    copy_gbs = nbytes.value / (cms / cn * 1e-3) / 1e9
+    log(f"copy kernel: {nbytes.value / 1e6:.1f} MB per launch, {copy_gbs:.0f} GB/s")
+
+    # roofline of the dominant kernel of the batched step
+    ms_p, n_p = fams32["dipole_predict"]
+    ms_c, n_c = fams32["dipole_correct"]
+    dom, ms, n, bytes_per_site = ("dipole_correct", ms_c, n_c, 36) if ms_c >= ms_p else \
+        ("dipole_predict", ms_p, n_p, 24)
+    avg_ms = ms / max(n, 1)
+    alg_bytes = bytes_per_site * batch.nsites
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    step_ms = dt32 * 1e3 / args.steps
+    step_achieved = 24 * batch.nsites / (step_ms * 1e-3) / 1e9
+
+    line = {
+        "metric": "spin-updates/s, Langevin midpoint step, FP32 batched vs FP64 plain; % HBM peak",
+        "value": ups32,
+        "unit": "spin-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD, "replicas": R, "spins_per_replica": base.nsites,
+                   "integrator": "Langevin (stochastic Heun), device Philox noise"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                     "bytes_per_spin": bytes_per_site, "avg_launch_ms": avg_ms},
+        "roofline_step": {"bytes_per_spin_update": 24, "achieved": step_achieved, "unit": "GB/s",
+                          "frac": step_achieved / HBM_PEAK_GBS},
+        "kernels_ms": {"dipole_predict": ms_p / max(n_p, 1), "dipole_correct": ms_c / max(n_c, 1)},
+        "copy_kernel": {"bytes_per_launch": nbytes.value, "GB/s": copy_gbs},
+        "fp64_plain": {"value": ups64, "unit": "spin-updates/s", "ms_per_step": dt64 * 1e3 / steps64,
+                       "steps": steps64, "dtype": "f64"},
+        "speedup_fp32_batched_vs_fp64_plain": ups32 / ups64,
+        "fp32_vs_fp64_after_10_steps": {"rel_energy": abs(Ea - Eb) / abs(Eb), "max_abs_dS": dS},
+        "cpu_baseline": None,
+    }
+    if not args.skip_cpu:
+        log("CPU baseline ...")
+        line["cpu_baseline"] = cpu_baseline(base, args.cpu_seconds)
+        log(f"CPU: {line['cpu_baseline']['value']:.3e} spin-updates/s on {line['cpu_baseline']['cores']} cores")
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

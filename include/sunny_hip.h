@@ -130,6 +130,12 @@ int snh_synchronize(snh_system *sys);
 int snh_enable_timing(snh_system *sys, int32_t enable);
 int snh_kernel_time(snh_system *sys, const char *family, double *ms, int64_t *launches);
 
+/* Diagnostics with no counterpart in the reference: copies the state planes
+ * into the stage buffer `repeats` times with a plain streaming kernel
+ * (k_copy_planes), whose bytes per launch are returned in *bytes (read +
+ * write).  Used to calibrate HBM traffic counters against a known size. */
+int snh_copy_calibration(snh_system *sys, int32_t repeats, int64_t *bytes);
+
 #ifdef __cplusplus
 }
 #endif

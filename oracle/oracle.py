@@ -170,6 +170,15 @@ class Oracle:
         return out
 
 
+def expected_spin(N, Z):
+    """expected_spin of every row of Z ([sites][N] complex)."""
+    Z = np.atleast_2d(Z)
+    Zr = np.ascontiguousarray(np.stack([Z.real, Z.imag], -1), float)
+    S = np.zeros((Z.shape[0], 3))
+    lib().so_expected_spin(N, Z.shape[0], _d(Zr), _d(S))
+    return S
+
+
 def stevens_energy_grad(S, c, kmax):
     S = np.ascontiguousarray(S, float)
     c = np.ascontiguousarray(c, float)

@@ -146,6 +146,7 @@ struct snh_system {
     virtual void synchronize() = 0;
     virtual void enable_timing(bool on) = 0;
     virtual void kernel_time(const char* fam, double* ms, int64_t* n) = 0;
+    virtual void copy_calibration(int repeats, int64_t* bytes) = 0;
 };
 
 namespace {
@@ -154,7 +155,8 @@ template <typename T>
 struct SystemImpl final : snh_system {
     hipStream_t st = nullptr;
     int R = 0, na = 0, N = 0, nb = 0, nblocks = 0;
-    long long total = 0;
+    uint32_t total = 0;  // padded sites over all replicas (plane stride)
+    Ghost G{};
     bool ewald = false;
     std::vector<ReplicaInfo> reps;
     std::vector<int> rblk;
@@ -164,8 +166,8 @@ struct SystemImpl final : snh_system {
     int* d_rblk = nullptr;
     // state
     T* X = nullptr;     // current state (3 or 2N planes)
-    T* Xp = nullptr;    // Heun stage
-    T* Xo = nullptr;    // ping-pong output
+    T* Xp = nullptr;    // Heun stage S' (with ghosts)
+    T* Xo = nullptr;    // Heun half-update H (own sites only); scratch planes
     T* B1 = nullptr;    // midpoint buffers (lazy)
     T* B2 = nullptr;
     T* Bh = nullptr;    // materialised midpoint state (Ewald only)
@@ -181,7 +183,7 @@ struct SystemImpl final : snh_system {
     int planes = 3;
     int last_iters = 4;
     // Ewald
-    EwaldGeom G{};
+    EwaldGeom EG{};
     void* M = nullptr;
     void* Fmu = nullptr;
     void* Fphi = nullptr;
@@ -216,78 +218,134 @@ struct SystemImpl final : snh_system {
         planes = (N == 0) ? 3 : 2 * N;
         SNH_HIP(hipStreamCreate(&st));
 
+        // ghost widths: the longest bond offset along each axis
+        int w[3] = {0, 0, 0};
+        for (int b = 0; b < nb; b++) {
+            const int* bd = sp.bond + 6 * b;
+            SNH_REQUIRE(bd[0] >= 0 && bd[0] < na && bd[1] >= 0 && bd[1] < na, SNH_ERR_INVALID,
+                        "bond atom index out of range");
+            for (int a = 0; a < 3; a++) w[a] = std::max(w[a], std::abs(bd[2 + a]));
+        }
+        G.w1 = w[0];
+        G.w2 = w[1];
+        G.w3 = w[2];
+        G.general = 0;
+
         // replicas and dims classes
         std::map<std::vector<int>, int> dcls_of;
         std::vector<std::vector<int>> dcls_dims;
-        std::vector<int> rdcls(R);
-        long long off = 0;
+        unsigned long long off = 0;
         for (int r = 0; r < R; r++) {
             std::vector<int> d = {sp.dims[3 * r], sp.dims[3 * r + 1], sp.dims[3 * r + 2]};
-            for (int a = 0; a < 3; a++)
+            for (int a = 0; a < 3; a++) {
                 SNH_REQUIRE(d[a] >= 1, SNH_ERR_INVALID, "replica dims must be positive");
+                if (w[a] > d[a]) G.general = 1;
+            }
             auto it = dcls_of.find(d);
             if (it == dcls_of.end()) {
                 it = dcls_of.emplace(d, (int)dcls_dims.size()).first;
                 dcls_dims.push_back(d);
             }
-            rdcls[r] = it->second;
-            ReplicaInfo ri;
+            ReplicaInfo ri{};
             ri.L1 = d[0];
             ri.L2 = d[1];
             ri.L3 = d[2];
-            long long nc = (long long)d[0] * d[1] * d[2];
-            SNH_REQUIRE(nc * na < (1ll << 31), SNH_ERR_UNSUPPORTED, "a replica must have fewer than 2^31 sites");
+            ri.P1 = d[0] + 2 * w[0];
+            ri.P2 = d[1] + 2 * w[1];
+            ri.P3 = d[2] + 2 * w[2];
+            const unsigned long long nc = (unsigned long long)d[0] * d[1] * d[2];
+            const unsigned long long pc = (unsigned long long)ri.P1 * ri.P2 * ri.P3;
             ri.ncell = (int)nc;
-            ri.off = off;
-            off += nc * na;
+            ri.pcell = (uint32_t)pc;
+            ri.dcls = it->second;
+            ri.off = (uint32_t)off;
+            ri.inv1 = std::nextafter(1.0f / (float)d[0], 2.0f);
+            ri.inv2 = std::nextafter(1.0f / (float)d[1], 2.0f);
+            SNH_REQUIRE(nc < (1ull << 24), SNH_ERR_UNSUPPORTED, "a replica must have fewer than 2^24 cells");
+            off += pc * na;
+            SNH_REQUIRE(off * sizeof(T) < (1ull << 32), SNH_ERR_UNSUPPORTED,
+                        "state planes must stay below 4 GiB (32-bit offsets); split the batch");
             reps.push_back(ri);
         }
-        total = off;
-        // blocks: replica-major, then sublattice, then chunks of kBlock cells
+        total = (uint32_t)off;
+        // blocks: replica-major, then chunks of kBlock interior cells, then
+        // sublattice: consecutive blocks cover the same cells of different
+        // sublattices, which is where a site's bonded neighbours live
         std::vector<BlockInfo> blocks;
         rblk.assign(R + 1, 0);
         for (int r = 0; r < R; r++) {
             rblk[r] = (int)blocks.size();
-            for (int i = 0; i < na; i++)
-                for (int c0 = 0; c0 < reps[r].ncell; c0 += kBlock) blocks.push_back({r, i, c0, rdcls[r]});
+            for (int c0 = 0; c0 < reps[r].ncell; c0 += kBlock)
+                for (int i = 0; i < na; i++) blocks.push_back({r, i, c0, 0});
         }
         rblk[R] = (int)blocks.size();
         nblocks = (int)blocks.size();
 
-        // bonds: group by source sublattice (gather lists)
+        // bonds: group by source sublattice (gather lists); pad every list to a
+        // multiple of the gather chunk with zero couplings of the site to itself
         std::vector<int> order(nb);
-        for (int b = 0; b < nb; b++) {
-            order[b] = b;
-            const int* bd = sp.bond + 6 * b;
-            SNH_REQUIRE(bd[0] >= 0 && bd[0] < na && bd[1] >= 0 && bd[1] < na, SNH_ERR_INVALID,
-                        "bond atom index out of range");
-        }
+        for (int b = 0; b < nb; b++) order[b] = b;
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sp.bond[6 * a] < sp.bond[6 * b]; });
-        std::vector<int> bstart(na + 1, 0), bj(nb), bflag(nb);
-        std::vector<T> bJ(9 * (size_t)nb), bQ(25 * (size_t)nb), bsc(nb);
-        for (int k = 0; k < nb; k++) {
-            int b = order[k];
-            const int* bd = sp.bond + 6 * b;
-            bstart[bd[0] + 1]++;
-            bj[k] = bd[1];
-            const double* J = sp.bilin + 9 * b;
-            bool iso = J[1] == 0 && J[2] == 0 && J[3] == 0 && J[5] == 0 && J[6] == 0 && J[7] == 0 && J[0] == J[4] &&
-                       J[0] == J[8];
-            bool bq = sp.has_biquad && sp.has_biquad[b];
-            bflag[k] = (iso ? kBondIso : 0) | (bq ? kBondBiquad : 0) | (bd[5] ? kBondCulled : 0);
-            for (int p = 0; p < 9; p++) bJ[9 * (size_t)k + p] = (T)J[p];
-            for (int p = 0; p < 25; p++) bQ[25 * (size_t)k + p] = bq ? (T)sp.biquad[25 * b + p] : (T)0;
-            bsc[k] = sp.scalar ? (T)sp.scalar[b] : (T)0;
+        std::vector<int> count(na, 0);
+        for (int b = 0; b < nb; b++) count[sp.bond[6 * b]]++;
+        long long slots6 = 0, slots9 = 0, slots18 = 0;
+        for (int i = 0; i < na; i++) {
+            slots6 += (count[i] + 5) / 6 * 6;
+            slots9 += (count[i] + 8) / 9 * 9;
+            slots18 += (count[i] + 17) / 18 * 18;
         }
-        for (int i = 0; i < na; i++) bstart[i + 1] += bstart[i];
-        std::vector<int> bnred((size_t)dcls_dims.size() * nb * 3);
-        for (size_t d = 0; d < dcls_dims.size(); d++)
-            for (int k = 0; k < nb; k++)
-                for (int a = 0; a < 3; a++) {
-                    int L = dcls_dims[d][a];
-                    int n = sp.bond[6 * order[k] + 2 + a];
-                    bnred[(d * nb + k) * 3 + a] = ((n % L) + L) % L;
+        // neighbours per batch of loads: the chunk with the fewest padded slots
+        // (ties to the larger chunk: fewer round trips); SU(N) kernels use 6
+        int CH = 6;
+        if (N == 0) CH = slots18 <= slots6 ? 18 : (slots9 <= slots6 ? 9 : 6);
+        C.chunk = CH;
+        std::vector<int> bstart(na + 1, 0);
+        for (int i = 0; i < na; i++) bstart[i + 1] = bstart[i] + (count[i] + CH - 1) / CH * CH;
+        const int nbp = bstart[na];  // padded bond count
+        std::vector<int> bj(std::max(nbp, 1)), bflag(std::max(nbp, 1));
+        std::vector<int> bsrc(std::max(nbp, 1), -1);  // original bond index, -1 for padding
+        std::vector<T> bJ(9 * (size_t)std::max(nbp, 1), (T)0), bJ0(std::max(nbp, 1), (T)0),
+            bQ(25 * (size_t)std::max(nbp, 1), (T)0), bsc(std::max(nbp, 1), (T)0);
+        {
+            std::vector<int> fill(na, 0);
+            for (int k = 0; k < nb; k++) {
+                const int b = order[k];
+                const int* bd = sp.bond + 6 * b;
+                const int slot = bstart[bd[0]] + fill[bd[0]]++;
+                bsrc[slot] = b;
+                bj[slot] = bd[1];
+                const double* J = sp.bilin + 9 * b;
+                const bool iso = J[1] == 0 && J[2] == 0 && J[3] == 0 && J[5] == 0 && J[6] == 0 && J[7] == 0 &&
+                                 J[0] == J[4] && J[0] == J[8];
+                const bool bq = sp.has_biquad && sp.has_biquad[b];
+                bflag[slot] = (iso ? kBondIso : 0) | (bq ? kBondBiquad : 0) | (bd[5] ? kBondCulled : 0);
+                for (int p = 0; p < 9; p++) bJ[9 * (size_t)slot + p] = (T)J[p];
+                bJ0[slot] = (T)J[0];
+                for (int p = 0; p < 25; p++) bQ[25 * (size_t)slot + p] = bq ? (T)sp.biquad[25 * b + p] : (T)0;
+                bsc[slot] = sp.scalar ? (T)sp.scalar[b] : (T)0;
+            }
+            for (int i = 0; i < na; i++)
+                for (int slot = bstart[i] + count[i]; slot < bstart[i + 1]; slot++) {
+                    bj[slot] = i;
+                    bflag[slot] = kBondIso | kBondCulled;
                 }
+        }
+        nb = nbp;
+        C.iso = 1;
+        for (int k = 0; k < nbp; k++)
+            if (!(bflag[k] & kBondIso) || (bflag[k] & kBondBiquad)) C.iso = 0;
+        // padded-index offset of every bond, per dims class
+        std::vector<int> bdelta(std::max<size_t>(1, dcls_dims.size() * nbp), 0);
+        for (size_t d = 0; d < dcls_dims.size(); d++) {
+            const long long P1 = dcls_dims[d][0] + 2 * w[0], P2 = dcls_dims[d][1] + 2 * w[1],
+                            P3 = dcls_dims[d][2] + 2 * w[2];
+            for (int k = 0; k < nbp; k++) {
+                if (bsrc[k] < 0) continue;  // padding: the site itself
+                const int* bd = sp.bond + 6 * bsrc[k];
+                const long long v = bd[2] + P1 * (bd[3] + P2 * (long long)bd[4]) + P1 * P2 * P3 * (bd[1] - bd[0]);
+                bdelta[d * nbp + k] = (int)v;
+            }
+        }
         // per-sublattice tables
         std::vector<T> kap(na), zee(3 * na), stev(28 * na), gm(9 * na), fld(3 * na);
         std::vector<int> kmax(na, 0);
@@ -325,7 +383,8 @@ struct SystemImpl final : snh_system {
         C.bstart = own(dupload(bstart));
         C.bj = own(dupload(bj));
         C.bflag = own(dupload(bflag));
-        C.bnred = own(dupload(bnred));
+        C.bdelta = own(dupload(bdelta));
+        C.bJ0 = own(dupload(bJ0));
         C.bJ = own(dupload(bJ));
         C.bQ = own(dupload(bQ));
         C.bscalar = own(dupload(bsc));
@@ -336,18 +395,18 @@ struct SystemImpl final : snh_system {
         C.gmat = own(dupload(gm));
         C.field = own(dupload(fld));
         C.lam = own(dupload(lam));
-        C.spin_ops = nullptr;
         C.quad_ops = own(dupload(qops));
         C.na = na;
         C.nb = nb;
         C.N = N;
         C.total = total;
+        C.G = G;
         d_rblk = own(dupload(rblk));
 
         X = own(dalloc<T>((size_t)planes * total));
         Xp = own(dalloc<T>((size_t)planes * total));
         Xo = own(dalloc<T>((size_t)planes * total));
-        SNH_HIP(hipMemset(X, 0, sizeof(T) * planes * total));
+        SNH_HIP(hipMemset(X, 0, sizeof(T) * planes * (size_t)total));
         partial = own(dalloc<double>(nblocks));
         d_out = own(dalloc<double>(R));
         conv_iter = own(dalloc<int>(R));
@@ -367,14 +426,20 @@ struct SystemImpl final : snh_system {
         const int L[3] = {reps[0].L1, reps[0].L2, reps[0].L3};
         for (int a = 0; a < 3; a++)
             SNH_REQUIRE(L[a] <= 32, SNH_ERR_UNSUPPORTED, "Ewald FFT supports at most 32 cells per axis");
-        G.L1 = L[0];
-        G.L2 = L[1];
-        G.L3 = L[2];
-        G.H1 = L[0] / 2 + 1;
-        G.na = na;
-        G.ncell = (long long)L[0] * L[1] * L[2];
-        G.nk = (long long)G.H1 * L[1] * L[2];
-        SNH_REQUIRE(ewald_fft_lds_bytes(G.nk, 16) <= 65536, SNH_ERR_UNSUPPORTED,
+        EG.L1 = L[0];
+        EG.L2 = L[1];
+        EG.L3 = L[2];
+        EG.H1 = L[0] / 2 + 1;
+        EG.na = na;
+        EG.ncell = (long long)L[0] * L[1] * L[2];
+        EG.nk = (long long)EG.H1 * L[1] * L[2];
+        EG.P1 = reps[0].P1;
+        EG.P2 = reps[0].P2;
+        EG.pcell = reps[0].pcell;
+        EG.w1 = G.w1;
+        EG.w2 = G.w2;
+        EG.w3 = G.w3;
+        SNH_REQUIRE(ewald_fft_lds_bytes(EG.nk, 16) <= 65536, SNH_ERR_UNSUPPORTED,
                     "Ewald FFT grid too large for one workgroup's LDS");
         SNH_REQUIRE((size_t)R * 3 * na * sizeof(T) * 2 <= 65536, SNH_ERR_UNSUPPORTED,
                     "too many replicas x sublattices for the Ewald contraction");
@@ -416,17 +481,17 @@ struct SystemImpl final : snh_system {
         P.na = na;
         std::vector<double> pos(sp.positions, sp.positions + 3 * na);
         double* d_pos = dupload(pos);
-        double* d_A = dalloc<double>((size_t)na * na * G.ncell * 9);
+        double* d_A = dalloc<double>((size_t)na * na * EG.ncell * 9);
         const int n3 = 3 * na;
-        M = own(dalloc<char>((size_t)G.nk * n3 * n3 * 2 * sizeof(T)));
+        M = own(dalloc<char>((size_t)EG.nk * n3 * n3 * 2 * sizeof(T)));
         launch_ewald_A(P, d_pos, d_A, st);
-        launch_ewald_fwdA<T>(G, d_A, M, st);
+        launch_ewald_fwdA<T>(EG, d_A, M, st);
         SNH_HIP(hipGetLastError());
         SNH_HIP(hipStreamSynchronize(st));
         SNH_HIP(hipFree(d_A));
         SNH_HIP(hipFree(d_pos));
-        Fmu = own(dalloc<char>((size_t)R * n3 * G.nk * 2 * sizeof(T)));
-        Fphi = own(dalloc<char>((size_t)R * n3 * G.nk * 2 * sizeof(T)));
+        Fmu = own(dalloc<char>((size_t)R * n3 * EG.nk * 2 * sizeof(T)));
+        Fphi = own(dalloc<char>((size_t)R * n3 * EG.nk * 2 * sizeof(T)));
         phi1 = own(dalloc<T>(3 * (size_t)total));
         phi2 = own(dalloc<T>(3 * (size_t)total));
         if (N > 0) Sd = own(dalloc<T>(3 * (size_t)total));
@@ -457,11 +522,55 @@ struct SystemImpl final : snh_system {
             S = Sd;
         }
         hipEvent_t e = tbegin();
-        launch_ewald_field<T>(G, R, C.reps, total, C.gmat, S, M, Fmu, Fphi, phi, st);
+        launch_ewald_field<T>(EG, R, C.reps, total, C.gmat, S, M, Fmu, Fphi, phi, st);
         tend("ewald_field", e);
     }
 
-    int64_t total_sites() const override { return total; }
+    // Calls f(padded site, interior site) for every padded site of every
+    // replica, interior site = the periodic source of the (ghost) position.
+    template <typename F>
+    void for_each_padded(F f) const
+    {
+        long long isite0 = 0;
+        for (int r = 0; r < R; r++) {
+            const ReplicaInfo& ri = reps[r];
+            const int wv[3] = {G.w1, G.w2, G.w3};
+            for (int i = 0; i < na; i++)
+                for (int p3 = 0; p3 < ri.P3; p3++)
+                    for (int p2 = 0; p2 < ri.P2; p2++)
+                        for (int p1 = 0; p1 < ri.P1; p1++) {
+                            const int c1 = ((p1 - wv[0]) % ri.L1 + ri.L1) % ri.L1;
+                            const int c2 = ((p2 - wv[1]) % ri.L2 + ri.L2) % ri.L2;
+                            const int c3 = ((p3 - wv[2]) % ri.L3 + ri.L3) % ri.L3;
+                            const uint32_t g = ri.off + (uint32_t)(p1 + ri.P1 * (p2 + ri.P2 * p3)) + ri.pcell * i;
+                            const long long s = isite0 + c1 + (long long)ri.L1 * (c2 + (long long)ri.L2 * c3) +
+                                                (long long)ri.ncell * i;
+                            const bool interior = p1 - wv[0] == c1 && p2 - wv[1] == c2 && p3 - wv[2] == c3;
+                            f(g, s, interior);
+                        }
+            isite0 += (long long)ri.ncell * na;
+        }
+    }
+
+    long long interior_sites() const
+    {
+        long long n = 0;
+        for (const auto& ri : reps) n += (long long)ri.ncell * na;
+        return n;
+    }
+
+    void download_planes(const T* P, int np, double* x)
+    {
+        std::vector<T> h((size_t)np * total);
+        SNH_HIP(hipMemcpyAsync(h.data(), P, sizeof(T) * h.size(), hipMemcpyDeviceToHost, st));
+        SNH_HIP(hipStreamSynchronize(st));
+        for_each_padded([&](uint32_t g, long long s, bool interior) {
+            if (!interior) return;
+            for (int p = 0; p < np; p++) x[(size_t)np * s + p] = (double)h[(size_t)p * total + g];
+        });
+    }
+
+    int64_t total_sites() const override { return interior_sites(); }
     int replicas() const override { return R; }
     int64_t replica_sites(int r) const override { return (int64_t)reps[r].ncell * na; }
 
@@ -469,32 +578,24 @@ struct SystemImpl final : snh_system {
     {
         SNH_REQUIRE(coherent == (N > 0), SNH_ERR_INVALID,
                     coherent ? "coherent states require :SUN mode" : "in :SUN mode set coherent states");
-        std::vector<T> h((size_t)planes * total);
-        for (long long g = 0; g < total; g++)
-            for (int p = 0; p < planes; p++) h[(size_t)p * total + g] = (T)x[(size_t)planes * g + p];
+        std::vector<T> h((size_t)planes * total, (T)0);
+        for_each_padded([&](uint32_t g, long long s, bool) {
+            for (int p = 0; p < planes; p++) h[(size_t)p * total + g] = (T)x[(size_t)planes * s + p];
+        });
         SNH_HIP(hipMemcpyAsync(X, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
         SNH_HIP(hipStreamSynchronize(st));
     }
 
     void get_state(double* x, bool coherent) override
     {
-        std::vector<T> h((size_t)planes * total);
         if (coherent || N == 0) {
             SNH_REQUIRE(!coherent || N > 0, SNH_ERR_INVALID, "coherent states require :SUN mode");
-            SNH_HIP(hipMemcpyAsync(h.data(), X, sizeof(T) * h.size(), hipMemcpyDeviceToHost, st));
-            SNH_HIP(hipStreamSynchronize(st));
-            for (long long g = 0; g < total; g++)
-                for (int p = 0; p < planes; p++) x[(size_t)planes * g + p] = (double)h[(size_t)p * total + g];
+            download_planes(X, planes, x);
         } else {
             // dipoles of a coherent state: expected_spin(Z) (Integrators.jl:354)
-            T* tmp = Xp;  // stage buffer is free between steps
-            launch_sun_expected_spin<T>(C, nblocks, X, tmp, st);
+            launch_sun_expected_spin<T>(C, nblocks, X, Xp, st);  // stage buffer is free between steps
             SNH_HIP(hipGetLastError());
-            std::vector<T> d(3 * (size_t)total);
-            SNH_HIP(hipMemcpyAsync(d.data(), tmp, sizeof(T) * d.size(), hipMemcpyDeviceToHost, st));
-            SNH_HIP(hipStreamSynchronize(st));
-            for (long long g = 0; g < total; g++)
-                for (int p = 0; p < 3; p++) x[3 * g + p] = (double)d[(size_t)p * total + g];
+            download_planes(Xp, 3, x);
         }
     }
 
@@ -518,11 +619,16 @@ struct SystemImpl final : snh_system {
         NoiseArgs<T> nz{};
         nz.scale = nscale;
         nz.seed = seed;
-        nz.per_site = planes;
         nz.mode = anynoise ? (noise ? 1 : 2) : 0;
+        std::vector<double> hn;
         if (nz.mode == 1) {
             if (!host_noise) host_noise = own(dalloc<double>((size_t)planes * total));
-            SNH_HIP(hipMemcpyAsync(host_noise, noise, sizeof(double) * planes * total, hipMemcpyHostToDevice, st));
+            hn.assign((size_t)planes * total, 0.0);
+            for_each_padded([&](uint32_t g, long long s, bool interior) {
+                if (interior)
+                    for (int p = 0; p < planes; p++) hn[(size_t)planes * g + p] = noise[(size_t)planes * s + p];
+            });
+            SNH_HIP(hipMemcpyAsync(host_noise, hn.data(), sizeof(double) * hn.size(), hipMemcpyHostToDevice, st));
             nz.host = host_noise;
         }
         SNH_HIP(hipStreamSynchronize(st));  // host vectors above are pageable
@@ -537,21 +643,20 @@ struct SystemImpl final : snh_system {
             }
             hipEvent_t e = tbegin();
             if (N == 0)
-                launch_dipole_predict<T>(C, nblocks, X, Xp, ewS, nz, tdt, tl, st);
+                launch_dipole_predict<T>(C, nblocks, X, Xp, Xo, ewS, nz, tdt, tl, st);
             else
-                launch_sun_predict<T>(C, nblocks, X, Xp, ewS, nz, tdt, tl, st);
+                launch_sun_predict<T>(C, nblocks, X, Xp, Xo, ewS, nz, tdt, tl, st);
             tend(N == 0 ? "dipole_predict" : "sun_predict", e);
             if (ewald) {
                 ewald_field(Xp, phi2);
                 ewSp = phi2;
             }
-            e = tbegin();
+            e = tbegin();  // the corrector overwrites the state in place (the predictor consumed it)
             if (N == 0)
-                launch_dipole_correct<T>(C, nblocks, X, Xp, Xo, ewS, ewSp, nz, tdt, tl, st);
+                launch_dipole_correct<T>(C, nblocks, Xp, Xo, X, ewSp, tdt, tl, st);
             else
-                launch_sun_correct<T>(C, nblocks, X, Xp, Xo, ewS, ewSp, nz, tdt, tl, st);
+                launch_sun_correct<T>(C, nblocks, Xp, Xo, X, ewSp, tdt, tl, st);
             tend(N == 0 ? "dipole_correct" : "sun_correct", e);
-            std::swap(X, Xo);
         }
         SNH_HIP(hipGetLastError());
     }
@@ -577,8 +682,8 @@ struct SystemImpl final : snh_system {
         const T tdt = (T)dt, tl = (T)damping;
         T* bufs[2] = {B2, B1};  // S'' of iteration m lives in bufs[m % 2]: m odd -> B1
         for (int s = 0; s < nsteps; s++) {
-            SNH_HIP(hipMemsetAsync(conv_iter, 0, sizeof(int) * R, st));
-            SNH_HIP(hipMemcpyAsync(B2, X, sizeof(T) * planes * total, hipMemcpyDeviceToDevice, st));  // S' = S
+            launch_fill_int(conv_iter, R, 0, st);
+            SNH_HIP(hipMemcpyAsync(B2, X, sizeof(T) * planes * (size_t)total, hipMemcpyDeviceToDevice, st));  // S' = S
             int it = 0;
             bool done = false;
             int chunk = std::max(1, std::min(last_iters, max_iters));
@@ -603,9 +708,9 @@ struct SystemImpl final : snh_system {
                     hipEvent_t e = tbegin();
                     if (N == 0)
                         launch_dipole_midpoint_iter<T>(C, nblocks, X, S1, S2, hat, phi, tdt, tl, partial, conv_iter,
-                                                       it, st);
+                                                       st);
                     else
-                        launch_sun_midpoint_iter<T>(C, nblocks, X, S1, S2, phi, tdt, tl, partial, conv_iter, it, st);
+                        launch_sun_midpoint_iter<T>(C, nblocks, X, S1, S2, phi, tdt, tl, partial, conv_iter, st);
                     tend("midpoint_iter", e);
                     launch_midpoint_check(partial, d_rblk, R, atol2, conv_iter, it, st);
                     if (N == 0)
@@ -672,11 +777,7 @@ struct SystemImpl final : snh_system {
         else
             launch_sun_grad<T>(C, nblocks, X, phi, Xp, st);
         SNH_HIP(hipGetLastError());
-        std::vector<T> h((size_t)planes * total);
-        SNH_HIP(hipMemcpyAsync(h.data(), Xp, sizeof(T) * h.size(), hipMemcpyDeviceToHost, st));
-        SNH_HIP(hipStreamSynchronize(st));
-        for (long long g = 0; g < total; g++)
-            for (int p = 0; p < planes; p++) out[(size_t)planes * g + p] = (double)h[(size_t)p * total + g];
+        download_planes(Xp, planes, out);
     }
 
     void synchronize() override { SNH_HIP(hipStreamSynchronize(st)); }
@@ -686,6 +787,19 @@ struct SystemImpl final : snh_system {
         SNH_HIP(hipStreamSynchronize(st));
         timer.clear();
         timer.enabled = on;
+    }
+
+    void copy_calibration(int repeats, int64_t* bytes) override
+    {
+        const size_t b = (sizeof(T) * planes * (size_t)total) / 16 * 16;
+        for (int k = 0; k < repeats; k++) {
+            hipEvent_t e = tbegin();
+            launch_copy_planes(X, Xp, b, st);
+            tend("copy_planes", e);
+        }
+        SNH_HIP(hipGetLastError());
+        SNH_HIP(hipStreamSynchronize(st));
+        *bytes = (int64_t)(2 * b);
     }
 
     void kernel_time(const char* fam, double* ms, int64_t* n) override
@@ -874,5 +988,13 @@ extern "C" int snh_kernel_time(snh_system* sys, const char* fam, double* ms, int
     return guarded(sys, [&] {
         SNH_REQUIRE(fam && ms && n, SNH_ERR_INVALID, "null argument");
         sys->kernel_time(fam, ms, n);
+    });
+}
+
+extern "C" int snh_copy_calibration(snh_system* sys, int32_t repeats, int64_t* bytes)
+{
+    return guarded(sys, [&] {
+        SNH_REQUIRE(bytes && repeats >= 0, SNH_ERR_INVALID, "invalid argument");
+        sys->copy_calibration(repeats, bytes);
     });
 }

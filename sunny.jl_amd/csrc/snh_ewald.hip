@@ -115,10 +115,16 @@ __device__ inline C2<T>* lds_spec()
     return reinterpret_cast<C2<T>*>(snh_lds);
 }
 
+// Padded state index of interior cell (c1, c2, c3) of sublattice j.
+__device__ __forceinline__ uint32_t ewald_site(const EwaldGeom& G, uint32_t off, int j, int c1, int c2, int c3)
+{
+    return off + (uint32_t)((c1 + G.w1) + G.P1 * ((c2 + G.w2) + G.P2 * (c3 + G.w3))) + G.pcell * (uint32_t)j;
+}
+
 // Forward transform of mu = -g S.  grid = (3 na, R); field f = 3 j + beta.
 template <typename T>
 __global__ __launch_bounds__(256) void k_ewald_fwd(EwaldGeom G, const ReplicaInfo* __restrict__ reps,
-                                                   long long total, const T* __restrict__ gmat,
+                                                   uint32_t total, const T* __restrict__ gmat,
                                                    const T* __restrict__ S, C2<T>* __restrict__ Fmu)
 {
     const int f = blockIdx.x, r = blockIdx.y;
@@ -127,11 +133,11 @@ __global__ __launch_bounds__(256) void k_ewald_fwd(EwaldGeom G, const ReplicaInf
     C2<T>* tw1 = spec + G.nk;
     C2<T>* tw2 = tw1 + kLmax;
     C2<T>* tw3 = tw2 + kLmax;
-    const long long base = reps[r].off + G.ncell * j;
+    const uint32_t off = reps[r].off;
     const T g0 = gmat[9 * j + 3 * beta], g1 = gmat[9 * j + 3 * beta + 1], g2 = gmat[9 * j + 3 * beta + 2];
     fwd3d<T>(G, spec, tw1, tw2, tw3, [&](int c1, int c2, int c3) -> T {
-        long long g = base + c1 + (long long)G.L1 * (c2 + (long long)G.L2 * c3);
-        return -(g0 * S[g] + g1 * S[total + g] + g2 * S[2 * total + g]);
+        const uint32_t g = ewald_site(G, off, j, c1, c2, c3);
+        return -(g0 * ldg(S, g) + g1 * ldg(S + total, g) + g2 * ldg(S + 2 * (size_t)total, g));
     });
     C2<T>* out = Fmu + ((long long)r * 3 * G.na + f) * G.nk;
     for (long long k = threadIdx.x; k < G.nk; k += blockDim.x) out[k] = spec[k];
@@ -193,7 +199,7 @@ __global__ __launch_bounds__(256) void k_ewald_contract(EwaldGeom G, int R, cons
 // Inverse: c2r 3D DFT of F[phi] scaled by 1/Ncell -> phi plane alpha of
 // sublattice i.  grid = (3 na, R); field = 3 i + alpha.
 template <typename T>
-__global__ __launch_bounds__(256) void k_ewald_inv(EwaldGeom G, const ReplicaInfo* __restrict__ reps, long long total,
+__global__ __launch_bounds__(256) void k_ewald_inv(EwaldGeom G, const ReplicaInfo* __restrict__ reps, uint32_t total,
                                                    const C2<T>* __restrict__ Fphi, T* __restrict__ phi)
 {
     const int f = blockIdx.x, r = blockIdx.y;
@@ -218,7 +224,8 @@ __global__ __launch_bounds__(256) void k_ewald_inv(EwaldGeom G, const ReplicaInf
     __syncthreads();
     // axis 1, half complex -> real, using Hermitian symmetry X[L1-k] = conj(X[k]).
     const T scale = (T)(1.0 / (double)G.ncell);
-    const long long base = reps[r].off + G.ncell * i + (long long)alpha * total;
+    T* const out = phi + (size_t)alpha * total;
+    const uint32_t off = reps[r].off;
     const bool even = (G.L1 % 2) == 0;
     for (int line = threadIdx.x; line < G.L2 * G.L3; line += blockDim.x) {
         int c2 = line % G.L2, c3 = line / G.L2;
@@ -234,7 +241,7 @@ __global__ __launch_bounds__(256) void k_ewald_inv(EwaldGeom G, const ReplicaInf
                 if (m >= G.L1) m -= G.L1;
             }
             if (even && G.H1 > 1) acc += X[G.H1 - 1].re * ((x & 1) ? (T)-1 : (T)1);
-            phi[base + x + (long long)G.L1 * (c2 + (long long)G.L2 * c3)] = acc * scale;
+            stg(out, ewald_site(G, off, i, x, c2, c3), acc * scale);
         }
     }
 }
@@ -325,7 +332,7 @@ void launch_ewald_fwdA(const EwaldGeom& G, const double* A, void* M, hipStream_t
 }
 
 template <typename T>
-void launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, long long total, const T* gmat,
+void launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint32_t total, const T* gmat,
                         const T* S, const void* M, void* Fmu, void* Fphi, T* phi, hipStream_t st)
 {
     size_t lds = ewald_fft_lds_bytes(G.nk, sizeof(C2<T>));
@@ -341,9 +348,9 @@ void launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, long
 
 template void launch_ewald_fwdA<float>(const EwaldGeom&, const double*, void*, hipStream_t);
 template void launch_ewald_fwdA<double>(const EwaldGeom&, const double*, void*, hipStream_t);
-template void launch_ewald_field<float>(const EwaldGeom&, int, const ReplicaInfo*, long long, const float*,
+template void launch_ewald_field<float>(const EwaldGeom&, int, const ReplicaInfo*, uint32_t, const float*,
                                         const float*, const void*, void*, void*, float*, hipStream_t);
-template void launch_ewald_field<double>(const EwaldGeom&, int, const ReplicaInfo*, long long, const double*,
+template void launch_ewald_field<double>(const EwaldGeom&, int, const ReplicaInfo*, uint32_t, const double*,
                                          const double*, const void*, void*, void*, double*, hipStream_t);
 
 }  // namespace snh

@@ -2,9 +2,6 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include <string>
-#include <vector>
-
 #include "snh_device.h"
 
 namespace snh {
@@ -18,7 +15,8 @@ struct Couplings {
     const int* bstart;   // [na+1] directed-bond range per sublattice (gather form)
     const int* bj;       // [nb]   target sublattice
     const int* bflag;    // [nb]   bit0 isotropic, bit1 biquad, bit2 culled
-    const int* bnred;    // [ndcls][nb][3] offsets reduced into [0, L)
+    const int* bdelta;   // [ndcls][nb] padded-index offset of the bond
+    const T* bJ0;        // [nb] J[0][0] (the coupling of an isotropic bond)
     const T* bJ;         // [nb][9] row-major, J for bond i -> j
     const T* bQ;         // [nb][25] row-major quadrupole coupling
     const T* bscalar;    // [nb]
@@ -29,20 +27,23 @@ struct Couplings {
     const T* gmat;       // [na][9]
     const T* field;      // [na][3]
     const T* lam;        // [na][N][N][2] SU(N) onsite matrix
-    const T* spin_ops;   // [3][N][N][2] spin matrices Sx, Sy, Sz
     const T* quad_ops;   // [5][N][N][2] Stevens quadrupoles O[2,q]
     int na, nb, N;
-    long long total;     // plane stride (global sites, padded)
+    int chunk;           // neighbours gathered per batch of loads (6, 9 or 18); every
+                         // sublattice's bond list is padded to a multiple of it
+                         // with zero couplings to the site itself (flag culled)
+    int iso;             // every bond isotropic and without biquadratic coupling
+    uint32_t total;      // plane stride (padded sites over all replicas)
+    Ghost G;
 };
 
 template <typename T>
 struct NoiseArgs {
     int mode;              // 0 zero, 1 host-supplied raw normals, 2 device Philox
-    const double* host;    // [total][per_site] raw draws (mode 1), by global site
+    const double* host;    // [total][per_site] raw draws (mode 1), by padded site
     const T* scale;        // [R] sqrt(2 dt damping kT_r)
     unsigned long long seed;
     unsigned long long step;
-    int per_site;          // 3 (dipole) or 2N (SU(N))
 };
 
 enum { kBondIso = 1, kBondBiquad = 2, kBondCulled = 4 };
@@ -51,6 +52,9 @@ enum { kBondIso = 1, kBondBiquad = 2, kBondCulled = 4 };
 struct EwaldGeom {
     int L1, L2, L3, H1, na;  // H1 = L1/2 + 1 (half spectrum along axis 1)
     long long ncell, nk;     // nk = H1 * L2 * L3
+    int P1, P2;              // padded dims of the state layout
+    uint32_t pcell;
+    int w1, w2, w3;
 };
 
 // Parameters of the real-space Ewald tensor precompute (FP64).
@@ -64,34 +68,33 @@ struct EwaldAParams {
     int L1, L2, L3, na;
 };
 
-// Per-precision launchers for dipole mode (snh_dipole.hip).
+// :dipole mode (snh_dipole.hip).  Buffers are plane sets [3][total].
 template <typename T>
-void launch_dipole_predict(const Couplings<T>& C, int nblocks, const T* S, T* Sp, const T* ew,
+void launch_dipole_predict(const Couplings<T>& C, int nblocks, const T* S, T* Sp, T* H, const T* phi,
                            const NoiseArgs<T>& nz, T dt, T lambda, hipStream_t st);
 template <typename T>
-void launch_dipole_correct(const Couplings<T>& C, int nblocks, const T* S, const T* Sp, T* Sout,
-                           const T* ewS, const T* ewSp, const NoiseArgs<T>& nz, T dt, T lambda,
-                           hipStream_t st);
-template <typename T>
-void launch_dipole_midpoint_iter(const Couplings<T>& C, int nblocks, const T* S, const T* S1, T* S2,
-                                 const T* Shat, const T* ew, T dt, T lambda, double* partial,
-                                 const int* conv_iter, int iter, hipStream_t st);
+void launch_dipole_correct(const Couplings<T>& C, int nblocks, const T* Sp, const T* H, T* Sout, const T* phi,
+                           T dt, T lambda, hipStream_t st);
 template <typename T>
 void launch_dipole_midpoint_hat(const Couplings<T>& C, int nblocks, const T* S, const T* S1, T* Shat,
                                 const int* conv_iter, hipStream_t st);
 template <typename T>
+void launch_dipole_midpoint_iter(const Couplings<T>& C, int nblocks, const T* S, const T* S1, T* S2,
+                                 const T* Shat, const T* phi, T dt, T lambda, double* partial,
+                                 const int* conv_iter, hipStream_t st);
+template <typename T>
 void launch_dipole_midpoint_finalize(const Couplings<T>& C, int nblocks, T* S, const T* Sa, const T* Sb,
                                      const int* conv_iter, int iter, hipStream_t st);
 template <typename T>
-void launch_dipole_grad(const Couplings<T>& C, int nblocks, const T* S, const T* ew, T* dE,
-                        hipStream_t st);
+void launch_dipole_grad(const Couplings<T>& C, int nblocks, const T* S, const T* phi, T* dE, hipStream_t st);
 template <typename T>
-void launch_dipole_energy(const Couplings<T>& C, int nblocks, const T* S, const T* ew, double* partial,
+void launch_dipole_energy(const Couplings<T>& C, int nblocks, const T* S, const T* phi, double* partial,
                           hipStream_t st);
-// Per-replica reduction of block partials, and the convergence test.
 void launch_reduce_partials(const double* partial, const int* rblk, int R, double* out, hipStream_t st);
-void launch_midpoint_check(const double* partial, const int* rblk, int R, const double* atol2,
-                           int* conv_iter, int iter, hipStream_t st);
+void launch_midpoint_check(const double* partial, const int* rblk, int R, const double* atol2, int* conv_iter,
+                           int iter, hipStream_t st);
+void launch_fill_int(int* p, int n, int v, hipStream_t st);
+void launch_copy_planes(const void* src, void* dst, size_t bytes, hipStream_t st);
 
 // Ewald (snh_ewald.hip).
 size_t ewald_fft_lds_bytes(long long nk, size_t elem);
@@ -99,33 +102,31 @@ void launch_ewald_A(const EwaldAParams& P, const double* pos, double* A, hipStre
 template <typename T>
 void launch_ewald_fwdA(const EwaldGeom& G, const double* A, void* M, hipStream_t st);
 template <typename T>
-void launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, long long total, const T* gmat,
+void launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint32_t total, const T* gmat,
                         const T* S, const void* M, void* Fmu, void* Fphi, T* phi, hipStream_t st);
 
-// SU(N) mode (snh_sun.hip).
+// :SUN mode (snh_sun.hip).  Buffers are plane sets [2N][total].
 template <typename T>
 void launch_sun_expected_spin(const Couplings<T>& C, int nblocks, const T* Z, T* S, hipStream_t st);
 template <typename T>
-void launch_sun_predict(const Couplings<T>& C, int nblocks, const T* Z, T* Zp, const T* ew,
+void launch_sun_predict(const Couplings<T>& C, int nblocks, const T* Z, T* Zp, T* H, const T* phi,
                         const NoiseArgs<T>& nz, T dt, T lambda, hipStream_t st);
 template <typename T>
-void launch_sun_correct(const Couplings<T>& C, int nblocks, const T* Z, const T* Zp, T* Zout,
-                        const T* ewZ, const T* ewZp, const NoiseArgs<T>& nz, T dt, T lambda,
-                        hipStream_t st);
-template <typename T>
-void launch_sun_midpoint_iter(const Couplings<T>& C, int nblocks, const T* Z, const T* Z1, T* Z2,
-                              const T* ew, T dt, T lambda, double* partial, const int* conv_iter,
-                              int iter, hipStream_t st);
+void launch_sun_correct(const Couplings<T>& C, int nblocks, const T* Zp, const T* H, T* Zout, const T* phi, T dt,
+                        T lambda, hipStream_t st);
 template <typename T>
 void launch_sun_midpoint_bar(const Couplings<T>& C, int nblocks, const T* Z, const T* Z1, T* Zbar,
                              const int* conv_iter, hipStream_t st);
 template <typename T>
+void launch_sun_midpoint_iter(const Couplings<T>& C, int nblocks, const T* Z, const T* Z1, T* Z2, const T* phi,
+                              T dt, T lambda, double* partial, const int* conv_iter, hipStream_t st);
+template <typename T>
 void launch_sun_midpoint_finalize(const Couplings<T>& C, int nblocks, T* Z, const T* Za, const T* Zb,
                                   const int* conv_iter, int iter, hipStream_t st);
 template <typename T>
-void launch_sun_grad(const Couplings<T>& C, int nblocks, const T* Z, const T* ew, T* HZ, hipStream_t st);
+void launch_sun_grad(const Couplings<T>& C, int nblocks, const T* Z, const T* phi, T* HZ, hipStream_t st);
 template <typename T>
-void launch_sun_energy(const Couplings<T>& C, int nblocks, const T* Z, const T* ew, double* partial,
+void launch_sun_energy(const Couplings<T>& C, int nblocks, const T* Z, const T* phi, double* partial,
                        hipStream_t st);
 
 }  // namespace snh

@@ -14,6 +14,9 @@
 // The spin matrices are tridiagonal (Sx, Sy) and diagonal (Sz) in the
 // descending-m basis, so (dE.S) Z is applied in O(N) without a table; the
 // onsite matrix Lambda and the quadrupole operators are dense N x N tables.
+// Heun folding as in :dipole mode: rhs_sun is linear in the noise,
+// -proj(zeta + dt(i+lambda)HZ, Z) = -proj(zeta, Z) - proj(dt(i+lambda)HZ, Z),
+// so the predictor stores H = Z + dZ1/2 - proj(zeta, Z')/2.
 #include "snh_internal.h"
 
 namespace snh {
@@ -24,7 +27,7 @@ struct Cx {
 };
 
 template <typename T>
-__device__ inline Cx<T> cmul(Cx<T> a, Cx<T> b)
+__device__ __forceinline__ Cx<T> cmul(Cx<T> a, Cx<T> b)
 {
     return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
@@ -35,29 +38,37 @@ struct Ket {
 };
 
 template <typename T, int N>
-__device__ inline Ket<T, N> load_ket(const T* __restrict__ P, long long total, long long g)
+__device__ __forceinline__ Ket<T, N> load_ket(const T* __restrict__ P, uint32_t total, uint32_t g)
 {
     Ket<T, N> k;
 #pragma unroll
     for (int a = 0; a < N; a++) {
-        k.z[a].re = P[(2 * a) * total + g];
-        k.z[a].im = P[(2 * a + 1) * total + g];
+        k.z[a].re = ldg(P + (size_t)(2 * a) * total, g);
+        k.z[a].im = ldg(P + (size_t)(2 * a + 1) * total, g);
     }
     return k;
 }
 
 template <typename T, int N>
-__device__ inline void store_ket(T* __restrict__ P, long long total, long long g, const Ket<T, N>& k)
+__device__ __forceinline__ void store_ket(T* __restrict__ P, uint32_t total, uint32_t g, const Ket<T, N>& k)
 {
 #pragma unroll
     for (int a = 0; a < N; a++) {
-        P[(2 * a) * total + g] = k.z[a].re;
-        P[(2 * a + 1) * total + g] = k.z[a].im;
+        stg(P + (size_t)(2 * a) * total, g, k.z[a].re);
+        stg(P + (size_t)(2 * a + 1) * total, g, k.z[a].im);
     }
 }
 
+template <typename T, int N>
+__device__ __forceinline__ void store_ket_images(T* __restrict__ P, uint32_t total, const SiteCoord& sc,
+                                                 const Ghost& G, const Ket<T, N>& k)
+{
+    store_ket<T, N>(P, total, sc.g, k);
+    for_each_ghost(sc, G, [&](uint32_t gi) { store_ket<T, N>(P, total, gi, k); });
+}
+
 template <int N>
-__device__ inline double spin_off(int k)  // k = 1..N-1
+__device__ __forceinline__ double spin_off(int k)  // k = 1..N-1
 {
     const double s = (N - 1) / 2.0;
     return sqrt(2 * (s + 1) * k - (double)k * (k + 1)) / 2;
@@ -65,13 +76,12 @@ __device__ inline double spin_off(int k)  // k = 1..N-1
 
 // expected_spin: (2 Re c, 2 Im c, sum m_k |Z_k|^2), c = sum conj(Z_k) off_k Z_{k+1}
 template <typename T, int N>
-__device__ inline void expected_spin(const Ket<T, N>& Z, T& sx, T& sy, T& sz)
+__device__ __forceinline__ void expected_spin(const Ket<T, N>& Z, T& sx, T& sy, T& sz)
 {
     T cr = 0, ci = 0, nz = 0;
 #pragma unroll
     for (int k = 0; k < N - 1; k++) {
         const T off = (T)spin_off<N>(k + 1);
-        // conj(Zk) * off * Zk+1
         cr += off * (Z.z[k].re * Z.z[k + 1].re + Z.z[k].im * Z.z[k + 1].im);
         ci += off * (Z.z[k].re * Z.z[k + 1].im - Z.z[k].im * Z.z[k + 1].re);
     }
@@ -87,7 +97,7 @@ __device__ inline void expected_spin(const Ket<T, N>& Z, T& sx, T& sy, T& sz)
 
 // expected_quadrupole from the dense O[2,q] table ([5][N][N][2]).
 template <typename T, int N>
-__device__ inline void expected_quadrupole(const Ket<T, N>& Z, const T* __restrict__ Q, T out[5])
+__device__ __forceinline__ void expected_quadrupole(const Ket<T, N>& Z, const T* __restrict__ Q, T out[5])
 {
 #pragma unroll
     for (int p = 0; p < 5; p++) {
@@ -95,7 +105,7 @@ __device__ inline void expected_quadrupole(const Ket<T, N>& Z, const T* __restri
         for (int a = 0; a < N; a++)
             for (int b = 0; b < N; b++) {
                 const T* q = Q + 2 * ((p * N + a) * N + b);
-                Cx<T> qz = cmul(Cx<T>{q[0], q[1]}, Z.z[b]);
+                const Cx<T> qz = cmul(Cx<T>{q[0], q[1]}, Z.z[b]);
                 acc += Z.z[a].re * qz.re + Z.z[a].im * qz.im;  // Re(conj(Za) qz)
             }
         out[p] = acc;
@@ -107,12 +117,12 @@ template <typename T, int N, bool kMid>
 struct KetSrc {
     const T* Z;
     const T* Z1;
-    long long total;
-    __device__ inline Ket<T, N> get(long long g) const
+    uint32_t total;
+    __device__ __forceinline__ Ket<T, N> get(uint32_t g) const
     {
         Ket<T, N> a = load_ket<T, N>(Z, total, g);
         if (!kMid) return a;
-        Ket<T, N> b = load_ket<T, N>(Z1, total, g);
+        const Ket<T, N> b = load_ket<T, N>(Z1, total, g);
 #pragma unroll
         for (int k = 0; k < N; k++) {
             a.z[k].re = (a.z[k].re + b.z[k].re) / 2;
@@ -122,80 +132,58 @@ struct KetSrc {
     }
 };
 
-struct SiteCoordS {
-    int r, i, c, c1, c2, c3, dcls;
-    long long g;
-    ReplicaInfo R;
-    bool valid;
-};
-
-__device__ inline SiteCoordS site_coord_s(const BlockInfo* blocks, const ReplicaInfo* reps)
-{
-    SiteCoordS sc;
-    BlockInfo bi = blocks[blockIdx.x];
-    sc.r = bi.r;
-    sc.i = bi.i;
-    sc.dcls = bi.dcls;
-    sc.R = reps[bi.r];
-    sc.c = bi.c0 + (int)threadIdx.x;
-    sc.valid = sc.c < sc.R.ncell;
-    int c = sc.valid ? sc.c : 0;
-    sc.c1 = c % sc.R.L1;
-    int t = c / sc.R.L1;
-    sc.c2 = t % sc.R.L2;
-    sc.c3 = t / sc.R.L2;
-    sc.g = sc.R.off + c + (long long)sc.R.ncell * sc.i;
-    return sc;
-}
-
-__device__ inline long long neighbor_site_s(const SiteCoordS& sc, const int* __restrict__ n, int j)
-{
-    int a = wrap_add(sc.c1, n[0], sc.R.L1);
-    int b = wrap_add(sc.c2, n[1], sc.R.L2);
-    int d = wrap_add(sc.c3, n[2], sc.R.L3);
-    return sc.R.off + a + (long long)sc.R.L1 * (b + (long long)sc.R.L2 * d) + (long long)sc.R.ncell * j;
-}
+// Neighbours gathered per batch of loads (bond lists are padded to a multiple
+// of it with zero self-couplings).
+constexpr int kSunChunk = 6;
 
 // H Z at one site (set_energy_grad_coherents!): dE/dS from Zeeman, bilinear
 // exchange on expected dipoles and Ewald; then (Lambda + dE.S) Z plus
 // biquadratic (dE/dQ . Q) Z.
 template <typename T, int N, bool kMid>
-__device__ inline Ket<T, N> local_hz(const Couplings<T>& C, const SiteCoordS& sc, const KetSrc<T, N, kMid>& src,
-                                     const Ket<T, N>& Z, const T* __restrict__ ew)
+__device__ __forceinline__ Ket<T, N> local_hz(const Couplings<T>& C, const SiteCoord& sc,
+                                              const KetSrc<T, N, kMid>& src, const Ket<T, N>& Z,
+                                              const T* __restrict__ phi)
 {
     const int i = sc.i;
     T gx = C.zee[3 * i], gy = C.zee[3 * i + 1], gz = C.zee[3 * i + 2];
     T wq[5] = {0, 0, 0, 0, 0};
     bool anyq = false;
-    const int* nred = C.bnred + (long long)sc.dcls * C.nb * 3;
-    for (int b = C.bstart[i]; b < C.bstart[i + 1]; b++) {
-        const int j = C.bj[b];
-        const int fl = C.bflag[b];
-        const Ket<T, N> Zj = src.get(neighbor_site_s(sc, nred + 3 * b, j));
-        T sx, sy, sz;
-        expected_spin<T, N>(Zj, sx, sy, sz);
-        const T* J = C.bJ + 9 * b;
-        if (fl & kBondIso) {
-            gx += J[0] * sx;
-            gy += J[0] * sy;
-            gz += J[0] * sz;
-        } else {
-            gx += J[0] * sx + J[1] * sy + J[2] * sz;
-            gy += J[3] * sx + J[4] * sy + J[5] * sz;
-            gz += J[6] * sx + J[7] * sy + J[8] * sz;
-        }
-        if (fl & kBondBiquad) {
-            T Qj[5];
-            expected_quadrupole<T, N>(Zj, C.quad_ops, Qj);
-            const T* Bq = C.bQ + 25 * b;
-            for (int p = 0; p < 5; p++)
-                for (int q = 0; q < 5; q++) wq[p] += Bq[5 * p + q] * Qj[q];
-            anyq = true;
+    const int* __restrict__ dl = C.bdelta + sc.R.dcls * C.nb;
+    const int b0 = C.bstart[i], b1 = C.bstart[i + 1];
+    for (int c0 = b0; c0 < b1; c0 += kSunChunk) {
+        Ket<T, N> nbr[kSunChunk];
+#pragma unroll
+        for (int k = 0; k < kSunChunk; k++) nbr[k] = src.get(sc.g + (uint32_t)dl[c0 + k]);
+#pragma unroll
+        for (int k = 0; k < kSunChunk; k++) {
+            const int b = c0 + k;
+            const int fl = C.bflag[b];
+            T sx, sy, sz;
+            expected_spin<T, N>(nbr[k], sx, sy, sz);
+            const T* J = C.bJ + 9 * b;
+            if (fl & kBondIso) {
+                const T J0 = C.bJ0[b];
+                gx += J0 * sx;
+                gy += J0 * sy;
+                gz += J0 * sz;
+            } else {
+                gx += J[0] * sx + J[1] * sy + J[2] * sz;
+                gy += J[3] * sx + J[4] * sy + J[5] * sz;
+                gz += J[6] * sx + J[7] * sy + J[8] * sz;
+            }
+            if (fl & kBondBiquad) {
+                T Qj[5];
+                expected_quadrupole<T, N>(nbr[k], C.quad_ops, Qj);
+                const T* Bq = C.bQ + 25 * b;
+                for (int p = 0; p < 5; p++)
+                    for (int q = 0; q < 5; q++) wq[p] += Bq[5 * p + q] * Qj[q];
+                anyq = true;
+            }
         }
     }
-    if (ew) {  // Ewald potential phi of the expected dipoles: dE/dS -= g' phi
+    if (phi) {  // Ewald potential phi of the expected dipoles: dE/dS -= g' phi
         const T* gm = C.gmat + 9 * i;
-        const T px = ew[sc.g], py = ew[C.total + sc.g], pz = ew[2 * C.total + sc.g];
+        const T px = ldg(phi, sc.g), py = ldg(phi + C.total, sc.g), pz = ldg(phi + 2 * (size_t)C.total, sc.g);
         gx -= gm[0] * px + gm[3] * py + gm[6] * pz;
         gy -= gm[1] * px + gm[4] * py + gm[7] * pz;
         gz -= gm[2] * px + gm[5] * py + gm[8] * pz;
@@ -207,7 +195,7 @@ __device__ inline Ket<T, N> local_hz(const Couplings<T>& C, const SiteCoordS& sc
         T hr = 0, hi = 0;
 #pragma unroll
         for (int b = 0; b < N; b++) {
-            Cx<T> p = cmul(Cx<T>{L[2 * (a * N + b)], L[2 * (a * N + b) + 1]}, Z.z[b]);
+            const Cx<T> p = cmul(Cx<T>{L[2 * (a * N + b)], L[2 * (a * N + b) + 1]}, Z.z[b]);
             hr += p.re;
             hi += p.im;
         }
@@ -237,7 +225,7 @@ __device__ inline Ket<T, N> local_hz(const Couplings<T>& C, const SiteCoordS& sc
                     qr += wq[p] * q[0];
                     qi += wq[p] * q[1];
                 }
-                Cx<T> v = cmul(Cx<T>{qr, qi}, Z.z[b]);
+                const Cx<T> v = cmul(Cx<T>{qr, qi}, Z.z[b]);
                 H.z[a].re += v.re;
                 H.z[a].im += v.im;
             }
@@ -245,9 +233,30 @@ __device__ inline Ket<T, N> local_hz(const Couplings<T>& C, const SiteCoordS& sc
     return H;
 }
 
-// rhs_sun!: lambda == 0: dZ = -i dt HZ; else dZ = -proj(zeta + dt (i + lambda) HZ, Z).
+// -proj(v, Z) = -(v - Z (Z'v)/(Z'Z))
 template <typename T, int N>
-__device__ inline Ket<T, N> rhs_sun(const Ket<T, N>& Z, const Ket<T, N>& zeta, const Ket<T, N>& H, T dt, T lambda)
+__device__ __forceinline__ Ket<T, N> neg_proj(const Ket<T, N>& v, const Ket<T, N>& Z)
+{
+    T nr = 0, ni = 0, nn = 0;
+#pragma unroll
+    for (int a = 0; a < N; a++) {
+        nr += Z.z[a].re * v.z[a].re + Z.z[a].im * v.z[a].im;
+        ni += Z.z[a].re * v.z[a].im - Z.z[a].im * v.z[a].re;
+        nn += Z.z[a].re * Z.z[a].re + Z.z[a].im * Z.z[a].im;
+    }
+    const T fr = nr / nn, fi = ni / nn;
+    Ket<T, N> d;
+#pragma unroll
+    for (int a = 0; a < N; a++) {
+        d.z[a].re = -(v.z[a].re - (Z.z[a].re * fr - Z.z[a].im * fi));
+        d.z[a].im = -(v.z[a].im - (Z.z[a].re * fi + Z.z[a].im * fr));
+    }
+    return d;
+}
+
+// Deterministic part of rhs_sun!: lambda == 0: -i dt HZ; else -proj(dt (i + lambda) HZ, Z).
+template <typename T, int N>
+__device__ __forceinline__ Ket<T, N> rhs_det(const Ket<T, N>& Z, const Ket<T, N>& H, T dt, T lambda)
 {
     Ket<T, N> d;
     if (lambda == (T)0) {
@@ -259,28 +268,17 @@ __device__ inline Ket<T, N> rhs_sun(const Ket<T, N>& Z, const Ket<T, N>& zeta, c
         return d;
     }
     Ket<T, N> v;
-    T nr = 0, ni = 0, nn = 0;
 #pragma unroll
-    for (int a = 0; a < N; a++) {
-        // (i + lambda) H = (lambda Hr - Hi) + i (Hr + lambda Hi)
-        v.z[a].re = zeta.z[a].re + dt * (lambda * H.z[a].re - H.z[a].im);
-        v.z[a].im = zeta.z[a].im + dt * (H.z[a].re + lambda * H.z[a].im);
-        nr += Z.z[a].re * v.z[a].re + Z.z[a].im * v.z[a].im;
-        ni += Z.z[a].re * v.z[a].im - Z.z[a].im * v.z[a].re;
-        nn += Z.z[a].re * Z.z[a].re + Z.z[a].im * Z.z[a].im;
+    for (int a = 0; a < N; a++) {  // (i + lambda) H = (lambda Hr - Hi) + i (Hr + lambda Hi)
+        v.z[a].re = dt * (lambda * H.z[a].re - H.z[a].im);
+        v.z[a].im = dt * (H.z[a].re + lambda * H.z[a].im);
     }
-    const T fr = nr / nn, fi = ni / nn;
-#pragma unroll
-    for (int a = 0; a < N; a++) {
-        d.z[a].re = -(v.z[a].re - (Z.z[a].re * fr - Z.z[a].im * fi));
-        d.z[a].im = -(v.z[a].im - (Z.z[a].re * fi + Z.z[a].im * fr));
-    }
-    return d;
+    return neg_proj<T, N>(v, Z);
 }
 
 // normalize_ket: Z / sqrt(Z'Z / kappa), norm in FP64.
 template <typename T, int N>
-__device__ inline Ket<T, N> normalize_ket(Ket<T, N> Z, T kappa)
+__device__ __forceinline__ Ket<T, N> normalize_ket(Ket<T, N> Z, T kappa)
 {
     if (kappa == (T)0) {
 #pragma unroll
@@ -290,7 +288,7 @@ __device__ inline Ket<T, N> normalize_ket(Ket<T, N> Z, T kappa)
     double n2 = 0;
 #pragma unroll
     for (int a = 0; a < N; a++) n2 += (double)Z.z[a].re * Z.z[a].re + (double)Z.z[a].im * Z.z[a].im;
-    double f = 1.0 / sqrt(n2 / (double)kappa);
+    const double f = 1.0 / sqrt(n2 / (double)kappa);
 #pragma unroll
     for (int a = 0; a < N; a++) {
         Z.z[a].re = (T)(Z.z[a].re * f);
@@ -299,23 +297,22 @@ __device__ inline Ket<T, N> normalize_ket(Ket<T, N> Z, T kappa)
     return Z;
 }
 
+// fill_noise! for one site: zeta = sqrt(2 dt lambda kT) * randn(ComplexF64),
+// each component with variance 1/2.
 template <typename T, int N>
-__device__ inline Ket<T, N> site_noise_sun(const NoiseArgs<T>& nz, const SiteCoordS& sc)
+__device__ __forceinline__ Ket<T, N> site_noise_sun(const NoiseArgs<T>& nz, const SiteCoord& sc)
 {
     Ket<T, N> k;
-    if (nz.mode == 0) {
 #pragma unroll
-        for (int a = 0; a < N; a++) k.z[a] = {0, 0};
-        return k;
-    }
+    for (int a = 0; a < N; a++) k.z[a] = {0, 0};
+    if (nz.mode == 0) return k;
     const T s = nz.scale[sc.r];
     if (nz.mode == 1) {
-        const double* h = nz.host + 2 * N * sc.g;
+        const double* h = nz.host + 2 * N * (size_t)sc.g;
 #pragma unroll
         for (int a = 0; a < N; a++) k.z[a] = {(T)h[2 * a] * s, (T)h[2 * a + 1] * s};
         return k;
     }
-    // randn(ComplexF64): each component has variance 1/2
     const T s2 = s * (T)0.70710678118654752440;
     const unsigned long long key = nz.seed + (unsigned long long)sc.r * 0x9E3779B97F4A7C15ull;
     const uint32_t site = (uint32_t)(sc.c + sc.R.ncell * sc.i);
@@ -337,20 +334,6 @@ __device__ inline Ket<T, N> site_noise_sun(const NoiseArgs<T>& nz, const SiteCoo
     return k;
 }
 
-__device__ inline double block_sum_s(double v)
-{
-    __shared__ double red[kBlock / 64];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) red[w] = v;
-    __syncthreads();
-    double t = 0;
-    if (threadIdx.x == 0)
-        for (int k = 0; k < kBlock / 64; k++) t += red[k];
-    return t;
-}
-
 // ---------------------------------------------------------------------------
 // Kernels
 
@@ -358,55 +341,75 @@ template <typename T, int N>
 __global__ __launch_bounds__(kBlock) void k_sun_expected_spin(Couplings<T> C, const T* __restrict__ Z,
                                                               T* __restrict__ S)
 {
-    SiteCoordS sc = site_coord_s(C.blocks, C.reps);
+    const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
     if (!sc.valid) return;
     T sx, sy, sz;
     expected_spin<T, N>(load_ket<T, N>(Z, C.total, sc.g), sx, sy, sz);
-    S[sc.g] = sx;
-    S[C.total + sc.g] = sy;
-    S[2 * C.total + sc.g] = sz;
+    stg(S, sc.g, sx);
+    stg(S + C.total, sc.g, sy);
+    stg(S + 2 * (size_t)C.total, sc.g, sz);
 }
 
+// Heun predictor: dZ1 = rhs(Z, zeta, HZ); Z' = normalize(Z + dZ1);
+// H = Z + dZ1/2 - proj(zeta, Z')/2.
 template <typename T, int N>
 __global__ __launch_bounds__(kBlock) void k_sun_predict(Couplings<T> C, const T* __restrict__ Z, T* __restrict__ Zp,
-                                                        const T* __restrict__ ew, NoiseArgs<T> nz, T dt, T lambda)
-{
-    SiteCoordS sc = site_coord_s(C.blocks, C.reps);
-    if (!sc.valid) return;
-    KetSrc<T, N, false> src{Z, nullptr, C.total};
-    Ket<T, N> z = load_ket<T, N>(Z, C.total, sc.g);
-    Ket<T, N> H = local_hz<T, N, false>(C, sc, src, z, ew);
-    Ket<T, N> zeta = site_noise_sun<T, N>(nz, sc);
-    Ket<T, N> d = rhs_sun<T, N>(z, zeta, H, dt, lambda);
-#pragma unroll
-    for (int a = 0; a < N; a++) {
-        d.z[a].re += z.z[a].re;
-        d.z[a].im += z.z[a].im;
-    }
-    store_ket<T, N>(Zp, C.total, sc.g, normalize_ket<T, N>(d, C.kappa[sc.i]));
-}
-
-template <typename T, int N>
-__global__ __launch_bounds__(kBlock) void k_sun_correct(Couplings<T> C, const T* __restrict__ Z,
-                                                        const T* __restrict__ Zp, T* __restrict__ Zout,
-                                                        const T* __restrict__ ewZ, const T* __restrict__ ewZp,
+                                                        T* __restrict__ Hb, const T* __restrict__ phi,
                                                         NoiseArgs<T> nz, T dt, T lambda)
 {
-    SiteCoordS sc = site_coord_s(C.blocks, C.reps);
+    const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
     if (!sc.valid) return;
-    Ket<T, N> zeta = site_noise_sun<T, N>(nz, sc);
-    KetSrc<T, N, false> s0{Z, nullptr, C.total};
-    Ket<T, N> z = load_ket<T, N>(Z, C.total, sc.g);
-    Ket<T, N> d1 = rhs_sun<T, N>(z, zeta, local_hz<T, N, false>(C, sc, s0, z, ewZ), dt, lambda);
-    KetSrc<T, N, false> s1{Zp, nullptr, C.total};
-    Ket<T, N> zp = load_ket<T, N>(Zp, C.total, sc.g);
-    Ket<T, N> d2 = rhs_sun<T, N>(zp, zeta, local_hz<T, N, false>(C, sc, s1, zp, ewZp), dt, lambda);
+    const KetSrc<T, N, false> src{Z, nullptr, C.total};
+    const Ket<T, N> z = load_ket<T, N>(Z, C.total, sc.g);
+    Ket<T, N> d = rhs_det<T, N>(z, local_hz<T, N, false>(C, sc, src, z, phi), dt, lambda);
+    Ket<T, N> zeta;
+    const bool noisy = lambda != (T)0 && nz.mode != 0;
+    if (noisy) {
+        zeta = site_noise_sun<T, N>(nz, sc);
+        const Ket<T, N> pz = neg_proj<T, N>(zeta, z);
+#pragma unroll
+        for (int a = 0; a < N; a++) {
+            d.z[a].re += pz.z[a].re;
+            d.z[a].im += pz.z[a].im;
+        }
+    }
+    Ket<T, N> zp;
+#pragma unroll
+    for (int a = 0; a < N; a++) zp.z[a] = {z.z[a].re + d.z[a].re, z.z[a].im + d.z[a].im};
+    zp = normalize_ket<T, N>(zp, C.kappa[sc.i]);
+    store_ket_images<T, N>(Zp, C.total, sc, C.G, zp);
+    Ket<T, N> h;
+#pragma unroll
+    for (int a = 0; a < N; a++) h.z[a] = {z.z[a].re + d.z[a].re / 2, z.z[a].im + d.z[a].im / 2};
+    if (noisy) {
+        const Ket<T, N> pz = neg_proj<T, N>(zeta, zp);
+#pragma unroll
+        for (int a = 0; a < N; a++) {
+            h.z[a].re += pz.z[a].re / 2;
+            h.z[a].im += pz.z[a].im / 2;
+        }
+    }
+    store_ket<T, N>(Hb, C.total, sc.g, h);
+}
+
+// Heun corrector: Z <- normalize(H + rhs_det(Z', HZ')/2).
+template <typename T, int N>
+__global__ __launch_bounds__(kBlock) void k_sun_correct(Couplings<T> C, const T* __restrict__ Zp,
+                                                        const T* __restrict__ Hb, T* __restrict__ Zout,
+                                                        const T* __restrict__ phi, T dt, T lambda)
+{
+    const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
+    if (!sc.valid) return;
+    const KetSrc<T, N, false> src{Zp, nullptr, C.total};
+    const Ket<T, N> zp = load_ket<T, N>(Zp, C.total, sc.g);
+    const Ket<T, N> d = rhs_det<T, N>(zp, local_hz<T, N, false>(C, sc, src, zp, phi), dt, lambda);
+    Ket<T, N> h = load_ket<T, N>(Hb, C.total, sc.g);
 #pragma unroll
     for (int a = 0; a < N; a++) {
-        z.z[a].re = z.z[a].re + (d1.z[a].re + d2.z[a].re) / 2;
-        z.z[a].im = z.z[a].im + (d1.z[a].im + d2.z[a].im) / 2;
+        h.z[a].re += d.z[a].re / 2;
+        h.z[a].im += d.z[a].im / 2;
     }
-    store_ket<T, N>(Zout, C.total, sc.g, normalize_ket<T, N>(z, C.kappa[sc.i]));
+    store_ket_images<T, N>(Zout, C.total, sc, C.G, normalize_ket<T, N>(h, C.kappa[sc.i]));
 }
 
 template <typename T, int N>
@@ -414,42 +417,39 @@ __global__ __launch_bounds__(kBlock) void k_sun_mid_bar(Couplings<T> C, const T*
                                                         const T* __restrict__ Z1, T* __restrict__ Zbar,
                                                         const int* __restrict__ conv_iter)
 {
-    SiteCoordS sc = site_coord_s(C.blocks, C.reps);
+    const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
     if (!sc.valid || conv_iter[sc.r] != 0) return;
-    KetSrc<T, N, true> src{Z, Z1, C.total};
-    store_ket<T, N>(Zbar, C.total, sc.g, src.get(sc.g));
+    const KetSrc<T, N, true> src{Z, Z1, C.total};
+    store_ket_images<T, N>(Zbar, C.total, sc, C.G, src.get(sc.g));
 }
 
 template <typename T, int N>
 __global__ __launch_bounds__(kBlock) void k_sun_mid_iter(Couplings<T> C, const T* __restrict__ Z,
                                                          const T* __restrict__ Z1, T* __restrict__ Z2,
-                                                         const T* __restrict__ ew, T dt, T lambda,
+                                                         const T* __restrict__ phi, T dt, T lambda,
                                                          double* __restrict__ partial,
                                                          const int* __restrict__ conv_iter)
 {
-    SiteCoordS sc = site_coord_s(C.blocks, C.reps);
+    const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
     if (conv_iter[sc.r] != 0) return;  // block-uniform
     double d2 = 0;
     if (sc.valid) {
-        KetSrc<T, N, true> src{Z, Z1, C.total};
-        Ket<T, N> zb = src.get(sc.g);
-        Ket<T, N> zero;
-#pragma unroll
-        for (int a = 0; a < N; a++) zero.z[a] = {0, 0};
-        Ket<T, N> d = rhs_sun<T, N>(zb, zero, local_hz<T, N, true>(C, sc, src, zb, ew), dt, lambda);
+        const KetSrc<T, N, true> src{Z, Z1, C.total};
+        const Ket<T, N> zb = src.get(sc.g);
+        const Ket<T, N> d = rhs_det<T, N>(zb, local_hz<T, N, true>(C, sc, src, zb, phi), dt, lambda);
         Ket<T, N> z = load_ket<T, N>(Z, C.total, sc.g);
-        Ket<T, N> z1 = load_ket<T, N>(Z1, C.total, sc.g);
+        const Ket<T, N> z1 = load_ket<T, N>(Z1, C.total, sc.g);
 #pragma unroll
         for (int a = 0; a < N; a++) {
             z.z[a].re += d.z[a].re;
             z.z[a].im += d.z[a].im;
-            double er = (double)z1.z[a].re - (double)z.z[a].re, ei = (double)z1.z[a].im - (double)z.z[a].im;
+            const double er = (double)z1.z[a].re - (double)z.z[a].re, ei = (double)z1.z[a].im - (double)z.z[a].im;
             d2 += er * er + ei * ei;
         }
-        store_ket<T, N>(Z2, C.total, sc.g, z);
+        store_ket_images<T, N>(Z2, C.total, sc, C.G, z);
     }
-    double t = block_sum_s(d2);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    const double t = block_sum(d2);
+    if (threadIdx.x == 0) partial[sc.blk] = t;
 }
 
 template <typename T, int N>
@@ -457,52 +457,51 @@ __global__ __launch_bounds__(kBlock) void k_sun_mid_finalize(Couplings<T> C, T* 
                                                              const T* __restrict__ Za, const T* __restrict__ Zb,
                                                              const int* __restrict__ conv_iter, int iter)
 {
-    SiteCoordS sc = site_coord_s(C.blocks, C.reps);
-    int m = conv_iter[sc.r];
-    if (!sc.valid || m != iter) return;
-    const T* src = (m & 1) ? Za : Zb;
-    store_ket<T, N>(Z, C.total, sc.g, normalize_ket<T, N>(load_ket<T, N>(src, C.total, sc.g), C.kappa[sc.i]));
+    const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
+    if (!sc.valid || conv_iter[sc.r] != iter) return;
+    const T* src = (iter & 1) ? Za : Zb;
+    store_ket_images<T, N>(Z, C.total, sc, C.G,
+                           normalize_ket<T, N>(load_ket<T, N>(src, C.total, sc.g), C.kappa[sc.i]));
 }
 
 template <typename T, int N>
 __global__ __launch_bounds__(kBlock) void k_sun_grad(Couplings<T> C, const T* __restrict__ Z,
-                                                     const T* __restrict__ ew, T* __restrict__ HZ)
+                                                     const T* __restrict__ phi, T* __restrict__ HZ)
 {
-    SiteCoordS sc = site_coord_s(C.blocks, C.reps);
+    const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
     if (!sc.valid) return;
-    KetSrc<T, N, false> src{Z, nullptr, C.total};
-    Ket<T, N> z = load_ket<T, N>(Z, C.total, sc.g);
-    store_ket<T, N>(HZ, C.total, sc.g, local_hz<T, N, false>(C, sc, src, z, ew));
+    const KetSrc<T, N, false> src{Z, nullptr, C.total};
+    const Ket<T, N> z = load_ket<T, N>(Z, C.total, sc.g);
+    store_ket<T, N>(HZ, C.total, sc.g, local_hz<T, N, false>(C, sc, src, z, phi));
 }
 
 template <typename T, int N>
 __global__ __launch_bounds__(kBlock) void k_sun_energy(Couplings<T> C, const T* __restrict__ Z,
-                                                       const T* __restrict__ ew, double* __restrict__ partial)
+                                                       const T* __restrict__ phi, double* __restrict__ partial)
 {
-    SiteCoordS sc = site_coord_s(C.blocks, C.reps);
+    const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
     double e = 0;
     if (sc.valid) {
         const int i = sc.i;
-        Ket<T, N> z = load_ket<T, N>(Z, C.total, sc.g);
+        const Ket<T, N> z = load_ket<T, N>(Z, C.total, sc.g);
         T sx, sy, sz;
         expected_spin<T, N>(z, sx, sy, sz);
         const T* gm = C.gmat + 9 * i;
         const T* B = C.field + 3 * i;
         for (int a = 0; a < 3; a++)
             e += (double)B[a] * ((double)gm[3 * a] * sx + (double)gm[3 * a + 1] * sy + (double)gm[3 * a + 2] * sz);
-        // onsite: Re(Z' Lambda Z)
-        const T* L = C.lam + 2 * N * N * i;
+        const T* L = C.lam + 2 * N * N * i;  // onsite: Re(Z' Lambda Z)
         for (int a = 0; a < N; a++)
             for (int b = 0; b < N; b++) {
-                Cx<T> p = cmul(Cx<T>{L[2 * (a * N + b)], L[2 * (a * N + b) + 1]}, z.z[b]);
+                const Cx<T> p = cmul(Cx<T>{L[2 * (a * N + b)], L[2 * (a * N + b) + 1]}, z.z[b]);
                 e += (double)z.z[a].re * p.re + (double)z.z[a].im * p.im;
             }
-        const int* nred = C.bnred + (long long)sc.dcls * C.nb * 3;
+        const int* __restrict__ dl = C.bdelta + sc.R.dcls * C.nb;
         for (int b = C.bstart[i]; b < C.bstart[i + 1]; b++) {
             const int fl = C.bflag[b];
             if (fl & kBondCulled) continue;
             const int j = C.bj[b];
-            Ket<T, N> zj = load_ket<T, N>(Z, C.total, neighbor_site_s(sc, nred + 3 * b, j));
+            const Ket<T, N> zj = load_ket<T, N>(Z, C.total, sc.g + (uint32_t)dl[b]);
             T tx, ty, tz;
             expected_spin<T, N>(zj, tx, ty, tz);
             const T* J = C.bJ + 9 * b;
@@ -520,29 +519,29 @@ __global__ __launch_bounds__(kBlock) void k_sun_energy(Couplings<T> C, const T* 
             }
             e += v;
         }
-        if (ew) {  // 1/2 mu.phi with mu = -g <S>
-            double mx = -((double)gm[0] * sx + (double)gm[1] * sy + (double)gm[2] * sz);
-            double my = -((double)gm[3] * sx + (double)gm[4] * sy + (double)gm[5] * sz);
-            double mz = -((double)gm[6] * sx + (double)gm[7] * sy + (double)gm[8] * sz);
-            e += 0.5 * (mx * ew[sc.g] + my * ew[C.total + sc.g] + mz * ew[2 * C.total + sc.g]);
+        if (phi) {  // 1/2 mu.phi with mu = -g <S>
+            const double mx = -((double)gm[0] * sx + (double)gm[1] * sy + (double)gm[2] * sz);
+            const double my = -((double)gm[3] * sx + (double)gm[4] * sy + (double)gm[5] * sz);
+            const double mz = -((double)gm[6] * sx + (double)gm[7] * sy + (double)gm[8] * sz);
+            e += 0.5 * (mx * ldg(phi, sc.g) + my * ldg(phi + C.total, sc.g) + mz * ldg(phi + 2 * (size_t)C.total, sc.g));
         }
     }
-    double t = block_sum_s(e);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    const double t = block_sum(e);
+    if (threadIdx.x == 0) partial[sc.blk] = t;
 }
 
 // ---------------------------------------------------------------------------
 // Launchers: dispatch the run-time N to a compiled specialisation.
 
-#define SNH_SUN_DISPATCH(N, KERNEL, ...)                                                            \
-    switch (N) {                                                                                    \
-    case 2: hipLaunchKernelGGL((KERNEL<T, 2>), __VA_ARGS__); break;                                 \
-    case 3: hipLaunchKernelGGL((KERNEL<T, 3>), __VA_ARGS__); break;                                 \
-    case 4: hipLaunchKernelGGL((KERNEL<T, 4>), __VA_ARGS__); break;                                 \
-    case 5: hipLaunchKernelGGL((KERNEL<T, 5>), __VA_ARGS__); break;                                 \
-    case 6: hipLaunchKernelGGL((KERNEL<T, 6>), __VA_ARGS__); break;                                 \
-    case 7: hipLaunchKernelGGL((KERNEL<T, 7>), __VA_ARGS__); break;                                 \
-    default: break;                                                                                 \
+#define SNH_SUN_DISPATCH(N, KERNEL, ...)                             \
+    switch (N) {                                                     \
+    case 2: hipLaunchKernelGGL((KERNEL<T, 2>), __VA_ARGS__); break;  \
+    case 3: hipLaunchKernelGGL((KERNEL<T, 3>), __VA_ARGS__); break;  \
+    case 4: hipLaunchKernelGGL((KERNEL<T, 4>), __VA_ARGS__); break;  \
+    case 5: hipLaunchKernelGGL((KERNEL<T, 5>), __VA_ARGS__); break;  \
+    case 6: hipLaunchKernelGGL((KERNEL<T, 6>), __VA_ARGS__); break;  \
+    case 7: hipLaunchKernelGGL((KERNEL<T, 7>), __VA_ARGS__); break;  \
+    default: break;                                                  \
     }
 
 template <typename T>
@@ -552,18 +551,17 @@ void launch_sun_expected_spin(const Couplings<T>& C, int nblocks, const T* Z, T*
 }
 
 template <typename T>
-void launch_sun_predict(const Couplings<T>& C, int nblocks, const T* Z, T* Zp, const T* ew, const NoiseArgs<T>& nz,
-                        T dt, T lambda, hipStream_t st)
+void launch_sun_predict(const Couplings<T>& C, int nblocks, const T* Z, T* Zp, T* H, const T* phi,
+                        const NoiseArgs<T>& nz, T dt, T lambda, hipStream_t st)
 {
-    SNH_SUN_DISPATCH(C.N, k_sun_predict, dim3(nblocks), dim3(kBlock), 0, st, C, Z, Zp, ew, nz, dt, lambda)
+    SNH_SUN_DISPATCH(C.N, k_sun_predict, dim3(nblocks), dim3(kBlock), 0, st, C, Z, Zp, H, phi, nz, dt, lambda)
 }
 
 template <typename T>
-void launch_sun_correct(const Couplings<T>& C, int nblocks, const T* Z, const T* Zp, T* Zout, const T* ewZ,
-                        const T* ewZp, const NoiseArgs<T>& nz, T dt, T lambda, hipStream_t st)
+void launch_sun_correct(const Couplings<T>& C, int nblocks, const T* Zp, const T* H, T* Zout, const T* phi, T dt,
+                        T lambda, hipStream_t st)
 {
-    SNH_SUN_DISPATCH(C.N, k_sun_correct, dim3(nblocks), dim3(kBlock), 0, st, C, Z, Zp, Zout, ewZ, ewZp, nz, dt,
-                     lambda)
+    SNH_SUN_DISPATCH(C.N, k_sun_correct, dim3(nblocks), dim3(kBlock), 0, st, C, Zp, H, Zout, phi, dt, lambda)
 }
 
 template <typename T>
@@ -574,11 +572,10 @@ void launch_sun_midpoint_bar(const Couplings<T>& C, int nblocks, const T* Z, con
 }
 
 template <typename T>
-void launch_sun_midpoint_iter(const Couplings<T>& C, int nblocks, const T* Z, const T* Z1, T* Z2, const T* ew, T dt,
-                              T lambda, double* partial, const int* conv_iter, int iter, hipStream_t st)
+void launch_sun_midpoint_iter(const Couplings<T>& C, int nblocks, const T* Z, const T* Z1, T* Z2, const T* phi, T dt,
+                              T lambda, double* partial, const int* conv_iter, hipStream_t st)
 {
-    (void)iter;
-    SNH_SUN_DISPATCH(C.N, k_sun_mid_iter, dim3(nblocks), dim3(kBlock), 0, st, C, Z, Z1, Z2, ew, dt, lambda, partial,
+    SNH_SUN_DISPATCH(C.N, k_sun_mid_iter, dim3(nblocks), dim3(kBlock), 0, st, C, Z, Z1, Z2, phi, dt, lambda, partial,
                      conv_iter)
 }
 
@@ -590,30 +587,30 @@ void launch_sun_midpoint_finalize(const Couplings<T>& C, int nblocks, T* Z, cons
 }
 
 template <typename T>
-void launch_sun_grad(const Couplings<T>& C, int nblocks, const T* Z, const T* ew, T* HZ, hipStream_t st)
+void launch_sun_grad(const Couplings<T>& C, int nblocks, const T* Z, const T* phi, T* HZ, hipStream_t st)
 {
-    SNH_SUN_DISPATCH(C.N, k_sun_grad, dim3(nblocks), dim3(kBlock), 0, st, C, Z, ew, HZ)
+    SNH_SUN_DISPATCH(C.N, k_sun_grad, dim3(nblocks), dim3(kBlock), 0, st, C, Z, phi, HZ)
 }
 
 template <typename T>
-void launch_sun_energy(const Couplings<T>& C, int nblocks, const T* Z, const T* ew, double* partial, hipStream_t st)
+void launch_sun_energy(const Couplings<T>& C, int nblocks, const T* Z, const T* phi, double* partial, hipStream_t st)
 {
-    SNH_SUN_DISPATCH(C.N, k_sun_energy, dim3(nblocks), dim3(kBlock), 0, st, C, Z, ew, partial)
+    SNH_SUN_DISPATCH(C.N, k_sun_energy, dim3(nblocks), dim3(kBlock), 0, st, C, Z, phi, partial)
 }
 
-#define SNH_SUN_INST(T)                                                                                           \
-    template void launch_sun_expected_spin<T>(const Couplings<T>&, int, const T*, T*, hipStream_t);               \
-    template void launch_sun_predict<T>(const Couplings<T>&, int, const T*, T*, const T*, const NoiseArgs<T>&, T, \
-                                        T, hipStream_t);                                                          \
-    template void launch_sun_correct<T>(const Couplings<T>&, int, const T*, const T*, T*, const T*, const T*,     \
-                                        const NoiseArgs<T>&, T, T, hipStream_t);                                  \
-    template void launch_sun_midpoint_bar<T>(const Couplings<T>&, int, const T*, const T*, T*, const int*,        \
-                                             hipStream_t);                                                        \
-    template void launch_sun_midpoint_iter<T>(const Couplings<T>&, int, const T*, const T*, T*, const T*, T, T,   \
-                                              double*, const int*, int, hipStream_t);                             \
-    template void launch_sun_midpoint_finalize<T>(const Couplings<T>&, int, T*, const T*, const T*, const int*,   \
-                                                  int, hipStream_t);                                                   \
-    template void launch_sun_grad<T>(const Couplings<T>&, int, const T*, const T*, T*, hipStream_t);              \
+#define SNH_SUN_INST(T)                                                                                          \
+    template void launch_sun_expected_spin<T>(const Couplings<T>&, int, const T*, T*, hipStream_t);              \
+    template void launch_sun_predict<T>(const Couplings<T>&, int, const T*, T*, T*, const T*,                    \
+                                        const NoiseArgs<T>&, T, T, hipStream_t);                                 \
+    template void launch_sun_correct<T>(const Couplings<T>&, int, const T*, const T*, T*, const T*, T, T,        \
+                                        hipStream_t);                                                            \
+    template void launch_sun_midpoint_bar<T>(const Couplings<T>&, int, const T*, const T*, T*, const int*,       \
+                                             hipStream_t);                                                       \
+    template void launch_sun_midpoint_iter<T>(const Couplings<T>&, int, const T*, const T*, T*, const T*, T, T,  \
+                                              double*, const int*, hipStream_t);                                 \
+    template void launch_sun_midpoint_finalize<T>(const Couplings<T>&, int, T*, const T*, const T*, const int*,  \
+                                                  int, hipStream_t);                                             \
+    template void launch_sun_grad<T>(const Couplings<T>&, int, const T*, const T*, T*, hipStream_t);             \
     template void launch_sun_energy<T>(const Couplings<T>&, int, const T*, const T*, double*, hipStream_t);
 
 SNH_SUN_INST(float)

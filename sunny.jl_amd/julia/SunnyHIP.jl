@@ -1,0 +1,133 @@
+# SunnyHIP: device-backed `step!` for Sunny.jl systems on MI355X.
+#
+# The binding a Sunny maintainer would add: a wrapper type built from a
+# `System` (the precedent is `EntangledSystem`, src/EntangledUnits/
+# TypesAndAliasing.jl:282-286, which forwards `step!`), one `ccall` per
+# entry point of include/sunny_hip.h, and `step!` methods that keep
+# `Langevin` / `ImplicitMidpoint` unchanged (src/Integrators.jl:56-135).
+# Julia is not available in the build container, so this file is the
+# documented binding rather than a tested one; the same entry points are
+# exercised through Python ctypes in tests/.
+module SunnyHIP
+
+import Sunny
+using Sunny: System, Langevin, ImplicitMidpoint
+
+const lib = joinpath(@__DIR__, "..", "sunnyhip", "libsunnyhip.so")
+
+struct SnhSpec
+    na::Int32; N::Int32; precision::Int32
+    kappa::Ptr{Float64}; g::Ptr{Float64}; field::Ptr{Float64}
+    stevens::Ptr{Float64}; kmax::Ptr{Int32}; Lambda::Ptr{Float64}
+    nbonds::Int32; bond::Ptr{Int32}; bilin::Ptr{Float64}; biquad::Ptr{Float64}
+    has_biquad::Ptr{Int32}; scalar::Ptr{Float64}
+    ewald::Int32; mu0_muB2::Float64; demag::NTuple{9,Float64}; latvecs::NTuple{9,Float64}
+    positions::Ptr{Float64}; nreplicas::Int32; dims::Ptr{Int32}
+end
+
+mutable struct DeviceSystem{N}
+    sys::System{N}
+    handle::Ptr{Cvoid}
+    step::UInt64
+    seed::UInt64
+end
+
+check(rc, h) = rc == 0 || (msg = unsafe_string(ccall((:snh_last_error, lib), Cstring, (Ptr{Cvoid},), h));
+                            rc == 3 ? error(msg) : error("libsunnyhip: " * msg))
+
+# Coupling tables in the layout of include/sunny_hip.h, built from the
+# homogeneous interactions of `sys` (src/System/Types.jl:46-74).
+function coupling_tables(sys::System{N}) where N
+    ints = sys.interactions_union
+    na = length(ints)
+    bonds = Int32[]; bilin = Float64[]; biquad = Float64[]; hasbq = Int32[]; scalar = Float64[]
+    for i in 1:na, pc in ints[i].pair
+        (; bond, isculled) = pc
+        append!(bonds, Int32[bond.i - 1, bond.j - 1, bond.n..., isculled])
+        J = pc.bilin isa Number ? pc.bilin * Sunny.Mat3(Sunny.I) : pc.bilin
+        append!(bilin, vec(permutedims(J)))                 # row-major
+        B = pc.biquad isa Number ? pc.biquad * Sunny.diagm(Sunny.scalar_biquad_metric) : pc.biquad
+        append!(biquad, vec(permutedims(Matrix(B))))
+        push!(hasbq, iszero(pc.biquad) ? 0 : 1)
+        push!(scalar, pc.scalar)
+    end
+    return (; na, bonds, bilin, biquad, hasbq, scalar)
+end
+
+function to_device(sys::System{N}; precision=64, seed=rand(UInt64)) where N
+    t = coupling_tables(sys)
+    na = t.na
+    kappa = Float64[sys.κs[1, 1, 1, i] for i in 1:na]
+    g = reduce(vcat, [vec(permutedims(sys.gs[1, 1, 1, i])) for i in 1:na])
+    field = reduce(vcat, [Vector(sys.extfield[1, 1, 1, i]) for i in 1:na])
+    stevens = zeros(28 * na); kmax = zeros(Int32, na)
+    Lambda = zeros(2 * max(N, 1)^2 * na)
+    for i in 1:na
+        os = sys.interactions_union[i].onsite
+        if N == 0
+            stevens[28(i-1)+1:28i] = vcat(os.c0, os.c2, os.c4, os.c6); kmax[i] = os.kmax
+        else
+            M = Matrix(os)
+            Lambda[2N^2*(i-1)+1:2N^2*i] = vec(permutedims(cat(real(permutedims(M)), imag(permutedims(M)); dims=3), (3, 1, 2)))
+        end
+    end
+    dims = Int32[sys.dims...]
+    ew = sys.ewald
+    pos = reduce(vcat, sys.crystal.positions)
+    GC.@preserve kappa g field stevens kmax Lambda t dims pos begin
+        spec = SnhSpec(na, N, precision, pointer(kappa), pointer(g), pointer(field), pointer(stevens),
+                       pointer(kmax), pointer(Lambda), length(t.scalar), pointer(t.bonds), pointer(t.bilin),
+                       pointer(t.biquad), pointer(t.hasbq), pointer(t.scalar), isnothing(ew) ? 0 : 1,
+                       isnothing(ew) ? 0.0 : ew.μ0_μB², isnothing(ew) ? ntuple(_ -> 0.0, 9) : Tuple(vec(permutedims(ew.demag))),
+                       Tuple(vec(permutedims(sys.crystal.latvecs))), pointer(pos), 1, pointer(dims))
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        check(ccall((:snh_create, lib), Cint, (Ref{SnhSpec}, Ref{Ptr{Cvoid}}), spec, h), C_NULL)
+    end
+    d = DeviceSystem{N}(sys, h[], 0, seed)
+    finalizer(x -> ccall((:snh_destroy, lib), Cint, (Ptr{Cvoid},), x.handle), d)
+    upload!(d)
+    return d
+end
+
+# Zero-copy: Array{Vec3,4} is 3 contiguous Float64 per site in column-major
+# (L1, L2, L3, na) order, exactly the C-ABI layout.
+function upload!(d::DeviceSystem{N}) where N
+    if N == 0
+        check(ccall((:snh_set_dipoles, lib), Cint, (Ptr{Cvoid}, Ptr{Float64}), d.handle, d.sys.dipoles), d.handle)
+    else
+        check(ccall((:snh_set_coherents, lib), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}), d.handle, d.sys.coherents), d.handle)
+    end
+end
+
+function download!(d::DeviceSystem{N}) where N
+    check(ccall((:snh_get_dipoles, lib), Cint, (Ptr{Cvoid}, Ptr{Float64}), d.handle, d.sys.dipoles), d.handle)
+    N > 0 && check(ccall((:snh_get_coherents, lib), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}), d.handle, d.sys.coherents), d.handle)
+    return d.sys
+end
+
+function Sunny.step!(d::DeviceSystem, integrator::Langevin)
+    Sunny.check_timestep_available(integrator)
+    kT = Ref(integrator.kT)
+    check(ccall((:snh_step_langevin, lib), Cint,
+                (Ptr{Cvoid}, Float64, Float64, Ptr{Float64}, Ptr{Float64}, UInt64, UInt64, Int32),
+                d.handle, integrator.dt, integrator.damping, kT, C_NULL, d.seed, d.step, 1), d.handle)
+    d.step += 1
+    return
+end
+
+function Sunny.step!(d::DeviceSystem, integrator::ImplicitMidpoint; max_iters=100)
+    Sunny.check_timestep_available(integrator)
+    iters = Ref{Int32}(0)
+    check(ccall((:snh_step_midpoint, lib), Cint,
+                (Ptr{Cvoid}, Float64, Float64, Float64, Int32, Int32, Ptr{Int32}),
+                d.handle, integrator.dt, integrator.damping, integrator.tol, max_iters, 1, iters), d.handle)
+    return
+end
+
+function Sunny.energy(d::DeviceSystem)
+    E = Ref(0.0)
+    check(ccall((:snh_energy, lib), Cint, (Ptr{Cvoid}, Ptr{Float64}), d.handle, E), d.handle)
+    return E[]
+end
+
+end

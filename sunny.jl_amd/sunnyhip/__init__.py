@@ -9,7 +9,7 @@ from .integrators import (ConvergenceError, ImplicitMidpoint, Langevin, ReplicaB
                           energy_grad_dipoles, energy_per_site, step_)
 from .operators import (SCALAR_BIQUAD_METRIC, StevensExpansion, matrix_to_stevens_coefficients, rcs_factors,
                         spin_matrices, stevens_matrices)
-from .system import (VACUUM_PERMEABILITY, Moment, System, enable_dipole_dipole_, expected_spin_host,
+from .system import (VACUUM_PERMEABILITY, Moment, System, clone_system, enable_dipole_dipole_, expected_spin_host,
                      ket_from_dipole, polarize_spins_, randomize_spins_, set_coherent_, set_dipole_, set_exchange_,
                      set_field_, set_onsite_coupling_, set_pair_biquad_, set_spin_rescaling_,
                      set_stevens_expansion_)

@@ -22,7 +22,7 @@ EXPORTS = [
     "snh_create", "snh_destroy", "snh_last_error", "snh_num_sites", "snh_replica_sites",
     "snh_set_dipoles", "snh_get_dipoles", "snh_set_coherents", "snh_get_coherents",
     "snh_step_langevin", "snh_step_midpoint", "snh_energy", "snh_energy_grad", "snh_synchronize",
-    "snh_enable_timing", "snh_kernel_time",
+    "snh_enable_timing", "snh_kernel_time", "snh_copy_calibration",
 ]
 
 
@@ -74,6 +74,7 @@ def lib():
         L.snh_synchronize.argtypes = [vp]
         L.snh_enable_timing.argtypes = [vp, C.c_int32]
         L.snh_kernel_time.argtypes = [vp, C.c_char_p, dp, C.POINTER(C.c_int64)]
+        L.snh_copy_calibration.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
         for f in EXPORTS:
             if f != "snh_last_error":
                 getattr(L, f).restype = C.c_int

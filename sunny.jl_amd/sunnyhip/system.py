@@ -209,6 +209,28 @@ class System:
         return f"System({mode}, Supercell ({'x'.join(map(str, self.dims))})x{self.na})"
 
 
+def clone_system(sys: System, seed=None) -> System:
+    """Independent copy sharing no mutable state (src/System/System.jl:138-174).
+    With `seed`, the clone gets its own generator (and noise key)."""
+    c = System.__new__(System)
+    c.__dict__.update({k: v for k, v in sys.__dict__.items() if not k.startswith("_")})
+    c.onsite = [o.copy() if isinstance(o, np.ndarray) else o for o in sys.onsite]
+    c.pairs = {b: [sc, np.array(J), None if B is None else np.array(B)] for b, (sc, J, B) in sys.pairs.items()}
+    c.extfield = sys.extfield.copy()
+    c.kappas = sys.kappas.copy()
+    sys._pull()
+    c._dip = sys._dip.copy()
+    c._coh = sys._coh.copy()
+    c._handle = None
+    c._host_dirty = True
+    c._dev_dirty = False
+    if seed is not None:
+        c.rng = np.random.default_rng(seed)
+        c.noise_seed = int(c.rng.integers(0, 2**63))
+    c.step_counter = 0
+    return c
+
+
 # ---------------------------------------------------------------------------
 # Host-side SU(N) helpers for setting states (src/Operators/Spin.jl:72-116)
 

@@ -23,6 +23,12 @@ def models(precision):
     yield "pyro2", sh.workloads.pyrochlore_model(L=2, seed=5, precision=precision)
     s = sh.workloads.pyrochlore_model(dims=(1, 2, 3), seed=6, precision=precision)
     yield "pyro_ragged", s
+    # bonds longer than the box: ghost images wrap more than once
+    s = sh.workloads.cubic_model(dims=(1, 2, 3), seed=8, precision=precision)
+    sh.set_exchange_(s, 0.3, sh.Bond(0, 0, (2, 1, 0)), propagate=[sh.Bond(0, 0, (2, 1, 0))])
+    sh.set_exchange_(s, -0.2, sh.Bond(0, 0, (0, 3, 4)), propagate=[sh.Bond(0, 0, (0, 3, 4))])
+    sh.randomize_spins_(s)
+    yield "cubic_long_bonds", s
 
 
 def aniso_model(precision):

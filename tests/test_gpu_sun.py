@@ -34,6 +34,7 @@ def spin2_model(precision):
 
 def cases(precision):
     yield "tri6x5", tri_model(precision)
+    yield "tri1x3", tri_model(precision, L1=1, L2=3, seed=5)
     yield "su5_diamond", spin2_model(precision)
 
 
