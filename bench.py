@@ -106,6 +106,9 @@ def main():
     ap.add_argument("--L", type=int, default=32)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--layout", choices=["auto", "ghost"], default="auto",
+                    help="device gather layout (auto: LDS tiles when they fit)")
+    ap.add_argument("--fp32-only", action="store_true", help="time only the FP32 batch (tuning runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (replicas shard across GPUs "
@@ -123,15 +126,23 @@ def main():
     reps = [base] + [sh.clone_system(base, seed=100 + r) for r in range(1, R)]
     for r in range(1, R):
         sh.randomize_spins_(reps[r])
-    batch = sh.ReplicaBatch(reps, precision=32)
-    log(f"FP32 batch: {R} replicas, {batch.nsites} spins")
+    layout = 1 if args.layout == "ghost" else 0
+    batch = sh.ReplicaBatch(reps, precision=32, layout=layout)
+    hb = batch._device()
+    plan = {k: hb.query(k) for k in ("layout", "tile_T1", "tile_T2", "tile_threads", "tile_work", "tile_lds")}
+    log(f"FP32 batch: {R} replicas, {batch.nsites} spins, plan {plan}")
     dt32, fams32 = timed_steps(batch, integ, args.steps, args.warmup)
     ups32 = batch.nsites * args.steps / dt32
-    log(f"FP32 batched: {dt32 * 1e3 / args.steps:.3f} ms/step, {ups32:.3e} spin-updates/s")
+    log(f"FP32 batched: {dt32 * 1e3 / args.steps:.3f} ms/step, {ups32:.3e} spin-updates/s, "
+        f"predict {fams32['dipole_predict'][0] / max(fams32['dipole_predict'][1], 1):.4f} ms, "
+        f"correct {fams32['dipole_correct'][0] / max(fams32['dipole_correct'][1], 1):.4f} ms")
+    if args.fp32_only:
+        return
 
     # FP64 plain (one replica, same initial state as replica 0)
     plain = sh.clone_system(base)
     plain.precision = 64
+    plain.layout = layout
     steps64 = max(10, args.steps // 4)
     dt64, fams64 = timed_steps(plain, integ, steps64, args.warmup)
     ups64 = plain.nsites * steps64 / dt64
@@ -149,7 +160,6 @@ def main():
 
     # plain copy of the batch's state planes: calibrates traffic counters
     # (profiles/) and gives the achievable streaming bandwidth
-    hb = batch._device()
     hb.call("snh_enable_timing", 1)
     nbytes = ctypes.c_int64()
     hb.call("snh_copy_calibration", 20, ctypes.byref(nbytes))
@@ -161,8 +171,10 @@ def main():
     # roofline of the dominant kernel of the batched step
     ms_p, n_p = fams32["dipole_predict"]
     ms_c, n_c = fams32["dipole_correct"]
+    # both stages move 36 B per spin: predict reads S, writes S' and H;
+    # correct reads S' and H, writes S
     dom, ms, n, bytes_per_site = ("dipole_correct", ms_c, n_c, 36) if ms_c >= ms_p else \
-        ("dipole_predict", ms_p, n_p, 24)
+        ("dipole_predict", ms_p, n_p, 36)
     avg_ms = ms / max(n, 1)
     alg_bytes = bytes_per_site * batch.nsites
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
@@ -183,7 +195,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": WORKLOAD, "replicas": R, "spins_per_replica": base.nsites,
-                   "integrator": "Langevin (stochastic Heun), device Philox noise"},
+                   "integrator": "Langevin (stochastic Heun), device Philox noise", "device_plan": plan},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
                      "bytes_per_spin": bytes_per_site, "avg_launch_ms": avg_ms},

@@ -65,7 +65,14 @@ typedef struct {
     const double *positions;    /* [na][3] fractional positions (Ewald only) */
     int32_t nreplicas;          /* R >= 1 */
     const int32_t *dims;        /* [R][3] supercell dims per replica (may differ without Ewald) */
+    int32_t layout;             /* SNH_LAYOUT_AUTO or SNH_LAYOUT_GHOST (below) */
 } snh_spec;
+
+/* Device layouts of the :dipole gather.  AUTO picks the LDS-tiled sweep
+ * (unpadded planes) whenever its tile fits the LDS, else the ghost-padded
+ * global gather; GHOST forces the latter.  Results agree to rounding. */
+#define SNH_LAYOUT_AUTO 0
+#define SNH_LAYOUT_GHOST 1
 
 /* Builds the device copy: replaces System construction plus the couplings set
  * by set_exchange!/set_onsite_coupling!/enable_dipole_dipole!
@@ -135,6 +142,12 @@ int snh_kernel_time(snh_system *sys, const char *family, double *ms, int64_t *la
  * (k_copy_planes), whose bytes per launch are returned in *bytes (read +
  * write).  Used to calibrate HBM traffic counters against a known size. */
 int snh_copy_calibration(snh_system *sys, int32_t repeats, int64_t *bytes);
+
+/* Diagnostics with no counterpart in the reference: integer facts about the
+ * device plan.  Keys: "layout" (SNH_LAYOUT_*, as chosen), "tile_T1",
+ * "tile_T2", "tile_threads", "tile_work" (workgroups per tiled launch),
+ * "tile_lds" (bytes of LDS per workgroup), "gather_chunk". */
+int snh_query(snh_system *sys, const char *key, int64_t *value);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <climits>
+#include <cstdlib>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -95,6 +98,137 @@ std::vector<double> quadrupole_ops(int N)
     return out;
 }
 
+// LDS tile plan for the :dipole gather: tile (T1, T2), workgroup size and
+// the split of axis 3 into segments, chosen by a cost model over all
+// candidates that fit the LDS.  Cost of one round of workgroups =
+// (layers loaded) x (positions per layer) + 3 x (site lanes computed, idle
+// lanes included), each divided by the fraction of a 128-byte line its rows
+// fill; rounds = workgroups / (CUs x workgroups per CU).
+struct TilePlan {
+    TileArgs A{};
+    int threads = 0;
+    size_t lds = 0;
+    std::vector<TileWork> work;
+    std::vector<int> rwork;  // [R+1] workgroup range per replica
+};
+
+int env_int(const char* name, int dflt)
+{
+    const char* v = std::getenv(name);
+    return (v && *v) ? std::atoi(v) : dflt;
+}
+
+bool plan_tiles(int na, const int w[3], const std::vector<std::array<int, 3>>& dims, size_t elem, size_t lds_wg,
+                size_t lds_cu, int ncu, int max_threads, int sites_per_thread, TilePlan& P)
+{
+    auto pow2 = [](int v) {
+        int p = 1;
+        while (p < v) p <<= 1;
+        return p;
+    };
+    int L1max = 1, L2max = 1, L3min = INT_MAX;
+    for (const auto& d : dims) {
+        L1max = std::max(L1max, d[0]);
+        L2max = std::max(L2max, d[1]);
+        L3min = std::min(L3min, d[2]);
+    }
+    const int nslot = 2 * w[2] + 1;
+    // overrides for tuning and tests: SNH_TILE_T1, SNH_TILE_T2, SNH_TILE_NSEG
+    const int want_T1 = env_int("SNH_TILE_T1", 0), want_T2 = env_int("SNH_TILE_T2", 0),
+              want_seg = env_int("SNH_TILE_NSEG", 0);
+    double best = 1e300;
+    bool found = false;
+    for (int T1 = std::min(64, pow2(L1max)); T1 >= 1; T1 >>= 1)
+        for (int T2 = 1; T2 <= std::min(64, pow2(L2max)); T2 <<= 1) {
+            if ((want_T1 && T1 != want_T1) || (want_T2 && T2 != want_T2)) continue;
+            const long long ROW = T1 + 2 * w[0], ROWS = T2 + 2 * w[1];
+            const long long npos = ROWS * na * ROW;
+            const long long spl = (T1 * T2 + 63) / 64 * 64;
+            const long long nsite = na * spl;
+            long long thr = std::max((nsite + sites_per_thread - 1) / sites_per_thread,
+                                     (npos + kTileLoadPer - 1) / kTileLoadPer);
+            thr = (thr + 63) / 64 * 64;
+            if (thr > max_threads) continue;
+            const size_t bytes = (size_t)nslot * npos * 3 * elem;
+            if (bytes > lds_wg) continue;
+            const int per_cu = (int)std::max<long long>(1, std::min<long long>(lds_cu / bytes, 2048 / thr));
+            long long base = 0;
+            for (const auto& d : dims) base += (long long)((d[0] + T1 - 1) / T1) * ((d[1] + T2 - 1) / T2);
+            for (int nseg = want_seg ? std::min(want_seg, L3min) : 1; nseg <= L3min; nseg *= 2) {
+                if (want_seg && nseg != std::min(want_seg, L3min)) break;
+                const long long nwork = base * nseg;
+                const long long slots = (long long)ncu * per_cu;
+                const long long rounds = (nwork + slots - 1) / slots;
+                const long long kseg = (L3min + nseg - 1) / nseg;
+                // rows shorter than a 128-byte line waste load and store bandwidth
+                const double eff_ld = std::min(1.0, (double)ROW * elem / 128.0);
+                const double eff_st = std::min(1.0, (double)T1 * elem / 128.0);
+                const double cost = (double)rounds * ((double)(kseg + 2 * w[2]) * npos / eff_ld +
+                                                      3.0 * kseg * nsite / eff_st);
+                if (cost < best) {
+                    best = cost;
+                    found = true;
+                    TileArgs& A = P.A;
+                    A = TileArgs{};
+                    A.T1 = T1;
+                    A.T2 = T2;
+                    A.lt1 = 0;
+                    while ((1 << A.lt1) < T1) A.lt1++;
+                    A.w1 = w[0];
+                    A.w2 = w[1];
+                    A.w3 = w[2];
+                    A.ROW = (int)ROW;
+                    A.nslot = nslot;
+                    A.npos = (int)npos;
+                    A.cap = (int)npos;
+                    A.spl = (int)spl;
+                    A.nsite = (int)nsite;
+                    P.threads = (int)thr;
+                    P.lds = bytes;
+                    A.nwork = nseg;  // segments per column until the work list is built
+                }
+            }
+        }
+    if (!found) return false;
+    const int nseg = P.A.nwork;
+    P.work.clear();
+    P.rwork.assign(dims.size() + 1, 0);
+    for (size_t r = 0; r < dims.size(); r++) {
+        P.rwork[r] = (int)P.work.size();
+        const int L3 = dims[r][2];
+        const int ns = std::min(nseg, L3);
+        for (int sgm = 0; sgm < ns; sgm++) {
+            const int k0 = (int)((long long)L3 * sgm / ns), k1 = (int)((long long)L3 * (sgm + 1) / ns);
+            for (int t2 = 0; t2 < dims[r][1]; t2 += P.A.T2)
+                for (int t1 = 0; t1 < dims[r][0]; t1 += P.A.T1) P.work.push_back({(int)r, t1, t2, k0, k1, {0, 0, 0}});
+        }
+    }
+    P.rwork[dims.size()] = (int)P.work.size();
+    P.A.nwork = (int)P.work.size();
+    return true;
+}
+
+// LDS byte offset of every (padded) bond from its site, per ring phase p:
+// neighbour (j, c + n) of a site (i, c) in layer k sits n1 + ROW ((j - i) +
+// na n2) positions away within a slot, in the slot of layer k + n3, which is
+// ((p + n3) mod nslot - p) slots away.  bsrc[k] < 0 marks padding (offset 0).
+std::vector<int> tile_lds_offsets(const TileArgs& A, int na, const int32_t* bond, const std::vector<int>& bsrc,
+                                  size_t elem)
+{
+    const size_t nbp = bsrc.size();
+    std::vector<int> off((size_t)A.nslot * nbp, 0);
+    for (int ph = 0; ph < A.nslot; ph++)
+        for (size_t k = 0; k < nbp; k++) {
+            if (bsrc[k] < 0) continue;
+            const int32_t* bd = bond + 6 * bsrc[k];
+            const int dslot = ((ph + bd[4]) % A.nslot + A.nslot) % A.nslot - ph;
+            const long long pos =
+                bd[2] + (long long)A.ROW * ((bd[1] - bd[0]) + (long long)na * bd[3]) + (long long)dslot * A.cap;
+            off[(size_t)ph * nbp + k] = (int)(pos * 3 * (long long)elem);
+        }
+    return off;
+}
+
 struct KernelTimer {
     bool enabled = false;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> events;
@@ -147,6 +281,7 @@ struct snh_system {
     virtual void enable_timing(bool on) = 0;
     virtual void kernel_time(const char* fam, double* ms, int64_t* n) = 0;
     virtual void copy_calibration(int repeats, int64_t* bytes) = 0;
+    virtual int64_t query(const std::string& key) const = 0;
 };
 
 namespace {
@@ -160,6 +295,10 @@ struct SystemImpl final : snh_system {
     bool ewald = false;
     std::vector<ReplicaInfo> reps;
     std::vector<int> rblk;
+    // LDS-tiled gather (unpadded layout) when it fits; else ghost-padded gather
+    bool tiled = false;
+    TilePlan tp;
+    int* d_trblk = nullptr;
     // device tables
     std::vector<void*> owned;
     Couplings<T> C{};
@@ -226,9 +365,30 @@ struct SystemImpl final : snh_system {
                         "bond atom index out of range");
             for (int a = 0; a < 3; a++) w[a] = std::max(w[a], std::abs(bd[2 + a]));
         }
-        G.w1 = w[0];
-        G.w2 = w[1];
-        G.w3 = w[2];
+        SNH_REQUIRE(sp.layout == SNH_LAYOUT_AUTO || sp.layout == SNH_LAYOUT_GHOST, SNH_ERR_INVALID,
+                    "layout must be SNH_LAYOUT_AUTO or SNH_LAYOUT_GHOST");
+        if (N == 0 && sp.layout == SNH_LAYOUT_AUTO) {
+            int dev = 0;
+            SNH_HIP(hipGetDevice(&dev));
+            hipDeviceProp_t prop;
+            SNH_HIP(hipGetDeviceProperties(&prop, dev));
+            std::vector<std::array<int, 3>> dv(R);
+            for (int r = 0; r < R; r++) {
+                dv[r] = {sp.dims[3 * r], sp.dims[3 * r + 1], sp.dims[3 * r + 2]};
+                for (int a = 0; a < 3; a++) SNH_REQUIRE(dv[r][a] >= 1, SNH_ERR_INVALID, "replica dims must be positive");
+            }
+            const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
+                                                                         : prop.sharedMemPerBlock;
+            // 256 bytes stay free for the kernels' static LDS (block reductions)
+            tiled = plan_tiles(na, w, dv, sizeof(T), prop.sharedMemPerBlock - 256, lds_cu,
+                               std::max(1, prop.multiProcessorCount), tile_max_threads<T>(),
+                               tile_sites_per_thread<T>(), tp);
+        }
+        // the tiled sweep fetches halos with the periodic wrap itself: no ghosts
+        const int lw[3] = {tiled ? 0 : w[0], tiled ? 0 : w[1], tiled ? 0 : w[2]};
+        G.w1 = lw[0];
+        G.w2 = lw[1];
+        G.w3 = lw[2];
         G.general = 0;
 
         // replicas and dims classes
@@ -239,7 +399,7 @@ struct SystemImpl final : snh_system {
             std::vector<int> d = {sp.dims[3 * r], sp.dims[3 * r + 1], sp.dims[3 * r + 2]};
             for (int a = 0; a < 3; a++) {
                 SNH_REQUIRE(d[a] >= 1, SNH_ERR_INVALID, "replica dims must be positive");
-                if (w[a] > d[a]) G.general = 1;
+                if (lw[a] > d[a]) G.general = 1;
             }
             auto it = dcls_of.find(d);
             if (it == dcls_of.end()) {
@@ -250,9 +410,9 @@ struct SystemImpl final : snh_system {
             ri.L1 = d[0];
             ri.L2 = d[1];
             ri.L3 = d[2];
-            ri.P1 = d[0] + 2 * w[0];
-            ri.P2 = d[1] + 2 * w[1];
-            ri.P3 = d[2] + 2 * w[2];
+            ri.P1 = d[0] + 2 * lw[0];
+            ri.P2 = d[1] + 2 * lw[1];
+            ri.P3 = d[2] + 2 * lw[2];
             const unsigned long long nc = (unsigned long long)d[0] * d[1] * d[2];
             const unsigned long long pc = (unsigned long long)ri.P1 * ri.P2 * ri.P3;
             ri.ncell = (int)nc;
@@ -337,17 +497,21 @@ struct SystemImpl final : snh_system {
         // padded-index offset of every bond, per dims class
         std::vector<int> bdelta(std::max<size_t>(1, dcls_dims.size() * nbp), 0);
         for (size_t d = 0; d < dcls_dims.size(); d++) {
-            const long long P1 = dcls_dims[d][0] + 2 * w[0], P2 = dcls_dims[d][1] + 2 * w[1],
-                            P3 = dcls_dims[d][2] + 2 * w[2];
+            const long long P1 = dcls_dims[d][0] + 2 * lw[0], P2 = dcls_dims[d][1] + 2 * lw[1],
+                            P3 = dcls_dims[d][2] + 2 * lw[2];
             for (int k = 0; k < nbp; k++) {
-                if (bsrc[k] < 0) continue;  // padding: the site itself
+                // padding is the site itself; so is every bond of a tiled
+                // layout, whose ghost-padded gather must never run
+                if (bsrc[k] < 0 || tiled) continue;
                 const int* bd = sp.bond + 6 * bsrc[k];
                 const long long v = bd[2] + P1 * (bd[3] + P2 * (long long)bd[4]) + P1 * P2 * P3 * (bd[1] - bd[0]);
                 bdelta[d * nbp + k] = (int)v;
             }
         }
+        std::vector<int> ldsoff(1, 0);
+        if (tiled) ldsoff = tile_lds_offsets(tp.A, na, sp.bond, bsrc, sizeof(T));
         // per-sublattice tables
-        std::vector<T> kap(na), zee(3 * na), stev(28 * na), gm(9 * na), fld(3 * na);
+        std::vector<T> kap(na), zee(3 * na), stev(28 * na), gm(9 * na), fld(3 * na), q2(9 * na, (T)0);
         std::vector<int> kmax(na, 0);
         for (int i = 0; i < na; i++) {
             kap[i] = (T)sp.kappa[i];
@@ -364,6 +528,11 @@ struct SystemImpl final : snh_system {
                 SNH_REQUIRE(km == 0 || km == 2 || km == 4 || km == 6, SNH_ERR_INVALID, "kmax must be 0, 2, 4 or 6");
                 kmax[i] = km;
                 for (int p = 0; p < 28; p++) stev[28 * i + p] = (T)sp.stevens[28 * i + p];
+                // kmax = 2: gradient 2 Q2 S of c2 . (x^2-y^2, zx, 3z^2-X, zy, 2xy),
+                // |S|^2 = X held constant (OnsiteCoupling.jl:316-425)
+                const double* c2 = sp.stevens + 28 * i + 1;
+                const double Q[9] = {2 * c2[0], 2 * c2[4], c2[1], 2 * c2[4], -2 * c2[0], c2[3], c2[1], c2[3], 6 * c2[2]};
+                for (int p = 0; p < 9; p++) q2[9 * i + p] = (T)Q[p];
             }
         }
         std::vector<T> lam, qops;
@@ -391,6 +560,7 @@ struct SystemImpl final : snh_system {
         C.kappa = own(dupload(kap));
         C.zee = own(dupload(zee));
         C.stev = own(dupload(stev));
+        C.stev_q2 = own(dupload(q2));
         C.kmax = own(dupload(kmax));
         C.gmat = own(dupload(gm));
         C.field = own(dupload(fld));
@@ -407,7 +577,12 @@ struct SystemImpl final : snh_system {
         Xp = own(dalloc<T>((size_t)planes * total));
         Xo = own(dalloc<T>((size_t)planes * total));
         SNH_HIP(hipMemset(X, 0, sizeof(T) * planes * (size_t)total));
-        partial = own(dalloc<double>(nblocks));
+        if (tiled) {
+            tp.A.work = own(dupload(tp.work));
+            tp.A.ldsoff = own(dupload(ldsoff));
+            d_trblk = own(dupload(tp.rwork));
+        }
+        partial = own(dalloc<double>(std::max<size_t>(nblocks, tp.work.size())));
         d_out = own(dalloc<double>(R));
         conv_iter = own(dalloc<int>(R));
         atol2 = own(dalloc<double>(R));
@@ -642,7 +817,9 @@ struct SystemImpl final : snh_system {
                 ewS = phi1;
             }
             hipEvent_t e = tbegin();
-            if (N == 0)
+            if (tiled)
+                launch_tile_predict<T>(C, tp.A, tp.threads, X, Xp, Xo, ewS, nz, tdt, tl, st);
+            else if (N == 0)
                 launch_dipole_predict<T>(C, nblocks, X, Xp, Xo, ewS, nz, tdt, tl, st);
             else
                 launch_sun_predict<T>(C, nblocks, X, Xp, Xo, ewS, nz, tdt, tl, st);
@@ -652,7 +829,9 @@ struct SystemImpl final : snh_system {
                 ewSp = phi2;
             }
             e = tbegin();  // the corrector overwrites the state in place (the predictor consumed it)
-            if (N == 0)
+            if (tiled)
+                launch_tile_correct<T>(C, tp.A, tp.threads, Xp, Xo, X, ewSp, tdt, tl, st);
+            else if (N == 0)
                 launch_dipole_correct<T>(C, nblocks, Xp, Xo, X, ewSp, tdt, tl, st);
             else
                 launch_sun_correct<T>(C, nblocks, Xp, Xo, X, ewSp, tdt, tl, st);
@@ -706,13 +885,16 @@ struct SystemImpl final : snh_system {
                         phi = phi1;
                     }
                     hipEvent_t e = tbegin();
-                    if (N == 0)
+                    if (tiled)
+                        launch_tile_midpoint_iter<T>(C, tp.A, tp.threads, X, S1, S2, hat, phi, tdt, tl, partial,
+                                                     conv_iter, st);
+                    else if (N == 0)
                         launch_dipole_midpoint_iter<T>(C, nblocks, X, S1, S2, hat, phi, tdt, tl, partial, conv_iter,
                                                        st);
                     else
                         launch_sun_midpoint_iter<T>(C, nblocks, X, S1, S2, phi, tdt, tl, partial, conv_iter, st);
                     tend("midpoint_iter", e);
-                    launch_midpoint_check(partial, d_rblk, R, atol2, conv_iter, it, st);
+                    launch_midpoint_check(partial, tiled ? d_trblk : d_rblk, R, atol2, conv_iter, it, st);
                     if (N == 0)
                         launch_dipole_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, it, st);
                     else
@@ -754,11 +936,13 @@ struct SystemImpl final : snh_system {
             ewald_field(X, phi1);
             phi = phi1;
         }
-        if (N == 0)
+        if (tiled)
+            launch_tile_energy<T>(C, tp.A, tp.threads, X, phi, partial, st);
+        else if (N == 0)
             launch_dipole_energy<T>(C, nblocks, X, phi, partial, st);
         else
             launch_sun_energy<T>(C, nblocks, X, phi, partial, st);
-        launch_reduce_partials(partial, d_rblk, R, d_out, st);
+        launch_reduce_partials(partial, tiled ? d_trblk : d_rblk, R, d_out, st);
         SNH_HIP(hipGetLastError());
         SNH_HIP(hipMemcpyAsync(pinned, d_out, sizeof(double) * R, hipMemcpyDeviceToHost, st));
         SNH_HIP(hipStreamSynchronize(st));
@@ -772,7 +956,9 @@ struct SystemImpl final : snh_system {
             ewald_field(X, phi1);
             phi = phi1;
         }
-        if (N == 0)
+        if (tiled)
+            launch_tile_grad<T>(C, tp.A, tp.threads, X, phi, Xp, st);
+        else if (N == 0)
             launch_dipole_grad<T>(C, nblocks, X, phi, Xp, st);
         else
             launch_sun_grad<T>(C, nblocks, X, phi, Xp, st);
@@ -800,6 +986,18 @@ struct SystemImpl final : snh_system {
         SNH_HIP(hipGetLastError());
         SNH_HIP(hipStreamSynchronize(st));
         *bytes = (int64_t)(2 * b);
+    }
+
+    int64_t query(const std::string& key) const override
+    {
+        if (key == "layout") return tiled ? SNH_LAYOUT_AUTO : SNH_LAYOUT_GHOST;
+        if (key == "tile_T1") return tiled ? tp.A.T1 : 0;
+        if (key == "tile_T2") return tiled ? tp.A.T2 : 0;
+        if (key == "tile_threads") return tiled ? tp.threads : 0;
+        if (key == "tile_work") return tiled ? tp.A.nwork : 0;
+        if (key == "tile_lds") return tiled ? (int64_t)tp.lds : 0;
+        if (key == "gather_chunk") return C.chunk;
+        throw Failure{SNH_ERR_INVALID, "unknown query key: " + key};
     }
 
     void kernel_time(const char* fam, double* ms, int64_t* n) override
@@ -996,5 +1194,13 @@ extern "C" int snh_copy_calibration(snh_system* sys, int32_t repeats, int64_t* b
     return guarded(sys, [&] {
         SNH_REQUIRE(bytes && repeats >= 0, SNH_ERR_INVALID, "invalid argument");
         sys->copy_calibration(repeats, bytes);
+    });
+}
+
+extern "C" int snh_query(snh_system* sys, const char* key, int64_t* value)
+{
+    return guarded(sys, [&] {
+        SNH_REQUIRE(key && value, SNH_ERR_INVALID, "null argument");
+        *value = sys->query(key);
     });
 }

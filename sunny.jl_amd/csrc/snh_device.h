@@ -67,6 +67,16 @@ __device__ __forceinline__ void stg(T* __restrict__ base, uint32_t idx, T v)
     *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + idx * (uint32_t)sizeof(T)) = v;
 }
 
+// Load from a read-only table through the constant address space: the
+// compiler may then keep a wavefront-uniform index in scalar registers and
+// use scalar loads even where the kernel also stores to global memory.
+template <typename X>
+__device__ __forceinline__ X ldc(const X* p, int i)
+{
+    typedef __attribute__((address_space(4))) const X* cptr;
+    return ((cptr)p)[i];
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b runs on
 // XCD b % 8), each with its own L2.  Remap so that every XCD walks a
 // contiguous range of logical blocks: blocks that share neighbour data then

@@ -18,14 +18,9 @@
 //   S + (dS1 + dS2)/2,  dS2 = -S' x xi  -  S' x (dt g' - dt lambda S' x g')
 // is linear in the noise term, so the predictor, which knows S' and xi,
 // stores H = S + dS1/2 - (S' x xi)/2 and the corrector only gathers S'.
-#include "snh_internal.h"
+#include "snh_vec.h"
 
 namespace snh {
-
-template <typename T>
-struct V3 {
-    T x, y, z;
-};
 
 template <typename T>
 struct Planes {
@@ -55,57 +50,6 @@ __device__ __forceinline__ void store3_images(T* P, uint32_t total, const SiteCo
 {
     store3(P, total, sc.g, v);
     for_each_ghost(sc, G, [&](uint32_t gi) { store3(P, total, gi, v); });
-}
-
-template <typename T>
-__device__ __forceinline__ V3<T> cross3(V3<T> a, V3<T> b)
-{
-    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-
-// kappa * s/|s| with the norm in FP64 (normalize_dipole, System.jl:421-423).
-// FP64 state: correctly rounded sqrt and divide.  FP32 state: the FP64
-// reciprocal square root (v_rsq_f64) refined by one Newton step, far below
-// the FP32 rounding of the result.
-__device__ __forceinline__ double inv_norm(double n2, double kappa, double)
-{
-    return kappa / sqrt(n2);
-}
-
-__device__ __forceinline__ double inv_norm(double n2, double kappa, float)
-{
-    double r = __builtin_amdgcn_rsq(n2);
-    r = r * (1.5 - 0.5 * n2 * r * r);
-    return kappa * r;
-}
-
-template <typename T>
-__device__ __forceinline__ V3<T> normalize_dipole(V3<T> s, T kappa)
-{
-    if (kappa == (T)0) return {0, 0, 0};
-    const double x = s.x, y = s.y, z = s.z;
-    const double f = inv_norm(x * x + y * y + z * z, (double)kappa, (T)0);
-    return {(T)(x * f), (T)(y * f), (T)(z * f)};
-}
-
-// Stevens quadrupoles of a classical dipole and their gradient, Spin.jl:48-69.
-template <typename T>
-__device__ __forceinline__ void quadrupole(V3<T> s, T Q[5])
-{
-    Q[0] = s.x * s.x - s.y * s.y;
-    Q[1] = s.z * s.x;
-    Q[2] = -s.x * s.x - s.y * s.y + 2 * s.z * s.z;
-    Q[3] = s.z * s.y;
-    Q[4] = 2 * s.y * s.x;
-}
-
-// g += sum_p w[p] * grad Q_p(s)
-template <typename T>
-__device__ __forceinline__ void add_grad_quadrupole(V3<T> s, const T w[5], V3<T>& g)
-{
-    g.x += w[0] * 2 * s.x + w[1] * s.z - w[2] * 2 * s.x + w[4] * 2 * s.y;
-    g.y += -w[0] * 2 * s.y - w[2] * 2 * s.y + w[3] * s.z + w[4] * 2 * s.x;
-    g.z += w[1] * s.x + w[2] * 4 * s.z + w[3] * s.y;
 }
 
 // Spin source for the gather: stored planes, or the midpoint
@@ -160,73 +104,12 @@ __device__ __forceinline__ V3<T> local_field(const Couplings<T>& C, const SiteCo
                 g.z += J0 * sj.z;
                 continue;
             }
-            const int fl = C.bflag[b];
-            if (fl & kBondIso) {
-                const T J0 = C.bJ0[b];
-                g.x += J0 * sj.x;
-                g.y += J0 * sj.y;
-                g.z += J0 * sj.z;
-            } else {
-                const T* J = C.bJ + 9 * b;
-                g.x += J[0] * sj.x + J[1] * sj.y + J[2] * sj.z;
-                g.y += J[3] * sj.x + J[4] * sj.y + J[5] * sj.z;
-                g.z += J[6] * sj.x + J[7] * sj.y + J[8] * sj.z;
-            }
-            if (fl & kBondBiquad) {
-                T Qj[5], w[5];
-                quadrupole(sj, Qj);
-                const T* B = C.bQ + 25 * b;
-#pragma unroll
-                for (int p = 0; p < 5; p++) {
-                    T acc = 0;
-#pragma unroll
-                    for (int q = 0; q < 5; q++) acc += B[5 * p + q] * Qj[q];
-                    w[p] = acc;
-                }
-                add_grad_quadrupole(s, w, g);
-            }
+            add_bond_field(C, b, s, sj, g);
         }
     }
-    stevens_grad<T>(s.x, s.y, s.z, C.stev + 28 * i, C.kmax[i], g.x, g.y, g.z, nullptr);
-    if (phi) {
-        const V3<T> p = planes(phi, C.total)(sc.g);
-        const T* gm = C.gmat + 9 * i;
-        g.x -= gm[0] * p.x + gm[3] * p.y + gm[6] * p.z;
-        g.y -= gm[1] * p.x + gm[4] * p.y + gm[7] * p.z;
-        g.z -= gm[2] * p.x + gm[5] * p.y + gm[8] * p.z;
-    }
+    add_onsite_field(C, i, s, g);
+    if (phi) add_ewald_field(C, i, planes(phi, C.total)(sc.g), g);
     return g;
-}
-
-// Deterministic part of rhs_dipole!: -S x (dt g - dt lambda S x g).
-template <typename T>
-__device__ __forceinline__ V3<T> rhs_det(V3<T> s, V3<T> g, T dt, T lambda)
-{
-    V3<T> v;
-    if (lambda == (T)0) {
-        v = {dt * g.x, dt * g.y, dt * g.z};
-    } else {
-        const V3<T> sg = cross3(s, g);
-        v = {dt * g.x - dt * lambda * sg.x, dt * g.y - dt * lambda * sg.y, dt * g.z - dt * lambda * sg.z};
-    }
-    const V3<T> o = cross3(s, v);
-    return {-o.x, -o.y, -o.z};
-}
-
-// fill_noise! for one site: xi = sqrt(2 dt lambda kT) * randn.
-template <typename T>
-__device__ __forceinline__ V3<T> site_noise(const NoiseArgs<T>& nz, const SiteCoord& sc)
-{
-    if (nz.mode == 0) return {0, 0, 0};
-    const T s = nz.scale[sc.r];
-    if (nz.mode == 1) {
-        const double* h = nz.host + 3 * (size_t)sc.g;
-        return {(T)h[0] * s, (T)h[1] * s, (T)h[2] * s};
-    }
-    T n[4];
-    const unsigned long long key = nz.seed + (unsigned long long)sc.r * 0x9E3779B97F4A7C15ull;
-    philox_normals4<T>(key, nz.step, (uint32_t)(sc.c + sc.R.ncell * sc.i), 0u, n);
-    return {n[0] * s, n[1] * s, n[2] * s};
 }
 
 // ---------------------------------------------------------------------------
@@ -246,17 +129,12 @@ __global__ __launch_bounds__(kBlock) void k_dipole_predict(Couplings<T> C, const
     const SpinSrc<T, false> src{Sv, Sv};
     const V3<T> s = Sv(sc.g);
     const V3<T> g = local_field<T, CH, false, kIso>(C, sc, src, s, phi);
-    V3<T> d = rhs_det(s, g, dt, lambda);
-    V3<T> xi = {0, 0, 0};
-    if (lambda != (T)0) {
-        xi = site_noise(nz, sc);
-        const V3<T> sx = cross3(s, xi);
-        d = {d.x - sx.x, d.y - sx.y, d.z - sx.z};
-    }
-    const V3<T> sp = normalize_dipole(V3<T>{s.x + d.x, s.y + d.y, s.z + d.z}, C.kappa[sc.i]);
+    const bool noisy = lambda != (T)0;
+    const V3<T> xi = noisy ? dipole_noise(nz, sc.r, (uint32_t)(sc.c + sc.R.ncell * sc.i), sc.g) : V3<T>{0, 0, 0};
+    V3<T> sp, h;
+    heun_predict(s, g, xi, noisy, C.kappa[sc.i], dt, lambda, sp, h);
     store3_images(Sp, C.total, sc, C.G, sp);
-    const V3<T> spx = cross3(sp, xi);
-    store3(H, C.total, sc.g, V3<T>{s.x + d.x / 2 - spx.x / 2, s.y + d.y / 2 - spx.y / 2, s.z + d.z / 2 - spx.z / 2});
+    store3(H, C.total, sc.g, h);
 }
 
 // Heun corrector: S <- kappa*normalize(H + rhs_det(S', dE(S'))/2).

@@ -23,6 +23,7 @@ struct Couplings {
     const T* kappa;      // [na]
     const T* zee;        // [na][3] = g^T B
     const T* stev;       // [na][28]
+    const T* stev_q2;    // [na][9] gradient matrix 2 Q2 of a kmax = 2 anisotropy (row-major)
     const int* kmax;     // [na]
     const T* gmat;       // [na][9]
     const T* field;      // [na][3]
@@ -47,6 +48,50 @@ struct NoiseArgs {
 };
 
 enum { kBondIso = 1, kBondBiquad = 2, kBondCulled = 4 };
+
+// Largest tiled workgroup: 1024 threads in FP32 (<= 128 VGPRs), 512 in FP64.
+template <typename T>
+constexpr int tile_max_threads()
+{
+    return sizeof(T) == 8 ? 512 : 1024;
+}
+// Sites computed per thread per layer: 1 in FP32, 2 in FP64 (so that 1024
+// sites of a layer fit 512 threads).
+template <typename T>
+constexpr int tile_sites_per_thread()
+{
+    return sizeof(T) == 8 ? 2 : 1;
+}
+constexpr int kTileLoadPer = 4;  // layer positions fetched per thread
+
+// LDS-tiled gather (snh_tile.hip).  State planes use the unpadded layout
+// (ghost width 0): site (cell c, sublattice i) of replica r sits at
+// off_r + c + ncell_r * i.  A workgroup owns a column of T1 x T2 cells of
+// one replica and sweeps it along axis 3 over layers [k0, k1): the layers
+// k - w3 .. k + w3 of a tile widened by (w1, w2) cells on each side live in
+// an LDS ring of 2 w3 + 1 slots, so every neighbour of a layer-k site is in
+// LDS at a bond-constant offset.  Halo cells are fetched with the periodic
+// wrap of bonded_site (src/System/System.jl:204-212).
+struct TileWork {
+    int r;       // replica
+    int t1, t2;  // first cell of the tile along axes 1 and 2
+    int k0, k1;  // layer range along axis 3
+    int pad[3];
+};
+
+struct TileArgs {
+    const TileWork* work;
+    const int* ldsoff;  // [nslot][nb] byte offset of bond b from the site, ring phase p = k mod nslot
+    int T1, T2, lt1;    // tile cells along axes 1, 2; T1 = 1 << lt1
+    int w1, w2, w3;     // halo widths
+    int ROW;            // T1 + 2 w1 positions per tile row
+    int nslot;          // 2 w3 + 1
+    int cap;            // positions per slot (>= ROWS na ROW)
+    int npos;           // (T2 + 2 w2) * na * ROW positions loaded per layer
+    int spl;            // compute lanes per sublattice per layer: T1 T2 rounded up to 64
+    int nsite;          // na * spl
+    int nwork;          // workgroups
+};
 
 // Geometry of the Ewald FFT grid (one replica; all replicas share it).
 struct EwaldGeom {
@@ -95,6 +140,25 @@ void launch_midpoint_check(const double* partial, const int* rblk, int R, const 
                            int iter, hipStream_t st);
 void launch_fill_int(int* p, int n, int v, hipStream_t st);
 void launch_copy_planes(const void* src, void* dst, size_t bytes, hipStream_t st);
+
+// LDS-tiled :dipole kernels (snh_tile.hip); nthreads = workgroup size.
+size_t tile_lds_bytes(const TileArgs& A, size_t elem);
+template <typename T>
+void launch_tile_predict(const Couplings<T>& C, const TileArgs& A, int nthreads, const T* S, T* Sp, T* H,
+                         const T* phi, const NoiseArgs<T>& nz, T dt, T lambda, hipStream_t st);
+template <typename T>
+void launch_tile_correct(const Couplings<T>& C, const TileArgs& A, int nthreads, const T* Sp, const T* H, T* Sout,
+                         const T* phi, T dt, T lambda, hipStream_t st);
+template <typename T>
+void launch_tile_midpoint_iter(const Couplings<T>& C, const TileArgs& A, int nthreads, const T* S, const T* S1,
+                               T* S2, const T* Shat, const T* phi, T dt, T lambda, double* partial,
+                               const int* conv_iter, hipStream_t st);
+template <typename T>
+void launch_tile_grad(const Couplings<T>& C, const TileArgs& A, int nthreads, const T* S, const T* phi, T* dE,
+                      hipStream_t st);
+template <typename T>
+void launch_tile_energy(const Couplings<T>& C, const TileArgs& A, int nthreads, const T* S, const T* phi,
+                        double* partial, hipStream_t st);
 
 // Ewald (snh_ewald.hip).
 size_t ewald_fft_lds_bytes(long long nk, size_t elem);

@@ -23,6 +23,7 @@ struct SnhSpec
     has_biquad::Ptr{Int32}; scalar::Ptr{Float64}
     ewald::Int32; mu0_muB2::Float64; demag::NTuple{9,Float64}; latvecs::NTuple{9,Float64}
     positions::Ptr{Float64}; nreplicas::Int32; dims::Ptr{Int32}
+    layout::Int32   # 0 = SNH_LAYOUT_AUTO
 end
 
 mutable struct DeviceSystem{N}
@@ -79,7 +80,7 @@ function to_device(sys::System{N}; precision=64, seed=rand(UInt64)) where N
                        pointer(kmax), pointer(Lambda), length(t.scalar), pointer(t.bonds), pointer(t.bilin),
                        pointer(t.biquad), pointer(t.hasbq), pointer(t.scalar), isnothing(ew) ? 0 : 1,
                        isnothing(ew) ? 0.0 : ew.μ0_μB², isnothing(ew) ? ntuple(_ -> 0.0, 9) : Tuple(vec(permutedims(ew.demag))),
-                       Tuple(vec(permutedims(sys.crystal.latvecs))), pointer(pos), 1, pointer(dims))
+                       Tuple(vec(permutedims(sys.crystal.latvecs))), pointer(pos), 1, pointer(dims), 0)
         h = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:snh_create, lib), Cint, (Ref{SnhSpec}, Ref{Ptr{Cvoid}}), spec, h), C_NULL)
     end

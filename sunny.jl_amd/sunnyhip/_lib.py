@@ -10,7 +10,9 @@ import os
 
 import numpy as np
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsunnyhip.so")
+# SUNNYHIP_LIB selects another build of the library (A/B timing of variants).
+LIB_PATH = os.environ.get("SUNNYHIP_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                          "libsunnyhip.so")
 
 SNH_OK = 0
 SNH_ERR_INVALID = 1
@@ -22,7 +24,7 @@ EXPORTS = [
     "snh_create", "snh_destroy", "snh_last_error", "snh_num_sites", "snh_replica_sites",
     "snh_set_dipoles", "snh_get_dipoles", "snh_set_coherents", "snh_get_coherents",
     "snh_step_langevin", "snh_step_midpoint", "snh_energy", "snh_energy_grad", "snh_synchronize",
-    "snh_enable_timing", "snh_kernel_time", "snh_copy_calibration",
+    "snh_enable_timing", "snh_kernel_time", "snh_copy_calibration", "snh_query",
 ]
 
 
@@ -35,6 +37,7 @@ class SnhSpec(C.Structure):
         ("biquad", C.POINTER(C.c_double)), ("has_biquad", C.POINTER(C.c_int32)), ("scalar", C.POINTER(C.c_double)),
         ("ewald", C.c_int32), ("mu0_muB2", C.c_double), ("demag", C.c_double * 9), ("latvecs", C.c_double * 9),
         ("positions", C.POINTER(C.c_double)), ("nreplicas", C.c_int32), ("dims", C.POINTER(C.c_int32)),
+        ("layout", C.c_int32),
     ]
 
 
@@ -75,6 +78,7 @@ def lib():
         L.snh_enable_timing.argtypes = [vp, C.c_int32]
         L.snh_kernel_time.argtypes = [vp, C.c_char_p, dp, C.POINTER(C.c_int64)]
         L.snh_copy_calibration.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
+        L.snh_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
         for f in EXPORTS:
             if f != "snh_last_error":
                 getattr(L, f).restype = C.c_int
@@ -127,6 +131,7 @@ class Handle:
         s.positions = dptr(spec["positions"])
         s.nreplicas = spec["nreplicas"]
         s.dims = iptr(spec["dims"])
+        s.layout = spec.get("layout", 0)
         h = C.c_void_p()
         rc = L.snh_create(C.byref(s), C.byref(h))
         if rc != SNH_OK:
@@ -143,3 +148,9 @@ class Handle:
 
     def call(self, name, *args):
         check(getattr(lib(), name)(self.h, *args), self.h)
+
+    def query(self, key):
+        """Integer fact about the device plan (snh_query), e.g. "layout"."""
+        v = C.c_int64()
+        self.call("snh_query", key.encode(), C.byref(v))
+        return v.value

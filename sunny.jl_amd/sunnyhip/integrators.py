@@ -135,7 +135,7 @@ class ReplicaBatch:
     This is the device counterpart of stepping clones of a System under
     `Threads.@threads` (src/MonteCarlo/ParallelTempering.jl:80-95)."""
 
-    def __init__(self, systems, precision=32):
+    def __init__(self, systems, precision=32, layout=0):
         self.systems = list(systems)
         s0 = self.systems[0]
         self.N = s0.N
@@ -150,6 +150,7 @@ class ReplicaBatch:
                 if not np.array_equal(sp[k], ref[k]):
                     raise ValueError("replicas of a ReplicaBatch must share their couplings")
         self._spec = s0.spec(precision=self.precision, dims_list=[s.dims for s in self.systems])
+        self._spec["layout"] = int(layout)
         self._handle = None
         self._host_dirty = True
         self._dev_dirty = False

@@ -72,6 +72,7 @@ class System:
         self.pairs = {}  # Bond -> [scalar, bilin 3x3, biquad 5x5 or None]
         self.ewald = None
         self.precision = int(precision)
+        self.layout = 0  # device gather layout: 0 automatic (LDS tiles when they fit), 1 ghost-padded
         self.rng = np.random.default_rng(seed)
         self.noise_seed = int(self.rng.integers(0, 2**63))
         self.step_counter = 0
@@ -187,6 +188,7 @@ class System:
             "latvecs": self.crystal.latvecs.reshape(9).copy(),
             "positions": np.ascontiguousarray(self.crystal.positions, float),
             "nreplicas": len(dims_list), "dims": np.ascontiguousarray(np.array(dims_list, np.int32).reshape(-1, 3)),
+            "layout": int(getattr(self, "layout", 0)),
         }
 
     # ---- device copy

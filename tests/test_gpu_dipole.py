@@ -52,10 +52,11 @@ def aniso_model(precision):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
-def test_grad_and_energy(precision):
+def test_grad_and_energy(precision, layout):
     cases = list(models(precision)) + [("diamond_aniso", aniso_model(precision))]
     for name, s in cases:
-        announce(f"grad/energy {name} fp{precision}")
+        s.layout = layout
+        announce(f"grad/energy {name} fp{precision} layout {layout}")
         S = s.dipoles_flat()
         o = O.Oracle(s.spec())
         g_dev = sh.energy_grad_dipoles(s)
@@ -66,9 +67,10 @@ def test_grad_and_energy(precision):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
-def test_langevin_host_noise(precision):
+def test_langevin_host_noise(precision, layout):
     for name, s in list(models(precision)) + [("diamond_aniso", aniso_model(precision))]:
-        announce(f"langevin host-noise {name} fp{precision}")
+        s.layout = layout
+        announce(f"langevin host-noise {name} fp{precision} layout {layout}")
         o = O.Oracle(s.spec())
         rng = np.random.default_rng(11)
         integ = sh.Langevin(0.02, damping=0.1, kT=0.5)
@@ -100,9 +102,10 @@ def test_langevin_device_philox_trajectory():
 
 
 @pytest.mark.parametrize("precision", [64, 32])
-def test_midpoint(precision):
+def test_midpoint(precision, layout):
     for name, s in list(models(precision)) + [("diamond_aniso", aniso_model(precision))]:
-        announce(f"midpoint {name} fp{precision}")
+        s.layout = layout
+        announce(f"midpoint {name} fp{precision} layout {layout}")
         o = O.Oracle(s.spec())
         damping = 0.1 if name.startswith("cubic") else 0.0
         tol = 1e-12 if precision == 64 else 1e-6
@@ -132,12 +135,12 @@ def test_midpoint_nonconvergence_message():
 
 
 @pytest.mark.parametrize("precision", [64, 32])
-def test_replica_batch_uneven(precision):
+def test_replica_batch_uneven(precision, layout):
     """Replicas of different dims in one launch, each matching its own oracle."""
     dims = [(4, 4, 4), (3, 2, 5), (6, 1, 2)]
     systems = [sh.workloads.cubic_model(dims=d, seed=20 + r, precision=precision) for r, d in enumerate(dims)]
-    announce(f"replica batch cubic {dims} fp{precision}")
-    batch = sh.ReplicaBatch(systems, precision=precision)
+    announce(f"replica batch cubic {dims} fp{precision} layout {layout}")
+    batch = sh.ReplicaBatch(systems, precision=precision, layout=layout)
     kTs = np.array([0.1, 0.5, 1.0])
     integ = sh.Langevin(0.03, damping=0.1, kT=0.0)
     refs = [s.dipoles_flat() for s in systems]
@@ -153,3 +156,36 @@ def test_replica_batch_uneven(precision):
             S = o.step_langevin(S=S, noise=noise, dt=integ.dt, damping=integ.damping, kT=kTs[r])
         assert rel_err(s.dipoles_flat(), S, scale=1.0) <= (1e-10 if precision == 64 else 1e-4)
         assert abs(E_dev[r] - o.energy(S)) <= (1e-10 if precision == 64 else 1e-4) * energy_scale(s)
+
+
+def test_layout_selection():
+    """The automatic plan picks the LDS-tiled sweep for :dipole models and
+    the ghost-padded gather when asked or in :SUN mode."""
+    s = sh.workloads.pyrochlore_model(L=4, seed=1, precision=32)
+    h = s._device()
+    assert h.query("layout") == 0 and h.query("tile_T1") >= 1 and h.query("tile_work") >= 1
+    assert h.query("tile_lds") > 0
+    g = sh.workloads.pyrochlore_model(L=4, seed=1, precision=32)
+    g.layout = 1
+    assert g._device().query("layout") == 1
+    np.testing.assert_allclose(sh.energy(s), sh.energy(g), rtol=1e-6)
+    u = sh.workloads.triangular_sun_model(L1=4, L2=4)
+    assert u._device().query("layout") == 1
+
+
+def test_tiled_bench_shape_step():
+    """One FP32 Heun step of the 32^3 pyrochlore replica with the automatic
+    (benchmark) tile plan against the ghost-padded gather on the same state
+    and noise, plus the energy after the step against the FP64 oracle."""
+    a = sh.workloads.pyrochlore_model(L=32, seed=3, precision=32)
+    b = sh.clone_system(a)
+    b.layout = 1
+    announce(f"bench-shape tile plan T1={a._device().query('tile_T1')} T2={a._device().query('tile_T2')}")
+    integ = sh.Langevin(0.025, damping=0.2, kT=0.3)
+    noise = np.random.default_rng(4).standard_normal((a.nsites, 3))
+    sh.step_(a, integ, noise=noise)
+    sh.step_(b, integ, noise=noise)
+    assert rel_err(a.dipoles_flat(), b.dipoles_flat(), scale=1.5) <= 2e-5
+    o = O.Oracle(a.spec())
+    E_ref = o.energy(a.dipoles_flat())
+    assert abs(sh.energy(a) - E_ref) <= 1e-6 * energy_scale(a)
