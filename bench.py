@@ -109,6 +109,7 @@ def main():
     ap.add_argument("--layout", choices=["auto", "ghost"], default="auto",
                     help="device gather layout (auto: LDS tiles when they fit)")
     ap.add_argument("--fp32-only", action="store_true", help="time only the FP32 batch (tuning runs)")
+    ap.add_argument("--fp64-only", action="store_true", help="time only the plain FP64 step (tuning runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (replicas shard across GPUs "
@@ -122,14 +123,26 @@ def main():
     integ = sh.Langevin(0.025, damping=0.2, kT=0.3)
     R = args.replicas
 
+    layout = 1 if args.layout == "ghost" else 0
+    if args.fp64_only:
+        plain = sh.clone_system(base)
+        plain.precision = 64
+        plain.layout = layout
+        h = plain._device()
+        plan = {k: h.query(k) for k in ("layout", "tile_sp", "tile_T1", "tile_T2", "tile_threads", "tile_work")}
+        steps64 = max(10, args.steps // 4)
+        dt64, f64 = timed_steps(plain, integ, steps64, args.warmup)
+        log(f"FP64 plain {plan}: {dt64 * 1e3 / steps64:.4f} ms/step, {plain.nsites * steps64 / dt64:.3e} spin-updates/s")
+        return
+
     # FP32 batched
     reps = [base] + [sh.clone_system(base, seed=100 + r) for r in range(1, R)]
     for r in range(1, R):
         sh.randomize_spins_(reps[r])
-    layout = 1 if args.layout == "ghost" else 0
     batch = sh.ReplicaBatch(reps, precision=32, layout=layout)
     hb = batch._device()
-    plan = {k: hb.query(k) for k in ("layout", "tile_T1", "tile_T2", "tile_threads", "tile_work", "tile_lds")}
+    plan = {k: hb.query(k) for k in ("layout", "tile_sp", "tile_T1", "tile_T2", "tile_threads", "tile_work",
+                                     "tile_lds")}
     log(f"FP32 batch: {R} replicas, {batch.nsites} spins, plan {plan}")
     dt32, fams32 = timed_steps(batch, integ, args.steps, args.warmup)
     ups32 = batch.nsites * args.steps / dt32

@@ -100,10 +100,10 @@ std::vector<double> quadrupole_ops(int N)
 
 // LDS tile plan for the :dipole gather: tile (T1, T2), workgroup size and
 // the split of axis 3 into segments, chosen by a cost model over all
-// candidates that fit the LDS.  Cost of one round of workgroups =
-// (layers loaded) x (positions per layer) + 3 x (site lanes computed, idle
-// lanes included), each divided by the fraction of a 128-byte line its rows
-// fill; rounds = workgroups / (CUs x workgroups per CU).
+// candidates that fit the LDS.  Cost of one workgroup = (layers loaded) x
+// (positions per layer) + 3 x (site lanes computed, idle lanes included),
+// each divided by the fraction of a 128-byte line its rows fill; a CU works
+// through ceil(workgroups / CUs) of them, 1.3x slower when only one fits.
 struct TilePlan {
     TileArgs A{};
     int threads = 0;
@@ -119,7 +119,7 @@ int env_int(const char* name, int dflt)
 }
 
 bool plan_tiles(int na, const int w[3], const std::vector<std::array<int, 3>>& dims, size_t elem, size_t lds_wg,
-                size_t lds_cu, int ncu, int max_threads, int sites_per_thread, TilePlan& P)
+                size_t lds_cu, int ncu, TilePlan& P)
 {
     auto pow2 = [](int v) {
         int p = 1;
@@ -135,36 +135,54 @@ bool plan_tiles(int na, const int w[3], const std::vector<std::array<int, 3>>& d
     const int nslot = 2 * w[2] + 1;
     // overrides for tuning and tests: SNH_TILE_T1, SNH_TILE_T2, SNH_TILE_NSEG
     const int want_T1 = env_int("SNH_TILE_T1", 0), want_T2 = env_int("SNH_TILE_T2", 0),
-              want_seg = env_int("SNH_TILE_NSEG", 0);
+              want_seg = env_int("SNH_TILE_NSEG", 0), want_sp = env_int("SNH_TILE_SP", 0);
     double best = 1e300;
     bool found = false;
+    for (int sp = 1; sp <= 2; sp++)
     for (int T1 = std::min(64, pow2(L1max)); T1 >= 1; T1 >>= 1)
         for (int T2 = 1; T2 <= std::min(64, pow2(L2max)); T2 <<= 1) {
-            if ((want_T1 && T1 != want_T1) || (want_T2 && T2 != want_T2)) continue;
+            if ((want_T1 && T1 != want_T1) || (want_T2 && T2 != want_T2) || (want_sp && sp != want_sp)) continue;
+            // workgroup size limits of the kernels (tile_threads_max)
+            const int max_threads = (elem == 4 && sp == 1) ? 1024 : 512;
+            const int sites_per_thread = sp;
             const long long ROW = T1 + 2 * w[0], ROWS = T2 + 2 * w[1];
             const long long npos = ROWS * na * ROW;
             const long long spl = (T1 * T2 + 63) / 64 * 64;
             const long long nsite = na * spl;
+            const int load_per = tile_load_per(sp);
             long long thr = std::max((nsite + sites_per_thread - 1) / sites_per_thread,
-                                     (npos + kTileLoadPer - 1) / kTileLoadPer);
+                                     (npos + load_per - 1) / load_per);
             thr = (thr + 63) / 64 * 64;
             if (thr > max_threads) continue;
             const size_t bytes = (size_t)nslot * npos * 3 * elem;
             if (bytes > lds_wg) continue;
-            const int per_cu = (int)std::max<long long>(1, std::min<long long>(lds_cu / bytes, 2048 / thr));
+            // workgroups resident per CU: LDS, 2048 threads, and the VGPR
+            // budget (about 88 / 128 VGPRs for FP32 with 1 / 2 sites per
+            // thread, 150 for FP64; 512 VGPRs per SIMD lane, 4 SIMDs)
+            const int vgpr = elem == 4 ? (sp == 1 ? 88 : 128) : 150;
+            const long long waves_cu = 4LL * (512 / vgpr), waves_wg = thr / 64;
+            const int per_cu = (int)std::max<long long>(
+                1, std::min({(long long)(lds_cu / bytes), 2048 / thr, waves_cu / waves_wg}));
+            // below 16 resident waves per CU the sweep cannot hide its load
+            // latency; the ghost-padded gather then does better (an explicit
+            // tile request is honoured regardless)
+            const bool forced = want_T1 || want_T2 || want_seg || want_sp;
+            if (!forced && per_cu * waves_wg < 16) continue;
             long long base = 0;
             for (const auto& d : dims) base += (long long)((d[0] + T1 - 1) / T1) * ((d[1] + T2 - 1) / T2);
             for (int nseg = want_seg ? std::min(want_seg, L3min) : 1; nseg <= L3min; nseg *= 2) {
                 if (want_seg && nseg != std::min(want_seg, L3min)) break;
                 const long long nwork = base * nseg;
-                const long long slots = (long long)ncu * per_cu;
-                const long long rounds = (nwork + slots - 1) / slots;
+                // a CU retires workgroups one after another; a lone workgroup
+                // per CU idles at its barriers (factor 1.3)
+                const long long rounds = (nwork + ncu - 1) / ncu;
                 const long long kseg = (L3min + nseg - 1) / nseg;
                 // rows shorter than a 128-byte line waste load and store bandwidth
                 const double eff_ld = std::min(1.0, (double)ROW * elem / 128.0);
                 const double eff_st = std::min(1.0, (double)T1 * elem / 128.0);
-                const double cost = (double)rounds * ((double)(kseg + 2 * w[2]) * npos / eff_ld +
-                                                      3.0 * kseg * nsite / eff_st);
+                const double bubble = per_cu >= 2 ? 1.0 : 1.3;
+                const double cost = (double)rounds * bubble *
+                                    ((double)(kseg + 2 * w[2]) * npos / eff_ld + 3.0 * kseg * nsite / eff_st);
                 if (cost < best) {
                     best = cost;
                     found = true;
@@ -183,6 +201,7 @@ bool plan_tiles(int na, const int w[3], const std::vector<std::array<int, 3>>& d
                     A.cap = (int)npos;
                     A.spl = (int)spl;
                     A.nsite = (int)nsite;
+                    A.sp = sp;
                     P.threads = (int)thr;
                     P.lds = bytes;
                     A.nwork = nseg;  // segments per column until the work list is built
@@ -381,8 +400,7 @@ struct SystemImpl final : snh_system {
                                                                          : prop.sharedMemPerBlock;
             // 256 bytes stay free for the kernels' static LDS (block reductions)
             tiled = plan_tiles(na, w, dv, sizeof(T), prop.sharedMemPerBlock - 256, lds_cu,
-                               std::max(1, prop.multiProcessorCount), tile_max_threads<T>(),
-                               tile_sites_per_thread<T>(), tp);
+                               std::max(1, prop.multiProcessorCount), tp);
         }
         // the tiled sweep fetches halos with the periodic wrap itself: no ghosts
         const int lw[3] = {tiled ? 0 : w[0], tiled ? 0 : w[1], tiled ? 0 : w[2]};
@@ -994,6 +1012,7 @@ struct SystemImpl final : snh_system {
         if (key == "tile_T1") return tiled ? tp.A.T1 : 0;
         if (key == "tile_T2") return tiled ? tp.A.T2 : 0;
         if (key == "tile_threads") return tiled ? tp.threads : 0;
+        if (key == "tile_sp") return tiled ? tp.A.sp : 0;
         if (key == "tile_work") return tiled ? tp.A.nwork : 0;
         if (key == "tile_lds") return tiled ? (int64_t)tp.lds : 0;
         if (key == "gather_chunk") return C.chunk;

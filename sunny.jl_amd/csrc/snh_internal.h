@@ -49,20 +49,18 @@ struct NoiseArgs {
 
 enum { kBondIso = 1, kBondBiquad = 2, kBondCulled = 4 };
 
-// Largest tiled workgroup: 1024 threads in FP32 (<= 128 VGPRs), 512 in FP64.
-template <typename T>
-constexpr int tile_max_threads()
+// Largest tiled workgroup for SP sites per thread: 1024 threads (<= 128
+// VGPRs) for FP32 with one site per thread, else 512.
+template <typename T, int SP>
+constexpr int tile_threads_max()
 {
-    return sizeof(T) == 8 ? 512 : 1024;
+    return (sizeof(T) == 4 && SP == 1) ? 1024 : 512;
 }
-// Sites computed per thread per layer: 1 in FP32, 2 in FP64 (so that 1024
-// sites of a layer fit 512 threads).
-template <typename T>
-constexpr int tile_sites_per_thread()
+// Layer positions fetched per thread: 4 with one site per thread, 6 with two.
+constexpr int tile_load_per(int sp)
 {
-    return sizeof(T) == 8 ? 2 : 1;
+    return sp == 1 ? 4 : 6;
 }
-constexpr int kTileLoadPer = 4;  // layer positions fetched per thread
 
 // LDS-tiled gather (snh_tile.hip).  State planes use the unpadded layout
 // (ghost width 0): site (cell c, sublattice i) of replica r sits at
@@ -91,6 +89,7 @@ struct TileArgs {
     int spl;            // compute lanes per sublattice per layer: T1 T2 rounded up to 64
     int nsite;          // na * spl
     int nwork;          // workgroups
+    int sp;             // sites computed per thread per layer (1 or 2)
 };
 
 // Geometry of the Ewald FFT grid (one replica; all replicas share it).

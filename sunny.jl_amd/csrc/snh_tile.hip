@@ -22,6 +22,9 @@
 //   energy_aux                src/System/Interactions.jl:435-508
 #include "snh_vec.h"
 
+#include <map>
+#include <mutex>
+
 namespace snh {
 
 constexpr int kTileThreads = 1024;
@@ -35,7 +38,6 @@ constexpr int kTileThreads = 1024;
 #else
 #define SNH_TILE_OCC
 #endif
-constexpr int kLoadPer = kTileLoadPer;
 
 __device__ __forceinline__ int wrap(int v, int L)
 {
@@ -106,11 +108,12 @@ struct TileSite {
 // pre(k) (own-site global loads), issues the next layer's loads, then calls
 // body(k, phase).  NSRC = 2 loads the midpoint state
 // kappa*normalize((S0 + S1)/2) instead of S0.
-template <typename T, int NSRC, typename Pre, typename Body>
+template <typename T, int NSRC, int SP, typename Pre, typename Body>
 __device__ __forceinline__ void tile_sweep(const Couplings<T>& C, const TileArgs& A, const TileWork& W,
                                            const ReplicaInfo& R, const T* __restrict__ s0, const T* __restrict__ s1,
                                            unsigned char* smem, Pre pre, Body body)
 {
+    constexpr int kLoadPer = tile_load_per(SP);
     T* lds = reinterpret_cast<T*>(smem);
     const int NT = blockDim.x;
     const uint32_t total = C.total;
@@ -133,24 +136,25 @@ __device__ __forceinline__ void tile_sweep(const Couplings<T>& C, const TileArgs
         lk[m] = NSRC == 2 ? ldc(C.kappa, j) : (T)0;
     }
     T buf[kLoadPer][3 * NSRC];
-    auto fetch = [&](int kk) {
+    auto fetch = [&](int kk) __attribute__((always_inline)) {
         const uint32_t o = lay * (uint32_t)wrap(kk, R.L3);
 #pragma unroll
         for (int m = 0; m < kLoadPer; m++) {
             const uint32_t g = lg[m] + o;
 #pragma unroll
             for (int a = 0; a < 3; a++) buf[m][a] = ldg(s0, g + a * total);
-            if (NSRC == 2)
+            if constexpr (NSRC == 2) {
 #pragma unroll
                 for (int a = 0; a < 3; a++) buf[m][3 + a] = ldg(s1, g + a * total);
+            }
         }
     };
-    auto put = [&](int slot) {
+    auto put = [&](int slot) __attribute__((always_inline)) {
 #pragma unroll
         for (int m = 0; m < kLoadPer; m++) {
             if (lp[m] < 0) continue;
             V3<T> v = {buf[m][0], buf[m][1], buf[m][2]};
-            if (NSRC == 2)
+            if constexpr (NSRC == 2)
                 v = normalize_dipole(
                     V3<T>{(v.x + buf[m][3]) / 2, (v.y + buf[m][4]) / 2, (v.z + buf[m][5]) / 2}, lk[m]);
             T* d = lds + 3 * (slot * A.cap + lp[m]);
@@ -213,8 +217,7 @@ __device__ __forceinline__ V3<T> tile_field(const Couplings<T>& C, const int* __
 // ---------------------------------------------------------------------------
 // Kernels
 //
-// Every kernel computes up to tile_sites_per_thread<T>() sites per thread and
-// layer.  Own-site global inputs (H, S, S1, the Ewald potential) are loaded
+// Every kernel computes up to SP sites per thread and layer.  Own-site global inputs (H, S, S1, the Ewald potential) are loaded
 // by the sweep's pre(k) hook into OwnIn registers, ahead of the next layer's
 // prefetch.
 
@@ -226,14 +229,14 @@ struct OwnIn {
 #define SNH_TILE_SITES(T_)                                                                          \
     const TileSite ts0(A, W, R, C.na, (int)sizeof(T_), (int)threadIdx.x);                           \
     const TileSite ts1(A, W, R, C.na, (int)sizeof(T_), (int)(threadIdx.x + blockDim.x));            \
-    constexpr bool kTwo = tile_sites_per_thread<T_>() > 1;                                          \
+    constexpr bool kTwo = SP > 1;                                                                   \
     const uint32_t lay = (uint32_t)R.L1 * (uint32_t)R.L2;                                           \
     const uint32_t slot = 3u * (uint32_t)sizeof(T_) * (uint32_t)A.cap;                              \
     OwnIn<T_> in0{}, in1{}
 
 // Heun predictor (see heun_predict): S' and H from S.
-template <typename T, int CH, bool kIso>
-__global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_predict(Couplings<T> C, TileArgs A,
+template <typename T, int CH, bool kIso, int SP>
+__global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_predict(Couplings<T> C, TileArgs A,
                                                                         const T* __restrict__ S, T* __restrict__ Sp,
                                                                         T* __restrict__ H, const T* __restrict__ phi,
                                                                         NoiseArgs<T> nz, T dt, T lambda)
@@ -243,10 +246,10 @@ __global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_pre
     const ReplicaInfo R = C.reps[W.r];
     SNH_TILE_SITES(T);
     const bool noisy = lambda != (T)0;
-    auto load = [&](const TileSite& ts, int k, OwnIn<T>& in) {
+    auto load = [&](const TileSite& ts, int k, OwnIn<T>& in) __attribute__((always_inline)) {
         if (phi) in.p = plane_v3(phi, C.total, ts.gb + lay * (uint32_t)k);
     };
-    auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) {
+    auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) __attribute__((always_inline)) {
         if (!ts.ok) return;
         const uint32_t own = ts.own + slot * (uint32_t)ph;
         const uint32_t g = ts.gb + lay * (uint32_t)k;
@@ -260,7 +263,7 @@ __global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_pre
         plane_store(Sp, C.total, g, sp);
         plane_store(H, C.total, g, h);
     };
-    tile_sweep<T, 1>(
+    tile_sweep<T, 1, SP>(
         C, A, W, R, S, nullptr, smem,
         [&](int k) {
             load(ts0, k, in0);
@@ -274,8 +277,8 @@ __global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_pre
 }
 
 // Heun corrector: S <- kappa*normalize(H + rhs_det(S', dE(S'))/2).
-template <typename T, int CH, bool kIso>
-__global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_correct(Couplings<T> C, TileArgs A,
+template <typename T, int CH, bool kIso, int SP>
+__global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_correct(Couplings<T> C, TileArgs A,
                                                                         const T* __restrict__ Sp,
                                                                         const T* __restrict__ H, T* __restrict__ Sout,
                                                                         const T* __restrict__ phi, T dt, T lambda)
@@ -284,12 +287,12 @@ __global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_cor
     const TileWork W = A.work[logical_block()];
     const ReplicaInfo R = C.reps[W.r];
     SNH_TILE_SITES(T);
-    auto load = [&](const TileSite& ts, int k, OwnIn<T>& in) {
+    auto load = [&](const TileSite& ts, int k, OwnIn<T>& in) __attribute__((always_inline)) {
         const uint32_t g = ts.gb + lay * (uint32_t)k;
         in.a = plane_v3(H, C.total, g);
         if (phi) in.p = plane_v3(phi, C.total, g);
     };
-    auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) {
+    auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) __attribute__((always_inline)) {
         if (!ts.ok) return;
         const uint32_t own = ts.own + slot * (uint32_t)ph;
         const uint32_t g = ts.gb + lay * (uint32_t)k;
@@ -302,7 +305,7 @@ __global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_cor
         plane_store(Sout, C.total, g,
                     normalize_dipole(V3<T>{h.x + d.x / 2, h.y + d.y / 2, h.z + d.z / 2}, ldc(C.kappa, i)));
     };
-    tile_sweep<T, 1>(
+    tile_sweep<T, 1, SP>(
         C, A, W, R, Sp, nullptr, smem,
         [&](int k) {
             load(ts0, k, in0);
@@ -318,8 +321,8 @@ __global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_cor
 // One fixed-point sweep of the spherical midpoint method on the ring of
 // Shat = kappa*normalize((S + S1)/2) (or a materialised Shat when a global
 // term needs it), with the FP64 residual sum |S1 - S2|^2 per workgroup.
-template <typename T, int CH, bool kHatGiven>
-__global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_mid_iter(Couplings<T> C, TileArgs A,
+template <typename T, int CH, bool kHatGiven, int SP>
+__global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_mid_iter(Couplings<T> C, TileArgs A,
                                                                          const T* __restrict__ S,
                                                                          const T* __restrict__ S1,
                                                                          T* __restrict__ S2,
@@ -335,13 +338,13 @@ __global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_mid
     const ReplicaInfo R = C.reps[W.r];
     SNH_TILE_SITES(T);
     double d2 = 0;
-    auto load = [&](const TileSite& ts, int k, OwnIn<T>& in) {
+    auto load = [&](const TileSite& ts, int k, OwnIn<T>& in) __attribute__((always_inline)) {
         const uint32_t g = ts.gb + lay * (uint32_t)k;
         in.a = plane_v3(S, C.total, g);
         in.b = plane_v3(S1, C.total, g);
         if (phi) in.p = plane_v3(phi, C.total, g);
     };
-    auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) {
+    auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) __attribute__((always_inline)) {
         if (!ts.ok) return;
         const uint32_t own = ts.own + slot * (uint32_t)ph;
         const uint32_t g = ts.gb + lay * (uint32_t)k;
@@ -356,25 +359,25 @@ __global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_mid
                      ez = (double)in.b.z - (double)o.z;
         d2 += ex * ex + ey * ey + ez * ez;
     };
-    auto pre = [&](int k) {
+    auto pre = [&](int k) __attribute__((always_inline)) {
         load(ts0, k, in0);
         if (kTwo) load(ts1, k, in1);
     };
-    auto body = [&](int k, int ph) {
+    auto body = [&](int k, int ph) __attribute__((always_inline)) {
         const int* offs = A.ldsoff + ph * C.nb;
         site(ts0, k, ph, offs, in0);
         if (kTwo) site(ts1, k, ph, offs, in1);
     };
     if (kHatGiven)
-        tile_sweep<T, 1>(C, A, W, R, Shat, nullptr, smem, pre, body);
+        tile_sweep<T, 1, SP>(C, A, W, R, Shat, nullptr, smem, pre, body);
     else
-        tile_sweep<T, 2>(C, A, W, R, S, S1, smem, pre, body);
+        tile_sweep<T, 2, SP>(C, A, W, R, S, S1, smem, pre, body);
     const double t = tile_block_sum(d2);
     if (threadIdx.x == 0) partial[wg] = t;
 }
 
-template <typename T, int CH>
-__global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_grad(Couplings<T> C, TileArgs A,
+template <typename T, int CH, int SP>
+__global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_grad(Couplings<T> C, TileArgs A,
                                                                      const T* __restrict__ S,
                                                                      const T* __restrict__ phi, T* __restrict__ dE)
 {
@@ -382,10 +385,10 @@ __global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_gra
     const TileWork W = A.work[logical_block()];
     const ReplicaInfo R = C.reps[W.r];
     SNH_TILE_SITES(T);
-    auto load = [&](const TileSite& ts, int k, OwnIn<T>& in) {
+    auto load = [&](const TileSite& ts, int k, OwnIn<T>& in) __attribute__((always_inline)) {
         if (phi) in.p = plane_v3(phi, C.total, ts.gb + lay * (uint32_t)k);
     };
-    auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) {
+    auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) __attribute__((always_inline)) {
         if (!ts.ok) return;
         const uint32_t own = ts.own + slot * (uint32_t)ph;
         const uint32_t g = ts.gb + lay * (uint32_t)k;
@@ -394,7 +397,7 @@ __global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_gra
         if (phi) add_ewald_field(C, i, in.p, f);
         plane_store(dE, C.total, g, f);
     };
-    tile_sweep<T, 1>(
+    tile_sweep<T, 1, SP>(
         C, A, W, R, S, nullptr, smem,
         [&](int k) {
             load(ts0, k, in0);
@@ -408,8 +411,8 @@ __global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_gra
 }
 
 // energy_aux per workgroup: Zeeman, anisotropy, non-culled bonds, Ewald 1/2 mu.phi.
-template <typename T>
-__global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_energy(Couplings<T> C, TileArgs A,
+template <typename T, int SP>
+__global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_energy(Couplings<T> C, TileArgs A,
                                                                        const T* __restrict__ S,
                                                                        const T* __restrict__ phi,
                                                                        double* __restrict__ partial)
@@ -420,10 +423,10 @@ __global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_ene
     const ReplicaInfo R = C.reps[W.r];
     SNH_TILE_SITES(T);
     double e = 0;
-    auto load = [&](const TileSite& ts, int k, OwnIn<T>& in) {
+    auto load = [&](const TileSite& ts, int k, OwnIn<T>& in) __attribute__((always_inline)) {
         if (phi) in.p = plane_v3(phi, C.total, ts.gb + lay * (uint32_t)k);
     };
-    auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) {
+    auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) __attribute__((always_inline)) {
         if (!ts.ok) return;
         const uint32_t own = ts.own + slot * (uint32_t)ph;
         const int i = ts.i;
@@ -463,7 +466,7 @@ __global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_ene
             e += 0.5 * (mx * p.x + my * p.y + mz * p.z);
         }
     };
-    tile_sweep<T, 1>(
+    tile_sweep<T, 1, SP>(
         C, A, W, R, S, nullptr, smem,
         [&](int k) {
             load(ts0, k, in0);
@@ -483,20 +486,34 @@ __global__ SNH_TILE_OCC __launch_bounds__(tile_max_threads<T>()) void k_tile_ene
 
 size_t tile_lds_bytes(const TileArgs& A, size_t elem) { return (size_t)A.nslot * A.cap * 3 * elem; }
 
-template <typename K>
-static void launch_tile(K kernel, const TileArgs& A, int nthreads, size_t lds, hipStream_t st)
+// Raises a kernel's dynamic-LDS limit to what a plan needs (once per kernel
+// and size; HIP keeps the attribute per function).
+static void allow_dynamic_lds(const void* fn, size_t bytes)
 {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
-    (void)st;
+    static std::mutex mu;
+    static std::map<const void*, size_t> done;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t& have = done[fn];
+    if (have >= bytes) return;
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    have = bytes;
+}
+
+template <typename K>
+static void launch_tile(K kernel, const TileArgs&, int, size_t lds, hipStream_t)
+{
+    allow_dynamic_lds(reinterpret_cast<const void*>(kernel), lds);
 }
 
 // LDS reads have short latency: batches of 9 (or 6) neighbours, a divisor
-// of the bond-list padding.
-#define SNH_TILE_CH(MACRO, ISO)                                  \
-    do {                                                         \
-        if (C.chunk % 9 == 0) MACRO(9, ISO);                     \
-        else MACRO(6, ISO);                                      \
+// of the bond-list padding.  SP (sites per thread) comes from the plan.
+#define SNH_TILE_DISPATCH(MACRO, ISO)                                  \
+    do {                                                               \
+        if (C.chunk % 9 == 0) {                                        \
+            if (A.sp == 2) MACRO(9, ISO, 2); else MACRO(9, ISO, 1);    \
+        } else {                                                       \
+            if (A.sp == 2) MACRO(6, ISO, 2); else MACRO(6, ISO, 1);    \
+        }                                                              \
     } while (0)
 
 template <typename T>
@@ -504,13 +521,13 @@ void launch_tile_predict(const Couplings<T>& C, const TileArgs& A, int nthreads,
                          const T* phi, const NoiseArgs<T>& nz, T dt, T lambda, hipStream_t st)
 {
     const size_t lds = tile_lds_bytes(A, sizeof(T));
-#define SNH_TP(CH, ISO)                                                                                        \
-    do {                                                                                                       \
-        launch_tile(k_tile_predict<T, CH, ISO>, A, nthreads, lds, st);                                         \
-        hipLaunchKernelGGL((k_tile_predict<T, CH, ISO>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, S, Sp, H, \
-                           phi, nz, dt, lambda);                                                               \
+#define SNH_TP(CH, ISO, SP)                                                                                      \
+    do {                                                                                                         \
+        launch_tile(k_tile_predict<T, CH, ISO, SP>, A, nthreads, lds, st);                                       \
+        hipLaunchKernelGGL((k_tile_predict<T, CH, ISO, SP>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, S, Sp, \
+                           H, phi, nz, dt, lambda);                                                              \
     } while (0)
-    if (C.iso) SNH_TILE_CH(SNH_TP, true); else SNH_TILE_CH(SNH_TP, false);
+    if (C.iso) SNH_TILE_DISPATCH(SNH_TP, true); else SNH_TILE_DISPATCH(SNH_TP, false);
 #undef SNH_TP
 }
 
@@ -519,13 +536,13 @@ void launch_tile_correct(const Couplings<T>& C, const TileArgs& A, int nthreads,
                          const T* phi, T dt, T lambda, hipStream_t st)
 {
     const size_t lds = tile_lds_bytes(A, sizeof(T));
-#define SNH_TC(CH, ISO)                                                                                          \
+#define SNH_TC(CH, ISO, SP)                                                                                      \
     do {                                                                                                         \
-        launch_tile(k_tile_correct<T, CH, ISO>, A, nthreads, lds, st);                                           \
-        hipLaunchKernelGGL((k_tile_correct<T, CH, ISO>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, Sp, H, Sout, \
-                           phi, dt, lambda);                                                                     \
+        launch_tile(k_tile_correct<T, CH, ISO, SP>, A, nthreads, lds, st);                                       \
+        hipLaunchKernelGGL((k_tile_correct<T, CH, ISO, SP>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, Sp, H, \
+                           Sout, phi, dt, lambda);                                                               \
     } while (0)
-    if (C.iso) SNH_TILE_CH(SNH_TC, true); else SNH_TILE_CH(SNH_TC, false);
+    if (C.iso) SNH_TILE_DISPATCH(SNH_TC, true); else SNH_TILE_DISPATCH(SNH_TC, false);
 #undef SNH_TC
 }
 
@@ -535,15 +552,18 @@ void launch_tile_midpoint_iter(const Couplings<T>& C, const TileArgs& A, int nth
                                const int* conv_iter, hipStream_t st)
 {
     const size_t lds = tile_lds_bytes(A, sizeof(T));
+#define SNH_TM(HAT, SP)                                                                                        \
+    do {                                                                                                       \
+        launch_tile(k_tile_mid_iter<T, 6, HAT, SP>, A, nthreads, lds, st);                                     \
+        hipLaunchKernelGGL((k_tile_mid_iter<T, 6, HAT, SP>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, S, S1, \
+                           S2, Shat, phi, dt, lambda, partial, conv_iter);                                     \
+    } while (0)
     if (Shat) {
-        launch_tile(k_tile_mid_iter<T, 6, true>, A, nthreads, lds, st);
-        hipLaunchKernelGGL((k_tile_mid_iter<T, 6, true>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, S, S1, S2,
-                           Shat, phi, dt, lambda, partial, conv_iter);
+        if (A.sp == 2) SNH_TM(true, 2); else SNH_TM(true, 1);
     } else {
-        launch_tile(k_tile_mid_iter<T, 6, false>, A, nthreads, lds, st);
-        hipLaunchKernelGGL((k_tile_mid_iter<T, 6, false>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, S, S1, S2,
-                           Shat, phi, dt, lambda, partial, conv_iter);
+        if (A.sp == 2) SNH_TM(false, 2); else SNH_TM(false, 1);
     }
+#undef SNH_TM
 }
 
 template <typename T>
@@ -551,8 +571,13 @@ void launch_tile_grad(const Couplings<T>& C, const TileArgs& A, int nthreads, co
                       hipStream_t st)
 {
     const size_t lds = tile_lds_bytes(A, sizeof(T));
-    launch_tile(k_tile_grad<T, 6>, A, nthreads, lds, st);
-    hipLaunchKernelGGL((k_tile_grad<T, 6>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, S, phi, dE);
+    if (A.sp == 2) {
+        launch_tile(k_tile_grad<T, 6, 2>, A, nthreads, lds, st);
+        hipLaunchKernelGGL((k_tile_grad<T, 6, 2>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, S, phi, dE);
+    } else {
+        launch_tile(k_tile_grad<T, 6, 1>, A, nthreads, lds, st);
+        hipLaunchKernelGGL((k_tile_grad<T, 6, 1>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, S, phi, dE);
+    }
 }
 
 template <typename T>
@@ -560,8 +585,13 @@ void launch_tile_energy(const Couplings<T>& C, const TileArgs& A, int nthreads, 
                         double* partial, hipStream_t st)
 {
     const size_t lds = tile_lds_bytes(A, sizeof(T));
-    launch_tile(k_tile_energy<T>, A, nthreads, lds, st);
-    hipLaunchKernelGGL(k_tile_energy<T>, dim3(A.nwork), dim3(nthreads), lds, st, C, A, S, phi, partial);
+    if (A.sp == 2) {
+        launch_tile(k_tile_energy<T, 2>, A, nthreads, lds, st);
+        hipLaunchKernelGGL((k_tile_energy<T, 2>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, S, phi, partial);
+    } else {
+        launch_tile(k_tile_energy<T, 1>, A, nthreads, lds, st);
+        hipLaunchKernelGGL((k_tile_energy<T, 1>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, S, phi, partial);
+    }
 }
 
 #define SNH_TILE_INST(T)                                                                                         \
