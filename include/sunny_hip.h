@@ -128,6 +128,33 @@ int snh_energy_grad(snh_system *sys, double *out);
 /* Blocks until all work queued on the handle's stream has finished. */
 int snh_synchronize(snh_system *sys);
 
+/* Gradients through a deterministic implicit-midpoint trajectory.
+ *
+ * snh_midpoint_record runs nsteps of step!(sys, ImplicitMidpoint(dt; tol))
+ * with damping 0 (src/Integrators.jl:379-413) from the current state and
+ * keeps the state on the device every checkpoint_every steps; the handle
+ * ends at the final state, from which the caller evaluates its loss.
+ *
+ * snh_midpoint_adjoint then sweeps the recorded trajectory backwards
+ * (recomputing each segment from its checkpoint) and returns, for the loss
+ * gradient dLdS_final at the final state ([sites][3], host layout):
+ *   grad_out[p]   dL/dtheta_p for nparams coupling parameters, each given
+ *                 as the spec of its unit couplings (same na, kappa and bond
+ *                 list as the system's spec; the system's couplings are
+ *                 linear in the parameters, as ModelParam,
+ *                 src/System/Types.jl:76-86, src/System/Interactions.jl:1-42);
+ *   dLdS_initial  dL/dS at the recorded initial state ([sites][3]; may be NULL).
+ * The handle's state is left at the final state.
+ *
+ * Supported: one replica in :dipole mode at precision 64, bilinear pair
+ * couplings, Zeeman, onsite anisotropy of order <= 2, no Ewald term.
+ * Sunny has no reverse rule for step!; these entry points are what a
+ * ChainRulesCore rrule would call (the pattern of src/Fitting.jl:123-152). */
+int snh_midpoint_record(snh_system *sys, double dt, double tol, int32_t max_iters, int32_t nsteps,
+                        int32_t checkpoint_every);
+int snh_midpoint_adjoint(snh_system *sys, int32_t nparams, const snh_spec *params, const double *dLdS_final,
+                         double *grad_out, double *dLdS_initial);
+
 /* Diagnostics with no counterpart in the reference: per-kernel timing by HIP
  * events recorded on the handle's stream around every launch of a kernel
  * family ("dipole_predict", "dipole_correct", "sun_predict", "sun_correct",

@@ -301,6 +301,9 @@ struct snh_system {
     virtual void kernel_time(const char* fam, double* ms, int64_t* n) = 0;
     virtual void copy_calibration(int repeats, int64_t* bytes) = 0;
     virtual int64_t query(const std::string& key) const = 0;
+    virtual void midpoint_record(double dt, double tol, int max_iters, int nsteps, int every) = 0;
+    virtual void midpoint_adjoint(int nparams, const snh_spec* params, const double* dLdS, double* grad,
+                                  double* dLdS0) = 0;
 };
 
 namespace {
@@ -318,6 +321,17 @@ struct SystemImpl final : snh_system {
     bool tiled = false;
     TilePlan tp;
     int* d_trblk = nullptr;
+    // bond tables as given (for the adjoint's per-parameter unit couplings)
+    std::vector<int> bsrc_h;      // padded slot -> original bond, -1 for padding
+    std::vector<int32_t> bond_h;  // [nb_in][6]
+    int nb_in = 0;
+    std::vector<int> kmax_h;
+    bool any_biquad = false;
+    // recorded midpoint trajectory (checkpoints on the device)
+    std::vector<T*> ckpt;
+    std::vector<int> rec_iters;
+    int rec_every = 0, rec_n = 0, rec_maxit = 0;
+    double rec_dt = 0, rec_tol = 0;
     // device tables
     std::vector<void*> owned;
     Couplings<T> C{};
@@ -360,6 +374,7 @@ struct SystemImpl final : snh_system {
     {
         if (st) (void)hipStreamSynchronize(st);
         timer.clear();
+        for (T* p : ckpt) (void)hipFree(p);
         for (void* p : owned) (void)hipFree(p);
         if (pinned) (void)hipHostFree(pinned);
         if (pinned_i) (void)hipHostFree(pinned_i);
@@ -508,6 +523,10 @@ struct SystemImpl final : snh_system {
                     bflag[slot] = kBondIso | kBondCulled;
                 }
         }
+        bsrc_h = bsrc;
+        nb_in = sp.nbonds;
+        bond_h.assign(sp.bond, sp.bond + 6 * (size_t)sp.nbonds);
+        for (int b = 0; b < sp.nbonds; b++) any_biquad |= sp.has_biquad && sp.has_biquad[b];
         nb = nbp;
         C.iso = 1;
         for (int k = 0; k < nbp; k++)
@@ -553,6 +572,7 @@ struct SystemImpl final : snh_system {
                 for (int p = 0; p < 9; p++) q2[9 * i + p] = (T)Q[p];
             }
         }
+        kmax_h = kmax;
         std::vector<T> lam, qops;
         if (N > 0) {
             SNH_REQUIRE(sp.Lambda, SNH_ERR_INVALID, ":SUN mode needs the onsite matrices Lambda");
@@ -947,6 +967,204 @@ struct SystemImpl final : snh_system {
         }
     }
 
+    // ---- adjoint of the deterministic midpoint trajectory
+
+    void require_adjoint_support() const
+    {
+        SNH_REQUIRE(N == 0, SNH_ERR_UNSUPPORTED, "the midpoint adjoint supports :dipole mode");
+        SNH_REQUIRE(sizeof(T) == 8, SNH_ERR_UNSUPPORTED, "the midpoint adjoint runs at precision 64");
+        SNH_REQUIRE(R == 1, SNH_ERR_UNSUPPORTED, "the midpoint adjoint takes one replica");
+        SNH_REQUIRE(!tiled, SNH_ERR_UNSUPPORTED, "the midpoint adjoint needs the ghost-padded layout");
+        SNH_REQUIRE(!ewald, SNH_ERR_UNSUPPORTED, "the midpoint adjoint does not support the Ewald term");
+        SNH_REQUIRE(!any_biquad, SNH_ERR_UNSUPPORTED, "the midpoint adjoint supports bilinear pair couplings");
+        for (int k : kmax_h)
+            SNH_REQUIRE(k <= 2, SNH_ERR_UNSUPPORTED, "the midpoint adjoint supports onsite anisotropy of order <= 2");
+    }
+
+    // One midpoint step (damping 0); returns the buffer holding S'' of the
+    // converged iteration and its iteration count.
+    T* midpoint_one(double dt, double tol, int max_iters, int* it_out)
+    {
+        step_midpoint(dt, 0.0, tol, max_iters, 1, it_out);
+        return (*it_out % 2 == 1) ? B1 : B2;  // S'' of iteration m lives in B1 for odd m, B2 for even m
+    }
+
+    void midpoint_record(double dt, double tol, int max_iters, int nsteps, int every) override
+    {
+        require_adjoint_support();
+        SNH_REQUIRE(nsteps >= 1 && every >= 1, SNH_ERR_INVALID, "nsteps and checkpoint_every must be positive");
+        const int ncp = (nsteps + every - 1) / every;
+        const size_t bytes = sizeof(T) * planes * (size_t)total;
+        for (T* p : ckpt) (void)hipFree(p);
+        ckpt.clear();
+        rec_n = 0;
+        for (int c = 0; c < ncp; c++) ckpt.push_back(dalloc<T>((size_t)planes * total));
+        rec_iters.assign(nsteps, 0);
+        rec_every = every;
+        rec_dt = dt;
+        rec_tol = tol;
+        rec_maxit = max_iters;
+        for (int n = 0; n < nsteps; n++) {
+            if (n % every == 0) SNH_HIP(hipMemcpyAsync(ckpt[n / every], X, bytes, hipMemcpyDeviceToDevice, st));
+            int it = 0;
+            midpoint_one(dt, tol, max_iters, &it);
+            rec_iters[n] = it;
+        }
+        SNH_HIP(hipStreamSynchronize(st));
+        rec_n = nsteps;
+    }
+
+    // Unit-coupling tables of one parameter, laid out like C (same bond slots).
+    Couplings<T> param_tables(const snh_spec& ps, const T* zeros3)
+    {
+        SNH_REQUIRE(ps.na == na && ps.N == 0, SNH_ERR_INVALID, "parameter spec: na and mode must match the system");
+        SNH_REQUIRE(ps.nbonds == nb_in, SNH_ERR_INVALID, "parameter spec: bond list must match the system's");
+        for (int k = 0; k < 6 * nb_in; k++)
+            if (k % 6 != 5)
+                SNH_REQUIRE(ps.bond[k] == bond_h[k], SNH_ERR_INVALID,
+                            "parameter spec: bond list must match the system's");
+        const size_t nbp = bsrc_h.size();
+        std::vector<int> flag(nbp, kBondIso | kBondCulled);
+        std::vector<T> J(9 * nbp, (T)0), J0(nbp, (T)0), q2(9 * na, (T)0);
+        std::vector<int> km(na, 0);
+        for (size_t k = 0; k < nbp; k++) {
+            const int b = bsrc_h[k];
+            if (b < 0) continue;
+            SNH_REQUIRE(!(ps.has_biquad && ps.has_biquad[b]), SNH_ERR_UNSUPPORTED,
+                        "parameter spec: biquadratic couplings are not supported by the adjoint");
+            const double* Jb = ps.bilin + 9 * b;
+            const bool iso = Jb[1] == 0 && Jb[2] == 0 && Jb[3] == 0 && Jb[5] == 0 && Jb[6] == 0 && Jb[7] == 0 &&
+                             Jb[0] == Jb[4] && Jb[0] == Jb[8];
+            flag[k] = iso ? kBondIso : 0;
+            for (int p = 0; p < 9; p++) J[9 * k + p] = (T)Jb[p];
+            J0[k] = (T)Jb[0];
+        }
+        for (int i = 0; i < na; i++) {
+            const int k = ps.kmax ? ps.kmax[i] : 0;
+            SNH_REQUIRE(k == 0 || k == 2, SNH_ERR_UNSUPPORTED,
+                        "parameter spec: onsite anisotropy of order <= 2 only");
+            km[i] = k;
+            if (k == 2) {
+                const double* c2 = ps.stevens + 28 * i + 1;
+                const double Q[9] = {2 * c2[0], 2 * c2[4], c2[1], 2 * c2[4], -2 * c2[0], c2[3], c2[1], c2[3], 6 * c2[2]};
+                for (int p = 0; p < 9; p++) q2[9 * i + p] = (T)Q[p];
+            }
+        }
+        Couplings<T> Cp = C;
+        Cp.bflag = own(dupload(flag));
+        Cp.bJ = own(dupload(J));
+        Cp.bJ0 = own(dupload(J0));
+        Cp.stev_q2 = own(dupload(q2));
+        Cp.kmax = own(dupload(km));
+        Cp.zee = zeros3;
+        Cp.iso = 0;
+        return Cp;
+    }
+
+    void midpoint_adjoint(int nparams, const snh_spec* params, const double* dLdS, double* grad,
+                          double* dLdS0) override
+    {
+        if constexpr (sizeof(T) != 8) {
+            throw Failure{SNH_ERR_UNSUPPORTED, "the midpoint adjoint runs at precision 64"};
+        } else {
+            require_adjoint_support();
+            SNH_REQUIRE(rec_n > 0, SNH_ERR_INVALID, "no recorded trajectory: call snh_midpoint_record first");
+            SNH_REQUIRE(nparams >= 0 && (nparams == 0 || (params && grad)), SNH_ERR_INVALID, "invalid parameters");
+            SNH_REQUIRE(dLdS, SNH_ERR_INVALID, "null loss gradient");
+            const size_t np = (size_t)planes * total, bytes = sizeof(T) * np;
+            // scratch for this call (freed below)
+            std::vector<void*> tmp;
+            auto alloc = [&](size_t n) {
+                T* p = dalloc<T>(n);
+                tmp.push_back(p);
+                return p;
+            };
+            struct Guard {
+                std::vector<void*>& v;
+                ~Guard()
+                {
+                    for (void* p : v) (void)hipFree(p);
+                }
+            } guard{tmp};
+            const size_t owned0 = owned.size();
+            std::vector<T> z3(3 * (size_t)na, (T)0);
+            T* zeros3 = dalloc<T>(z3.size());
+            tmp.push_back(zeros3);
+            SNH_HIP(hipMemcpy(zeros3, z3.data(), sizeof(T) * z3.size(), hipMemcpyHostToDevice));
+            std::vector<Couplings<T>> Cps;
+            for (int p = 0; p < nparams; p++) Cps.push_back(param_tables(params[p], zeros3));
+            // param_tables' uploads belong to this call too
+            for (size_t k = owned0; k < owned.size(); k++) tmp.push_back(owned[k]);
+            owned.resize(owned0);
+            Couplings<T> Clin = C;
+            Clin.zee = zeros3;
+            Clin.iso = 0;
+
+            T *Xsave = alloc(np), *lam = alloc(np), *mu = alloc(np), *nu = alloc(np), *nu2 = alloc(np),
+              *Shat = alloc(np), *w = alloc(np), *g = alloc(np);
+            double* dgrad = (double*)dalloc<double>(std::max(1, nparams));
+            tmp.push_back(dgrad);
+            const int K = rec_every;
+            std::vector<T*> segS, segD;
+            for (int k = 0; k < K; k++) {
+                segS.push_back(alloc(np));
+                segD.push_back(alloc(np));
+            }
+            SNH_HIP(hipMemcpyAsync(Xsave, X, bytes, hipMemcpyDeviceToDevice, st));
+            SNH_HIP(hipMemsetAsync(dgrad, 0, sizeof(double) * std::max(1, nparams), st));
+            SNH_HIP(hipMemsetAsync(lam, 0, bytes, st));
+            {
+                std::vector<T> h(np, (T)0);
+                for_each_padded([&](uint32_t gi, long long si, bool interior) {
+                    if (!interior) return;
+                    for (int a = 0; a < 3; a++) h[(size_t)a * total + gi] = (T)dLdS[3 * (size_t)si + a];
+                });
+                SNH_HIP(hipMemcpyAsync(lam, h.data(), bytes, hipMemcpyHostToDevice, st));
+                SNH_HIP(hipStreamSynchronize(st));
+            }
+            const T tdt = (T)rec_dt;
+            const int ncp = (int)ckpt.size();
+            for (int seg = ncp - 1; seg >= 0; seg--) {
+                const int n0 = seg * K, n1 = std::min(n0 + K, rec_n);
+                SNH_HIP(hipMemcpyAsync(X, ckpt[seg], bytes, hipMemcpyDeviceToDevice, st));
+                for (int n = n0; n < n1; n++) {
+                    SNH_HIP(hipMemcpyAsync(segS[n - n0], X, bytes, hipMemcpyDeviceToDevice, st));
+                    int it = 0;
+                    T* Sdd = midpoint_one(rec_dt, rec_tol, rec_maxit, &it);
+                    SNH_HIP(hipMemcpyAsync(segD[n - n0], Sdd, bytes, hipMemcpyDeviceToDevice, st));
+                    rec_iters[n] = std::max(rec_iters[n], it);
+                }
+                for (int n = n1 - 1; n >= n0; n--) {
+                    const T* S = segS[n - n0];
+                    const T* Sdd = segD[n - n0];
+                    launch_adj_prep<T>(C, nblocks, S, Sdd, lam, Shat, mu, st);
+                    launch_dipole_grad<T>(C, nblocks, Shat, nullptr, g, st);
+                    SNH_HIP(hipMemcpyAsync(nu, mu, bytes, hipMemcpyDeviceToDevice, st));
+                    // the linear sweep contracts like the forward fixed point
+                    const int sweeps = rec_iters[n] + 2;
+                    for (int k = 0; k < sweeps; k++) {
+                        launch_adj_w<T>(C, nblocks, nu, Shat, tdt, w, st);
+                        launch_adj_apply<T>(Clin, nblocks, w, S, Sdd, g, nu, mu, tdt, nu2, partial, st);
+                        std::swap(nu, nu2);
+                    }
+                    launch_adj_w<T>(C, nblocks, nu, Shat, tdt, w, st);
+                    for (int p = 0; p < nparams; p++) {
+                        launch_adj_param<T>(Cps[p], nblocks, Shat, w, partial, st);
+                        launch_adj_accumulate(partial, nblocks, dgrad, p, st);
+                    }
+                    launch_adj_lambda<T>(C, nblocks, nu, mu, lam, st);
+                }
+                SNH_HIP(hipGetLastError());
+            }
+            SNH_HIP(hipMemcpyAsync(X, Xsave, bytes, hipMemcpyDeviceToDevice, st));
+            if (nparams > 0) {
+                SNH_HIP(hipMemcpyAsync(grad, dgrad, sizeof(double) * nparams, hipMemcpyDeviceToHost, st));
+            }
+            SNH_HIP(hipStreamSynchronize(st));
+            if (dLdS0) download_planes(lam, 3, dLdS0);
+        }
+    }
+
     void energy(double* E) override
     {
         const T* phi = nullptr;
@@ -1222,4 +1440,16 @@ extern "C" int snh_query(snh_system* sys, const char* key, int64_t* value)
         SNH_REQUIRE(key && value, SNH_ERR_INVALID, "null argument");
         *value = sys->query(key);
     });
+}
+
+extern "C" int snh_midpoint_record(snh_system* sys, double dt, double tol, int32_t max_iters, int32_t nsteps,
+                                   int32_t checkpoint_every)
+{
+    return guarded(sys, [&] { sys->midpoint_record(dt, tol, max_iters, nsteps, checkpoint_every); });
+}
+
+extern "C" int snh_midpoint_adjoint(snh_system* sys, int32_t nparams, const snh_spec* params,
+                                    const double* dLdS_final, double* grad_out, double* dLdS_initial)
+{
+    return guarded(sys, [&] { sys->midpoint_adjoint(nparams, params, dLdS_final, grad_out, dLdS_initial); });
 }

@@ -159,6 +159,22 @@ template <typename T>
 void launch_tile_energy(const Couplings<T>& C, const TileArgs& A, int nthreads, const T* S, const T* phi,
                         double* partial, hipStream_t st);
 
+// Adjoint of the deterministic :dipole midpoint step (snh_adjoint.hip, FP64).
+template <typename T>
+void launch_adj_prep(const Couplings<T>& C, int nblocks, const T* S, const T* Sdd, const T* lam, T* Shat, T* mu,
+                     hipStream_t st);
+template <typename T>
+void launch_adj_w(const Couplings<T>& C, int nblocks, const T* nu, const T* Shat, T dt, T* w, hipStream_t st);
+template <typename T>
+void launch_adj_apply(const Couplings<T>& Clin, int nblocks, const T* w, const T* S, const T* Sdd, const T* g,
+                      const T* nu, const T* mu, T dt, T* nu_out, double* partial, hipStream_t st);
+template <typename T>
+void launch_adj_param(const Couplings<T>& Cp, int nblocks, const T* Shat, const T* w, double* partial,
+                      hipStream_t st);
+template <typename T>
+void launch_adj_lambda(const Couplings<T>& C, int nblocks, const T* nu, const T* mu, T* lam, hipStream_t st);
+void launch_adj_accumulate(const double* partial, int n, double* out, int slot, hipStream_t st);
+
 // Ewald (snh_ewald.hip).
 size_t ewald_fft_lds_bytes(long long nk, size_t elem);
 void launch_ewald_A(const EwaldAParams& P, const double* pos, double* A, hipStream_t st);

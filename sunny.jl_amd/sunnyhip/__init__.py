@@ -9,10 +9,12 @@ from .integrators import (ConvergenceError, ImplicitMidpoint, Langevin, ReplicaB
                           energy_grad_dipoles, energy_per_site, step_)
 from .operators import (SCALAR_BIQUAD_METRIC, StevensExpansion, matrix_to_stevens_coefficients, rcs_factors,
                         spin_matrices, stevens_matrices)
-from .system import (VACUUM_PERMEABILITY, Moment, System, clone_system, enable_dipole_dipole_, expected_spin_host,
-                     ket_from_dipole, polarize_spins_, randomize_spins_, set_coherent_, set_dipole_, set_exchange_,
-                     set_field_, set_onsite_coupling_, set_pair_biquad_, set_spin_rescaling_,
-                     set_stevens_expansion_)
+from .system import (VACUUM_PERMEABILITY, ModelParam, Moment, System, clone_system, enable_dipole_dipole_,
+                     expected_spin_host, get_param, get_params, ket_from_dipole, param_spec, polarize_spins_,
+                     randomize_spins_, repopulate_couplings_from_params_, set_coherent_, set_dipole_, set_exchange_,
+                     set_field_, set_onsite_coupling_, set_pair_biquad_, set_param_, set_params_,
+                     set_spin_rescaling_, set_stevens_expansion_)
 from . import workloads  # noqa: F401
+from .adjoint import midpoint_gradient, structure_factor, structure_factor_loss
 
 __all__ = [n for n in dir() if not n.startswith("_")]

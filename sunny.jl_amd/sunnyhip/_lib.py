@@ -25,6 +25,7 @@ EXPORTS = [
     "snh_set_dipoles", "snh_get_dipoles", "snh_set_coherents", "snh_get_coherents",
     "snh_step_langevin", "snh_step_midpoint", "snh_energy", "snh_energy_grad", "snh_synchronize",
     "snh_enable_timing", "snh_kernel_time", "snh_copy_calibration", "snh_query",
+    "snh_midpoint_record", "snh_midpoint_adjoint",
 ]
 
 
@@ -79,6 +80,8 @@ def lib():
         L.snh_kernel_time.argtypes = [vp, C.c_char_p, dp, C.POINTER(C.c_int64)]
         L.snh_copy_calibration.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
         L.snh_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
+        L.snh_midpoint_record.argtypes = [vp, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32]
+        L.snh_midpoint_adjoint.argtypes = [vp, C.c_int32, C.POINTER(SnhSpec), dp, dp, dp]
         for f in EXPORTS:
             if f != "snh_last_error":
                 getattr(L, f).restype = C.c_int
@@ -111,27 +114,33 @@ def check(rc, handle=None):
     raise ValueError(msg)
 
 
+def pack_spec(spec: dict) -> SnhSpec:
+    """snh_spec pointing into the arrays of `spec` (which must outlive it)."""
+    s = SnhSpec()
+    s.na, s.N, s.precision = spec["na"], spec["N"], spec["precision"]
+    s.kappa, s.g, s.field = dptr(spec["kappa"]), dptr(spec["g"]), dptr(spec["field"])
+    s.stevens, s.kmax, s.Lambda = dptr(spec["stevens"]), iptr(spec["kmax"]), dptr(spec["Lambda"])
+    s.nbonds = spec["nbonds"]
+    s.bond, s.bilin, s.biquad = iptr(spec["bond"]), dptr(spec["bilin"]), dptr(spec["biquad"])
+    s.has_biquad, s.scalar = iptr(spec["has_biquad"]), dptr(spec["scalar"])
+    s.ewald = spec["ewald"]
+    s.mu0_muB2 = spec["mu0_muB2"]
+    s.demag[:] = list(spec["demag"])
+    s.latvecs[:] = list(spec["latvecs"])
+    s.positions = dptr(spec["positions"])
+    s.nreplicas = spec["nreplicas"]
+    s.dims = iptr(spec["dims"])
+    s.layout = spec.get("layout", 0)
+    return s
+
+
 class Handle:
     """Owns one snh_system*; keeps the spec arrays alive for its lifetime."""
 
     def __init__(self, spec: dict):
         L = lib()
         self._keep = spec
-        s = SnhSpec()
-        s.na, s.N, s.precision = spec["na"], spec["N"], spec["precision"]
-        s.kappa, s.g, s.field = dptr(spec["kappa"]), dptr(spec["g"]), dptr(spec["field"])
-        s.stevens, s.kmax, s.Lambda = dptr(spec["stevens"]), iptr(spec["kmax"]), dptr(spec["Lambda"])
-        s.nbonds = spec["nbonds"]
-        s.bond, s.bilin, s.biquad = iptr(spec["bond"]), dptr(spec["bilin"]), dptr(spec["biquad"])
-        s.has_biquad, s.scalar = iptr(spec["has_biquad"]), dptr(spec["scalar"])
-        s.ewald = spec["ewald"]
-        s.mu0_muB2 = spec["mu0_muB2"]
-        s.demag[:] = list(spec["demag"])
-        s.latvecs[:] = list(spec["latvecs"])
-        s.positions = dptr(spec["positions"])
-        s.nreplicas = spec["nreplicas"]
-        s.dims = iptr(spec["dims"])
-        s.layout = spec.get("layout", 0)
+        s = pack_spec(spec)
         h = C.c_void_p()
         rc = L.snh_create(C.byref(s), C.byref(h))
         if rc != SNH_OK:

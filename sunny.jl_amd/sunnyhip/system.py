@@ -69,7 +69,10 @@ class System:
         self.extfield = np.zeros((na, 3))
         self.onsite = [StevensExpansion.zero() if self.N == 0 else np.zeros((self.N, self.N), complex)
                        for _ in range(na)]
-        self.pairs = {}  # Bond -> [scalar, bilin 3x3, biquad 5x5 or None]
+        self.pairs = {}  # Bond -> [scalar, bilin 3x3, biquad 5x5 or None], accumulated from params
+        # label -> ModelParam: couplings are linear in the parameter values
+        # (src/System/Types.jl:76-86); unlabeled couplings get "Unnamed" labels
+        self.params = {}
         self.ewald = None
         self.precision = int(precision)
         self.layout = 0  # device gather layout: 0 automatic (LDS tiles when they fit), 1 ghost-padded
@@ -218,6 +221,7 @@ def clone_system(sys: System, seed=None) -> System:
     c.__dict__.update({k: v for k, v in sys.__dict__.items() if not k.startswith("_")})
     c.onsite = [o.copy() if isinstance(o, np.ndarray) else o for o in sys.onsite]
     c.pairs = {b: [sc, np.array(J), None if B is None else np.array(B)] for b, (sc, J, B) in sys.pairs.items()}
+    c.params = {k: p.copy() for k, p in sys.params.items()}
     c.extfield = sys.extfield.copy()
     c.kappas = sys.kappas.copy()
     sys._pull()
@@ -330,9 +334,113 @@ def _to_float_or_mat3(J):
     return J
 
 
-def set_exchange_(sys: System, J, bond: Bond, biquad=0.0, propagate=None):
+class ModelParam:
+    """A labeled coupling strength `val` and the unit couplings it scales
+    (src/System/Types.jl:76-86).  pairs: Bond -> [scalar, J 3x3, B 5x5 or
+    None]; onsites: atom -> StevensExpansion (:dipole) or N x N matrix (:SUN)."""
+
+    def __init__(self, label, val, pairs=None, onsites=None):
+        self.label = label
+        self.val = float(val)
+        self.pairs = pairs or {}
+        self.onsites = onsites or {}
+
+    def copy(self):
+        return ModelParam(self.label, self.val,
+                          {b: [sc, np.array(J), None if B is None else np.array(B)]
+                           for b, (sc, J, B) in self.pairs.items()},
+                          {i: (o.copy() if isinstance(o, np.ndarray) else o) for i, o in self.onsites.items()})
+
+
+def _param_label(sys, param, key):
+    """`param` is (label, val) like Julia's `:label => val`; None gives the
+    coupling an "Unnamed" label tied to `key` (a bond or atom), so that a
+    later unlabeled call on the same bond or atom replaces it
+    (src/System/ModelParams.jl:9-48)."""
+    if param is not None:
+        label, val = param
+        return str(label), float(val)
+    for lab, p in sys.params.items():
+        if lab.startswith("Unnamed") and getattr(p, "key", None) == key:
+            return lab, 1.0
+    n = sum(1 for lab in sys.params if lab.startswith("Unnamed"))
+    return f"Unnamed{n + 1}", 1.0
+
+
+def _replace_param(sys, label, val, pairs=None, onsites=None, key=None):
+    p = ModelParam(label, val, pairs, onsites)
+    p.key = key
+    sys.params.pop(label, None)
+    sys.params[label] = p
+    repopulate_couplings_from_params_(sys)
+
+
+def repopulate_couplings_from_params_(sys: System):
+    """Couplings = sum over parameters of value x unit couplings
+    (src/System/Interactions.jl:1-42)."""
+    sys.pairs = {}
+    sys.onsite = [StevensExpansion.zero() if sys.N == 0 else np.zeros((sys.N, sys.N), complex)
+                  for _ in range(sys.na)]
+    for p in sys.params.values():
+        for b, (sc, J, B) in p.pairs.items():
+            e = sys.pairs.setdefault(b, [0.0, np.zeros((3, 3)), None])
+            e[0] += p.val * sc
+            e[1] = e[1] + p.val * np.asarray(J)
+            if B is not None:
+                e[2] = p.val * np.asarray(B) if e[2] is None else e[2] + p.val * np.asarray(B)
+        for i, o in p.onsites.items():
+            sys.onsite[i] = sys.onsite[i] + (o.scaled(p.val) if sys.N == 0 else p.val * o)
+    sys._invalidate()
+
+
+def get_param(sys: System, label):
+    return sys.params[str(label)].val
+
+
+def get_params(sys: System, labels):
+    return [get_param(sys, l) for l in labels]
+
+
+def set_params_(sys: System, labels, vals):
+    """Sets each labeled parameter and rebuilds the couplings
+    (src/System/ModelParams.jl:130-150)."""
+    if len(labels) != len(vals):
+        raise ValueError("Mismatched lengths")
+    for l, v in zip(labels, vals):
+        if str(l) not in sys.params:
+            raise KeyError(f"No parameter {l!r}")
+        sys.params[str(l)].val = float(v)
+    repopulate_couplings_from_params_(sys)
+
+
+def set_param_(sys: System, label, val):
+    set_params_(sys, [label], [val])
+
+
+def param_spec(sys: System, label):
+    """Coupling spec of one parameter's unit couplings on the system's bond
+    list (the derivative of the couplings with respect to that parameter)."""
+    p = sys.params[str(label)]
+    unit = System.__new__(System)
+    unit.__dict__.update({k: v for k, v in sys.__dict__.items() if not k.startswith("_")})
+    unit.params = {p.label: ModelParam(p.label, 1.0, p.pairs, p.onsites)}
+    unit.extfield = np.zeros_like(sys.extfield)
+    unit.ewald = None
+    unit.pairs = {b: [0.0, np.zeros((3, 3)), None] for b in sys.pairs}
+    for b, (sc, J, B) in p.pairs.items():
+        unit.pairs[b] = [sc, np.asarray(J, float), None if B is None else np.asarray(B, float)]
+    unit.onsite = [StevensExpansion.zero() if sys.N == 0 else np.zeros((sys.N, sys.N), complex)
+                   for _ in range(sys.na)]
+    for i, o in p.onsites.items():
+        unit.onsite[i] = o
+    return unit.spec()
+
+
+def set_exchange_(sys: System, J, bond: Bond, biquad=0.0, propagate=None, param=None):
     """Exchange S_i.J S_j (+ biquad (S_i.S_j)^2) on `bond` and its equivalents
-    (src/System/PairExchange.jl:379-385, 322-340, 185-260).
+    (src/System/PairExchange.jl:379-385, 322-340, 185-260).  `param` =
+    (label, val) scales the coupling by a labeled parameter (the reference's
+    trailing `:label => val`).
 
     Scalar J propagates to every bond of the same distance shell (exact for
     Heisenberg couplings).  Matrix J applies to `bond` in every cell (pass
@@ -351,6 +459,7 @@ def set_exchange_(sys: System, J, bond: Bond, biquad=0.0, propagate=None):
         targets = sys.crystal.equivalent_bonds(bond) if propagate is None else list(propagate)
     else:
         targets = [bond] if propagate is None else list(propagate)
+    pairs = {}
     for b in targets:
         Jm = Jv * np.eye(3) if isinstance(Jv, float) else np.asarray(Jv)
         bq = None
@@ -359,30 +468,31 @@ def set_exchange_(sys: System, J, bond: Bond, biquad=0.0, propagate=None):
             if sys.mode == "dipole":
                 f *= rcs_factors(sys.spins[b.i])[2] * rcs_factors(sys.spins[b.j])[2]
             bq = f * np.diag(SCALAR_BIQUAD_METRIC)
-        sys.pairs[b] = [scalar, Jm, bq]
-        sys.pairs[b.reverse()] = [scalar, Jm.T.copy(), None if bq is None else bq.T.copy()]
-    sys._invalidate()
+        pairs[b] = [scalar, Jm, bq]
+        pairs[b.reverse()] = [scalar, Jm.T.copy(), None if bq is None else bq.T.copy()]
+    key = ("pair", min(bond, bond.reverse(), key=lambda x: (x.i, x.j, x.n)))
+    label, val = _param_label(sys, param, key)
+    _replace_param(sys, label, val, pairs=pairs, key=key)
 
 
-def set_pair_biquad_(sys: System, B, bond: Bond):
+def set_pair_biquad_(sys: System, B, bond: Bond, param=None):
     """Coupling Q_i . B Q_j of Stevens quadrupoles (5x5, q descending) on one
     bond in every cell; the reverse bond carries B' (PairExchange.jl:447-468).
     In :dipole mode B is renormalised by rcs_2(s_i) rcs_2(s_j)."""
     B = np.asarray(B, float).reshape(5, 5)
     if sys.mode == "dipole":
         B = B * rcs_factors(sys.spins[bond.i])[2] * rcs_factors(sys.spins[bond.j])[2]
-    for b, M in ((bond, B), (bond.reverse(), B.T)):
-        ent = sys.pairs.get(b, [0.0, np.zeros((3, 3)), None])
-        ent[2] = M.copy() if ent[2] is None else ent[2] + M
-        sys.pairs[b] = ent
-    sys._invalidate()
+    pairs = {bond: [0.0, np.zeros((3, 3)), B.copy()], bond.reverse(): [0.0, np.zeros((3, 3)), B.T.copy()]}
+    key = ("biquad", min(bond, bond.reverse(), key=lambda x: (x.i, x.j, x.n)))
+    label, val = _param_label(sys, param, key)
+    _replace_param(sys, label, val, pairs=pairs, key=key)
 
 
-def set_onsite_coupling_(sys: System, op, i: int):
+def set_onsite_coupling_(sys: System, op, i: int, param=None):
     """Onsite anisotropy on atom i (src/System/OnsiteCoupling.jl:1-23, 159-205).
     `op` is an N x N matrix in the spin-s irrep, or a function of the spin
-    matrices.  No symmetry propagation: call once per atom (with the operator
-    rotated as required)."""
+    matrices; `param` = (label, val) as in set_exchange_.  No symmetry
+    propagation: call once per atom (with the operator rotated as required)."""
     s = sys.spins[i]
     if callable(op):
         op = op(spin_matrices(s))
@@ -395,20 +505,23 @@ def set_onsite_coupling_(sys: System, op, i: int):
     if N == 2:
         raise ValueError("Onsite coupling is trivial for quantum spin 1/2.")
     if sys.mode == "SUN":
-        sys.onsite[i] = A.copy()
+        unit = A.copy()
     else:
         c = matrix_to_stevens_coefficients((A + A.conj().T) / 2)
         lam = rcs_factors(s)
-        sys.onsite[i] = StevensExpansion.from_coefficients({k: lam[k] * c[k] for k in range(7)})
-    sys._invalidate()
+        unit = StevensExpansion.from_coefficients({k: lam[k] * c[k] for k in range(7)})
+    key = ("onsite", int(i))
+    label, val = _param_label(sys, param, key)
+    _replace_param(sys, label, val, onsites={int(i): unit}, key=key)
 
 
-def set_stevens_expansion_(sys: System, stvexp: StevensExpansion, i: int):
+def set_stevens_expansion_(sys: System, stvexp: StevensExpansion, i: int, param=None):
     """Sets already-renormalised classical Stevens coefficients (:dipole)."""
     if sys.mode != "dipole":
         raise ValueError("Stevens expansions apply to :dipole mode")
-    sys.onsite[i] = stvexp
-    sys._invalidate()
+    key = ("onsite", int(i))
+    label, val = _param_label(sys, param, key)
+    _replace_param(sys, label, val, onsites={int(i): stvexp}, key=key)
 
 
 def enable_dipole_dipole_(sys: System, mu0_muB2=VACUUM_PERMEABILITY, demag=1 / 3):

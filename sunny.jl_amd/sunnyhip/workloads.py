@@ -22,14 +22,19 @@ def bond_at_distance(cryst, dist, i=0, tol=1e-6):
     raise ValueError(f"no bond of length {dist}")
 
 
-def cubic_model(L=16, J=1.0, D=-0.1, s=1.0, g=2.0, seed=0, precision=64, dims=None):
+def cubic_model(L=16, J=1.0, D=-0.1, s=1.0, g=2.0, seed=0, precision=64, dims=None, params=False):
     """Simple cubic, a = 1 A, one site per cell: nearest-neighbour Heisenberg
-    J and onsite D*Sz^2."""
+    J and onsite D*Sz^2.  With params=True the two strengths are the labeled
+    parameters "J" and "D" (set_exchange!(sys, 1.0, bond, :J => J))."""
     cryst = cubic_crystal(1.0)
     sys = System(cryst, [(0, Moment(s=s, g=g))], "dipole", dims=dims or (L, L, L), seed=seed, precision=precision)
-    set_exchange_(sys, J, Bond(0, 0, (1, 0, 0)))
-    if D != 0:
-        set_onsite_coupling_(sys, lambda S: D * S[2] @ S[2], 0)
+    if params:
+        set_exchange_(sys, 1.0, Bond(0, 0, (1, 0, 0)), param=("J", J))
+        set_onsite_coupling_(sys, lambda S: S[2] @ S[2], 0, param=("D", D))
+    else:
+        set_exchange_(sys, J, Bond(0, 0, (1, 0, 0)))
+        if D != 0:
+            set_onsite_coupling_(sys, lambda S: D * S[2] @ S[2], 0)
     randomize_spins_(sys)
     return sys
 

@@ -1,0 +1,97 @@
+"""Pins the checkers of the midpoint adjoint (CPU only): the forward-mode
+dual-number restatement against the C oracle and central differences, the
+structure-factor loss and its gradient, and the labeled-parameter API
+(couplings linear in the parameter values, src/System/Interactions.jl:1-42)."""
+import numpy as np
+import pytest
+
+import sunnyhip as sh
+from oracle import oracle as O
+from oracle.adjoint_ref import MidpointAD
+
+
+def _run_oracle(J, D, S0, nsteps, dt=0.05, L=4, seed=3):
+    s = sh.workloads.cubic_model(L=L, J=J, D=D, seed=seed, params=True)
+    o = O.Oracle(s.spec())
+    S = S0.copy()
+    for _ in range(nsteps):
+        S, it = o.step_midpoint(S=S, dt=dt, damping=0.0, tol=1e-13)
+        assert it > 0
+    return S
+
+
+def test_dual_restatement_matches_oracle_and_fd():
+    s = sh.workloads.cubic_model(L=4, J=0.9, D=-0.08, seed=3, params=True)
+    S0 = s.dipoles_flat()
+    ad = MidpointAD(s.spec(), [sh.param_spec(s, "J"), sh.param_spec(s, "D")])
+    Sd = ad.trajectory(S0, 0.05, 20)
+    np.testing.assert_allclose(Sd.v, _run_oracle(0.9, -0.08, S0, 20), atol=1e-13)
+    h = 1e-5
+    fdJ = (_run_oracle(0.9 + h, -0.08, S0, 20) - _run_oracle(0.9 - h, -0.08, S0, 20)) / (2 * h)
+    fdD = (_run_oracle(0.9, -0.08 + h, S0, 20) - _run_oracle(0.9, -0.08 - h, S0, 20)) / (2 * h)
+    assert np.abs(Sd.d[..., 0] - fdJ).max() <= 1e-7 * np.abs(fdJ).max()
+    assert np.abs(Sd.d[..., 1] - fdD).max() <= 1e-7 * np.abs(fdD).max()
+
+
+def test_dual_initial_state_direction():
+    s = sh.workloads.cubic_model(L=4, J=0.9, D=-0.08, seed=4, params=True)
+    S0 = s.dipoles_flat()
+    v = np.random.default_rng(1).standard_normal(S0.shape)
+    v -= (v * S0).sum(1, keepdims=True) * S0  # tangent to the spheres
+    ad = MidpointAD(s.spec(), [])
+    Sd = ad.trajectory(S0, 0.05, 10, dS0=v)
+    h = 1e-6
+    o = O.Oracle(s.spec())
+
+    def run(S):
+        for _ in range(10):
+            S, _ = o.step_midpoint(S=S, dt=0.05, damping=0.0, tol=1e-13)
+        return S
+    fd = (run(S0 + h * v) - run(S0 - h * v)) / (2 * h)
+    assert np.abs(Sd.d[..., -1] - fd).max() <= 1e-6 * np.abs(fd).max()
+
+
+def test_structure_factor_direct_sum():
+    dims = (8, 4, 4)
+    rng = np.random.default_rng(0)
+    S = rng.standard_normal((128, 3))
+    Sq, _ = sh.structure_factor(S, dims, nq=4)
+    c = np.arange(128)
+    r = np.stack([c % 8, (c // 8) % 4, c // 32], 1)
+    for m in [(0, 0, 0), (1, 2, 3), (3, 0, 1)]:
+        q = 2 * np.pi * np.array(m) / 4
+        F = (np.exp(1j * r @ q)[:, None] * S).sum(0)
+        ref = (np.abs(F) ** 2).sum() / 128
+        assert abs(Sq[m[2], m[1], m[0]] - ref) < 1e-10
+
+
+def test_structure_factor_loss_gradient():
+    dims = (8, 8, 8)
+    rng = np.random.default_rng(2)
+    S = rng.standard_normal((512, 3))
+    T = rng.random((4, 4, 4))
+    L, g = sh.structure_factor_loss(S, dims, T, nq=4)
+    for _ in range(4):
+        v = rng.standard_normal(S.shape)
+        h = 1e-6
+        fd = (sh.structure_factor_loss(S + h * v, dims, T, nq=4)[0] -
+              sh.structure_factor_loss(S - h * v, dims, T, nq=4)[0]) / (2 * h)
+        assert abs((g * v).sum() - fd) <= 1e-7 * max(1.0, abs(fd))
+
+
+def test_labeled_parameters_are_linear():
+    s = sh.workloads.cubic_model(L=2, J=0.9, D=-0.08, seed=1, params=True)
+    assert sh.get_params(s, ["J", "D"]) == [0.9, -0.08]
+    b = sh.Bond(0, 0, (1, 0, 0))
+    np.testing.assert_allclose(s.pairs[b][1], 0.9 * np.eye(3))
+    c_D = s.onsite[0].packed().copy()
+    sh.set_params_(s, ["J", "D"], [1.0, -0.1])
+    np.testing.assert_allclose(s.pairs[b][1], np.eye(3))
+    np.testing.assert_allclose(s.onsite[0].packed(), c_D * (-0.1 / -0.08), atol=1e-15)
+    # the unit spec is the derivative of the couplings
+    uJ, uD = sh.param_spec(s, "J"), sh.param_spec(s, "D")
+    full = s.spec()
+    np.testing.assert_allclose(full["bilin"], 1.0 * uJ["bilin"] - 0.1 * uD["bilin"], atol=1e-15)
+    np.testing.assert_allclose(full["stevens"], 1.0 * uJ["stevens"] - 0.1 * uD["stevens"], atol=1e-15)
+    with pytest.raises(KeyError):
+        sh.set_params_(s, ["K"], [1.0])
