@@ -37,7 +37,7 @@ check(rc, h) = rc == 0 || (msg = unsafe_string(ccall((:snh_last_error, lib), Cst
                             rc == 3 ? error(msg) : error("libsunnyhip: " * msg))
 
 # Coupling tables in the layout of include/sunny_hip.h, built from the
-# homogeneous interactions of `sys` (src/System/Types.jl:46-74).
+# homogeneous interactions of `sys` (src/System/Types.jl:46-74, 106-140).
 function coupling_tables(sys::System{N}) where N
     ints = sys.interactions_union
     na = length(ints)
@@ -52,19 +52,13 @@ function coupling_tables(sys::System{N}) where N
         push!(hasbq, iszero(pc.biquad) ? 0 : 1)
         push!(scalar, pc.scalar)
     end
-    return (; na, bonds, bilin, biquad, hasbq, scalar)
-end
-
-function to_device(sys::System{N}; precision=64, seed=rand(UInt64)) where N
-    t = coupling_tables(sys)
-    na = t.na
     kappa = Float64[sys.κs[1, 1, 1, i] for i in 1:na]
     g = reduce(vcat, [vec(permutedims(sys.gs[1, 1, 1, i])) for i in 1:na])
     field = reduce(vcat, [Vector(sys.extfield[1, 1, 1, i]) for i in 1:na])
     stevens = zeros(28 * na); kmax = zeros(Int32, na)
     Lambda = zeros(2 * max(N, 1)^2 * na)
     for i in 1:na
-        os = sys.interactions_union[i].onsite
+        os = ints[i].onsite
         if N == 0
             stevens[28(i-1)+1:28i] = vcat(os.c0, os.c2, os.c4, os.c6); kmax[i] = os.kmax
         else
@@ -73,14 +67,19 @@ function to_device(sys::System{N}; precision=64, seed=rand(UInt64)) where N
         end
     end
     dims = Int32[sys.dims...]
-    ew = sys.ewald
     pos = reduce(vcat, sys.crystal.positions)
-    GC.@preserve kappa g field stevens kmax Lambda t dims pos begin
-        spec = SnhSpec(na, N, precision, pointer(kappa), pointer(g), pointer(field), pointer(stevens),
-                       pointer(kmax), pointer(Lambda), length(t.scalar), pointer(t.bonds), pointer(t.bilin),
+    return (; na, bonds, bilin, biquad, hasbq, scalar, kappa, g, field, stevens, kmax, Lambda, dims, pos)
+end
+
+function to_device(sys::System{N}; precision=64, seed=rand(UInt64)) where N
+    t = coupling_tables(sys)
+    ew = sys.ewald
+    GC.@preserve t begin
+        spec = SnhSpec(t.na, N, precision, pointer(t.kappa), pointer(t.g), pointer(t.field), pointer(t.stevens),
+                       pointer(t.kmax), pointer(t.Lambda), length(t.scalar), pointer(t.bonds), pointer(t.bilin),
                        pointer(t.biquad), pointer(t.hasbq), pointer(t.scalar), isnothing(ew) ? 0 : 1,
                        isnothing(ew) ? 0.0 : ew.μ0_μB², isnothing(ew) ? ntuple(_ -> 0.0, 9) : Tuple(vec(permutedims(ew.demag))),
-                       Tuple(vec(permutedims(sys.crystal.latvecs))), pointer(pos), 1, pointer(dims), 0)
+                       Tuple(vec(permutedims(sys.crystal.latvecs))), pointer(t.pos), 1, pointer(t.dims), 0)
         h = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:snh_create, lib), Cint, (Ref{SnhSpec}, Ref{Ptr{Cvoid}}), spec, h), C_NULL)
     end
@@ -123,6 +122,59 @@ function Sunny.step!(d::DeviceSystem, integrator::ImplicitMidpoint; max_iters=10
                 (Ptr{Cvoid}, Float64, Float64, Float64, Int32, Int32, Ptr{Int32}),
                 d.handle, integrator.dt, integrator.damping, integrator.tol, max_iters, 1, iters), d.handle)
     return
+end
+
+# ---------------------------------------------------------------------------
+# Reverse rule for a deterministic ImplicitMidpoint trajectory (no such rule
+# exists in Sunny; src/Fitting.jl:123-152 is the pattern).  `trajectory`
+# runs nsteps of step! and returns the final dipoles; its pullback maps the
+# cotangent of the final dipoles to cotangents of the labeled parameters
+# (as a SystemTangent over `labels`, src/System/Types.jl:143-146) and of the
+# initial dipoles.
+
+using ChainRulesCore
+
+function trajectory(d::DeviceSystem{0}, integrator::ImplicitMidpoint, nsteps::Int; checkpoint_every=50, max_iters=100)
+    iszero(integrator.damping) || error("the midpoint adjoint needs damping = 0")
+    check(ccall((:snh_midpoint_record, lib), Cint, (Ptr{Cvoid}, Float64, Float64, Int32, Int32, Int32),
+                d.handle, integrator.dt, integrator.tol, max_iters, nsteps, checkpoint_every), d.handle)
+    return copy(download!(d).dipoles)
+end
+
+# Unit couplings of parameter `label`: the system with only that parameter set to 1.
+function param_tables(sys::System{0}, label::Symbol)
+    unit = Sunny.clone_system(sys)
+    for p in unit.params
+        p.val = (p.label == label) ? 1.0 : 0.0
+    end
+    Sunny.repopulate_couplings_from_params!(unit)
+    return coupling_tables(unit)
+end
+
+function ChainRulesCore.rrule(::typeof(trajectory), d::DeviceSystem{0}, integrator::ImplicitMidpoint, nsteps::Int;
+                              labels::Vector{Symbol}, kws...)
+    S_final = trajectory(d, integrator, nsteps; kws...)
+    function trajectory_pullback(dS)
+        dLdS = collect(reinterpret(Float64, unthunk(dS)))
+        tabs = [param_tables(d.sys, l) for l in labels]
+        # snh_spec per parameter on the system's bond list (layout of to_device)
+        specs = map(tabs) do t
+            SnhSpec(t.na, 0, 64, pointer(t.kappa), pointer(t.g), pointer(t.field), pointer(t.stevens), pointer(t.kmax),
+                    pointer(t.Lambda), length(t.scalar), pointer(t.bonds), pointer(t.bilin), pointer(t.biquad), pointer(t.hasbq),
+                    pointer(t.scalar), 0, 0.0, ntuple(_ -> 0.0, 9), ntuple(_ -> 0.0, 9), pointer(t.pos), 1,
+                    pointer(t.dims), 0)
+        end
+        grad = zeros(length(labels))
+        dS0 = zeros(length(dLdS))
+        GC.@preserve tabs dLdS begin
+            check(ccall((:snh_midpoint_adjoint, lib), Cint,
+                        (Ptr{Cvoid}, Int32, Ptr{SnhSpec}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                        d.handle, length(labels), specs, dLdS, grad, dS0), d.handle)
+        end
+        tangent = Sunny.SystemTangent(Dict(zip(labels, grad)))
+        return NoTangent(), tangent, NoTangent(), NoTangent()
+    end
+    return S_final, trajectory_pullback
 end
 
 function Sunny.energy(d::DeviceSystem)
