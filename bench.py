@@ -126,6 +126,7 @@ def main():
                     help="device gather layout (auto: LDS tiles when they fit)")
     ap.add_argument("--fp32-only", action="store_true", help="time only the FP32 batch (tuning runs)")
     ap.add_argument("--fp64-only", action="store_true", help="time only the plain FP64 step (tuning runs)")
+    ap.add_argument("--kT", type=float, default=0.3, help="temperature of the timed runs (tuning: 0 disables noise)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (replicas shard across GPUs "
@@ -136,7 +137,7 @@ def main():
     t0 = time.perf_counter()
     base = sh.workloads.pyrochlore_model(L=args.L, seed=0, precision=32)
     log(f"model built in {time.perf_counter() - t0:.1f}s: {base.nsites} spins, {base.spec()['nbonds']} directed bonds")
-    integ = sh.Langevin(0.025, damping=0.2, kT=0.3)
+    integ = sh.Langevin(0.025, damping=0.2, kT=args.kT)
     R = args.replicas
 
     layout = 1 if args.layout == "ghost" else 0

@@ -105,9 +105,10 @@ struct TileSite {
 };
 
 // Sweeps the workgroup's column: loads the ring, and for every layer k calls
-// pre(k) (own-site global loads), issues the next layer's loads, then calls
-// body(k, phase).  NSRC = 2 loads the midpoint state
-// kappa*normalize((S0 + S1)/2) instead of S0.
+// pre(k + 1) (which moves the own-site inputs of layer k to the registers the
+// body reads and issues the loads of layer k + 1), issues the next ring
+// layer's loads, then calls body(k, phase).  NSRC = 2 loads the midpoint
+// state kappa*normalize((S0 + S1)/2) instead of S0.
 template <typename T, int NSRC, int SP, typename Pre, typename Body>
 __device__ __forceinline__ void tile_sweep(const Couplings<T>& C, const TileArgs& A, const TileWork& W,
                                            const ReplicaInfo& R, const T* __restrict__ s0, const T* __restrict__ s1,
@@ -163,6 +164,7 @@ __device__ __forceinline__ void tile_sweep(const Couplings<T>& C, const TileArgs
             d[2] = v.z;
         }
     };
+    pre(W.k0);
     for (int d = 0; d < A.nslot; d++) {
         fetch(W.k0 - A.w3 + d);
         put(d);
@@ -171,10 +173,10 @@ __device__ __forceinline__ void tile_sweep(const Couplings<T>& C, const TileArgs
     for (int k = W.k0; k < W.k1; k++) {
         const int rel = k - W.k0;
         const bool more = k + 1 < W.k1;
-        // the layer's own-site global inputs are issued before the prefetch:
-        // vector-memory counters retire in order, so waiting for them does
-        // not wait for the next layer's loads
-        pre(k);
+        // own-site inputs of layer k arrived with the previous layer's
+        // prefetch (vector-memory counters retire in order); issue those of
+        // layer k + 1, then the next ring layer
+        pre(more ? k + 1 : k);
         fetch(k + A.w3 + 1);  // wraps in bounds; unused after the last layer
         body(k, (rel + A.w3) % A.nslot);
         if (more) {
@@ -217,9 +219,10 @@ __device__ __forceinline__ V3<T> tile_field(const Couplings<T>& C, const int* __
 // ---------------------------------------------------------------------------
 // Kernels
 //
-// Every kernel computes up to SP sites per thread and layer.  Own-site global inputs (H, S, S1, the Ewald potential) are loaded
-// by the sweep's pre(k) hook into OwnIn registers, ahead of the next layer's
-// prefetch.
+// Every kernel computes up to SP sites per thread and layer.  Own-site global
+// inputs (H, S, S1, the Ewald potential) of layer k + 1 are loaded by the
+// sweep's pre hook into the nx registers while layer k is computed, and move
+// to the in registers the body reads at the next layer.
 
 template <typename T>
 struct OwnIn {
@@ -232,7 +235,7 @@ struct OwnIn {
     constexpr bool kTwo = SP > 1;                                                                   \
     const uint32_t lay = (uint32_t)R.L1 * (uint32_t)R.L2;                                           \
     const uint32_t slot = 3u * (uint32_t)sizeof(T_) * (uint32_t)A.cap;                              \
-    OwnIn<T_> in0{}, in1{}
+    OwnIn<T_> in0{}, in1{}, nx0{}, nx1{}
 
 // Heun predictor (see heun_predict): S' and H from S.
 template <typename T, int CH, bool kIso, int SP>
@@ -265,11 +268,13 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
     };
     tile_sweep<T, 1, SP>(
         C, A, W, R, S, nullptr, smem,
-        [&](int k) {
-            load(ts0, k, in0);
-            if (kTwo) load(ts1, k, in1);
+        [&](int k) __attribute__((always_inline)) {
+            in0 = nx0;
+            in1 = nx1;
+            load(ts0, k, nx0);
+            if (kTwo) load(ts1, k, nx1);
         },
-        [&](int k, int ph) {
+        [&](int k, int ph) __attribute__((always_inline)) {
             const int* offs = A.ldsoff + ph * C.nb;
             site(ts0, k, ph, offs, in0);
             if (kTwo) site(ts1, k, ph, offs, in1);
@@ -307,11 +312,13 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
     };
     tile_sweep<T, 1, SP>(
         C, A, W, R, Sp, nullptr, smem,
-        [&](int k) {
-            load(ts0, k, in0);
-            if (kTwo) load(ts1, k, in1);
+        [&](int k) __attribute__((always_inline)) {
+            in0 = nx0;
+            in1 = nx1;
+            load(ts0, k, nx0);
+            if (kTwo) load(ts1, k, nx1);
         },
-        [&](int k, int ph) {
+        [&](int k, int ph) __attribute__((always_inline)) {
             const int* offs = A.ldsoff + ph * C.nb;
             site(ts0, k, ph, offs, in0);
             if (kTwo) site(ts1, k, ph, offs, in1);
@@ -360,8 +367,10 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
         d2 += ex * ex + ey * ey + ez * ez;
     };
     auto pre = [&](int k) __attribute__((always_inline)) {
-        load(ts0, k, in0);
-        if (kTwo) load(ts1, k, in1);
+        in0 = nx0;
+        in1 = nx1;
+        load(ts0, k, nx0);
+        if (kTwo) load(ts1, k, nx1);
     };
     auto body = [&](int k, int ph) __attribute__((always_inline)) {
         const int* offs = A.ldsoff + ph * C.nb;
@@ -399,11 +408,13 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
     };
     tile_sweep<T, 1, SP>(
         C, A, W, R, S, nullptr, smem,
-        [&](int k) {
-            load(ts0, k, in0);
-            if (kTwo) load(ts1, k, in1);
+        [&](int k) __attribute__((always_inline)) {
+            in0 = nx0;
+            in1 = nx1;
+            load(ts0, k, nx0);
+            if (kTwo) load(ts1, k, nx1);
         },
-        [&](int k, int ph) {
+        [&](int k, int ph) __attribute__((always_inline)) {
             const int* offs = A.ldsoff + ph * C.nb;
             site(ts0, k, ph, offs, in0);
             if (kTwo) site(ts1, k, ph, offs, in1);
@@ -468,11 +479,13 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
     };
     tile_sweep<T, 1, SP>(
         C, A, W, R, S, nullptr, smem,
-        [&](int k) {
-            load(ts0, k, in0);
-            if (kTwo) load(ts1, k, in1);
+        [&](int k) __attribute__((always_inline)) {
+            in0 = nx0;
+            in1 = nx1;
+            load(ts0, k, nx0);
+            if (kTwo) load(ts1, k, nx1);
         },
-        [&](int k, int ph) {
+        [&](int k, int ph) __attribute__((always_inline)) {
             const int* offs = A.ldsoff + ph * C.nb;
             site(ts0, k, ph, offs, in0);
             if (kTwo) site(ts1, k, ph, offs, in1);
