@@ -113,6 +113,34 @@ def cpu_baseline(base_sys, seconds=15.0):
                       f"FP64 CPU port (oracle), one replica per thread, {t1 - t0:.1f} s"}
 
 
+def bench_sun3(args):
+    """Auxiliary line: 64 SU(3) triangular replicas, L_r uniform in 32..256,
+    kT geometric 0.1..2.0, Langevin dt = 0.02, damping 0.1, FP32 batch."""
+    import sunnyhip as sh
+    systems, kTs = sh.workloads.sun3_tempering_batch(seed=0)
+    batch = sh.ReplicaBatch(systems, precision=32)
+    integ = sh.Langevin(0.02, damping=0.1, kT=0.0)
+    h = batch._device()
+    sh.step_(batch, integ, nsteps=args.warmup, kTs=kTs)
+    h.call("snh_synchronize")
+    h.call("snh_enable_timing", 1)
+    t0 = time.perf_counter()
+    sh.step_(batch, integ, nsteps=args.steps, kTs=kTs)
+    h.call("snh_synchronize")
+    dt = time.perf_counter() - t0
+    fams = {f: kernel_time(h, f) for f in ("sun_predict", "sun_correct")}
+    h.call("snh_enable_timing", 0)
+    ups = batch.nsites * args.steps / dt
+    per = {f: ms / max(n, 1) for f, (ms, n) in fams.items()}
+    log(f"SU(3) batch: {batch.nsites} sites, {dt * 1e3 / args.steps:.3f} ms/step, {ups:.3e} spin-updates/s, {per}")
+    print(json.dumps({"metric": "spin-updates/s (auxiliary: SU(3) tempering batch)", "value": ups,
+                      "unit": "spin-updates/s", "ms_per_step": dt * 1e3 / args.steps, "steps": args.steps,
+                      "dtype": "f32", "kernels_ms": per,
+                      "config": {"workload": "64 SU(3) triangular replicas, L uniform 32..256 (seed 0), "
+                                             f"{batch.nsites} sites, kT 0.1..2.0 geometric, dt 0.02, damping 0.1"}}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +155,8 @@ def main():
     ap.add_argument("--fp32-only", action="store_true", help="time only the FP32 batch (tuning runs)")
     ap.add_argument("--fp64-only", action="store_true", help="time only the plain FP64 step (tuning runs)")
     ap.add_argument("--kT", type=float, default=0.3, help="temperature of the timed runs (tuning: 0 disables noise)")
+    ap.add_argument("--workload", choices=["pyrochlore", "sun3"], default="pyrochlore",
+                    help="pyrochlore: the headline batch; sun3: 64 uneven SU(3) tempering replicas (auxiliary)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (replicas shard across GPUs "
@@ -134,6 +164,8 @@ def main():
         sys.exit(2)
 
     import sunnyhip as sh
+    if args.workload == "sun3":
+        return bench_sun3(args)
     t0 = time.perf_counter()
     base = sh.workloads.pyrochlore_model(L=args.L, seed=0, precision=32)
     log(f"model built in {time.perf_counter() - t0:.1f}s: {base.nsites} spins, {base.spec()['nbonds']} directed bonds")

@@ -344,6 +344,8 @@ struct SystemImpl final : snh_system {
     T* B2 = nullptr;
     T* Bh = nullptr;    // materialised midpoint state (Ewald only)
     T* Sd = nullptr;    // expected dipoles of a coherent state (SU(N) + Ewald)
+    T* Dz = nullptr;    // SU(N) Langevin without biquadratic bonds: expected dipoles
+    T* Dzp = nullptr;   //   of the state and of the predictor stage, with images
     double* partial = nullptr;
     double* d_out = nullptr;   // [R]
     int* conv_iter = nullptr;  // [R]
@@ -726,11 +728,12 @@ struct SystemImpl final : snh_system {
         timer.events.push_back({fam, {e0, e1}});
     }
 
-    // Ewald potential phi of state planes Xs (dipoles; expected dipoles for SU(N)).
-    void ewald_field(const T* Xs, T* phi)
+    // Ewald potential phi of state planes Xs (dipoles; expected dipoles for
+    // SU(N), computed here unless the caller holds them in dip).
+    void ewald_field(const T* Xs, T* phi, const T* dip = nullptr)
     {
-        const T* S = Xs;
-        if (N > 0) {
+        const T* S = dip ? dip : Xs;
+        if (N > 0 && !dip) {
             launch_sun_expected_spin<T>(C, nblocks, Xs, Sd, st);
             S = Sd;
         }
@@ -846,12 +849,22 @@ struct SystemImpl final : snh_system {
         }
         SNH_HIP(hipStreamSynchronize(st));  // host vectors above are pageable
         const T tdt = (T)dt, tl = (T)damping;
+        // SU(N) with bilinear couplings only: the stages exchange expected
+        // dipoles (3 planes) instead of kets for the neighbour gather.
+        const bool dip = N > 0 && !any_biquad && !tiled;
+        if (dip && nsteps > 0) {
+            if (!Dz) {
+                Dz = own(dalloc<T>(3 * (size_t)total));
+                Dzp = own(dalloc<T>(3 * (size_t)total));
+            }
+            launch_sun_expected_spin<T>(C, nblocks, X, Dz, st);
+        }
         for (int s = 0; s < nsteps; s++) {
             nz.step = step0 + (uint64_t)s;
             const T* ewS = nullptr;
             const T* ewSp = nullptr;
             if (ewald) {
-                ewald_field(X, phi1);
+                ewald_field(X, phi1, dip ? Dz : nullptr);
                 ewS = phi1;
             }
             hipEvent_t e = tbegin();
@@ -860,10 +873,11 @@ struct SystemImpl final : snh_system {
             else if (N == 0)
                 launch_dipole_predict<T>(C, nblocks, X, Xp, Xo, ewS, nz, tdt, tl, st);
             else
-                launch_sun_predict<T>(C, nblocks, X, Xp, Xo, ewS, nz, tdt, tl, st);
+                launch_sun_predict<T>(C, nblocks, X, Xp, Xo, ewS, nz, tdt, tl, dip ? Dz : nullptr,
+                                      dip ? Dzp : nullptr, st);
             tend(N == 0 ? "dipole_predict" : "sun_predict", e);
             if (ewald) {
-                ewald_field(Xp, phi2);
+                ewald_field(Xp, phi2, dip ? Dzp : nullptr);
                 ewSp = phi2;
             }
             e = tbegin();  // the corrector overwrites the state in place (the predictor consumed it)
@@ -872,7 +886,8 @@ struct SystemImpl final : snh_system {
             else if (N == 0)
                 launch_dipole_correct<T>(C, nblocks, Xp, Xo, X, ewSp, tdt, tl, st);
             else
-                launch_sun_correct<T>(C, nblocks, Xp, Xo, X, ewSp, tdt, tl, st);
+                launch_sun_correct<T>(C, nblocks, Xp, Xo, X, ewSp, tdt, tl, dip ? Dzp : nullptr,
+                                      dip ? Dz : nullptr, st);
             tend(N == 0 ? "dipole_correct" : "sun_correct", e);
         }
         SNH_HIP(hipGetLastError());

@@ -184,15 +184,19 @@ template <typename T>
 void launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint32_t total, const T* gmat,
                         const T* S, const void* M, void* Fmu, void* Fphi, T* phi, hipStream_t st);
 
-// :SUN mode (snh_sun.hip).  Buffers are plane sets [2N][total].
+// :SUN mode (snh_sun.hip).  Buffers are plane sets [2N][total]; expected
+// dipoles are [3][total] planes with ghost images.
 template <typename T>
 void launch_sun_expected_spin(const Couplings<T>& C, int nblocks, const T* Z, T* S, hipStream_t st);
+// D/Dp (and Dp/Dout) non-null: neighbours are gathered from the expected
+// dipoles D of Z (bilinear couplings only) and the stage writes the dipoles
+// of its output; null: neighbours are gathered from the kets.
 template <typename T>
 void launch_sun_predict(const Couplings<T>& C, int nblocks, const T* Z, T* Zp, T* H, const T* phi,
-                        const NoiseArgs<T>& nz, T dt, T lambda, hipStream_t st);
+                        const NoiseArgs<T>& nz, T dt, T lambda, const T* D, T* Dp, hipStream_t st);
 template <typename T>
 void launch_sun_correct(const Couplings<T>& C, int nblocks, const T* Zp, const T* H, T* Zout, const T* phi, T dt,
-                        T lambda, hipStream_t st);
+                        T lambda, const T* Dp, T* Dout, hipStream_t st);
 template <typename T>
 void launch_sun_midpoint_bar(const Couplings<T>& C, int nblocks, const T* Z, const T* Z1, T* Zbar,
                              const int* conv_iter, hipStream_t st);

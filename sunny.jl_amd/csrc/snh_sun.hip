@@ -115,6 +115,8 @@ __device__ __forceinline__ void expected_quadrupole(const Ket<T, N>& Z, const T*
 // Source of kets for the gather: stored planes, or Zbar = (Z + Z1)/2.
 template <typename T, int N, bool kMid>
 struct KetSrc {
+    static constexpr bool kKets = true;
+    using Raw = Ket<T, N>;
     const T* Z;
     const T* Z1;
     uint32_t total;
@@ -130,7 +132,46 @@ struct KetSrc {
         }
         return a;
     }
+    __device__ __forceinline__ Raw load(uint32_t g) const { return get(g); }
+    __device__ __forceinline__ void spin(const Raw& r, T& sx, T& sy, T& sz) const { expected_spin<T, N>(r, sx, sy, sz); }
 };
+
+// Neighbour source of stored expected dipoles (3 planes with ghost images),
+// written by the kernel that produced the state: the gather reads 3 values
+// per neighbour instead of a ket, and recomputes nothing.  Bilinear
+// couplings only (a biquadratic bond needs the neighbour's ket).
+template <typename T, int N>
+struct DipSrc {
+    static constexpr bool kKets = false;
+    struct Raw {
+        T x, y, z;
+    };
+    const T* D;
+    uint32_t total;
+    __device__ __forceinline__ Raw load(uint32_t g) const { return {ldg(D, g), ldg(D, g + total), ldg(D, g + 2 * total)}; }
+    __device__ __forceinline__ void spin(const Raw& r, T& sx, T& sy, T& sz) const
+    {
+        sx = r.x;
+        sy = r.y;
+        sz = r.z;
+    }
+};
+
+// Stores the expected dipole of ket z at the site and its ghost images.
+template <typename T, int N>
+__device__ __forceinline__ void store_dipole_images(T* __restrict__ D, uint32_t total, const SiteCoord& sc,
+                                                    const Ghost& G, const Ket<T, N>& z)
+{
+    T sx, sy, sz;
+    expected_spin<T, N>(z, sx, sy, sz);
+    auto st = [&](uint32_t g) {
+        stg(D, g, sx);
+        stg(D, g + total, sy);
+        stg(D, g + 2 * total, sz);
+    };
+    st(sc.g);
+    for_each_ghost(sc, G, st);
+}
 
 // Neighbours gathered per batch of loads (bond lists are padded to a multiple
 // of it with zero self-couplings).
@@ -139,39 +180,38 @@ constexpr int kSunChunk = 6;
 // H Z at one site (set_energy_grad_coherents!): dE/dS from Zeeman, bilinear
 // exchange on expected dipoles and Ewald; then (Lambda + dE.S) Z plus
 // biquadratic (dE/dQ . Q) Z.
-template <typename T, int N, bool kMid>
-__device__ __forceinline__ Ket<T, N> local_hz(const Couplings<T>& C, const SiteCoord& sc,
-                                              const KetSrc<T, N, kMid>& src, const Ket<T, N>& Z,
-                                              const T* __restrict__ phi)
+template <typename T, int N, class Src>
+__device__ __forceinline__ Ket<T, N> local_hz(const Couplings<T>& C, const SiteCoord& sc, const Src& src,
+                                              const Ket<T, N>& Z, const T* __restrict__ phi)
 {
     const int i = sc.i;
-    T gx = C.zee[3 * i], gy = C.zee[3 * i + 1], gz = C.zee[3 * i + 2];
+    T gx = ldc(C.zee, 3 * i), gy = ldc(C.zee, 3 * i + 1), gz = ldc(C.zee, 3 * i + 2);
     T wq[5] = {0, 0, 0, 0, 0};
     bool anyq = false;
     const int* __restrict__ dl = C.bdelta + sc.R.dcls * C.nb;
-    const int b0 = C.bstart[i], b1 = C.bstart[i + 1];
+    const int b0 = ldc(C.bstart, i), b1 = ldc(C.bstart, i + 1);
     for (int c0 = b0; c0 < b1; c0 += kSunChunk) {
-        Ket<T, N> nbr[kSunChunk];
+        typename Src::Raw nbr[kSunChunk];
 #pragma unroll
-        for (int k = 0; k < kSunChunk; k++) nbr[k] = src.get(sc.g + (uint32_t)dl[c0 + k]);
+        for (int k = 0; k < kSunChunk; k++) nbr[k] = src.load(sc.g + (uint32_t)ldc(dl, c0 + k));
 #pragma unroll
         for (int k = 0; k < kSunChunk; k++) {
             const int b = c0 + k;
-            const int fl = C.bflag[b];
+            const int fl = ldc(C.bflag, b);
             T sx, sy, sz;
-            expected_spin<T, N>(nbr[k], sx, sy, sz);
+            src.spin(nbr[k], sx, sy, sz);
             const T* J = C.bJ + 9 * b;
             if (fl & kBondIso) {
-                const T J0 = C.bJ0[b];
+                const T J0 = ldc(C.bJ0, b);
                 gx += J0 * sx;
                 gy += J0 * sy;
                 gz += J0 * sz;
             } else {
-                gx += J[0] * sx + J[1] * sy + J[2] * sz;
-                gy += J[3] * sx + J[4] * sy + J[5] * sz;
-                gz += J[6] * sx + J[7] * sy + J[8] * sz;
+                gx += ldc(J, 0) * sx + ldc(J, 1) * sy + ldc(J, 2) * sz;
+                gy += ldc(J, 3) * sx + ldc(J, 4) * sy + ldc(J, 5) * sz;
+                gz += ldc(J, 6) * sx + ldc(J, 7) * sy + ldc(J, 8) * sz;
             }
-            if (fl & kBondBiquad) {
+            if constexpr (Src::kKets) if (fl & kBondBiquad) {
                 T Qj[5];
                 expected_quadrupole<T, N>(nbr[k], C.quad_ops, Qj);
                 const T* Bq = C.bQ + 25 * b;
@@ -343,25 +383,48 @@ __global__ __launch_bounds__(kBlock) void k_sun_expected_spin(Couplings<T> C, co
 {
     const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
     if (!sc.valid) return;
-    T sx, sy, sz;
-    expected_spin<T, N>(load_ket<T, N>(Z, C.total, sc.g), sx, sy, sz);
-    stg(S, sc.g, sx);
-    stg(S + C.total, sc.g, sy);
-    stg(S + 2 * (size_t)C.total, sc.g, sz);
+    store_dipole_images<T, N>(S, C.total, sc, C.G, load_ket<T, N>(Z, C.total, sc.g));
+}
+
+// H Z at one site with neighbours gathered from their kets (kDip false) or
+// from the expected-dipole planes D (kDip true).
+template <typename T, int N, bool kDip>
+__device__ __forceinline__ Ket<T, N> site_hz(const Couplings<T>& C, const SiteCoord& sc, const T* __restrict__ Z,
+                                             const T* __restrict__ D, const Ket<T, N>& z, const T* __restrict__ phi)
+{
+    if constexpr (kDip)
+        return local_hz<T, N>(C, sc, DipSrc<T, N>{D, C.total}, z, phi);
+    else
+        return local_hz<T, N>(C, sc, KetSrc<T, N, false>{Z, nullptr, C.total}, z, phi);
+}
+
+// Stores an updated ket.  With kDip the neighbours of the next stage read
+// expected dipoles: the ket goes to its own site only and the dipoles (with
+// ghost images) to Dout; otherwise the ket carries its ghost images.
+template <typename T, int N, bool kDip>
+__device__ __forceinline__ void store_state(T* __restrict__ Zo, T* __restrict__ Dout, const Couplings<T>& C,
+                                            const SiteCoord& sc, const Ket<T, N>& z)
+{
+    if constexpr (kDip) {
+        store_ket<T, N>(Zo, C.total, sc.g, z);
+        store_dipole_images<T, N>(Dout, C.total, sc, C.G, z);
+    } else {
+        store_ket_images<T, N>(Zo, C.total, sc, C.G, z);
+    }
 }
 
 // Heun predictor: dZ1 = rhs(Z, zeta, HZ); Z' = normalize(Z + dZ1);
 // H = Z + dZ1/2 - proj(zeta, Z')/2.
-template <typename T, int N>
+template <typename T, int N, bool kDip>
 __global__ __launch_bounds__(kBlock) void k_sun_predict(Couplings<T> C, const T* __restrict__ Z, T* __restrict__ Zp,
                                                         T* __restrict__ Hb, const T* __restrict__ phi,
-                                                        NoiseArgs<T> nz, T dt, T lambda)
+                                                        NoiseArgs<T> nz, T dt, T lambda, const T* __restrict__ D,
+                                                        T* __restrict__ Dp)
 {
     const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
     if (!sc.valid) return;
-    const KetSrc<T, N, false> src{Z, nullptr, C.total};
     const Ket<T, N> z = load_ket<T, N>(Z, C.total, sc.g);
-    Ket<T, N> d = rhs_det<T, N>(z, local_hz<T, N, false>(C, sc, src, z, phi), dt, lambda);
+    Ket<T, N> d = rhs_det<T, N>(z, site_hz<T, N, kDip>(C, sc, Z, D, z, phi), dt, lambda);
     Ket<T, N> zeta;
     const bool noisy = lambda != (T)0 && nz.mode != 0;
     if (noisy) {
@@ -376,8 +439,8 @@ __global__ __launch_bounds__(kBlock) void k_sun_predict(Couplings<T> C, const T*
     Ket<T, N> zp;
 #pragma unroll
     for (int a = 0; a < N; a++) zp.z[a] = {z.z[a].re + d.z[a].re, z.z[a].im + d.z[a].im};
-    zp = normalize_ket<T, N>(zp, C.kappa[sc.i]);
-    store_ket_images<T, N>(Zp, C.total, sc, C.G, zp);
+    zp = normalize_ket<T, N>(zp, ldc(C.kappa, sc.i));
+    store_state<T, N, kDip>(Zp, Dp, C, sc, zp);
     Ket<T, N> h;
 #pragma unroll
     for (int a = 0; a < N; a++) h.z[a] = {z.z[a].re + d.z[a].re / 2, z.z[a].im + d.z[a].im / 2};
@@ -393,23 +456,26 @@ __global__ __launch_bounds__(kBlock) void k_sun_predict(Couplings<T> C, const T*
 }
 
 // Heun corrector: Z <- normalize(H + rhs_det(Z', HZ')/2).
-template <typename T, int N>
+template <typename T, int N, bool kDip>
 __global__ __launch_bounds__(kBlock) void k_sun_correct(Couplings<T> C, const T* __restrict__ Zp,
                                                         const T* __restrict__ Hb, T* __restrict__ Zout,
-                                                        const T* __restrict__ phi, T dt, T lambda)
+                                                        const T* __restrict__ phi, T dt, T lambda,
+                                                        const T* __restrict__ Dp, T* __restrict__ Dout)
 {
     const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
     if (!sc.valid) return;
-    const KetSrc<T, N, false> src{Zp, nullptr, C.total};
     const Ket<T, N> zp = load_ket<T, N>(Zp, C.total, sc.g);
-    const Ket<T, N> d = rhs_det<T, N>(zp, local_hz<T, N, false>(C, sc, src, zp, phi), dt, lambda);
+    const Ket<T, N> d = rhs_det<T, N>(zp, site_hz<T, N, kDip>(C, sc, Zp, Dp, zp, phi), dt, lambda);
     Ket<T, N> h = load_ket<T, N>(Hb, C.total, sc.g);
 #pragma unroll
     for (int a = 0; a < N; a++) {
         h.z[a].re += d.z[a].re / 2;
         h.z[a].im += d.z[a].im / 2;
     }
-    store_ket_images<T, N>(Zout, C.total, sc, C.G, normalize_ket<T, N>(h, C.kappa[sc.i]));
+    // the state keeps its ket images (energy, gradients and midpoint gather kets)
+    const Ket<T, N> zo = normalize_ket<T, N>(h, ldc(C.kappa, sc.i));
+    store_ket_images<T, N>(Zout, C.total, sc, C.G, zo);
+    if constexpr (kDip) store_dipole_images<T, N>(Dout, C.total, sc, C.G, zo);
 }
 
 template <typename T, int N>
@@ -436,7 +502,7 @@ __global__ __launch_bounds__(kBlock) void k_sun_mid_iter(Couplings<T> C, const T
     if (sc.valid) {
         const KetSrc<T, N, true> src{Z, Z1, C.total};
         const Ket<T, N> zb = src.get(sc.g);
-        const Ket<T, N> d = rhs_det<T, N>(zb, local_hz<T, N, true>(C, sc, src, zb, phi), dt, lambda);
+        const Ket<T, N> d = rhs_det<T, N>(zb, local_hz<T, N>(C, sc, src, zb, phi), dt, lambda);
         Ket<T, N> z = load_ket<T, N>(Z, C.total, sc.g);
         const Ket<T, N> z1 = load_ket<T, N>(Z1, C.total, sc.g);
 #pragma unroll
@@ -472,7 +538,7 @@ __global__ __launch_bounds__(kBlock) void k_sun_grad(Couplings<T> C, const T* __
     if (!sc.valid) return;
     const KetSrc<T, N, false> src{Z, nullptr, C.total};
     const Ket<T, N> z = load_ket<T, N>(Z, C.total, sc.g);
-    store_ket<T, N>(HZ, C.total, sc.g, local_hz<T, N, false>(C, sc, src, z, phi));
+    store_ket<T, N>(HZ, C.total, sc.g, local_hz<T, N>(C, sc, src, z, phi));
 }
 
 template <typename T, int N>
@@ -544,6 +610,18 @@ __global__ __launch_bounds__(kBlock) void k_sun_energy(Couplings<T> C, const T* 
     default: break;                                                  \
     }
 
+// As SNH_SUN_DISPATCH, with one more template argument X after N.
+#define SNH_SUN_DISPATCH_X(N, KERNEL, X, ...)                           \
+    switch (N) {                                                        \
+    case 2: hipLaunchKernelGGL((KERNEL<T, 2, X>), __VA_ARGS__); break;  \
+    case 3: hipLaunchKernelGGL((KERNEL<T, 3, X>), __VA_ARGS__); break;  \
+    case 4: hipLaunchKernelGGL((KERNEL<T, 4, X>), __VA_ARGS__); break;  \
+    case 5: hipLaunchKernelGGL((KERNEL<T, 5, X>), __VA_ARGS__); break;  \
+    case 6: hipLaunchKernelGGL((KERNEL<T, 6, X>), __VA_ARGS__); break;  \
+    case 7: hipLaunchKernelGGL((KERNEL<T, 7, X>), __VA_ARGS__); break;  \
+    default: break;                                                     \
+    }
+
 template <typename T>
 void launch_sun_expected_spin(const Couplings<T>& C, int nblocks, const T* Z, T* S, hipStream_t st)
 {
@@ -552,16 +630,32 @@ void launch_sun_expected_spin(const Couplings<T>& C, int nblocks, const T* Z, T*
 
 template <typename T>
 void launch_sun_predict(const Couplings<T>& C, int nblocks, const T* Z, T* Zp, T* H, const T* phi,
-                        const NoiseArgs<T>& nz, T dt, T lambda, hipStream_t st)
+                        const NoiseArgs<T>& nz, T dt, T lambda, const T* D, T* Dp, hipStream_t st)
 {
-    SNH_SUN_DISPATCH(C.N, k_sun_predict, dim3(nblocks), dim3(kBlock), 0, st, C, Z, Zp, H, phi, nz, dt, lambda)
+#define SNH_SP(KD)                                                                                    \
+    SNH_SUN_DISPATCH_X(C.N, k_sun_predict, KD, dim3(nblocks), dim3(kBlock), 0, st, C, Z, Zp, H, phi, nz, dt, \
+                       lambda, D, Dp)
+    if (D) {
+        SNH_SP(true)
+    } else {
+        SNH_SP(false)
+    }
+#undef SNH_SP
 }
 
 template <typename T>
 void launch_sun_correct(const Couplings<T>& C, int nblocks, const T* Zp, const T* H, T* Zout, const T* phi, T dt,
-                        T lambda, hipStream_t st)
+                        T lambda, const T* Dp, T* Dout, hipStream_t st)
 {
-    SNH_SUN_DISPATCH(C.N, k_sun_correct, dim3(nblocks), dim3(kBlock), 0, st, C, Zp, H, Zout, phi, dt, lambda)
+#define SNH_SC(KD)                                                                                   \
+    SNH_SUN_DISPATCH_X(C.N, k_sun_correct, KD, dim3(nblocks), dim3(kBlock), 0, st, C, Zp, H, Zout, phi, dt, \
+                       lambda, Dp, Dout)
+    if (Dp) {
+        SNH_SC(true)
+    } else {
+        SNH_SC(false)
+    }
+#undef SNH_SC
 }
 
 template <typename T>
@@ -601,9 +695,9 @@ void launch_sun_energy(const Couplings<T>& C, int nblocks, const T* Z, const T* 
 #define SNH_SUN_INST(T)                                                                                          \
     template void launch_sun_expected_spin<T>(const Couplings<T>&, int, const T*, T*, hipStream_t);              \
     template void launch_sun_predict<T>(const Couplings<T>&, int, const T*, T*, T*, const T*,                    \
-                                        const NoiseArgs<T>&, T, T, hipStream_t);                                 \
+                                        const NoiseArgs<T>&, T, T, const T*, T*, hipStream_t);                   \
     template void launch_sun_correct<T>(const Couplings<T>&, int, const T*, const T*, T*, const T*, T, T,        \
-                                        hipStream_t);                                                            \
+                                        const T*, T*, hipStream_t);                                              \
     template void launch_sun_midpoint_bar<T>(const Couplings<T>&, int, const T*, const T*, T*, const int*,       \
                                              hipStream_t);                                                       \
     template void launch_sun_midpoint_iter<T>(const Couplings<T>&, int, const T*, const T*, T*, const T*, T, T,  \

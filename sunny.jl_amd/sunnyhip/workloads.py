@@ -77,3 +77,14 @@ def triangular_sun_model(L1=32, L2=None, J1=-0.24, J2=0.05, J3=0.2, D=2.2, s=1.0
         set_onsite_coupling_(sys, lambda S: D * S[2] @ S[2], 0)
     randomize_spins_(sys)
     return sys
+
+
+def sun3_tempering_batch(nreplicas=64, Lmin=32, Lmax=256, kT_min=0.1, kT_max=2.0, seed=0, precision=32):
+    """Parallel-tempering set of SU(3) triangular replicas of uneven size:
+    L_r drawn uniformly from {Lmin..Lmax} with the seed, L_r x L_r x 1 cells,
+    temperatures geometric from kT_min to kT_max.  Returns (systems, kTs)."""
+    rng = np.random.default_rng(seed)
+    Ls = rng.integers(Lmin, Lmax + 1, size=nreplicas)
+    systems = [triangular_sun_model(L1=int(L), seed=seed + 1 + r, precision=precision) for r, L in enumerate(Ls)]
+    kTs = kT_min * (kT_max / kT_min) ** (np.arange(nreplicas) / max(1, nreplicas - 1))
+    return systems, kTs

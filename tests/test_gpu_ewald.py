@@ -87,3 +87,11 @@ def test_ewald_sun():
     o = O.Oracle(s.spec())
     assert rel_err(sh.energy_grad_coherents(s), o.grad_coherents(Z)) <= 1e-11
     assert abs(sh.energy(s) - o.energy(Z=Z)) <= 1e-11
+    # Langevin: both Heun stages take the Ewald potential of expected dipoles
+    rng = np.random.default_rng(8)
+    integ = sh.Langevin(0.02, damping=0.1, kT=0.3)
+    for _ in range(2):
+        noise = (rng.standard_normal((s.nsites, s.N)) + 1j * rng.standard_normal((s.nsites, s.N))) / np.sqrt(2)
+        Z = o.step_langevin(Z=Z, noise=noise, dt=integ.dt, damping=integ.damping, kT=integ.kT)
+        sh.step_(s, integ, noise=noise)
+    assert rel_err(s.coherents_flat(), Z, scale=1.0) <= 1e-11
