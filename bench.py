@@ -141,6 +141,41 @@ def bench_sun3(args):
           flush=True)
 
 
+def bench_ewald(args):
+    """Auxiliary line: pyrochlore 16^3 cells x 16 = 65,536 spins, s = 7/2,
+    J1 = 0.05 meV, Ewald dipole-dipole (mu0 muB^2 = 0.6745817653324668,
+    demag 1/3), FP32 Langevin dt = 0.025, damping 0.2, kT 0.3; one replica
+    and a batch sharing the transformed Ewald tensor."""
+    import sunnyhip as sh
+    out = {}
+    for R in (1, args.ewald_replicas):
+        systems = [sh.workloads.pyrochlore_model(L=16, J1=0.05, J2=0.0, A=0.0, s=3.5, seed=40 + r, precision=32,
+                                                 dipolar=True, mu0_muB2=0.6745817653324668, demag=1 / 3)
+                   for r in range(R)]
+        obj = systems[0] if R == 1 else sh.ReplicaBatch(systems, precision=32)
+        integ = sh.Langevin(0.025, damping=0.2, kT=0.3)
+        h = obj._device()
+        sh.step_(obj, integ, nsteps=args.warmup)
+        h.call("snh_synchronize")
+        h.call("snh_enable_timing", 1)
+        t0 = time.perf_counter()
+        sh.step_(obj, integ, nsteps=args.steps)
+        h.call("snh_synchronize")
+        dt = time.perf_counter() - t0
+        fams = {f: kernel_time(h, f) for f in ("dipole_predict", "dipole_correct", "ewald_field")}
+        h.call("snh_enable_timing", 0)
+        ns = sum(s.nsites for s in systems)
+        per = {f: ms / max(n, 1) for f, (ms, n) in fams.items()}
+        ups = ns * args.steps / dt
+        log(f"Ewald R={R}: {ns} sites, {dt * 1e3 / args.steps:.3f} ms/step, {ups:.3e} spin-updates/s, {per}")
+        out[R] = {"ms_per_step": dt * 1e3 / args.steps, "value": ups, "kernels_ms": per}
+    print(json.dumps({"metric": "spin-updates/s (auxiliary: Ewald pyrochlore 16^3)", "unit": "spin-updates/s",
+                      "dtype": "f32", "steps": args.steps, "by_replicas": out,
+                      "config": {"workload": "pyrochlore 16^3 cells x 16 = 65536 spins, s=7/2, J1=0.05, "
+                                             "Ewald mu0muB2=0.6745817653324668 demag 1/3, Langevin dt 0.025"}}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,8 +190,10 @@ def main():
     ap.add_argument("--fp32-only", action="store_true", help="time only the FP32 batch (tuning runs)")
     ap.add_argument("--fp64-only", action="store_true", help="time only the plain FP64 step (tuning runs)")
     ap.add_argument("--kT", type=float, default=0.3, help="temperature of the timed runs (tuning: 0 disables noise)")
-    ap.add_argument("--workload", choices=["pyrochlore", "sun3"], default="pyrochlore",
-                    help="pyrochlore: the headline batch; sun3: 64 uneven SU(3) tempering replicas (auxiliary)")
+    ap.add_argument("--workload", choices=["pyrochlore", "sun3", "ewald"], default="pyrochlore",
+                    help="pyrochlore: the headline batch; sun3: 64 uneven SU(3) tempering replicas; "
+                         "ewald: dipolar pyrochlore 16^3 (auxiliary lines)")
+    ap.add_argument("--ewald-replicas", type=int, default=32, help="batch size of the Ewald auxiliary line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (replicas shard across GPUs "
@@ -166,6 +203,8 @@ def main():
     import sunnyhip as sh
     if args.workload == "sun3":
         return bench_sun3(args)
+    if args.workload == "ewald":
+        return bench_ewald(args)
     t0 = time.perf_counter()
     base = sh.workloads.pyrochlore_model(L=args.L, seed=0, precision=32)
     log(f"model built in {time.perf_counter() - t0:.1f}s: {base.nsites} spins, {base.spec()['nbonds']} directed bonds")

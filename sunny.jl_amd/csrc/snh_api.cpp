@@ -654,10 +654,19 @@ struct SystemImpl final : snh_system {
         EG.w1 = G.w1;
         EG.w2 = G.w2;
         EG.w3 = G.w3;
-        SNH_REQUIRE(ewald_fft_lds_bytes(EG.nk, 16) <= 65536, SNH_ERR_UNSUPPORTED,
-                    "Ewald FFT grid too large for one workgroup's LDS");
-        SNH_REQUIRE((size_t)R * 3 * na * sizeof(T) * 2 <= 65536, SNH_ERR_UNSUPPORTED,
-                    "too many replicas x sublattices for the Ewald contraction");
+        {
+            int dev = 0;
+            SNH_HIP(hipGetDevice(&dev));
+            hipDeviceProp_t prop;
+            SNH_HIP(hipGetDeviceProperties(&prop, dev));
+            const size_t lim = prop.sharedMemPerBlock;
+            // the FP64 transform of the tensor A is the largest FFT workgroup
+            SNH_REQUIRE(ewald_fft_lds_bytes(EG, sizeof(double)) <= lim, SNH_ERR_UNSUPPORTED,
+                        "Ewald FFT grid too large for one workgroup's LDS");
+            const size_t lc = ewald_use_mfma(EG, R, sizeof(T)) ? ewald_mfma_lds_bytes(EG)
+                                                               : (size_t)R * 3 * na * sizeof(T) * 2;
+            SNH_REQUIRE(lc <= lim, SNH_ERR_UNSUPPORTED, "too many replicas x sublattices for the Ewald contraction");
+        }
 
         // A precompute parameters, Ewald.jl:61-91
         EwaldAParams P{};
@@ -705,8 +714,8 @@ struct SystemImpl final : snh_system {
         SNH_HIP(hipStreamSynchronize(st));
         SNH_HIP(hipFree(d_A));
         SNH_HIP(hipFree(d_pos));
-        Fmu = own(dalloc<char>((size_t)R * n3 * EG.nk * 2 * sizeof(T)));
-        Fphi = own(dalloc<char>((size_t)R * n3 * EG.nk * 2 * sizeof(T)));
+        Fmu = own(dalloc<char>(ewald_spectrum_bytes(EG, R, sizeof(T))));
+        Fphi = own(dalloc<char>(ewald_spectrum_bytes(EG, R, sizeof(T))));
         phi1 = own(dalloc<T>(3 * (size_t)total));
         phi2 = own(dalloc<T>(3 * (size_t)total));
         if (N > 0) Sd = own(dalloc<T>(3 * (size_t)total));

@@ -8,55 +8,264 @@
 //   ewald_energy                          src/System/Ewald.jl:155-181
 //
 // Pipeline per evaluation (one real field per (sublattice, component)):
-//   k_ewald_fwd    mu = -g S, 3D r2c DFT of each field inside one workgroup's LDS
-//   k_ewald_contract  F[phi]_k = M_k F[mu]_k with M_k = conj(F[A]_k), a
+//   k_ewald_fwd    mu = -g S staged in LDS, 3D r2c FFT of each field inside
+//                  one workgroup (radix-2 in registers per line for
+//                  power-of-two axes, direct DFT otherwise)
+//   k_ewald_contract / k_ewald_contract_mfma
+//                  F[phi]_k = M_k F[mu]_k with M_k = conj(F[A]_k), a
 //                  (3na x 3na) complex block per half-spectrum wavevector k,
-//                  stored column-major so each column read is coalesced
-//   k_ewald_inv    3D c2r DFT of F[phi] (1/Ncell scaling) -> phi planes
+//                  stored column-major so each column read is coalesced.
+//                  Few replicas: one matrix-vector product per k (streams M,
+//                  HBM-bound).  Many replicas: a complex GEMM per k on the
+//                  FP32 matrix cores (v_mfma_f32_16x16x4_f32), every M_k read
+//                  once for up to 32 replicas.
+//   k_ewald_inv    3D c2r FFT of F[phi] (1/Ncell scaling) -> phi planes
 // The local-field kernels then add dE/dS -= g' phi and the energy 1/2 mu.phi.
 // Wavevectors: k1 in [0, L1/2] (half spectrum along the fastest cell axis,
 // as rfft), k2 in [0, L2), k3 in [0, L3); flat index k1 + H1*(k2 + L2*k3).
+// Spectra of all fields live in HBM in blocks of kSpecKB wavevectors,
+// [k / kSpecKB][replica][field][k % kSpecKB]: each transform workgroup reads
+// and writes whole 128-byte runs, and the contraction of a block for a group
+// of replicas reads one compact range.
 #include "snh_internal.h"
+
+#include <cstdlib>
+#include <map>
+#include <mutex>
 
 namespace snh {
 
 constexpr int kLmax = 32;  // longest supported axis of the FFT grid
+constexpr int kFftThreads = 256;
+constexpr int kSpecKB = 16;  // wavevectors per spectrum block (128-byte runs in FP32)
 
 template <typename T>
 struct C2 {
     T re, im;
 };
 
-
-
-// 1D DFT of one line held in registers, written back to LDS with stride.
-// sign = -1 forward, +1 backward.  tw[m] = exp(sign * 2 pi i m / L).
 template <typename T>
-__device__ inline void dft_line_c2c(C2<T>* buf, long long base, long long stride, int L, const C2<T>* tw)
+__device__ __forceinline__ C2<T> cmul2(C2<T> a, C2<T> b)
 {
-    C2<T> in[kLmax];
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// ---------------------------------------------------------------------------
+// One line of the 3D transform.  tw[m] = exp(sign 2 pi i m / L), m < L, in LDS.
+
+constexpr int bit_reverse(int i, int L)
+{
+    int r = 0;
+    for (int b = 1; b < L; b <<= 1) {
+        r = (r << 1) | (i & 1);
+        i >>= 1;
+    }
+    return r;
+}
+
+constexpr int ilog2(int L) { return L <= 1 ? 0 : 1 + ilog2(L >> 1); }
+
+// In-place radix-2 decimation-in-time FFT of L (a power of two) values held
+// in registers; all indices are compile-time after unrolling.
+template <typename T, int L>
+__device__ __forceinline__ void fft_regs(C2<T> (&x)[L], const C2<T>* __restrict__ tw)
+{
 #pragma unroll
-    for (int x = 0; x < kLmax; x++)
-        if (x < L) in[x] = buf[base + x * stride];
-    for (int k = 0; k < L; k++) {
-        T ar = 0, ai = 0;
-        int m = 0;
+    for (int i = 0; i < L; i++) {
+        const int j = bit_reverse(i, L);
+        if (j > i) {
+            const C2<T> t = x[i];
+            x[i] = x[j];
+            x[j] = t;
+        }
+    }
 #pragma unroll
-        for (int x = 0; x < kLmax; x++) {
-            if (x < L) {
-                C2<T> w = tw[m];
-                ar += in[x].re * w.re - in[x].im * w.im;
-                ai += in[x].re * w.im + in[x].im * w.re;
+    for (int s = 1; s <= ilog2(L); s++) {
+        const int len = 1 << s, half = len >> 1, step = L / len;
+#pragma unroll
+        for (int i = 0; i < L; i += len)
+#pragma unroll
+            for (int j = 0; j < half; j++) {
+                const C2<T> u = x[i + j];
+                const C2<T> v = j == 0 ? x[i + j + half] : cmul2(x[i + j + half], tw[j * step]);
+                x[i + j] = {u.re + v.re, u.im + v.im};
+                x[i + j + half] = {u.re - v.re, u.im - v.im};
+            }
+    }
+}
+
+// c2c transform of the L values buf[base + x stride] in place.
+template <typename T, int L>
+__device__ __forceinline__ void line_c2c_pow2(C2<T>* buf, int base, int stride, const C2<T>* tw)
+{
+    C2<T> x[L];
+#pragma unroll
+    for (int i = 0; i < L; i++) x[i] = buf[base + i * stride];
+    fft_regs<T, L>(x, tw);
+#pragma unroll
+    for (int i = 0; i < L; i++) buf[base + i * stride] = x[i];
+}
+
+// Direct DFT of a line of any length L <= kLmax, in place: inputs are read
+// from LDS for every output, outputs held in registers until all are formed.
+template <typename T>
+__device__ __forceinline__ void line_c2c_any(C2<T>* buf, int base, int stride, int L, const C2<T>* tw)
+{
+    C2<T> out[kLmax];
+#pragma unroll
+    for (int k = 0; k < kLmax; k++) {
+        if (k < L) {
+            T ar = 0, ai = 0;
+            int m = 0;
+            for (int x = 0; x < L; x++) {
+                const C2<T> v = buf[base + x * stride], w = tw[m];
+                ar += v.re * w.re - v.im * w.im;
+                ai += v.re * w.im + v.im * w.re;
                 m += k;
                 if (m >= L) m -= L;
             }
+            out[k] = {ar, ai};
         }
-        buf[base + k * stride] = {ar, ai};
+    }
+#pragma unroll
+    for (int k = 0; k < kLmax; k++)
+        if (k < L) buf[base + k * stride] = out[k];
+}
+
+template <typename T>
+__device__ __forceinline__ void line_c2c(C2<T>* buf, int base, int stride, int L, const C2<T>* tw)
+{
+    switch (L) {
+    case 1: return;
+    case 2: line_c2c_pow2<T, 2>(buf, base, stride, tw); return;
+    case 4: line_c2c_pow2<T, 4>(buf, base, stride, tw); return;
+    case 8: line_c2c_pow2<T, 8>(buf, base, stride, tw); return;
+    case 16: line_c2c_pow2<T, 16>(buf, base, stride, tw); return;
+    case 32: line_c2c_pow2<T, 32>(buf, base, stride, tw); return;
+    default: line_c2c_any<T>(buf, base, stride, L, tw); return;
+    }
+}
+
+// Real line fin[0..L) -> half spectrum out[0..L/2] (forward).
+template <typename T, int L>
+__device__ __forceinline__ void line_r2c_pow2(const T* fin, C2<T>* out, const C2<T>* tw)
+{
+    C2<T> x[L];
+#pragma unroll
+    for (int i = 0; i < L; i++) x[i] = {fin[i], (T)0};
+    fft_regs<T, L>(x, tw);
+#pragma unroll
+    for (int k = 0; k <= L / 2; k++) out[k] = x[k];
+}
+
+template <typename T>
+__device__ __forceinline__ void line_r2c_any(const T* fin, C2<T>* out, int L, const C2<T>* tw)
+{
+    for (int k = 0; k <= L / 2; k++) {
+        T ar = 0, ai = 0;
+        int m = 0;
+        for (int x = 0; x < L; x++) {
+            const T v = fin[x];
+            ar += v * tw[m].re;
+            ai += v * tw[m].im;
+            m += k;
+            if (m >= L) m -= L;
+        }
+        out[k] = {ar, ai};
     }
 }
 
 template <typename T>
-__device__ inline void make_twiddles(C2<T>* tw, int L, int sign)
+__device__ __forceinline__ void line_r2c(const T* fin, C2<T>* out, int L, const C2<T>* tw)
+{
+    switch (L) {
+    case 1: out[0] = {fin[0], (T)0}; return;
+    case 2: line_r2c_pow2<T, 2>(fin, out, tw); return;
+    case 4: line_r2c_pow2<T, 4>(fin, out, tw); return;
+    case 8: line_r2c_pow2<T, 8>(fin, out, tw); return;
+    case 16: line_r2c_pow2<T, 16>(fin, out, tw); return;
+    case 32: line_r2c_pow2<T, 32>(fin, out, tw); return;
+    default: line_r2c_any<T>(fin, out, L, tw); return;
+    }
+}
+
+// Half spectrum in[0..L/2] -> real line fout[0..L) (backward, unscaled): the
+// real part of the inverse transform of the Hermitian extension
+// X[L-k] = conj(X[k]); imaginary parts of X[0] and X[L/2] drop out, as c2r.
+template <typename T, int L>
+__device__ __forceinline__ void line_c2r_pow2(const C2<T>* in, T* fout, const C2<T>* tw)
+{
+    C2<T> x[L];
+#pragma unroll
+    for (int k = 0; k <= L / 2; k++) x[k] = in[k];
+#pragma unroll
+    for (int k = L / 2 + 1; k < L; k++) x[k] = {x[L - k].re, -x[L - k].im};
+    fft_regs<T, L>(x, tw);
+#pragma unroll
+    for (int i = 0; i < L; i++) fout[i] = x[i].re;
+}
+
+template <typename T>
+__device__ __forceinline__ void line_c2r_any(const C2<T>* in, T* fout, int L, const C2<T>* tw)
+{
+    for (int x = 0; x < L; x++) {
+        T acc = 0;
+        int m = 0;  // k x mod L
+        for (int k = 0; k < L; k++) {
+            const C2<T> X = k <= L / 2 ? in[k] : C2<T>{in[L - k].re, -in[L - k].im};
+            acc += X.re * tw[m].re - X.im * tw[m].im;
+            m += x;
+            if (m >= L) m -= L;
+        }
+        fout[x] = acc;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void line_c2r(const C2<T>* in, T* fout, int L, const C2<T>* tw)
+{
+    switch (L) {
+    case 1: fout[0] = in[0].re; return;
+    case 2: line_c2r_pow2<T, 2>(in, fout, tw); return;
+    case 4: line_c2r_pow2<T, 4>(in, fout, tw); return;
+    case 8: line_c2r_pow2<T, 8>(in, fout, tw); return;
+    case 16: line_c2r_pow2<T, 16>(in, fout, tw); return;
+    case 32: line_c2r_pow2<T, 32>(in, fout, tw); return;
+    default: line_c2r_any<T>(in, fout, L, tw); return;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One workgroup transforms one field.  Dynamic LDS: spectrum [nk] complex,
+// three twiddle tables [kLmax], the real field [L2 L3][L1 + 1] (row padded
+// by one word against bank conflicts).
+
+template <typename T>
+struct FftLds {
+    C2<T>* spec;
+    C2<T>* tw1;
+    C2<T>* tw2;
+    C2<T>* tw3;
+    T* fld;
+    int row;  // L1 + 1
+};
+
+template <typename T>
+__device__ __forceinline__ FftLds<T> fft_lds(const EwaldGeom& G)
+{
+    extern __shared__ __align__(16) unsigned char snh_lds[];
+    FftLds<T> F;
+    F.spec = reinterpret_cast<C2<T>*>(snh_lds);
+    F.tw1 = F.spec + G.nk;
+    F.tw2 = F.tw1 + kLmax;
+    F.tw3 = F.tw2 + kLmax;
+    F.fld = reinterpret_cast<T*>(F.tw3 + kLmax);
+    F.row = G.L1 + 1;
+    return F;
+}
+
+template <typename T>
+__device__ __forceinline__ void make_twiddles(C2<T>* tw, int L, int sign)
 {
     for (int m = threadIdx.x; m < L; m += blockDim.x) {
         double s, c;
@@ -65,54 +274,45 @@ __device__ inline void make_twiddles(C2<T>* tw, int L, int sign)
     }
 }
 
-// Forward r2c 3D DFT of the field values produced by `load(c1, c2, c3)`,
-// leaving the half spectrum in LDS `spec` (layout k1 + H1*(k2 + L2*k3)).
-template <typename T, typename Loader>
-__device__ inline void fwd3d(const EwaldGeom& G, C2<T>* spec, C2<T>* tw1, C2<T>* tw2, C2<T>* tw3, Loader load)
+// Forward r2c 3D FFT of F.fld (filled by the caller), leaving the half
+// spectrum in F.spec.  Twiddles must be ready (sign -1).
+template <typename T>
+__device__ __forceinline__ void fwd3d(const EwaldGeom& G, const FftLds<T>& F)
 {
-    make_twiddles(tw1, G.L1, -1);
-    make_twiddles(tw2, G.L2, -1);
-    make_twiddles(tw3, G.L3, -1);
-    __syncthreads();
-    // axis 1, real -> half complex
-    for (int line = threadIdx.x; line < G.L2 * G.L3; line += blockDim.x) {
-        int c2 = line % G.L2, c3 = line / G.L2;
-        T in[kLmax];
-#pragma unroll
-        for (int x = 0; x < kLmax; x++)
-            if (x < G.L1) in[x] = load(x, c2, c3);
-        for (int k = 0; k < G.H1; k++) {
-            T ar = 0, ai = 0;
-            int m = 0;
-#pragma unroll
-            for (int x = 0; x < kLmax; x++) {
-                if (x < G.L1) {
-                    ar += in[x] * tw1[m].re;
-                    ai += in[x] * tw1[m].im;
-                    m += k;
-                    if (m >= G.L1) m -= G.L1;
-                }
-            }
-            spec[k + (long long)G.H1 * (c2 + (long long)G.L2 * c3)] = {ar, ai};
-        }
-    }
+    for (int line = threadIdx.x; line < G.L2 * G.L3; line += blockDim.x)
+        line_r2c<T>(F.fld + line * F.row, F.spec + G.H1 * line, G.L1, F.tw1);
     __syncthreads();
     for (int line = threadIdx.x; line < G.H1 * G.L3; line += blockDim.x) {
-        int k1 = line % G.H1, c3 = line / G.H1;
-        dft_line_c2c(spec, k1 + (long long)G.H1 * G.L2 * c3, G.H1, G.L2, tw2);
+        const int k1 = line % G.H1, c3 = line / G.H1;
+        line_c2c<T>(F.spec, k1 + G.H1 * G.L2 * c3, G.H1, G.L2, F.tw2);
     }
     __syncthreads();
     for (int line = threadIdx.x; line < G.H1 * G.L2; line += blockDim.x)
-        dft_line_c2c(spec, line, (long long)G.H1 * G.L2, G.L3, tw3);
+        line_c2c<T>(F.spec, line, G.H1 * G.L2, G.L3, F.tw3);
     __syncthreads();
 }
 
-// Dynamic LDS: spectrum (nk complex) followed by three twiddle tables.
+// Backward c2r 3D FFT of F.spec into F.fld (unscaled).  Twiddles sign +1.
 template <typename T>
-__device__ inline C2<T>* lds_spec()
+__device__ __forceinline__ void inv3d(const EwaldGeom& G, const FftLds<T>& F)
 {
-    extern __shared__ __align__(16) unsigned char snh_lds[];
-    return reinterpret_cast<C2<T>*>(snh_lds);
+    for (int line = threadIdx.x; line < G.H1 * G.L2; line += blockDim.x)
+        line_c2c<T>(F.spec, line, G.H1 * G.L2, G.L3, F.tw3);
+    __syncthreads();
+    for (int line = threadIdx.x; line < G.H1 * G.L3; line += blockDim.x) {
+        const int k1 = line % G.H1, c3 = line / G.H1;
+        line_c2c<T>(F.spec, k1 + G.H1 * G.L2 * c3, G.H1, G.L2, F.tw2);
+    }
+    __syncthreads();
+    for (int line = threadIdx.x; line < G.L2 * G.L3; line += blockDim.x)
+        line_c2r<T>(F.spec + G.H1 * line, F.fld + line * F.row, G.L1, F.tw1);
+    __syncthreads();
+}
+
+// Position of (replica r, field f, wavevector k) in the blocked spectra.
+__device__ __forceinline__ long long spec_index(const EwaldGeom& G, int R, int r, int f, long long k)
+{
+    return (((k / kSpecKB) * R + r) * (3 * G.na) + f) * kSpecKB + (k % kSpecKB);
 }
 
 // Padded state index of interior cell (c1, c2, c3) of sublattice j.
@@ -123,51 +323,67 @@ __device__ __forceinline__ uint32_t ewald_site(const EwaldGeom& G, uint32_t off,
 
 // Forward transform of mu = -g S.  grid = (3 na, R); field f = 3 j + beta.
 template <typename T>
-__global__ __launch_bounds__(256) void k_ewald_fwd(EwaldGeom G, const ReplicaInfo* __restrict__ reps,
-                                                   uint32_t total, const T* __restrict__ gmat,
-                                                   const T* __restrict__ S, C2<T>* __restrict__ Fmu)
+__global__ __launch_bounds__(kFftThreads) void k_ewald_fwd(EwaldGeom G, int R, const ReplicaInfo* __restrict__ reps,
+                                                           uint32_t total, const T* __restrict__ gmat,
+                                                           const T* __restrict__ S, C2<T>* __restrict__ Fmu)
 {
     const int f = blockIdx.x, r = blockIdx.y;
     const int j = f / 3, beta = f % 3;
-    C2<T>* spec = lds_spec<T>();
-    C2<T>* tw1 = spec + G.nk;
-    C2<T>* tw2 = tw1 + kLmax;
-    C2<T>* tw3 = tw2 + kLmax;
+    const FftLds<T> F = fft_lds<T>(G);
+    make_twiddles(F.tw1, G.L1, -1);
+    make_twiddles(F.tw2, G.L2, -1);
+    make_twiddles(F.tw3, G.L3, -1);
     const uint32_t off = reps[r].off;
     const T g0 = gmat[9 * j + 3 * beta], g1 = gmat[9 * j + 3 * beta + 1], g2 = gmat[9 * j + 3 * beta + 2];
-    fwd3d<T>(G, spec, tw1, tw2, tw3, [&](int c1, int c2, int c3) -> T {
-        const uint32_t g = ewald_site(G, off, j, c1, c2, c3);
-        return -(g0 * ldg(S, g) + g1 * ldg(S + total, g) + g2 * ldg(S + 2 * (size_t)total, g));
-    });
-    C2<T>* out = Fmu + ((long long)r * 3 * G.na + f) * G.nk;
-    for (long long k = threadIdx.x; k < G.nk; k += blockDim.x) out[k] = spec[k];
+    constexpr int kBatch = sizeof(T) == 4 ? 8 : 4;  // loads in flight per thread
+    for (int c0 = 0; c0 < G.ncell; c0 += kBatch * kFftThreads) {
+        T v[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; q++) {
+            const int c = c0 + q * kFftThreads + threadIdx.x;
+            v[q] = 0;
+            if (c < G.ncell) {
+                const int c1 = c % G.L1, l = c / G.L1;
+                const uint32_t g = ewald_site(G, off, j, c1, l % G.L2, l / G.L2);
+                v[q] = -(g0 * ldg(S, g) + g1 * ldg(S + total, g) + g2 * ldg(S + 2 * (size_t)total, g));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kBatch; q++) {
+            const int c = c0 + q * kFftThreads + threadIdx.x;
+            if (c < G.ncell) F.fld[(c / G.L1) * F.row + c % G.L1] = v[q];
+        }
+    }
+    __syncthreads();
+    fwd3d<T>(G, F);
+    for (int k = threadIdx.x; k < G.nk; k += blockDim.x) Fmu[spec_index(G, R, r, f, k)] = F.spec[k];
 }
 
 // Forward transform of the real-space tensor A (FP64), written as the
 // conjugated, column-major contraction blocks M_k[col = 3j+beta][row = 3i+alpha].
 // grid = 9 na^2: field = ((j*na + i)*3 + alpha)*3 + beta.
 template <typename T>
-__global__ __launch_bounds__(256) void k_ewald_fwdA(EwaldGeom G, const double* __restrict__ A,
-                                                    C2<T>* __restrict__ M)
+__global__ __launch_bounds__(kFftThreads) void k_ewald_fwdA(EwaldGeom G, const double* __restrict__ A,
+                                                            C2<T>* __restrict__ M)
 {
     const int f = blockIdx.x;
     const int beta = f % 3, alpha = (f / 3) % 3, i = (f / 9) % G.na, j = f / (9 * G.na);
-    C2<double>* spec = lds_spec<double>();
-    C2<double>* tw1 = spec + G.nk;
-    C2<double>* tw2 = tw1 + kLmax;
-    C2<double>* tw3 = tw2 + kLmax;
+    const FftLds<double> F = fft_lds<double>(G);
+    make_twiddles(F.tw1, G.L1, -1);
+    make_twiddles(F.tw2, G.L2, -1);
+    make_twiddles(F.tw3, G.L3, -1);
     const double* Aij = A + ((long long)j * G.na + i) * G.ncell * 9;
-    fwd3d<double>(G, spec, tw1, tw2, tw3, [&](int c1, int c2, int c3) -> double {
-        long long c = c1 + (long long)G.L1 * (c2 + (long long)G.L2 * c3);
-        return Aij[c * 9 + 3 * alpha + beta];
-    });
+    for (int c = threadIdx.x; c < G.ncell; c += blockDim.x)
+        F.fld[(c / G.L1) * F.row + c % G.L1] = Aij[(long long)c * 9 + 3 * alpha + beta];
+    __syncthreads();
+    fwd3d<double>(G, F);
     const int n3 = 3 * G.na;
     const long long col = 3 * j + beta, row = 3 * i + alpha;
-    for (long long k = threadIdx.x; k < G.nk; k += blockDim.x)
-        M[(k * n3 + col) * n3 + row] = {(T)spec[k].re, (T)(-spec[k].im)};
+    for (int k = threadIdx.x; k < G.nk; k += blockDim.x)
+        M[(k * n3 + col) * n3 + row] = {(T)F.spec[k].re, (T)(-F.spec[k].im)};
 }
 
-// F[phi]_k = M_k F[mu]_k for every replica.  One workgroup per k.
+// F[phi]_k = M_k F[mu]_k for every replica; one workgroup per k (few replicas).
 template <typename T>
 __global__ __launch_bounds__(256) void k_ewald_contract(EwaldGeom G, int R, const C2<T>* __restrict__ M,
                                                         const C2<T>* __restrict__ Fmu, C2<T>* __restrict__ Fphi)
@@ -178,7 +394,7 @@ __global__ __launch_bounds__(256) void k_ewald_contract(EwaldGeom G, int R, cons
     C2<T>* v = reinterpret_cast<C2<T>*>(snh_lds);  // [R][n3]
     for (int t = threadIdx.x; t < R * n3; t += blockDim.x) {
         int r = t / n3, col = t % n3;
-        v[t] = Fmu[((long long)r * n3 + col) * G.nk + k];
+        v[t] = Fmu[spec_index(G, R, r, col, k)];
     }
     __syncthreads();
     const C2<T>* Mk = M + k * n3 * n3;
@@ -192,57 +408,156 @@ __global__ __launch_bounds__(256) void k_ewald_contract(EwaldGeom G, int R, cons
             ar += m.re * x.re - m.im * x.im;
             ai += m.re * x.im + m.im * x.re;
         }
-        Fphi[((long long)r * n3 + row) * G.nk + k] = {ar, ai};
+        Fphi[spec_index(G, R, r, row, k)] = {ar, ai};
     }
 }
 
-// Inverse: c2r 3D DFT of F[phi] scaled by 1/Ncell -> phi plane alpha of
+// The same contraction on the matrix cores for many replicas: a workgroup
+// takes kMfmaKB wavevectors and up to 16 NC replicas, with one wave per
+// (wavevector, tile of 16 output fields): wave w computes rows
+// 16 (w / kMfmaKB) .. +15 of Y = M_k X for k = k0 + w % kMfmaKB as real
+// 16x16x4 MFMA tiles,
+//   Yr = Mr Xr - Mi Xi,  Yi = Mi Xr + Mr Xi,
+// output rows and the reduction (input fields) padded to 16 with zeros.
+// X and Y pass through LDS.  Operand layouts of v_mfma_f32_16x16x4_f32
+// (wave of 64, lane l): A[m][kk]: m = l % 16, kk = l / 16; B[kk][n]:
+// kk = l / 16, n = l % 16; D[m][n] (4 registers v): m = 4 (l / 16) + v,
+// n = l % 16.
+constexpr int kMfmaKB = 4;  // wavevectors per workgroup
+constexpr int kMfmaMaxRowTiles = 4;  // 3 na <= 64
+static_assert(kSpecKB % kMfmaKB == 0, "a contraction block lies inside one spectrum block");
+
+// Workgroup -> block of wavevectors such that the kSpecKB / kMfmaKB blocks
+// sharing 128-byte spectrum runs land on the same XCD (workgroups are
+// dispatched to the 8 XCDs round robin) one after another, and meet in its L2.
+__device__ __forceinline__ long long xcd_block(long long b, long long nb)
+{
+    const long long x = b % 8, i = b / 8, lo = nb / 8, rem = nb % 8;
+    return x * lo + (x < rem ? x : rem) + i;
+}
+
+template <int NRT, int NC>
+__global__ __launch_bounds__(256 * NRT) void k_ewald_contract_mfma(EwaldGeom G, int R,
+                                                                   const C2<float>* __restrict__ M,
+                                                                   const C2<float>* __restrict__ Fmu,
+                                                                   C2<float>* __restrict__ Fphi)
+{
+    constexpr int NT = 256 * NRT;  // threads: kMfmaKB x NRT waves
+    constexpr int RC = 16 * NC;    // replicas per workgroup
+    constexpr int N3R = 16 * NRT;  // rows padded to the tile
+    constexpr int KS = 4 * NRT;    // reduction steps of 4 (covers the padded fields)
+    constexpr int XR = RC + 1;     // LDS row of one (field, k), padded against bank conflicts
+    const int n3 = 3 * G.na;
+    const long long k0 = xcd_block(blockIdx.x, gridDim.x) * kMfmaKB;
+    const int r0 = blockIdx.y * RC;
+    extern __shared__ __align__(16) unsigned char snh_lds[];
+    C2<float>* X = reinterpret_cast<C2<float>*>(snh_lds);  // [N3R][kMfmaKB][XR]
+
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int kk_w = wave % kMfmaKB, tile = wave / kMfmaKB;
+    const long long k = k0 + kk_w;
+    const int lm = lane % 16, lk = lane / 16;
+    const int row = 16 * tile + lm;
+    // this wave's operand A: rows of its tile of M_k, all loads in flight first
+    C2<float> a[KS];
+    {
+        const C2<float>* Mk = M + (k < G.nk ? k : 0) * n3 * n3;
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+            const int col = 4 * s + lk;
+            a[s] = (k < G.nk && row < n3 && col < n3) ? Mk[col * n3 + row] : C2<float>{0.f, 0.f};
+        }
+    }
+    // X block: 4 NC loads per thread, all issued before the first LDS write
+    constexpr int PER = kMfmaKB * N3R * RC / NT;
+    static_assert(PER * NT == kMfmaKB * N3R * RC, "whole loads per thread");
+    C2<float> v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int t = q * NT + threadIdx.x;
+        const int kk = t % kMfmaKB, f = (t / kMfmaKB) % N3R, rr = t / (kMfmaKB * N3R);
+        const int r = r0 + rr;
+        v[q] = (f < n3 && k0 + kk < G.nk && r < R) ? Fmu[spec_index(G, R, r, f, k0 + kk)] : C2<float>{0.f, 0.f};
+    }
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int t = q * NT + threadIdx.x;
+        const int kk = t % kMfmaKB, f = (t / kMfmaKB) % N3R, rr = t / (kMfmaKB * N3R);
+        X[(f * kMfmaKB + kk) * XR + rr] = v[q];
+    }
+    __syncthreads();
+
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    f4 yr[NC], yi[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        yr[c] = f4{0.f, 0.f, 0.f, 0.f};
+        yi[c] = f4{0.f, 0.f, 0.f, 0.f};
+    }
+    const C2<float>* Xk = X + kk_w * XR;
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+        const int col = 4 * s + lk;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            const C2<float> b = Xk[col * kMfmaKB * XR + 16 * c + lm];
+            yr[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].re, b.re, yr[c], 0, 0, 0);
+            yi[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].im, b.re, yi[c], 0, 0, 0);
+            yr[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].im, -b.im, yr[c], 0, 0, 0);
+            yi[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s].re, b.im, yi[c], 0, 0, 0);
+        }
+    }
+    __syncthreads();  // every wave is done reading X: reuse it for Y
+    C2<float>* Yk = X + kk_w * XR;
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int v4 = 0; v4 < 4; v4++)
+            Yk[(16 * tile + 4 * lk + v4) * kMfmaKB * XR + 16 * c + lm] = {yr[c][v4], yi[c][v4]};
+    __syncthreads();
+    const int nst = kMfmaKB * n3 * RC;
+    for (int t = threadIdx.x; t < nst; t += NT) {
+        const int kk = t % kMfmaKB, f = (t / kMfmaKB) % n3, rr = t / (kMfmaKB * n3);
+        const int r = r0 + rr;
+        if (k0 + kk < G.nk && r < R) Fphi[spec_index(G, R, r, f, k0 + kk)] = X[(f * kMfmaKB + kk) * XR + rr];
+    }
+}
+
+// Inverse: c2r 3D FFT of F[phi] scaled by 1/Ncell -> phi plane alpha of
 // sublattice i.  grid = (3 na, R); field = 3 i + alpha.
 template <typename T>
-__global__ __launch_bounds__(256) void k_ewald_inv(EwaldGeom G, const ReplicaInfo* __restrict__ reps, uint32_t total,
-                                                   const C2<T>* __restrict__ Fphi, T* __restrict__ phi)
+__global__ __launch_bounds__(kFftThreads) void k_ewald_inv(EwaldGeom G, int R, const ReplicaInfo* __restrict__ reps,
+                                                           uint32_t total, const C2<T>* __restrict__ Fphi,
+                                                           T* __restrict__ phi)
 {
     const int f = blockIdx.x, r = blockIdx.y;
     const int i = f / 3, alpha = f % 3;
-    C2<T>* spec = lds_spec<T>();
-    C2<T>* tw1 = spec + G.nk;
-    C2<T>* tw2 = tw1 + kLmax;
-    C2<T>* tw3 = tw2 + kLmax;
-    make_twiddles(tw1, G.L1, +1);
-    make_twiddles(tw2, G.L2, +1);
-    make_twiddles(tw3, G.L3, +1);
-    const C2<T>* in = Fphi + ((long long)r * 3 * G.na + f) * G.nk;
-    for (long long k = threadIdx.x; k < G.nk; k += blockDim.x) spec[k] = in[k];
-    __syncthreads();
-    for (int line = threadIdx.x; line < G.H1 * G.L2; line += blockDim.x)
-        dft_line_c2c(spec, line, (long long)G.H1 * G.L2, G.L3, tw3);
-    __syncthreads();
-    for (int line = threadIdx.x; line < G.H1 * G.L3; line += blockDim.x) {
-        int k1 = line % G.H1, c3 = line / G.H1;
-        dft_line_c2c(spec, k1 + (long long)G.H1 * G.L2 * c3, G.H1, G.L2, tw2);
+    const FftLds<T> F = fft_lds<T>(G);
+    make_twiddles(F.tw1, G.L1, +1);
+    make_twiddles(F.tw2, G.L2, +1);
+    make_twiddles(F.tw3, G.L3, +1);
+    constexpr int kBatch = sizeof(T) == 4 ? 8 : 4;  // loads in flight per thread
+    for (int k0 = 0; k0 < G.nk; k0 += kBatch * kFftThreads) {
+        C2<T> v[kBatch];
+#pragma unroll
+        for (int q = 0; q < kBatch; q++) {
+            const int k = k0 + q * kFftThreads + threadIdx.x;
+            if (k < G.nk) v[q] = Fphi[spec_index(G, R, r, f, k)];
+        }
+#pragma unroll
+        for (int q = 0; q < kBatch; q++) {
+            const int k = k0 + q * kFftThreads + threadIdx.x;
+            if (k < G.nk) F.spec[k] = v[q];
+        }
     }
     __syncthreads();
-    // axis 1, half complex -> real, using Hermitian symmetry X[L1-k] = conj(X[k]).
+    inv3d<T>(G, F);
     const T scale = (T)(1.0 / (double)G.ncell);
     T* const out = phi + (size_t)alpha * total;
     const uint32_t off = reps[r].off;
-    const bool even = (G.L1 % 2) == 0;
-    for (int line = threadIdx.x; line < G.L2 * G.L3; line += blockDim.x) {
-        int c2 = line % G.L2, c3 = line / G.L2;
-        const C2<T>* X = spec + (long long)G.H1 * (c2 + (long long)G.L2 * c3);
-        for (int x = 0; x < G.L1; x++) {
-            T acc = X[0].re;
-            int m = x;  // twiddle index k*x mod L1
-            int last = even ? G.H1 - 1 : G.H1;
-            for (int k = 1; k < last; k++) {
-                C2<T> w = tw1[m];
-                acc += 2 * (X[k].re * w.re - X[k].im * w.im);
-                m += x;
-                if (m >= G.L1) m -= G.L1;
-            }
-            if (even && G.H1 > 1) acc += X[G.H1 - 1].re * ((x & 1) ? (T)-1 : (T)1);
-            stg(out, ewald_site(G, off, i, x, c2, c3), acc * scale);
-        }
+    for (int c = threadIdx.x; c < G.ncell; c += blockDim.x) {
+        const int c1 = c % G.L1, l = c / G.L1;
+        stg(out, ewald_site(G, off, i, c1, l % G.L2, l / G.L2), F.fld[l * F.row + c1] * scale);
     }
 }
 
@@ -311,9 +626,37 @@ __global__ __launch_bounds__(256) void k_ewald_A(EwaldAParams P, const double* _
 // ---------------------------------------------------------------------------
 // Host-side launchers
 
-size_t ewald_fft_lds_bytes(long long nk, size_t elem)
+size_t ewald_fft_lds_bytes(const EwaldGeom& G, size_t elem)
 {
-    return (size_t)nk * elem + 3 * kLmax * elem;
+    return ((size_t)G.nk * 2 + 3 * 2 * kLmax + (size_t)G.L2 * G.L3 * (G.L1 + 1)) * elem;
+}
+
+size_t ewald_mfma_lds_bytes(const EwaldGeom& G)
+{
+    return (size_t)kMfmaKB * ((3 * G.na + 15) & ~15) * 33 * sizeof(C2<float>);
+}
+
+size_t ewald_spectrum_bytes(const EwaldGeom& G, int R, size_t elem)
+{
+    const long long nkp = (G.nk + kSpecKB - 1) / kSpecKB * kSpecKB;
+    return (size_t)nkp * R * 3 * G.na * 2 * elem;
+}
+
+bool ewald_use_mfma(const EwaldGeom& G, int R, size_t elem)
+{
+    return elem == sizeof(float) && R >= kEwaldMfmaMinReplicas && 3 * G.na <= 16 * kMfmaMaxRowTiles;
+}
+
+// Raises a kernel's dynamic-LDS limit (once per kernel and size).
+static void ewald_allow_lds(const void* fn, size_t bytes)
+{
+    static std::mutex mu;
+    static std::map<const void*, size_t> done;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t& have = done[fn];
+    if (have >= bytes) return;
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    have = bytes;
 }
 
 void launch_ewald_A(const EwaldAParams& P, const double* pos, double* A, hipStream_t st)
@@ -326,8 +669,9 @@ void launch_ewald_A(const EwaldAParams& P, const double* pos, double* A, hipStre
 template <typename T>
 void launch_ewald_fwdA(const EwaldGeom& G, const double* A, void* M, hipStream_t st)
 {
-    size_t lds = ewald_fft_lds_bytes(G.nk, sizeof(C2<double>));
-    hipLaunchKernelGGL(k_ewald_fwdA<T>, dim3(9 * G.na * G.na), dim3(256), lds, st, G, A,
+    const size_t lds = ewald_fft_lds_bytes(G, sizeof(double));
+    ewald_allow_lds(reinterpret_cast<const void*>(&k_ewald_fwdA<T>), lds);
+    hipLaunchKernelGGL(k_ewald_fwdA<T>, dim3(9 * G.na * G.na), dim3(kFftThreads), lds, st, G, A,
                        reinterpret_cast<C2<T>*>(M));
 }
 
@@ -335,14 +679,55 @@ template <typename T>
 void launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint32_t total, const T* gmat,
                         const T* S, const void* M, void* Fmu, void* Fphi, T* phi, hipStream_t st)
 {
-    size_t lds = ewald_fft_lds_bytes(G.nk, sizeof(C2<T>));
-    hipLaunchKernelGGL(k_ewald_fwd<T>, dim3(3 * G.na, R), dim3(256), lds, st, G, reps, total, gmat, S,
+    const size_t lds = ewald_fft_lds_bytes(G, sizeof(T));
+    ewald_allow_lds(reinterpret_cast<const void*>(&k_ewald_fwd<T>), lds);
+    ewald_allow_lds(reinterpret_cast<const void*>(&k_ewald_inv<T>), lds);
+    hipLaunchKernelGGL(k_ewald_fwd<T>, dim3(3 * G.na, R), dim3(kFftThreads), lds, st, G, R, reps, total, gmat, S,
                        reinterpret_cast<C2<T>*>(Fmu));
-    size_t lds2 = (size_t)R * 3 * G.na * sizeof(C2<T>);
+    if constexpr (sizeof(T) == sizeof(float)) {
+        if (ewald_use_mfma(G, R, sizeof(T))) {
+            const size_t lm = ewald_mfma_lds_bytes(G);
+            // column tiles of 16 replicas per workgroup: 2 reads each M_k once
+            // for up to 32 replicas; SNH_EWALD_NC=1 forces one (tuning)
+            static const int force_nc = [] {
+                const char* e = std::getenv("SNH_EWALD_NC");
+                return e ? std::atoi(e) : 0;
+            }();
+            const int nc = force_nc == 1 || (force_nc != 2 && R <= 16) ? 1 : 2;
+            const dim3 grid((unsigned)((G.nk + kMfmaKB - 1) / kMfmaKB), (unsigned)((R + 16 * nc - 1) / (16 * nc)));
+            const auto* m = reinterpret_cast<const C2<float>*>(M);
+            const auto* x = reinterpret_cast<const C2<float>*>(Fmu);
+            auto* y = reinterpret_cast<C2<float>*>(Fphi);
+            const int nrt = (3 * G.na + 15) / 16;
+#define SNH_EWM(NRT, NC)                                                                               \
+    do {                                                                                               \
+        ewald_allow_lds(reinterpret_cast<const void*>(&k_ewald_contract_mfma<NRT, NC>), lm);           \
+        hipLaunchKernelGGL((k_ewald_contract_mfma<NRT, NC>), grid, dim3(256 * NRT), lm, st, G, R, m, x, y); \
+    } while (0)
+#define SNH_EWM_NC(NRT) \
+    if (nc == 1)        \
+        SNH_EWM(NRT, 1); \
+    else                \
+        SNH_EWM(NRT, 2)
+            switch (nrt) {
+            case 1: SNH_EWM_NC(1); break;
+            case 2: SNH_EWM_NC(2); break;
+            case 3: SNH_EWM_NC(3); break;
+            default: SNH_EWM_NC(4); break;
+            }
+#undef SNH_EWM_NC
+#undef SNH_EWM
+            hipLaunchKernelGGL(k_ewald_inv<T>, dim3(3 * G.na, R), dim3(kFftThreads), lds, st, G, R, reps, total,
+                               reinterpret_cast<const C2<T>*>(Fphi), phi);
+            return;
+        }
+    }
+    const size_t lds2 = (size_t)R * 3 * G.na * sizeof(C2<T>);
+    ewald_allow_lds(reinterpret_cast<const void*>(&k_ewald_contract<T>), lds2);
     hipLaunchKernelGGL(k_ewald_contract<T>, dim3((unsigned)G.nk), dim3(256), lds2, st, G, R,
                        reinterpret_cast<const C2<T>*>(M), reinterpret_cast<const C2<T>*>(Fmu),
                        reinterpret_cast<C2<T>*>(Fphi));
-    hipLaunchKernelGGL(k_ewald_inv<T>, dim3(3 * G.na, R), dim3(256), lds, st, G, reps, total,
+    hipLaunchKernelGGL(k_ewald_inv<T>, dim3(3 * G.na, R), dim3(kFftThreads), lds, st, G, R, reps, total,
                        reinterpret_cast<const C2<T>*>(Fphi), phi);
 }
 

@@ -176,7 +176,14 @@ void launch_adj_lambda(const Couplings<T>& C, int nblocks, const T* nu, const T*
 void launch_adj_accumulate(const double* partial, int n, double* out, int slot, hipStream_t st);
 
 // Ewald (snh_ewald.hip).
-size_t ewald_fft_lds_bytes(long long nk, size_t elem);
+// Dynamic LDS of one field's 3D FFT workgroup (elem = sizeof of the FFT type).
+size_t ewald_fft_lds_bytes(const EwaldGeom& G, size_t elem);
+// The matrix-core contraction serves FP32 batches of at least this many replicas.
+constexpr int kEwaldMfmaMinReplicas = 8;
+bool ewald_use_mfma(const EwaldGeom& G, int R, size_t elem);
+size_t ewald_mfma_lds_bytes(const EwaldGeom& G);
+// Bytes of one set of spectra (all fields of R replicas, blocked wavevectors).
+size_t ewald_spectrum_bytes(const EwaldGeom& G, int R, size_t elem);
 void launch_ewald_A(const EwaldAParams& P, const double* pos, double* A, hipStream_t st);
 template <typename T>
 void launch_ewald_fwdA(const EwaldGeom& G, const double* A, void* M, hipStream_t st);
