@@ -43,19 +43,23 @@ def kernel_time(handle, family):
     return ms.value, n.value
 
 
-def timed_steps(obj, integ, steps, warmup):
-    """Warm up, then time exactly `steps` steps bracketed by synchronisation."""
+def timed_steps(obj, integ, steps, warmup, families=("dipole_predict", "dipole_correct"), kTs=None):
+    """Warm up, then time exactly `steps` steps bracketed by synchronisation
+    (whole chunks of steps replay the library's captured HIP graph).  Then
+    the same number of steps again with HIP events recorded on the library's
+    stream around every kernel (eager launches): per-kernel durations."""
     import sunnyhip as sh
     h = obj._device()
     if warmup:
-        sh.step_(obj, integ, nsteps=warmup)
+        sh.step_(obj, integ, nsteps=warmup, kTs=kTs)
     h.call("snh_synchronize")
-    h.call("snh_enable_timing", 1)
     t0 = time.perf_counter()
-    sh.step_(obj, integ, nsteps=steps)
+    sh.step_(obj, integ, nsteps=steps, kTs=kTs)
     h.call("snh_synchronize")
     t1 = time.perf_counter()
-    fams = {f: kernel_time(h, f) for f in ("dipole_predict", "dipole_correct")}
+    h.call("snh_enable_timing", 1)
+    sh.step_(obj, integ, nsteps=steps, kTs=kTs)
+    fams = {f: kernel_time(h, f) for f in families}
     h.call("snh_enable_timing", 0)
     return (t1 - t0), fams
 
@@ -120,16 +124,7 @@ def bench_sun3(args):
     systems, kTs = sh.workloads.sun3_tempering_batch(seed=0)
     batch = sh.ReplicaBatch(systems, precision=32)
     integ = sh.Langevin(0.02, damping=0.1, kT=0.0)
-    h = batch._device()
-    sh.step_(batch, integ, nsteps=args.warmup, kTs=kTs)
-    h.call("snh_synchronize")
-    h.call("snh_enable_timing", 1)
-    t0 = time.perf_counter()
-    sh.step_(batch, integ, nsteps=args.steps, kTs=kTs)
-    h.call("snh_synchronize")
-    dt = time.perf_counter() - t0
-    fams = {f: kernel_time(h, f) for f in ("sun_predict", "sun_correct")}
-    h.call("snh_enable_timing", 0)
+    dt, fams = timed_steps(batch, integ, args.steps, args.warmup, ("sun_predict", "sun_correct"), kTs)
     ups = batch.nsites * args.steps / dt
     per = {f: ms / max(n, 1) for f, (ms, n) in fams.items()}
     log(f"SU(3) batch: {batch.nsites} sites, {dt * 1e3 / args.steps:.3f} ms/step, {ups:.3e} spin-updates/s, {per}")
@@ -154,16 +149,7 @@ def bench_ewald(args):
                    for r in range(R)]
         obj = systems[0] if R == 1 else sh.ReplicaBatch(systems, precision=32)
         integ = sh.Langevin(0.025, damping=0.2, kT=0.3)
-        h = obj._device()
-        sh.step_(obj, integ, nsteps=args.warmup)
-        h.call("snh_synchronize")
-        h.call("snh_enable_timing", 1)
-        t0 = time.perf_counter()
-        sh.step_(obj, integ, nsteps=args.steps)
-        h.call("snh_synchronize")
-        dt = time.perf_counter() - t0
-        fams = {f: kernel_time(h, f) for f in ("dipole_predict", "dipole_correct", "ewald_field")}
-        h.call("snh_enable_timing", 0)
+        dt, fams = timed_steps(obj, integ, args.steps, args.warmup, ("dipole_predict", "dipole_correct", "ewald_field"))
         ns = sum(s.nsites for s in systems)
         per = {f: ms / max(n, 1) for f, (ms, n) in fams.items()}
         ups = ns * args.steps / dt

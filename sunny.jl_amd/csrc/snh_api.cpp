@@ -346,6 +346,7 @@ struct SystemImpl final : snh_system {
     T* Sd = nullptr;    // expected dipoles of a coherent state (SU(N) + Ewald)
     T* Dz = nullptr;    // SU(N) Langevin without biquadratic bonds: expected dipoles
     T* Dzp = nullptr;   //   of the state and of the predictor stage, with images
+    bool dz_valid = false;  // Dz holds the dipoles of X as the last corrector wrote them
     double* partial = nullptr;
     double* d_out = nullptr;   // [R]
     int* conv_iter = nullptr;  // [R]
@@ -364,6 +365,16 @@ struct SystemImpl final : snh_system {
     T* phi1 = nullptr;
     T* phi2 = nullptr;
     KernelTimer timer;
+    // Captured Langevin steps: a chunk of graph_steps steps with the noise
+    // step counter in device memory, replayed while the launch parameters
+    // (dt, damping, seed, noise mode, neighbour source) stay those of `gkey`.
+    unsigned long long* d_step = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    std::array<double, 5> gkey{};
+    int graph_steps = 8;
+    bool graph_ok = true;
+    bool warmed = false;
 
     template <typename X_>
     X_* own(X_* p)
@@ -375,6 +386,7 @@ struct SystemImpl final : snh_system {
     ~SystemImpl() override
     {
         if (st) (void)hipStreamSynchronize(st);
+        drop_graph();
         timer.clear();
         for (T* p : ckpt) (void)hipFree(p);
         for (void* p : owned) (void)hipFree(p);
@@ -385,6 +397,7 @@ struct SystemImpl final : snh_system {
 
     void build(const snh_spec& sp)
     {
+        graph_steps = std::max(0, env_int("SNH_GRAPH_STEPS", 8));  // 0: no graphs
         R = sp.nreplicas;
         na = sp.na;
         N = sp.N;
@@ -801,6 +814,7 @@ struct SystemImpl final : snh_system {
 
     void set_state(const double* x, bool coherent) override
     {
+        dz_valid = false;
         SNH_REQUIRE(coherent == (N > 0), SNH_ERR_INVALID,
                     coherent ? "coherent states require :SUN mode" : "in :SUN mode set coherent states");
         std::vector<T> h((size_t)planes * total, (T)0);
@@ -822,6 +836,73 @@ struct SystemImpl final : snh_system {
             SNH_HIP(hipGetLastError());
             download_planes(Xp, 3, x);
         }
+    }
+
+    void drop_graph()
+    {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+    }
+
+    // One Heun step: Ewald potentials (if any), predictor, corrector.
+    void langevin_step(const NoiseArgs<T>& nz, T tdt, T tl, bool dip)
+    {
+        const T* ewS = nullptr;
+        const T* ewSp = nullptr;
+        if (ewald) {
+            ewald_field(X, phi1, dip ? Dz : nullptr);
+            ewS = phi1;
+        }
+        hipEvent_t e = tbegin();
+        if (tiled)
+            launch_tile_predict<T>(C, tp.A, tp.threads, X, Xp, Xo, ewS, nz, tdt, tl, st);
+        else if (N == 0)
+            launch_dipole_predict<T>(C, nblocks, X, Xp, Xo, ewS, nz, tdt, tl, st);
+        else
+            launch_sun_predict<T>(C, nblocks, X, Xp, Xo, ewS, nz, tdt, tl, dip ? Dz : nullptr, dip ? Dzp : nullptr,
+                                  st);
+        tend(N == 0 ? "dipole_predict" : "sun_predict", e);
+        if (ewald) {
+            ewald_field(Xp, phi2, dip ? Dzp : nullptr);
+            ewSp = phi2;
+        }
+        e = tbegin();  // the corrector overwrites the state in place (the predictor consumed it)
+        if (tiled)
+            launch_tile_correct<T>(C, tp.A, tp.threads, Xp, Xo, X, ewSp, tdt, tl, st);
+        else if (N == 0)
+            launch_dipole_correct<T>(C, nblocks, Xp, Xo, X, ewSp, tdt, tl, st);
+        else
+            launch_sun_correct<T>(C, nblocks, Xp, Xo, X, ewSp, tdt, tl, dip ? Dzp : nullptr, dip ? Dz : nullptr, st);
+        tend(N == 0 ? "dipole_correct" : "sun_correct", e);
+    }
+
+    // Captures graph_steps Langevin steps (noise step counter on the device);
+    // on any capture failure graphs stay off for this handle.
+    bool capture_langevin(const NoiseArgs<T>& nz, T tdt, T tl, bool dip)
+    {
+        drop_graph();
+        if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            return graph_ok = false;
+        }
+        for (int k = 0; k < graph_steps; k++) {
+            langevin_step(nz, tdt, tl, dip);
+            launch_step_advance(d_step, st);
+        }
+        hipGraph_t g = nullptr;
+        if (hipStreamEndCapture(st, &g) != hipSuccess || !g) {
+            (void)hipGetLastError();
+            return graph_ok = false;
+        }
+        graph = g;
+        if (hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            drop_graph();
+            return graph_ok = false;
+        }
+        return true;
     }
 
     void step_langevin(double dt, double damping, const double* kT, const double* noise, uint64_t seed,
@@ -866,44 +947,45 @@ struct SystemImpl final : snh_system {
                 Dz = own(dalloc<T>(3 * (size_t)total));
                 Dzp = own(dalloc<T>(3 * (size_t)total));
             }
-            launch_sun_expected_spin<T>(C, nblocks, X, Dz, st);
+            // a run of such steps carries the corrector's dipoles from call to
+            // call, so that results do not depend on how steps are split
+            if (!dz_valid) launch_sun_expected_spin<T>(C, nblocks, X, Dz, st);
+            dz_valid = true;
         }
-        for (int s = 0; s < nsteps; s++) {
+        int s = 0;
+        // the first step of a handle runs eagerly (it also sets kernel attributes)
+        if (!warmed && nsteps > 0) {
+            nz.step = step0;
+            langevin_step(nz, tdt, tl, dip);
+            SNH_HIP(hipGetLastError());
+            warmed = true;
+            s = 1;
+        }
+        // whole chunks of graph_steps steps replay a captured graph (device
+        // noise or none, kernel timing off)
+        if (graph_ok && graph_steps > 0 && nz.mode != 1 && !timer.enabled && nsteps - s >= graph_steps) {
+            if (!d_step) d_step = own(dalloc<unsigned long long>(1));
+            const std::array<double, 5> key = {dt, damping, (double)(seed >> 32), (double)(seed & 0xffffffffu),
+                                               (double)(nz.mode + 4 * dip)};
+            NoiseArgs<T> nzg = nz;
+            nzg.step_ptr = d_step;
+            if (graph_exec && key != gkey) drop_graph();
+            if (!graph_exec && capture_langevin(nzg, tdt, tl, dip)) gkey = key;
+            if (graph_exec) {
+                launch_step_set(d_step, step0 + (uint64_t)s, st);
+                for (; nsteps - s >= graph_steps; s += graph_steps) SNH_HIP(hipGraphLaunch(graph_exec, st));
+            }
+        }
+        for (; s < nsteps; s++) {
             nz.step = step0 + (uint64_t)s;
-            const T* ewS = nullptr;
-            const T* ewSp = nullptr;
-            if (ewald) {
-                ewald_field(X, phi1, dip ? Dz : nullptr);
-                ewS = phi1;
-            }
-            hipEvent_t e = tbegin();
-            if (tiled)
-                launch_tile_predict<T>(C, tp.A, tp.threads, X, Xp, Xo, ewS, nz, tdt, tl, st);
-            else if (N == 0)
-                launch_dipole_predict<T>(C, nblocks, X, Xp, Xo, ewS, nz, tdt, tl, st);
-            else
-                launch_sun_predict<T>(C, nblocks, X, Xp, Xo, ewS, nz, tdt, tl, dip ? Dz : nullptr,
-                                      dip ? Dzp : nullptr, st);
-            tend(N == 0 ? "dipole_predict" : "sun_predict", e);
-            if (ewald) {
-                ewald_field(Xp, phi2, dip ? Dzp : nullptr);
-                ewSp = phi2;
-            }
-            e = tbegin();  // the corrector overwrites the state in place (the predictor consumed it)
-            if (tiled)
-                launch_tile_correct<T>(C, tp.A, tp.threads, Xp, Xo, X, ewSp, tdt, tl, st);
-            else if (N == 0)
-                launch_dipole_correct<T>(C, nblocks, Xp, Xo, X, ewSp, tdt, tl, st);
-            else
-                launch_sun_correct<T>(C, nblocks, Xp, Xo, X, ewSp, tdt, tl, dip ? Dzp : nullptr,
-                                      dip ? Dz : nullptr, st);
-            tend(N == 0 ? "dipole_correct" : "sun_correct", e);
+            langevin_step(nz, tdt, tl, dip);
         }
         SNH_HIP(hipGetLastError());
     }
 
     void step_midpoint(double dt, double damping, double tol, int max_iters, int nsteps, int* iters) override
     {
+        dz_valid = false;
         SNH_REQUIRE(std::isfinite(dt) && dt > 0, SNH_ERR_INVALID, "Select positive dt");
         SNH_REQUIRE(damping >= 0, SNH_ERR_INVALID, "Select nonnegative damping");
         SNH_REQUIRE(max_iters >= 1, SNH_ERR_INVALID, "max_iters must be at least 1");
@@ -1015,6 +1097,7 @@ struct SystemImpl final : snh_system {
 
     void midpoint_record(double dt, double tol, int max_iters, int nsteps, int every) override
     {
+        dz_valid = false;
         require_adjoint_support();
         SNH_REQUIRE(nsteps >= 1 && every >= 1, SNH_ERR_INVALID, "nsteps and checkpoint_every must be positive");
         const int ncp = (nsteps + every - 1) / every;
@@ -1258,6 +1341,8 @@ struct SystemImpl final : snh_system {
         if (key == "tile_work") return tiled ? tp.A.nwork : 0;
         if (key == "tile_lds") return tiled ? (int64_t)tp.lds : 0;
         if (key == "gather_chunk") return C.chunk;
+        if (key == "graph_steps") return graph_steps;
+        if (key == "graph_live") return graph_exec ? 1 : 0;
         throw Failure{SNH_ERR_INVALID, "unknown query key: " + key};
     }
 

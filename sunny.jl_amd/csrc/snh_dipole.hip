@@ -233,6 +233,16 @@ __global__ void k_fill_int(int* __restrict__ p, int n, int v)
     if (k < n) p[k] = v;
 }
 
+__global__ void k_step_set(unsigned long long* __restrict__ p, unsigned long long v)
+{
+    if (threadIdx.x == 0) p[0] = v;
+}
+
+__global__ void k_step_advance(unsigned long long* __restrict__ p)
+{
+    if (threadIdx.x == 0) p[0] += 1;
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 
@@ -324,6 +334,16 @@ void launch_midpoint_check(const double* partial, const int* rblk, int R, const 
 void launch_fill_int(int* p, int n, int v, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+}
+
+void launch_step_set(unsigned long long* p, unsigned long long v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_step_set, dim3(1), dim3(64), 0, st, p, v);
+}
+
+void launch_step_advance(unsigned long long* p, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, st, p);
 }
 
 #define SNH_INST(T)                                                                                              \

@@ -45,7 +45,18 @@ struct NoiseArgs {
     const T* scale;        // [R] sqrt(2 dt damping kT_r)
     unsigned long long seed;
     unsigned long long step;
+    // non-null: the step counter lives in device memory (advanced by
+    // k_step_advance after each step), so that a captured graph of several
+    // steps replays with fresh noise
+    const unsigned long long* step_ptr;
 };
+
+// Step number of the noise stream (kernel argument, or the device counter).
+template <typename T>
+__device__ __forceinline__ unsigned long long noise_step(const NoiseArgs<T>& nz)
+{
+    return nz.step_ptr ? ldc(nz.step_ptr, 0) : nz.step;
+}
 
 enum { kBondIso = 1, kBondBiquad = 2, kBondCulled = 4 };
 
@@ -138,6 +149,9 @@ void launch_reduce_partials(const double* partial, const int* rblk, int R, doubl
 void launch_midpoint_check(const double* partial, const int* rblk, int R, const double* atol2, int* conv_iter,
                            int iter, hipStream_t st);
 void launch_fill_int(int* p, int n, int v, hipStream_t st);
+// Device step counter of the noise stream: set, and advance by one.
+void launch_step_set(unsigned long long* p, unsigned long long v, hipStream_t st);
+void launch_step_advance(unsigned long long* p, hipStream_t st);
 void launch_copy_planes(const void* src, void* dst, size_t bytes, hipStream_t st);
 
 // LDS-tiled :dipole kernels (snh_tile.hip); nthreads = workgroup size.

@@ -359,7 +359,7 @@ __device__ __forceinline__ Ket<T, N> site_noise_sun(const NoiseArgs<T>& nz, cons
 #pragma unroll
     for (int blk = 0; blk < (2 * N + 3) / 4; blk++) {
         T n[4];
-        philox_normals4<T>(key, nz.step, site, (uint32_t)blk, n);
+        philox_normals4<T>(key, noise_step(nz), site, (uint32_t)blk, n);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int idx = 4 * blk + q;

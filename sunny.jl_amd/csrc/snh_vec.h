@@ -154,7 +154,7 @@ __device__ __forceinline__ V3<T> dipole_noise(const NoiseArgs<T>& nz, int r, uin
     }
     T n[4];
     const unsigned long long key = nz.seed + (unsigned long long)r * 0x9E3779B97F4A7C15ull;
-    philox_normals4<T>(key, nz.step, site, 0u, n);
+    philox_normals4<T>(key, noise_step(nz), site, 0u, n);
     return {n[0] * s, n[1] * s, n[2] * s};
 }
 
