@@ -239,6 +239,39 @@ def main():
     ups64 = plain.nsites * steps64 / dt64
     log(f"FP64 plain: {dt64 * 1e3 / steps64:.3f} ms/step, {ups64:.3e} spin-updates/s")
 
+    # ImplicitMidpoint (deterministic, damping 0) on the same batch and plain
+    # system: spin-updates/s = spins x steps / time; iterations reported
+    mp = sh.ImplicitMidpoint(0.025, damping=0.0, tol=1e-5)
+    msteps = max(4, args.steps // 5)
+    sh.step_(batch, mp, nsteps=2)
+    hb.call("snh_synchronize")
+    t0 = time.perf_counter()
+    it32 = sh.step_(batch, mp, nsteps=msteps)
+    hb.call("snh_synchronize")
+    dtm32 = time.perf_counter() - t0
+    hb.call("snh_enable_timing", 1)
+    sh.step_(batch, mp, nsteps=msteps)
+    mi_ms, mi_n = kernel_time(hb, "midpoint_iter")
+    hb.call("snh_enable_timing", 0)
+    mp64 = sh.ImplicitMidpoint(0.025, damping=0.0, tol=1e-10)
+    sh.step_(plain, mp64, nsteps=2)
+    hp = plain._device()
+    hp.call("snh_synchronize")
+    t0 = time.perf_counter()
+    it64 = sh.step_(plain, mp64, nsteps=msteps)
+    hp.call("snh_synchronize")
+    dtm64 = time.perf_counter() - t0
+    midpoint = {"fp32_batched": batch.nsites * msteps / dtm32, "fp64_plain": plain.nsites * msteps / dtm64,
+                "unit": "spin-updates/s", "steps": msteps, "iterations_fp32": int(np.max(it32)),
+                "iterations_fp64": int(np.max(it64)), "tol_fp32": 1e-5, "tol_fp64": 1e-10,
+                # one fixed-point sweep reads Shat (gathered), S, S'; writes S'', next Shat
+                "iter_kernel_ms": mi_ms / max(mi_n, 1), "iter_kernel_bytes_per_spin": 60,
+                "iter_kernel_GBs": 60 * batch.nsites / (mi_ms / max(mi_n, 1) * 1e-3) / 1e9}
+    midpoint["speedup"] = midpoint["fp32_batched"] / midpoint["fp64_plain"]
+    log(f"midpoint: FP32 batched {midpoint['fp32_batched']:.3e} ({midpoint['iterations_fp32']} it), "
+        f"FP64 plain {midpoint['fp64_plain']:.3e} ({midpoint['iterations_fp64']} it), "
+        f"iter kernel {midpoint['iter_kernel_ms']:.4f} ms = {midpoint['iter_kernel_GBs']:.0f} GB/s")
+
     # FP32 vs FP64 on the same start and noise: energy deviation after k steps
     a = sh.clone_system(base)
     b = sh.clone_system(base)
@@ -298,6 +331,7 @@ def main():
                        "steps": steps64, "dtype": "f64"},
         "speedup_fp32_batched_vs_fp64_plain": ups32 / ups64,
         "fp32_vs_fp64_after_10_steps": {"rel_energy": abs(Ea - Eb) / abs(Eb), "max_abs_dS": dS},
+        "implicit_midpoint": midpoint,
         "cpu_baseline": None,
     }
     if not args.skip_cpu:

@@ -105,9 +105,10 @@ int snh_get_coherents(snh_system *sys, double *coherents);
  * 3 standard normals per site (:dipole) or N complex normals with variance
  * 1/2 per component (:SUN).  Scaling by sqrt(2 dt damping kT) happens here,
  * as in fill_noise! (src/Integrators.jl:288-298).
- * With device noise (or none) and kernel timing off, runs of whole chunks of
- * steps replay one captured HIP graph per handle (the noise step counter
- * lives in device memory); results are identical to step-by-step calls. */
+ * With SNH_GRAPH_STEPS=n > 0 in the environment at snh_create, device noise
+ * (or none) and kernel timing off, runs of whole chunks of n steps replay
+ * one captured HIP graph per handle (the noise step counter lives in device
+ * memory); results are identical to step-by-step calls. */
 int snh_step_langevin(snh_system *sys, double dt, double damping, const double *kT, const double *noise,
                       uint64_t seed, uint64_t step0, int32_t nsteps);
 
@@ -177,9 +178,9 @@ int snh_copy_calibration(snh_system *sys, int32_t repeats, int64_t *bytes);
  * device plan.  Keys: "layout" (SNH_LAYOUT_*, as chosen), "tile_T1",
  * "tile_T2", "tile_threads", "tile_work" (workgroups per tiled launch),
  * "tile_lds" (bytes of LDS per workgroup), "gather_chunk", "graph_steps"
- * (Langevin steps per captured HIP graph; 0 = graphs off, set by the
- * environment variable SNH_GRAPH_STEPS at snh_create), "graph_live" (1 when
- * a captured graph is instantiated). */
+ * (Langevin steps per captured HIP graph; 0 = graphs off, the default; set
+ * by the environment variable SNH_GRAPH_STEPS at snh_create), "graph_live"
+ * (1 when a captured graph is instantiated). */
 int snh_query(snh_system *sys, const char *key, int64_t *value);
 
 #ifdef __cplusplus

@@ -342,7 +342,8 @@ struct SystemImpl final : snh_system {
     T* Xo = nullptr;    // Heun half-update H (own sites only); scratch planes
     T* B1 = nullptr;    // midpoint buffers (lazy)
     T* B2 = nullptr;
-    T* Bh = nullptr;    // materialised midpoint state (Ewald only)
+    T* Bh = nullptr;    // materialised midpoint state Shat (Ewald, or tiled)
+    T* Bh2 = nullptr;   // tiled: the next iteration's Shat
     T* Sd = nullptr;    // expected dipoles of a coherent state (SU(N) + Ewald)
     T* Dz = nullptr;    // SU(N) Langevin without biquadratic bonds: expected dipoles
     T* Dzp = nullptr;   //   of the state and of the predictor stage, with images
@@ -372,7 +373,7 @@ struct SystemImpl final : snh_system {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     std::array<double, 5> gkey{};
-    int graph_steps = 8;
+    int graph_steps = 0;
     bool graph_ok = true;
     bool warmed = false;
 
@@ -397,7 +398,9 @@ struct SystemImpl final : snh_system {
 
     void build(const snh_spec& sp)
     {
-        graph_steps = std::max(0, env_int("SNH_GRAPH_STEPS", 8));  // 0: no graphs
+        // graph replays measured slower than eager launches on ROCm 7.2 (about
+        // 3.7 us more per kernel node): off unless SNH_GRAPH_STEPS asks
+        graph_steps = std::max(0, env_int("SNH_GRAPH_STEPS", 0));
         R = sp.nreplicas;
         na = sp.na;
         N = sp.N;
@@ -992,7 +995,8 @@ struct SystemImpl final : snh_system {
         if (!B1) {
             B1 = own(dalloc<T>((size_t)planes * total));
             B2 = own(dalloc<T>((size_t)planes * total));
-            if (ewald) Bh = own(dalloc<T>((size_t)planes * total));
+            if (ewald || tiled) Bh = own(dalloc<T>((size_t)planes * total));
+            if (tiled) Bh2 = own(dalloc<T>((size_t)planes * total));
         }
         // atol = tol * sqrt(length(S)), length counting sites (Integrators.jl:383)
         std::vector<double> a2(R);
@@ -1018,7 +1022,19 @@ struct SystemImpl final : snh_system {
                     T* S2 = bufs[it % 2];
                     const T* phi = nullptr;
                     const T* hat = nullptr;
-                    if (ewald) {
+                    T* hat_next = nullptr;
+                    if (tiled) {
+                        // Shat of iteration it lives in hats[it % 2]: formed from S and
+                        // S1 = S at it = 1, then written by the previous iteration
+                        T* hats[2] = {Bh2, Bh};
+                        if (it == 1) launch_dipole_midpoint_hat<T>(C, nblocks, X, S1, hats[1], conv_iter, st);
+                        hat = hats[it % 2];
+                        hat_next = hats[(it + 1) % 2];
+                        if (ewald) {
+                            ewald_field(hat, phi1);
+                            phi = phi1;
+                        }
+                    } else if (ewald) {
                         if (N == 0) {
                             launch_dipole_midpoint_hat<T>(C, nblocks, X, S1, Bh, conv_iter, st);
                             hat = Bh;
@@ -1031,7 +1047,7 @@ struct SystemImpl final : snh_system {
                     hipEvent_t e = tbegin();
                     if (tiled)
                         launch_tile_midpoint_iter<T>(C, tp.A, tp.threads, X, S1, S2, hat, phi, tdt, tl, partial,
-                                                     conv_iter, st);
+                                                     conv_iter, hat_next, st);
                     else if (N == 0)
                         launch_dipole_midpoint_iter<T>(C, nblocks, X, S1, S2, hat, phi, tdt, tl, partial, conv_iter,
                                                        st);

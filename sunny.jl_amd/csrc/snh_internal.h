@@ -163,9 +163,12 @@ template <typename T>
 void launch_tile_correct(const Couplings<T>& C, const TileArgs& A, int nthreads, const T* Sp, const T* H, T* Sout,
                          const T* phi, T dt, T lambda, hipStream_t st);
 template <typename T>
+// Shat non-null: the ring gathers these planes; null: it forms
+// kappa n((S + S1)/2) at load.  hat_out non-null: the next iteration's Shat
+// is written for every site.
 void launch_tile_midpoint_iter(const Couplings<T>& C, const TileArgs& A, int nthreads, const T* S, const T* S1,
                                T* S2, const T* Shat, const T* phi, T dt, T lambda, double* partial,
-                               const int* conv_iter, hipStream_t st);
+                               const int* conv_iter, T* hat_out, hipStream_t st);
 template <typename T>
 void launch_tile_grad(const Couplings<T>& C, const TileArgs& A, int nthreads, const T* S, const T* phi, T* dE,
                       hipStream_t st);

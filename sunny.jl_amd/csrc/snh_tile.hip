@@ -336,7 +336,8 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
                                                                          const T* __restrict__ Shat,
                                                                          const T* __restrict__ phi, T dt, T lambda,
                                                                          double* __restrict__ partial,
-                                                                         const int* __restrict__ conv_iter)
+                                                                         const int* __restrict__ conv_iter,
+                                                                         T* __restrict__ hat_out)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wg = logical_block();
@@ -362,6 +363,11 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
         const V3<T> d = rhs_det(hat, f, dt, lambda);
         const V3<T> o = {in.a.x + d.x, in.a.y + d.y, in.a.z + d.z};
         plane_store(S2, C.total, g, o);
+        // the next iteration's Shat = kappa n((S + S'')/2) at this site
+        if (hat_out)
+            plane_store(hat_out, C.total, g,
+                        normalize_dipole(V3<T>{(in.a.x + o.x) / 2, (in.a.y + o.y) / 2, (in.a.z + o.z) / 2},
+                                         ldc(C.kappa, i)));
         const double ex = (double)in.b.x - (double)o.x, ey = (double)in.b.y - (double)o.y,
                      ez = (double)in.b.z - (double)o.z;
         d2 += ex * ex + ey * ey + ez * ez;
@@ -562,14 +568,14 @@ void launch_tile_correct(const Couplings<T>& C, const TileArgs& A, int nthreads,
 template <typename T>
 void launch_tile_midpoint_iter(const Couplings<T>& C, const TileArgs& A, int nthreads, const T* S, const T* S1,
                                T* S2, const T* Shat, const T* phi, T dt, T lambda, double* partial,
-                               const int* conv_iter, hipStream_t st)
+                               const int* conv_iter, T* hat_out, hipStream_t st)
 {
     const size_t lds = tile_lds_bytes(A, sizeof(T));
 #define SNH_TM(HAT, SP)                                                                                        \
     do {                                                                                                       \
         launch_tile(k_tile_mid_iter<T, 6, HAT, SP>, A, nthreads, lds, st);                                     \
         hipLaunchKernelGGL((k_tile_mid_iter<T, 6, HAT, SP>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, S, S1, \
-                           S2, Shat, phi, dt, lambda, partial, conv_iter);                                     \
+                           S2, Shat, phi, dt, lambda, partial, conv_iter, hat_out);                            \
     } while (0)
     if (Shat) {
         if (A.sp == 2) SNH_TM(true, 2); else SNH_TM(true, 1);
@@ -613,7 +619,7 @@ void launch_tile_energy(const Couplings<T>& C, const TileArgs& A, int nthreads, 
     template void launch_tile_correct<T>(const Couplings<T>&, const TileArgs&, int, const T*, const T*, T*,       \
                                          const T*, T, T, hipStream_t);                                           \
     template void launch_tile_midpoint_iter<T>(const Couplings<T>&, const TileArgs&, int, const T*, const T*, T*, \
-                                               const T*, const T*, T, T, double*, const int*, hipStream_t);      \
+                                               const T*, const T*, T, T, double*, const int*, T*, hipStream_t);  \
     template void launch_tile_grad<T>(const Couplings<T>&, const TileArgs&, int, const T*, const T*, T*,          \
                                       hipStream_t);                                                              \
     template void launch_tile_energy<T>(const Couplings<T>&, const TileArgs&, int, const T*, const T*, double*,   \

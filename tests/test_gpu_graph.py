@@ -2,6 +2,7 @@
 launches: a handle stepped in long calls (graph chunks) and a twin stepped
 one call per step (eager launches) stay bitwise equal, across replays
 separated by other work on the stream (energy evaluations)."""
+
 import numpy as np
 import pytest
 
@@ -38,9 +39,15 @@ def state(obj):
 
 
 @pytest.mark.parametrize("kind", ["tile_fp32_batch", "ghost_fp64", "sun3_batch", "ewald_fp32"])
-def test_graph_replays_match_eager(kind):
+def test_graph_replays_match_eager(kind, monkeypatch):
     announce(f"graph vs eager {kind}")
-    a, b = twins(kind)
+    # graphs are opt-in, read at handle creation
+    monkeypatch.setenv("SNH_GRAPH_STEPS", "8")
+    a = twins(kind)[0]
+    a._device()
+    monkeypatch.setenv("SNH_GRAPH_STEPS", "0")
+    b = twins(kind)[1]
+    b._device()
     R = getattr(a, "nreplicas", 1)
     kTs = np.linspace(0.2, 0.9, R)
     integ = sh.Langevin(0.02, damping=0.1, kT=0.5)
@@ -55,4 +62,4 @@ def test_graph_replays_match_eager(kind):
     h = a._device()
     assert h.query("graph_steps") == 8
     assert h.query("graph_live") == 1
-    assert b._device().query("graph_live") == 0
+    assert b._device().query("graph_steps") == 0 and b._device().query("graph_live") == 0
