@@ -342,8 +342,8 @@ struct SystemImpl final : snh_system {
     T* Xo = nullptr;    // Heun half-update H (own sites only); scratch planes
     T* B1 = nullptr;    // midpoint buffers (lazy)
     T* B2 = nullptr;
-    T* Bh = nullptr;    // materialised midpoint state Shat (Ewald, or tiled)
-    T* Bh2 = nullptr;   // tiled: the next iteration's Shat
+    T* Bh = nullptr;    // materialised midpoint state: Shat (:dipole) or Zbar (SU(N) + Ewald)
+    T* Bh2 = nullptr;   // :dipole: the next iteration's Shat
     T* Sd = nullptr;    // expected dipoles of a coherent state (SU(N) + Ewald)
     T* Dz = nullptr;    // SU(N) Langevin without biquadratic bonds: expected dipoles
     T* Dzp = nullptr;   //   of the state and of the predictor stage, with images
@@ -995,8 +995,8 @@ struct SystemImpl final : snh_system {
         if (!B1) {
             B1 = own(dalloc<T>((size_t)planes * total));
             B2 = own(dalloc<T>((size_t)planes * total));
-            if (ewald || tiled) Bh = own(dalloc<T>((size_t)planes * total));
-            if (tiled) Bh2 = own(dalloc<T>((size_t)planes * total));
+            if (ewald || N == 0) Bh = own(dalloc<T>((size_t)planes * total));
+            if (N == 0) Bh2 = own(dalloc<T>((size_t)planes * total));
         }
         // atol = tol * sqrt(length(S)), length counting sites (Integrators.jl:383)
         std::vector<double> a2(R);
@@ -1023,7 +1023,7 @@ struct SystemImpl final : snh_system {
                     const T* phi = nullptr;
                     const T* hat = nullptr;
                     T* hat_next = nullptr;
-                    if (tiled) {
+                    if (N == 0) {
                         // Shat of iteration it lives in hats[it % 2]: formed from S and
                         // S1 = S at it = 1, then written by the previous iteration
                         T* hats[2] = {Bh2, Bh};
@@ -1035,12 +1035,7 @@ struct SystemImpl final : snh_system {
                             phi = phi1;
                         }
                     } else if (ewald) {
-                        if (N == 0) {
-                            launch_dipole_midpoint_hat<T>(C, nblocks, X, S1, Bh, conv_iter, st);
-                            hat = Bh;
-                        } else {
-                            launch_sun_midpoint_bar<T>(C, nblocks, X, S1, Bh, conv_iter, st);
-                        }
+                        launch_sun_midpoint_bar<T>(C, nblocks, X, S1, Bh, conv_iter, st);
                         ewald_field(Bh, phi1);
                         phi = phi1;
                     }
@@ -1050,7 +1045,7 @@ struct SystemImpl final : snh_system {
                                                      conv_iter, hat_next, st);
                     else if (N == 0)
                         launch_dipole_midpoint_iter<T>(C, nblocks, X, S1, S2, hat, phi, tdt, tl, partial, conv_iter,
-                                                       st);
+                                                       hat_next, st);
                     else
                         launch_sun_midpoint_iter<T>(C, nblocks, X, S1, S2, phi, tdt, tl, partial, conv_iter, st);
                     tend("midpoint_iter", e);

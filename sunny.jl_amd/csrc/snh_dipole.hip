@@ -79,12 +79,16 @@ __global__ __launch_bounds__(kBlock) void k_dipole_mid_hat(Couplings<T> C, const
 // One fixed-point sweep of the spherical midpoint method:
 //   Shat = kappa*normalize((S + S1)/2);  S2 = S + rhs(Shat, 0, dE(Shat)),
 // plus the FP64 residual sum |S1 - S2|^2 per block (fast_isapprox).
+// hat_out non-null (with kHatGiven): the next iteration's Shat =
+// kappa*normalize((S + S2)/2) is written with its ghost images, and S2 is
+// only read at its own site afterwards (no images).
 template <typename T, int CH, bool kHatGiven>
 __global__ __launch_bounds__(kBlock) void k_dipole_mid_iter(Couplings<T> C, const T* __restrict__ S,
                                                             const T* __restrict__ S1, T* __restrict__ S2,
                                                             const T* __restrict__ Shat, const T* __restrict__ phi,
                                                             T dt, T lambda, double* __restrict__ partial,
-                                                            const int* __restrict__ conv_iter)
+                                                            const int* __restrict__ conv_iter,
+                                                            T* __restrict__ hat_out)
 {
     const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
     if (conv_iter[sc.r] != 0) return;  // block-uniform: the whole block belongs to replica r
@@ -104,7 +108,13 @@ __global__ __launch_bounds__(kBlock) void k_dipole_mid_iter(Couplings<T> C, cons
         const V3<T> d = rhs_det(h, g, dt, lambda);
         const V3<T> s = planes(S, C.total)(sc.g);
         const V3<T> o = {s.x + d.x, s.y + d.y, s.z + d.z};
-        store3_images(S2, C.total, sc, C.G, o);
+        if (hat_out) {
+            store3(S2, C.total, sc.g, o);
+            store3_images(hat_out, C.total, sc, C.G,
+                          normalize_dipole(V3<T>{(s.x + o.x) / 2, (s.y + o.y) / 2, (s.z + o.z) / 2}, ldc(C.kappa, sc.i)));
+        } else {
+            store3_images(S2, C.total, sc, C.G, o);
+        }
         const V3<T> p = planes(S1, C.total)(sc.g);
         const double ex = (double)p.x - (double)o.x, ey = (double)p.y - (double)o.y, ez = (double)p.z - (double)o.z;
         d2 = ex * ex + ey * ey + ez * ez;
@@ -290,14 +300,14 @@ void launch_dipole_midpoint_hat(const Couplings<T>& C, int nblocks, const T* S, 
 template <typename T>
 void launch_dipole_midpoint_iter(const Couplings<T>& C, int nblocks, const T* S, const T* S1, T* S2,
                                  const T* Shat, const T* phi, T dt, T lambda, double* partial, const int* conv_iter,
-                                 hipStream_t st)
+                                 T* hat_out, hipStream_t st)
 {
     if (Shat)
         hipLaunchKernelGGL((k_dipole_mid_iter<T, 6, true>), dim3(nblocks), dim3(kBlock), 0, st, C, S, S1, S2, Shat,
-                           phi, dt, lambda, partial, conv_iter);
+                           phi, dt, lambda, partial, conv_iter, hat_out);
     else
         hipLaunchKernelGGL((k_dipole_mid_iter<T, 6, false>), dim3(nblocks), dim3(kBlock), 0, st, C, S, S1, S2, Shat,
-                           phi, dt, lambda, partial, conv_iter);
+                           phi, dt, lambda, partial, conv_iter, (T*)nullptr);
 }
 
 template <typename T>
@@ -354,7 +364,7 @@ void launch_step_advance(unsigned long long* p, hipStream_t st)
     template void launch_dipole_midpoint_hat<T>(const Couplings<T>&, int, const T*, const T*, T*, const int*,    \
                                                 hipStream_t);                                                    \
     template void launch_dipole_midpoint_iter<T>(const Couplings<T>&, int, const T*, const T*, T*, const T*,     \
-                                                 const T*, T, T, double*, const int*, hipStream_t);              \
+                                                 const T*, T, T, double*, const int*, T*, hipStream_t);          \
     template void launch_dipole_midpoint_finalize<T>(const Couplings<T>&, int, T*, const T*, const T*,           \
                                                      const int*, int, hipStream_t);                              \
     template void launch_dipole_grad<T>(const Couplings<T>&, int, const T*, const T*, T*, hipStream_t);          \

@@ -134,9 +134,11 @@ template <typename T>
 void launch_dipole_midpoint_hat(const Couplings<T>& C, int nblocks, const T* S, const T* S1, T* Shat,
                                 const int* conv_iter, hipStream_t st);
 template <typename T>
+// Shat non-null: gathered with its images (hat_out, when non-null, receives
+// the next iteration's Shat with images); null: formed from S and S1.
 void launch_dipole_midpoint_iter(const Couplings<T>& C, int nblocks, const T* S, const T* S1, T* S2,
                                  const T* Shat, const T* phi, T dt, T lambda, double* partial,
-                                 const int* conv_iter, hipStream_t st);
+                                 const int* conv_iter, T* hat_out, hipStream_t st);
 template <typename T>
 void launch_dipole_midpoint_finalize(const Couplings<T>& C, int nblocks, T* S, const T* Sa, const T* Sb,
                                      const int* conv_iter, int iter, hipStream_t st);
