@@ -175,6 +175,7 @@ def main():
                     help="device gather layout (auto: LDS tiles when they fit)")
     ap.add_argument("--fp32-only", action="store_true", help="time only the FP32 batch (tuning runs)")
     ap.add_argument("--fp64-only", action="store_true", help="time only the plain FP64 step (tuning runs)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the cost model's first tile plan")
     ap.add_argument("--kT", type=float, default=0.3, help="temperature of the timed runs (tuning: 0 disables noise)")
     ap.add_argument("--workload", choices=["pyrochlore", "sun3", "ewald"], default="pyrochlore",
                     help="pyrochlore: the headline batch; sun3: 64 uneven SU(3) tempering replicas; "
@@ -215,8 +216,12 @@ def main():
         sh.randomize_spins_(reps[r])
     batch = sh.ReplicaBatch(reps, precision=32, layout=layout)
     hb = batch._device()
+    if not args.no_autotune:
+        tune_ms = hb.autotune(candidates=8, launches=3)
+        log(f"autotune: kept plan rank {hb.query('tile_tuned')} of {hb.query('tile_candidates')}, "
+            f"{tune_ms:.4f} ms per predictor + corrector pair")
     plan = {k: hb.query(k) for k in ("layout", "tile_sp", "tile_T1", "tile_T2", "tile_threads", "tile_work",
-                                     "tile_lds")}
+                                     "tile_lds", "tile_tuned")}
     log(f"FP32 batch: {R} replicas, {batch.nsites} spins, plan {plan}")
     dt32, fams32 = timed_steps(batch, integ, args.steps, args.warmup)
     ups32 = batch.nsites * args.steps / dt32

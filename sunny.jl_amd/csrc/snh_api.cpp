@@ -118,9 +118,31 @@ int env_int(const char* name, int dflt)
     return (v && *v) ? std::atoi(v) : dflt;
 }
 
-bool plan_tiles(int na, const int w[3], const std::vector<std::array<int, 3>>& dims, size_t elem, size_t lds_wg,
-                size_t lds_cu, int ncu, TilePlan& P)
+// Work list of a plan whose A.nwork holds its segment count along axis 3.
+void tile_work_list(TilePlan& P, const std::vector<std::array<int, 3>>& dims)
 {
+    const int nseg = P.A.nwork;
+    P.work.clear();
+    P.rwork.assign(dims.size() + 1, 0);
+    for (size_t r = 0; r < dims.size(); r++) {
+        P.rwork[r] = (int)P.work.size();
+        const int L3 = dims[r][2];
+        const int ns = std::min(nseg, L3);
+        for (int sgm = 0; sgm < ns; sgm++) {
+            const int k0 = (int)((long long)L3 * sgm / ns), k1 = (int)((long long)L3 * (sgm + 1) / ns);
+            for (int t2 = 0; t2 < dims[r][1]; t2 += P.A.T2)
+                for (int t1 = 0; t1 < dims[r][0]; t1 += P.A.T1) P.work.push_back({(int)r, t1, t2, k0, k1, {0, 0, 0}});
+        }
+    }
+    P.rwork[dims.size()] = (int)P.work.size();
+    P.A.nwork = (int)P.work.size();
+}
+
+// Tile plans of the cost model, best first (at most `keep`, work lists built).
+bool plan_tiles(int na, const int w[3], const std::vector<std::array<int, 3>>& dims, size_t elem, size_t lds_wg,
+                size_t lds_cu, int ncu, std::vector<TilePlan>& ranked, size_t keep)
+{
+    std::vector<std::pair<double, TilePlan>> cands;
     auto pow2 = [](int v) {
         int p = 1;
         while (p < v) p <<= 1;
@@ -136,8 +158,6 @@ bool plan_tiles(int na, const int w[3], const std::vector<std::array<int, 3>>& d
     // overrides for tuning and tests: SNH_TILE_T1, SNH_TILE_T2, SNH_TILE_NSEG
     const int want_T1 = env_int("SNH_TILE_T1", 0), want_T2 = env_int("SNH_TILE_T2", 0),
               want_seg = env_int("SNH_TILE_NSEG", 0), want_sp = env_int("SNH_TILE_SP", 0);
-    double best = 1e300;
-    bool found = false;
     for (int sp = 1; sp <= 2; sp++)
     for (int T1 = std::min(64, pow2(L1max)); T1 >= 1; T1 >>= 1)
         for (int T2 = 1; T2 <= std::min(64, pow2(L2max)); T2 <<= 1) {
@@ -183,11 +203,9 @@ bool plan_tiles(int na, const int w[3], const std::vector<std::array<int, 3>>& d
                 const double bubble = per_cu >= 2 ? 1.0 : 1.3;
                 const double cost = (double)rounds * bubble *
                                     ((double)(kseg + 2 * w[2]) * npos / eff_ld + 3.0 * kseg * nsite / eff_st);
-                if (cost < best) {
-                    best = cost;
-                    found = true;
+                {
+                    TilePlan P;
                     TileArgs& A = P.A;
-                    A = TileArgs{};
                     A.T1 = T1;
                     A.T2 = T2;
                     A.lt1 = 0;
@@ -205,25 +223,20 @@ bool plan_tiles(int na, const int w[3], const std::vector<std::array<int, 3>>& d
                     P.threads = (int)thr;
                     P.lds = bytes;
                     A.nwork = nseg;  // segments per column until the work list is built
+                    cands.push_back({cost, std::move(P)});
                 }
             }
         }
-    if (!found) return false;
-    const int nseg = P.A.nwork;
-    P.work.clear();
-    P.rwork.assign(dims.size() + 1, 0);
-    for (size_t r = 0; r < dims.size(); r++) {
-        P.rwork[r] = (int)P.work.size();
-        const int L3 = dims[r][2];
-        const int ns = std::min(nseg, L3);
-        for (int sgm = 0; sgm < ns; sgm++) {
-            const int k0 = (int)((long long)L3 * sgm / ns), k1 = (int)((long long)L3 * (sgm + 1) / ns);
-            for (int t2 = 0; t2 < dims[r][1]; t2 += P.A.T2)
-                for (int t1 = 0; t1 < dims[r][0]; t1 += P.A.T1) P.work.push_back({(int)r, t1, t2, k0, k1, {0, 0, 0}});
-        }
+    if (cands.empty()) return false;
+    std::stable_sort(cands.begin(), cands.end(),
+                     [](const std::pair<double, TilePlan>& a, const std::pair<double, TilePlan>& b) {
+                         return a.first < b.first;
+                     });
+    ranked.clear();
+    for (size_t c = 0; c < cands.size() && c < keep; c++) {
+        ranked.push_back(std::move(cands[c].second));
+        tile_work_list(ranked.back(), dims);
     }
-    P.rwork[dims.size()] = (int)P.work.size();
-    P.A.nwork = (int)P.work.size();
     return true;
 }
 
@@ -301,6 +314,7 @@ struct snh_system {
     virtual void kernel_time(const char* fam, double* ms, int64_t* n) = 0;
     virtual void copy_calibration(int repeats, int64_t* bytes) = 0;
     virtual int64_t query(const std::string& key) const = 0;
+    virtual void autotune(int candidates, int launches, double* best_ms) = 0;
     virtual void midpoint_record(double dt, double tol, int max_iters, int nsteps, int every) = 0;
     virtual void midpoint_adjoint(int nparams, const snh_spec* params, const double* dLdS, double* grad,
                                   double* dLdS0) = 0;
@@ -320,6 +334,10 @@ struct SystemImpl final : snh_system {
     // LDS-tiled gather (unpadded layout) when it fits; else ghost-padded gather
     bool tiled = false;
     TilePlan tp;
+    std::vector<TilePlan> tile_cands;  // the cost model's best plans, best first
+    std::vector<void*> plan_mem;       // device tables of the plan in use
+    size_t partial_cap = 0;            // entries of `partial`
+    int tuned_index = -1;              // candidate kept by autotune (-1: the cost model's first)
     int* d_trblk = nullptr;
     // bond tables as given (for the adjoint's per-parameter unit couplings)
     std::vector<int> bsrc_h;      // padded slot -> original bond, -1 for padding
@@ -433,7 +451,8 @@ struct SystemImpl final : snh_system {
                                                                          : prop.sharedMemPerBlock;
             // 256 bytes stay free for the kernels' static LDS (block reductions)
             tiled = plan_tiles(na, w, dv, sizeof(T), prop.sharedMemPerBlock - 256, lds_cu,
-                               std::max(1, prop.multiProcessorCount), tp);
+                               std::max(1, prop.multiProcessorCount), tile_cands, 16);
+            if (tiled) tp = tile_cands[0];
         }
         // the tiled sweep fetches halos with the periodic wrap itself: no ghosts
         const int lw[3] = {tiled ? 0 : w[0], tiled ? 0 : w[1], tiled ? 0 : w[2]};
@@ -563,8 +582,7 @@ struct SystemImpl final : snh_system {
                 bdelta[d * nbp + k] = (int)v;
             }
         }
-        std::vector<int> ldsoff(1, 0);
-        if (tiled) ldsoff = tile_lds_offsets(tp.A, na, sp.bond, bsrc, sizeof(T));
+
         // per-sublattice tables
         std::vector<T> kap(na), zee(3 * na), stev(28 * na), gm(9 * na), fld(3 * na), q2(9 * na, (T)0);
         std::vector<int> kmax(na, 0);
@@ -633,12 +651,11 @@ struct SystemImpl final : snh_system {
         Xp = own(dalloc<T>((size_t)planes * total));
         Xo = own(dalloc<T>((size_t)planes * total));
         SNH_HIP(hipMemset(X, 0, sizeof(T) * planes * (size_t)total));
-        if (tiled) {
-            tp.A.work = own(dupload(tp.work));
-            tp.A.ldsoff = own(dupload(ldsoff));
-            d_trblk = own(dupload(tp.rwork));
+        if (tiled) apply_plan(tile_cands[0]);
+        if (!partial) {
+            partial_cap = std::max<size_t>(nblocks, 1);
+            partial = own(dalloc<double>(partial_cap));
         }
-        partial = own(dalloc<double>(std::max<size_t>(nblocks, tp.work.size())));
         d_out = own(dalloc<double>(R));
         conv_iter = own(dalloc<int>(R));
         atol2 = own(dalloc<double>(R));
@@ -841,6 +858,94 @@ struct SystemImpl final : snh_system {
         }
     }
 
+    // The midpoint's buffers, each allocated on first need: S'' ping-pong B1/B2,
+    // and Shat (:dipole, ping-pong Bh/Bh2) or Zbar (SU(N) + Ewald) in Bh.
+    void ensure_midpoint_buffers()
+    {
+        const size_t n = (size_t)planes * total;
+        if (!B1) B1 = own(dalloc<T>(n));
+        if (!B2) B2 = own(dalloc<T>(n));
+        if ((ewald || N == 0) && !Bh) Bh = own(dalloc<T>(n));
+        if (N == 0 && !Bh2) Bh2 = own(dalloc<T>(n));
+        SNH_REQUIRE(B1 && B2 && (Bh || (!ewald && N != 0)) && (Bh2 || N != 0), SNH_ERR_HIP,
+                    "midpoint buffers missing");
+    }
+
+    // Makes P the tiled plan in use: device work list, LDS bond offsets per
+    // ring phase, per-replica workgroup ranges, and room for its partial sums.
+    void apply_plan(const TilePlan& P)
+    {
+        if (st) SNH_HIP(hipStreamSynchronize(st));
+        for (void* p : plan_mem) (void)hipFree(p);
+        plan_mem.clear();
+        tp = P;
+        const std::vector<int> ldsoff = tile_lds_offsets(tp.A, na, bond_h.data(), bsrc_h, sizeof(T));
+        TileWork* w = dupload(tp.work);
+        int* o = dupload(ldsoff);
+        int* rb = dupload(tp.rwork);
+        plan_mem = {w, o, rb};
+        tp.A.work = w;
+        tp.A.ldsoff = o;
+        d_trblk = rb;
+        const size_t need = std::max<size_t>(nblocks, tp.work.size());
+        if (need > partial_cap) {
+            if (partial) {
+                owned.erase(std::remove(owned.begin(), owned.end(), (void*)partial), owned.end());
+                (void)hipFree(partial);
+            }
+            partial = own(dalloc<double>(need));
+            partial_cap = need;
+        }
+        drop_graph();  // captured launches carry the old plan
+    }
+
+    // Times up to `candidates` of the cost model's plans on this handle (one
+    // warm-up, then `launches` Heun predictor + corrector pairs each, the
+    // corrector writing into a scratch buffer so the state is untouched) and
+    // keeps the fastest.
+    void autotune(int candidates, int launches, double* best_ms) override
+    {
+        SNH_REQUIRE(candidates >= 1 && launches >= 1, SNH_ERR_INVALID, "candidates and launches must be positive");
+        if (!tiled) {
+            if (best_ms) *best_ms = 0;
+            return;
+        }
+        ensure_midpoint_buffers();  // B1 is the corrector's scratch output here
+        NoiseArgs<T> nz{};
+        nz.mode = 2;  // device noise: the predictor's full work
+        nz.scale = nscale;
+        const T dt = (T)0.01, lam = (T)0.1;
+        hipEvent_t e0, e1;
+        SNH_HIP(hipEventCreate(&e0));
+        SNH_HIP(hipEventCreate(&e1));
+        const int nc = std::min<int>(candidates, (int)tile_cands.size());
+        int best = 0;
+        float best_t = 0;
+        for (int c = 0; c < nc; c++) {
+            apply_plan(tile_cands[c]);
+            auto pair = [&] {
+                launch_tile_predict<T>(C, tp.A, tp.threads, X, Xp, Xo, nullptr, nz, dt, lam, st);
+                launch_tile_correct<T>(C, tp.A, tp.threads, Xp, Xo, B1, nullptr, dt, lam, st);
+            };
+            pair();
+            SNH_HIP(hipEventRecord(e0, st));
+            for (int k = 0; k < launches; k++) pair();
+            SNH_HIP(hipEventRecord(e1, st));
+            SNH_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            SNH_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (c == 0 || ms < best_t) {
+                best_t = ms;
+                best = c;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        apply_plan(tile_cands[best]);
+        tuned_index = best;
+        if (best_ms) *best_ms = best_t / launches;
+    }
+
     void drop_graph()
     {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -992,12 +1097,7 @@ struct SystemImpl final : snh_system {
         SNH_REQUIRE(std::isfinite(dt) && dt > 0, SNH_ERR_INVALID, "Select positive dt");
         SNH_REQUIRE(damping >= 0, SNH_ERR_INVALID, "Select nonnegative damping");
         SNH_REQUIRE(max_iters >= 1, SNH_ERR_INVALID, "max_iters must be at least 1");
-        if (!B1) {
-            B1 = own(dalloc<T>((size_t)planes * total));
-            B2 = own(dalloc<T>((size_t)planes * total));
-            if (ewald || N == 0) Bh = own(dalloc<T>((size_t)planes * total));
-            if (N == 0) Bh2 = own(dalloc<T>((size_t)planes * total));
-        }
+        ensure_midpoint_buffers();
         // atol = tol * sqrt(length(S)), length counting sites (Integrators.jl:383)
         std::vector<double> a2(R);
         for (int r = 0; r < R; r++) {
@@ -1351,6 +1451,8 @@ struct SystemImpl final : snh_system {
         if (key == "tile_sp") return tiled ? tp.A.sp : 0;
         if (key == "tile_work") return tiled ? tp.A.nwork : 0;
         if (key == "tile_lds") return tiled ? (int64_t)tp.lds : 0;
+        if (key == "tile_candidates") return (int64_t)tile_cands.size();
+        if (key == "tile_tuned") return tuned_index;
         if (key == "gather_chunk") return C.chunk;
         if (key == "graph_steps") return graph_steps;
         if (key == "graph_live") return graph_exec ? 1 : 0;
@@ -1560,6 +1662,11 @@ extern "C" int snh_query(snh_system* sys, const char* key, int64_t* value)
         SNH_REQUIRE(key && value, SNH_ERR_INVALID, "null argument");
         *value = sys->query(key);
     });
+}
+
+extern "C" int snh_autotune(snh_system* sys, int32_t candidates, int32_t launches, double* best_ms)
+{
+    return guarded(sys, [&] { sys->autotune(candidates, launches, best_ms); });
 }
 
 extern "C" int snh_midpoint_record(snh_system* sys, double dt, double tol, int32_t max_iters, int32_t nsteps,

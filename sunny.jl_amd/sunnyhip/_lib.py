@@ -25,7 +25,7 @@ EXPORTS = [
     "snh_set_dipoles", "snh_get_dipoles", "snh_set_coherents", "snh_get_coherents",
     "snh_step_langevin", "snh_step_midpoint", "snh_energy", "snh_energy_grad", "snh_synchronize",
     "snh_enable_timing", "snh_kernel_time", "snh_copy_calibration", "snh_query",
-    "snh_midpoint_record", "snh_midpoint_adjoint",
+    "snh_midpoint_record", "snh_midpoint_adjoint", "snh_autotune",
 ]
 
 
@@ -80,6 +80,7 @@ def lib():
         L.snh_kernel_time.argtypes = [vp, C.c_char_p, dp, C.POINTER(C.c_int64)]
         L.snh_copy_calibration.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
         L.snh_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
+        L.snh_autotune.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
         L.snh_midpoint_record.argtypes = [vp, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32]
         L.snh_midpoint_adjoint.argtypes = [vp, C.c_int32, C.POINTER(SnhSpec), dp, dp, dp]
         for f in EXPORTS:
@@ -163,3 +164,11 @@ class Handle:
         v = C.c_int64()
         self.call("snh_query", key.encode(), C.byref(v))
         return v.value
+
+    def autotune(self, candidates=8, launches=3):
+        """Times the cost model's best tile plans on the device and keeps the
+        fastest (snh_autotune); returns its ms per predictor + corrector pair
+        (0 when the handle does not use the tiled sweep)."""
+        ms = C.c_double()
+        self.call("snh_autotune", int(candidates), int(launches), C.byref(ms))
+        return ms.value

@@ -189,3 +189,40 @@ def test_tiled_bench_shape_step():
     o = O.Oracle(a.spec())
     E_ref = o.energy(a.dipoles_flat())
     assert abs(sh.energy(a) - E_ref) <= 1e-6 * energy_scale(a)
+
+
+def test_autotune_keeps_results():
+    """The empirical autotuner times the cost model's best tile plans and
+    keeps the fastest; every plan computes the same per-site arithmetic, so
+    the stepped states stay bitwise equal to those of the default plan."""
+    dims = [(8, 8, 8), (6, 4, 10), (5, 5, 5)]
+
+    def batch():
+        systems = [sh.workloads.cubic_model(dims=d, seed=40 + r, precision=32) for r, d in enumerate(dims)]
+        return sh.ReplicaBatch(systems, precision=32)
+
+    announce(f"autotune cubic {dims} fp32")
+    a, b = batch(), batch()
+    hb = b._device()
+    assert hb.query("tile_candidates") >= 1 and hb.query("tile_tuned") == -1
+    ms = hb.autotune(candidates=6, launches=2)
+    assert ms > 0
+    assert 0 <= hb.query("tile_tuned") < min(6, hb.query("tile_candidates"))
+    integ = sh.Langevin(0.03, damping=0.1, kT=0.4)
+    kTs = np.array([0.2, 0.5, 0.8])
+    sh.step_(a, integ, nsteps=5, kTs=kTs)
+    sh.step_(b, integ, nsteps=5, kTs=kTs)
+    da = np.empty((a._device().total, 3))
+    db = np.empty_like(da)
+    a._device().call("snh_get_dipoles", sh._lib.dptr(da))
+    hb.call("snh_get_dipoles", sh._lib.dptr(db))
+    assert np.array_equal(da, db)
+    np.testing.assert_allclose(sh.energy(a), sh.energy(b), rtol=1e-12)
+    # the midpoint after autotuning (which allocates scratch buffers of its
+    # own) finds every buffer it needs
+    mp = sh.ImplicitMidpoint(0.03, tol=1e-5)
+    sh.step_(a, mp, nsteps=2)
+    sh.step_(b, mp, nsteps=2)
+    a._device().call("snh_get_dipoles", sh._lib.dptr(da))
+    hb.call("snh_get_dipoles", sh._lib.dptr(db))
+    np.testing.assert_allclose(da, db, rtol=0, atol=1e-6)
