@@ -175,19 +175,26 @@ int snh_kernel_time(snh_system *sys, const char *family, double *ms, int64_t *la
 int snh_copy_calibration(snh_system *sys, int32_t repeats, int64_t *bytes);
 
 /* Tuning with no counterpart in the reference: times up to `candidates` of
- * the tile plans the cost model ranks best (one warm-up, then `launches`
- * Heun predictor + corrector pairs each, on this handle's replicas, with the
- * corrector writing to a scratch buffer so the state is untouched) and keeps
- * the fastest for all later calls.  best_ms (may be NULL): its time per pair.
- * A handle on the ghost-padded layout is left as it is (best_ms = 0).  The
- * chosen plan is visible through snh_query ("tile_tuned": its rank). */
+ * the tile plans the cost model ranks best on this handle's replicas (one
+ * warm-up, then `launches` launches each of the Heun predictor and of the
+ * corrector, the corrector writing to a scratch buffer so the state is
+ * untouched) and keeps the fastest plan for each kernel for all later calls
+ * (the predictor's plan also serves energy and gradients, the corrector's the
+ * midpoint iteration); a later-ranked plan must be 2% faster to be kept.
+ * best_ms (may be NULL): predictor + corrector time of the kept plans.  A
+ * handle on the ghost-padded layout is left as it is (best_ms = 0).  The
+ * kept ranks are visible through snh_query ("tile_tuned",
+ * "tile_tuned_correct"). */
 int snh_autotune(snh_system *sys, int32_t candidates, int32_t launches, double *best_ms);
 
 /* Diagnostics with no counterpart in the reference: integer facts about the
  * device plan.  Keys: "layout" (SNH_LAYOUT_*, as chosen), "tile_T1",
  * "tile_T2", "tile_threads", "tile_work" (workgroups per tiled launch),
- * "tile_lds" (bytes of LDS per workgroup), "tile_candidates" (plans ranked
- * by the cost model), "tile_tuned" (rank kept by snh_autotune, -1 before),
+ * "tile_lds" (bytes of LDS per workgroup) of the predictor's plan;
+ * "correct_T1", "correct_T2", "correct_sp", "correct_threads",
+ * "correct_work" of the corrector's plan; "tile_candidates" (plans ranked by
+ * the cost model), "tile_tuned" / "tile_tuned_correct" (ranks kept by
+ * snh_autotune, -1 before),
  * "gather_chunk", "graph_steps"
  * (Langevin steps per captured HIP graph; 0 = graphs off, the default; set
  * by the environment variable SNH_GRAPH_STEPS at snh_create), "graph_live"

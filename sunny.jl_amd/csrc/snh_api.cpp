@@ -333,11 +333,15 @@ struct SystemImpl final : snh_system {
     std::vector<int> rblk;
     // LDS-tiled gather (unpadded layout) when it fits; else ghost-padded gather
     bool tiled = false;
-    TilePlan tp;
+    // Tiled plans: tp for the predictor, energy and gradient; tc for the
+    // corrector and the midpoint iteration (the kernels share the state
+    // layout, so each may run on its own plan).
+    TilePlan tp, tc;
     std::vector<TilePlan> tile_cands;  // the cost model's best plans, best first
-    std::vector<void*> plan_mem;       // device tables of the plan in use
+    std::vector<void*> plan_mem[2];    // device tables of tp and tc
+    int* d_tcblk = nullptr;            // per-replica workgroup ranges of tc
     size_t partial_cap = 0;            // entries of `partial`
-    int tuned_index = -1;              // candidate kept by autotune (-1: the cost model's first)
+    int tuned_index[2] = {-1, -1};     // candidates kept by autotune (-1: the cost model's first)
     int* d_trblk = nullptr;
     // bond tables as given (for the adjoint's per-parameter unit couplings)
     std::vector<int> bsrc_h;      // padded slot -> original bond, -1 for padding
@@ -651,7 +655,10 @@ struct SystemImpl final : snh_system {
         Xp = own(dalloc<T>((size_t)planes * total));
         Xo = own(dalloc<T>((size_t)planes * total));
         SNH_HIP(hipMemset(X, 0, sizeof(T) * planes * (size_t)total));
-        if (tiled) apply_plan(tile_cands[0]);
+        if (tiled) {
+            apply_plan(tile_cands[0], 0);
+            apply_plan(tile_cands[0], 1);
+        }
         if (!partial) {
             partial_cap = std::max<size_t>(nblocks, 1);
             partial = own(dalloc<double>(partial_cap));
@@ -871,23 +878,25 @@ struct SystemImpl final : snh_system {
                     "midpoint buffers missing");
     }
 
-    // Makes P the tiled plan in use: device work list, LDS bond offsets per
+    // Makes P the tiled plan of the predictor side (which = 0) or of the
+    // corrector side (which = 1): device work list, LDS bond offsets per
     // ring phase, per-replica workgroup ranges, and room for its partial sums.
-    void apply_plan(const TilePlan& P)
+    void apply_plan(const TilePlan& P, int which)
     {
         if (st) SNH_HIP(hipStreamSynchronize(st));
-        for (void* p : plan_mem) (void)hipFree(p);
-        plan_mem.clear();
-        tp = P;
-        const std::vector<int> ldsoff = tile_lds_offsets(tp.A, na, bond_h.data(), bsrc_h, sizeof(T));
-        TileWork* w = dupload(tp.work);
+        for (void* p : plan_mem[which]) (void)hipFree(p);
+        plan_mem[which].clear();
+        TilePlan& D = which ? tc : tp;
+        D = P;
+        const std::vector<int> ldsoff = tile_lds_offsets(D.A, na, bond_h.data(), bsrc_h, sizeof(T));
+        TileWork* w = dupload(D.work);
         int* o = dupload(ldsoff);
-        int* rb = dupload(tp.rwork);
-        plan_mem = {w, o, rb};
-        tp.A.work = w;
-        tp.A.ldsoff = o;
-        d_trblk = rb;
-        const size_t need = std::max<size_t>(nblocks, tp.work.size());
+        int* rb = dupload(D.rwork);
+        plan_mem[which] = {w, o, rb};
+        D.A.work = w;
+        D.A.ldsoff = o;
+        (which ? d_tcblk : d_trblk) = rb;
+        const size_t need = std::max<size_t>(nblocks, D.work.size());
         if (need > partial_cap) {
             if (partial) {
                 owned.erase(std::remove(owned.begin(), owned.end(), (void*)partial), owned.end());
@@ -899,10 +908,11 @@ struct SystemImpl final : snh_system {
         drop_graph();  // captured launches carry the old plan
     }
 
-    // Times up to `candidates` of the cost model's plans on this handle (one
-    // warm-up, then `launches` Heun predictor + corrector pairs each, the
-    // corrector writing into a scratch buffer so the state is untouched) and
-    // keeps the fastest.
+    // Times up to `candidates` of the cost model's plans on this handle, the
+    // Heun predictor and corrector separately (one warm-up, then `launches`
+    // launches each; the corrector writes into a scratch buffer so the state
+    // is untouched), and keeps the fastest for each.  A later-ranked plan
+    // must win by 2% to displace an earlier one (timing noise).
     void autotune(int candidates, int launches, double* best_ms) override
     {
         SNH_REQUIRE(candidates >= 1 && launches >= 1, SNH_ERR_INVALID, "candidates and launches must be positive");
@@ -918,32 +928,38 @@ struct SystemImpl final : snh_system {
         hipEvent_t e0, e1;
         SNH_HIP(hipEventCreate(&e0));
         SNH_HIP(hipEventCreate(&e1));
-        const int nc = std::min<int>(candidates, (int)tile_cands.size());
-        int best = 0;
-        float best_t = 0;
-        for (int c = 0; c < nc; c++) {
-            apply_plan(tile_cands[c]);
-            auto pair = [&] {
-                launch_tile_predict<T>(C, tp.A, tp.threads, X, Xp, Xo, nullptr, nz, dt, lam, st);
-                launch_tile_correct<T>(C, tp.A, tp.threads, Xp, Xo, B1, nullptr, dt, lam, st);
-            };
-            pair();
+        auto timed = [&](auto&& launch) {
+            launch();
             SNH_HIP(hipEventRecord(e0, st));
-            for (int k = 0; k < launches; k++) pair();
+            for (int k = 0; k < launches; k++) launch();
             SNH_HIP(hipEventRecord(e1, st));
             SNH_HIP(hipEventSynchronize(e1));
             float ms = 0;
             SNH_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (c == 0 || ms < best_t) {
-                best_t = ms;
-                best = c;
-            }
+            return ms / launches;
+        };
+        const int nc = std::min<int>(candidates, (int)tile_cands.size());
+        int best[2] = {0, 0};
+        float best_t[2] = {0, 0};
+        for (int c = 0; c < nc; c++) {
+            apply_plan(tile_cands[c], 0);
+            apply_plan(tile_cands[c], 1);
+            const float t[2] = {
+                timed([&] { launch_tile_predict<T>(C, tp.A, tp.threads, X, Xp, Xo, nullptr, nz, dt, lam, st); }),
+                timed([&] { launch_tile_correct<T>(C, tc.A, tc.threads, Xp, Xo, B1, nullptr, dt, lam, st); })};
+            for (int k = 0; k < 2; k++)
+                if (c == 0 || t[k] < 0.98f * best_t[k]) {
+                    best_t[k] = t[k];
+                    best[k] = c;
+                }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        apply_plan(tile_cands[best]);
-        tuned_index = best;
-        if (best_ms) *best_ms = best_t / launches;
+        apply_plan(tile_cands[best[0]], 0);
+        apply_plan(tile_cands[best[1]], 1);
+        tuned_index[0] = best[0];
+        tuned_index[1] = best[1];
+        if (best_ms) *best_ms = best_t[0] + best_t[1];
     }
 
     void drop_graph()
@@ -978,7 +994,7 @@ struct SystemImpl final : snh_system {
         }
         e = tbegin();  // the corrector overwrites the state in place (the predictor consumed it)
         if (tiled)
-            launch_tile_correct<T>(C, tp.A, tp.threads, Xp, Xo, X, ewSp, tdt, tl, st);
+            launch_tile_correct<T>(C, tc.A, tc.threads, Xp, Xo, X, ewSp, tdt, tl, st);
         else if (N == 0)
             launch_dipole_correct<T>(C, nblocks, Xp, Xo, X, ewSp, tdt, tl, st);
         else
@@ -1141,7 +1157,7 @@ struct SystemImpl final : snh_system {
                     }
                     hipEvent_t e = tbegin();
                     if (tiled)
-                        launch_tile_midpoint_iter<T>(C, tp.A, tp.threads, X, S1, S2, hat, phi, tdt, tl, partial,
+                        launch_tile_midpoint_iter<T>(C, tc.A, tc.threads, X, S1, S2, hat, phi, tdt, tl, partial,
                                                      conv_iter, hat_next, st);
                     else if (N == 0)
                         launch_dipole_midpoint_iter<T>(C, nblocks, X, S1, S2, hat, phi, tdt, tl, partial, conv_iter,
@@ -1149,7 +1165,7 @@ struct SystemImpl final : snh_system {
                     else
                         launch_sun_midpoint_iter<T>(C, nblocks, X, S1, S2, phi, tdt, tl, partial, conv_iter, st);
                     tend("midpoint_iter", e);
-                    launch_midpoint_check(partial, tiled ? d_trblk : d_rblk, R, atol2, conv_iter, it, st);
+                    launch_midpoint_check(partial, tiled ? d_tcblk : d_rblk, R, atol2, conv_iter, it, st);
                     if (N == 0)
                         launch_dipole_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, it, st);
                     else
@@ -1452,7 +1468,13 @@ struct SystemImpl final : snh_system {
         if (key == "tile_work") return tiled ? tp.A.nwork : 0;
         if (key == "tile_lds") return tiled ? (int64_t)tp.lds : 0;
         if (key == "tile_candidates") return (int64_t)tile_cands.size();
-        if (key == "tile_tuned") return tuned_index;
+        if (key == "tile_tuned") return tuned_index[0];
+        if (key == "tile_tuned_correct") return tuned_index[1];
+        if (key == "correct_T1") return tiled ? tc.A.T1 : 0;
+        if (key == "correct_T2") return tiled ? tc.A.T2 : 0;
+        if (key == "correct_sp") return tiled ? tc.A.sp : 0;
+        if (key == "correct_threads") return tiled ? tc.threads : 0;
+        if (key == "correct_work") return tiled ? tc.A.nwork : 0;
         if (key == "gather_chunk") return C.chunk;
         if (key == "graph_steps") return graph_steps;
         if (key == "graph_live") return graph_exec ? 1 : 0;

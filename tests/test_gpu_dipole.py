@@ -208,6 +208,7 @@ def test_autotune_keeps_results():
     ms = hb.autotune(candidates=6, launches=2)
     assert ms > 0
     assert 0 <= hb.query("tile_tuned") < min(6, hb.query("tile_candidates"))
+    assert 0 <= hb.query("tile_tuned_correct") < min(6, hb.query("tile_candidates"))
     integ = sh.Langevin(0.03, damping=0.1, kT=0.4)
     kTs = np.array([0.2, 0.5, 0.8])
     sh.step_(a, integ, nsteps=5, kTs=kTs)
