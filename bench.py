@@ -126,14 +126,110 @@ def bench_sun3(args):
     integ = sh.Langevin(0.02, damping=0.1, kT=0.0)
     dt, fams = timed_steps(batch, integ, args.steps, args.warmup, ("sun_predict", "sun_correct"), kTs)
     ups = batch.nsites * args.steps / dt
+    # Schroedinger midpoint (deterministic) on the same batch
+    mp = sh.ImplicitMidpoint(0.02, tol=1e-5)
+    h = batch._device()
+    sh.step_(batch, mp, nsteps=2)
+    h.call("snh_synchronize")
+    msteps = 20
+    t0 = time.perf_counter()
+    its = sh.step_(batch, mp, nsteps=msteps)
+    h.call("snh_synchronize")
+    dtm = time.perf_counter() - t0
+    mid = {"value": batch.nsites * msteps / dtm, "steps": msteps, "iterations": int(np.max(its)), "tol": 1e-5}
+    log(f"SU(3) midpoint: {mid['value']:.3e} spin-updates/s, {mid['iterations']} iterations")
     per = {f: ms / max(n, 1) for f, (ms, n) in fams.items()}
     log(f"SU(3) batch: {batch.nsites} sites, {dt * 1e3 / args.steps:.3f} ms/step, {ups:.3e} spin-updates/s, {per}")
     print(json.dumps({"metric": "spin-updates/s (auxiliary: SU(3) tempering batch)", "value": ups,
                       "unit": "spin-updates/s", "ms_per_step": dt * 1e3 / args.steps, "steps": args.steps,
-                      "dtype": "f32", "kernels_ms": per,
+                      "dtype": "f32", "kernels_ms": per, "implicit_midpoint": mid,
                       "config": {"workload": "64 SU(3) triangular replicas, L uniform 32..256 (seed 0), "
                                              f"{batch.nsites} sites, kT 0.1..2.0 geometric, dt 0.02, damping 0.1"}}),
           flush=True)
+
+
+def bench_cubic16(args):
+    """Auxiliary line: simple cubic 16^3 = 4096 spins, :dipole s = 1, g = 2,
+    NN J = 1 and -0.1 Sz^2, plain FP64; Langevin(0.05; damping 0.1, kT 0.5)
+    and ImplicitMidpoint(0.05; damping 0.1), 1000 steps each (launch- and
+    latency-bound); beside them the FP64 CPU port on one core."""
+    import sunnyhip as sh
+    from oracle import oracle as O
+    nst = 1000
+    s = sh.workloads.cubic_model(L=16, seed=0, precision=64)
+    lang = sh.Langevin(0.05, damping=0.1, kT=0.5)
+    dt_l, _ = timed_steps(s, lang, nst, 50)
+    mp = sh.ImplicitMidpoint(0.05, damping=0.1)
+    sh.step_(s, mp, nsteps=20)
+    h = s._device()
+    h.call("snh_synchronize")
+    t0 = time.perf_counter()
+    its = sh.step_(s, mp, nsteps=nst)
+    h.call("snh_synchronize")
+    dt_m = time.perf_counter() - t0
+    o = O.Oracle(s.spec())
+    S = s.dipoles_flat()
+    nz = np.random.default_rng(1).standard_normal((s.nsites, 3))
+    ncpu = 200
+    t0 = time.perf_counter()
+    for _ in range(ncpu):
+        S = o.step_langevin(S=S, noise=nz, dt=0.05, damping=0.1, kT=0.5)
+    dt_c = time.perf_counter() - t0
+    out = {"langevin": s.nsites * nst / dt_l, "midpoint": s.nsites * nst / dt_m,
+           "midpoint_iterations_last": int(its[0]), "cpu_port_langevin_1core": s.nsites * ncpu / dt_c,
+           "ms_per_step": {"langevin": dt_l * 1e3 / nst, "midpoint": dt_m * 1e3 / nst}}
+    log(f"cubic 16^3 FP64: Langevin {out['langevin']:.3e}, midpoint {out['midpoint']:.3e} spin-updates/s "
+        f"({out['ms_per_step']}); CPU port {out['cpu_port_langevin_1core']:.3e}")
+    print(json.dumps({"metric": "spin-updates/s (auxiliary: cubic 16^3 plain FP64)", "unit": "spin-updates/s",
+                      "dtype": "f64", "steps": nst, **out,
+                      "config": {"workload": "simple cubic 16^3 = 4096 spins, dipole s=1, J=1, -0.1 Sz^2, "
+                                             "Langevin dt 0.05 damping 0.1 kT 0.5; ImplicitMidpoint dt 0.05 "
+                                             "damping 0.1"}}), flush=True)
+
+
+def bench_adjoint(args):
+    """Auxiliary line: simple cubic 64^3 = 262,144 spins, plain FP64, labelled
+    parameters J (NN exchange) and D (D Sz^2) at (0.9, -0.08);
+    ImplicitMidpoint(0.05) with damping 0 for 500 steps, checkpoints every
+    50; loss = mean over the 8^3 grid q = 2 pi m / 8 of (S(q) - S_target(q))^2
+    with the target from (1.0, -0.1).  Times the recorded forward pass, the
+    host loss and the reverse sweep (which recomputes every segment)."""
+    import ctypes
+
+    import sunnyhip as sh
+    L, nq, nsteps, dt, every = 64, 8, 500, 0.05, 50
+    dims = (L, L, L)
+    ref = sh.workloads.cubic_model(L=L, J=1.0, D=-0.1, seed=11, params=True)
+    sh.step_(ref, sh.ImplicitMidpoint(dt), nsteps=20)
+    target, _ = sh.structure_factor(ref.dipoles_flat(), dims, nq)
+    s = sh.workloads.cubic_model(L=L, J=0.9, D=-0.08, seed=11, params=True)
+    h = s._device()
+    h.call("snh_synchronize")
+    t0 = time.perf_counter()
+    h.call("snh_midpoint_record", dt, 1e-12, 100, nsteps, every)
+    h.call("snh_synchronize")
+    t1 = time.perf_counter()
+    s._dev_dirty = True
+    loss, dLdS = sh.structure_factor_loss(s.dipoles_flat(), dims, target, nq)
+    t2 = time.perf_counter()
+    specs = [sh.param_spec(s, lab) for lab in ("J", "D")]
+    from sunnyhip import _lib
+    packed = (_lib.SnhSpec * 2)(*[_lib.pack_spec(sp) for sp in specs])
+    grad = np.zeros(2)
+    h.call("snh_midpoint_adjoint", 2, ctypes.cast(packed, ctypes.POINTER(_lib.SnhSpec)),
+           _lib.dptr(np.ascontiguousarray(dLdS, float)), _lib.dptr(grad), None)
+    h.call("snh_synchronize")
+    t3 = time.perf_counter()
+    out = {"forward_record_s": t1 - t0, "loss_host_s": t2 - t1, "reverse_s": t3 - t2,
+           "forward_spin_updates_per_s": s.nsites * nsteps / (t1 - t0),
+           "gradient_per_s": 1.0 / (t3 - t0), "loss": loss, "grad": grad.tolist()}
+    log(f"adjoint 64^3 x {nsteps}: forward {t1 - t0:.3f} s, loss {t2 - t1:.3f} s, reverse {t3 - t2:.3f} s, "
+        f"grad {grad}")
+    print(json.dumps({"metric": "trajectory gradients/s (auxiliary: midpoint adjoint, cubic 64^3 x 500 steps)",
+                      "unit": "gradients/s", "value": out["gradient_per_s"], "dtype": "f64", **out,
+                      "config": {"workload": "simple cubic 64^3 = 262144 spins, J=0.9 D=-0.08 labelled, "
+                                             "ImplicitMidpoint dt 0.05 damping 0, 500 steps, checkpoints every 50, "
+                                             "structure-factor loss on the 8^3 q grid"}}), flush=True)
 
 
 def bench_ewald(args):
@@ -177,9 +273,11 @@ def main():
     ap.add_argument("--fp64-only", action="store_true", help="time only the plain FP64 step (tuning runs)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the cost model's first tile plan")
     ap.add_argument("--kT", type=float, default=0.3, help="temperature of the timed runs (tuning: 0 disables noise)")
-    ap.add_argument("--workload", choices=["pyrochlore", "sun3", "ewald"], default="pyrochlore",
-                    help="pyrochlore: the headline batch; sun3: 64 uneven SU(3) tempering replicas; "
-                         "ewald: dipolar pyrochlore 16^3 (auxiliary lines)")
+    ap.add_argument("--workload", choices=["pyrochlore", "sun3", "ewald", "cubic16", "adjoint"],
+                    default="pyrochlore",
+                    help="pyrochlore: the headline batch; auxiliary lines: sun3 (64 uneven SU(3) tempering "
+                         "replicas), ewald (dipolar pyrochlore 16^3), cubic16 (plain FP64 16^3), adjoint "
+                         "(midpoint trajectory gradient, cubic 64^3 x 500 steps)")
     ap.add_argument("--ewald-replicas", type=int, default=32, help="batch size of the Ewald auxiliary line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -192,6 +290,10 @@ def main():
         return bench_sun3(args)
     if args.workload == "ewald":
         return bench_ewald(args)
+    if args.workload == "cubic16":
+        return bench_cubic16(args)
+    if args.workload == "adjoint":
+        return bench_adjoint(args)
     t0 = time.perf_counter()
     base = sh.workloads.pyrochlore_model(L=args.L, seed=0, precision=32)
     log(f"model built in {time.perf_counter() - t0:.1f}s: {base.nsites} spins, {base.spec()['nbonds']} directed bonds")
