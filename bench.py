@@ -74,7 +74,11 @@ def measured_traffic(kernel, plan):
         return None
     with open(path) as f:
         d = json.load(f)
-    if d.get("plan") != plan or kernel not in d.get("kernels", {}):
+    # the launch geometry must match (the autotuner's ranks do not matter)
+    geom = ("layout", "tile_sp", "tile_T1", "tile_T2", "tile_threads", "tile_work", "tile_lds", "correct_sp",
+            "correct_T1", "correct_T2", "correct_threads", "correct_work")
+    have = d.get("plan") or {}
+    if any(have.get(k) != plan.get(k) for k in geom) or kernel not in d.get("kernels", {}):
         return None
     k = d["kernels"][kernel]
     return k["fetch_bytes"] + k["write_bytes"]

@@ -71,7 +71,7 @@ function coupling_tables(sys::System{N}) where N
     return (; na, bonds, bilin, biquad, hasbq, scalar, kappa, g, field, stevens, kmax, Lambda, dims, pos)
 end
 
-function to_device(sys::System{N}; precision=64, seed=rand(UInt64)) where N
+function to_device(sys::System{N}; precision=64, seed=rand(UInt64), autotune=false) where N
     t = coupling_tables(sys)
     ew = sys.ewald
     GC.@preserve t begin
@@ -86,6 +86,10 @@ function to_device(sys::System{N}; precision=64, seed=rand(UInt64)) where N
     d = DeviceSystem{N}(sys, h[], 0, seed)
     finalizer(x -> ccall((:snh_destroy, lib), Cint, (Ptr{Cvoid},), x.handle), d)
     upload!(d)
+    # measured choice of the tile plans (no reference counterpart; results
+    # do not depend on it)
+    autotune && check(ccall((:snh_autotune, lib), Cint, (Ptr{Cvoid}, Int32, Int32, Ptr{Float64}),
+                            d.handle, 8, 3, C_NULL), d.handle)
     return d
 end
 
