@@ -1126,7 +1126,6 @@ struct SystemImpl final : snh_system {
         T* bufs[2] = {B2, B1};  // S'' of iteration m lives in bufs[m % 2]: m odd -> B1
         for (int s = 0; s < nsteps; s++) {
             launch_fill_int(conv_iter, R, 0, st);
-            SNH_HIP(hipMemcpyAsync(B2, X, sizeof(T) * planes * (size_t)total, hipMemcpyDeviceToDevice, st));  // S' = S
             int it = 0;
             bool done = false;
             int chunk = std::max(1, std::min(last_iters, max_iters));
@@ -1134,7 +1133,7 @@ struct SystemImpl final : snh_system {
                 int upto = std::min(max_iters, it + chunk);
                 for (; it < upto;) {
                     ++it;
-                    T* S1 = bufs[(it + 1) % 2];
+                    T* S1 = it == 1 ? X : bufs[(it + 1) % 2];  // the first guess S' = S
                     T* S2 = bufs[it % 2];
                     const T* phi = nullptr;
                     const T* hat = nullptr;
@@ -1166,16 +1165,21 @@ struct SystemImpl final : snh_system {
                         launch_sun_midpoint_iter<T>(C, nblocks, X, S1, S2, phi, tdt, tl, partial, conv_iter, st);
                     tend("midpoint_iter", e);
                     launch_midpoint_check(partial, tiled ? d_tcblk : d_rblk, R, atol2, conv_iter, it, st);
-                    if (N == 0)
-                        launch_dipole_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, it, st);
-                    else
-                        launch_sun_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, it, st);
                 }
                 SNH_HIP(hipGetLastError());
                 SNH_HIP(hipMemcpyAsync(pinned_i, conv_iter, sizeof(int) * R, hipMemcpyDeviceToHost, st));
                 SNH_HIP(hipStreamSynchronize(st));
                 bool all = true;
                 for (int r = 0; r < R; r++) all &= pinned_i[r] != 0;
+                // S = normalize(S'') for every converged replica, once per step
+                // (also before reporting a replica that did not converge)
+                if (all || it >= max_iters) {
+                    if (N == 0)
+                        launch_dipole_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, 0, st);
+                    else
+                        launch_sun_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, 0, st);
+                    SNH_HIP(hipGetLastError());
+                }
                 if (all) {
                     done = true;
                 } else if (it >= max_iters) {

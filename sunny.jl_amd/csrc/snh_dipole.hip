@@ -131,8 +131,9 @@ __global__ __launch_bounds__(kBlock) void k_dipole_mid_finalize(Couplings<T> C, 
                                                                 const int* __restrict__ conv_iter, int iter)
 {
     const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
-    if (!sc.valid || conv_iter[sc.r] != iter) return;
-    const T* src = (iter & 1) ? Sa : Sb;
+    const int c = sc.valid ? conv_iter[sc.r] : 0;  // iter 0: every converged replica
+    if (c == 0 || (iter != 0 && c != iter)) return;
+    const T* src = (c & 1) ? Sa : Sb;
     store3_images(S, C.total, sc, C.G, normalize_dipole(planes(src, C.total)(sc.g), C.kappa[sc.i]));
 }
 

@@ -139,6 +139,8 @@ template <typename T>
 void launch_dipole_midpoint_iter(const Couplings<T>& C, int nblocks, const T* S, const T* S1, T* S2,
                                  const T* Shat, const T* phi, T dt, T lambda, double* partial,
                                  const int* conv_iter, T* hat_out, hipStream_t st);
+// S = kappa n(S'') for replicas that converged at `iter` (0: at any
+// iteration); S'' of iteration m is in Sa for odd m, Sb for even m.
 template <typename T>
 void launch_dipole_midpoint_finalize(const Couplings<T>& C, int nblocks, T* S, const T* Sa, const T* Sb,
                                      const int* conv_iter, int iter, hipStream_t st);

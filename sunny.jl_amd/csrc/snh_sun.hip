@@ -524,8 +524,9 @@ __global__ __launch_bounds__(kBlock) void k_sun_mid_finalize(Couplings<T> C, T* 
                                                              const int* __restrict__ conv_iter, int iter)
 {
     const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
-    if (!sc.valid || conv_iter[sc.r] != iter) return;
-    const T* src = (iter & 1) ? Za : Zb;
+    const int c = sc.valid ? conv_iter[sc.r] : 0;  // iter 0: every converged replica
+    if (c == 0 || (iter != 0 && c != iter)) return;
+    const T* src = (c & 1) ? Za : Zb;
     store_ket_images<T, N>(Z, C.total, sc, C.G,
                            normalize_ket<T, N>(load_ket<T, N>(src, C.total, sc.g), C.kappa[sc.i]));
 }
