@@ -380,6 +380,11 @@ struct SystemImpl final : snh_system {
     int* pinned_i = nullptr;   // [R]
     int planes = 3;
     int last_iters = 4;
+    // Small :dipole replicas on the ghost layout (no Ewald) step in one
+    // persistent workgroup per replica: no launch per stage, no host check
+    // per midpoint step.
+    bool persist = false;
+    int* fail_step = nullptr;  // [R] persistent midpoint: step that did not converge
     // Ewald
     EwaldGeom EG{};
     void* M = nullptr;
@@ -670,6 +675,13 @@ struct SystemImpl final : snh_system {
         SNH_HIP(hipHostMalloc((void**)&pinned, sizeof(double) * R, hipHostMallocDefault));
         SNH_HIP(hipHostMalloc((void**)&pinned_i, sizeof(int) * R, hipHostMallocDefault));
         if (ewald) build_ewald(sp);
+        long long max_sites = 0;
+        for (const auto& ri : reps) max_sites = std::max(max_sites, (long long)ri.ncell * na);
+        // measured crossover: faster up to one site per thread of the
+        // workgroup (64 sites: 5.8 vs 8.8 us per Langevin step), slower beyond
+        // (4096 sites: 64 vs 10.7 us)
+        persist = N == 0 && !ewald && !tiled && env_int("SNH_PERSIST", 1) != 0 &&
+                  max_sites <= env_int("SNH_PERSIST_MAX_SITES", kPersist);
     }
 
     void build_ewald(const snh_spec& sp)
@@ -1076,6 +1088,18 @@ struct SystemImpl final : snh_system {
             if (!dz_valid) launch_sun_expected_spin<T>(C, nblocks, X, Dz, st);
             dz_valid = true;
         }
+        if (persist) {
+            // chunks bound the length of one launch
+            for (int s = 0; s < nsteps; s += 4096) {
+                nz.step = step0 + (uint64_t)s;
+                hipEvent_t e = tbegin();
+                launch_dipole_langevin_persist<T>(C, R, d_rblk, X, Xp, Xo, nz, tdt, tl, std::min(4096, nsteps - s),
+                                                  st);
+                tend("dipole_persist", e);
+            }
+            SNH_HIP(hipGetLastError());
+            return;
+        }
         int s = 0;
         // the first step of a handle runs eagerly (it also sets kernel attributes)
         if (!warmed && nsteps > 0) {
@@ -1123,6 +1147,32 @@ struct SystemImpl final : snh_system {
         SNH_HIP(hipMemcpyAsync(atol2, a2.data(), sizeof(double) * R, hipMemcpyHostToDevice, st));
         SNH_HIP(hipStreamSynchronize(st));
         const T tdt = (T)dt, tl = (T)damping;
+        if (persist && R == 1 && nsteps > 0) {
+            // every step iterated to convergence on the device; one host check per call
+            if (!fail_step) fail_step = own(dalloc<int>(R));
+            launch_fill_int(fail_step, R, -1, st);
+            launch_fill_int(conv_iter, R, 0, st);
+            hipEvent_t e = tbegin();
+            launch_dipole_midpoint_persist<T>(C, R, d_rblk, X, B1, B2, Bh, Bh2, tdt, tl, atol2, max_iters, nsteps,
+                                              conv_iter, fail_step, st);
+            tend("midpoint_persist", e);
+            SNH_HIP(hipGetLastError());
+            SNH_HIP(hipMemcpyAsync(pinned_i, conv_iter, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+            SNH_HIP(hipStreamSynchronize(st));
+            const int its = pinned_i[0];
+            SNH_HIP(hipMemcpyAsync(pinned_i, fail_step, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+            SNH_HIP(hipStreamSynchronize(st));
+            if (iters) iters[0] = its;
+            last_iters = its;
+            if (pinned_i[0] >= 0) {
+                char msg[256];
+                snprintf(msg, sizeof msg,
+                         "Spherical midpoint method failed to converge to tolerance %g after %d iterations.", tol,
+                         max_iters);
+                throw Failure{SNH_ERR_NOCONVERGE, msg};
+            }
+            return;
+        }
         T* bufs[2] = {B2, B1};  // S'' of iteration m lives in bufs[m % 2]: m odd -> B1
         for (int s = 0; s < nsteps; s++) {
             launch_fill_int(conv_iter, R, 0, st);
@@ -1481,6 +1531,7 @@ struct SystemImpl final : snh_system {
         if (key == "correct_work") return tiled ? tc.A.nwork : 0;
         if (key == "gather_chunk") return C.chunk;
         if (key == "graph_steps") return graph_steps;
+        if (key == "persistent") return persist ? 1 : 0;
         if (key == "graph_live") return graph_exec ? 1 : 0;
         throw Failure{SNH_ERR_INVALID, "unknown query key: " + key};
     }

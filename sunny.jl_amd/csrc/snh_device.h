@@ -21,6 +21,10 @@
 namespace snh {
 
 constexpr int kBlock = 256;
+// Persistent small-replica kernels: one workgroup of kPersist threads per
+// replica, kPersistLanes blocks of kBlock sites in flight.
+constexpr int kPersist = 512;
+constexpr int kPersistLanes = kPersist / kBlock;
 
 // One work block = kBlock consecutive interior cells of one (replica, sublattice).
 struct BlockInfo {
@@ -100,16 +104,18 @@ struct SiteCoord {
     ReplicaInfo R;
 };
 
-__device__ __forceinline__ SiteCoord site_coord(const BlockInfo* __restrict__ blocks,
-                                                const ReplicaInfo* __restrict__ reps, const Ghost& G)
+// Site of lane `lane` (0..kBlock-1) in logical block `blk`.
+__device__ __forceinline__ SiteCoord site_coord_at(const BlockInfo* __restrict__ blocks,
+                                                   const ReplicaInfo* __restrict__ reps, const Ghost& G, int blk,
+                                                   int lane)
 {
     SiteCoord sc;
-    sc.blk = logical_block();
+    sc.blk = blk;
     const BlockInfo bi = blocks[sc.blk];
     sc.r = bi.r;
     sc.i = bi.i;
     sc.R = reps[bi.r];
-    sc.c = bi.c0 + (int)threadIdx.x;
+    sc.c = bi.c0 + lane;
     sc.valid = sc.c < sc.R.ncell;
     const int c = sc.valid ? sc.c : 0;
     const int t = fdiv(c, sc.R.L1, sc.R.inv1);
@@ -121,6 +127,12 @@ __device__ __forceinline__ SiteCoord site_coord(const BlockInfo* __restrict__ bl
     sc.edge = (sc.c1 < G.w1) | (sc.c1 >= sc.R.L1 - G.w1) | (sc.c2 < G.w2) | (sc.c2 >= sc.R.L2 - G.w2) |
               (sc.c3 < G.w3) | (sc.c3 >= sc.R.L3 - G.w3);
     return sc;
+}
+
+__device__ __forceinline__ SiteCoord site_coord(const BlockInfo* __restrict__ blocks,
+                                                const ReplicaInfo* __restrict__ reps, const Ghost& G)
+{
+    return site_coord_at(blocks, reps, G, logical_block(), (int)threadIdx.x);
 }
 
 // Range of image shifts m (in units of L) with 0 <= c + w + m L < L + 2w.

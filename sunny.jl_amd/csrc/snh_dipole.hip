@@ -25,16 +25,14 @@ namespace snh {
 // ---------------------------------------------------------------------------
 // Kernels
 
-// Heun predictor: dS1 = rhs(S, xi, dE(S)); S' = kappa*normalize(S + dS1);
+// Heun predictor at one site: dS1 = rhs(S, xi, dE(S)); S' = kappa*normalize(S + dS1);
 // H = S + dS1/2 - (S' x xi)/2.
 template <typename T, int CH, bool kIso>
-__global__ __launch_bounds__(kBlock) void k_dipole_predict(Couplings<T> C, const T* __restrict__ S,
-                                                           T* __restrict__ Sp, T* __restrict__ H,
-                                                           const T* __restrict__ phi, NoiseArgs<T> nz, T dt,
-                                                           T lambda)
+__device__ __forceinline__ void dipole_predict_site(const Couplings<T>& C, const SiteCoord& sc,
+                                                    const T* __restrict__ S, T* __restrict__ Sp, T* __restrict__ H,
+                                                    const T* __restrict__ phi, const NoiseArgs<T>& nz, T dt,
+                                                    T lambda)
 {
-    const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
-    if (!sc.valid) return;
     const Planes<T> Sv = planes(S, C.total);
     const SpinSrc<T, false> src{Sv, Sv};
     const V3<T> s = Sv(sc.g);
@@ -47,14 +45,22 @@ __global__ __launch_bounds__(kBlock) void k_dipole_predict(Couplings<T> C, const
     store3(H, C.total, sc.g, h);
 }
 
-// Heun corrector: S <- kappa*normalize(H + rhs_det(S', dE(S'))/2).
 template <typename T, int CH, bool kIso>
-__global__ __launch_bounds__(kBlock) void k_dipole_correct(Couplings<T> C, const T* __restrict__ Sp,
-                                                           const T* __restrict__ H, T* __restrict__ Sout,
-                                                           const T* __restrict__ phi, T dt, T lambda)
+__global__ __launch_bounds__(kBlock) void k_dipole_predict(Couplings<T> C, const T* __restrict__ S,
+                                                           T* __restrict__ Sp, T* __restrict__ H,
+                                                           const T* __restrict__ phi, NoiseArgs<T> nz, T dt,
+                                                           T lambda)
 {
     const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
-    if (!sc.valid) return;
+    if (sc.valid) dipole_predict_site<T, CH, kIso>(C, sc, S, Sp, H, phi, nz, dt, lambda);
+}
+
+// Heun corrector at one site: S <- kappa*normalize(H + rhs_det(S', dE(S'))/2).
+template <typename T, int CH, bool kIso>
+__device__ __forceinline__ void dipole_correct_site(const Couplings<T>& C, const SiteCoord& sc,
+                                                    const T* __restrict__ Sp, const T* __restrict__ H,
+                                                    T* __restrict__ Sout, const T* __restrict__ phi, T dt, T lambda)
+{
     const Planes<T> Pv = planes(Sp, C.total);
     const SpinSrc<T, false> src{Pv, Pv};
     const V3<T> sp = Pv(sc.g);
@@ -62,6 +68,15 @@ __global__ __launch_bounds__(kBlock) void k_dipole_correct(Couplings<T> C, const
     const V3<T> h = planes(H, C.total)(sc.g);
     store3_images(Sout, C.total, sc, C.G,
                   normalize_dipole(V3<T>{h.x + d.x / 2, h.y + d.y / 2, h.z + d.z / 2}, C.kappa[sc.i]));
+}
+
+template <typename T, int CH, bool kIso>
+__global__ __launch_bounds__(kBlock) void k_dipole_correct(Couplings<T> C, const T* __restrict__ Sp,
+                                                           const T* __restrict__ H, T* __restrict__ Sout,
+                                                           const T* __restrict__ phi, T dt, T lambda)
+{
+    const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
+    if (sc.valid) dipole_correct_site<T, CH, kIso>(C, sc, Sp, H, Sout, phi, dt, lambda);
 }
 
 // Materialised midpoint Shat (when a global term, Ewald, needs it).
@@ -74,6 +89,34 @@ __global__ __launch_bounds__(kBlock) void k_dipole_mid_hat(Couplings<T> C, const
     if (!sc.valid || conv_iter[sc.r] != 0) return;
     const SpinSrc<T, true> src{planes(S, C.total), planes(S1, C.total)};
     store3_images(Shat, C.total, sc, C.G, src.get(sc.g, C.kappa[sc.i]));
+}
+
+// One fixed-point sweep at one site with Shat given (gathered with its
+// images): S2 = S + rhs(Shat) at the site, the next iteration's Shat =
+// kappa*normalize((S + S2)/2) with images when hat_out is non-null (S2 then
+// has no images); returns |S1 - S2|^2 in FP64.
+template <typename T, int CH>
+__device__ __forceinline__ double dipole_mid_sweep_site(const Couplings<T>& C, const SiteCoord& sc, const T* S,
+                                                        const T* S1, T* S2, const T* Shat, const T* phi, T dt,
+                                                        T lambda, T* hat_out)
+{
+    const Planes<T> Hv = planes(Shat, C.total);
+    const SpinSrc<T, false> src{Hv, Hv};
+    const V3<T> h = Hv(sc.g);
+    const V3<T> g = local_field<T, CH>(C, sc, src, h, phi);
+    const V3<T> d = rhs_det(h, g, dt, lambda);
+    const V3<T> s = planes(S, C.total)(sc.g);
+    const V3<T> o = {s.x + d.x, s.y + d.y, s.z + d.z};
+    if (hat_out) {
+        store3(S2, C.total, sc.g, o);
+        store3_images(hat_out, C.total, sc, C.G,
+                      normalize_dipole(V3<T>{(s.x + o.x) / 2, (s.y + o.y) / 2, (s.z + o.z) / 2}, ldc(C.kappa, sc.i)));
+    } else {
+        store3_images(S2, C.total, sc, C.G, o);
+    }
+    const V3<T> p = planes(S1, C.total)(sc.g);
+    const double ex = (double)p.x - (double)o.x, ey = (double)p.y - (double)o.y, ez = (double)p.z - (double)o.z;
+    return ex * ex + ey * ey + ez * ez;
 }
 
 // One fixed-point sweep of the spherical midpoint method:
@@ -93,28 +136,16 @@ __global__ __launch_bounds__(kBlock) void k_dipole_mid_iter(Couplings<T> C, cons
     const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
     if (conv_iter[sc.r] != 0) return;  // block-uniform: the whole block belongs to replica r
     double d2 = 0;
-    if (sc.valid) {
-        V3<T> h, g;
-        if (kHatGiven) {
-            const Planes<T> Hv = planes(Shat, C.total);
-            const SpinSrc<T, false> src{Hv, Hv};
-            h = Hv(sc.g);
-            g = local_field<T, CH>(C, sc, src, h, phi);
-        } else {
-            const SpinSrc<T, true> src{planes(S, C.total), planes(S1, C.total)};
-            h = src.get(sc.g, C.kappa[sc.i]);
-            g = local_field<T, CH>(C, sc, src, h, phi);
-        }
+    if (sc.valid && kHatGiven) {
+        d2 = dipole_mid_sweep_site<T, CH>(C, sc, S, S1, S2, Shat, phi, dt, lambda, hat_out);
+    } else if (sc.valid) {
+        const SpinSrc<T, true> src{planes(S, C.total), planes(S1, C.total)};
+        const V3<T> h = src.get(sc.g, C.kappa[sc.i]);
+        const V3<T> g = local_field<T, CH>(C, sc, src, h, phi);
         const V3<T> d = rhs_det(h, g, dt, lambda);
         const V3<T> s = planes(S, C.total)(sc.g);
         const V3<T> o = {s.x + d.x, s.y + d.y, s.z + d.z};
-        if (hat_out) {
-            store3(S2, C.total, sc.g, o);
-            store3_images(hat_out, C.total, sc, C.G,
-                          normalize_dipole(V3<T>{(s.x + o.x) / 2, (s.y + o.y) / 2, (s.z + o.z) / 2}, ldc(C.kappa, sc.i)));
-        } else {
-            store3_images(S2, C.total, sc, C.G, o);
-        }
+        store3_images(S2, C.total, sc, C.G, o);
         const V3<T> p = planes(S1, C.total)(sc.g);
         const double ex = (double)p.x - (double)o.x, ey = (double)p.y - (double)o.y, ez = (double)p.z - (double)o.z;
         d2 = ex * ex + ey * ey + ez * ez;
@@ -135,6 +166,115 @@ __global__ __launch_bounds__(kBlock) void k_dipole_mid_finalize(Couplings<T> C, 
     if (c == 0 || (iter != 0 && c != iter)) return;
     const T* src = (c & 1) ? Sa : Sb;
     store3_images(S, C.total, sc, C.G, normalize_dipole(planes(src, C.total)(sc.g), C.kappa[sc.i]));
+}
+
+// ---------------------------------------------------------------------------
+// Persistent kernels for small replicas: one workgroup of kPersist threads
+// per replica runs many steps, its 256-site blocks kPersistLanes at a time,
+// with workgroup barriers between stages instead of kernel launches (and,
+// for the midpoint, instead of a host convergence check per step).  A
+// workgroup runs on one CU, and __syncthreads orders global memory at
+// workgroup scope, so one stage's stores are visible to the next.
+
+template <typename T, typename F>
+__device__ __forceinline__ void for_replica_sites(const Couplings<T>& C, int b0, int b1, F f)
+{
+    const int sub = (int)threadIdx.x / kBlock, lane = (int)threadIdx.x % kBlock;
+    for (int b = b0 + sub; b < b1; b += kPersistLanes) {
+        const SiteCoord sc = site_coord_at(C.blocks, C.reps, C.G, b, lane);
+        if (sc.valid) f(sc);
+    }
+}
+
+// Sum over the workgroup, returned to every thread (two barriers).
+__device__ __forceinline__ double persist_sum(double v, double* red)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0;
+#pragma unroll
+    for (int k = 0; k < kPersist / 64; k++) t += red[k];
+    __syncthreads();
+    return t;
+}
+
+// nsteps Heun steps of replica blockIdx.x (steps nz.step, nz.step + 1, ...).
+template <typename T, int CH, bool kIso>
+__global__ __launch_bounds__(kPersist) void k_dipole_langevin_persist(Couplings<T> C, const int* __restrict__ rblk,
+                                                                      T* X, T* Xp, T* Xo, NoiseArgs<T> nz, T dt,
+                                                                      T lambda, int nsteps)
+{
+    const int b0 = rblk[blockIdx.x], b1 = rblk[blockIdx.x + 1];
+    const unsigned long long step0 = nz.step;
+    for (int s = 0; s < nsteps; s++) {
+        nz.step = step0 + (unsigned long long)s;
+        for_replica_sites(C, b0, b1, [&](const SiteCoord& sc) __attribute__((always_inline)) {
+            dipole_predict_site<T, CH, kIso>(C, sc, X, Xp, Xo, nullptr, nz, dt, lambda);
+        });
+        __syncthreads();
+        for_replica_sites(C, b0, b1, [&](const SiteCoord& sc) __attribute__((always_inline)) {
+            dipole_correct_site<T, CH, kIso>(C, sc, Xp, Xo, X, nullptr, dt, lambda);
+        });
+        __syncthreads();
+    }
+}
+
+// nsteps spherical-midpoint steps of replica r = blockIdx.x, each iterated
+// to fast_isapprox convergence on the device: Shat of iteration m lives in
+// hats[m % 2] (hats = {Bh2, Bh}), S'' in bufs[m % 2] (bufs = {B2, B1}).
+// iters_out[r]: iterations of the last step; a step that does not converge
+// within max_iters sets fail_step[r] and leaves X at the previous step.
+template <typename T, int CH>
+__global__ __launch_bounds__(kPersist) void k_dipole_midpoint_persist(Couplings<T> C, const int* __restrict__ rblk,
+                                                                      T* X, T* B1, T* B2, T* Bh, T* Bh2, T dt,
+                                                                      T lambda, const double* __restrict__ atol2,
+                                                                      int max_iters, int nsteps,
+                                                                      int* __restrict__ iters_out,
+                                                                      int* __restrict__ fail_step)
+{
+    __shared__ double red[kPersist / 64];
+    const int r = blockIdx.x;
+    const int b0 = rblk[r], b1 = rblk[r + 1];
+    T* const bufs[2] = {B2, B1};
+    T* const hats[2] = {Bh2, Bh};
+    const double a2 = atol2[r];
+    for (int s = 0; s < nsteps; s++) {
+        // the first guess S' = S: Shat = kappa n((S + S)/2) = kappa n(S)
+        for_replica_sites(C, b0, b1, [&](const SiteCoord& sc) __attribute__((always_inline)) {
+            store3_images(Bh, C.total, sc, C.G, normalize_dipole(planes(X, C.total)(sc.g), ldc(C.kappa, sc.i)));
+        });
+        __syncthreads();
+        int it = 0;
+        bool conv = false;
+        while (!conv && it < max_iters) {
+            ++it;
+            const T* S1 = it == 1 ? X : bufs[(it + 1) % 2];
+            T* S2 = bufs[it % 2];
+            const T* hat = hats[it % 2];
+            T* hat_next = hats[(it + 1) % 2];
+            double d2 = 0;
+            for_replica_sites(C, b0, b1, [&](const SiteCoord& sc) __attribute__((always_inline)) {
+                d2 += dipole_mid_sweep_site<T, CH>(C, sc, X, S1, S2, hat, nullptr, dt, lambda, hat_next);
+            });
+            const double t = persist_sum(d2, red);  // uniform: every thread sums the same partials
+            conv = t <= a2 && !isnan(t);
+        }
+        if (!conv) {
+            if (threadIdx.x == 0) {
+                fail_step[r] = s;
+                iters_out[r] = it;
+            }
+            return;
+        }
+        const T* S2 = bufs[it % 2];
+        for_replica_sites(C, b0, b1, [&](const SiteCoord& sc) __attribute__((always_inline)) {
+            store3_images(X, C.total, sc, C.G, normalize_dipole(planes(S2, C.total)(sc.g), ldc(C.kappa, sc.i)));
+        });
+        __syncthreads();
+        if (threadIdx.x == 0) iters_out[r] = it;
+    }
 }
 
 template <typename T, int CH>
@@ -357,6 +497,32 @@ void launch_step_advance(unsigned long long* p, hipStream_t st)
     hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, st, p);
 }
 
+template <typename T>
+void launch_dipole_langevin_persist(const Couplings<T>& C, int R, const int* rblk, T* X, T* Xp, T* Xo,
+                                   const NoiseArgs<T>& nz, T dt, T lambda, int nsteps, hipStream_t st)
+{
+#define SNH_LP(CH, ISO)                                                                                      \
+    hipLaunchKernelGGL((k_dipole_langevin_persist<T, CH, ISO>), dim3(R), dim3(kPersist), 0, st, C, rblk, X, Xp, \
+                       Xo, nz, dt, lambda, nsteps)
+    if (C.chunk == 18) {
+        if (C.iso) SNH_LP(18, true); else SNH_LP(18, false);
+    } else if (C.chunk == 9) {
+        if (C.iso) SNH_LP(9, true); else SNH_LP(9, false);
+    } else {
+        if (C.iso) SNH_LP(6, true); else SNH_LP(6, false);
+    }
+#undef SNH_LP
+}
+
+template <typename T>
+void launch_dipole_midpoint_persist(const Couplings<T>& C, int R, const int* rblk, T* X, T* B1, T* B2, T* Bh, T* Bh2,
+                                    T dt, T lambda, const double* atol2, int max_iters, int nsteps, int* iters_out,
+                                    int* fail_step, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_dipole_midpoint_persist<T, 6>), dim3(R), dim3(kPersist), 0, st, C, rblk, X, B1, B2, Bh,
+                       Bh2, dt, lambda, atol2, max_iters, nsteps, iters_out, fail_step);
+}
+
 #define SNH_INST(T)                                                                                              \
     template void launch_dipole_predict<T>(const Couplings<T>&, int, const T*, T*, T*, const T*,                \
                                            const NoiseArgs<T>&, T, T, hipStream_t);                             \
@@ -369,6 +535,10 @@ void launch_step_advance(unsigned long long* p, hipStream_t st)
     template void launch_dipole_midpoint_finalize<T>(const Couplings<T>&, int, T*, const T*, const T*,           \
                                                      const int*, int, hipStream_t);                              \
     template void launch_dipole_grad<T>(const Couplings<T>&, int, const T*, const T*, T*, hipStream_t);          \
+    template void launch_dipole_langevin_persist<T>(const Couplings<T>&, int, const int*, T*, T*, T*,             \
+                                                    const NoiseArgs<T>&, T, T, int, hipStream_t);                \
+    template void launch_dipole_midpoint_persist<T>(const Couplings<T>&, int, const int*, T*, T*, T*, T*, T*, T, \
+                                                    T, const double*, int, int, int*, int*, hipStream_t);        \
     template void launch_dipole_energy<T>(const Couplings<T>&, int, const T*, const T*, double*, hipStream_t);
 
 SNH_INST(float)

@@ -153,6 +153,17 @@ void launch_reduce_partials(const double* partial, const int* rblk, int R, doubl
 void launch_midpoint_check(const double* partial, const int* rblk, int R, const double* atol2, int* conv_iter,
                            int iter, hipStream_t st);
 void launch_fill_int(int* p, int n, int v, hipStream_t st);
+// Persistent small-replica kernels (one workgroup per replica, rblk: its
+// range of logical blocks): nsteps Langevin steps, or nsteps midpoint steps
+// iterated to convergence on the device (iters_out / fail_step per replica).
+template <typename T>
+void launch_dipole_langevin_persist(const Couplings<T>& C, int R, const int* rblk, T* X, T* Xp, T* Xo,
+                                   const NoiseArgs<T>& nz, T dt, T lambda, int nsteps, hipStream_t st);
+template <typename T>
+void launch_dipole_midpoint_persist(const Couplings<T>& C, int R, const int* rblk, T* X, T* B1, T* B2, T* Bh, T* Bh2,
+                                    T dt, T lambda, const double* atol2, int max_iters, int nsteps, int* iters_out,
+                                    int* fail_step, hipStream_t st);
+
 // Device step counter of the noise stream: set, and advance by one.
 void launch_step_set(unsigned long long* p, unsigned long long v, hipStream_t st);
 void launch_step_advance(unsigned long long* p, hipStream_t st);

@@ -227,3 +227,53 @@ def test_autotune_keeps_results():
     a._device().call("snh_get_dipoles", sh._lib.dptr(da))
     hb.call("snh_get_dipoles", sh._lib.dptr(db))
     np.testing.assert_allclose(da, db, rtol=0, atol=1e-6)
+
+
+def test_persistent_small_replicas_match_launches(monkeypatch):
+    """Small :dipole replicas on the ghost layout step in one persistent
+    workgroup per replica; the results equal the per-stage launches (same
+    per-site arithmetic) and the oracle."""
+    dims = [(6, 6, 6), (4, 5, 3), (8, 2, 4)]
+
+    def batch(persist):
+        monkeypatch.setenv("SNH_PERSIST", "1" if persist else "0")
+        systems = [sh.workloads.cubic_model(dims=d, seed=50 + r, precision=64) for r, d in enumerate(dims)]
+        b = sh.ReplicaBatch(systems, precision=64, layout=1)
+        assert b._device().query("persistent") == int(persist)
+        return b, systems
+
+    announce(f"persistent vs launches cubic {dims} fp64")
+    (a, sa), (b, sb) = batch(True), batch(False)
+    integ = sh.Langevin(0.03, damping=0.1, kT=0.4)
+    kTs = np.array([0.2, 0.5, 0.8])
+    refs = [s.dipoles_flat() for s in sa]
+    sh.step_(a, integ, nsteps=7, kTs=kTs)
+    sh.step_(b, integ, nsteps=7, kTs=kTs)
+    a.sync_to_systems()
+    b.sync_to_systems()
+    for r in range(3):
+        assert rel_err(sa[r].dipoles_flat(), sb[r].dipoles_flat(), scale=1.0) <= 1e-13
+        o = O.Oracle(sa[r].spec())
+        S = refs[r]
+        key = O.replica_key(a.noise_seed, r)
+        for k in range(7):
+            S = o.step_langevin(S=S, noise=O.philox_normals(key, k, sa[r].nsites, 3), dt=integ.dt,
+                                damping=integ.damping, kT=kTs[r])
+        assert rel_err(sa[r].dipoles_flat(), S, scale=1.0) <= 1e-11
+    # midpoint on one replica: every step iterated to convergence on the device
+    monkeypatch.setenv("SNH_PERSIST", "1")
+    p = sh.workloads.cubic_model(L=6, seed=61, precision=64)
+    assert p._device().query("persistent") == 1
+    monkeypatch.setenv("SNH_PERSIST", "0")
+    q = sh.workloads.cubic_model(L=6, seed=61, precision=64)
+    assert q._device().query("persistent") == 0
+    mp = sh.ImplicitMidpoint(0.03, damping=0.05)
+    its_p = sh.step_(p, mp, nsteps=5)
+    its_q = sh.step_(q, mp, nsteps=5)
+    assert abs(int(its_p[0]) - int(its_q[0])) <= 1
+    assert rel_err(p.dipoles_flat(), q.dipoles_flat(), scale=1.0) <= 1e-12
+    o = O.Oracle(p.spec())
+    S = sh.workloads.cubic_model(L=6, seed=61, precision=64).dipoles_flat()
+    for _ in range(5):
+        S, _ = o.step_midpoint(S=S, dt=0.03, damping=0.05)
+    assert rel_err(p.dipoles_flat(), S, scale=1.0) <= 1e-11

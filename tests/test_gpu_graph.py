@@ -41,7 +41,10 @@ def state(obj):
 @pytest.mark.parametrize("kind", ["tile_fp32_batch", "ghost_fp64", "sun3_batch", "ewald_fp32"])
 def test_graph_replays_match_eager(kind, monkeypatch):
     announce(f"graph vs eager {kind}")
-    # graphs are opt-in, read at handle creation
+    # graphs are opt-in, read at handle creation; they replay the per-stage
+    # launches, which small ghost-layout replicas would otherwise replace by
+    # the persistent kernel
+    monkeypatch.setenv("SNH_PERSIST", "0")
     monkeypatch.setenv("SNH_GRAPH_STEPS", "8")
     a = twins(kind)[0]
     a._device()
