@@ -16,5 +16,6 @@ from .system import (VACUUM_PERMEABILITY, ModelParam, Moment, System, clone_syst
                      set_spin_rescaling_, set_stevens_expansion_)
 from . import workloads  # noqa: F401
 from .adjoint import midpoint_gradient, structure_factor, structure_factor_loss
+from .tempering import ParallelTempering
 
 __all__ = [n for n in dir() if not n.startswith("_")]
