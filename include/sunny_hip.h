@@ -174,6 +174,34 @@ int snh_kernel_time(snh_system *sys, const char *family, double *ms, int64_t *la
  * write).  Used to calibrate HBM traffic counters against a known size. */
 int snh_copy_calibration(snh_system *sys, int32_t repeats, int64_t *bytes);
 
+/* Sampled dynamical correlations of one :dipole replica, on the device.
+ *
+ * snh_corr_init replaces SampledCorrelations(sys; measure, energies, dt)
+ * (src/SampledCorrelations/SampledCorrelations.jl:194-268) for the buffers:
+ *   nobs observables, obs [nobs][na][3]: the value of observable a at a site
+ *   of atom i is obs[a][i] . S (Sunny's dipole-mode observables,
+ *   src/Measurements/MeasureSpec.jl:53-64: rows of g, or of the identity);
+ *   ncorr correlation pairs, pairs [ncorr][2] (0-based observable indices);
+ *   n_all_omega = 2 (n_omega - 1) time slots (even, <= 4096), measperiod
+ *   integration steps per snapshot, window 1 (cosine) or 0 (rectangular).
+ * The caller computes dt and measperiod as adjusted_dt_and_downsampling_factor
+ * (SampledCorrelations.jl:152-173).  Lattice axes of at most 32 cells.
+ *
+ * snh_corr_add_sample replaces add_sample!(sc, sys)
+ * (src/SampledCorrelations/CorrelationSampling.jl:1-175): a trajectory of
+ * step!(sys, ImplicitMidpoint(dt)) (damping 0, tolerance tol) records
+ * n_all_omega/2 + 1 snapshots of the observables, measperiod steps apart, and
+ * accumulates the sample's correlations into the running mean (the handle
+ * ends at the trajectory's last state).
+ *
+ * snh_corr_data returns sc.data, complex [ncorr][na][na][L1][L2][L3][n_all_omega]
+ * in Julia's column-major order (ncorr fastest, (re, im) pairs), and the
+ * number of samples (either pointer may be NULL). */
+int snh_corr_init(snh_system *sys, int32_t nobs, const double *obs, int32_t ncorr, const int32_t *pairs,
+                  int32_t n_all_omega, int32_t measperiod, int32_t window);
+int snh_corr_add_sample(snh_system *sys, double dt, double tol, int32_t max_iters);
+int snh_corr_data(snh_system *sys, double *data, int64_t *nsamples);
+
 /* Tuning with no counterpart in the reference: times up to `candidates` of
  * the tile plans the cost model ranks best on this handle's replicas (one
  * warm-up, then `launches` launches each of the Heun predictor and of the

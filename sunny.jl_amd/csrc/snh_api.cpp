@@ -315,6 +315,10 @@ struct snh_system {
     virtual void copy_calibration(int repeats, int64_t* bytes) = 0;
     virtual int64_t query(const std::string& key) const = 0;
     virtual void autotune(int candidates, int launches, double* best_ms) = 0;
+    virtual void corr_init(int nobs, const double* obs, int ncorr, const int32_t* pairs, int T, int measperiod,
+                           int window) = 0;
+    virtual void corr_add_sample(double dt, double tol, int max_iters) = 0;
+    virtual void corr_data(double* out, int64_t* nsamples) = 0;
     virtual void midpoint_record(double dt, double tol, int max_iters, int nsteps, int every) = 0;
     virtual void midpoint_adjoint(int nparams, const snh_spec* params, const double* dLdS, double* grad,
                                   double* dLdS0) = 0;
@@ -385,6 +389,22 @@ struct SystemImpl final : snh_system {
     // per midpoint step.
     bool persist = false;
     int* fail_step = nullptr;  // [R] persistent midpoint: step that did not converge
+    // Sampled correlations of one :dipole replica (snh_corr.hip)
+    struct Corr {
+        int nobs = 0, ncorr = 0, nt = 0, measperiod = 0;  // nt: time slots T = n_all_omega
+        long long ncell = 0;
+        int L[3] = {1, 1, 1};
+        std::vector<int> pairs;  // [ncorr][2] observable indices (alpha, beta)
+        std::vector<void*> mem;
+        double* obs = nullptr;   // [nobs][na][3]
+        void* samp = nullptr;    // [T][nobs][na][ncell] complex: A(r, t), then A(q, t)
+        void* samp2 = nullptr;   // [T][nobs][na][ncell] complex: A(q, w)
+        void* P = nullptr;       // [T][na][ncell] one channel's products
+        void* Y = nullptr;       // [T][na][ncell] its windowed real-time correlation
+        void* data = nullptr;    // [ncorr][na][T][na][ncell] running mean
+        double* post = nullptr;  // [T] window / (T n_contrib)
+        long long nsamples = 0;
+    } corr;
     // Ewald
     EwaldGeom EG{};
     void* M = nullptr;
@@ -415,6 +435,7 @@ struct SystemImpl final : snh_system {
     {
         if (st) (void)hipStreamSynchronize(st);
         drop_graph();
+        for (void* p : corr.mem) (void)hipFree(p);
         timer.clear();
         for (T* p : ckpt) (void)hipFree(p);
         for (void* p : owned) (void)hipFree(p);
@@ -972,6 +993,123 @@ struct SystemImpl final : snh_system {
         tuned_index[0] = best[0];
         tuned_index[1] = best[1];
         if (best_ms) *best_ms = best_t[0] + best_t[1];
+    }
+
+    // SampledCorrelations(sys; measure, energies, dt) buffers
+    // (SampledCorrelations.jl:194-268): T = n_all_omega time slots,
+    // observables o_a,i (A = o . S at atom i), correlation pairs (alpha, beta).
+    void corr_init(int nobs, const double* obs, int ncorr, const int32_t* pairs, int nt, int measperiod,
+                   int window) override
+    {
+        SNH_REQUIRE(N == 0 && R == 1, SNH_ERR_UNSUPPORTED, "sampled correlations support one :dipole replica");
+        SNH_REQUIRE(nobs >= 1 && obs && ncorr >= 1 && pairs, SNH_ERR_INVALID, "observables and pairs required");
+        SNH_REQUIRE(nt >= 2 && nt % 2 == 0 && nt <= 4096, SNH_ERR_INVALID, "n_all_omega must be even, 2..4096");
+        SNH_REQUIRE(measperiod >= 1, SNH_ERR_INVALID, "measperiod must be positive");
+        const ReplicaInfo& ri = reps[0];
+        const int L[3] = {ri.L1, ri.L2, ri.L3};
+        for (int a = 0; a < 3; a++)
+            SNH_REQUIRE(L[a] <= 32, SNH_ERR_UNSUPPORTED, "sampled correlations support at most 32 cells per axis");
+        for (int c = 0; c < 2 * ncorr; c++)
+            SNH_REQUIRE(pairs[c] >= 0 && pairs[c] < nobs, SNH_ERR_INVALID, "correlation pair out of range");
+        if (st) SNH_HIP(hipStreamSynchronize(st));
+        for (void* p : corr.mem) (void)hipFree(p);
+        corr = Corr{};
+        corr.nobs = nobs;
+        corr.ncorr = ncorr;
+        corr.nt = nt;
+        corr.measperiod = measperiod;
+        corr.ncell = ri.ncell;
+        for (int a = 0; a < 3; a++) corr.L[a] = L[a];
+        corr.pairs.assign(pairs, pairs + 2 * ncorr);
+        const size_t cplx = 2 * sizeof(double);
+        const size_t nsamp = (size_t)nt * nobs * na * corr.ncell, nchan = (size_t)nt * na * corr.ncell;
+        auto alloc = [&](size_t bytes) {
+            void* p = dalloc<char>(bytes);
+            corr.mem.push_back(p);
+            return p;
+        };
+        corr.obs = static_cast<double*>(alloc(sizeof(double) * 3 * nobs * na));
+        SNH_HIP(hipMemcpy(corr.obs, obs, sizeof(double) * 3 * nobs * na, hipMemcpyHostToDevice));
+        corr.samp = alloc(nsamp * cplx);
+        corr.samp2 = alloc(nsamp * cplx);
+        corr.P = alloc(nchan * cplx);
+        corr.Y = alloc(nchan * cplx);
+        corr.data = alloc((size_t)ncorr * na * nchan * cplx);
+        SNH_HIP(hipMemset(corr.data, 0, (size_t)ncorr * na * nchan * cplx));
+        // Time offsets fftfreq(T, T) = 0, 1, ..., T/2 - 1, -T/2, ..., -1 of a
+        // correlation of two signals of Tsnap = T/2 + 1 snapshots: n_contrib =
+        // Tsnap - |offset| estimates each; optional cos^2 window over the slots
+        // (CorrelationSampling.jl:51-117); 1/T of the normalised inverse FFT.
+        std::vector<double> post(nt);
+        const int Tsnap = nt / 2 + 1;
+        for (int t = 0; t < nt; t++) {
+            const int off = t < nt / 2 ? t : t - nt;
+            const double c = std::cos(M_PI * t / nt);
+            const double w = window ? c * c : 1.0;
+            post[t] = w / ((double)nt * (Tsnap - std::abs(off)));
+        }
+        corr.post = static_cast<double*>(alloc(sizeof(double) * nt));
+        SNH_HIP(hipMemcpy(corr.post, post.data(), sizeof(double) * nt, hipMemcpyHostToDevice));
+    }
+
+    // add_sample!(sc, sys) = new_sample! + accum_sample!
+    // (CorrelationSampling.jl:21-138, :154-175): a trajectory of
+    // ImplicitMidpoint(dt) with damping 0 records Tsnap snapshots every
+    // measperiod steps (the handle ends at its last state), then the sample's
+    // correlations enter the running mean.
+    void corr_add_sample(double dt, double tol, int max_iters) override
+    {
+        SNH_REQUIRE(corr.nt > 0, SNH_ERR_INVALID, "sampled correlations are not initialised");
+        const int nt = corr.nt, Tsnap = nt / 2 + 1, nobs = corr.nobs;
+        const long long ncell = corr.ncell;
+        const size_t nsamp = (size_t)nt * nobs * na * ncell;
+        SNH_HIP(hipMemsetAsync(corr.samp, 0, nsamp * 2 * sizeof(double), st));
+        const double scale = 1.0 / std::sqrt((double)ncell);  // the 1/sqrt(prod(dims)) of space_fft!
+        launch_corr_observe<T>(C, nblocks, X, nobs, corr.obs, corr.samp, 0, scale, st);
+        for (int n = 1; n < Tsnap; n++) {
+            step_midpoint(dt, 0.0, tol, max_iters, corr.measperiod, nullptr);
+            launch_corr_observe<T>(C, nblocks, X, nobs, corr.obs, corr.samp, n, scale, st);
+        }
+        // lattice transform of the recorded slots (the padding stays zero)
+        launch_corr_space_fft(corr.samp, (long long)Tsnap * nobs * na, corr.L[0], corr.L[1], corr.L[2], st);
+        launch_corr_tdft(corr.samp, corr.samp2, (long long)nobs * na * ncell, nt, -1, nullptr, nullptr, 0, st);
+        const long long count = ++corr.nsamples;
+        const long long nl = (long long)na * ncell;
+        const size_t chan = (size_t)nt * nl * 2 * sizeof(double);
+        for (int c = 0; c < corr.ncorr; c++)
+            for (int i = 0; i < na; i++) {
+                launch_corr_product(corr.samp2, corr.P, nt, nobs, na, ncell, corr.pairs[2 * c], i,
+                                    corr.pairs[2 * c + 1], st);
+                launch_corr_tdft(corr.P, corr.Y, nl, nt, +1, nullptr, corr.post, 0, st);
+                launch_corr_tdft(corr.Y, static_cast<char*>(corr.data) + ((size_t)c * na + i) * chan, nl, nt, -1,
+                                 nullptr, nullptr, count, st);
+            }
+        SNH_HIP(hipGetLastError());
+    }
+
+    // sc.data in Julia's column-major order data[c, i, j, q1, q2, q3, w]
+    // (SampledCorrelations.jl:3), complex as (re, im) pairs.
+    void corr_data(double* out, int64_t* nsamples) override
+    {
+        SNH_REQUIRE(corr.nt > 0, SNH_ERR_INVALID, "sampled correlations are not initialised");
+        const int nt = corr.nt, nc = corr.ncorr;
+        const long long ncell = corr.ncell;
+        const size_t n = (size_t)nc * na * nt * na * ncell;
+        std::vector<double> h(2 * n);
+        SNH_HIP(hipMemcpyAsync(h.data(), corr.data, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+        SNH_HIP(hipStreamSynchronize(st));
+        if (out)
+            for (int c = 0; c < nc; c++)
+                for (int i = 0; i < na; i++)
+                    for (int k = 0; k < nt; k++)
+                        for (int j = 0; j < na; j++)
+                            for (long long q = 0; q < ncell; q++) {
+                                const size_t src = ((((size_t)c * na + i) * nt + k) * na + j) * ncell + q;
+                                const size_t dst = c + (size_t)nc * (i + (size_t)na * (j + (size_t)na * (q + ncell * k)));
+                                out[2 * dst] = h[2 * src];
+                                out[2 * dst + 1] = h[2 * src + 1];
+                            }
+        if (nsamples) *nsamples = corr.nsamples;
     }
 
     void drop_graph()
@@ -1739,6 +1877,22 @@ extern "C" int snh_query(snh_system* sys, const char* key, int64_t* value)
         SNH_REQUIRE(key && value, SNH_ERR_INVALID, "null argument");
         *value = sys->query(key);
     });
+}
+
+extern "C" int snh_corr_init(snh_system* sys, int32_t nobs, const double* obs, int32_t ncorr, const int32_t* pairs,
+                             int32_t n_all_omega, int32_t measperiod, int32_t window)
+{
+    return guarded(sys, [&] { sys->corr_init(nobs, obs, ncorr, pairs, n_all_omega, measperiod, window); });
+}
+
+extern "C" int snh_corr_add_sample(snh_system* sys, double dt, double tol, int32_t max_iters)
+{
+    return guarded(sys, [&] { sys->corr_add_sample(dt, tol, max_iters); });
+}
+
+extern "C" int snh_corr_data(snh_system* sys, double* data, int64_t* nsamples)
+{
+    return guarded(sys, [&] { sys->corr_data(data, nsamples); });
 }
 
 extern "C" int snh_autotune(snh_system* sys, int32_t candidates, int32_t launches, double* best_ms)

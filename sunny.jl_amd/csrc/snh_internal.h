@@ -164,6 +164,16 @@ void launch_dipole_midpoint_persist(const Couplings<T>& C, int R, const int* rbl
                                     T dt, T lambda, const double* atol2, int max_iters, int nsteps, int* iters_out,
                                     int* fail_step, hipStream_t st);
 
+// Sampled correlations (snh_corr.hip): buffers are FP64 complex (re, im).
+template <typename T>
+void launch_corr_observe(const Couplings<T>& C, int nblocks, const T* S, int nobs, const double* obs, void* samp,
+                         int t, double scale, hipStream_t st);
+void launch_corr_space_fft(void* buf, long long nfield, int L1, int L2, int L3, hipStream_t st);
+void launch_corr_tdft(const void* in, void* out, long long nl, int T, int sign, const double* pre, const double* post,
+                      long long mean_count, hipStream_t st);
+void launch_corr_product(const void* samp, void* P, int T, int nobs, int na, long long ncell, int a, int i, int b,
+                         hipStream_t st);
+
 // Device step counter of the noise stream: set, and advance by one.
 void launch_step_set(unsigned long long* p, unsigned long long v, hipStream_t st);
 void launch_step_advance(unsigned long long* p, hipStream_t st);

@@ -17,5 +17,6 @@ from .system import (VACUUM_PERMEABILITY, ModelParam, Moment, System, clone_syst
 from . import workloads  # noqa: F401
 from .adjoint import midpoint_gradient, structure_factor, structure_factor_loss
 from .tempering import ParallelTempering
+from .correlations import SampledCorrelations, add_sample_
 
 __all__ = [n for n in dir() if not n.startswith("_")]

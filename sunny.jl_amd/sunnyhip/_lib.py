@@ -25,7 +25,8 @@ EXPORTS = [
     "snh_set_dipoles", "snh_get_dipoles", "snh_set_coherents", "snh_get_coherents",
     "snh_step_langevin", "snh_step_midpoint", "snh_energy", "snh_energy_grad", "snh_synchronize",
     "snh_enable_timing", "snh_kernel_time", "snh_copy_calibration", "snh_query",
-    "snh_midpoint_record", "snh_midpoint_adjoint", "snh_autotune",
+    "snh_midpoint_record", "snh_midpoint_adjoint", "snh_autotune", "snh_corr_init", "snh_corr_add_sample",
+    "snh_corr_data",
 ]
 
 
@@ -81,6 +82,10 @@ def lib():
         L.snh_copy_calibration.argtypes = [vp, C.c_int32, C.POINTER(C.c_int64)]
         L.snh_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
         L.snh_autotune.argtypes = [vp, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+        L.snh_corr_init.argtypes = [vp, C.c_int32, dp, C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
+                                    C.c_int32]
+        L.snh_corr_add_sample.argtypes = [vp, C.c_double, C.c_double, C.c_int32]
+        L.snh_corr_data.argtypes = [vp, dp, C.POINTER(C.c_int64)]
         L.snh_midpoint_record.argtypes = [vp, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32]
         L.snh_midpoint_adjoint.argtypes = [vp, C.c_int32, C.POINTER(SnhSpec), dp, dp, dp]
         for f in EXPORTS:

@@ -1,0 +1,53 @@
+"""GPU parity of the device SampledCorrelations with the restated
+accum_sample! (oracle/correlations_ref.py) on trajectories of the FP64
+oracle integrator."""
+import numpy as np
+import pytest
+
+import sunnyhip as sh
+from oracle import oracle as O
+from oracle.correlations_ref import SampledCorrelationsRef
+
+from helpers import announce, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def model(name):
+    if name == "cubic4":
+        return sh.workloads.cubic_model(L=4, seed=5, precision=64)
+    return sh.workloads.pyrochlore_model(L=2, J1=1.0, J2=0.1, A=-0.05, seed=6, precision=64)
+
+
+@pytest.mark.parametrize("name,window", [("cubic4", "cosine"), ("pyro2", "cosine"), ("cubic4", "rectangular")])
+def test_sampled_correlations_match_oracle(name, window):
+    s = model(name)
+    # n_all_omega = 10; omega_max = 20 keeps the trajectory short (measperiod 4,
+    # 40 steps per two samples) so that rounding differences between two FP64
+    # integrators stay far below the tolerance despite chaotic dynamics
+    energies = np.linspace(0.0, 20.0, 6)
+    sc = sh.SampledCorrelations(s, energies=energies, dt=0.05, window=window)
+    announce(f"sampled correlations {name} {window}, T={sc.n_all_omega}, measperiod={sc.measperiod}")
+    na = s.crystal.natoms
+    ref = SampledCorrelationsRef(s.dims, na, sc.observables, sc.corr_pairs, sc.n_all_omega, window)
+    o = O.Oracle(s.spec())
+    S = s.dipoles_flat()
+    Tsnap = sc.n_all_omega // 2 + 1
+    for _ in range(2):
+        snaps = [S]
+        for _ in range(Tsnap - 1):
+            for _ in range(sc.measperiod):
+                S, _ = o.step_midpoint(S=S, dt=sc.dt, damping=0.0, tol=sc.tol)
+            snaps.append(S)
+        ref.accum_sample(ref.new_sample(snaps))
+        sh.add_sample_(sc, s)
+    assert sc.nsamples == 2
+    assert rel_err(s.dipoles_flat(), S, scale=1.0) <= 1e-10
+    data = sc.data
+    scale = np.abs(ref.data).max()
+    np.testing.assert_allclose(data, ref.data, rtol=0, atol=1e-9 * scale)
+    # frequency sum rule on the device data: (1/T) sum_w S(q, w) is real and
+    # non-negative on the diagonal pairs (a, a), i = j
+    diag = [c for c, (a, b) in enumerate(sc.corr_pairs) if a == b]
+    m = data[diag][:, np.arange(na), np.arange(na)].sum(-1) / sc.n_all_omega
+    assert np.all(m.real >= -1e-12) and np.abs(m.imag).max() <= 1e-10 * scale
