@@ -236,6 +236,45 @@ def bench_adjoint(args):
                                              "structure-factor loss on the 8^3 q grid"}}), flush=True)
 
 
+def bench_corr(args):
+    """Auxiliary line: sampled dynamical correlations (SampledCorrelations,
+    ssf observables, six pairs) of pyrochlore 8^3 x 16 = 8192 spins, plain
+    FP64, energies 0..4 in 100 steps (198 time slots); one sample = the
+    midpoint trajectory plus the correlation accumulation; beside it the
+    numpy restatement of the accumulation on the host."""
+    import sunnyhip as sh
+    from oracle.correlations_ref import SampledCorrelationsRef
+    s = sh.workloads.pyrochlore_model(L=8, seed=3, precision=64)
+    sh.step_(s, sh.Langevin(0.02, damping=0.2, kT=0.3), nsteps=200)
+    sc = sh.SampledCorrelations(s, energies=np.linspace(0.0, 4.0, 100), dt=0.05)
+    h = s._device()
+    sh.add_sample_(sc, s)  # warm-up sample
+    h.call("snh_synchronize")
+    h.call("snh_enable_timing", 1)
+    t0 = time.perf_counter()
+    sh.add_sample_(sc, s)
+    h.call("snh_synchronize")
+    t1 = time.perf_counter()
+    traj_ms, _ = kernel_time(h, "midpoint_iter")
+    h.call("snh_enable_timing", 0)
+    nsteps = sc.measperiod * (sc.n_all_omega // 2)
+    # host restatement of accum_sample! on the same sizes (zero trajectory)
+    ref = SampledCorrelationsRef(s.dims, s.crystal.natoms, sc.observables, sc.corr_pairs, sc.n_all_omega)
+    buf = np.zeros((3, *s.dims, s.crystal.natoms, sc.n_all_omega), complex)
+    t2 = time.perf_counter()
+    ref.accum_sample(buf)
+    t3 = time.perf_counter()
+    out = {"sample_s": t1 - t0, "trajectory_steps": nsteps, "measperiod": sc.measperiod,
+           "n_all_omega": sc.n_all_omega, "midpoint_iter_kernel_ms": traj_ms,
+           "host_numpy_accumulation_s": t3 - t2, "value": 1.0 / (t1 - t0)}
+    log(f"correlations pyrochlore 8^3: sample {t1 - t0:.3f} s ({nsteps} midpoint steps), host numpy "
+        f"accumulation alone {t3 - t2:.3f} s")
+    print(json.dumps({"metric": "samples/s (auxiliary: sampled correlations, pyrochlore 8^3 x 16)",
+                      "unit": "samples/s", "dtype": "f64", **out,
+                      "config": {"workload": "pyrochlore 8^3 x 16 = 8192 spins, FP64, ssf observables, "
+                                             "energies 0..4 (100), dt 0.05 adjusted, ImplicitMidpoint"}}), flush=True)
+
+
 def bench_ewald(args):
     """Auxiliary line: pyrochlore 16^3 cells x 16 = 65,536 spins, s = 7/2,
     J1 = 0.05 meV, Ewald dipole-dipole (mu0 muB^2 = 0.6745817653324668,
@@ -277,11 +316,11 @@ def main():
     ap.add_argument("--fp64-only", action="store_true", help="time only the plain FP64 step (tuning runs)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the cost model's first tile plan")
     ap.add_argument("--kT", type=float, default=0.3, help="temperature of the timed runs (tuning: 0 disables noise)")
-    ap.add_argument("--workload", choices=["pyrochlore", "sun3", "ewald", "cubic16", "adjoint"],
+    ap.add_argument("--workload", choices=["pyrochlore", "sun3", "ewald", "cubic16", "adjoint", "corr"],
                     default="pyrochlore",
                     help="pyrochlore: the headline batch; auxiliary lines: sun3 (64 uneven SU(3) tempering "
                          "replicas), ewald (dipolar pyrochlore 16^3), cubic16 (plain FP64 16^3), adjoint "
-                         "(midpoint trajectory gradient, cubic 64^3 x 500 steps)")
+                         "(midpoint trajectory gradient, cubic 64^3 x 500 steps), corr (sampled correlations)")
     ap.add_argument("--ewald-replicas", type=int, default=32, help="batch size of the Ewald auxiliary line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -298,6 +337,8 @@ def main():
         return bench_cubic16(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "corr":
+        return bench_corr(args)
     t0 = time.perf_counter()
     base = sh.workloads.pyrochlore_model(L=args.L, seed=0, precision=32)
     log(f"model built in {time.perf_counter() - t0:.1f}s: {base.nsites} spins, {base.spec()['nbonds']} directed bonds")
