@@ -187,4 +187,47 @@ function Sunny.energy(d::DeviceSystem)
     return E[]
 end
 
+# SampledCorrelations accumulated on the device (:dipole): the measure's
+# observables and pairs, dt and measperiod as Sunny computes them
+# (adjusted_dt_and_downsampling_factor), then add_sample! and sc.data.
+struct DeviceCorrelations
+    d::DeviceSystem
+    dt::Float64
+    measperiod::Int
+    dims::NTuple{4, Int}   # (ncorr, na, na) sizes and n_all_omega follow sc.data
+    n_all_omega::Int
+    ncorr::Int
+end
+
+function device_correlations(d::DeviceSystem{0}, measure; energies, dt, window=:cosine)
+    nω = length(energies)
+    dt, measperiod = Sunny.adjusted_dt_and_downsampling_factor(dt, nω, energies[end])
+    na = Sunny.natoms(d.sys.crystal)
+    obs = zeros(3, na, 3)                 # observable a of atom i: obs[:, i, a] in Julia order
+    for i in 1:na, a in 1:length(measure.observables[:, 1, 1, 1, i])
+        obs[:, i, a] = measure.observables[a, 1, 1, 1, i]
+    end
+    pairs = Int32[p[k] - 1 for p in measure.corr_pairs for k in 1:2]
+    n_all = 2(nω - 1)
+    check(ccall((:snh_corr_init, lib), Cint,
+                (Ptr{Cvoid}, Int32, Ptr{Float64}, Int32, Ptr{Int32}, Int32, Int32, Int32),
+                d.handle, size(obs, 3), obs, length(measure.corr_pairs), pairs, n_all, measperiod,
+                window == :cosine ? 1 : 0), d.handle)
+    return DeviceCorrelations(d, dt, measperiod, (length(measure.corr_pairs), na, na, 0), n_all,
+                              length(measure.corr_pairs))
+end
+
+function Sunny.add_sample!(c::DeviceCorrelations, ::DeviceSystem; tol=1e-12, max_iters=100)
+    check(ccall((:snh_corr_add_sample, lib), Cint, (Ptr{Cvoid}, Float64, Float64, Int32),
+                c.d.handle, c.dt, tol, max_iters), c.d.handle)
+end
+
+function correlation_data(c::DeviceCorrelations)
+    na = c.dims[2]
+    data = zeros(ComplexF64, c.ncorr, na, na, c.d.sys.dims..., c.n_all_omega)
+    check(ccall((:snh_corr_data, lib), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{Int64}),
+                c.d.handle, data, C_NULL), c.d.handle)
+    return data   # the layout of sc.data: zero-copy
+end
+
 end
