@@ -18,5 +18,6 @@ from . import workloads  # noqa: F401
 from .adjoint import midpoint_gradient, structure_factor, structure_factor_loss
 from .tempering import ParallelTempering
 from .correlations import SampledCorrelations, add_sample_
+from .optimize import MinimizationResult, minimize_energy_, suggest_timestep
 
 __all__ = [n for n in dir() if not n.startswith("_")]
