@@ -275,6 +275,59 @@ def bench_corr(args):
                                              "energies 0..4 (100), dt 0.05 adjusted, ImplicitMidpoint"}}), flush=True)
 
 
+def bench_local(args):
+    """Auxiliary line: LocalSampler (Metropolis trials, propose_delta(0.3),
+    kT 0.3) on the headline pyrochlore: one replica in FP32 and in FP64, and a
+    batch of FP32 replicas.  trials/s counts the expected nsweeps * nsites
+    trials of a call.  Beside it the oracle's scalar restatement on a bounded
+    sample (pyrochlore 8^3 x 16 = 8192 spins, same colour classes)."""
+    import sunnyhip as sh
+    from oracle import oracle as O
+    out = {}
+    cases = (("fp32_r1", 32, 1), ("fp64_r1", 64, 1), (f"fp32_r{args.local_replicas}", 32, args.local_replicas))
+    for label, prec, R in cases:
+        systems = [sh.workloads.pyrochlore_model(L=args.L, seed=50 + r, precision=prec) for r in range(R)]
+        obj = systems[0] if R == 1 else sh.ReplicaBatch(systems, precision=prec)
+        sampler = sh.LocalSampler(kT=0.3, propose=sh.propose_delta(0.3))
+        h = obj._device()
+        sh.step_(obj, sampler, nsteps=2)
+        h.call("snh_synchronize")
+        t0 = time.perf_counter()
+        sh.step_(obj, sampler, nsteps=args.local_sweeps)
+        h.call("snh_synchronize")
+        t1 = time.perf_counter()
+        h.call("snh_enable_timing", 1)
+        sh.step_(obj, sampler, nsteps=args.local_sweeps)
+        ms, n = kernel_time(h, "local_color")
+        h.call("snh_enable_timing", 0)
+        m, nc, _ = sh.local_coloring(obj)
+        ns = sum(s.nsites for s in systems)
+        rate = ns * args.local_sweeps / (t1 - t0)
+        out[label] = {"sites": ns, "ms_per_sweep": (t1 - t0) * 1e3 / args.local_sweeps, "value": rate,
+                      "colors": nc, "period": list(m), "color_kernel_ms": ms / max(n, 1),
+                      "color_launches_per_sweep": n / args.local_sweeps}
+        log(f"LocalSampler {label}: {ns} sites, {nc} colours, {(t1 - t0) * 1e3 / args.local_sweeps:.3f} ms/sweep, "
+            f"{rate:.3e} trials/s, colour kernel {ms / max(n, 1):.4f} ms")
+        del obj, systems
+    small = sh.workloads.pyrochlore_model(L=8, seed=1, precision=64)
+    m, nc, col = sh.local_coloring(small)
+    o = O.Oracle(small.spec())
+    S = small.dipoles_flat()
+    sweeps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < args.cpu_seconds / 3:
+        S, _ = o.local_sweeps(S, m, col.reshape(-1), nc, [(2, 0.3, 1.0)], 0.3, 7, sweeps, 1.0)
+        sweeps += 1
+    cpu = small.nsites * sweeps / (time.perf_counter() - t0)
+    log(f"LocalSampler CPU oracle: {cpu:.3e} trials/s (1 core, {sweeps} sweeps of {small.nsites})")
+    print(json.dumps({"metric": "Metropolis trials/s (auxiliary: LocalSampler, pyrochlore 32^3 x 16)",
+                      "unit": "trials/s", "sweeps": args.local_sweeps, "by_case": out,
+                      "cpu_baseline": {"value": cpu, "unit": "trials/s", "cores": 1, "kind": "port",
+                                       "sample": f"oracle colour passes, pyrochlore 8^3 x 16, {sweeps} sweeps"},
+                      "config": {"workload": "pyrochlore 32^3 x 16 = 524288 spins/replica, dipole s=3/2, "
+                                             "J1 1.0 J2 0.1 A -0.05, LocalSampler kT 0.3 propose_delta(0.3)"}}),
+          flush=True)
+
+
 def bench_ewald(args):
     """Auxiliary line: pyrochlore 16^3 cells x 16 = 65,536 spins, s = 7/2,
     J1 = 0.05 meV, Ewald dipole-dipole (mu0 muB^2 = 0.6745817653324668,
@@ -316,12 +369,15 @@ def main():
     ap.add_argument("--fp64-only", action="store_true", help="time only the plain FP64 step (tuning runs)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the cost model's first tile plan")
     ap.add_argument("--kT", type=float, default=0.3, help="temperature of the timed runs (tuning: 0 disables noise)")
-    ap.add_argument("--workload", choices=["pyrochlore", "sun3", "ewald", "cubic16", "adjoint", "corr"],
+    ap.add_argument("--workload", choices=["pyrochlore", "sun3", "ewald", "cubic16", "adjoint", "corr", "local"],
                     default="pyrochlore",
                     help="pyrochlore: the headline batch; auxiliary lines: sun3 (64 uneven SU(3) tempering "
                          "replicas), ewald (dipolar pyrochlore 16^3), cubic16 (plain FP64 16^3), adjoint "
-                         "(midpoint trajectory gradient, cubic 64^3 x 500 steps), corr (sampled correlations)")
+                         "(midpoint trajectory gradient, cubic 64^3 x 500 steps), corr (sampled correlations), local "
+                         "(LocalSampler Metropolis sweeps)")
     ap.add_argument("--ewald-replicas", type=int, default=32, help="batch size of the Ewald auxiliary line")
+    ap.add_argument("--local-replicas", type=int, default=16, help="batch size of the LocalSampler line")
+    ap.add_argument("--local-sweeps", type=int, default=20, help="timed sweeps of the LocalSampler line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (replicas shard across GPUs "
@@ -339,6 +395,8 @@ def main():
         return bench_adjoint(args)
     if args.workload == "corr":
         return bench_corr(args)
+    if args.workload == "local":
+        return bench_local(args)
     t0 = time.perf_counter()
     base = sh.workloads.pyrochlore_model(L=args.L, seed=0, precision=32)
     log(f"model built in {time.perf_counter() - t0:.1f}s: {base.nsites} spins, {base.spec()['nbonds']} directed bonds")

@@ -202,6 +202,34 @@ int snh_corr_init(snh_system *sys, int32_t nobs, const double *obs, int32_t ncor
 int snh_corr_add_sample(snh_system *sys, double dt, double tol, int32_t max_iters);
 int snh_corr_data(snh_system *sys, double *data, int64_t *nsamples);
 
+/* step!(sys, LocalSampler(; kT, nsweeps, propose)) for every replica
+ * (src/MonteCarlo/Samplers.jl:133-153), :dipole mode without Ewald.
+ * Proposal p of the mixture (@mix_proposals, Samplers.jl:67-100) has
+ * kinds[p] = 0 propose_uniform (randspin, src/System/System.jl:444-447),
+ * 1 propose_flip (flip, System.jl:440-442) or 2 propose_delta(magnitudes[p])
+ * (perturbed_spin, System.jl:454-460; an infinite magnitude is uniform),
+ * drawn with probability weights[p] / sum(weights); 1 <= nprop <= 4.
+ * kT: [R].  A trial's energy change is local_energy_change
+ * (src/System/Interactions.jl:311-380) in FP64, and it is accepted when
+ * kT = 0 and dE <= 0, or rand <= exp(-dE/kT).
+ * Trial order: the reference draws ceil(nsweeps * nsites) random sites; here
+ * floor(nsweeps) sweeps try every site once, colour class by colour class
+ * (snh_local_coloring), and a fractional remainder f adds a sweep that tries
+ * each site with probability f.  Draws are Philox4x32-10 keyed by
+ * (seed, replica) and counted by (site, sweep0 + k) on a stream disjoint from
+ * snh_step_langevin's.  dE_dS (may be NULL): [R][4], the sums of dE and of
+ * dS = S_new - S_old over the accepted trials of this call (LocalSampler's
+ * fields dE, dS accumulate them). */
+int snh_step_local(snh_system *sys, const double *kT, int32_t nprop, const int32_t *kinds,
+                   const double *magnitudes, const double *weights, uint64_t seed, uint64_t sweep0,
+                   double nsweeps, double *dE_dS);
+
+/* The colour classes of snh_step_local (no counterpart in the reference):
+ * class (i, r) holds the sites (c, i) with c = r mod m componentwise, and
+ * color [na][m3][m2][m1] (may be NULL) its colour in 0..ncolors-1; no bond
+ * joins two sites of one colour. */
+int snh_local_coloring(snh_system *sys, int32_t m[3], int32_t *ncolors, int32_t *color);
+
 /* Tuning with no counterpart in the reference: times up to `candidates` of
  * the tile plans the cost model ranks best on this handle's replicas (one
  * warm-up, then `launches` launches each of the Heun predictor and of the

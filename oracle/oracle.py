@@ -58,6 +58,12 @@ def lib():
         L.so_ewald_grad_bruteforce.argtypes = [sp, _dp, _dp]
         L.so_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.so_philox_normals.argtypes = [C.c_uint64, C.c_uint64, C.c_long, C.c_int, _dp]
+        L.so_local_energy_change.argtypes = [sp, _dp, C.c_long, _dp]
+        L.so_local_energy_change.restype = C.c_double
+        L.so_local_sweeps.argtypes = [sp, _dp, _ip, _ip, C.c_int, C.c_int, _ip, _dp, _dp, C.c_double, C.c_uint64,
+                                      C.c_uint64, C.c_double, _dp]
+        L.so_local_random_sites.argtypes = [sp, _dp, C.c_int, _ip, _dp, _dp, C.c_double, C.c_uint64, C.c_double,
+                                            _dp]
         _lib = L
     return _lib
 
@@ -128,6 +134,44 @@ class Oracle:
             Zr = None
         S = np.ascontiguousarray(S, float).reshape(self.ns, 3)
         return lib().so_energy(self._p(), _d(S), _d(Zr))
+
+    def local_energy_change(self, S, site, Snew):
+        """local_energy_change(sys, site, state) for a :dipole state."""
+        S = np.ascontiguousarray(S, float).reshape(self.ns, 3)
+        sn = np.ascontiguousarray(Snew, float).reshape(3)
+        return lib().so_local_energy_change(self._p(), _d(S), int(site), _d(sn))
+
+    @staticmethod
+    def _props(proposals):
+        """[(kind, magnitude, weight), ...] -> kind, mag, cumulative probability arrays."""
+        kind = np.array([p[0] for p in proposals], np.int32)
+        mag = np.array([p[1] for p in proposals], float)
+        w = np.array([p[2] for p in proposals], float)
+        return kind, mag, np.cumsum(w / w.sum())
+
+    def local_sweeps(self, S, m, color, ncolors, proposals, kT, key, sweep0, nsweeps):
+        """LocalSampler sweeps in colour-class passes; returns (S, [dE, dSx, dSy, dSz])."""
+        S = np.ascontiguousarray(S, float).reshape(self.ns, 3).copy()
+        kind, mag, cum = self._props(proposals)
+        mm = np.ascontiguousarray(m, np.int32)
+        col = np.ascontiguousarray(color, np.int32)
+        acc = np.zeros(4)
+        rc = lib().so_local_sweeps(self._p(), _d(S), _i(mm), _i(col), int(ncolors), len(kind), _i(kind), _d(mag),
+                                   _d(cum), float(kT), int(key) % 2**64, int(sweep0), float(nsweeps), _d(acc))
+        if rc:
+            raise ValueError("Energy delta for self-interaction not supported")
+        return S, acc
+
+    def local_random_sites(self, S, proposals, kT, seed, nsweeps):
+        """LocalSampler trials at random sites (the reference's order)."""
+        S = np.ascontiguousarray(S, float).reshape(self.ns, 3).copy()
+        kind, mag, cum = self._props(proposals)
+        acc = np.zeros(4)
+        rc = lib().so_local_random_sites(self._p(), _d(S), len(kind), _i(kind), _d(mag), _d(cum), float(kT),
+                                         int(seed), float(nsweeps), _d(acc))
+        if rc:
+            raise ValueError("Energy delta for self-interaction not supported")
+        return S, acc
 
     def expected_spin(self, Z):
         Zr = np.ascontiguousarray(np.stack([Z.real, Z.imag], -1), float)

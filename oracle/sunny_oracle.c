@@ -1067,3 +1067,209 @@ void so_philox_normals(uint64_t seed, uint64_t step, long ns, int per_site, doub
             }
         }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Metropolis updates of single dipoles.                                     */
+
+/* local_energy_change, Interactions.jl:311-380 (:dipole mode), with
+   ewald_energy_delta, Ewald.jl:234-241 (field at the site by the pairwise
+   sum of Ewald.jl:211-230).  NAN when a bond maps the site onto itself
+   (the reference raises "Energy delta for self-interaction not supported"). */
+double so_local_energy_change(const so_spec *sp, const double *S, long site, const double Snew[3])
+{
+    const long nc = ncells(sp);
+    const int *L = sp->L;
+    const int i = (int)(site / nc);
+    const long c = site % nc;
+    const int c1 = (int)(c % L[0]), c2 = (int)((c / L[0]) % L[1]), c3 = (int)(c / ((long)L[0] * L[1]));
+    const double *S0 = S + 3 * site;
+    double dS[3] = {Snew[0] - S0[0], Snew[1] - S0[1], Snew[2] - S0[2]};
+    double dE = 0;
+    /* Zeeman: dot(B, g, dS) */
+    const double *g = sp->g + 9 * i, *B = sp->field + 3 * i;
+    for (int r = 0; r < 3; r++) dE += B[r] * (g[3 * r] * dS[0] + g[3 * r + 1] * dS[1] + g[3 * r + 2] * dS[2]);
+    /* single-ion anisotropy */
+    double En, Eo, gr[3];
+    so_stevens_energy_grad(Snew, sp->stevens + 28 * i, sp->kmax[i], &En, gr);
+    so_stevens_energy_grad(S0, sp->stevens + 28 * i, sp->kmax[i], &Eo, gr);
+    dE += En - Eo;
+    /* every pair coupling of the site's atom, culled or not */
+    for (int b = 0; b < sp->nb; b++) {
+        const int *bd = sp->bond + 6 * b;
+        if (bd[0] != i) continue;
+        long sj = site_index(L, wrap(c1 + bd[2], L[0]), wrap(c2 + bd[3], L[1]), wrap(c3 + bd[4], L[2]), bd[1]);
+        if (sj == site) return NAN;
+        const double *Sj = S + 3 * sj, *J = sp->bilin + 9 * b;
+        for (int r = 0; r < 3; r++) dE += dS[r] * (J[3 * r] * Sj[0] + J[3 * r + 1] * Sj[1] + J[3 * r + 2] * Sj[2]);
+        if (sp->has_biquad[b]) {
+            double Qn[5], Qo[5], Qj[5];
+            quadrupole(Snew, Qn);
+            quadrupole(S0, Qo);
+            quadrupole(Sj, Qj);
+            const double *Bq = sp->biquad + 25 * b;
+            for (int p = 0; p < 5; p++)
+                for (int q = 0; q < 5; q++) dE += (Qn[p] - Qo[p]) * Bq[5 * p + q] * Qj[q];
+        }
+    }
+    if (sp->ewald_A) {
+        const int na = sp->na;
+        double grad[3] = {0, 0, 0};
+        for (int i2 = 0; i2 < na; i2++) {
+            const double *g2 = sp->g + 9 * i2;
+            for (int b3 = 0; b3 < L[2]; b3++)
+                for (int b2 = 0; b2 < L[1]; b2++)
+                    for (int b1 = 0; b1 < L[0]; b1++) {
+                        long off = site_index(L, wrap(b1 - c1, L[0]), wrap(b2 - c2, L[1]), wrap(b3 - c3, L[2]), 0);
+                        const double *A = sp->ewald_A + (((long)i2 * na + i) * nc + off) * 9;
+                        const double *s2 = S + 3 * site_index(L, b1, b2, b3, i2);
+                        double gs[3], Ags[3];
+                        for (int r = 0; r < 3; r++)
+                            gs[r] = g2[3 * r] * s2[0] + g2[3 * r + 1] * s2[1] + g2[3 * r + 2] * s2[2];
+                        for (int r = 0; r < 3; r++)
+                            Ags[r] = A[3 * r] * gs[0] + A[3 * r + 1] * gs[1] + A[3 * r + 2] * gs[2];
+                        for (int r = 0; r < 3; r++) grad[r] += g[r] * Ags[0] + g[3 + r] * Ags[1] + g[6 + r] * Ags[2];
+                    }
+        }
+        double dmu[3], Adm[3];
+        for (int r = 0; r < 3; r++) dmu[r] = -(g[3 * r] * dS[0] + g[3 * r + 1] * dS[1] + g[3 * r + 2] * dS[2]);
+        const double *A0 = sp->ewald_A + (((long)i * na + i) * nc) * 9;
+        for (int r = 0; r < 3; r++) Adm[r] = A0[3 * r] * dmu[0] + A0[3 * r + 1] * dmu[1] + A0[3 * r + 2] * dmu[2];
+        for (int r = 0; r < 3; r++) dE += dS[r] * grad[r] + dmu[r] * Adm[r] / 2;
+    }
+    return dE;
+}
+
+/* The proposal of a LocalSampler (Samplers.jl:1-100): kind 0 propose_uniform
+   (randspin, System.jl:444-447), 1 propose_flip (flip, System.jl:440-442),
+   2 propose_delta(mag) (perturbed_spin, System.jl:454-460; infinite mag is
+   randspin).  n: three standard normals. */
+static void local_proposal(int kind, double mag, double kappa, const double *S0, const double n[3], double out[3])
+{
+    if (kind == 1) {
+        for (int r = 0; r < 3; r++) out[r] = -S0[r];
+        return;
+    }
+    if (kind == 0 || isinf(mag)) {
+        for (int r = 0; r < 3; r++) out[r] = n[r];
+    } else {
+        for (int r = 0; r < 3; r++) out[r] = S0[r] + mag * kappa * n[r];
+    }
+    normalize_dipole(out, kappa);
+}
+
+static void box_muller_pair(uint32_t a, uint32_t b, double *x, double *y)
+{
+    double ua = ((double)a + 1.0) * (1.0 / 4294967296.0);
+    double ub = (double)b * (1.0 / 4294967296.0);
+    double r = sqrt(-2.0 * log(ua)), th = 2.0 * M_PI * ub;
+    *x = r * cos(th);
+    *y = r * sin(th);
+}
+
+/* One Metropolis trial at `site` with the given draws; accumulates dE, dS of
+   accepted moves into acc[4] (LocalSampler's step!, Samplers.jl:133-153).
+   Returns -1 when the energy change is undefined (self-interaction). */
+static int local_trial(const so_spec *sp, double *S, long site, int nprop, const int *kind, const double *mag,
+                        const double *cum, double kT, double u_mix, double u_acc, const double n[3], double acc[4])
+{
+    int k = nprop - 1;
+    for (int p = 0; p < nprop - 1; p++)
+        if (u_mix < cum[p]) {
+            k = p;
+            break;
+        }
+    const int i = (int)(site / ncells(sp));
+    double Sn[3];
+    local_proposal(kind[k], mag[k], sp->kappa[i], S + 3 * site, n, Sn);
+    const double dE = so_local_energy_change(sp, S, site, Sn);
+    if (isnan(dE)) return -1;
+    const int accept = kT == 0 ? dE <= 0 : u_acc <= exp(-dE / kT);
+    if (accept) {
+        acc[0] += dE;
+        for (int r = 0; r < 3; r++) {
+            acc[1 + r] += Sn[r] - S[3 * site + r];
+            S[3 * site + r] = Sn[r];
+        }
+    }
+    return 0;
+}
+
+/* Colour-class passes: the device's ordering of LocalSampler trials.  A
+   sweep is two passes; a pass visits the colour classes 0..ncolors-1 in turn
+   and tries every site of a class with probability 1/2 (no two sites of one
+   class share a bond, so their order is immaterial).  Class of site (c, i):
+   color[i][c3 % m3][c2 % m2][c1 % m1].  nsweeps may be fractional: there are
+   ceil(2 nsweeps) passes, the last with probability (2 nsweeps - passes + 1)/2,
+   so a call makes nsweeps * nsites trials on average, as the reference's
+   ceil(nsweeps * nsites) does.  (With every site tried in a fixed order,
+   Metropolis flips of Ising-like spins keep the domain walls of a chain
+   moving deterministically and the sweep is not ergodic.)
+   Draws of site s (Julia index) in pass P = 2 sweep0 + k: Philox4x32-10
+   under `key` of the counters (s, 0x4D430000, P) -> four normals by
+   Box-Muller (the first three are used) and (s, 0x4D430001, P) -> uniforms
+   u_acc, u_mix, u_try (each word / 2^32).  Returns -1 on a self-interaction. */
+int so_local_sweeps(const so_spec *sp, double *S, const int m[3], const int *color, int ncolors, int nprop,
+                    const int *kind, const double *mag, const double *cum, double kT, uint64_t key,
+                    uint64_t sweep0, double nsweeps, double acc[4])
+{
+    const int *L = sp->L;
+    const uint32_t kk[2] = {(uint32_t)key, (uint32_t)(key >> 32)};
+    const long npass = (long)ceil(2 * nsweeps);
+    for (long k = 0; k < npass; k++) {
+        const uint64_t P = 2 * sweep0 + (uint64_t)k;
+        const double prob = k == npass - 1 ? (2 * nsweeps - (double)(npass - 1)) / 2 : 0.5;
+        for (int col = 0; col < ncolors; col++)
+            for (int i = 0; i < sp->na; i++)
+                for (int c3 = 0; c3 < L[2]; c3++)
+                    for (int c2 = 0; c2 < L[1]; c2++)
+                        for (int c1 = 0; c1 < L[0]; c1++) {
+                            const int cls = c1 % m[0] + m[0] * (c2 % m[1] + m[1] * (c3 % m[2] + m[2] * i));
+                            if (color[cls] != col) continue;
+                            const long site = site_index(L, c1, c2, c3, i);
+                            uint32_t ctr[4] = {(uint32_t)site, 0x4D430000u, (uint32_t)P, (uint32_t)(P >> 32)}, u[4];
+                            so_philox4x32(ctr, kk, u);
+                            double n[4];
+                            box_muller_pair(u[0], u[1], &n[0], &n[1]);
+                            box_muller_pair(u[2], u[3], &n[2], &n[3]);
+                            ctr[1] = 0x4D430001u;
+                            so_philox4x32(ctr, kk, u);
+                            const double inv = 1.0 / 4294967296.0;
+                            if (!((double)u[2] * inv < prob)) continue;
+                            if (local_trial(sp, S, site, nprop, kind, mag, cum, kT, (double)u[1] * inv,
+                                            (double)u[0] * inv, n, acc))
+                                return -1;
+                        }
+    }
+    return 0;
+}
+
+/* LocalSampler's step! as the reference orders it (Samplers.jl:133-153):
+   ceil(nsweeps * nsites) trials at uniformly random sites.  The draws come
+   from splitmix64 seeded by `seed` (the reference draws from sys.rng). */
+static uint64_t splitmix64(uint64_t *x)
+{
+    uint64_t z = (*x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static double splitmix_u01(uint64_t *x) { return (double)(splitmix64(x) >> 11) * (1.0 / 9007199254740992.0); }
+
+int so_local_random_sites(const so_spec *sp, double *S, int nprop, const int *kind, const double *mag,
+                          const double *cum, double kT, uint64_t seed, double nsweeps, double acc[4])
+{
+    const long ns = nsites(sp);
+    const long niters = (long)ceil(nsweeps * (double)ns);
+    uint64_t st = seed;
+    for (long t = 0; t < niters; t++) {
+        const long site = (long)(splitmix64(&st) % (uint64_t)ns);
+        const double u_mix = splitmix_u01(&st);
+        double n[4];
+        box_muller_pair((uint32_t)splitmix64(&st), (uint32_t)splitmix64(&st), &n[0], &n[1]);
+        box_muller_pair((uint32_t)splitmix64(&st), (uint32_t)splitmix64(&st), &n[2], &n[3]);
+        const double u_acc = splitmix_u01(&st);
+        if (local_trial(sp, S, site, nprop, kind, mag, cum, kT, u_mix, u_acc, n, acc)) return -1;
+    }
+    return 0;
+}

@@ -13,7 +13,8 @@
  *   src/System/OnsiteCoupling.jl:316-425 classical Stevens anisotropy
  *   src/Operators/Spin.jl:1-167         spin matrices, expected_spin, ...
  *   src/System/Ewald.jl:1-241           A precompute, FFT convolution, energy
- *   src/System/System.jl:204-212,416-423 bonded_site, normalize_*
+ *   src/System/System.jl:204-212,416-468 bonded_site, normalize_*, proposals
+ *   src/MonteCarlo/Samplers.jl:1-153    LocalSampler (Metropolis trials)
  *
  * Layout (identical to Julia's column-major Array{Vec3,4}(L1,L2,L3,na)):
  *   site = c1 + L1*(c2 + L2*(c3 + L3*i)), all 0-based
@@ -98,6 +99,23 @@ void so_ewald_grad_bruteforce(const so_spec *sp, const double *S, double *dE);
 void so_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void so_philox_normals(uint64_t seed, uint64_t step, long nsites, int per_site,
                        double *out);
+
+/* local_energy_change (Interactions.jl:311-380, Ewald.jl:234-241), :dipole
+   mode: energy change when the dipole of `site` becomes Snew.  NAN for a
+   bond that maps the site onto itself. */
+double so_local_energy_change(const so_spec *sp, const double *S, long site, const double Snew[3]);
+/* LocalSampler trials (Samplers.jl:1-153) in the order of the device:
+   ceil(2 nsweeps) passes over the colour classes color[na][m3][m2][m1], each
+   trying every site with probability 1/2 (the last pass: the remainder).
+   Proposal p has kind[p] (0 uniform, 1 flip, 2 delta with mag[p]) and
+   cumulative probability cum[p].  acc += (dE, dS) of accepted trials. */
+int so_local_sweeps(const so_spec *sp, double *S, const int m[3], const int *color, int ncolors, int nprop,
+                    const int *kind, const double *mag, const double *cum, double kT, uint64_t key,
+                    uint64_t sweep0, double nsweeps, double acc[4]);
+/* The same trials at uniformly random sites, as Samplers.jl:133-153 orders
+   them (ceil(nsweeps * nsites) trials; splitmix64 draws from seed). */
+int so_local_random_sites(const so_spec *sp, double *S, int nprop, const int *kind, const double *mag,
+                          const double *cum, double kT, uint64_t seed, double nsweeps, double acc[4]);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -292,6 +293,75 @@ struct KernelTimer {
     }
 };
 
+// Colour classes for parallel Metropolis sweeps.  Class (i, r) holds the
+// sites (c, i) with c = r mod m (componentwise); m divides every replica's
+// dims, so a bond (i -> j, n) joins class (i, r) only to class (j, r + n mod m).
+// A proper colouring of that quotient graph leaves no bond inside a colour.
+// Tries periods m up to 8 along each axis (and the full common period) and
+// keeps the colouring with the fewest colours (then the smallest period).
+struct Coloring {
+    int m[3] = {1, 1, 1};
+    int ncolors = 0;
+    std::vector<int> color;  // [na][m3][m2][m1]
+};
+
+Coloring color_classes(int na, int nbonds, const int32_t* bond, const std::vector<std::array<int, 3>>& dims)
+{
+    int g[3] = {0, 0, 0};
+    for (const auto& d : dims)
+        for (int a = 0; a < 3; a++) g[a] = std::gcd(g[a], d[a]);
+    std::vector<int> cand[3];
+    for (int a = 0; a < 3; a++)
+        for (int m = 1; m <= g[a]; m++)
+            if (g[a] % m == 0 && (m <= 8 || m == g[a])) cand[a].push_back(m);
+    Coloring best;
+    long long best_nodes = 0;
+    for (int m1 : cand[0])
+        for (int m2 : cand[1])
+            for (int m3 : cand[2]) {
+                const long long nodes = (long long)na * m1 * m2 * m3;
+                if (nodes > (1 << 20)) continue;
+                const int m[3] = {m1, m2, m3};
+                auto node = [&](int i, const int r[3]) { return r[0] + m1 * (r[1] + m2 * (r[2] + m3 * i)); };
+                std::vector<int> color(nodes, -1);
+                bool loop = false;
+                int nc = 0;
+                std::vector<char> used;
+                for (long long u = 0; u < nodes && !loop; u++) {
+                    const int i = (int)(u / ((long long)m1 * m2 * m3));
+                    long long t = u % ((long long)m1 * m2 * m3);
+                    const int r[3] = {(int)(t % m1), (int)((t / m1) % m2), (int)(t / ((long long)m1 * m2))};
+                    used.assign(nc + 1, 0);
+                    for (int b = 0; b < nbonds; b++) {
+                        const int32_t* bd = bond + 6 * b;
+                        if (bd[0] != i) continue;
+                        int rr[3];
+                        for (int a = 0; a < 3; a++) rr[a] = (((r[a] + bd[2 + a]) % m[a]) + m[a]) % m[a];
+                        const int v = node(bd[1], rr);
+                        if (v == u) {
+                            loop = true;
+                            break;
+                        }
+                        if (color[v] >= 0) used[color[v]] = 1;
+                    }
+                    int c = 0;
+                    while (c < nc && used[c]) c++;
+                    color[u] = c;
+                    nc = std::max(nc, c + 1);
+                }
+                if (loop) continue;
+                if (best.ncolors == 0 || nc < best.ncolors || (nc == best.ncolors && nodes < best_nodes)) {
+                    best.m[0] = m1;
+                    best.m[1] = m2;
+                    best.m[2] = m3;
+                    best.ncolors = nc;
+                    best.color = color;
+                    best_nodes = nodes;
+                }
+            }
+    return best;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -319,6 +389,10 @@ struct snh_system {
                            int window) = 0;
     virtual void corr_add_sample(double dt, double tol, int max_iters) = 0;
     virtual void corr_data(double* out, int64_t* nsamples) = 0;
+    virtual void step_local(const double* kT, int nprop, const int32_t* kinds, const double* mags,
+                            const double* weights, uint64_t seed, uint64_t sweep0, double nsweeps,
+                            double* dE_dS) = 0;
+    virtual void local_coloring(int32_t* m, int32_t* ncolors, int32_t* color) = 0;
     virtual void midpoint_record(double dt, double tol, int max_iters, int nsteps, int every) = 0;
     virtual void midpoint_adjoint(int nparams, const snh_spec* params, const double* dLdS, double* grad,
                                   double* dLdS0) = 0;
@@ -405,6 +479,16 @@ struct SystemImpl final : snh_system {
         double* post = nullptr;  // [T] window / (T n_contrib)
         long long nsamples = 0;
     } corr;
+    // Metropolis sweeps (snh_mc.hip): colour classes and tables, built at first use
+    struct Mc {
+        bool ready = false;
+        Coloring col;
+        int4* bn = nullptr;      // [nb]
+        int* color = nullptr;    // [na][m3][m2][m1]
+        double* kT = nullptr;    // [R]
+        double* acc = nullptr;   // [nblocks][4]
+        double* out = nullptr;   // [R][4]
+    } mc;
     // Ewald
     EwaldGeom EG{};
     void* M = nullptr;
@@ -1110,6 +1194,110 @@ struct SystemImpl final : snh_system {
                                 out[2 * dst + 1] = h[2 * src + 1];
                             }
         if (nsamples) *nsamples = corr.nsamples;
+    }
+
+    // Tables of the Metropolis sweeps: colour classes, and the cell offset
+    // and target sublattice of every gather slot.
+    void mc_prepare()
+    {
+        if (mc.ready) return;
+        SNH_REQUIRE(N == 0, SNH_ERR_UNSUPPORTED, "the device LocalSampler supports :dipole mode");
+        SNH_REQUIRE(!ewald, SNH_ERR_UNSUPPORTED,
+                    "the device LocalSampler does not support dipole-dipole (Ewald) interactions: they couple every "
+                    "pair of sites, so no two trials are independent");
+        std::vector<std::array<int, 3>> dims;
+        for (const auto& ri : reps) dims.push_back({ri.L1, ri.L2, ri.L3});
+        // a bond that maps a site onto itself (local_energy_change raises)
+        for (const auto& d : dims)
+            for (int b = 0; b < nb_in; b++) {
+                const int32_t* bd = bond_h.data() + 6 * b;
+                SNH_REQUIRE(!(bd[0] == bd[1] && bd[2] % d[0] == 0 && bd[3] % d[1] == 0 && bd[4] % d[2] == 0),
+                            SNH_ERR_INVALID, "Energy delta for self-interaction not supported");
+            }
+        mc.col = color_classes(na, nb_in, bond_h.data(), dims);
+        SNH_REQUIRE(mc.col.ncolors > 0, SNH_ERR_UNSUPPORTED,
+                    "no colouring of the sites has a period that divides every replica's dims");
+        std::vector<int4> bn(std::max<size_t>(bsrc_h.size(), 1), make_int4(0, 0, 0, -1));
+        for (size_t k = 0; k < bsrc_h.size(); k++) {
+            const int b = bsrc_h[k];
+            if (b < 0) continue;
+            const int32_t* bd = bond_h.data() + 6 * b;
+            bn[k] = make_int4(bd[2], bd[3], bd[4], bd[1]);
+        }
+        mc.bn = own(dupload(bn));
+        mc.color = own(dupload(mc.col.color));
+        mc.kT = own(dalloc<double>(R));
+        mc.acc = own(dalloc<double>(4 * (size_t)nblocks));
+        mc.out = own(dalloc<double>(4 * (size_t)R));
+        mc.ready = true;
+    }
+
+    // step!(sys, LocalSampler(; kT, nsweeps, propose)) for every replica:
+    // ceil(2 nsweeps) passes over the colour classes, each trying every site
+    // with probability 1/2 (the last pass: the remainder), numbered from
+    // 2 sweep0.
+    void step_local(const double* kT, int nprop, const int32_t* kinds, const double* mags, const double* weights,
+                    uint64_t seed, uint64_t sweep0, double nsweeps, double* dE_dS) override
+    {
+        SNH_REQUIRE(kT, SNH_ERR_INVALID, "null temperatures");
+        SNH_REQUIRE(nprop >= 1 && nprop <= 4 && kinds && weights, SNH_ERR_INVALID, "give 1 to 4 proposals");
+        SNH_REQUIRE(std::isfinite(nsweeps) && nsweeps >= 0 && nsweeps < 1e9, SNH_ERR_INVALID,
+                    "nsweeps must be in [0, 1e9)");
+        for (int r = 0; r < R; r++)
+            SNH_REQUIRE(std::isfinite(kT[r]) && kT[r] >= 0, SNH_ERR_INVALID, "kT must be finite and >= 0");
+        McArgs A{};
+        double wsum = 0;
+        for (int p = 0; p < nprop; p++) {
+            SNH_REQUIRE(kinds[p] >= 0 && kinds[p] <= 2, SNH_ERR_INVALID, "proposal kind must be 0, 1 or 2");
+            const double mg = mags ? mags[p] : 0.0;
+            SNH_REQUIRE(kinds[p] != 2 || mg >= 0, SNH_ERR_INVALID, "perturbation magnitude must be >= 0");
+            SNH_REQUIRE(std::isfinite(weights[p]) && weights[p] >= 0, SNH_ERR_INVALID,
+                        "proposal weights must be finite and >= 0");
+            A.kind[p] = kinds[p];
+            A.mag[p] = mg;
+            wsum += weights[p];
+        }
+        SNH_REQUIRE(wsum > 0, SNH_ERR_INVALID, "proposal weights must not all be zero");
+        double c = 0;
+        for (int p = 0; p < nprop; p++) {
+            c += weights[p] / wsum;
+            A.cum[p] = c;
+        }
+        mc_prepare();
+        A.nprop = nprop;
+        A.bn = mc.bn;
+        A.color = mc.color;
+        A.kT = mc.kT;
+        A.seed = seed;
+        A.m1 = mc.col.m[0];
+        A.m2 = mc.col.m[1];
+        A.m3 = mc.col.m[2];
+        SNH_HIP(hipMemcpyAsync(mc.kT, kT, sizeof(double) * R, hipMemcpyHostToDevice, st));
+        launch_fill_double(mc.acc, 4 * (long long)nblocks, 0.0, st);
+        const long long npass = (long long)std::ceil(2 * nsweeps);
+        for (long long k = 0; k < npass; k++) {
+            const double prob = k == npass - 1 ? (2 * nsweeps - (double)(npass - 1)) / 2 : 0.5;
+            for (int col = 0; col < mc.col.ncolors; col++) {
+                hipEvent_t e = tbegin();
+                launch_mc_color<T>(C, nblocks, A, X, col, 2 * sweep0 + (uint64_t)k, prob, mc.acc, st);
+                tend("local_color", e);
+            }
+        }
+        launch_mc_reduce(mc.acc, d_rblk, R, mc.out, st);
+        SNH_HIP(hipGetLastError());
+        std::vector<double> h(4 * (size_t)R);
+        SNH_HIP(hipMemcpyAsync(h.data(), mc.out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+        SNH_HIP(hipStreamSynchronize(st));
+        if (dE_dS) std::copy(h.begin(), h.end(), dE_dS);
+    }
+
+    void local_coloring(int32_t* m, int32_t* ncolors, int32_t* color) override
+    {
+        mc_prepare();
+        if (m)
+            for (int a = 0; a < 3; a++) m[a] = mc.col.m[a];
+        if (ncolors) *ncolors = mc.col.ncolors;
+        if (color) std::copy(mc.col.color.begin(), mc.col.color.end(), color);
     }
 
     void drop_graph()
@@ -1893,6 +2081,18 @@ extern "C" int snh_corr_add_sample(snh_system* sys, double dt, double tol, int32
 extern "C" int snh_corr_data(snh_system* sys, double* data, int64_t* nsamples)
 {
     return guarded(sys, [&] { sys->corr_data(data, nsamples); });
+}
+
+extern "C" int snh_step_local(snh_system* sys, const double* kT, int32_t nprop, const int32_t* kinds,
+                              const double* magnitudes, const double* weights, uint64_t seed, uint64_t sweep0,
+                              double nsweeps, double* dE_dS)
+{
+    return guarded(sys, [&] { sys->step_local(kT, nprop, kinds, magnitudes, weights, seed, sweep0, nsweeps, dE_dS); });
+}
+
+extern "C" int snh_local_coloring(snh_system* sys, int32_t* m, int32_t* ncolors, int32_t* color)
+{
+    return guarded(sys, [&] { sys->local_coloring(m, ncolors, color); });
 }
 
 extern "C" int snh_autotune(snh_system* sys, int32_t candidates, int32_t launches, double* best_ms)

@@ -174,6 +174,26 @@ void launch_corr_tdft(const void* in, void* out, long long nl, int T, int sign, 
 void launch_corr_product(const void* samp, void* P, int T, int nobs, int na, long long ncell, int a, int i, int b,
                          hipStream_t st);
 
+// Metropolis sweeps of :dipole replicas (snh_mc.hip).  A colour class is a
+// set (sublattice i, c1 mod m1, c2 mod m2, c3 mod m3) of sites no two of
+// which share a bond.
+struct McArgs {
+    const int4* bn;      // [nb] gather slot b: cell offset (x, y, z) and target sublattice w (< 0: padding)
+    const int* color;    // [na][m3][m2][m1] colour of every class
+    const double* kT;    // [R]
+    unsigned long long seed;
+    int m1, m2, m3;
+    int nprop;           // proposals of the mixture (1..4)
+    int kind[4];         // 0 uniform, 1 flip, 2 Gaussian perturbation
+    double mag[4];       // perturbation magnitude (kind 2)
+    double cum[4];       // cumulative probabilities
+};
+template <typename T>
+void launch_mc_color(const Couplings<T>& C, int nblocks, const McArgs& A, T* S, int color, unsigned long long P,
+                     double prob, double* acc, hipStream_t st);
+void launch_mc_reduce(const double* acc, const int* rblk, int R, double* out, hipStream_t st);
+void launch_fill_double(double* p, long long n, double v, hipStream_t st);
+
 // Device step counter of the noise stream: set, and advance by one.
 void launch_step_set(unsigned long long* p, unsigned long long v, hipStream_t st);
 void launch_step_advance(unsigned long long* p, hipStream_t st);

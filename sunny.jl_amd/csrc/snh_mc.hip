@@ -1,0 +1,236 @@
+// Metropolis updates of single dipoles (:dipole mode) on the device.
+//
+// Reference semantics (Sunny.jl 0.9.2):
+//   LocalSampler, step!                       src/MonteCarlo/Samplers.jl:133-153
+//   propose_uniform / propose_flip / propose_delta / @mix_proposals
+//                                             src/MonteCarlo/Samplers.jl:1-100
+//   randspin, flip, perturbed_spin            src/System/System.jl:440-468
+//   local_energy_change                       src/System/Interactions.jl:311-380
+//
+// The reference tries ceil(nsweeps * nsites) sites drawn at random, one at a
+// time.  Here a sweep is two passes; a pass visits the colour classes in
+// turn and tries each site of a class with probability 1/2.  The host colours
+// the classes (sublattice i, c mod m) so that no bond joins two sites of one
+// class, hence all sites of a class update in parallel with exactly the
+// energy change a sequential update would see.  Each trial is a Metropolis
+// step of the reference's proposal and acceptance rule, so the Boltzmann
+// distribution stays stationary, and a call makes nsweeps * nsites trials on
+// average.  The random subsets matter: with every site tried in a fixed
+// order, Metropolis flips of Ising-like spins move the domain walls of a
+// chain deterministically (moves with dE = 0 are always accepted) and the
+// sweeps are not ergodic.
+//
+// Draws of site s (index within its replica, Julia order) in pass P come
+// from Philox4x32-10 under key = seed + r * 0x9E3779B97F4A7C15:
+//   counter (s, 0x4D430000, P) -> four normals (Box-Muller; three are used)
+//   counter (s, 0x4D430001, P) -> u_acc, u_mix, u_try (word / 2^32)
+// disjoint from the Langevin stream (second counter word 0).
+#include "snh_internal.h"
+#include "snh_vec.h"
+
+namespace snh {
+
+constexpr uint32_t kMcTagNormals = 0x4D430000u;
+constexpr uint32_t kMcTagUniforms = 0x4D430001u;
+
+__device__ __forceinline__ int mc_wrap(int v, int L)
+{
+    const int m = v % L;
+    return m < 0 ? m + L : m;
+}
+
+// Classical Stevens energy in FP64 from the T-typed coefficients.
+template <typename T>
+__device__ __forceinline__ double stevens_energy_d(const Couplings<T>& C, int i, double x, double y, double z)
+{
+    const int km = ldc(C.kmax, i);
+    double c[28];
+#pragma unroll
+    for (int p = 0; p < 28; p++) c[p] = (double)ldc(C.stev + 28 * i, p);
+    double gx = 0, gy = 0, gz = 0, E = 0;
+    stevens_grad<double>(x, y, z, c, km, gx, gy, gz, &E);
+    return E;
+}
+
+__device__ __forceinline__ void quadrupole_d(double x, double y, double z, double Q[5])
+{
+    Q[0] = x * x - y * y;
+    Q[1] = z * x;
+    Q[2] = -x * x - y * y + 2 * z * z;
+    Q[3] = z * y;
+    Q[4] = 2 * y * x;
+}
+
+// Sum over a kBlock workgroup of four doubles per thread; thread 0 returns it.
+__device__ __forceinline__ void block_sum4(double v[4])
+{
+    __shared__ double red[kBlock / 64][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int q = 0; q < 4; q++) red[w][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; q++) {
+            double t = 0;
+            for (int k = 0; k < kBlock / 64; k++) t += red[k][q];
+            v[q] = t;
+        }
+}
+
+// One colour class of pass P over every replica: each site of the class is
+// tried with probability prob; sites of other classes return at once.
+// acc[4 * blk + q] += (dE, dS) of the block's accepted trials (each
+// workgroup owns its four entries).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mc_color(Couplings<T> C, McArgs A, T* __restrict__ S, int color,
+                                                     unsigned long long P, double prob, double* __restrict__ acc)
+{
+    const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
+    bool active = false;
+    if (sc.valid) {
+        const int cls = sc.c1 % A.m1 + A.m1 * (sc.c2 % A.m2 + A.m2 * (sc.c3 % A.m3 + A.m3 * sc.i));
+        active = ldc(A.color, cls) == color;
+    }
+    double v[4] = {0, 0, 0, 0};
+    if (active) {
+        const int i = sc.i;
+        const uint32_t total = C.total;
+        const unsigned long long key = A.seed + (unsigned long long)sc.r * 0x9E3779B97F4A7C15ull;
+        const uint32_t site = (uint32_t)sc.c + (uint32_t)sc.R.ncell * (uint32_t)i;
+        uint32_t un[4] = {site, kMcTagUniforms, (uint32_t)P, (uint32_t)(P >> 32)};
+        philox4x32(un, (uint32_t)key, (uint32_t)(key >> 32));
+        const double inv32 = 1.0 / 4294967296.0;
+        const double u_acc = (double)un[0] * inv32, u_mix = (double)un[1] * inv32, u_try = (double)un[2] * inv32;
+        if (u_try < prob) {
+            int k = A.nprop - 1;
+            for (int p = 0; p < A.nprop - 1; p++)
+                if (u_mix < A.cum[p]) {
+                    k = p;
+                    break;
+                }
+            const V3<T> s0 = {ldg(S, sc.g), ldg(S, sc.g + total), ldg(S, sc.g + 2 * total)};
+            const T kap = ldc(C.kappa, i);
+            V3<T> sn;
+            if (A.kind[k] == 1) {  // propose_flip
+                sn = {-s0.x, -s0.y, -s0.z};
+            } else {
+                uint32_t cn[4] = {site, kMcTagNormals, (uint32_t)P, (uint32_t)(P >> 32)};
+                philox4x32(cn, (uint32_t)key, (uint32_t)(key >> 32));
+                T n[4];
+                box_muller(cn[0], cn[1], n[0], n[1]);
+                box_muller(cn[2], cn[3], n[2], n[3]);
+                const double mag = A.mag[k];
+                if (A.kind[k] == 0 || __builtin_isinf(mag))  // randspin
+                    sn = normalize_dipole(V3<T>{n[0], n[1], n[2]}, kap);
+                else {  // perturbed_spin: S + mag kappa randn
+                    const T f = (T)(mag * (double)kap);
+                    sn = normalize_dipole(V3<T>{s0.x + f * n[0], s0.y + f * n[1], s0.z + f * n[2]}, kap);
+                }
+            }
+            // local_energy_change in FP64
+            const double ox = s0.x, oy = s0.y, oz = s0.z, nx = sn.x, ny = sn.y, nz = sn.z;
+            const double dx = nx - ox, dy = ny - oy, dz = nz - oz;
+            double dE = (double)ldc(C.zee, 3 * i) * dx + (double)ldc(C.zee, 3 * i + 1) * dy +
+                        (double)ldc(C.zee, 3 * i + 2) * dz;
+            dE += stevens_energy_d(C, i, nx, ny, nz) - stevens_energy_d(C, i, ox, oy, oz);
+            double Qn[5], Qo[5];
+            bool have_q = false;
+            const ReplicaInfo& R = sc.R;
+            for (int b = ldc(C.bstart, i); b < ldc(C.bstart, i + 1); b++) {
+                const int4 t = A.bn[b];
+                if (t.w < 0) continue;  // padding of the gather lists
+                const int c1 = mc_wrap(sc.c1 + t.x, R.L1), c2 = mc_wrap(sc.c2 + t.y, R.L2),
+                          c3 = mc_wrap(sc.c3 + t.z, R.L3);
+                const uint32_t gj = R.off +
+                                    (uint32_t)((c1 + C.G.w1) + R.P1 * ((c2 + C.G.w2) + R.P2 * (c3 + C.G.w3))) +
+                                    R.pcell * (uint32_t)t.w;
+                const double jx = ldg(S, gj), jy = ldg(S, gj + total), jz = ldg(S, gj + 2 * total);
+                const T* J = C.bJ + 9 * b;
+                dE += dx * ((double)ldc(J, 0) * jx + (double)ldc(J, 1) * jy + (double)ldc(J, 2) * jz) +
+                      dy * ((double)ldc(J, 3) * jx + (double)ldc(J, 4) * jy + (double)ldc(J, 5) * jz) +
+                      dz * ((double)ldc(J, 6) * jx + (double)ldc(J, 7) * jy + (double)ldc(J, 8) * jz);
+                if (ldc(C.bflag, b) & kBondBiquad) {
+                    if (!have_q) {
+                        quadrupole_d(nx, ny, nz, Qn);
+                        quadrupole_d(ox, oy, oz, Qo);
+                        have_q = true;
+                    }
+                    double Qj[5];
+                    quadrupole_d(jx, jy, jz, Qj);
+                    const T* Bq = C.bQ + 25 * b;
+                    for (int p = 0; p < 5; p++) {
+                        double w = 0;
+                        for (int q = 0; q < 5; q++) w += (double)ldc(Bq, 5 * p + q) * Qj[q];
+                        dE += (Qn[p] - Qo[p]) * w;
+                    }
+                }
+            }
+            const double kT = A.kT[sc.r];
+            const bool accept = kT == 0.0 ? dE <= 0.0 : u_acc <= exp(-dE / kT);
+            if (accept) {
+                stg(S, sc.g, sn.x);
+                stg(S, sc.g + total, sn.y);
+                stg(S, sc.g + 2 * total, sn.z);
+                for_each_ghost(sc, C.G, [&](uint32_t gi) {
+                    stg(S, gi, sn.x);
+                    stg(S, gi + total, sn.y);
+                    stg(S, gi + 2 * total, sn.z);
+                });
+                v[0] = dE;
+                v[1] = dx;
+                v[2] = dy;
+                v[3] = dz;
+            }
+        }
+    }
+    block_sum4(v);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; q++) acc[4 * (size_t)sc.blk + q] += v[q];
+}
+
+// out[4 r + q] = sum of acc[4 k + q] over the blocks k of replica r.
+__global__ __launch_bounds__(kBlock) void k_mc_reduce(const double* __restrict__ acc, const int* __restrict__ rblk,
+                                                      double* __restrict__ out)
+{
+    const int r = blockIdx.x;
+    double v[4] = {0, 0, 0, 0};
+    for (int k = rblk[r] + threadIdx.x; k < rblk[r + 1]; k += kBlock)
+        for (int q = 0; q < 4; q++) v[q] += acc[4 * (size_t)k + q];
+    block_sum4(v);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; q++) out[4 * r + q] = v[q];
+}
+
+__global__ void k_fill_double(double* __restrict__ p, long long n, double v)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+template <typename T>
+void launch_mc_color(const Couplings<T>& C, int nblocks, const McArgs& A, T* S, int color, unsigned long long P,
+                     double prob, double* acc, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mc_color<T>, dim3(nblocks), dim3(kBlock), 0, st, C, A, S, color, P, prob, acc);
+}
+
+void launch_mc_reduce(const double* acc, const int* rblk, int R, double* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mc_reduce, dim3(R), dim3(kBlock), 0, st, acc, rblk, out);
+}
+
+void launch_fill_double(double* p, long long n, double v, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+
+template void launch_mc_color<float>(const Couplings<float>&, int, const McArgs&, float*, int, unsigned long long,
+                                     double, double*, hipStream_t);
+template void launch_mc_color<double>(const Couplings<double>&, int, const McArgs&, double*, int, unsigned long long,
+                                      double, double*, hipStream_t);
+
+}  // namespace snh

@@ -230,4 +230,37 @@ function correlation_data(c::DeviceCorrelations)
     return data   # the layout of sc.data: zero-copy
 end
 
+# LocalSampler (src/MonteCarlo/Samplers.jl:133-153) on the device, :dipole
+# without Ewald.  The proposal function is translated to the device's
+# mixture: propose_uniform, propose_flip, the closure of propose_delta (its
+# captured `magnitude`) and the closure of @mix_proposals (its `cumprobs`
+# and `fns`).
+function device_proposal(f)
+    f === Sunny.propose_uniform && return [(Int32(0), 0.0, 1.0)]
+    f === Sunny.propose_flip && return [(Int32(1), 0.0, 1.0)]
+    hasfield(typeof(f), :magnitude) && return [(Int32(2), Float64(f.magnitude), 1.0)]
+    if hasfield(typeof(f), :fns) && hasfield(typeof(f), :cumprobs)
+        probs = diff([0.0; collect(f.cumprobs)])
+        return [(k, m, p * w) for (g, p) in zip(f.fns, probs) for (k, m, w) in device_proposal(g)]
+    end
+    error("the device LocalSampler supports propose_uniform, propose_flip, propose_delta and @mix_proposals of them")
+end
+
+function Sunny.step!(d::DeviceSystem{0}, sampler::Sunny.LocalSampler)
+    terms = device_proposal(sampler.propose)
+    kinds = Int32[t[1] for t in terms]
+    mags = Float64[t[2] for t in terms]
+    wts = Float64[t[3] for t in terms]
+    kT = Ref(sampler.kT)
+    out = zeros(4)
+    check(ccall((:snh_step_local, lib), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Int32, Ptr{Int32}, Ptr{Float64}, Ptr{Float64}, UInt64, UInt64, Float64,
+                 Ptr{Float64}),
+                d.handle, kT, length(kinds), kinds, mags, wts, d.seed, d.step, sampler.nsweeps, out), d.handle)
+    d.step += ceil(UInt64, sampler.nsweeps)
+    sampler.ΔE += out[1]
+    sampler.ΔS += Sunny.Vec3(out[2], out[3], out[4])
+    return
+end
+
 end

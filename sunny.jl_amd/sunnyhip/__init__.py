@@ -19,5 +19,7 @@ from .adjoint import midpoint_gradient, structure_factor, structure_factor_loss
 from .tempering import ParallelTempering
 from .correlations import SampledCorrelations, add_sample_
 from .optimize import MinimizationResult, minimize_energy_, suggest_timestep
+from .samplers import (LocalSampler, Proposal, local_coloring, mix_proposals, propose_delta, propose_flip,
+                       propose_uniform)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -26,7 +26,7 @@ EXPORTS = [
     "snh_step_langevin", "snh_step_midpoint", "snh_energy", "snh_energy_grad", "snh_synchronize",
     "snh_enable_timing", "snh_kernel_time", "snh_copy_calibration", "snh_query",
     "snh_midpoint_record", "snh_midpoint_adjoint", "snh_autotune", "snh_corr_init", "snh_corr_add_sample",
-    "snh_corr_data",
+    "snh_corr_data", "snh_step_local", "snh_local_coloring",
 ]
 
 
@@ -86,6 +86,9 @@ def lib():
                                     C.c_int32]
         L.snh_corr_add_sample.argtypes = [vp, C.c_double, C.c_double, C.c_int32]
         L.snh_corr_data.argtypes = [vp, dp, C.POINTER(C.c_int64)]
+        ip = C.POINTER(C.c_int32)
+        L.snh_step_local.argtypes = [vp, dp, C.c_int32, ip, dp, dp, C.c_uint64, C.c_uint64, C.c_double, dp]
+        L.snh_local_coloring.argtypes = [vp, ip, ip, ip]
         L.snh_midpoint_record.argtypes = [vp, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32]
         L.snh_midpoint_adjoint.argtypes = [vp, C.c_int32, C.POINTER(SnhSpec), dp, dp, dp]
         for f in EXPORTS:

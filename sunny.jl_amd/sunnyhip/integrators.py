@@ -1,5 +1,7 @@
 """`Langevin`, `ImplicitMidpoint`, `step_` (Sunny's `step!`), energies.
 
+`step_` also runs a `LocalSampler` (samplers.py).
+
 Constructors validate exactly as src/Integrators.jl:56-135; `step_` runs the
 stochastic Heun or implicit-midpoint step on the device
 (src/Integrators.jl:313-452).
@@ -12,6 +14,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import ConvergenceError  # noqa: F401  (re-exported)
+from .samplers import LocalSampler, step_local_
 from .system import System
 
 
@@ -67,7 +70,10 @@ def step_(sys, integrator, *, max_iters=100, noise=None, nsteps=1, kTs=None):
     noise: raw standard-normal draws for one Langevin step, laid out like the
     state (the draws of randn!(rng, xi) before scaling); None uses the device
     Philox stream.  kTs: per-replica temperatures for a ReplicaBatch.
-    Returns the midpoint iteration counts of the last step (ImplicitMidpoint)."""
+    Returns the midpoint iteration counts of the last step (ImplicitMidpoint),
+    or the [R][4] sums (dE, dS) of accepted trials (LocalSampler)."""
+    if isinstance(integrator, LocalSampler):
+        return step_local_(sys, integrator, nsteps=nsteps, kTs=kTs)
     _check_dt(integrator)
     h = sys._device()
     R = getattr(sys, "nreplicas", 1)

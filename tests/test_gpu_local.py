@@ -1,0 +1,207 @@
+"""GPU parity of the device LocalSampler against the oracle.
+
+The device runs colour-class passes of Metropolis trials with Philox draws;
+the oracle's restatement (oracle/sunny_oracle.c: so_local_sweeps, whose
+local_energy_change and sampling are pinned in test_oracle_local.py) makes
+the same trials with the same draws, so FP64 states agree to rounding
+(<= 1e-12 of kappa) and FP32 states to 2e-5 of kappa.  Known answers at
+sizes the oracle would take minutes for: a paramagnet's Langevin function
+and periodic Ising chains with flip proposals.  All calls go through
+libsunnyhip's C ABI.
+"""
+import numpy as np
+import pytest
+
+import sunnyhip as sh
+from oracle import oracle as O
+
+from helpers import announce, coloring_is_proper, energy_scale, langevin_function
+from test_gpu_dipole import aniso_model
+from test_oracle_local import ising_chains
+
+pytestmark = pytest.mark.gpu
+
+TOL = {64: 1e-12, 32: 2e-5}
+
+PROPOSALS = {
+    "uniform": sh.propose_uniform,
+    "flip": sh.propose_flip,
+    "delta": sh.propose_delta(0.3),
+    "mix": sh.mix_proposals(0.4, sh.propose_flip, 0.6, sh.propose_delta(0.2)),
+}
+
+
+def models(precision):
+    yield "cubic4", sh.workloads.cubic_model(L=4, seed=3, precision=precision)
+    yield "cubic_ragged", sh.workloads.cubic_model(dims=(3, 5, 2), seed=4, precision=precision)
+    yield "pyro2", sh.workloads.pyrochlore_model(L=2, seed=5, precision=precision)
+    yield "pyro_ragged", sh.workloads.pyrochlore_model(dims=(1, 2, 3), seed=6, precision=precision)
+    s = sh.workloads.cubic_model(dims=(2, 2, 3), seed=8, precision=precision)
+    sh.set_exchange_(s, 0.3, sh.Bond(0, 0, (2, 1, 0)), propagate=[sh.Bond(0, 0, (2, 1, 0))])
+    sh.set_exchange_(s, -0.2, sh.Bond(0, 0, (0, 3, 4)), propagate=[sh.Bond(0, 0, (0, 3, 4))])
+    sh.randomize_spins_(s)
+    yield "cubic_long_bonds", s
+    yield "diamond_aniso", aniso_model(precision)
+
+
+def oracle_props(p):
+    return [(k, m, w) for k, m, w in p.terms]
+
+
+def run_oracle(s, S, kT, prop, key, sweep0, nsweeps):
+    m, nc, col = sh.local_coloring(s)
+    return O.Oracle(s.spec()).local_sweeps(S, m, col.reshape(-1), nc, oracle_props(prop), kT, key, sweep0, nsweeps)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_sweeps_match_oracle(precision, layout):
+    for name, s in models(precision):
+        s.layout = layout
+        kap = np.repeat(s.kappas, s.ncells)
+        for pname, prop in PROPOSALS.items():
+            if precision == 32 and pname != "flip":
+                # FP32 draws and proposals differ from the oracle's by rounding, and a
+                # trial near its acceptance threshold could then go the other way
+                continue
+            announce(f"local sweeps {name} {pname} fp{precision} layout {layout}")
+            S0 = s.dipoles_flat()
+            E0 = sh.energy(s)
+            key = O.replica_key(s.noise_seed, 0)
+            sweep0 = s.step_counter
+            nsweeps = 1.5 if pname == "mix" else 2.0
+            S_ref, acc = run_oracle(s, S0, 0.7, prop, key, sweep0, nsweeps)
+            sampler = sh.LocalSampler(kT=0.7, nsweeps=nsweeps, propose=prop)
+            sh.step_(s, sampler)
+            S_dev = s.dipoles_flat()
+            assert np.max(np.abs(S_dev - S_ref)) <= TOL[precision] * kap.max(), (name, pname)
+            assert np.sum(np.any(np.abs(S_dev - S0) > 1e-9, axis=1)) > 0, (name, pname)
+            scale = energy_scale(s)
+            assert abs(sampler.dE - acc[0]) <= 10 * TOL[precision] * scale, (name, pname, sampler.dE, acc[0])
+            np.testing.assert_allclose(sampler.dS, acc[1:], atol=10 * TOL[precision] * s.nsites * kap.max())
+            # the accumulated change is the change of the energy
+            assert abs(sh.energy(s) - E0 - sampler.dE) <= 10 * TOL[precision] * scale, (name, pname)
+
+
+def test_zero_temperature_descends():
+    s = sh.workloads.pyrochlore_model(L=2, seed=9, precision=64)
+    announce("local sweeps kT=0 pyro2")
+    E = [sh.energy(s)]
+    sampler = sh.LocalSampler(kT=0.0, propose=sh.propose_delta(0.5))
+    key = O.replica_key(s.noise_seed, 0)
+    S_ref, _ = run_oracle(s, s.dipoles_flat(), 0.0, sampler.propose, key, s.step_counter, 3.0)
+    for _ in range(3):
+        sh.step_(s, sampler)
+        E.append(sh.energy(s))
+    assert all(b <= a + 1e-12 for a, b in zip(E, E[1:])), E
+    assert np.max(np.abs(s.dipoles_flat() - S_ref)) <= 1e-12 * 1.5
+
+
+def test_replica_batch_matches_oracle():
+    dims = [(2, 2, 2), (2, 2, 4), (4, 2, 2)]
+    systems = [sh.workloads.pyrochlore_model(dims=d, seed=30 + r, precision=64) for r, d in enumerate(dims)]
+    announce(f"local sweeps replica batch pyrochlore {dims}")
+    batch = sh.ReplicaBatch(systems, precision=64, layout=1)
+    kTs = np.array([0.2, 0.6, 1.5])
+    refs = [s.dipoles_flat() for s in systems]
+    prop = sh.mix_proposals(0.5, sh.propose_uniform, 0.5, sh.propose_delta(0.25))
+    sampler = sh.LocalSampler(kT=0.0, nsweeps=2, propose=prop)
+    m, nc, col = sh.local_coloring(batch)
+    for d in dims:
+        assert coloring_is_proper(systems[0].spec(), d, m, col)
+    sh.step_(batch, sampler, kTs=kTs)
+    batch.sync_to_systems()
+    for r, s in enumerate(systems):
+        o = O.Oracle(s.spec())
+        S, acc = o.local_sweeps(refs[r], m, col.reshape(-1), nc, oracle_props(prop), kTs[r],
+                                O.replica_key(batch.noise_seed, r), 0, 2.0)
+        assert np.max(np.abs(s.dipoles_flat() - S)) <= 1e-12 * 1.5, r
+        assert abs(sampler.dE[r] - acc[0]) <= 1e-9 * energy_scale(s), r
+
+
+def test_colorings_are_proper():
+    for name, s in models(64):
+        announce(f"local colouring {name}")
+        m, nc, col = sh.local_coloring(s)
+        assert coloring_is_proper(s.spec(), s.dims, m, col), name
+        assert 2 <= nc <= 4 * s.crystal.natoms + 8, (name, nc)
+
+
+def test_langevin_after_sweeps_sees_updated_images():
+    """Ghost images written by accepted trials feed the next Langevin step."""
+    s = sh.workloads.cubic_model(L=4, seed=13, precision=64)
+    s.layout = 1
+    announce("local sweeps then langevin, ghost layout")
+    sh.step_(s, sh.LocalSampler(kT=0.5, propose=sh.propose_flip))
+    S = s.dipoles_flat()
+    o = O.Oracle(s.spec())
+    noise = np.random.default_rng(2).standard_normal((s.nsites, 3))
+    integ = sh.Langevin(0.02, damping=0.1, kT=0.3)
+    S_ref = o.step_langevin(S=S, noise=noise, dt=integ.dt, damping=integ.damping, kT=integ.kT)
+    sh.step_(s, integ, noise=noise)
+    assert np.max(np.abs(s.dipoles_flat() - S_ref)) <= 1e-11
+
+
+def test_unsupported_and_invalid():
+    announce("local sampler errors")
+    sun = sh.workloads.triangular_sun_model(L1=3, L2=3, seed=1, precision=64)
+    with pytest.raises(ValueError, match=":dipole"):
+        sh.step_(sun, sh.LocalSampler(kT=1.0))
+    ew = sh.workloads.pyrochlore_model(L=1, seed=2, precision=64, dipolar=True)
+    with pytest.raises(ValueError, match="Ewald"):
+        sh.step_(ew, sh.LocalSampler(kT=1.0))
+    selfint = sh.workloads.cubic_model(dims=(1, 2, 2), seed=3, precision=64)
+    with pytest.raises(ValueError, match="self-interaction"):
+        sh.step_(selfint, sh.LocalSampler(kT=1.0))
+    s = sh.workloads.cubic_model(L=2, seed=4, precision=64)
+    bad = sh.LocalSampler(kT=1.0)
+    bad.propose = sh.Proposal([(3, 0.0, 1.0)], "bad")
+    with pytest.raises(ValueError, match="kind"):
+        sh.step_(s, bad)
+    with pytest.raises(ValueError, match="kT"):
+        sh.step_(s, sh.LocalSampler(kT=-1.0))
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_paramagnet_langevin_function(precision):
+    s = sh.workloads.cubic_model(L=16, J=0.0, D=0.0, seed=5, precision=precision)
+    sh.set_field_(s, [0.0, 0.3, 0.4])
+    announce(f"local sampler paramagnet 16^3 fp{precision}")
+    spec = s.spec()
+    kap = spec["kappa"][0]
+    h = np.asarray(spec["g"][0]).reshape(3, 3).T @ np.asarray(spec["field"][0])
+    hn = h / np.linalg.norm(h)
+    kT = 0.6
+    sampler = sh.LocalSampler(kT=kT, propose=sh.propose_uniform)
+    sh.step_(s, sampler, nsteps=20)
+    vals = []
+    for _ in range(200):
+        sh.step_(s, sampler)
+        vals.append(np.mean(s.dipoles_flat() @ hn))
+    blocks = np.asarray(vals).reshape(20, -1).mean(1)
+    err = blocks.std(ddof=1) / np.sqrt(20)
+    exact = -kap * langevin_function(kap * np.linalg.norm(h) / kT)
+    assert abs(np.mean(vals) - exact) < max(4 * err, 1e-3), (np.mean(vals), exact, err)
+
+
+def test_ising_chains_with_flips():
+    L = 16
+    s = ising_chains(L, dims=(L, 8, 8), precision=32)
+    announce("local sampler Ising chains 16x8x8 fp32")
+    kap = s.spec()["kappa"][0]
+    rng = np.random.default_rng(4)
+    S = np.zeros((s.nsites, 3))
+    S[:, 2] = kap * rng.choice([-1.0, 1.0], s.nsites)
+    s.set_state_flat(dip=S)
+    kT = 1.0
+    sampler = sh.LocalSampler(kT=kT, propose=sh.propose_flip)
+    sh.step_(s, sampler, nsteps=200)
+    vals = []
+    for _ in range(1000):
+        sh.step_(s, sampler)
+        vals.append(sh.energy(s) / s.nsites)
+    vals = np.asarray(vals)
+    blocks = vals.reshape(20, -1).mean(1)
+    err = blocks.std(ddof=1) / np.sqrt(20)
+    t = np.tanh(kap**2 / kT)
+    exact = -kap**2 * (t + t ** (L - 1)) / (1 + t**L)
+    assert abs(vals.mean() - exact) < max(4 * err, 3e-3), (vals.mean(), exact, err)
