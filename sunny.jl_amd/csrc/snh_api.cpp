@@ -483,11 +483,14 @@ struct SystemImpl final : snh_system {
     struct Mc {
         bool ready = false;
         Coloring col;
-        int4* bn = nullptr;      // [nb]
-        int* color = nullptr;    // [na][m3][m2][m1]
-        double* kT = nullptr;    // [R]
-        double* acc = nullptr;   // [nblocks][4]
-        double* out = nullptr;   // [R][4]
+        bool wide = false;            // some |n| exceeds a replica's dims
+        std::vector<int> coff;        // [ncolors + 1] work items of each colour
+        int* bn = nullptr;            // [nb][4]
+        McWork* work = nullptr;       // colour-major, then replica, then class
+        int* start = nullptr;         // [ncolors][R + 1] work ranges of each replica
+        double* kT = nullptr;         // [R]
+        double* acc = nullptr;        // [work][4]
+        double* out = nullptr;        // [R][4]
     } mc;
     // Ewald
     EwaldGeom EG{};
@@ -1217,17 +1220,47 @@ struct SystemImpl final : snh_system {
         mc.col = color_classes(na, nb_in, bond_h.data(), dims);
         SNH_REQUIRE(mc.col.ncolors > 0, SNH_ERR_UNSUPPORTED,
                     "no colouring of the sites has a period that divides every replica's dims");
-        std::vector<int4> bn(std::max<size_t>(bsrc_h.size(), 1), make_int4(0, 0, 0, -1));
-        for (size_t k = 0; k < bsrc_h.size(); k++) {
-            const int b = bsrc_h[k];
-            if (b < 0) continue;
-            const int32_t* bd = bond_h.data() + 6 * b;
-            bn[k] = make_int4(bd[2], bd[3], bd[4], bd[1]);
+        // gather slots: cell offset and target; padding is the site itself (zero coupling)
+        std::vector<int> bn(4 * std::max<size_t>(bsrc_h.size(), 1), 0);
+        {
+            std::vector<int> bst(na + 1, 0);  // slot ranges per sublattice, as uploaded in C.bstart
+            SNH_HIP(hipMemcpy(bst.data(), C.bstart, sizeof(int) * (na + 1), hipMemcpyDeviceToHost));
+            for (int i = 0; i < na; i++)
+                for (int k = bst[i]; k < bst[i + 1]; k++) {
+                    const int b = bsrc_h[k];
+                    const int32_t* bd = bond_h.data() + 6 * (b < 0 ? 0 : b);
+                    const int v[4] = {b < 0 ? 0 : bd[2], b < 0 ? 0 : bd[3], b < 0 ? 0 : bd[4], b < 0 ? i : bd[1]};
+                    for (int a = 0; a < 4; a++) bn[4 * (size_t)k + a] = v[a];
+                }
         }
+        mc.wide = false;
+        for (const auto& d : dims)
+            for (int b = 0; b < nb_in; b++)
+                for (int a = 0; a < 3; a++) mc.wide |= std::abs(bond_h[6 * b + 2 + a]) > d[a];
+        // work lists: per colour, per replica, kBlock sites of one class at a time
+        const int M = mc.col.m[0] * mc.col.m[1] * mc.col.m[2];
+        const int nc = mc.col.ncolors;
+        std::vector<McWork> work;
+        std::vector<int> start((size_t)nc * (R + 1));
+        mc.coff.assign(nc + 1, 0);
+        for (int c = 0; c < nc; c++) {
+            mc.coff[c] = (int)work.size();
+            for (int r = 0; r < R; r++) {
+                start[(size_t)c * (R + 1) + r] = (int)work.size();
+                const int nq = reps[r].ncell / M;
+                for (int cls = 0; cls < na * M; cls++) {
+                    if (mc.col.color[cls] != c) continue;
+                    for (int q0 = 0; q0 < nq; q0 += kBlock) work.push_back({r, cls, q0, 0});
+                }
+            }
+            start[(size_t)c * (R + 1) + R] = (int)work.size();
+        }
+        mc.coff[nc] = (int)work.size();
         mc.bn = own(dupload(bn));
-        mc.color = own(dupload(mc.col.color));
+        mc.work = own(dupload(work));
+        mc.start = own(dupload(start));
         mc.kT = own(dalloc<double>(R));
-        mc.acc = own(dalloc<double>(4 * (size_t)nblocks));
+        mc.acc = own(dalloc<double>(4 * std::max<size_t>(work.size(), 1)));
         mc.out = own(dalloc<double>(4 * (size_t)R));
         mc.ready = true;
     }
@@ -1266,24 +1299,25 @@ struct SystemImpl final : snh_system {
         mc_prepare();
         A.nprop = nprop;
         A.bn = mc.bn;
-        A.color = mc.color;
         A.kT = mc.kT;
+        A.wide = mc.wide ? 1 : 0;
         A.seed = seed;
         A.m1 = mc.col.m[0];
         A.m2 = mc.col.m[1];
         A.m3 = mc.col.m[2];
         SNH_HIP(hipMemcpyAsync(mc.kT, kT, sizeof(double) * R, hipMemcpyHostToDevice, st));
-        launch_fill_double(mc.acc, 4 * (long long)nblocks, 0.0, st);
+        launch_fill_double(mc.acc, 4 * (long long)std::max(mc.coff.back(), 1), 0.0, st);
         const long long npass = (long long)std::ceil(2 * nsweeps);
         for (long long k = 0; k < npass; k++) {
             const double prob = k == npass - 1 ? (2 * nsweeps - (double)(npass - 1)) / 2 : 0.5;
             for (int col = 0; col < mc.col.ncolors; col++) {
                 hipEvent_t e = tbegin();
-                launch_mc_color<T>(C, nblocks, A, X, col, 2 * sweep0 + (uint64_t)k, prob, mc.acc, st);
+                launch_mc_color<T>(C, A, X, mc.work + mc.coff[col], mc.coff[col + 1] - mc.coff[col],
+                                   2 * sweep0 + (uint64_t)k, prob, mc.acc + 4 * (size_t)mc.coff[col], st);
                 tend("local_color", e);
             }
         }
-        launch_mc_reduce(mc.acc, d_rblk, R, mc.out, st);
+        launch_mc_reduce(mc.acc, mc.start, mc.col.ncolors, R, mc.out, st);
         SNH_HIP(hipGetLastError());
         std::vector<double> h(4 * (size_t)R);
         SNH_HIP(hipMemcpyAsync(h.data(), mc.out, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));

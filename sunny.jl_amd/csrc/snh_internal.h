@@ -178,20 +178,24 @@ void launch_corr_product(const void* samp, void* P, int T, int nobs, int na, lon
 // set (sublattice i, c1 mod m1, c2 mod m2, c3 mod m3) of sites no two of
 // which share a bond.
 struct McArgs {
-    const int4* bn;      // [nb] gather slot b: cell offset (x, y, z) and target sublattice w (< 0: padding)
-    const int* color;    // [na][m3][m2][m1] colour of every class
+    const int* bn;       // [nb][4] gather slot b: cell offset n1, n2, n3 and target sublattice (padding: the site itself)
     const double* kT;    // [R]
     unsigned long long seed;
     int m1, m2, m3;
+    int wide;            // some bond offset exceeds a replica's dims: wrap by division
     int nprop;           // proposals of the mixture (1..4)
     int kind[4];         // 0 uniform, 1 flip, 2 Gaussian perturbation
     double mag[4];       // perturbation magnitude (kind 2)
     double cum[4];       // cumulative probabilities
 };
+// kBlock sites q0.. of class cls = rho1 + m1 (rho2 + m2 (rho3 + m3 i)) of replica r
+struct McWork {
+    int r, cls, q0, pad;
+};
 template <typename T>
-void launch_mc_color(const Couplings<T>& C, int nblocks, const McArgs& A, T* S, int color, unsigned long long P,
-                     double prob, double* acc, hipStream_t st);
-void launch_mc_reduce(const double* acc, const int* rblk, int R, double* out, hipStream_t st);
+void launch_mc_color(const Couplings<T>& C, const McArgs& A, T* S, const McWork* work, int nwork,
+                     unsigned long long P, double prob, double* acc, hipStream_t st);
+void launch_mc_reduce(const double* acc, const int* start, int ncolors, int R, double* out, hipStream_t st);
 void launch_fill_double(double* p, long long n, double v, hipStream_t st);
 
 // Device step counter of the noise stream: set, and advance by one.

@@ -81,125 +81,175 @@ __device__ __forceinline__ void block_sum4(double v[4])
         }
 }
 
-// One colour class of pass P over every replica: each site of the class is
-// tried with probability prob; sites of other classes return at once.
-// acc[4 * blk + q] += (dE, dS) of the block's accepted trials (each
-// workgroup owns its four entries).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mc_color(Couplings<T> C, McArgs A, T* __restrict__ S, int color,
-                                                     unsigned long long P, double prob, double* __restrict__ acc)
+// Periodic wrap of c + n for 0 <= c < L: without a division when |n| <= L
+// (every bond of the handle, unless `wide`).
+__device__ __forceinline__ int mc_wrap_to(int v, int L, bool wide)
 {
-    const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
-    bool active = false;
-    if (sc.valid) {
-        const int cls = sc.c1 % A.m1 + A.m1 * (sc.c2 % A.m2 + A.m2 * (sc.c3 % A.m3 + A.m3 * sc.i));
-        active = ldc(A.color, cls) == color;
-    }
+    if (wide) return mc_wrap(v, L);
+    v = v >= L ? v - L : v;
+    return v < 0 ? v + L : v;
+}
+
+// Site (cell c1, c2, c3; sublattice i) of replica R as the gather kernels
+// see it: padded index and whether it has ghost images.
+__device__ __forceinline__ SiteCoord mc_site(const ReplicaInfo& R, const Ghost& G, int i, int c1, int c2, int c3,
+                                             bool valid)
+{
+    SiteCoord sc;
+    sc.blk = 0;
+    sc.r = 0;
+    sc.i = i;
+    sc.R = R;
+    sc.c1 = c1;
+    sc.c2 = c2;
+    sc.c3 = c3;
+    sc.c = c1 + R.L1 * (c2 + R.L2 * c3);
+    sc.valid = valid;
+    sc.g = R.off + (uint32_t)((c1 + G.w1) + R.P1 * ((c2 + G.w2) + R.P2 * (c3 + G.w3))) + R.pcell * (uint32_t)i;
+    sc.edge = (c1 < G.w1) | (c1 >= R.L1 - G.w1) | (c2 < G.w2) | (c2 >= R.L2 - G.w2) | (c3 < G.w3) |
+              (c3 >= R.L3 - G.w3);
+    return sc;
+}
+
+// One colour of pass P: workgroup b takes work[b] = kBlock consecutive sites
+// q of one class (replica r, sublattice i, cell residue rho mod m), cell
+// c = rho + m * (q1, q2, q3); each site is tried with probability prob.
+// acc[4 b + q] += (dE, dS) of the workgroup's accepted trials.  Neighbour
+// states are gathered G bonds at a time (the bond lists are padded with
+// zero couplings of the site to itself to a multiple of G).
+template <typename T, int G>
+__global__ __launch_bounds__(kBlock) void k_mc_color(Couplings<T> C, McArgs A, T* __restrict__ S,
+                                                     const McWork* __restrict__ work, unsigned long long P,
+                                                     double prob, double* __restrict__ acc)
+{
+    const McWork W = work[blockIdx.x];
+    const ReplicaInfo R = C.reps[W.r];
+    const int M = A.m1 * A.m2 * A.m3;
+    const int i = W.cls / M;
+    int rho = W.cls - i * M;
+    const int r1 = rho % A.m1;
+    rho /= A.m1;
+    const int r2 = rho % A.m2, r3 = rho / A.m2;
+    const int Q1 = R.L1 / A.m1, Q2 = R.L2 / A.m2, nq = R.ncell / M;
+    const int q = W.q0 + (int)threadIdx.x;
+    const int qq = q < nq ? q : 0;
+    const int t = qq / Q1;
+    const int q1 = qq - t * Q1, q3 = t / Q2, q2 = t - q3 * Q2;
+    const SiteCoord sc = mc_site(R, C.G, i, r1 + A.m1 * q1, r2 + A.m2 * q2, r3 + A.m3 * q3, q < nq);
+    const bool wide = A.wide != 0;
     double v[4] = {0, 0, 0, 0};
-    if (active) {
-        const int i = sc.i;
-        const uint32_t total = C.total;
-        const unsigned long long key = A.seed + (unsigned long long)sc.r * 0x9E3779B97F4A7C15ull;
-        const uint32_t site = (uint32_t)sc.c + (uint32_t)sc.R.ncell * (uint32_t)i;
-        uint32_t un[4] = {site, kMcTagUniforms, (uint32_t)P, (uint32_t)(P >> 32)};
-        philox4x32(un, (uint32_t)key, (uint32_t)(key >> 32));
-        const double inv32 = 1.0 / 4294967296.0;
-        const double u_acc = (double)un[0] * inv32, u_mix = (double)un[1] * inv32, u_try = (double)un[2] * inv32;
-        if (u_try < prob) {
-            int k = A.nprop - 1;
-            for (int p = 0; p < A.nprop - 1; p++)
-                if (u_mix < A.cum[p]) {
-                    k = p;
-                    break;
-                }
-            const V3<T> s0 = {ldg(S, sc.g), ldg(S, sc.g + total), ldg(S, sc.g + 2 * total)};
-            const T kap = ldc(C.kappa, i);
-            V3<T> sn;
-            if (A.kind[k] == 1) {  // propose_flip
-                sn = {-s0.x, -s0.y, -s0.z};
-            } else {
-                uint32_t cn[4] = {site, kMcTagNormals, (uint32_t)P, (uint32_t)(P >> 32)};
-                philox4x32(cn, (uint32_t)key, (uint32_t)(key >> 32));
-                T n[4];
-                box_muller(cn[0], cn[1], n[0], n[1]);
-                box_muller(cn[2], cn[3], n[2], n[3]);
-                const double mag = A.mag[k];
-                if (A.kind[k] == 0 || __builtin_isinf(mag))  // randspin
-                    sn = normalize_dipole(V3<T>{n[0], n[1], n[2]}, kap);
-                else {  // perturbed_spin: S + mag kappa randn
-                    const T f = (T)(mag * (double)kap);
-                    sn = normalize_dipole(V3<T>{s0.x + f * n[0], s0.y + f * n[1], s0.z + f * n[2]}, kap);
-                }
+    const uint32_t total = C.total;
+    const unsigned long long key = A.seed + (unsigned long long)W.r * 0x9E3779B97F4A7C15ull;
+    const uint32_t site = (uint32_t)sc.c + (uint32_t)R.ncell * (uint32_t)i;
+    uint32_t un[4] = {site, kMcTagUniforms, (uint32_t)P, (uint32_t)(P >> 32)};
+    philox4x32(un, (uint32_t)key, (uint32_t)(key >> 32));
+    const double inv32 = 1.0 / 4294967296.0;
+    const double u_acc = (double)un[0] * inv32, u_mix = (double)un[1] * inv32, u_try = (double)un[2] * inv32;
+    if (sc.valid && u_try < prob) {
+        int k = A.nprop - 1;
+        for (int p = 0; p < A.nprop - 1; p++)
+            if (u_mix < A.cum[p]) {
+                k = p;
+                break;
             }
-            // local_energy_change in FP64
-            const double ox = s0.x, oy = s0.y, oz = s0.z, nx = sn.x, ny = sn.y, nz = sn.z;
-            const double dx = nx - ox, dy = ny - oy, dz = nz - oz;
-            double dE = (double)ldc(C.zee, 3 * i) * dx + (double)ldc(C.zee, 3 * i + 1) * dy +
-                        (double)ldc(C.zee, 3 * i + 2) * dz;
-            dE += stevens_energy_d(C, i, nx, ny, nz) - stevens_energy_d(C, i, ox, oy, oz);
-            double Qn[5], Qo[5];
-            bool have_q = false;
-            const ReplicaInfo& R = sc.R;
-            for (int b = ldc(C.bstart, i); b < ldc(C.bstart, i + 1); b++) {
-                const int4 t = A.bn[b];
-                if (t.w < 0) continue;  // padding of the gather lists
-                const int c1 = mc_wrap(sc.c1 + t.x, R.L1), c2 = mc_wrap(sc.c2 + t.y, R.L2),
-                          c3 = mc_wrap(sc.c3 + t.z, R.L3);
+        const V3<T> s0 = {ldg(S, sc.g), ldg(S, sc.g + total), ldg(S, sc.g + 2 * total)};
+        const T kap = ldc(C.kappa, i);
+        V3<T> sn;
+        if (A.kind[k] == 1) {  // propose_flip
+            sn = {-s0.x, -s0.y, -s0.z};
+        } else {
+            uint32_t cn[4] = {site, kMcTagNormals, (uint32_t)P, (uint32_t)(P >> 32)};
+            philox4x32(cn, (uint32_t)key, (uint32_t)(key >> 32));
+            T n[4];
+            box_muller(cn[0], cn[1], n[0], n[1]);
+            box_muller(cn[2], cn[3], n[2], n[3]);
+            const double mag = A.mag[k];
+            if (A.kind[k] == 0 || __builtin_isinf(mag))  // randspin
+                sn = normalize_dipole(V3<T>{n[0], n[1], n[2]}, kap);
+            else {  // perturbed_spin: S + mag kappa randn
+                const T f = (T)(mag * (double)kap);
+                sn = normalize_dipole(V3<T>{s0.x + f * n[0], s0.y + f * n[1], s0.z + f * n[2]}, kap);
+            }
+        }
+        // local_energy_change in FP64
+        const double ox = s0.x, oy = s0.y, oz = s0.z, nx = sn.x, ny = sn.y, nz = sn.z;
+        const double dx = nx - ox, dy = ny - oy, dz = nz - oz;
+        double dE = (double)ldc(C.zee, 3 * i) * dx + (double)ldc(C.zee, 3 * i + 1) * dy +
+                    (double)ldc(C.zee, 3 * i + 2) * dz;
+        dE += stevens_energy_d(C, i, nx, ny, nz) - stevens_energy_d(C, i, ox, oy, oz);
+        double Qn[5], Qo[5];
+        quadrupole_d(nx, ny, nz, Qn);
+        quadrupole_d(ox, oy, oz, Qo);
+        const int b0 = ldc(C.bstart, i), b1 = ldc(C.bstart, i + 1);
+        for (int c0 = b0; c0 < b1; c0 += G) {
+            T jx[G], jy[G], jz[G];
+#pragma unroll
+            for (int kk = 0; kk < G; kk++) {
+                const int* tb = A.bn + 4 * (c0 + kk);
+                const int a1 = mc_wrap_to(sc.c1 + ldc(tb, 0), R.L1, wide),
+                          a2 = mc_wrap_to(sc.c2 + ldc(tb, 1), R.L2, wide),
+                          a3 = mc_wrap_to(sc.c3 + ldc(tb, 2), R.L3, wide);
                 const uint32_t gj = R.off +
-                                    (uint32_t)((c1 + C.G.w1) + R.P1 * ((c2 + C.G.w2) + R.P2 * (c3 + C.G.w3))) +
-                                    R.pcell * (uint32_t)t.w;
-                const double jx = ldg(S, gj), jy = ldg(S, gj + total), jz = ldg(S, gj + 2 * total);
+                                    (uint32_t)((a1 + C.G.w1) + R.P1 * ((a2 + C.G.w2) + R.P2 * (a3 + C.G.w3))) +
+                                    R.pcell * (uint32_t)ldc(tb, 3);
+                jx[kk] = ldg(S, gj);
+                jy[kk] = ldg(S, gj + total);
+                jz[kk] = ldg(S, gj + 2 * total);
+            }
+#pragma unroll
+            for (int kk = 0; kk < G; kk++) {
+                const int b = c0 + kk;
+                const double x = jx[kk], y = jy[kk], z = jz[kk];
                 const T* J = C.bJ + 9 * b;
-                dE += dx * ((double)ldc(J, 0) * jx + (double)ldc(J, 1) * jy + (double)ldc(J, 2) * jz) +
-                      dy * ((double)ldc(J, 3) * jx + (double)ldc(J, 4) * jy + (double)ldc(J, 5) * jz) +
-                      dz * ((double)ldc(J, 6) * jx + (double)ldc(J, 7) * jy + (double)ldc(J, 8) * jz);
+                dE += dx * ((double)ldc(J, 0) * x + (double)ldc(J, 1) * y + (double)ldc(J, 2) * z) +
+                      dy * ((double)ldc(J, 3) * x + (double)ldc(J, 4) * y + (double)ldc(J, 5) * z) +
+                      dz * ((double)ldc(J, 6) * x + (double)ldc(J, 7) * y + (double)ldc(J, 8) * z);
                 if (ldc(C.bflag, b) & kBondBiquad) {
-                    if (!have_q) {
-                        quadrupole_d(nx, ny, nz, Qn);
-                        quadrupole_d(ox, oy, oz, Qo);
-                        have_q = true;
-                    }
                     double Qj[5];
-                    quadrupole_d(jx, jy, jz, Qj);
+                    quadrupole_d(x, y, z, Qj);
                     const T* Bq = C.bQ + 25 * b;
                     for (int p = 0; p < 5; p++) {
                         double w = 0;
-                        for (int q = 0; q < 5; q++) w += (double)ldc(Bq, 5 * p + q) * Qj[q];
+                        for (int qv = 0; qv < 5; qv++) w += (double)ldc(Bq, 5 * p + qv) * Qj[qv];
                         dE += (Qn[p] - Qo[p]) * w;
                     }
                 }
             }
-            const double kT = A.kT[sc.r];
-            const bool accept = kT == 0.0 ? dE <= 0.0 : u_acc <= exp(-dE / kT);
-            if (accept) {
-                stg(S, sc.g, sn.x);
-                stg(S, sc.g + total, sn.y);
-                stg(S, sc.g + 2 * total, sn.z);
-                for_each_ghost(sc, C.G, [&](uint32_t gi) {
-                    stg(S, gi, sn.x);
-                    stg(S, gi + total, sn.y);
-                    stg(S, gi + 2 * total, sn.z);
-                });
-                v[0] = dE;
-                v[1] = dx;
-                v[2] = dy;
-                v[3] = dz;
-            }
+        }
+        const double kT = ldc(A.kT, W.r);
+        const bool accept = kT == 0.0 ? dE <= 0.0 : u_acc <= exp(-dE / kT);
+        if (accept) {
+            stg(S, sc.g, sn.x);
+            stg(S, sc.g + total, sn.y);
+            stg(S, sc.g + 2 * total, sn.z);
+            for_each_ghost(sc, C.G, [&](uint32_t gi) {
+                stg(S, gi, sn.x);
+                stg(S, gi + total, sn.y);
+                stg(S, gi + 2 * total, sn.z);
+            });
+            v[0] = dE;
+            v[1] = dx;
+            v[2] = dy;
+            v[3] = dz;
         }
     }
     block_sum4(v);
     if (threadIdx.x == 0)
-        for (int q = 0; q < 4; q++) acc[4 * (size_t)sc.blk + q] += v[q];
+        for (int qv = 0; qv < 4; qv++) acc[4 * (size_t)blockIdx.x + qv] += v[qv];
 }
 
-// out[4 r + q] = sum of acc[4 k + q] over the blocks k of replica r.
-__global__ __launch_bounds__(kBlock) void k_mc_reduce(const double* __restrict__ acc, const int* __restrict__ rblk,
-                                                      double* __restrict__ out)
+// out[4 r + q] = sum of acc[4 k + q] over the work items k of replica r:
+// ranges [start[c (R + 1) + r], start[c (R + 1) + r + 1]) of every colour c.
+__global__ __launch_bounds__(kBlock) void k_mc_reduce(const double* __restrict__ acc, const int* __restrict__ start,
+                                                      int ncolors, int R, double* __restrict__ out)
 {
     const int r = blockIdx.x;
     double v[4] = {0, 0, 0, 0};
-    for (int k = rblk[r] + threadIdx.x; k < rblk[r + 1]; k += kBlock)
-        for (int q = 0; q < 4; q++) v[q] += acc[4 * (size_t)k + q];
+    for (int c = 0; c < ncolors; c++) {
+        const int* st = start + c * (R + 1);
+        for (int k = st[r] + threadIdx.x; k < st[r + 1]; k += kBlock)
+            for (int q = 0; q < 4; q++) v[q] += acc[4 * (size_t)k + q];
+    }
     block_sum4(v);
     if (threadIdx.x == 0)
         for (int q = 0; q < 4; q++) out[4 * r + q] = v[q];
@@ -212,15 +262,19 @@ __global__ void k_fill_double(double* __restrict__ p, long long n, double v)
 }
 
 template <typename T>
-void launch_mc_color(const Couplings<T>& C, int nblocks, const McArgs& A, T* S, int color, unsigned long long P,
-                     double prob, double* acc, hipStream_t st)
+void launch_mc_color(const Couplings<T>& C, const McArgs& A, T* S, const McWork* work, int nwork,
+                     unsigned long long P, double prob, double* acc, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mc_color<T>, dim3(nblocks), dim3(kBlock), 0, st, C, A, S, color, P, prob, acc);
+    if (nwork <= 0) return;
+    if (C.chunk == 6)
+        hipLaunchKernelGGL((k_mc_color<T, 6>), dim3(nwork), dim3(kBlock), 0, st, C, A, S, work, P, prob, acc);
+    else  // bond lists padded to 9 or 18
+        hipLaunchKernelGGL((k_mc_color<T, 9>), dim3(nwork), dim3(kBlock), 0, st, C, A, S, work, P, prob, acc);
 }
 
-void launch_mc_reduce(const double* acc, const int* rblk, int R, double* out, hipStream_t st)
+void launch_mc_reduce(const double* acc, const int* start, int ncolors, int R, double* out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mc_reduce, dim3(R), dim3(kBlock), 0, st, acc, rblk, out);
+    hipLaunchKernelGGL(k_mc_reduce, dim3(R), dim3(kBlock), 0, st, acc, start, ncolors, R, out);
 }
 
 void launch_fill_double(double* p, long long n, double v, hipStream_t st)
@@ -228,9 +282,9 @@ void launch_fill_double(double* p, long long n, double v, hipStream_t st)
     hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
 }
 
-template void launch_mc_color<float>(const Couplings<float>&, int, const McArgs&, float*, int, unsigned long long,
-                                     double, double*, hipStream_t);
-template void launch_mc_color<double>(const Couplings<double>&, int, const McArgs&, double*, int, unsigned long long,
-                                      double, double*, hipStream_t);
+template void launch_mc_color<float>(const Couplings<float>&, const McArgs&, float*, const McWork*, int,
+                                     unsigned long long, double, double*, hipStream_t);
+template void launch_mc_color<double>(const Couplings<double>&, const McArgs&, double*, const McWork*, int,
+                                      unsigned long long, double, double*, hipStream_t);
 
 }  // namespace snh
