@@ -254,7 +254,10 @@ int snh_autotune(snh_system *sys, int32_t candidates, int32_t launches, double *
  * "gather_chunk", "graph_steps"
  * (Langevin steps per captured HIP graph; 0 = graphs off, the default; set
  * by the environment variable SNH_GRAPH_STEPS at snh_create), "graph_live"
- * (1 when a captured graph is instantiated). */
+ * (1 when a captured graph is instantiated), "pair_split" (first replica of
+ * the second replica group of tiled Langevin batches, stepped on a second
+ * stream in anti-phase with the first; 0 = one stream, as with SNH_PAIR=0 at
+ * snh_create). */
 int snh_query(snh_system *sys, const char *key, int64_t *value);
 
 #ifdef __cplusplus

@@ -462,6 +462,12 @@ struct SystemImpl final : snh_system {
     // persistent workgroup per replica: no launch per stage, no host check
     // per midpoint step.
     bool persist = false;
+    // Tiled Langevin batches of two or more replicas (no Ewald) step as two
+    // replica groups on two streams in anti-phase: one group's corrector
+    // (latency-bound) runs beside the other's predictor (VALU-bound).
+    int pair_split = 0;        // first replica of the second group (0: off)
+    hipStream_t st2 = nullptr;
+    hipEvent_t pev[3] = {nullptr, nullptr, nullptr};  // start, first group's predictor, second group's predictor
     int* fail_step = nullptr;  // [R] persistent midpoint: step that did not converge
     // Sampled correlations of one :dipole replica (snh_corr.hip)
     struct Corr {
@@ -521,6 +527,10 @@ struct SystemImpl final : snh_system {
     ~SystemImpl() override
     {
         if (st) (void)hipStreamSynchronize(st);
+        if (st2) (void)hipStreamSynchronize(st2);
+        for (hipEvent_t e : pev)
+            if (e) (void)hipEventDestroy(e);
+        if (st2) (void)hipStreamDestroy(st2);
         drop_graph();
         for (void* p : corr.mem) (void)hipFree(p);
         timer.clear();
@@ -790,6 +800,55 @@ struct SystemImpl final : snh_system {
         // (4096 sites: 64 vs 10.7 us)
         persist = N == 0 && !ewald && !tiled && env_int("SNH_PERSIST", 1) != 0 &&
                   max_sites <= env_int("SNH_PERSIST_MAX_SITES", kPersist);
+        if (tiled && !ewald && R >= 2 && env_int("SNH_PAIR", 1) != 0) {
+            // split the replicas into two groups of about equal sites
+            long long all = 0, acc = 0;
+            for (int r = 0; r < R; r++) all += replica_sites(r);
+            int h = 1;
+            for (int r = 0; r < R - 1; r++) {
+                acc += replica_sites(r);
+                h = r + 1;
+                if (2 * acc >= all) break;
+            }
+            pair_split = h;
+            SNH_HIP(hipStreamCreate(&st2));
+            for (hipEvent_t& e : pev) SNH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+    }
+
+    // The part of a tile plan's work list that covers replicas [r0, r1).
+    static TileArgs plan_part(const TilePlan& P, int r0, int r1)
+    {
+        TileArgs A = P.A;
+        A.work = P.A.work + P.rwork[r0];
+        A.nwork = P.rwork[r1] - P.rwork[r0];
+        return A;
+    }
+
+    // Steps [s0, s1) of the tiled Langevin path as two replica groups:
+    // group 0 on st, group 1 on st2; group 1's predictor waits for group 0's,
+    // and group 0's next predictor for group 1's, so each corrector overlaps
+    // the other group's predictor.  Per-site arithmetic is that of
+    // langevin_step, so results are identical.
+    void langevin_pairs(NoiseArgs<T> nz, T tdt, T tl, uint64_t step0, int s0, int s1)
+    {
+        const TileArgs p0 = plan_part(tp, 0, pair_split), p1 = plan_part(tp, pair_split, R);
+        const TileArgs c0 = plan_part(tc, 0, pair_split), c1 = plan_part(tc, pair_split, R);
+        SNH_HIP(hipEventRecord(pev[0], st));
+        SNH_HIP(hipStreamWaitEvent(st2, pev[0], 0));
+        for (int s = s0; s < s1; s++) {
+            nz.step = step0 + (uint64_t)s;
+            if (s > s0) SNH_HIP(hipStreamWaitEvent(st, pev[2], 0));
+            launch_tile_predict<T>(C, p0, tp.threads, X, Xp, Xo, nullptr, nz, tdt, tl, st);
+            SNH_HIP(hipEventRecord(pev[1], st));
+            launch_tile_correct<T>(C, c0, tc.threads, Xp, Xo, X, nullptr, tdt, tl, st);
+            SNH_HIP(hipStreamWaitEvent(st2, pev[1], 0));
+            launch_tile_predict<T>(C, p1, tp.threads, X, Xp, Xo, nullptr, nz, tdt, tl, st2);
+            SNH_HIP(hipEventRecord(pev[2], st2));
+            launch_tile_correct<T>(C, c1, tc.threads, Xp, Xo, X, nullptr, tdt, tl, st2);
+        }
+        SNH_HIP(hipEventRecord(pev[0], st2));
+        SNH_HIP(hipStreamWaitEvent(st, pev[0], 0));
     }
 
     void build_ewald(const snh_spec& sp)
@@ -1484,6 +1543,10 @@ struct SystemImpl final : snh_system {
                 for (; nsteps - s >= graph_steps; s += graph_steps) SNH_HIP(hipGraphLaunch(graph_exec, st));
             }
         }
+        if (pair_split > 0 && !timer.enabled && s < nsteps) {
+            langevin_pairs(nz, tdt, tl, step0, s, nsteps);
+            s = nsteps;
+        }
         for (; s < nsteps; s++) {
             nz.step = step0 + (uint64_t)s;
             langevin_step(nz, tdt, tl, dip);
@@ -1893,6 +1956,7 @@ struct SystemImpl final : snh_system {
         if (key == "graph_steps") return graph_steps;
         if (key == "persistent") return persist ? 1 : 0;
         if (key == "graph_live") return graph_exec ? 1 : 0;
+        if (key == "pair_split") return pair_split;
         throw Failure{SNH_ERR_INVALID, "unknown query key: " + key};
     }
 

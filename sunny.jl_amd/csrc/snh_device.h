@@ -168,8 +168,16 @@ __device__ __forceinline__ void for_each_ghost(const SiteCoord& sc, const Ghost&
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) + Box-Muller.
+//
+// Every caller's key is uniform over its workgroup (one replica), so the key
+// lives in scalar registers.  The round keys are advanced in place behind an
+// empty asm barrier: otherwise the compiler precomputes all twenty of them
+// outside the caller's loops, runs out of scalar registers and reloads them
+// from vector-register lanes in every round.
 __device__ __forceinline__ void philox4x32(uint32_t c[4], uint32_t k0, uint32_t k1)
 {
+    k0 = __builtin_amdgcn_readfirstlane(k0);
+    k1 = __builtin_amdgcn_readfirstlane(k1);
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
@@ -182,6 +190,7 @@ __device__ __forceinline__ void philox4x32(uint32_t c[4], uint32_t k0, uint32_t 
         c[3] = (uint32_t)p0;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
+        asm volatile("" : "+s"(k0), "+s"(k1));
     }
 }
 
