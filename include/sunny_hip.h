@@ -20,6 +20,8 @@
  *  - Host pointers are borrowed for the duration of the call only.
  *  - One handle = one HIP stream; a handle is not internally synchronised
  *    (the same contract as a Sunny `System`, src/System/System.jl:138-174).
+ *    Tiled Langevin batches also use a second stream of the handle, which
+ *    the handle's stream waits for before any later work.
  */
 #ifndef SUNNY_HIP_H
 #define SUNNY_HIP_H
