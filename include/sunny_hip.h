@@ -205,11 +205,12 @@ int snh_corr_add_sample(snh_system *sys, double dt, double tol, int32_t max_iter
 int snh_corr_data(snh_system *sys, double *data, int64_t *nsamples);
 
 /* step!(sys, LocalSampler(; kT, nsweeps, propose)) for every replica
- * (src/MonteCarlo/Samplers.jl:133-153), :dipole mode without Ewald.
+ * (src/MonteCarlo/Samplers.jl:133-153), :dipole or :SUN, without Ewald.
  * Proposal p of the mixture (@mix_proposals, Samplers.jl:67-100) has
  * kinds[p] = 0 propose_uniform (randspin, src/System/System.jl:444-447),
- * 1 propose_flip (flip, System.jl:440-442) or 2 propose_delta(magnitudes[p])
- * (perturbed_spin, System.jl:454-460; an infinite magnitude is uniform),
+ * 1 propose_flip (flip, System.jl:440-442; flip_ket for :SUN, Spin.jl:119-125)
+ * or 2 propose_delta(magnitudes[p]) (perturbed_spin, System.jl:454-468; an
+ * infinite magnitude is uniform),
  * drawn with probability weights[p] / sum(weights); 1 <= nprop <= 4.
  * kT: [R].  A trial's energy change is local_energy_change
  * (src/System/Interactions.jl:311-380) in FP64, and it is accepted when

@@ -62,6 +62,10 @@ def lib():
         L.so_local_energy_change.restype = C.c_double
         L.so_local_sweeps.argtypes = [sp, _dp, _ip, _ip, C.c_int, C.c_int, _ip, _dp, _dp, C.c_double, C.c_uint64,
                                       C.c_uint64, C.c_double, _dp]
+        L.so_local_energy_change_sun.argtypes = [sp, _dp, _dp, C.c_long, _dp]
+        L.so_local_energy_change_sun.restype = C.c_double
+        L.so_local_sweeps_sun.argtypes = [sp, _dp, _dp, _ip, _ip, C.c_int, C.c_int, _ip, _dp, _dp, C.c_double,
+                                          C.c_uint64, C.c_uint64, C.c_double, _dp]
         L.so_local_random_sites.argtypes = [sp, _dp, C.c_int, _ip, _dp, _dp, C.c_double, C.c_uint64, C.c_double,
                                             _dp]
         _lib = L
@@ -161,6 +165,33 @@ class Oracle:
         if rc:
             raise ValueError("Energy delta for self-interaction not supported")
         return S, acc
+
+    def _ket_arrays(self, Z):
+        Zr = np.ascontiguousarray(np.stack([np.asarray(Z).real, np.asarray(Z).imag], -1), float).reshape(
+            self.ns, self.N, 2)
+        return Zr
+
+    def local_energy_change_sun(self, Z, site, Znew):
+        """local_energy_change(sys, site, state) for a :SUN state Z [sites][N]."""
+        Zr = self._ket_arrays(Z)
+        S = self.expected_spin(Z)
+        zn = np.ascontiguousarray(np.stack([np.asarray(Znew).real, np.asarray(Znew).imag], -1), float)
+        return lib().so_local_energy_change_sun(self._p(), _d(Zr), _d(S), int(site), _d(zn))
+
+    def local_sweeps_sun(self, Z, m, color, ncolors, proposals, kT, key, sweep0, nsweeps):
+        """:SUN LocalSampler passes; returns (Z, [dE, dSx, dSy, dSz])."""
+        Zr = self._ket_arrays(Z).copy()
+        S = self.expected_spin(Z)
+        kind, mag, cum = self._props(proposals)
+        mm = np.ascontiguousarray(m, np.int32)
+        col = np.ascontiguousarray(color, np.int32)
+        acc = np.zeros(4)
+        rc = lib().so_local_sweeps_sun(self._p(), _d(Zr), _d(S), _i(mm), _i(col), int(ncolors), len(kind), _i(kind),
+                                       _d(mag), _d(cum), float(kT), int(key) % 2**64, int(sweep0), float(nsweeps),
+                                       _d(acc))
+        if rc:
+            raise ValueError("Energy delta for self-interaction not supported")
+        return Zr[..., 0] + 1j * Zr[..., 1], acc
 
     def local_random_sites(self, S, proposals, kT, seed, nsweeps):
         """LocalSampler trials at random sites (the reference's order)."""

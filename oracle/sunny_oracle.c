@@ -1071,6 +1071,39 @@ void so_philox_normals(uint64_t seed, uint64_t step, long ns, int per_site, doub
 /* ------------------------------------------------------------------------ */
 /* Metropolis updates of single dipoles.                                     */
 
+/* ewald_energy_delta, Ewald.jl:234-241: dS . (field at the site by the
+   pairwise sum of Ewald.jl:211-230) + dmu . A[0, i, i] dmu / 2, dmu = -g dS. */
+static double ewald_delta(const so_spec *sp, const double *S, long site, const double dS[3])
+{
+    const long nc = ncells(sp);
+    const int *L = sp->L;
+    const int na = sp->na, i = (int)(site / nc);
+    const long c = site % nc;
+    const int c1 = (int)(c % L[0]), c2 = (int)((c / L[0]) % L[1]), c3 = (int)(c / ((long)L[0] * L[1]));
+    const double *g = sp->g + 9 * i;
+    double grad[3] = {0, 0, 0};
+    for (int i2 = 0; i2 < na; i2++) {
+        const double *g2 = sp->g + 9 * i2;
+        for (int b3 = 0; b3 < L[2]; b3++)
+            for (int b2 = 0; b2 < L[1]; b2++)
+                for (int b1 = 0; b1 < L[0]; b1++) {
+                    long off = site_index(L, wrap(b1 - c1, L[0]), wrap(b2 - c2, L[1]), wrap(b3 - c3, L[2]), 0);
+                    const double *A = sp->ewald_A + (((long)i2 * na + i) * nc + off) * 9;
+                    const double *s2 = S + 3 * site_index(L, b1, b2, b3, i2);
+                    double gs[3], Ags[3];
+                    for (int r = 0; r < 3; r++) gs[r] = g2[3 * r] * s2[0] + g2[3 * r + 1] * s2[1] + g2[3 * r + 2] * s2[2];
+                    for (int r = 0; r < 3; r++) Ags[r] = A[3 * r] * gs[0] + A[3 * r + 1] * gs[1] + A[3 * r + 2] * gs[2];
+                    for (int r = 0; r < 3; r++) grad[r] += g[r] * Ags[0] + g[3 + r] * Ags[1] + g[6 + r] * Ags[2];
+                }
+    }
+    double dmu[3], Adm[3], dE = 0;
+    for (int r = 0; r < 3; r++) dmu[r] = -(g[3 * r] * dS[0] + g[3 * r + 1] * dS[1] + g[3 * r + 2] * dS[2]);
+    const double *A0 = sp->ewald_A + (((long)i * na + i) * nc) * 9;
+    for (int r = 0; r < 3; r++) Adm[r] = A0[3 * r] * dmu[0] + A0[3 * r + 1] * dmu[1] + A0[3 * r + 2] * dmu[2];
+    for (int r = 0; r < 3; r++) dE += dS[r] * grad[r] + dmu[r] * Adm[r] / 2;
+    return dE;
+}
+
 /* local_energy_change, Interactions.jl:311-380 (:dipole mode), with
    ewald_energy_delta, Ewald.jl:234-241 (field at the site by the pairwise
    sum of Ewald.jl:211-230).  NAN when a bond maps the site onto itself
@@ -1111,31 +1144,7 @@ double so_local_energy_change(const so_spec *sp, const double *S, long site, con
                 for (int q = 0; q < 5; q++) dE += (Qn[p] - Qo[p]) * Bq[5 * p + q] * Qj[q];
         }
     }
-    if (sp->ewald_A) {
-        const int na = sp->na;
-        double grad[3] = {0, 0, 0};
-        for (int i2 = 0; i2 < na; i2++) {
-            const double *g2 = sp->g + 9 * i2;
-            for (int b3 = 0; b3 < L[2]; b3++)
-                for (int b2 = 0; b2 < L[1]; b2++)
-                    for (int b1 = 0; b1 < L[0]; b1++) {
-                        long off = site_index(L, wrap(b1 - c1, L[0]), wrap(b2 - c2, L[1]), wrap(b3 - c3, L[2]), 0);
-                        const double *A = sp->ewald_A + (((long)i2 * na + i) * nc + off) * 9;
-                        const double *s2 = S + 3 * site_index(L, b1, b2, b3, i2);
-                        double gs[3], Ags[3];
-                        for (int r = 0; r < 3; r++)
-                            gs[r] = g2[3 * r] * s2[0] + g2[3 * r + 1] * s2[1] + g2[3 * r + 2] * s2[2];
-                        for (int r = 0; r < 3; r++)
-                            Ags[r] = A[3 * r] * gs[0] + A[3 * r + 1] * gs[1] + A[3 * r + 2] * gs[2];
-                        for (int r = 0; r < 3; r++) grad[r] += g[r] * Ags[0] + g[3 + r] * Ags[1] + g[6 + r] * Ags[2];
-                    }
-        }
-        double dmu[3], Adm[3];
-        for (int r = 0; r < 3; r++) dmu[r] = -(g[3 * r] * dS[0] + g[3 * r + 1] * dS[1] + g[3 * r + 2] * dS[2]);
-        const double *A0 = sp->ewald_A + (((long)i * na + i) * nc) * 9;
-        for (int r = 0; r < 3; r++) Adm[r] = A0[3 * r] * dmu[0] + A0[3 * r + 1] * dmu[1] + A0[3 * r + 2] * dmu[2];
-        for (int r = 0; r < 3; r++) dE += dS[r] * grad[r] + dmu[r] * Adm[r] / 2;
-    }
+    if (sp->ewald_A) dE += ewald_delta(sp, S, site, dS);
     return dE;
 }
 
@@ -1270,6 +1279,154 @@ int so_local_random_sites(const so_spec *sp, double *S, int nprop, const int *ki
         box_muller_pair((uint32_t)splitmix64(&st), (uint32_t)splitmix64(&st), &n[2], &n[3]);
         const double u_acc = splitmix_u01(&st);
         if (local_trial(sp, S, site, nprop, kind, mag, cum, kT, u_mix, u_acc, n, acc)) return -1;
+    }
+    return 0;
+}
+
+/* local_energy_change for :SUN (Interactions.jl:311-380): the coherent state
+   of `site` becomes Zn (|Zn|^2 = kappa); S holds the expected spins of Z.
+   Zeeman and bilinear terms act on expected spins, the onsite term is
+   Re(Z' Lambda Z), biquadratic terms act on expected quadrupoles.  Writes the
+   new expected spin to Sn.  NAN on a self-interaction. */
+static double local_delta_sun(const so_spec *sp, const sun_ops *o, const cplx *Z, const double *S, long site,
+                              const cplx *Zn, double Sn[3])
+{
+    const int N = sp->N;
+    const long nc = ncells(sp);
+    const int *L = sp->L;
+    const int i = (int)(site / nc);
+    const long c = site % nc;
+    const int c1 = (int)(c % L[0]), c2 = (int)((c / L[0]) % L[1]), c3 = (int)(c / ((long)L[0] * L[1]));
+    const cplx *Z0 = Z + (long)N * site;
+    expected_spin1(o, Zn, Sn);
+    const double *S0 = S + 3 * site;
+    double dS[3] = {Sn[0] - S0[0], Sn[1] - S0[1], Sn[2] - S0[2]};
+    double dE = 0;
+    const double *g = sp->g + 9 * i, *B = sp->field + 3 * i;
+    for (int r = 0; r < 3; r++) dE += B[r] * (g[3 * r] * dS[0] + g[3 * r + 1] * dS[1] + g[3 * r + 2] * dS[2]);
+    const double *Lam = sp->Lambda + (long)2 * N * N * i;
+    cplx en = 0, eo = 0;
+    for (int a = 0; a < N; a++)
+        for (int b = 0; b < N; b++) {
+            const cplx l = Lam[2 * (a * N + b)] + I * Lam[2 * (a * N + b) + 1];
+            en += conj(Zn[a]) * l * Zn[b];
+            eo += conj(Z0[a]) * l * Z0[b];
+        }
+    dE += creal(en) - creal(eo);
+    double Qn[5], Qo[5];
+    expected_quadrupole1(o, Zn, Qn);
+    expected_quadrupole1(o, Z0, Qo);
+    for (int b = 0; b < sp->nb; b++) {
+        const int *bd = sp->bond + 6 * b;
+        if (bd[0] != i) continue;
+        long sj = site_index(L, wrap(c1 + bd[2], L[0]), wrap(c2 + bd[3], L[1]), wrap(c3 + bd[4], L[2]), bd[1]);
+        if (sj == site) return NAN;
+        const double *Sj = S + 3 * sj, *J = sp->bilin + 9 * b;
+        for (int r = 0; r < 3; r++) dE += dS[r] * (J[3 * r] * Sj[0] + J[3 * r + 1] * Sj[1] + J[3 * r + 2] * Sj[2]);
+        if (sp->has_biquad[b]) {
+            double Qj[5];
+            expected_quadrupole1(o, Z + (long)N * sj, Qj);
+            const double *Bq = sp->biquad + 25 * b;
+            for (int p = 0; p < 5; p++)
+                for (int q = 0; q < 5; q++) dE += (Qn[p] - Qo[p]) * Bq[5 * p + q] * Qj[q];
+        }
+    }
+    if (sp->ewald_A) dE += ewald_delta(sp, S, site, dS);
+    return dE;
+}
+
+double so_local_energy_change_sun(const so_spec *sp, const double *Zr, const double *S, long site,
+                                  const double *Znr)
+{
+    sun_ops o;
+    sun_ops_init(&o, sp->N);
+    double Sn[3];
+    return local_delta_sun(sp, &o, (const cplx *)Zr, S, site, (const cplx *)Znr, Sn);
+}
+
+/* flip_ket (Spin.jl:119-125): reverse(parity .* conj(Z)), parity (1, -1, 1, ...). */
+static void flip_ket(int N, const cplx *Z, cplx *out)
+{
+    for (int k = 0; k < N; k++) {
+        const int j = N - 1 - k;
+        out[k] = (j % 2 ? -1.0 : 1.0) * conj(Z[j]);
+    }
+}
+
+/* LocalSampler passes for :SUN, the order and draws of so_local_sweeps.  The
+   2N normals of a site come from the Philox counters (s, 0x4D430000 + 2b, P),
+   b = 0, 1, ...; component a of the complex draw randn(CVec{N}) (variance
+   1/2 per part) is (n[2a] + i n[2a+1]) / sqrt(2).  Proposals: randspin
+   (System.jl:448-452), flip (flip_ket), perturbed_spin Z + mag sqrt(kappa)
+   randn (System.jl:461-468); S is kept equal to the expected spins of Z. */
+int so_local_sweeps_sun(const so_spec *sp, double *Zr, double *S, const int m[3], const int *color, int ncolors,
+                        int nprop, const int *kind, const double *mag, const double *cum, double kT, uint64_t key,
+                        uint64_t sweep0, double nsweeps, double acc[4])
+{
+    const int N = sp->N;
+    const int *L = sp->L;
+    cplx *Z = (cplx *)Zr;
+    sun_ops o;
+    sun_ops_init(&o, N);
+    const uint32_t kk[2] = {(uint32_t)key, (uint32_t)(key >> 32)};
+    const long npass = (long)ceil(2 * nsweeps);
+    const double inv = 1.0 / 4294967296.0;
+    for (long k = 0; k < npass; k++) {
+        const uint64_t P = 2 * sweep0 + (uint64_t)k;
+        const double prob = k == npass - 1 ? (2 * nsweeps - (double)(npass - 1)) / 2 : 0.5;
+        for (int col = 0; col < ncolors; col++)
+            for (int i = 0; i < sp->na; i++)
+                for (int c3 = 0; c3 < L[2]; c3++)
+                    for (int c2 = 0; c2 < L[1]; c2++)
+                        for (int c1 = 0; c1 < L[0]; c1++) {
+                            const int cls = c1 % m[0] + m[0] * (c2 % m[1] + m[1] * (c3 % m[2] + m[2] * i));
+                            if (color[cls] != col) continue;
+                            const long site = site_index(L, c1, c2, c3, i);
+                            uint32_t ctr[4] = {(uint32_t)site, 0x4D430001u, (uint32_t)P, (uint32_t)(P >> 32)}, u[4];
+                            so_philox4x32(ctr, kk, u);
+                            if (!((double)u[2] * inv < prob)) continue;
+                            const double u_acc = (double)u[0] * inv, u_mix = (double)u[1] * inv;
+                            int kp = nprop - 1;
+                            for (int p = 0; p < nprop - 1; p++)
+                                if (u_mix < cum[p]) {
+                                    kp = p;
+                                    break;
+                                }
+                            double n[4 * ((2 * NMAX + 3) / 4)];
+                            for (int b = 0; b < (2 * N + 3) / 4; b++) {
+                                uint32_t cn[4] = {(uint32_t)site, 0x4D430000u + 2u * (uint32_t)b, (uint32_t)P,
+                                                  (uint32_t)(P >> 32)},
+                                         w[4];
+                                so_philox4x32(cn, kk, w);
+                                box_muller_pair(w[0], w[1], &n[4 * b], &n[4 * b + 1]);
+                                box_muller_pair(w[2], w[3], &n[4 * b + 2], &n[4 * b + 3]);
+                            }
+                            const double kap = sp->kappa[i];
+                            const cplx *Z0 = Z + (long)N * site;
+                            cplx Zn[NMAX];
+                            if (kind[kp] == 1) {
+                                flip_ket(N, Z0, Zn);
+                            } else {
+                                const int unif = kind[kp] == 0 || isinf(mag[kp]);
+                                for (int a = 0; a < N; a++) {
+                                    const cplx r = (n[2 * a] + I * n[2 * a + 1]) / sqrt(2.0);
+                                    Zn[a] = unif ? r : Z0[a] + mag[kp] * sqrt(kap) * r;
+                                }
+                                normalize_ket(N, Zn, kap);
+                            }
+                            double Sn[3];
+                            const double dE = local_delta_sun(sp, &o, Z, S, site, Zn, Sn);
+                            if (isnan(dE)) return -1;
+                            const int accept = kT == 0 ? dE <= 0 : u_acc <= exp(-dE / kT);
+                            if (accept) {
+                                acc[0] += dE;
+                                for (int r = 0; r < 3; r++) {
+                                    acc[1 + r] += Sn[r] - S[3 * site + r];
+                                    S[3 * site + r] = Sn[r];
+                                }
+                                for (int a = 0; a < N; a++) Z[(long)N * site + a] = Zn[a];
+                            }
+                        }
     }
     return 0;
 }

@@ -112,6 +112,13 @@ double so_local_energy_change(const so_spec *sp, const double *S, long site, con
 int so_local_sweeps(const so_spec *sp, double *S, const int m[3], const int *color, int ncolors, int nprop,
                     const int *kind, const double *mag, const double *cum, double kT, uint64_t key,
                     uint64_t sweep0, double nsweeps, double acc[4]);
+/* :SUN versions (Interactions.jl:311-380, Samplers.jl:1-153): Z [ns][N][2],
+   S [ns][3] the expected spins of Z (kept in step by the sweeps). */
+double so_local_energy_change_sun(const so_spec *sp, const double *Z, const double *S, long site,
+                                  const double *Znew);
+int so_local_sweeps_sun(const so_spec *sp, double *Z, double *S, const int m[3], const int *color, int ncolors,
+                        int nprop, const int *kind, const double *mag, const double *cum, double kT, uint64_t key,
+                        uint64_t sweep0, double nsweeps, double acc[4]);
 /* The same trials at uniformly random sites, as Samplers.jl:133-153 orders
    them (ceil(nsweeps * nsites) trials; splitmix64 draws from seed). */
 int so_local_random_sites(const so_spec *sp, double *S, int nprop, const int *kind, const double *mag,

@@ -1263,7 +1263,6 @@ struct SystemImpl final : snh_system {
     void mc_prepare()
     {
         if (mc.ready) return;
-        SNH_REQUIRE(N == 0, SNH_ERR_UNSUPPORTED, "the device LocalSampler supports :dipole mode");
         SNH_REQUIRE(!ewald, SNH_ERR_UNSUPPORTED,
                     "the device LocalSampler does not support dipole-dipole (Ewald) interactions: they couple every "
                     "pair of sites, so no two trials are independent");
@@ -1371,8 +1370,12 @@ struct SystemImpl final : snh_system {
             const double prob = k == npass - 1 ? (2 * nsweeps - (double)(npass - 1)) / 2 : 0.5;
             for (int col = 0; col < mc.col.ncolors; col++) {
                 hipEvent_t e = tbegin();
-                launch_mc_color<T>(C, A, X, mc.work + mc.coff[col], mc.coff[col + 1] - mc.coff[col],
-                                   2 * sweep0 + (uint64_t)k, prob, mc.acc + 4 * (size_t)mc.coff[col], st);
+                if (N == 0)
+                    launch_mc_color<T>(C, A, X, mc.work + mc.coff[col], mc.coff[col + 1] - mc.coff[col],
+                                       2 * sweep0 + (uint64_t)k, prob, mc.acc + 4 * (size_t)mc.coff[col], st);
+                else
+                    launch_sun_mc_color<T>(C, A, X, mc.work + mc.coff[col], mc.coff[col + 1] - mc.coff[col],
+                                           2 * sweep0 + (uint64_t)k, prob, mc.acc + 4 * (size_t)mc.coff[col], st);
                 tend("local_color", e);
             }
         }

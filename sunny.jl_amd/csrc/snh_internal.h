@@ -195,6 +195,9 @@ struct McWork {
 template <typename T>
 void launch_mc_color(const Couplings<T>& C, const McArgs& A, T* S, const McWork* work, int nwork,
                      unsigned long long P, double prob, double* acc, hipStream_t st);
+template <typename T>
+void launch_sun_mc_color(const Couplings<T>& C, const McArgs& A, T* Z, const McWork* work, int nwork,
+                         unsigned long long P, double prob, double* acc, hipStream_t st);
 void launch_mc_reduce(const double* acc, const int* start, int ncolors, int R, double* out, hipStream_t st);
 void launch_fill_double(double* p, long long n, double v, hipStream_t st);
 

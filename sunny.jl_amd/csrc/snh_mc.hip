@@ -1,4 +1,5 @@
-// Metropolis updates of single dipoles (:dipole mode) on the device.
+// Metropolis updates of single dipoles (:dipole mode) on the device; the
+// :SUN kernel (k_sun_mc_color, snh_sun.hip) shares the scheme.
 //
 // Reference semantics (Sunny.jl 0.9.2):
 //   LocalSampler, step!                       src/MonteCarlo/Samplers.jl:133-153
@@ -25,91 +26,10 @@
 //   counter (s, 0x4D430000, P) -> four normals (Box-Muller; three are used)
 //   counter (s, 0x4D430001, P) -> u_acc, u_mix, u_try (word / 2^32)
 // disjoint from the Langevin stream (second counter word 0).
-#include "snh_internal.h"
+#include "snh_mc.h"
 #include "snh_vec.h"
 
 namespace snh {
-
-constexpr uint32_t kMcTagNormals = 0x4D430000u;
-constexpr uint32_t kMcTagUniforms = 0x4D430001u;
-
-__device__ __forceinline__ int mc_wrap(int v, int L)
-{
-    const int m = v % L;
-    return m < 0 ? m + L : m;
-}
-
-// Classical Stevens energy in FP64 from the T-typed coefficients.
-template <typename T>
-__device__ __forceinline__ double stevens_energy_d(const Couplings<T>& C, int i, double x, double y, double z)
-{
-    const int km = ldc(C.kmax, i);
-    double c[28];
-#pragma unroll
-    for (int p = 0; p < 28; p++) c[p] = (double)ldc(C.stev + 28 * i, p);
-    double gx = 0, gy = 0, gz = 0, E = 0;
-    stevens_grad<double>(x, y, z, c, km, gx, gy, gz, &E);
-    return E;
-}
-
-__device__ __forceinline__ void quadrupole_d(double x, double y, double z, double Q[5])
-{
-    Q[0] = x * x - y * y;
-    Q[1] = z * x;
-    Q[2] = -x * x - y * y + 2 * z * z;
-    Q[3] = z * y;
-    Q[4] = 2 * y * x;
-}
-
-// Sum over a kBlock workgroup of four doubles per thread; thread 0 returns it.
-__device__ __forceinline__ void block_sum4(double v[4])
-{
-    __shared__ double red[kBlock / 64][4];
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0)
-        for (int q = 0; q < 4; q++) red[w][q] = v[q];
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 4; q++) {
-            double t = 0;
-            for (int k = 0; k < kBlock / 64; k++) t += red[k][q];
-            v[q] = t;
-        }
-}
-
-// Periodic wrap of c + n for 0 <= c < L: without a division when |n| <= L
-// (every bond of the handle, unless `wide`).
-__device__ __forceinline__ int mc_wrap_to(int v, int L, bool wide)
-{
-    if (wide) return mc_wrap(v, L);
-    v = v >= L ? v - L : v;
-    return v < 0 ? v + L : v;
-}
-
-// Site (cell c1, c2, c3; sublattice i) of replica R as the gather kernels
-// see it: padded index and whether it has ghost images.
-__device__ __forceinline__ SiteCoord mc_site(const ReplicaInfo& R, const Ghost& G, int i, int c1, int c2, int c3,
-                                             bool valid)
-{
-    SiteCoord sc;
-    sc.blk = 0;
-    sc.r = 0;
-    sc.i = i;
-    sc.R = R;
-    sc.c1 = c1;
-    sc.c2 = c2;
-    sc.c3 = c3;
-    sc.c = c1 + R.L1 * (c2 + R.L2 * c3);
-    sc.valid = valid;
-    sc.g = R.off + (uint32_t)((c1 + G.w1) + R.P1 * ((c2 + G.w2) + R.P2 * (c3 + G.w3))) + R.pcell * (uint32_t)i;
-    sc.edge = (c1 < G.w1) | (c1 >= R.L1 - G.w1) | (c2 < G.w2) | (c2 >= R.L2 - G.w2) | (c3 < G.w3) |
-              (c3 >= R.L3 - G.w3);
-    return sc;
-}
 
 // One colour of pass P: workgroup b takes work[b] = kBlock consecutive sites
 // q of one class (replica r, sublattice i, cell residue rho mod m), cell
@@ -123,19 +43,9 @@ __global__ __launch_bounds__(kBlock) void k_mc_color(Couplings<T> C, McArgs A, T
                                                      double prob, double* __restrict__ acc)
 {
     const McWork W = work[blockIdx.x];
-    const ReplicaInfo R = C.reps[W.r];
-    const int M = A.m1 * A.m2 * A.m3;
-    const int i = W.cls / M;
-    int rho = W.cls - i * M;
-    const int r1 = rho % A.m1;
-    rho /= A.m1;
-    const int r2 = rho % A.m2, r3 = rho / A.m2;
-    const int Q1 = R.L1 / A.m1, Q2 = R.L2 / A.m2, nq = R.ncell / M;
-    const int q = W.q0 + (int)threadIdx.x;
-    const int qq = q < nq ? q : 0;
-    const int t = qq / Q1;
-    const int q1 = qq - t * Q1, q3 = t / Q2, q2 = t - q3 * Q2;
-    const SiteCoord sc = mc_site(R, C.G, i, r1 + A.m1 * q1, r2 + A.m2 * q2, r3 + A.m3 * q3, q < nq);
+    const SiteCoord sc = mc_work_site(C, A, W, (int)threadIdx.x);
+    const ReplicaInfo& R = sc.R;
+    const int i = sc.i;
     const bool wide = A.wide != 0;
     double v[4] = {0, 0, 0, 0};
     const uint32_t total = C.total;
@@ -146,12 +56,7 @@ __global__ __launch_bounds__(kBlock) void k_mc_color(Couplings<T> C, McArgs A, T
     const double inv32 = 1.0 / 4294967296.0;
     const double u_acc = (double)un[0] * inv32, u_mix = (double)un[1] * inv32, u_try = (double)un[2] * inv32;
     if (sc.valid && u_try < prob) {
-        int k = A.nprop - 1;
-        for (int p = 0; p < A.nprop - 1; p++)
-            if (u_mix < A.cum[p]) {
-                k = p;
-                break;
-            }
+        const int k = mc_pick(A, u_mix);
         const V3<T> s0 = {ldg(S, sc.g), ldg(S, sc.g + total), ldg(S, sc.g + 2 * total)};
         const T kap = ldc(C.kappa, i);
         V3<T> sn;
@@ -185,13 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_mc_color(Couplings<T> C, McArgs A, T
             T jx[G], jy[G], jz[G];
 #pragma unroll
             for (int kk = 0; kk < G; kk++) {
-                const int* tb = A.bn + 4 * (c0 + kk);
-                const int a1 = mc_wrap_to(sc.c1 + ldc(tb, 0), R.L1, wide),
-                          a2 = mc_wrap_to(sc.c2 + ldc(tb, 1), R.L2, wide),
-                          a3 = mc_wrap_to(sc.c3 + ldc(tb, 2), R.L3, wide);
-                const uint32_t gj = R.off +
-                                    (uint32_t)((a1 + C.G.w1) + R.P1 * ((a2 + C.G.w2) + R.P2 * (a3 + C.G.w3))) +
-                                    R.pcell * (uint32_t)ldc(tb, 3);
+                const uint32_t gj = mc_neighbour(A, C.G, sc, c0 + kk, wide);
                 jx[kk] = ldg(S, gj);
                 jy[kk] = ldg(S, gj + total);
                 jz[kk] = ldg(S, gj + 2 * total);

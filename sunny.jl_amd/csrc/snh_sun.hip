@@ -18,6 +18,7 @@
 // -proj(zeta + dt(i+lambda)HZ, Z) = -proj(zeta, Z) - proj(dt(i+lambda)HZ, Z),
 // so the predictor stores H = Z + dZ1/2 - proj(zeta, Z')/2.
 #include "snh_internal.h"
+#include "snh_mc.h"
 
 namespace snh {
 
@@ -598,6 +599,146 @@ __global__ __launch_bounds__(kBlock) void k_sun_energy(Couplings<T> C, const T* 
 }
 
 // ---------------------------------------------------------------------------
+// Metropolis trials of one colour class for :SUN (the pass scheme, work
+// lists and draws of snh_mc.hip): LocalSampler's step! (Samplers.jl:133-153)
+// with randspin (System.jl:448-452), flip (flip_ket, Spin.jl:119-125) or
+// perturbed_spin (System.jl:461-468) proposals and local_energy_change
+// (Interactions.jl:311-380) in FP64: Zeeman and bilinear terms on expected
+// spins, Re(Z' Lambda Z) onsite, biquadratic terms on expected quadrupoles.
+// Accepted kets are stored with their ghost images.  Neighbour kets are
+// gathered 3, 2 or 1 bonds at a time as N grows (the SU(N) gather lists are
+// padded to 6), which keeps the kernel in registers.
+
+// Re(Zn' Lambda Zn) - Re(Z0' Lambda Z0) in one pass over Lambda.
+template <typename T, int N>
+__device__ __forceinline__ double onsite_delta(const Couplings<T>& C, int i, const Ket<T, N>& zn, const Ket<T, N>& z0)
+{
+    const T* L = C.lam + 2 * N * N * i;
+    double e = 0;
+#pragma unroll
+    for (int a = 0; a < N; a++) {
+        T nr = 0, ni = 0, orr = 0, oi = 0;  // (Lambda Zn)_a, (Lambda Z0)_a
+#pragma unroll
+        for (int b = 0; b < N; b++) {
+            const T lr = ldc(L, 2 * (a * N + b)), li = ldc(L, 2 * (a * N + b) + 1);
+            nr += lr * zn.z[b].re - li * zn.z[b].im;
+            ni += lr * zn.z[b].im + li * zn.z[b].re;
+            orr += lr * z0.z[b].re - li * z0.z[b].im;
+            oi += lr * z0.z[b].im + li * z0.z[b].re;
+        }
+        e += (double)(zn.z[a].re * nr + zn.z[a].im * ni) - (double)(z0.z[a].re * orr + z0.z[a].im * oi);
+    }
+    return e;
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(kBlock) void k_sun_mc_color(Couplings<T> C, McArgs A, T* __restrict__ Z,
+                                                         const McWork* __restrict__ work, unsigned long long P,
+                                                         double prob, double* __restrict__ acc)
+{
+    constexpr int G = (sizeof(T) == 4 ? N <= 3 : N <= 2) ? 3 : (N <= 4 ? 2 : 1);
+    const McWork W = work[blockIdx.x];
+    const SiteCoord sc = mc_work_site(C, A, W, (int)threadIdx.x);
+    const int i = sc.i;
+    const bool wide = A.wide != 0;
+    const uint32_t total = C.total;
+    double v[4] = {0, 0, 0, 0};
+    const unsigned long long key = A.seed + (unsigned long long)W.r * 0x9E3779B97F4A7C15ull;
+    const uint32_t site = (uint32_t)sc.c + (uint32_t)sc.R.ncell * (uint32_t)i;
+    uint32_t un[4] = {site, kMcTagUniforms, (uint32_t)P, (uint32_t)(P >> 32)};
+    philox4x32(un, (uint32_t)key, (uint32_t)(key >> 32));
+    const double inv32 = 1.0 / 4294967296.0;
+    const double u_acc = (double)un[0] * inv32, u_mix = (double)un[1] * inv32, u_try = (double)un[2] * inv32;
+    if (sc.valid && u_try < prob) {
+        const int k = mc_pick(A, u_mix);
+        const Ket<T, N> z0 = load_ket<T, N>(Z, total, sc.g);
+        const T kap = ldc(C.kappa, i);
+        Ket<T, N> zn;
+        if (A.kind[k] == 1) {  // flip_ket: reverse(parity .* conj(Z))
+#pragma unroll
+            for (int a = 0; a < N; a++) {
+                const int j = N - 1 - a;
+                const T sg = (j & 1) ? (T)-1 : (T)1;
+                zn.z[a] = {sg * z0.z[j].re, -sg * z0.z[j].im};
+            }
+        } else {
+            constexpr int NB = (2 * N + 3) / 4;
+            T n[4 * NB];
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                uint32_t cn[4] = {site, kMcTagNormals + 2u * (uint32_t)b, (uint32_t)P, (uint32_t)(P >> 32)};
+                philox4x32(cn, (uint32_t)key, (uint32_t)(key >> 32));
+                box_muller(cn[0], cn[1], n[4 * b], n[4 * b + 1]);
+                box_muller(cn[2], cn[3], n[4 * b + 2], n[4 * b + 3]);
+            }
+            const double mag = A.mag[k];
+            const bool unif = A.kind[k] == 0 || __builtin_isinf(mag);
+            const T f = unif ? (T)0 : (T)(mag * sqrt((double)kap));
+            const T h = (T)0.70710678118654752440;  // randn(ComplexF64): variance 1/2 per part
+#pragma unroll
+            for (int a = 0; a < N; a++) {
+                const T nr = n[2 * a] * h, ni = n[2 * a + 1] * h;
+                zn.z[a] = unif ? Cx<T>{nr, ni} : Cx<T>{z0.z[a].re + f * nr, z0.z[a].im + f * ni};
+            }
+            zn = normalize_ket<T, N>(zn, kap);
+        }
+        T ox, oy, oz, nx, ny, nz;
+        expected_spin<T, N>(z0, ox, oy, oz);
+        expected_spin<T, N>(zn, nx, ny, nz);
+        const double dx = (double)nx - ox, dy = (double)ny - oy, dz = (double)nz - oz;
+        double dE = (double)ldc(C.zee, 3 * i) * dx + (double)ldc(C.zee, 3 * i + 1) * dy +
+                    (double)ldc(C.zee, 3 * i + 2) * dz;
+        dE += onsite_delta<T, N>(C, i, zn, z0);
+        T Qn[5], Qo[5];
+        bool have_q = false;
+        const int b0 = ldc(C.bstart, i), b1 = ldc(C.bstart, i + 1);
+        for (int c0 = b0; c0 < b1; c0 += G) {
+            Ket<T, N> zj[G];
+#pragma unroll
+            for (int kk = 0; kk < G; kk++) zj[kk] = load_ket<T, N>(Z, total, mc_neighbour(A, C.G, sc, c0 + kk, wide));
+#pragma unroll
+            for (int kk = 0; kk < G; kk++) {
+                const int b = c0 + kk;
+                T sx, sy, sz;
+                expected_spin<T, N>(zj[kk], sx, sy, sz);
+                const double x = sx, y = sy, z = sz;
+                const T* J = C.bJ + 9 * b;
+                dE += dx * ((double)ldc(J, 0) * x + (double)ldc(J, 1) * y + (double)ldc(J, 2) * z) +
+                      dy * ((double)ldc(J, 3) * x + (double)ldc(J, 4) * y + (double)ldc(J, 5) * z) +
+                      dz * ((double)ldc(J, 6) * x + (double)ldc(J, 7) * y + (double)ldc(J, 8) * z);
+                if (ldc(C.bflag, b) & kBondBiquad) {
+                    if (!have_q) {
+                        expected_quadrupole<T, N>(zn, C.quad_ops, Qn);
+                        expected_quadrupole<T, N>(z0, C.quad_ops, Qo);
+                        have_q = true;
+                    }
+                    T Qj[5];
+                    expected_quadrupole<T, N>(zj[kk], C.quad_ops, Qj);
+                    const T* Bq = C.bQ + 25 * b;
+                    for (int p = 0; p < 5; p++) {
+                        double w = 0;
+                        for (int q = 0; q < 5; q++) w += (double)ldc(Bq, 5 * p + q) * (double)Qj[q];
+                        dE += ((double)Qn[p] - (double)Qo[p]) * w;
+                    }
+                }
+            }
+        }
+        const double kT = ldc(A.kT, W.r);
+        const bool accept = kT == 0.0 ? dE <= 0.0 : u_acc <= exp(-dE / kT);
+        if (accept) {
+            store_ket_images<T, N>(Z, total, sc, C.G, zn);
+            v[0] = dE;
+            v[1] = dx;
+            v[2] = dy;
+            v[3] = dz;
+        }
+    }
+    block_sum4(v);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; q++) acc[4 * (size_t)blockIdx.x + q] += v[q];
+}
+
+// ---------------------------------------------------------------------------
 // Launchers: dispatch the run-time N to a compiled specialisation.
 
 #define SNH_SUN_DISPATCH(N, KERNEL, ...)                             \
@@ -693,7 +834,17 @@ void launch_sun_energy(const Couplings<T>& C, int nblocks, const T* Z, const T* 
     SNH_SUN_DISPATCH(C.N, k_sun_energy, dim3(nblocks), dim3(kBlock), 0, st, C, Z, phi, partial)
 }
 
+template <typename T>
+void launch_sun_mc_color(const Couplings<T>& C, const McArgs& A, T* Z, const McWork* work, int nwork,
+                         unsigned long long P, double prob, double* acc, hipStream_t st)
+{
+    if (nwork <= 0) return;
+    SNH_SUN_DISPATCH(C.N, k_sun_mc_color, dim3(nwork), dim3(kBlock), 0, st, C, A, Z, work, P, prob, acc)
+}
+
 #define SNH_SUN_INST(T)                                                                                          \
+    template void launch_sun_mc_color<T>(const Couplings<T>&, const McArgs&, T*, const McWork*, int,            \
+                                         unsigned long long, double, double*, hipStream_t);                      \
     template void launch_sun_expected_spin<T>(const Couplings<T>&, int, const T*, T*, hipStream_t);              \
     template void launch_sun_predict<T>(const Couplings<T>&, int, const T*, T*, T*, const T*,                    \
                                         const NoiseArgs<T>&, T, T, const T*, T*, hipStream_t);                   \

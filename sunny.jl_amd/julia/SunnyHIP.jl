@@ -231,7 +231,7 @@ function correlation_data(c::DeviceCorrelations)
 end
 
 # LocalSampler (src/MonteCarlo/Samplers.jl:133-153) on the device, :dipole
-# without Ewald.  The proposal function is translated to the device's
+# or :SUN, without Ewald.  The proposal function is translated to the device's
 # mixture: propose_uniform, propose_flip, the closure of propose_delta (its
 # captured `magnitude`) and the closure of @mix_proposals (its `cumprobs`
 # and `fns`).
@@ -246,7 +246,7 @@ function device_proposal(f)
     error("the device LocalSampler supports propose_uniform, propose_flip, propose_delta and @mix_proposals of them")
 end
 
-function Sunny.step!(d::DeviceSystem{0}, sampler::Sunny.LocalSampler)
+function Sunny.step!(d::DeviceSystem, sampler::Sunny.LocalSampler)
     terms = device_proposal(sampler.propose)
     kinds = Int32[t[1] for t in terms]
     mags = Float64[t[2] for t in terms]

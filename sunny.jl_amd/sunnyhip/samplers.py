@@ -1,4 +1,4 @@
-"""Metropolis sampling of single dipoles on the device.
+"""Metropolis sampling of single spins (dipoles or SU(N) coherent states) on the device.
 
 Host mirror of Sunny's `LocalSampler` and its proposal functions
 (src/MonteCarlo/Samplers.jl:1-153) over snh_step_local.  A proposal is a

@@ -17,7 +17,8 @@ from oracle import oracle as O
 
 from helpers import announce, coloring_is_proper, energy_scale, langevin_function
 from test_gpu_dipole import aniso_model
-from test_oracle_local import ising_chains
+from test_oracle_local import ising_chains, su5_mean_energy, sun_anisotropy_chain, sun_biquad_system
+from test_oracle_dynamics import su3_mean_energy
 
 pytestmark = pytest.mark.gpu
 
@@ -143,9 +144,6 @@ def test_langevin_after_sweeps_sees_updated_images():
 
 def test_unsupported_and_invalid():
     announce("local sampler errors")
-    sun = sh.workloads.triangular_sun_model(L1=3, L2=3, seed=1, precision=64)
-    with pytest.raises(ValueError, match=":dipole"):
-        sh.step_(sun, sh.LocalSampler(kT=1.0))
     ew = sh.workloads.pyrochlore_model(L=1, seed=2, precision=64, dipolar=True)
     with pytest.raises(ValueError, match="Ewald"):
         sh.step_(ew, sh.LocalSampler(kT=1.0))
@@ -205,3 +203,55 @@ def test_ising_chains_with_flips():
     t = np.tanh(kap**2 / kT)
     exact = -kap**2 * (t + t ** (L - 1)) / (1 + t**L)
     assert abs(vals.mean() - exact) < max(4 * err, 3e-3), (vals.mean(), exact, err)
+
+
+def sun_models(precision):
+    yield "tri_sun3_biquad", sun_biquad_system(seed=5)
+    s5 = sh.workloads.triangular_sun_model(L1=3, L2=3, s=2.0, D=0.7, seed=6, precision=precision)
+    sh.set_field_(s5, [0.0, 0.3, -0.1])
+    sh.randomize_spins_(s5)
+    yield "tri_sun5", s5
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_sun_sweeps_match_oracle(precision):
+    for name, s in sun_models(precision):
+        s.precision = precision
+        for pname, prop in PROPOSALS.items():
+            if precision == 32 and pname != "flip":
+                continue
+            announce(f"local sweeps {name} {pname} fp{precision}")
+            Z0 = s.coherents_flat()
+            key = O.replica_key(s.noise_seed, 0)
+            m, nc, col = sh.local_coloring(s)
+            o = O.Oracle(s.spec())
+            nsweeps = 1.5 if pname == "mix" else 2.0
+            Z_ref, acc = o.local_sweeps_sun(Z0, m, col.reshape(-1), nc, oracle_props(prop), 0.7, key,
+                                            s.step_counter, nsweeps)
+            sampler = sh.LocalSampler(kT=0.7, nsweeps=nsweeps, propose=prop)
+            sh.step_(s, sampler)
+            Z_dev = s.coherents_flat()
+            kap = s.spec()["kappa"][0]
+            assert np.max(np.abs(Z_dev - Z_ref)) <= TOL[precision] * np.sqrt(kap) * 10, (name, pname)
+            assert np.sum(np.any(np.abs(Z_dev - Z0) > 1e-9, axis=1)) > 0, (name, pname)
+            assert abs(sampler.dE - acc[0]) <= 10 * TOL[precision] * energy_scale(s), (name, pname)
+
+
+@pytest.mark.parametrize("N,kT", [(3, 0.125), (3, 0.5), (5, 0.25)])
+def test_sun_mean_energy_closed_form(N, kT):
+    """test/test_samplers.jl:6-15 closed forms from device Metropolis trials."""
+    announce(f"local sampler SU({N}) anisotropy kT={kT}")
+    D = 1.0
+    s = sun_anisotropy_chain(N, D, L=512)
+    s.precision = 32
+    sampler = sh.LocalSampler(kT=kT, propose=sh.propose_uniform)
+    sh.step_(s, sampler, nsteps=50)
+    vals = []
+    for _ in range(400):
+        sh.step_(s, sampler)
+        vals.append(sh.energy(s) / s.nsites)
+    vals = np.asarray(vals)
+    blocks = vals.reshape(20, -1).mean(1)
+    err = blocks.std(ddof=1) / np.sqrt(20)
+    exact = su3_mean_energy(kT, D) if N == 3 else su5_mean_energy(kT, D)
+    assert abs(vals.mean() - exact) < max(4 * err, 0.01 * abs(exact)), (vals.mean(), exact, err)

@@ -158,3 +158,101 @@ def test_colour_passes_and_random_site_orders_sample_alike():
     blocks = vals[: len(vals) // 20 * 20].reshape(20, -1).mean(1)
     e_rs, err_rs = vals.mean(), blocks.std(ddof=1) / np.sqrt(20)
     assert abs(e_cb - e_rs) < 4 * np.hypot(err_cb, err_rs) + 1e-3, (e_cb, err_cb, e_rs, err_rs)
+
+
+# --------------------------------------------------------------------------
+# :SUN mode
+
+
+def sun_biquad_system(seed=3):
+    """Triangular SU(3) with J1-J3, D Sz^2, a field and a biquadratic J1 bond."""
+    s = sh.workloads.triangular_sun_model(L1=3, L2=4, seed=seed, precision=64)
+    sh.set_field_(s, [0.1, -0.2, 0.3])
+    sh.set_exchange_(s, -0.24, sh.Bond(0, 0, (1, 0, 0)), biquad=0.15)
+    sh.randomize_spins_(s)
+    return s
+
+
+def kets_for(Z0, rng, kappa):
+    """A random, a time-reversed and a perturbed coherent state of one site."""
+    N = len(Z0)
+    z = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    yield np.sqrt(kappa) * z / np.linalg.norm(z)
+    par = np.array([1 - 2 * (j % 2) for j in range(N)])
+    yield (par * np.conj(Z0))[::-1]
+    d = Z0 + 0.3 * np.sqrt(kappa) * (rng.standard_normal(N) + 1j * rng.standard_normal(N)) / np.sqrt(2)
+    yield np.sqrt(kappa) * d / np.linalg.norm(d)
+
+
+def test_sun_local_energy_change_is_energy_difference():
+    s = sun_biquad_system()
+    o = O.Oracle(s.spec())
+    Z = s.coherents_flat()
+    E0 = o.energy(Z=Z)
+    kap = s.spec()["kappa"][0]
+    rng = np.random.default_rng(9)
+    for site in rng.choice(s.nsites, 6, replace=False):
+        for Zn in kets_for(Z[site], rng, kap):
+            Z2 = Z.copy()
+            Z2[site] = Zn
+            dE = o.local_energy_change_sun(Z, site, Zn)
+            assert abs(dE - (o.energy(Z=Z2) - E0)) <= 1e-11 * max(abs(E0), 1.0), site
+
+
+def test_sun_flip_reverses_expected_spin():
+    """Time reversal (flip_ket) negates <S>: one pass at a huge kT accepts every
+    flip it tries, so every site is either unchanged or reversed."""
+    s = sun_biquad_system(seed=4)
+    spec = s.spec()
+    o = O.Oracle(spec)
+    m, nc, col = site_coloring(spec, s.dims)
+    Z = s.coherents_flat()
+    S0 = o.expected_spin(Z)
+    Z2, acc = o.local_sweeps_sun(Z, m, col.reshape(-1), nc, [(FLIP, 0, 1)], 1e12, 3, 0, 0.5)
+    S2 = o.expected_spin(Z2)
+    same = np.all(np.abs(S2 - S0) < 1e-12, axis=1)
+    flipped = np.all(np.abs(S2 + S0) < 1e-12, axis=1)
+    assert np.all(same | flipped) and 0.2 < flipped.mean() < 0.8
+    np.testing.assert_allclose(acc[1:], S2.sum(0) - S0.sum(0), atol=1e-10)
+    np.testing.assert_allclose(np.linalg.norm(Z2, axis=1) ** 2, spec["kappa"][0], rtol=1e-13)
+
+
+def su5_mean_energy(kT, D):
+    """test/test_samplers.jl:11-15: SU(5) with Lambda = D (Sz^2 - Sz^4 / 5)."""
+    a = 4 * D / (5 * kT)
+    return 4 * D * (np.exp(-a) * (-a * (a * (a * (a + 4) + 12) + 24) - 24) + 24) / (
+        5 * a * (np.exp(-a) * (-a * (a * (a + 3) + 6) - 6) + 6))
+
+
+def sun_anisotropy_chain(N, D, L=20):
+    cryst = sh.Crystal(sh.lattice_vectors(1, 1, 1, 90, 90, 90), [[0, 0, 0]])
+    s = sh.System(cryst, [(0, sh.Moment(s=(N - 1) / 2, g=2))], "SUN", dims=(L, 1, 1), seed=0)
+    if N == 3:
+        sh.set_onsite_coupling_(s, lambda S: D * S[2] @ S[2], 0)
+    else:
+        sh.set_onsite_coupling_(s, lambda S: D * (S[2] @ S[2] - 0.2 * S[2] @ S[2] @ S[2] @ S[2]), 0)
+    sh.randomize_spins_(s)
+    return s
+
+
+@pytest.mark.parametrize("N,kT", [(3, 0.125), (3, 0.5), (5, 0.25)])
+def test_sun_local_sampler_mean_energy(N, kT):
+    """The closed-form mean energies of test/test_samplers.jl:6-15 (written for
+    Langevin sampling) from Metropolis trials with uniform proposals."""
+    from test_oracle_dynamics import su3_mean_energy
+    D = 1.0
+    s = sun_anisotropy_chain(N, D)
+    spec = s.spec()
+    o = O.Oracle(spec)
+    m, nc, col = site_coloring(spec, s.dims)
+    Z = s.coherents_flat()
+    vals = []
+    for q in range(6000):
+        Z, _ = o.local_sweeps_sun(Z, m, col.reshape(-1), nc, [(UNIFORM, 0, 1)], kT, 21, q, 1.0)
+        if q >= 200:
+            vals.append(o.energy(Z=Z) / s.nsites)
+    vals = np.asarray(vals)
+    blocks = vals[: len(vals) // 20 * 20].reshape(20, -1).mean(1)
+    err = blocks.std(ddof=1) / np.sqrt(20)
+    exact = su3_mean_energy(kT, D) if N == 3 else su5_mean_energy(kT, D)
+    assert abs(vals.mean() - exact) < max(4 * err, 0.01 * abs(exact)), (vals.mean(), exact, err)
