@@ -176,7 +176,9 @@ int snh_kernel_time(snh_system *sys, const char *family, double *ms, int64_t *la
  * write).  Used to calibrate HBM traffic counters against a known size. */
 int snh_copy_calibration(snh_system *sys, int32_t repeats, int64_t *bytes);
 
-/* Sampled dynamical correlations of one :dipole replica, on the device.
+/* Sampled dynamical correlations of one replica, on the device (:dipole, or
+ * :SUN with the observables acting on the expected spins, as
+ * observable_values! reads sys.dipoles).
  *
  * snh_corr_init replaces SampledCorrelations(sys; measure, energies, dt)
  * (src/SampledCorrelations/SampledCorrelations.jl:194-268) for the buffers:

@@ -1147,7 +1147,7 @@ struct SystemImpl final : snh_system {
     void corr_init(int nobs, const double* obs, int ncorr, const int32_t* pairs, int nt, int measperiod,
                    int window) override
     {
-        SNH_REQUIRE(N == 0 && R == 1, SNH_ERR_UNSUPPORTED, "sampled correlations support one :dipole replica");
+        SNH_REQUIRE(R == 1, SNH_ERR_UNSUPPORTED, "sampled correlations support one replica");
         SNH_REQUIRE(nobs >= 1 && obs && ncorr >= 1 && pairs, SNH_ERR_INVALID, "observables and pairs required");
         SNH_REQUIRE(nt >= 2 && nt % 2 == 0 && nt <= 4096, SNH_ERR_INVALID, "n_all_omega must be even, 2..4096");
         SNH_REQUIRE(measperiod >= 1, SNH_ERR_INVALID, "measperiod must be positive");
@@ -1211,10 +1211,16 @@ struct SystemImpl final : snh_system {
         const size_t nsamp = (size_t)nt * nobs * na * ncell;
         SNH_HIP(hipMemsetAsync(corr.samp, 0, nsamp * 2 * sizeof(double), st));
         const double scale = 1.0 / std::sqrt((double)ncell);  // the 1/sqrt(prod(dims)) of space_fft!
-        launch_corr_observe<T>(C, nblocks, X, nobs, corr.obs, corr.samp, 0, scale, st);
+        // observables act on the dipoles: :SUN states give their expected spins
+        if (N > 0 && !Sd) Sd = own(dalloc<T>(3 * (size_t)total));
+        auto observe = [&](int n) {
+            if (N > 0) launch_sun_expected_spin<T>(C, nblocks, X, Sd, st);
+            launch_corr_observe<T>(C, nblocks, N > 0 ? Sd : X, nobs, corr.obs, corr.samp, n, scale, st);
+        };
+        observe(0);
         for (int n = 1; n < Tsnap; n++) {
             step_midpoint(dt, 0.0, tol, max_iters, corr.measperiod, nullptr);
-            launch_corr_observe<T>(C, nblocks, X, nobs, corr.obs, corr.samp, n, scale, st);
+            observe(n);
         }
         // lattice transform of the recorded slots (the padding stays zero)
         launch_corr_space_fft(corr.samp, (long long)Tsnap * nobs * na, corr.L[0], corr.L[1], corr.L[2], st);

@@ -1,4 +1,5 @@
-// Sampled dynamical correlations of one :dipole replica on the device.
+// Sampled dynamical correlations of one replica on the device (:SUN states
+// are observed through their expected spins).
 //
 // Reference semantics (Sunny.jl 0.9.2, src/SampledCorrelations/):
 //   observable_values!, trajectory!, new_sample!   CorrelationSampling.jl:1-44

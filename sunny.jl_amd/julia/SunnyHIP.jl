@@ -187,7 +187,7 @@ function Sunny.energy(d::DeviceSystem)
     return E[]
 end
 
-# SampledCorrelations accumulated on the device (:dipole): the measure's
+# SampledCorrelations accumulated on the device (:dipole or :SUN): the measure's
 # observables and pairs, dt and measperiod as Sunny computes them
 # (adjusted_dt_and_downsampling_factor), then add_sample! and sc.data.
 struct DeviceCorrelations
@@ -199,7 +199,7 @@ struct DeviceCorrelations
     ncorr::Int
 end
 
-function device_correlations(d::DeviceSystem{0}, measure; energies, dt, window=:cosine)
+function device_correlations(d::DeviceSystem, measure; energies, dt, window=:cosine)
     nω = length(energies)
     dt, measperiod = Sunny.adjusted_dt_and_downsampling_factor(dt, nω, energies[end])
     na = Sunny.natoms(d.sys.crystal)

@@ -2,7 +2,8 @@
 
 Host mirror of Sunny's `SampledCorrelations` / `add_sample!`
 (src/SampledCorrelations/SampledCorrelations.jl, CorrelationSampling.jl)
-for one :dipole System: the trajectory, the lattice and time transforms,
+for one System (:dipole, or :SUN through the expected spins): the
+trajectory, the lattice and time transforms,
 and the running mean of every correlation run on the GPU
 (snh_corr_init / snh_corr_add_sample / snh_corr_data).
 """
@@ -52,8 +53,6 @@ class SampledCorrelations:
 
     def __init__(self, sys: System, *, energies, dt, observables=None, corr_pairs=None, window="cosine",
                  tol=1e-12, apply_g=True):
-        if sys.N != 0:
-            raise NotImplementedError("device sampled correlations support :dipole mode")
         energies = np.asarray(energies, float)
         n_omega = len(energies)
         omega_max = energies[-1]

@@ -51,3 +51,27 @@ def test_sampled_correlations_match_oracle(name, window):
     diag = [c for c, (a, b) in enumerate(sc.corr_pairs) if a == b]
     m = data[diag][:, np.arange(na), np.arange(na)].sum(-1) / sc.n_all_omega
     assert np.all(m.real >= -1e-12) and np.abs(m.imag).max() <= 1e-10 * scale
+
+
+def test_sun_sampled_correlations_match_oracle():
+    """:SUN trajectories: the dipole observables act on the expected spins
+    (observable_values!, CorrelationSampling.jl:1-8, reads sys.dipoles)."""
+    s = sh.workloads.triangular_sun_model(L1=4, L2=3, seed=7, precision=64)
+    energies = np.linspace(0.0, 8.0, 6)
+    sc = sh.SampledCorrelations(s, energies=energies, dt=0.05)
+    announce(f"sampled correlations SU(3) triangular 4x3, T={sc.n_all_omega}, measperiod={sc.measperiod}")
+    ref = SampledCorrelationsRef(s.dims, 1, sc.observables, sc.corr_pairs, sc.n_all_omega, "cosine")
+    o = O.Oracle(s.spec())
+    Z = s.coherents_flat()
+    Tsnap = sc.n_all_omega // 2 + 1
+    for _ in range(2):
+        snaps = [o.expected_spin(Z)]
+        for _ in range(Tsnap - 1):
+            for _ in range(sc.measperiod):
+                Z, _ = o.step_midpoint(Z=Z, dt=sc.dt, damping=0.0, tol=sc.tol)
+            snaps.append(o.expected_spin(Z))
+        ref.accum_sample(ref.new_sample(snaps))
+        sh.add_sample_(sc, s)
+    assert rel_err(s.coherents_flat(), Z, scale=1.0) <= 1e-10
+    scale = np.abs(ref.data).max()
+    np.testing.assert_allclose(sc.data, ref.data, rtol=0, atol=1e-9 * scale)
