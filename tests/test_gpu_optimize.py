@@ -53,3 +53,21 @@ def test_minimize_energy_finds_neel_state():
     S = s.dipoles_flat()
     assert np.allclose(np.abs(S[:, 2]), 1.0, atol=1e-5)
     assert np.allclose(np.sign(S[:, 2]) * stag, np.sign(S[0, 2]) * stag[0])
+
+
+def test_minimize_energy_sun_finds_polarised_state():
+    """SU(3) (s = 1) triangular lattice, easy-axis D Sz^2 (D = -1), field
+    B = 0.3 along z (g = 2) and ferromagnetic J1 = -0.2: the energy is minimal
+    on every site and bond at once in the m = -1 state, so the ground energy is
+    N (D - 2 B) + nbonds J1 = N (D - 2 B + 3 J1)."""
+    s = sh.workloads.triangular_sun_model(L1=4, L2=3, J1=-0.2, J2=0.0, J3=0.0, D=-1.0, seed=11, precision=64)
+    sh.set_field_(s, [0.0, 0.0, 0.3])
+    sh.randomize_spins_(s)
+    announce("minimize_energy SU(3) triangular 4x3")
+    res = minimize_energy_(s, maxiters=2000)
+    E_exact = s.nsites * (-1.0 - 2 * 0.3 + 3 * -0.2)
+    assert res.converged, res
+    assert abs(res.energy - E_exact) <= 1e-8 * abs(E_exact), (res.energy, E_exact)
+    # the oracle agrees with the device energy of the final state
+    o = O.Oracle(s.spec())
+    assert abs(o.energy(Z=s.coherents_flat()) - res.energy) <= 1e-10 * abs(E_exact)
