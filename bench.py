@@ -427,7 +427,7 @@ def main():
             f"(corrector) of {hb.query('tile_candidates')}, {tune_ms:.4f} ms per predictor + corrector")
     plan = {k: hb.query(k) for k in ("layout", "tile_sp", "tile_T1", "tile_T2", "tile_threads", "tile_work",
                                      "tile_lds", "tile_tuned", "correct_sp", "correct_T1", "correct_T2",
-                                     "correct_threads", "correct_work", "tile_tuned_correct")}
+                                     "correct_threads", "correct_work", "tile_tuned_correct", "pair_on")}
     log(f"FP32 batch: {R} replicas, {batch.nsites} spins, plan {plan}")
     dt32, fams32 = timed_steps(batch, integ, args.steps, args.warmup)
     ups32 = batch.nsites * args.steps / dt32

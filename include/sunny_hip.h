@@ -242,6 +242,9 @@ int snh_local_coloring(snh_system *sys, int32_t m[3], int32_t *ncolors, int32_t 
  * untouched) and keeps the fastest plan for each kernel for all later calls
  * (the predictor's plan also serves energy and gradients, the corrector's the
  * midpoint iteration); a later-ranked plan must be 2% faster to be kept.
+ * Tiled batches of two or more replicas are then also timed as two replica
+ * groups on two streams in anti-phase; that schedule is kept for Langevin
+ * steps when it is 2% faster ("pair_on").
  * best_ms (may be NULL): predictor + corrector time of the kept plans.  A
  * handle on the ghost-padded layout is left as it is (best_ms = 0).  The
  * kept ranks are visible through snh_query ("tile_tuned",
@@ -260,9 +263,10 @@ int snh_autotune(snh_system *sys, int32_t candidates, int32_t launches, double *
  * (Langevin steps per captured HIP graph; 0 = graphs off, the default; set
  * by the environment variable SNH_GRAPH_STEPS at snh_create), "graph_live"
  * (1 when a captured graph is instantiated), "pair_split" (first replica of
- * the second replica group of tiled Langevin batches, stepped on a second
- * stream in anti-phase with the first; 0 = one stream, as with SNH_PAIR=0 at
- * snh_create). */
+ * the second replica group of a tiled Langevin batch; 0 = no groups, as with
+ * SNH_PAIR=0 at snh_create), "pair_on" (1 when the groups step on two
+ * streams in anti-phase: chosen by snh_autotune when whole steps are
+ * measurably faster that way, or forced by SNH_PAIR=2). */
 int snh_query(snh_system *sys, const char *key, int64_t *value);
 
 #ifdef __cplusplus

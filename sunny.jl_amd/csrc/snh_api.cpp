@@ -465,7 +465,8 @@ struct SystemImpl final : snh_system {
     // Tiled Langevin batches of two or more replicas (no Ewald) step as two
     // replica groups on two streams in anti-phase: one group's corrector
     // (latency-bound) runs beside the other's predictor (VALU-bound).
-    int pair_split = 0;        // first replica of the second group (0: off)
+    int pair_split = 0;        // first replica of the second group (0: no groups)
+    bool pair_on = false;      // step in groups: SNH_PAIR=2, or chosen by snh_autotune (SNH_PAIR=1)
     hipStream_t st2 = nullptr;
     hipEvent_t pev[3] = {nullptr, nullptr, nullptr};  // start, first group's predictor, second group's predictor
     int* fail_step = nullptr;  // [R] persistent midpoint: step that did not converge
@@ -800,7 +801,8 @@ struct SystemImpl final : snh_system {
         // (4096 sites: 64 vs 10.7 us)
         persist = N == 0 && !ewald && !tiled && env_int("SNH_PERSIST", 1) != 0 &&
                   max_sites <= env_int("SNH_PERSIST_MAX_SITES", kPersist);
-        if (tiled && !ewald && R >= 2 && env_int("SNH_PAIR", 1) != 0) {
+        const int pair_env = env_int("SNH_PAIR", 1);
+        if (tiled && !ewald && R >= 2 && pair_env != 0) {
             // split the replicas into two groups of about equal sites
             long long all = 0, acc = 0;
             for (int r = 0; r < R; r++) all += replica_sites(r);
@@ -811,6 +813,7 @@ struct SystemImpl final : snh_system {
                 if (2 * acc >= all) break;
             }
             pair_split = h;
+            pair_on = pair_env == 2;
             SNH_HIP(hipStreamCreate(&st2));
             for (hipEvent_t& e : pev) SNH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         }
@@ -829,8 +832,9 @@ struct SystemImpl final : snh_system {
     // group 0 on st, group 1 on st2; group 1's predictor waits for group 0's,
     // and group 0's next predictor for group 1's, so each corrector overlaps
     // the other group's predictor.  Per-site arithmetic is that of
-    // langevin_step, so results are identical.
-    void langevin_pairs(NoiseArgs<T> nz, T tdt, T tl, uint64_t step0, int s0, int s1)
+    // langevin_step, so results are identical.  The correctors write to Sout
+    // (the state X, or a scratch buffer when timing).
+    void langevin_pairs(NoiseArgs<T> nz, T tdt, T tl, uint64_t step0, int s0, int s1, T* Sout)
     {
         const TileArgs p0 = plan_part(tp, 0, pair_split), p1 = plan_part(tp, pair_split, R);
         const TileArgs c0 = plan_part(tc, 0, pair_split), c1 = plan_part(tc, pair_split, R);
@@ -841,11 +845,11 @@ struct SystemImpl final : snh_system {
             if (s > s0) SNH_HIP(hipStreamWaitEvent(st, pev[2], 0));
             launch_tile_predict<T>(C, p0, tp.threads, X, Xp, Xo, nullptr, nz, tdt, tl, st);
             SNH_HIP(hipEventRecord(pev[1], st));
-            launch_tile_correct<T>(C, c0, tc.threads, Xp, Xo, X, nullptr, tdt, tl, st);
+            launch_tile_correct<T>(C, c0, tc.threads, Xp, Xo, Sout, nullptr, tdt, tl, st);
             SNH_HIP(hipStreamWaitEvent(st2, pev[1], 0));
             launch_tile_predict<T>(C, p1, tp.threads, X, Xp, Xo, nullptr, nz, tdt, tl, st2);
             SNH_HIP(hipEventRecord(pev[2], st2));
-            launch_tile_correct<T>(C, c1, tc.threads, Xp, Xo, X, nullptr, tdt, tl, st2);
+            launch_tile_correct<T>(C, c1, tc.threads, Xp, Xo, Sout, nullptr, tdt, tl, st2);
         }
         SNH_HIP(hipEventRecord(pev[0], st2));
         SNH_HIP(hipStreamWaitEvent(st, pev[0], 0));
@@ -1132,13 +1136,37 @@ struct SystemImpl final : snh_system {
                     best[k] = c;
                 }
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
         apply_plan(tile_cands[best[0]], 0);
         apply_plan(tile_cands[best[1]], 1);
         tuned_index[0] = best[0];
         tuned_index[1] = best[1];
         if (best_ms) *best_ms = best_t[0] + best_t[1];
+        // two replica groups in anti-phase pay off for some plan pairs and
+        // not for others: keep them when whole steps (correctors into the
+        // scratch buffer) are 2% faster than on one stream
+        if (pair_split > 0 && env_int("SNH_PAIR", 1) == 1) {
+            const int nstep = std::max(launches, 8);
+            auto steps = [&](auto&& run) {  // a warm-up run, then one timed run of nstep steps
+                run();
+                SNH_HIP(hipEventRecord(e0, st));
+                run();
+                SNH_HIP(hipEventRecord(e1, st));
+                SNH_HIP(hipEventSynchronize(e1));
+                float ms = 0;
+                SNH_HIP(hipEventElapsedTime(&ms, e0, e1));
+                return ms;
+            };
+            const float one = steps([&] {
+                for (int k = 0; k < nstep; k++) {
+                    launch_tile_predict<T>(C, tp.A, tp.threads, X, Xp, Xo, nullptr, nz, dt, lam, st);
+                    launch_tile_correct<T>(C, tc.A, tc.threads, Xp, Xo, B1, nullptr, dt, lam, st);
+                }
+            });
+            const float two = steps([&] { langevin_pairs(nz, dt, lam, 0, 0, nstep, B1); });
+            pair_on = two < 0.98f * one;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     }
 
     // SampledCorrelations(sys; measure, energies, dt) buffers
@@ -1552,8 +1580,8 @@ struct SystemImpl final : snh_system {
                 for (; nsteps - s >= graph_steps; s += graph_steps) SNH_HIP(hipGraphLaunch(graph_exec, st));
             }
         }
-        if (pair_split > 0 && !timer.enabled && s < nsteps) {
-            langevin_pairs(nz, tdt, tl, step0, s, nsteps);
+        if (pair_on && !timer.enabled && s < nsteps) {
+            langevin_pairs(nz, tdt, tl, step0, s, nsteps, X);
             s = nsteps;
         }
         for (; s < nsteps; s++) {
@@ -1966,6 +1994,7 @@ struct SystemImpl final : snh_system {
         if (key == "persistent") return persist ? 1 : 0;
         if (key == "graph_live") return graph_exec ? 1 : 0;
         if (key == "pair_split") return pair_split;
+        if (key == "pair_on") return pair_on ? 1 : 0;
         throw Failure{SNH_ERR_INVALID, "unknown query key: " + key};
     }
 

@@ -28,7 +28,7 @@ def state(obj):
 @pytest.mark.parametrize("precision", [32, 64])
 def test_paired_groups_match_one_stream(precision, monkeypatch):
     announce(f"paired replica groups fp{precision}")
-    monkeypatch.setenv("SNH_PAIR", "1")
+    monkeypatch.setenv("SNH_PAIR", "2")
     a = batch(precision)
     ha = a._device()
     monkeypatch.setenv("SNH_PAIR", "0")
@@ -36,7 +36,7 @@ def test_paired_groups_match_one_stream(precision, monkeypatch):
     hb = b._device()
     if ha.query("layout") != 0:
         pytest.skip("the batch does not fit the tiled layout here")
-    assert ha.query("pair_split") > 0 and hb.query("pair_split") == 0
+    assert ha.query("pair_on") == 1 and hb.query("pair_on") == 0
     kTs = np.linspace(0.2, 0.9, a.nreplicas)
     integ = sh.Langevin(0.02, damping=0.1, kT=0.5)
     for _ in range(3):
