@@ -74,9 +74,12 @@ def measured_traffic(kernel, plan):
         return None
     with open(path) as f:
         d = json.load(f)
-    # the launch geometry must match (the autotuner's ranks do not matter)
-    geom = ("layout", "tile_sp", "tile_T1", "tile_T2", "tile_threads", "tile_work", "tile_lds", "correct_sp",
-            "correct_T1", "correct_T2", "correct_threads", "correct_work")
+    # the launch geometry of this kernel's plan must match (the autotuner's
+    # ranks, and the other stage's plan, do not matter)
+    if kernel == "dipole_predict":
+        geom = ("layout", "tile_sp", "tile_T1", "tile_T2", "tile_threads", "tile_work", "tile_lds")
+    else:
+        geom = ("layout", "correct_sp", "correct_T1", "correct_T2", "correct_threads", "correct_work")
     have = d.get("plan") or {}
     if any(have.get(k) != plan.get(k) for k in geom) or kernel not in d.get("kernels", {}):
         return None
