@@ -626,6 +626,10 @@ struct SystemImpl final : snh_system {
             reps.push_back(ri);
         }
         total = (uint32_t)off;
+        // the tiled sweep reads the three planes of a state through one
+        // buffer descriptor with 32-bit byte offsets
+        SNH_REQUIRE(!tiled || 3ull * total * sizeof(T) < (1ull << 32), SNH_ERR_UNSUPPORTED,
+                    "a tiled batch's state must stay below 4 GiB (32-bit offsets); split the batch");
         // blocks: replica-major, then chunks of kBlock interior cells, then
         // sublattice: consecutive blocks cover the same cells of different
         // sublattices, which is where a site's bonded neighbours live

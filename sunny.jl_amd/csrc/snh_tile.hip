@@ -66,6 +66,52 @@ __device__ __forceinline__ void plane_store(T* __restrict__ P, uint32_t total, u
     stg(P, g + 2 * total, v.z);
 }
 
+// Buffer descriptor over `bytes` bytes from p (raw addressing, no stride),
+// and a load of one T at byte offset voff (per lane) + soff (wavefront
+// uniform, a scalar register): no per-load address arithmetic in vector
+// registers.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+template <typename T>
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, T v)
+{
+    typedef __attribute__((__vector_size__(2 * sizeof(unsigned int)))) unsigned int u32x2;
+    if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+
+// The three planes of a state through a descriptor over all of them: the
+// site at byte offset voff of layer 0, layer offset soff (bytes), plane
+// stride pb (bytes).
+template <typename T>
+__device__ __forceinline__ V3<T> buf_v3(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, uint32_t pb)
+{
+    return {buf_ld<T>(r, voff, soff), buf_ld<T>(r, voff, soff + pb), buf_ld<T>(r, voff, soff + 2 * pb)};
+}
+
+template <typename T>
+__device__ __forceinline__ void buf_store_v3(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, uint32_t pb,
+                                             V3<T> v)
+{
+    buf_st<T>(r, voff, soff, v.x);
+    buf_st<T>(r, voff, soff + pb, v.y);
+    buf_st<T>(r, voff, soff + 2 * pb, v.z);
+}
+
 // Sum of one double per thread over a workgroup of any multiple of 64 up to 1024.
 __device__ __forceinline__ double tile_block_sum(double v)
 {
@@ -119,7 +165,11 @@ __device__ __forceinline__ void tile_sweep(const Couplings<T>& C, const TileArgs
     const int NT = blockDim.x;
     const uint32_t total = C.total;
     const uint32_t lay = (uint32_t)R.L1 * (uint32_t)R.L2;
-    uint32_t lg[kLoadPer];
+    constexpr uint32_t esz = sizeof(T);
+    // one descriptor per source over its three planes (< 4 GiB, checked on the host)
+    const __amdgpu_buffer_rsrc_t r0 = plane_rsrc(s0, 3u * total * esz);
+    const __amdgpu_buffer_rsrc_t r1 = plane_rsrc(NSRC == 2 ? s1 : s0, 3u * total * esz);
+    uint32_t lg[kLoadPer];  // byte offset of the position in layer 0
     int lp[kLoadPer];
     T lk[kLoadPer];
 #pragma unroll
@@ -133,20 +183,19 @@ __device__ __forceinline__ void tile_sweep(const Couplings<T>& C, const TileArgs
         const int x = pp % A.ROW, t = pp / A.ROW;
         const int j = t % C.na, row = t / C.na;
         const int c1 = wrap(W.t1 + x - A.w1, R.L1), c2 = wrap(W.t2 + row - A.w2, R.L2);
-        lg[m] = R.off + (uint32_t)(c1 + R.L1 * c2) + (uint32_t)R.ncell * (uint32_t)j;
+        lg[m] = (R.off + (uint32_t)(c1 + R.L1 * c2) + (uint32_t)R.ncell * (uint32_t)j) * esz;
         lk[m] = NSRC == 2 ? ldc(C.kappa, j) : (T)0;
     }
     T buf[kLoadPer][3 * NSRC];
     auto fetch = [&](int kk) __attribute__((always_inline)) {
-        const uint32_t o = lay * (uint32_t)wrap(kk, R.L3);
+        const uint32_t o = lay * (uint32_t)wrap(kk, R.L3) * esz;
 #pragma unroll
         for (int m = 0; m < kLoadPer; m++) {
-            const uint32_t g = lg[m] + o;
 #pragma unroll
-            for (int a = 0; a < 3; a++) buf[m][a] = ldg(s0, g + a * total);
+            for (int a = 0; a < 3; a++) buf[m][a] = buf_ld<T>(r0, lg[m], o + a * total * esz);
             if constexpr (NSRC == 2) {
 #pragma unroll
-                for (int a = 0; a < 3; a++) buf[m][3 + a] = ldg(s1, g + a * total);
+                for (int a = 0; a < 3; a++) buf[m][3 + a] = buf_ld<T>(r1, lg[m], o + a * total * esz);
             }
         }
     };
@@ -292,23 +341,24 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
     const TileWork W = A.work[logical_block()];
     const ReplicaInfo R = C.reps[W.r];
     SNH_TILE_SITES(T);
+    const uint32_t pb = C.total * (uint32_t)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rH = plane_rsrc(H, 3 * pb), rS = plane_rsrc(Sout, 3 * pb);
     auto load = [&](const TileSite& ts, int k, OwnIn<T>& in) __attribute__((always_inline)) {
         const uint32_t g = ts.gb + lay * (uint32_t)k;
-        in.a = plane_v3(H, C.total, g);
+        in.a = buf_v3<T>(rH, ts.gb * (uint32_t)sizeof(T), lay * (uint32_t)k * (uint32_t)sizeof(T), pb);
         if (phi) in.p = plane_v3(phi, C.total, g);
     };
     auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) __attribute__((always_inline)) {
         if (!ts.ok) return;
         const uint32_t own = ts.own + slot * (uint32_t)ph;
-        const uint32_t g = ts.gb + lay * (uint32_t)k;
         const int i = ts.i;
         const V3<T> sp = lds_v3<T>(smem, own);
         V3<T> f = tile_field<T, CH, kIso>(C, offs, smem, own, i, sp);
         if (phi) add_ewald_field(C, i, in.p, f);
         const V3<T> d = rhs_det(sp, f, dt, lambda);
         const V3<T> h = in.a;
-        plane_store(Sout, C.total, g,
-                    normalize_dipole(V3<T>{h.x + d.x / 2, h.y + d.y / 2, h.z + d.z / 2}, ldc(C.kappa, i)));
+        buf_store_v3<T>(rS, ts.gb * (uint32_t)sizeof(T), lay * (uint32_t)k * (uint32_t)sizeof(T), pb,
+                        normalize_dipole(V3<T>{h.x + d.x / 2, h.y + d.y / 2, h.z + d.z / 2}, ldc(C.kappa, i)));
     };
     tile_sweep<T, 1, SP>(
         C, A, W, R, Sp, nullptr, smem,
