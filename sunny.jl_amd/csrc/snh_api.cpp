@@ -1637,49 +1637,115 @@ struct SystemImpl final : snh_system {
             }
             return;
         }
-        T* bufs[2] = {B2, B1};  // S'' of iteration m lives in bufs[m % 2]: m odd -> B1
-        for (int s = 0; s < nsteps; s++) {
+        // Groups of steps run optimistically: each step gets a fixed budget of
+        // iterations (the last count + 1; iterations after convergence are
+        // no-ops) and no host check; a group in which some replica needs more
+        // is undone from a checkpoint and rerun with a host check per step.
+        const bool optimistic = env_int("SNH_MID_SYNC", 0) == 0;
+        constexpr int kGroup = 64;
+        for (int s0 = 0; s0 < nsteps; s0 += kGroup) {
+            const int n = std::min(kGroup, nsteps - s0);
+            const int budget = std::min(max_iters, last_iters + 1);
+            if (optimistic && midpoint_budgeted(n, budget, tdt, tl, iters)) continue;
+            midpoint_checked(n, max_iters, tol, tdt, tl, iters);
+        }
+    }
+
+    // Fixed-point iteration `it` of a midpoint step: S'' of iteration it goes
+    // to bufs[it % 2] (m odd -> B1), then the per-replica convergence check.
+    void midpoint_iteration(int it, T tdt, T tl)
+    {
+        T* bufs[2] = {B2, B1};
+        T* S1 = it == 1 ? X : bufs[(it + 1) % 2];  // the first guess S' = S
+        T* S2 = bufs[it % 2];
+        const T* phi = nullptr;
+        const T* hat = nullptr;
+        T* hat_next = nullptr;
+        if (N == 0) {
+            // Shat of iteration it lives in hats[it % 2]: formed from S and
+            // S1 = S at it = 1, then written by the previous iteration
+            T* hats[2] = {Bh2, Bh};
+            if (it == 1) launch_dipole_midpoint_hat<T>(C, nblocks, X, S1, hats[1], conv_iter, st);
+            hat = hats[it % 2];
+            hat_next = hats[(it + 1) % 2];
+            if (ewald) {
+                ewald_field(hat, phi1);
+                phi = phi1;
+            }
+        } else if (ewald) {
+            launch_sun_midpoint_bar<T>(C, nblocks, X, S1, Bh, conv_iter, st);
+            ewald_field(Bh, phi1);
+            phi = phi1;
+        }
+        hipEvent_t e = tbegin();
+        if (tiled)
+            launch_tile_midpoint_iter<T>(C, tc.A, tc.threads, X, S1, S2, hat, phi, tdt, tl, partial, conv_iter,
+                                         hat_next, st);
+        else if (N == 0)
+            launch_dipole_midpoint_iter<T>(C, nblocks, X, S1, S2, hat, phi, tdt, tl, partial, conv_iter, hat_next,
+                                           st);
+        else
+            launch_sun_midpoint_iter<T>(C, nblocks, X, S1, S2, phi, tdt, tl, partial, conv_iter, st);
+        tend("midpoint_iter", e);
+        launch_midpoint_check(partial, tiled ? d_tcblk : d_rblk, R, atol2, conv_iter, it, st);
+    }
+
+    // S = normalize(S'') for every converged replica.
+    void midpoint_finalize()
+    {
+        if (N == 0)
+            launch_dipole_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, 0, st);
+        else
+            launch_sun_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, 0, st);
+    }
+
+    // n steps of `budget` iterations each, with one host check at the end.
+    // Returns false, with the state restored to the start, when a replica
+    // had not converged within the budget at some step.
+    bool midpoint_budgeted(int n, int budget, T tdt, T tl, int* iters)
+    {
+        if (!fail_step) fail_step = own(dalloc<int>(R));
+        const size_t bytes = sizeof(T) * planes * (size_t)total;
+        launch_copy_planes(X, Xo, bytes, st);  // checkpoint (Xo is Heun scratch)
+        launch_fill_int(fail_step, R, -1, st);
+        for (int s = 0; s < n; s++) {
+            launch_fill_int(conv_iter, R, 0, st);
+            for (int it = 1; it <= budget; it++) midpoint_iteration(it, tdt, tl);
+            midpoint_finalize();
+            launch_mid_budget(conv_iter, fail_step, R, s, st);
+        }
+        SNH_HIP(hipGetLastError());
+        SNH_HIP(hipMemcpyAsync(pinned_i, fail_step, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+        SNH_HIP(hipStreamSynchronize(st));
+        bool ok = true;
+        for (int r = 0; r < R; r++) ok &= pinned_i[r] < 0;
+        if (!ok) {
+            launch_copy_planes(Xo, X, bytes, st);
+            SNH_HIP(hipGetLastError());
+            return false;
+        }
+        SNH_HIP(hipMemcpyAsync(pinned_i, conv_iter, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+        SNH_HIP(hipStreamSynchronize(st));
+        int mx = 0;
+        for (int r = 0; r < R; r++) mx = std::max(mx, pinned_i[r]);
+        last_iters = mx;
+        if (iters)
+            for (int r = 0; r < R; r++) iters[r] = pinned_i[r];
+        return true;
+    }
+
+    // n steps, each iterated with host convergence checks until every replica
+    // converged (or max_iters: the reference's error).
+    void midpoint_checked(int n, int max_iters, double tol, T tdt, T tl, int* iters)
+    {
+        for (int s = 0; s < n; s++) {
             launch_fill_int(conv_iter, R, 0, st);
             int it = 0;
             bool done = false;
             int chunk = std::max(1, std::min(last_iters, max_iters));
             while (!done) {
-                int upto = std::min(max_iters, it + chunk);
-                for (; it < upto;) {
-                    ++it;
-                    T* S1 = it == 1 ? X : bufs[(it + 1) % 2];  // the first guess S' = S
-                    T* S2 = bufs[it % 2];
-                    const T* phi = nullptr;
-                    const T* hat = nullptr;
-                    T* hat_next = nullptr;
-                    if (N == 0) {
-                        // Shat of iteration it lives in hats[it % 2]: formed from S and
-                        // S1 = S at it = 1, then written by the previous iteration
-                        T* hats[2] = {Bh2, Bh};
-                        if (it == 1) launch_dipole_midpoint_hat<T>(C, nblocks, X, S1, hats[1], conv_iter, st);
-                        hat = hats[it % 2];
-                        hat_next = hats[(it + 1) % 2];
-                        if (ewald) {
-                            ewald_field(hat, phi1);
-                            phi = phi1;
-                        }
-                    } else if (ewald) {
-                        launch_sun_midpoint_bar<T>(C, nblocks, X, S1, Bh, conv_iter, st);
-                        ewald_field(Bh, phi1);
-                        phi = phi1;
-                    }
-                    hipEvent_t e = tbegin();
-                    if (tiled)
-                        launch_tile_midpoint_iter<T>(C, tc.A, tc.threads, X, S1, S2, hat, phi, tdt, tl, partial,
-                                                     conv_iter, hat_next, st);
-                    else if (N == 0)
-                        launch_dipole_midpoint_iter<T>(C, nblocks, X, S1, S2, hat, phi, tdt, tl, partial, conv_iter,
-                                                       hat_next, st);
-                    else
-                        launch_sun_midpoint_iter<T>(C, nblocks, X, S1, S2, phi, tdt, tl, partial, conv_iter, st);
-                    tend("midpoint_iter", e);
-                    launch_midpoint_check(partial, tiled ? d_tcblk : d_rblk, R, atol2, conv_iter, it, st);
-                }
+                const int upto = std::min(max_iters, it + chunk);
+                while (it < upto) midpoint_iteration(++it, tdt, tl);
                 SNH_HIP(hipGetLastError());
                 SNH_HIP(hipMemcpyAsync(pinned_i, conv_iter, sizeof(int) * R, hipMemcpyDeviceToHost, st));
                 SNH_HIP(hipStreamSynchronize(st));
@@ -1688,10 +1754,7 @@ struct SystemImpl final : snh_system {
                 // S = normalize(S'') for every converged replica, once per step
                 // (also before reporting a replica that did not converge)
                 if (all || it >= max_iters) {
-                    if (N == 0)
-                        launch_dipole_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, 0, st);
-                    else
-                        launch_sun_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, 0, st);
+                    midpoint_finalize();
                     SNH_HIP(hipGetLastError());
                 }
                 if (all) {

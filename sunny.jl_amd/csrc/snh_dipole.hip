@@ -384,6 +384,14 @@ __global__ void k_fill_int(int* __restrict__ p, int n, int v)
     if (k < n) p[k] = v;
 }
 
+// Marks replicas that have not converged by the end of step s's iteration
+// budget: fail_step[r] = s (the first such step).
+__global__ void k_mid_budget(const int* __restrict__ conv_iter, int* __restrict__ fail_step, int R, int s)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R && conv_iter[r] == 0 && fail_step[r] < 0) fail_step[r] = s;
+}
+
 __global__ void k_step_set(unsigned long long* __restrict__ p, unsigned long long v)
 {
     if (threadIdx.x == 0) p[0] = v;
@@ -485,6 +493,11 @@ void launch_midpoint_check(const double* partial, const int* rblk, int R, const 
 void launch_fill_int(int* p, int n, int v, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+}
+
+void launch_mid_budget(const int* conv_iter, int* fail_step, int R, int s, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mid_budget, dim3((R + 255) / 256), dim3(256), 0, st, conv_iter, fail_step, R, s);
 }
 
 void launch_step_set(unsigned long long* p, unsigned long long v, hipStream_t st)
