@@ -118,7 +118,11 @@ int snh_step_langevin(snh_system *sys, double dt, double damping, const double *
  * times (src/Integrators.jl:379-452).  iters_out: NULL or [R], the
  * fixed-point iterations of the last step.  A replica that does not converge
  * leaves its state at the last completed step and the call returns
- * SNH_ERR_NOCONVERGE with the reference's message. */
+ * SNH_ERR_NOCONVERGE with the reference's message.  Steps are queued in
+ * groups with an iteration budget and one host check per group (a group in
+ * which a replica needs more iterations is undone and rerun with a check per
+ * step; SNH_MID_SYNC=1 in the environment checks every step); the
+ * results do not depend on it. */
 int snh_step_midpoint(snh_system *sys, double dt, double damping, double tol, int32_t max_iters, int32_t nsteps,
                       int32_t *iters_out);
 
