@@ -121,7 +121,7 @@ int snh_step_langevin(snh_system *sys, double dt, double damping, const double *
  * SNH_ERR_NOCONVERGE with the reference's message.  Steps are queued in
  * groups with an iteration budget and one host check per group (a group in
  * which a replica needs more iterations is undone and rerun with a check per
- * step; SNH_MID_SYNC=1 in the environment checks every step); the
+ * step; SNH_MID_SYNC=1 in the environment at snh_create checks every step); the
  * results do not depend on it. */
 int snh_step_midpoint(snh_system *sys, double dt, double damping, double tol, int32_t max_iters, int32_t nsteps,
                       int32_t *iters_out);
@@ -270,7 +270,8 @@ int snh_autotune(snh_system *sys, int32_t candidates, int32_t launches, double *
  * the second replica group of a tiled Langevin batch; 0 = no groups, as with
  * SNH_PAIR=0 at snh_create), "pair_on" (1 when the groups step on two
  * streams in anti-phase: chosen by snh_autotune when whole steps are
- * measurably faster that way, or forced by SNH_PAIR=2). */
+ * measurably faster that way, or forced by SNH_PAIR=2). "mid_reruns"
+ * (midpoint step groups undone and rerun with a host check per step). */
 int snh_query(snh_system *sys, const char *key, int64_t *value);
 
 #ifdef __cplusplus

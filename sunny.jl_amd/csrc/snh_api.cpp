@@ -467,6 +467,8 @@ struct SystemImpl final : snh_system {
     // (latency-bound) runs beside the other's predictor (VALU-bound).
     int pair_split = 0;        // first replica of the second group (0: no groups)
     bool pair_on = false;      // step in groups: SNH_PAIR=2, or chosen by snh_autotune (SNH_PAIR=1)
+    bool mid_sync = false;     // SNH_MID_SYNC=1: host convergence check after every midpoint step
+    int64_t mid_reruns = 0;    // midpoint step groups undone and rerun with per-step checks
     hipStream_t st2 = nullptr;
     hipEvent_t pev[3] = {nullptr, nullptr, nullptr};  // start, first group's predictor, second group's predictor
     int* fail_step = nullptr;  // [R] persistent midpoint: step that did not converge
@@ -805,6 +807,7 @@ struct SystemImpl final : snh_system {
         // (4096 sites: 64 vs 10.7 us)
         persist = N == 0 && !ewald && !tiled && env_int("SNH_PERSIST", 1) != 0 &&
                   max_sites <= env_int("SNH_PERSIST_MAX_SITES", kPersist);
+        mid_sync = env_int("SNH_MID_SYNC", 0) != 0;
         const int pair_env = env_int("SNH_PAIR", 1);
         if (tiled && !ewald && R >= 2 && pair_env != 0) {
             // split the replicas into two groups of about equal sites
@@ -1641,7 +1644,7 @@ struct SystemImpl final : snh_system {
         // iterations (the last count + 1; iterations after convergence are
         // no-ops) and no host check; a group in which some replica needs more
         // is undone from a checkpoint and rerun with a host check per step.
-        const bool optimistic = env_int("SNH_MID_SYNC", 0) == 0;
+        const bool optimistic = !mid_sync;
         constexpr int kGroup = 64;
         for (int s0 = 0; s0 < nsteps; s0 += kGroup) {
             const int n = std::min(kGroup, nsteps - s0);
@@ -1720,6 +1723,7 @@ struct SystemImpl final : snh_system {
         bool ok = true;
         for (int r = 0; r < R; r++) ok &= pinned_i[r] < 0;
         if (!ok) {
+            ++mid_reruns;
             launch_copy_planes(Xo, X, bytes, st);
             SNH_HIP(hipGetLastError());
             return false;
@@ -2062,6 +2066,7 @@ struct SystemImpl final : snh_system {
         if (key == "graph_live") return graph_exec ? 1 : 0;
         if (key == "pair_split") return pair_split;
         if (key == "pair_on") return pair_on ? 1 : 0;
+        if (key == "mid_reruns") return mid_reruns;
         throw Failure{SNH_ERR_INVALID, "unknown query key: " + key};
     }
 

@@ -54,6 +54,7 @@ def test_budgeted_groups_match_checked_steps(kind, precision, monkeypatch):
         ib = sh.step_(b, integ, nsteps=n)
         np.testing.assert_array_equal(ia, ib)
         np.testing.assert_array_equal(state(a), state(b))
+    assert a._device().query("mid_reruns") >= 1 and b._device().query("mid_reruns") == 0
 
 
 def test_budgeted_groups_report_nonconvergence(monkeypatch):
