@@ -119,10 +119,11 @@ int snh_step_langevin(snh_system *sys, double dt, double damping, const double *
  * fixed-point iterations of the last step.  A replica that does not converge
  * leaves its state at the last completed step and the call returns
  * SNH_ERR_NOCONVERGE with the reference's message.  Steps are queued in
- * groups with an iteration budget and one host check per group (a group in
- * which a replica needs more iterations is undone and rerun with a check per
- * step; SNH_MID_SYNC=1 in the environment at snh_create checks every step); the
- * results do not depend on it. */
+ * groups of up to 16 with an iteration budget (the previous group's largest
+ * count + 1) and one host check per group; a group in which a replica needs
+ * more iterations is undone, tried once with two more, then rerun with a
+ * check per step.  SNH_MID_SYNC=1 in the environment at snh_create checks
+ * every step.  The results do not depend on any of this. */
 int snh_step_midpoint(snh_system *sys, double dt, double damping, double tol, int32_t max_iters, int32_t nsteps,
                       int32_t *iters_out);
 
@@ -271,7 +272,8 @@ int snh_autotune(snh_system *sys, int32_t candidates, int32_t launches, double *
  * SNH_PAIR=0 at snh_create), "pair_on" (1 when the groups step on two
  * streams in anti-phase: chosen by snh_autotune when whole steps are
  * measurably faster that way, or forced by SNH_PAIR=2). "mid_reruns"
- * (midpoint step groups undone and rerun with a host check per step). */
+ * (midpoint step groups undone after overrunning their iteration budget),
+ * "mid_fallbacks" (groups then rerun with a host check per step). */
 int snh_query(snh_system *sys, const char *key, int64_t *value);
 
 #ifdef __cplusplus

@@ -468,10 +468,12 @@ struct SystemImpl final : snh_system {
     int pair_split = 0;        // first replica of the second group (0: no groups)
     bool pair_on = false;      // step in groups: SNH_PAIR=2, or chosen by snh_autotune (SNH_PAIR=1)
     bool mid_sync = false;     // SNH_MID_SYNC=1: host convergence check after every midpoint step
-    int64_t mid_reruns = 0;    // midpoint step groups undone and rerun with per-step checks
+    int64_t mid_reruns = 0;     // midpoint step groups undone after overrunning their budget
+    int64_t mid_fallbacks = 0;  // midpoint step groups then run with per-step host checks
     hipStream_t st2 = nullptr;
     hipEvent_t pev[3] = {nullptr, nullptr, nullptr};  // start, first group's predictor, second group's predictor
-    int* fail_step = nullptr;  // [R] persistent midpoint: step that did not converge
+    int* fail_step = nullptr;  // [R] persistent midpoint / budgeted groups: step that did not converge
+    int* max_iter = nullptr;   // [R] budgeted midpoint groups: largest iteration count
     // Sampled correlations of one :dipole replica (snh_corr.hip)
     struct Corr {
         int nobs = 0, ncorr = 0, nt = 0, measperiod = 0;  // nt: time slots T = n_all_omega
@@ -1644,12 +1646,18 @@ struct SystemImpl final : snh_system {
         // iterations (the last count + 1; iterations after convergence are
         // no-ops) and no host check; a group in which some replica needs more
         // is undone from a checkpoint and rerun with a host check per step.
+        // A group that overran its budget is tried once more with two more
+        // iterations per step before falling back to per-step checks.
         const bool optimistic = !mid_sync;
-        constexpr int kGroup = 64;
+        constexpr int kGroup = 16;
         for (int s0 = 0; s0 < nsteps; s0 += kGroup) {
             const int n = std::min(kGroup, nsteps - s0);
             const int budget = std::min(max_iters, last_iters + 1);
             if (optimistic && midpoint_budgeted(n, budget, tdt, tl, iters)) continue;
+            if (optimistic && budget < max_iters &&
+                midpoint_budgeted(n, std::min(max_iters, budget + 2), tdt, tl, iters))
+                continue;
+            if (optimistic) ++mid_fallbacks;
             midpoint_checked(n, max_iters, tol, tdt, tl, iters);
         }
     }
@@ -1708,6 +1716,7 @@ struct SystemImpl final : snh_system {
     bool midpoint_budgeted(int n, int budget, T tdt, T tl, int* iters)
     {
         if (!fail_step) fail_step = own(dalloc<int>(R));
+        if (!max_iter) max_iter = own(dalloc<int>(R));
         const size_t bytes = sizeof(T) * planes * (size_t)total;
         launch_copy_planes(X, Xo, bytes, st);  // checkpoint (Xo is Heun scratch)
         launch_fill_int(fail_step, R, -1, st);
@@ -1715,7 +1724,7 @@ struct SystemImpl final : snh_system {
             launch_fill_int(conv_iter, R, 0, st);
             for (int it = 1; it <= budget; it++) midpoint_iteration(it, tdt, tl);
             midpoint_finalize();
-            launch_mid_budget(conv_iter, fail_step, R, s, st);
+            launch_mid_budget(conv_iter, fail_step, max_iter, R, s, st);
         }
         SNH_HIP(hipGetLastError());
         SNH_HIP(hipMemcpyAsync(pinned_i, fail_step, sizeof(int) * R, hipMemcpyDeviceToHost, st));
@@ -1728,11 +1737,14 @@ struct SystemImpl final : snh_system {
             SNH_HIP(hipGetLastError());
             return false;
         }
-        SNH_HIP(hipMemcpyAsync(pinned_i, conv_iter, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+        // the next group's budget covers the slowest step of this one
+        SNH_HIP(hipMemcpyAsync(pinned_i, max_iter, sizeof(int) * R, hipMemcpyDeviceToHost, st));
         SNH_HIP(hipStreamSynchronize(st));
         int mx = 0;
         for (int r = 0; r < R; r++) mx = std::max(mx, pinned_i[r]);
         last_iters = mx;
+        SNH_HIP(hipMemcpyAsync(pinned_i, conv_iter, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+        SNH_HIP(hipStreamSynchronize(st));
         if (iters)
             for (int r = 0; r < R; r++) iters[r] = pinned_i[r];
         return true;
@@ -2067,6 +2079,7 @@ struct SystemImpl final : snh_system {
         if (key == "pair_split") return pair_split;
         if (key == "pair_on") return pair_on ? 1 : 0;
         if (key == "mid_reruns") return mid_reruns;
+        if (key == "mid_fallbacks") return mid_fallbacks;
         throw Failure{SNH_ERR_INVALID, "unknown query key: " + key};
     }
 

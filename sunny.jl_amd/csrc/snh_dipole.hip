@@ -384,12 +384,17 @@ __global__ void k_fill_int(int* __restrict__ p, int n, int v)
     if (k < n) p[k] = v;
 }
 
-// Marks replicas that have not converged by the end of step s's iteration
-// budget: fail_step[r] = s (the first such step).
-__global__ void k_mid_budget(const int* __restrict__ conv_iter, int* __restrict__ fail_step, int R, int s)
+// After step s of a budgeted group: fail_step[r] = s for replicas that have
+// not converged within the budget (the first such step), and
+// max_iter[r] = the largest iteration count of the group so far.
+__global__ void k_mid_budget(const int* __restrict__ conv_iter, int* __restrict__ fail_step,
+                             int* __restrict__ max_iter, int R, int s)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < R && conv_iter[r] == 0 && fail_step[r] < 0) fail_step[r] = s;
+    if (r >= R) return;
+    const int c = conv_iter[r];
+    if (c == 0 && fail_step[r] < 0) fail_step[r] = s;
+    if (s == 0 || c > max_iter[r]) max_iter[r] = c;
 }
 
 __global__ void k_step_set(unsigned long long* __restrict__ p, unsigned long long v)
@@ -495,9 +500,10 @@ void launch_fill_int(int* p, int n, int v, hipStream_t st)
     hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
 }
 
-void launch_mid_budget(const int* conv_iter, int* fail_step, int R, int s, hipStream_t st)
+void launch_mid_budget(const int* conv_iter, int* fail_step, int* max_iter, int R, int s, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mid_budget, dim3((R + 255) / 256), dim3(256), 0, st, conv_iter, fail_step, R, s);
+    hipLaunchKernelGGL(k_mid_budget, dim3((R + 255) / 256), dim3(256), 0, st, conv_iter, fail_step, max_iter, R,
+                       s);
 }
 
 void launch_step_set(unsigned long long* p, unsigned long long v, hipStream_t st)

@@ -153,8 +153,9 @@ void launch_reduce_partials(const double* partial, const int* rblk, int R, doubl
 void launch_midpoint_check(const double* partial, const int* rblk, int R, const double* atol2, int* conv_iter,
                            int iter, hipStream_t st);
 void launch_fill_int(int* p, int n, int v, hipStream_t st);
-// fail_step[r] = s where replica r has not converged (first such step only).
-void launch_mid_budget(const int* conv_iter, int* fail_step, int R, int s, hipStream_t st);
+// fail_step[r] = s where replica r has not converged (first such step only);
+// max_iter[r]: largest iteration count of the group's steps 0..s.
+void launch_mid_budget(const int* conv_iter, int* fail_step, int* max_iter, int R, int s, hipStream_t st);
 // Persistent small-replica kernels (one workgroup per replica, rblk: its
 // range of logical blocks): nsteps Langevin steps, or nsteps midpoint steps
 // iterated to convergence on the device (iters_out / fail_step per replica).

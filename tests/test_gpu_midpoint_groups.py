@@ -1,6 +1,7 @@
 """Implicit-midpoint steps queued in groups with a fixed iteration budget and
-one host check (undone and rerun step by step when a replica needs more
-iterations) equal, bitwise, steps checked on the host one by one."""
+one host check (undone, retried with a larger budget, then rerun step by step
+when a replica needs more iterations) equal, bitwise, steps checked on the
+host one by one."""
 import numpy as np
 import pytest
 
@@ -55,6 +56,27 @@ def test_budgeted_groups_match_checked_steps(kind, precision, monkeypatch):
         np.testing.assert_array_equal(ia, ib)
         np.testing.assert_array_equal(state(a), state(b))
     assert a._device().query("mid_reruns") >= 1 and b._device().query("mid_reruns") == 0
+    assert b._device().query("mid_fallbacks") == 0
+
+
+def test_budgeted_groups_fall_back_to_checked_steps(monkeypatch):
+    """A tenfold dt jump overruns both the learned budget and its retry (two
+    more iterations): the group runs with per-step checks, bitwise equal to
+    a handle that checks every step."""
+    announce("midpoint groups fallback")
+    monkeypatch.setenv("SNH_PERSIST", "0")
+    a, b = twins("cubic", 64)
+    monkeypatch.setenv("SNH_MID_SYNC", "0")
+    a._device()
+    monkeypatch.setenv("SNH_MID_SYNC", "1")
+    b._device()
+    for dt, n in ((0.005, 3), (0.05, 4)):
+        integ = sh.ImplicitMidpoint(dt, damping=0.0, tol=1e-12)
+        ia = sh.step_(a, integ, nsteps=n)
+        ib = sh.step_(b, integ, nsteps=n)
+        np.testing.assert_array_equal(ia, ib)
+        np.testing.assert_array_equal(state(a), state(b))
+    assert a._device().query("mid_fallbacks") >= 1
 
 
 def test_budgeted_groups_report_nonconvergence(monkeypatch):
