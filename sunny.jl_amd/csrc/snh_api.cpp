@@ -745,6 +745,8 @@ struct SystemImpl final : snh_system {
             }
         }
         kmax_h = kmax;
+        C.onsite_hi = 0;
+        for (int k : kmax) C.onsite_hi |= k > 2;
         std::vector<T> lam, qops;
         if (N > 0) {
             SNH_REQUIRE(sp.Lambda, SNH_ERR_INVALID, ":SUN mode needs the onsite matrices Lambda");

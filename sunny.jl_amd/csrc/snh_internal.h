@@ -34,6 +34,7 @@ struct Couplings {
                          // sublattice's bond list is padded to a multiple of it
                          // with zero couplings to the site itself (flag culled)
     int iso;             // every bond isotropic and without biquadratic coupling
+    int onsite_hi = 1;   // some sublattice has Stevens terms beyond rank 2
     uint32_t total;      // plane stride (padded sites over all replicas)
     Ghost G;
 };

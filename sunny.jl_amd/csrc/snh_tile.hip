@@ -237,8 +237,9 @@ __device__ __forceinline__ void tile_sweep(const Couplings<T>& C, const TileArgs
 }
 
 // dE/dS of a site whose own LDS byte address is `own`: Zeeman, bonds from
-// the ring (CH per batch of LDS reads), onsite anisotropy.
-template <typename T, int CH, bool kIso>
+// the ring (CH per batch of LDS reads), onsite anisotropy (kQ2: of rank at
+// most 2 on every sublattice).
+template <typename T, int CH, bool kIso, bool kQ2 = false>
 __device__ __forceinline__ V3<T> tile_field(const Couplings<T>& C, const int* __restrict__ offs,
                                             const unsigned char* lds, uint32_t own, int i, V3<T> s)
 {
@@ -261,7 +262,16 @@ __device__ __forceinline__ V3<T> tile_field(const Couplings<T>& C, const int* __
             }
         }
     }
-    add_onsite_field(C, i, s, g);
+    if constexpr (kQ2) {
+        if (ldc(C.kmax, i) == 2) {
+            const T* q = C.stev_q2 + 9 * i;
+            g.x += ldc(q, 0) * s.x + ldc(q, 1) * s.y + ldc(q, 2) * s.z;
+            g.y += ldc(q, 3) * s.x + ldc(q, 4) * s.y + ldc(q, 5) * s.z;
+            g.z += ldc(q, 6) * s.x + ldc(q, 7) * s.y + ldc(q, 8) * s.z;
+        }
+    } else {
+        add_onsite_field(C, i, s, g);
+    }
     return g;
 }
 
@@ -286,8 +296,10 @@ struct OwnIn {
     const uint32_t slot = 3u * (uint32_t)sizeof(T_) * (uint32_t)A.cap;                              \
     OwnIn<T_> in0{}, in1{}, nx0{}, nx1{}
 
-// Heun predictor (see heun_predict): S' and H from S.
-template <typename T, int CH, bool kIso, int SP>
+// Heun predictor (see heun_predict): S' and H from S.  kLean: no Ewald
+// potential, device noise with the step as an argument (or none), onsite
+// terms of rank <= 2; fewer live pointers and branches in the layer loop.
+template <typename T, int CH, bool kIso, int SP, bool kLean>
 __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_predict(Couplings<T> C, TileArgs A,
                                                                         const T* __restrict__ S, T* __restrict__ Sp,
                                                                         T* __restrict__ H, const T* __restrict__ phi,
@@ -299,7 +311,7 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
     SNH_TILE_SITES(T);
     const bool noisy = lambda != (T)0;
     auto load = [&](const TileSite& ts, int k, OwnIn<T>& in) __attribute__((always_inline)) {
-        if (phi) in.p = plane_v3(phi, C.total, ts.gb + lay * (uint32_t)k);
+        if (!kLean && phi) in.p = plane_v3(phi, C.total, ts.gb + lay * (uint32_t)k);
     };
     auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) __attribute__((always_inline)) {
         if (!ts.ok) return;
@@ -307,9 +319,10 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
         const uint32_t g = ts.gb + lay * (uint32_t)k;
         const int i = ts.i;
         const V3<T> s = lds_v3<T>(smem, own);
-        V3<T> f = tile_field<T, CH, kIso>(C, offs, smem, own, i, s);
-        if (phi) add_ewald_field(C, i, in.p, f);
-        const V3<T> xi = noisy ? dipole_noise(nz, W.r, g - R.off, g) : V3<T>{0, 0, 0};
+        V3<T> f = tile_field<T, CH, kIso, kLean>(C, offs, smem, own, i, s);
+        if (!kLean && phi) add_ewald_field(C, i, in.p, f);
+        V3<T> xi = {0, 0, 0};
+        if (noisy) xi = kLean ? dipole_noise_philox(nz, W.r, g - R.off) : dipole_noise(nz, W.r, g - R.off, g);
         V3<T> sp, h;
         heun_predict(s, f, xi, noisy, ldc(C.kappa, i), dt, lambda, sp, h);
         plane_store(Sp, C.total, g, sp);
@@ -330,9 +343,19 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
         });
 }
 
-// Heun corrector: S <- kappa*normalize(H + rhs_det(S', dE(S'))/2).
-template <typename T, int CH, bool kIso, int SP>
-__global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_correct(Couplings<T> C, TileArgs A,
+// Waves per SIMD requested for the corrector: the lean FP32 one-site form
+// fits in 64 registers, so two 1024-thread workgroups share a CU.
+template <typename T, int SP, bool kLean>
+constexpr int correct_waves()
+{
+    return (kLean && SP == 1 && sizeof(T) == 4) ? 8 : 1;
+}
+
+// Heun corrector: S <- kappa*normalize(H + rhs_det(S', dE(S'))/2).  kLean:
+// no Ewald potential, onsite terms of rank <= 2.
+template <typename T, int CH, bool kIso, int SP, bool kLean>
+__global__ __attribute__((amdgpu_waves_per_eu(correct_waves<T, SP, kLean>())))
+__launch_bounds__((tile_threads_max<T, SP>())) void k_tile_correct(Couplings<T> C, TileArgs A,
                                                                         const T* __restrict__ Sp,
                                                                         const T* __restrict__ H, T* __restrict__ Sout,
                                                                         const T* __restrict__ phi, T dt, T lambda)
@@ -346,15 +369,15 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
     auto load = [&](const TileSite& ts, int k, OwnIn<T>& in) __attribute__((always_inline)) {
         const uint32_t g = ts.gb + lay * (uint32_t)k;
         in.a = buf_v3<T>(rH, ts.gb * (uint32_t)sizeof(T), lay * (uint32_t)k * (uint32_t)sizeof(T), pb);
-        if (phi) in.p = plane_v3(phi, C.total, g);
+        if (!kLean && phi) in.p = plane_v3(phi, C.total, g);
     };
     auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) __attribute__((always_inline)) {
         if (!ts.ok) return;
         const uint32_t own = ts.own + slot * (uint32_t)ph;
         const int i = ts.i;
         const V3<T> sp = lds_v3<T>(smem, own);
-        V3<T> f = tile_field<T, CH, kIso>(C, offs, smem, own, i, sp);
-        if (phi) add_ewald_field(C, i, in.p, f);
+        V3<T> f = tile_field<T, CH, kIso, kLean>(C, offs, smem, own, i, sp);
+        if (!kLean && phi) add_ewald_field(C, i, in.p, f);
         const V3<T> d = rhs_det(sp, f, dt, lambda);
         const V3<T> h = in.a;
         buf_store_v3<T>(rS, ts.gb * (uint32_t)sizeof(T), lay * (uint32_t)k * (uint32_t)sizeof(T), pb,
@@ -590,11 +613,18 @@ void launch_tile_predict(const Couplings<T>& C, const TileArgs& A, int nthreads,
                          const T* phi, const NoiseArgs<T>& nz, T dt, T lambda, hipStream_t st)
 {
     const size_t lds = tile_lds_bytes(A, sizeof(T));
-#define SNH_TP(CH, ISO, SP)                                                                                      \
-    do {                                                                                                         \
-        launch_tile(k_tile_predict<T, CH, ISO, SP>, A, nthreads, lds, st);                                       \
-        hipLaunchKernelGGL((k_tile_predict<T, CH, ISO, SP>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, S, Sp, \
-                           H, phi, nz, dt, lambda);                                                              \
+    const bool lean = !phi && nz.mode != 1 && !nz.step_ptr && !C.onsite_hi;
+#define SNH_TP(CH, ISO, SP)                                                                                        \
+    do {                                                                                                           \
+        if (lean) {                                                                                                \
+            launch_tile(k_tile_predict<T, CH, ISO, SP, true>, A, nthreads, lds, st);                               \
+            hipLaunchKernelGGL((k_tile_predict<T, CH, ISO, SP, true>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, \
+                               S, Sp, H, phi, nz, dt, lambda);                                                     \
+        } else {                                                                                                   \
+            launch_tile(k_tile_predict<T, CH, ISO, SP, false>, A, nthreads, lds, st);                              \
+            hipLaunchKernelGGL((k_tile_predict<T, CH, ISO, SP, false>), dim3(A.nwork), dim3(nthreads), lds, st, C,   \
+                               A, S, Sp, H, phi, nz, dt, lambda);                                                  \
+        }                                                                                                          \
     } while (0)
     if (C.iso) SNH_TILE_DISPATCH(SNH_TP, true); else SNH_TILE_DISPATCH(SNH_TP, false);
 #undef SNH_TP
@@ -605,11 +635,18 @@ void launch_tile_correct(const Couplings<T>& C, const TileArgs& A, int nthreads,
                          const T* phi, T dt, T lambda, hipStream_t st)
 {
     const size_t lds = tile_lds_bytes(A, sizeof(T));
-#define SNH_TC(CH, ISO, SP)                                                                                      \
-    do {                                                                                                         \
-        launch_tile(k_tile_correct<T, CH, ISO, SP>, A, nthreads, lds, st);                                       \
-        hipLaunchKernelGGL((k_tile_correct<T, CH, ISO, SP>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, Sp, H, \
-                           Sout, phi, dt, lambda);                                                               \
+    const bool lean = !phi && !C.onsite_hi;
+#define SNH_TC(CH, ISO, SP)                                                                                        \
+    do {                                                                                                           \
+        if (lean) {                                                                                                \
+            launch_tile(k_tile_correct<T, CH, ISO, SP, true>, A, nthreads, lds, st);                               \
+            hipLaunchKernelGGL((k_tile_correct<T, CH, ISO, SP, true>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, \
+                               Sp, H, Sout, phi, dt, lambda);                                                      \
+        } else {                                                                                                   \
+            launch_tile(k_tile_correct<T, CH, ISO, SP, false>, A, nthreads, lds, st);                              \
+            hipLaunchKernelGGL((k_tile_correct<T, CH, ISO, SP, false>), dim3(A.nwork), dim3(nthreads), lds, st, C,   \
+                               A, Sp, H, Sout, phi, dt, lambda);                                                   \
+        }                                                                                                          \
     } while (0)
     if (C.iso) SNH_TILE_DISPATCH(SNH_TC, true); else SNH_TILE_DISPATCH(SNH_TC, false);
 #undef SNH_TC

@@ -158,6 +158,19 @@ __device__ __forceinline__ V3<T> dipole_noise(const NoiseArgs<T>& nz, int r, uin
     return {n[0] * s, n[1] * s, n[2] * s};
 }
 
+// dipole_noise without host-supplied draws and with the step number passed
+// as a kernel argument (modes 0 and 2 only).
+template <typename T>
+__device__ __forceinline__ V3<T> dipole_noise_philox(const NoiseArgs<T>& nz, int r, uint32_t site)
+{
+    if (nz.mode == 0) return {0, 0, 0};
+    const T s = ldc(nz.scale, r);
+    T n[4];
+    const unsigned long long key = nz.seed + (unsigned long long)r * 0x9E3779B97F4A7C15ull;
+    philox_normals4<T>(key, nz.step, site, 0u, n);
+    return {n[0] * s, n[1] * s, n[2] * s};
+}
+
 // Heun predictor algebra for one site with field g and noise xi:
 //   dS1 = -S x xi + rhs_det(S, g);  S' = kappa*normalize(S + dS1);
 //   H = S + dS1/2 - (S' x xi)/2  (the corrector's noise term, folded in).
