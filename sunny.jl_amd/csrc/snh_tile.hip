@@ -343,8 +343,9 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
         });
 }
 
-// Waves per SIMD requested for the corrector: the lean FP32 one-site form
-// fits in 64 registers, so two 1024-thread workgroups share a CU.
+// Waves per SIMD requested for the corrector and the midpoint iteration: the
+// lean FP32 one-site forms fit in 64 registers, so two 1024-thread
+// workgroups share a CU.
 template <typename T, int SP, bool kLean>
 constexpr int correct_waves()
 {
@@ -401,8 +402,10 @@ __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_correct(Couplings<T> 
 // One fixed-point sweep of the spherical midpoint method on the ring of
 // Shat = kappa*normalize((S + S1)/2) (or a materialised Shat when a global
 // term needs it), with the FP64 residual sum |S1 - S2|^2 per workgroup.
-template <typename T, int CH, bool kHatGiven, int SP>
-__global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_mid_iter(Couplings<T> C, TileArgs A,
+// kLean: isotropic bilinear bonds, no Ewald potential, onsite rank <= 2.
+template <typename T, int CH, bool kHatGiven, int SP, bool kLean>
+__global__ __attribute__((amdgpu_waves_per_eu(correct_waves<T, SP, kLean && kHatGiven>())))
+__launch_bounds__((tile_threads_max<T, SP>())) void k_tile_mid_iter(Couplings<T> C, TileArgs A,
                                                                          const T* __restrict__ S,
                                                                          const T* __restrict__ S1,
                                                                          T* __restrict__ S2,
@@ -423,7 +426,7 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
         const uint32_t g = ts.gb + lay * (uint32_t)k;
         in.a = plane_v3(S, C.total, g);
         in.b = plane_v3(S1, C.total, g);
-        if (phi) in.p = plane_v3(phi, C.total, g);
+        if (!kLean && phi) in.p = plane_v3(phi, C.total, g);
     };
     auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) __attribute__((always_inline)) {
         if (!ts.ok) return;
@@ -431,8 +434,8 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
         const uint32_t g = ts.gb + lay * (uint32_t)k;
         const int i = ts.i;
         const V3<T> hat = lds_v3<T>(smem, own);
-        V3<T> f = tile_field<T, CH, false>(C, offs, smem, own, i, hat);
-        if (phi) add_ewald_field(C, i, in.p, f);
+        V3<T> f = tile_field<T, CH, kLean, kLean>(C, offs, smem, own, i, hat);
+        if (!kLean && phi) add_ewald_field(C, i, in.p, f);
         const V3<T> d = rhs_det(hat, f, dt, lambda);
         const V3<T> o = {in.a.x + d.x, in.a.y + d.y, in.a.z + d.z};
         plane_store(S2, C.total, g, o);
@@ -658,11 +661,18 @@ void launch_tile_midpoint_iter(const Couplings<T>& C, const TileArgs& A, int nth
                                const int* conv_iter, T* hat_out, hipStream_t st)
 {
     const size_t lds = tile_lds_bytes(A, sizeof(T));
-#define SNH_TM(HAT, SP)                                                                                        \
-    do {                                                                                                       \
-        launch_tile(k_tile_mid_iter<T, 6, HAT, SP>, A, nthreads, lds, st);                                     \
-        hipLaunchKernelGGL((k_tile_mid_iter<T, 6, HAT, SP>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, S, S1, \
-                           S2, Shat, phi, dt, lambda, partial, conv_iter, hat_out);                            \
+    const bool lean = !phi && C.iso && !C.onsite_hi;
+#define SNH_TM(HAT, SP)                                                                                            \
+    do {                                                                                                           \
+        if (lean) {                                                                                                \
+            launch_tile(k_tile_mid_iter<T, 6, HAT, SP, true>, A, nthreads, lds, st);                               \
+            hipLaunchKernelGGL((k_tile_mid_iter<T, 6, HAT, SP, true>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, \
+                               S, S1, S2, Shat, phi, dt, lambda, partial, conv_iter, hat_out);                     \
+        } else {                                                                                                   \
+            launch_tile(k_tile_mid_iter<T, 6, HAT, SP, false>, A, nthreads, lds, st);                              \
+            hipLaunchKernelGGL((k_tile_mid_iter<T, 6, HAT, SP, false>), dim3(A.nwork), dim3(nthreads), lds, st, C,   \
+                               A, S, S1, S2, Shat, phi, dt, lambda, partial, conv_iter, hat_out);                  \
+        }                                                                                                          \
     } while (0)
     if (Shat) {
         if (A.sp == 2) SNH_TM(true, 2); else SNH_TM(true, 1);
