@@ -110,7 +110,10 @@ int snh_get_coherents(snh_system *sys, double *coherents);
  * With SNH_GRAPH_STEPS=n > 0 in the environment at snh_create, device noise
  * (or none) and kernel timing off, runs of whole chunks of n steps replay
  * one captured HIP graph per handle (the noise step counter lives in device
- * memory); results are identical to step-by-step calls. */
+ * memory); results are identical to step-by-step calls.  The tiled kernels
+ * take lean forms (no Ewald potential, device noise with the step as an
+ * argument, onsite terms of rank <= 2) when a call needs nothing else;
+ * SNH_LEAN=0 at snh_create keeps the general forms (identical results). */
 int snh_step_langevin(snh_system *sys, double dt, double damping, const double *kT, const double *noise,
                       uint64_t seed, uint64_t step0, int32_t nsteps);
 

@@ -745,8 +745,10 @@ struct SystemImpl final : snh_system {
             }
         }
         kmax_h = kmax;
-        C.onsite_hi = 0;
-        for (int k : kmax) C.onsite_hi |= k > 2;
+        // SNH_LEAN=0 keeps the general tiled kernels (for comparisons)
+        C.lean_ok = env_int("SNH_LEAN", 1) != 0;
+        for (int k : kmax)
+            if (k > 2) C.lean_ok = 0;
         std::vector<T> lam, qops;
         if (N > 0) {
             SNH_REQUIRE(sp.Lambda, SNH_ERR_INVALID, ":SUN mode needs the onsite matrices Lambda");

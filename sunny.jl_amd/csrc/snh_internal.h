@@ -34,7 +34,8 @@ struct Couplings {
                          // sublattice's bond list is padded to a multiple of it
                          // with zero couplings to the site itself (flag culled)
     int iso;             // every bond isotropic and without biquadratic coupling
-    int onsite_hi = 1;   // some sublattice has Stevens terms beyond rank 2
+    int lean_ok = 0;     // tiled kernels may take their lean forms: every onsite
+                         // term of rank <= 2 (and SNH_LEAN is not 0)
     uint32_t total;      // plane stride (padded sites over all replicas)
     Ghost G;
 };

@@ -616,7 +616,7 @@ void launch_tile_predict(const Couplings<T>& C, const TileArgs& A, int nthreads,
                          const T* phi, const NoiseArgs<T>& nz, T dt, T lambda, hipStream_t st)
 {
     const size_t lds = tile_lds_bytes(A, sizeof(T));
-    const bool lean = !phi && nz.mode != 1 && !nz.step_ptr && !C.onsite_hi;
+    const bool lean = C.lean_ok && !phi && nz.mode != 1 && !nz.step_ptr;
 #define SNH_TP(CH, ISO, SP)                                                                                        \
     do {                                                                                                           \
         if (lean) {                                                                                                \
@@ -638,7 +638,7 @@ void launch_tile_correct(const Couplings<T>& C, const TileArgs& A, int nthreads,
                          const T* phi, T dt, T lambda, hipStream_t st)
 {
     const size_t lds = tile_lds_bytes(A, sizeof(T));
-    const bool lean = !phi && !C.onsite_hi;
+    const bool lean = C.lean_ok && !phi;
 #define SNH_TC(CH, ISO, SP)                                                                                        \
     do {                                                                                                           \
         if (lean) {                                                                                                \
@@ -661,7 +661,7 @@ void launch_tile_midpoint_iter(const Couplings<T>& C, const TileArgs& A, int nth
                                const int* conv_iter, T* hat_out, hipStream_t st)
 {
     const size_t lds = tile_lds_bytes(A, sizeof(T));
-    const bool lean = !phi && C.iso && !C.onsite_hi;
+    const bool lean = C.lean_ok && !phi && C.iso;
 #define SNH_TM(HAT, SP)                                                                                            \
     do {                                                                                                           \
         if (lean) {                                                                                                \
