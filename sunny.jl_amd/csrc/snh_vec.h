@@ -22,29 +22,22 @@ __device__ __forceinline__ V3<T> cross3(V3<T> a, V3<T> b)
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
-// kappa * s/|s| with the norm in FP64 (normalize_dipole, System.jl:421-423).
-// FP64 state: correctly rounded sqrt and divide.  FP32 state: the FP64
-// reciprocal square root (v_rsq_f64) refined by one Newton step, far below
-// the FP32 rounding of the result.
-__device__ __forceinline__ double inv_norm(double n2, double kappa, double)
+// kappa * s/|s| (normalize_dipole, System.jl:421-423).  FP64 state:
+// correctly rounded sqrt and divide.  FP32 state: FP32 throughout, with the
+// hardware reciprocal square root (v_rsq_f32, 1 ulp): |result| is within
+// about 2.5 ulps of kappa.
+__device__ __forceinline__ V3<double> normalize_dipole(V3<double> s, double kappa)
 {
-    return kappa / sqrt(n2);
+    if (kappa == 0.0) return {0, 0, 0};
+    const double f = kappa / sqrt(s.x * s.x + s.y * s.y + s.z * s.z);
+    return {s.x * f, s.y * f, s.z * f};
 }
 
-__device__ __forceinline__ double inv_norm(double n2, double kappa, float)
+__device__ __forceinline__ V3<float> normalize_dipole(V3<float> s, float kappa)
 {
-    double r = __builtin_amdgcn_rsq(n2);
-    r = r * (1.5 - 0.5 * n2 * r * r);
-    return kappa * r;
-}
-
-template <typename T>
-__device__ __forceinline__ V3<T> normalize_dipole(V3<T> s, T kappa)
-{
-    if (kappa == (T)0) return {0, 0, 0};
-    const double x = s.x, y = s.y, z = s.z;
-    const double f = inv_norm(x * x + y * y + z * z, (double)kappa, (T)0);
-    return {(T)(x * f), (T)(y * f), (T)(z * f)};
+    if (kappa == 0.0f) return {0, 0, 0};
+    const float f = kappa * __builtin_amdgcn_rsqf(s.x * s.x + s.y * s.y + s.z * s.z);
+    return {s.x * f, s.y * f, s.z * f};
 }
 
 // Stevens quadrupoles of a classical dipole and their gradient, Spin.jl:48-69.
