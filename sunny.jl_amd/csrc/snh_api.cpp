@@ -793,7 +793,7 @@ struct SystemImpl final : snh_system {
         SNH_HIP(hipMemset(X, 0, sizeof(T) * planes * (size_t)total));
         if (tiled) {
             apply_plan(tile_cands[0], 0);
-            apply_plan(tile_cands[0], 1);
+            apply_plan(tile_cands[default_correct_plan()], 1);
         }
         if (!partial) {
             partial_cap = std::max<size_t>(nblocks, 1);
@@ -1077,6 +1077,22 @@ struct SystemImpl final : snh_system {
     // Makes P the tiled plan of the predictor side (which = 0) or of the
     // corrector side (which = 1): device work list, LDS bond offsets per
     // ring phase, per-replica workgroup ranges, and room for its partial sums.
+    // The corrector's (and midpoint iteration's) plan before any autotuning:
+    // the cost model's first, except that lean FP32 kernels over isotropic
+    // bonds take the same tile with one site per thread where the model
+    // offers it (their one-site forms fit two 1024-thread workgroups per CU).
+    size_t default_correct_plan() const
+    {
+        if (sizeof(T) != 4 || !C.lean_ok || !C.iso || ewald || env_int("SNH_TILE_SP", 0)) return 0;
+        const TilePlan& b = tile_cands[0];
+        for (size_t k = 1; k < tile_cands.size(); k++) {
+            const TilePlan& a = tile_cands[k];
+            if (a.A.sp == 1 && b.A.sp == 2 && a.A.T1 == b.A.T1 && a.A.T2 == b.A.T2 && a.work.size() == b.work.size())
+                return k;
+        }
+        return 0;
+    }
+
     void apply_plan(const TilePlan& P, int which)
     {
         if (st) SNH_HIP(hipStreamSynchronize(st));
