@@ -349,6 +349,7 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
 template <typename T, int SP, bool kLean>
 constexpr int correct_waves()
 {
+    if (SNH_TILE_WAVES_PER_EU > 0) return SNH_TILE_WAVES_PER_EU;  // a build-time override
     return (kLean && SP == 1 && sizeof(T) == 4) ? 8 : 1;
 }
 
