@@ -68,6 +68,8 @@ def lib():
                                           C.c_uint64, C.c_uint64, C.c_double, _dp]
         L.so_local_random_sites.argtypes = [sp, _dp, C.c_int, _ip, _dp, _dp, C.c_double, C.c_uint64, C.c_double,
                                             _dp]
+        L.so_local_sequential.argtypes = [sp, _dp, C.c_int, _ip, _dp, _dp, C.c_double, C.c_uint64, C.c_uint64,
+                                          C.c_double, _dp]
         _lib = L
     return _lib
 
@@ -200,6 +202,18 @@ class Oracle:
         acc = np.zeros(4)
         rc = lib().so_local_random_sites(self._p(), _d(S), len(kind), _i(kind), _d(mag), _d(cum), float(kT),
                                          int(seed), float(nsweeps), _d(acc))
+        if rc:
+            raise ValueError("Energy delta for self-interaction not supported")
+        return S, acc
+
+    def local_sequential(self, S, proposals, kT, key, sweep0, nsweeps):
+        """Sequential LocalSampler trials with Philox draws (sunny_oracle.c
+        so_local_sequential: the device's form when the Ewald term is on)."""
+        S = np.ascontiguousarray(S, float).reshape(self.ns, 3).copy()
+        kind, mag, cum = self._props(proposals)
+        acc = np.zeros(4)
+        rc = lib().so_local_sequential(self._p(), _d(S), len(kind), _i(kind), _d(mag), _d(cum), float(kT),
+                                       int(key) % 2**64, int(sweep0), float(nsweeps), _d(acc))
         if rc:
             raise ValueError("Energy delta for self-interaction not supported")
         return S, acc

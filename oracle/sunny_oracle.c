@@ -1252,6 +1252,36 @@ int so_local_sweeps(const so_spec *sp, double *S, const int m[3], const int *col
     return 0;
 }
 
+/* LocalSampler's step! as sequential trials at random sites (Samplers.jl:
+   133-153), the device's form when the Ewald term couples every pair of
+   sites: ceil(nsweeps * ns) trials, trial T = sweep0 * ns + t drawing from
+   Philox4x32-10 under `key`: counter (T lo, 0x4D430002, T hi, 0) -> site =
+   (w0 * ns) >> 32, u_acc = w1 / 2^32, u_mix = w2 / 2^32; counter
+   (T lo, 0x4D430003, T hi, 0) -> four normals (Box-Muller). */
+int so_local_sequential(const so_spec *sp, double *S, int nprop, const int *kind, const double *mag,
+                        const double *cum, double kT, uint64_t key, uint64_t sweep0, double nsweeps, double acc[4])
+{
+    const long ns = nsites(sp);
+    const long niters = (long)ceil(nsweeps * (double)ns);
+    const uint32_t kk[2] = {(uint32_t)key, (uint32_t)(key >> 32)};
+    const double inv = 1.0 / 4294967296.0;
+    for (long t = 0; t < niters; t++) {
+        const uint64_t T = sweep0 * (uint64_t)ns + (uint64_t)t;
+        uint32_t ctr[4] = {(uint32_t)T, 0x4D430002u, (uint32_t)(T >> 32), 0u}, u[4];
+        so_philox4x32(ctr, kk, u);
+        const long site = (long)(((uint64_t)u[0] * (uint64_t)ns) >> 32);
+        const double u_acc = (double)u[1] * inv, u_mix = (double)u[2] * inv;
+        ctr[1] = 0x4D430003u;
+        uint32_t v[4];
+        so_philox4x32(ctr, kk, v);
+        double n[4];
+        box_muller_pair(v[0], v[1], &n[0], &n[1]);
+        box_muller_pair(v[2], v[3], &n[2], &n[3]);
+        if (local_trial(sp, S, site, nprop, kind, mag, cum, kT, u_mix, u_acc, n, acc)) return -1;
+    }
+    return 0;
+}
+
 /* LocalSampler's step! as the reference orders it (Samplers.jl:133-153):
    ceil(nsweeps * nsites) trials at uniformly random sites.  The draws come
    from splitmix64 seeded by `seed` (the reference draws from sys.rng). */

@@ -119,6 +119,12 @@ double so_local_energy_change_sun(const so_spec *sp, const double *Z, const doub
 int so_local_sweeps_sun(const so_spec *sp, double *Z, double *S, const int m[3], const int *color, int ncolors,
                         int nprop, const int *kind, const double *mag, const double *cum, double kT, uint64_t key,
                         uint64_t sweep0, double nsweeps, double acc[4]);
+/* Sequential trials at random sites with Philox draws (the device's form
+   when the Ewald term is on): trial T = sweep0 * ns + t, counters
+   (T lo, 0x4D430002, T hi, 0) for site and uniforms, (T lo, 0x4D430003,
+   T hi, 0) for the normals. */
+int so_local_sequential(const so_spec *sp, double *S, int nprop, const int *kind, const double *mag,
+                        const double *cum, double kT, uint64_t key, uint64_t sweep0, double nsweeps, double acc[4]);
 /* The same trials at uniformly random sites, as Samplers.jl:133-153 orders
    them (ceil(nsweeps * nsites) trials; splitmix64 draws from seed). */
 int so_local_random_sites(const so_spec *sp, double *S, int nprop, const int *kind, const double *mag,

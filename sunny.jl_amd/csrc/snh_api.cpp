@@ -500,11 +500,18 @@ struct SystemImpl final : snh_system {
         McWork* work = nullptr;       // colour-major, then replica, then class
         int* start = nullptr;         // [ncolors][R + 1] work ranges of each replica
         double* kT = nullptr;         // [R]
-        double* acc = nullptr;        // [work][4]
+        double* acc = nullptr;        // [max(work, R)][4]
         double* out = nullptr;        // [R][4]
+        // sequential trials with the Ewald term
+        double* Aew = nullptr;   // real-space tensor [i_s][i_j][cell][3][3]
+        double* A0 = nullptr;    // [na][9] self blocks A(0, i, i)
+        double* phi = nullptr;   // FP64 potential planes [3][total]
+        McTrial* trial = nullptr;  // [R]
     } mc;
     // Ewald
     EwaldGeom EG{};
+    EwaldAParams ewP{};          // parameters of the real-space tensor
+    std::vector<double> ew_pos;  // fractional positions [na][3]
     void* M = nullptr;
     void* Fmu = nullptr;
     void* Fphi = nullptr;
@@ -940,6 +947,8 @@ struct SystemImpl final : snh_system {
         P.L3 = L[2];
         P.na = na;
         std::vector<double> pos(sp.positions, sp.positions + 3 * na);
+        ewP = P;
+        ew_pos = pos;
         double* d_pos = dupload(pos);
         double* d_A = dalloc<double>((size_t)na * na * EG.ncell * 9);
         const int n3 = 3 * na;
@@ -1326,9 +1335,8 @@ struct SystemImpl final : snh_system {
     void mc_prepare()
     {
         if (mc.ready) return;
-        SNH_REQUIRE(!ewald, SNH_ERR_UNSUPPORTED,
-                    "the device LocalSampler does not support dipole-dipole (Ewald) interactions: they couple every "
-                    "pair of sites, so no two trials are independent");
+        SNH_REQUIRE(!ewald || N == 0, SNH_ERR_UNSUPPORTED,
+                    "the device LocalSampler supports dipole-dipole (Ewald) interactions in :dipole mode only");
         std::vector<std::array<int, 3>> dims;
         for (const auto& ri : reps) dims.push_back({ri.L1, ri.L2, ri.L3});
         // a bond that maps a site onto itself (local_energy_change raises)
@@ -1339,7 +1347,8 @@ struct SystemImpl final : snh_system {
                             SNH_ERR_INVALID, "Energy delta for self-interaction not supported");
             }
         mc.col = color_classes(na, nb_in, bond_h.data(), dims);
-        SNH_REQUIRE(mc.col.ncolors > 0, SNH_ERR_UNSUPPORTED,
+        // with the Ewald term trials run one at a time: no colouring needed
+        SNH_REQUIRE(ewald || mc.col.ncolors > 0, SNH_ERR_UNSUPPORTED,
                     "no colouring of the sites has a period that divides every replica's dims");
         // gather slots: cell offset and target; padding is the site itself (zero coupling)
         std::vector<int> bn(4 * std::max<size_t>(bsrc_h.size(), 1), 0);
@@ -1381,8 +1390,29 @@ struct SystemImpl final : snh_system {
         mc.work = own(dupload(work));
         mc.start = own(dupload(start));
         mc.kT = own(dalloc<double>(R));
-        mc.acc = own(dalloc<double>(4 * std::max<size_t>(work.size(), 1)));
+        mc.acc = own(dalloc<double>(4 * std::max<size_t>(work.size(), (size_t)R)));
         mc.out = own(dalloc<double>(4 * (size_t)R));
+        if (ewald) {
+            // the real-space tensor (freed after the Fourier transform at
+            // creation) is recomputed and kept: every accepted move updates
+            // the potential with one of its slices
+            for (int r = 1; r < R; r++)
+                SNH_REQUIRE(reps[r].ncell == reps[0].ncell, SNH_ERR_INVALID, "Ewald replicas share their dims");
+            const size_t nA = (size_t)na * na * (size_t)EG.ncell * 9;
+            double* d_pos = dupload(ew_pos);
+            mc.Aew = own(dalloc<double>(nA));
+            launch_ewald_A(ewP, d_pos, mc.Aew, st);
+            SNH_HIP(hipGetLastError());
+            SNH_HIP(hipStreamSynchronize(st));
+            SNH_HIP(hipFree(d_pos));
+            std::vector<double> a0(9 * (size_t)na);
+            for (int i = 0; i < na; i++)
+                SNH_HIP(hipMemcpy(a0.data() + 9 * (size_t)i, mc.Aew + ((size_t)(i * na + i) * EG.ncell) * 9,
+                                  9 * sizeof(double), hipMemcpyDeviceToHost));
+            mc.A0 = own(dupload(a0));
+            mc.phi = own(dalloc<double>(3 * (size_t)total));
+            mc.trial = own(dalloc<McTrial>(R));
+        }
         mc.ready = true;
     }
 
@@ -1427,6 +1457,27 @@ struct SystemImpl final : snh_system {
         A.m2 = mc.col.m[1];
         A.m3 = mc.col.m[2];
         SNH_HIP(hipMemcpyAsync(mc.kT, kT, sizeof(double) * R, hipMemcpyHostToDevice, st));
+        if (ewald) {
+            // ceil(nsweeps * ns) sequential trials per replica, numbered from
+            // sweep0 * ns; phi starts as the Ewald potential of the state
+            ewald_field(X, phi1);
+            launch_planes_to_double<T>(phi1, mc.phi, 3 * (long long)total, st);
+            launch_fill_double(mc.acc, 4 * (long long)R, 0.0, st);
+            const uint32_t ns = (uint32_t)reps[0].ncell * (uint32_t)na;
+            const long long niters = (long long)std::ceil(nsweeps * (double)ns);
+            hipEvent_t e = tbegin();
+            for (long long t = 0; t < niters; t++) {
+                launch_mc_ewald_trial<T>(C, A, X, mc.phi, mc.A0, sweep0 * ns + (uint64_t)t, mc.trial, mc.acc, R, st);
+                launch_mc_ewald_update(mc.Aew, C.reps, C.G, na, total, ns, mc.trial, mc.phi, R, st);
+            }
+            tend("local_ewald", e);
+            SNH_HIP(hipGetLastError());
+            std::vector<double> h(4 * (size_t)R);
+            SNH_HIP(hipMemcpyAsync(h.data(), mc.acc, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+            SNH_HIP(hipStreamSynchronize(st));
+            if (dE_dS) std::copy(h.begin(), h.end(), dE_dS);
+            return;
+        }
         launch_fill_double(mc.acc, 4 * (long long)std::max(mc.coff.back(), 1), 0.0, st);
         const long long npass = (long long)std::ceil(2 * nsweeps);
         for (long long k = 0; k < npass; k++) {

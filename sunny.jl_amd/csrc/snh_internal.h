@@ -204,6 +204,22 @@ template <typename T>
 void launch_sun_mc_color(const Couplings<T>& C, const McArgs& A, T* Z, const McWork* work, int nwork,
                          unsigned long long P, double prob, double* acc, hipStream_t st);
 void launch_mc_reduce(const double* acc, const int* start, int ncolors, int R, double* out, hipStream_t st);
+// Accepted sequential trial of a replica: site (sublattice i, cell c1..c3)
+// and the change of its moment dmu = -g dS (flag 0: rejected).
+struct McTrial {
+    int flag, i, c1, c2, c3, pad;
+    double dmu[3];
+};
+// Trial T0 of the Ewald form for every replica (phi: FP64 potential planes,
+// A0: [na][9] self blocks A(0, i, i)).
+template <typename T>
+void launch_mc_ewald_trial(const Couplings<T>& C, const McArgs& A, T* S, const double* phi, const double* A0,
+                           unsigned long long T0, McTrial* tr, double* acc, int R, hipStream_t st);
+// phi += A dmu at the ns sites of every replica whose trial was accepted.
+void launch_mc_ewald_update(const double* Aew, const ReplicaInfo* reps, const Ghost& G, int na, uint32_t total,
+                            uint32_t ns, const McTrial* tr, double* phi, int R, hipStream_t st);
+template <typename T>
+void launch_planes_to_double(const T* src, double* dst, long long n, hipStream_t st);
 void launch_fill_double(double* p, long long n, double v, hipStream_t st);
 
 // Device step counter of the noise stream: set, and advance by one.

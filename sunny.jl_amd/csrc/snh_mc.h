@@ -9,6 +9,7 @@ namespace snh {
 
 constexpr uint32_t kMcTagNormals = 0x4D430000u;
 constexpr uint32_t kMcTagUniforms = 0x4D430001u;
+constexpr uint32_t kMcTagSequential = 0x4D430002u;  // and + 1: sequential trials (Ewald)
 
 __device__ __forceinline__ int mc_wrap(int v, int L)
 {

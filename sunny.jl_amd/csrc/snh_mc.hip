@@ -26,10 +26,101 @@
 //   counter (s, 0x4D430000, P) -> four normals (Box-Muller; three are used)
 //   counter (s, 0x4D430001, P) -> u_acc, u_mix, u_try (word / 2^32)
 // disjoint from the Langevin stream (second counter word 0).
+//
+// With the Ewald term every pair of sites interacts, so no two trials are
+// independent: trials then run one at a time per replica (k_mc_ewald_trial),
+// each followed by an update of the FP64 Ewald potential phi at every site
+// of the replica (k_mc_ewald_update, O(N) per accepted move as in
+// Ewald.jl:212-241).
 #include "snh_mc.h"
 #include "snh_vec.h"
 
 namespace snh {
+
+// Proposal of LocalSampler for a site holding s0: propose_flip, randspin or
+// perturbed_spin (S + mag kappa randn), chosen by u_mix; normals from
+// Philox4x32-10 of counter ctr (four normals, three used).
+template <typename T>
+__device__ __forceinline__ V3<T> mc_propose(const McArgs& A, V3<T> s0, T kap, double u_mix, uint32_t ctr[4],
+                                            unsigned long long key)
+{
+    const int k = mc_pick(A, u_mix);
+    if (A.kind[k] == 1) return {-s0.x, -s0.y, -s0.z};  // propose_flip
+    philox4x32(ctr, (uint32_t)key, (uint32_t)(key >> 32));
+    T n[4];
+    box_muller(ctr[0], ctr[1], n[0], n[1]);
+    box_muller(ctr[2], ctr[3], n[2], n[3]);
+    const double mag = A.mag[k];
+    if (A.kind[k] == 0 || __builtin_isinf(mag))  // randspin
+        return normalize_dipole(V3<T>{n[0], n[1], n[2]}, kap);
+    const T f = (T)(mag * (double)kap);  // perturbed_spin
+    return normalize_dipole(V3<T>{s0.x + f * n[0], s0.y + f * n[1], s0.z + f * n[2]}, kap);
+}
+
+// local_energy_change in FP64 (Interactions.jl:311-380) of site sc going
+// from s0 to sn: Zeeman, Stevens, bilinear and biquadratic bonds; neighbour
+// states gathered G bonds at a time.
+template <typename T, int G>
+__device__ __forceinline__ double mc_local_delta(const Couplings<T>& C, const McArgs& A, const T* __restrict__ S,
+                                                 const SiteCoord& sc, V3<T> s0, V3<T> sn, bool wide)
+{
+    const int i = sc.i;
+    const uint32_t total = C.total;
+    const double ox = s0.x, oy = s0.y, oz = s0.z, nx = sn.x, ny = sn.y, nz = sn.z;
+    const double dx = nx - ox, dy = ny - oy, dz = nz - oz;
+    double dE = (double)ldc(C.zee, 3 * i) * dx + (double)ldc(C.zee, 3 * i + 1) * dy +
+                (double)ldc(C.zee, 3 * i + 2) * dz;
+    dE += stevens_energy_d(C, i, nx, ny, nz) - stevens_energy_d(C, i, ox, oy, oz);
+    double Qn[5], Qo[5];
+    quadrupole_d(nx, ny, nz, Qn);
+    quadrupole_d(ox, oy, oz, Qo);
+    const int b0 = ldc(C.bstart, i), b1 = ldc(C.bstart, i + 1);
+    for (int c0 = b0; c0 < b1; c0 += G) {
+        T jx[G], jy[G], jz[G];
+#pragma unroll
+        for (int kk = 0; kk < G; kk++) {
+            const uint32_t gj = mc_neighbour(A, C.G, sc, c0 + kk, wide);
+            jx[kk] = ldg(S, gj);
+            jy[kk] = ldg(S, gj + total);
+            jz[kk] = ldg(S, gj + 2 * total);
+        }
+#pragma unroll
+        for (int kk = 0; kk < G; kk++) {
+            const int b = c0 + kk;
+            const double x = jx[kk], y = jy[kk], z = jz[kk];
+            const T* J = C.bJ + 9 * b;
+            dE += dx * ((double)ldc(J, 0) * x + (double)ldc(J, 1) * y + (double)ldc(J, 2) * z) +
+                  dy * ((double)ldc(J, 3) * x + (double)ldc(J, 4) * y + (double)ldc(J, 5) * z) +
+                  dz * ((double)ldc(J, 6) * x + (double)ldc(J, 7) * y + (double)ldc(J, 8) * z);
+            if (ldc(C.bflag, b) & kBondBiquad) {
+                double Qj[5];
+                quadrupole_d(x, y, z, Qj);
+                const T* Bq = C.bQ + 25 * b;
+                for (int p = 0; p < 5; p++) {
+                    double w = 0;
+                    for (int qv = 0; qv < 5; qv++) w += (double)ldc(Bq, 5 * p + qv) * Qj[qv];
+                    dE += (Qn[p] - Qo[p]) * w;
+                }
+            }
+        }
+    }
+    return dE;
+}
+
+// Writes sn at site sc and its ghost images.
+template <typename T>
+__device__ __forceinline__ void mc_store(T* __restrict__ S, const Couplings<T>& C, const SiteCoord& sc, V3<T> sn)
+{
+    const uint32_t total = C.total;
+    stg(S, sc.g, sn.x);
+    stg(S, sc.g + total, sn.y);
+    stg(S, sc.g + 2 * total, sn.z);
+    for_each_ghost(sc, C.G, [&](uint32_t gi) {
+        stg(S, gi, sn.x);
+        stg(S, gi + total, sn.y);
+        stg(S, gi + 2 * total, sn.z);
+    });
+}
 
 // One colour of pass P: workgroup b takes work[b] = kBlock consecutive sites
 // q of one class (replica r, sublattice i, cell residue rho mod m), cell
@@ -56,85 +147,117 @@ __global__ __launch_bounds__(kBlock) void k_mc_color(Couplings<T> C, McArgs A, T
     const double inv32 = 1.0 / 4294967296.0;
     const double u_acc = (double)un[0] * inv32, u_mix = (double)un[1] * inv32, u_try = (double)un[2] * inv32;
     if (sc.valid && u_try < prob) {
-        const int k = mc_pick(A, u_mix);
         const V3<T> s0 = {ldg(S, sc.g), ldg(S, sc.g + total), ldg(S, sc.g + 2 * total)};
-        const T kap = ldc(C.kappa, i);
-        V3<T> sn;
-        if (A.kind[k] == 1) {  // propose_flip
-            sn = {-s0.x, -s0.y, -s0.z};
-        } else {
-            uint32_t cn[4] = {site, kMcTagNormals, (uint32_t)P, (uint32_t)(P >> 32)};
-            philox4x32(cn, (uint32_t)key, (uint32_t)(key >> 32));
-            T n[4];
-            box_muller(cn[0], cn[1], n[0], n[1]);
-            box_muller(cn[2], cn[3], n[2], n[3]);
-            const double mag = A.mag[k];
-            if (A.kind[k] == 0 || __builtin_isinf(mag))  // randspin
-                sn = normalize_dipole(V3<T>{n[0], n[1], n[2]}, kap);
-            else {  // perturbed_spin: S + mag kappa randn
-                const T f = (T)(mag * (double)kap);
-                sn = normalize_dipole(V3<T>{s0.x + f * n[0], s0.y + f * n[1], s0.z + f * n[2]}, kap);
-            }
-        }
-        // local_energy_change in FP64
-        const double ox = s0.x, oy = s0.y, oz = s0.z, nx = sn.x, ny = sn.y, nz = sn.z;
-        const double dx = nx - ox, dy = ny - oy, dz = nz - oz;
-        double dE = (double)ldc(C.zee, 3 * i) * dx + (double)ldc(C.zee, 3 * i + 1) * dy +
-                    (double)ldc(C.zee, 3 * i + 2) * dz;
-        dE += stevens_energy_d(C, i, nx, ny, nz) - stevens_energy_d(C, i, ox, oy, oz);
-        double Qn[5], Qo[5];
-        quadrupole_d(nx, ny, nz, Qn);
-        quadrupole_d(ox, oy, oz, Qo);
-        const int b0 = ldc(C.bstart, i), b1 = ldc(C.bstart, i + 1);
-        for (int c0 = b0; c0 < b1; c0 += G) {
-            T jx[G], jy[G], jz[G];
-#pragma unroll
-            for (int kk = 0; kk < G; kk++) {
-                const uint32_t gj = mc_neighbour(A, C.G, sc, c0 + kk, wide);
-                jx[kk] = ldg(S, gj);
-                jy[kk] = ldg(S, gj + total);
-                jz[kk] = ldg(S, gj + 2 * total);
-            }
-#pragma unroll
-            for (int kk = 0; kk < G; kk++) {
-                const int b = c0 + kk;
-                const double x = jx[kk], y = jy[kk], z = jz[kk];
-                const T* J = C.bJ + 9 * b;
-                dE += dx * ((double)ldc(J, 0) * x + (double)ldc(J, 1) * y + (double)ldc(J, 2) * z) +
-                      dy * ((double)ldc(J, 3) * x + (double)ldc(J, 4) * y + (double)ldc(J, 5) * z) +
-                      dz * ((double)ldc(J, 6) * x + (double)ldc(J, 7) * y + (double)ldc(J, 8) * z);
-                if (ldc(C.bflag, b) & kBondBiquad) {
-                    double Qj[5];
-                    quadrupole_d(x, y, z, Qj);
-                    const T* Bq = C.bQ + 25 * b;
-                    for (int p = 0; p < 5; p++) {
-                        double w = 0;
-                        for (int qv = 0; qv < 5; qv++) w += (double)ldc(Bq, 5 * p + qv) * Qj[qv];
-                        dE += (Qn[p] - Qo[p]) * w;
-                    }
-                }
-            }
-        }
+        uint32_t cn[4] = {site, kMcTagNormals, (uint32_t)P, (uint32_t)(P >> 32)};
+        const V3<T> sn = mc_propose<T>(A, s0, ldc(C.kappa, i), u_mix, cn, key);
+        const double dE = mc_local_delta<T, G>(C, A, S, sc, s0, sn, wide);
         const double kT = ldc(A.kT, W.r);
         const bool accept = kT == 0.0 ? dE <= 0.0 : u_acc <= exp(-dE / kT);
         if (accept) {
-            stg(S, sc.g, sn.x);
-            stg(S, sc.g + total, sn.y);
-            stg(S, sc.g + 2 * total, sn.z);
-            for_each_ghost(sc, C.G, [&](uint32_t gi) {
-                stg(S, gi, sn.x);
-                stg(S, gi + total, sn.y);
-                stg(S, gi + 2 * total, sn.z);
-            });
+            mc_store(S, C, sc, sn);
             v[0] = dE;
-            v[1] = dx;
-            v[2] = dy;
-            v[3] = dz;
+            v[1] = (double)sn.x - (double)s0.x;
+            v[2] = (double)sn.y - (double)s0.y;
+            v[3] = (double)sn.z - (double)s0.z;
         }
     }
     block_sum4(v);
     if (threadIdx.x == 0)
         for (int qv = 0; qv < 4; qv++) acc[4 * (size_t)blockIdx.x + qv] += v[qv];
+}
+
+// Sequential trial of the Ewald form, workgroup r = replica r (its first
+// lane only: philox4x32 takes its key as uniform over the wave): trial T0
+// draws the site and u_acc, u_mix from Philox4x32-10 counter
+// (T lo, kMcTagSequential, T hi, 0) and the normals from counter
+// (T lo, kMcTagSequential + 1, T hi, 0).  dE adds ewald_energy_delta
+// (Ewald.jl:234-241): dS . (-g^T phi) + dmu . A(0, i, i) dmu / 2 with
+// dmu = -g dS, phi the site's current potential.  tr[r] records an accepted
+// move (site, dmu) for k_mc_ewald_update; acc[4 r + q] += (dE, dS).
+template <typename T, int G>
+__global__ __launch_bounds__(64) void k_mc_ewald_trial(Couplings<T> C, McArgs A, T* __restrict__ S,
+                                                       const double* __restrict__ phi,
+                                                       const double* __restrict__ A0, unsigned long long T0,
+                                                       McTrial* __restrict__ tr, double* __restrict__ acc, int nrep)
+{
+    const int r = blockIdx.x;
+    if (threadIdx.x != 0 || r >= nrep) return;
+    const ReplicaInfo R = C.reps[r];
+    const uint32_t total = C.total;
+    const uint32_t ns = (uint32_t)R.ncell * (uint32_t)C.na;
+    const unsigned long long key = A.seed + (unsigned long long)r * 0x9E3779B97F4A7C15ull;
+    uint32_t un[4] = {(uint32_t)T0, kMcTagSequential, (uint32_t)(T0 >> 32), 0u};
+    philox4x32(un, (uint32_t)key, (uint32_t)(key >> 32));
+    const uint32_t site = (uint32_t)(((unsigned long long)un[0] * ns) >> 32);
+    const double inv32 = 1.0 / 4294967296.0;
+    const double u_acc = (double)un[1] * inv32, u_mix = (double)un[2] * inv32;
+    const int i = (int)(site / (uint32_t)R.ncell);
+    const int c = (int)(site - (uint32_t)i * (uint32_t)R.ncell);
+    const int c1 = c % R.L1, c2 = (c / R.L1) % R.L2, c3 = c / (R.L1 * R.L2);
+    SiteCoord sc = mc_site(R, C.G, i, c1, c2, c3, true);
+    sc.r = r;
+    const V3<T> s0 = {ldg(S, sc.g), ldg(S, sc.g + total), ldg(S, sc.g + 2 * total)};
+    uint32_t cn[4] = {(uint32_t)T0, kMcTagSequential + 1u, (uint32_t)(T0 >> 32), 0u};
+    const V3<T> sn = mc_propose<T>(A, s0, ldc(C.kappa, i), u_mix, cn, key);
+    double dE = mc_local_delta<T, G>(C, A, S, sc, s0, sn, A.wide != 0);
+    const double dS[3] = {(double)sn.x - (double)s0.x, (double)sn.y - (double)s0.y, (double)sn.z - (double)s0.z};
+    const double p[3] = {phi[sc.g], phi[sc.g + total], phi[sc.g + 2 * total]};
+    double g[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) g[q] = (double)ldc(C.gmat + 9 * i, q);
+    double dmu[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) dmu[a] = -(g[3 * a] * dS[0] + g[3 * a + 1] * dS[1] + g[3 * a + 2] * dS[2]);
+    const double* a0 = A0 + 9 * i;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double grad = -(g[a] * p[0] + g[3 + a] * p[1] + g[6 + a] * p[2]);
+        const double adm = a0[3 * a] * dmu[0] + a0[3 * a + 1] * dmu[1] + a0[3 * a + 2] * dmu[2];
+        dE += dS[a] * grad + dmu[a] * adm / 2;
+    }
+    const double kT = A.kT[r];
+    const bool accept = kT == 0.0 ? dE <= 0.0 : u_acc <= exp(-dE / kT);
+    McTrial out{};
+    if (accept) {
+        mc_store(S, C, sc, sn);
+        out = {1, i, c1, c2, c3, 0, {dmu[0], dmu[1], dmu[2]}};
+        acc[4 * r] += dE;
+        for (int a = 0; a < 3; a++) acc[4 * r + 1 + a] += dS[a];
+    }
+    tr[r] = out;
+}
+
+// After trial tr[r] was accepted: phi at every site j of replica r (grid.y)
+// gains A(c_s - c_j; i_s, i_j) dmu (phi = sum_k A mu_k, Ewald.jl:185-209).
+// Aew: real-space tensor [i_s][i_j][cell offset][3][3], FP64.
+__global__ __launch_bounds__(256) void k_mc_ewald_update(const double* __restrict__ Aew, const ReplicaInfo* reps,
+                                                         Ghost G, int na, uint32_t total,
+                                                         const McTrial* __restrict__ tr, double* __restrict__ phi)
+{
+    const int r = blockIdx.y;
+    const McTrial m = tr[r];
+    if (!m.flag) return;  // uniform over the workgroup
+    const ReplicaInfo R = reps[r];
+    const uint32_t ncell = (uint32_t)R.ncell;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ncell * (uint32_t)na) return;
+    const int ij = (int)(j / ncell);
+    const int c = (int)(j - (uint32_t)ij * ncell);
+    const int c1 = c % R.L1, c2 = (c / R.L1) % R.L2, c3 = c / (R.L1 * R.L2);
+    const int o1 = mc_wrap(m.c1 - c1, R.L1), o2 = mc_wrap(m.c2 - c2, R.L2), o3 = mc_wrap(m.c3 - c3, R.L3);
+    const size_t off = (size_t)o1 + (size_t)R.L1 * ((size_t)o2 + (size_t)R.L2 * (size_t)o3);
+    const double* a = Aew + (((size_t)m.i * na + ij) * ncell + off) * 9;
+    const SiteCoord sc = mc_site(R, G, ij, c1, c2, c3, true);
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+        phi[sc.g + q * total] += a[3 * q] * m.dmu[0] + a[3 * q + 1] * m.dmu[1] + a[3 * q + 2] * m.dmu[2];
+}
+
+template <typename T>
+__global__ void k_planes_to_double(const T* __restrict__ src, double* __restrict__ dst, long long n)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = (double)src[k];
 }
 
 // out[4 r + q] = sum of acc[4 k + q] over the work items k of replica r:
@@ -171,6 +294,30 @@ void launch_mc_color(const Couplings<T>& C, const McArgs& A, T* S, const McWork*
         hipLaunchKernelGGL((k_mc_color<T, 9>), dim3(nwork), dim3(kBlock), 0, st, C, A, S, work, P, prob, acc);
 }
 
+template <typename T>
+void launch_mc_ewald_trial(const Couplings<T>& C, const McArgs& A, T* S, const double* phi, const double* A0,
+                           unsigned long long T0, McTrial* tr, double* acc, int R, hipStream_t st)
+{
+    const dim3 grid(R);
+    if (C.chunk == 6)
+        hipLaunchKernelGGL((k_mc_ewald_trial<T, 6>), grid, dim3(64), 0, st, C, A, S, phi, A0, T0, tr, acc, R);
+    else
+        hipLaunchKernelGGL((k_mc_ewald_trial<T, 9>), grid, dim3(64), 0, st, C, A, S, phi, A0, T0, tr, acc, R);
+}
+
+void launch_mc_ewald_update(const double* Aew, const ReplicaInfo* reps, const Ghost& G, int na, uint32_t total,
+                            uint32_t ns, const McTrial* tr, double* phi, int R, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mc_ewald_update, dim3((ns + 255) / 256, R), dim3(256), 0, st, Aew, reps, G, na, total, tr,
+                       phi);
+}
+
+template <typename T>
+void launch_planes_to_double(const T* src, double* dst, long long n, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_planes_to_double<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
+}
+
 void launch_mc_reduce(const double* acc, const int* start, int ncolors, int R, double* out, hipStream_t st)
 {
     hipLaunchKernelGGL(k_mc_reduce, dim3(R), dim3(kBlock), 0, st, acc, start, ncolors, R, out);
@@ -181,6 +328,12 @@ void launch_fill_double(double* p, long long n, double v, hipStream_t st)
     hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
 }
 
+template void launch_mc_ewald_trial<float>(const Couplings<float>&, const McArgs&, float*, const double*,
+                                           const double*, unsigned long long, McTrial*, double*, int, hipStream_t);
+template void launch_mc_ewald_trial<double>(const Couplings<double>&, const McArgs&, double*, const double*,
+                                            const double*, unsigned long long, McTrial*, double*, int, hipStream_t);
+template void launch_planes_to_double<float>(const float*, double*, long long, hipStream_t);
+template void launch_planes_to_double<double>(const double*, double*, long long, hipStream_t);
 template void launch_mc_color<float>(const Couplings<float>&, const McArgs&, float*, const McWork*, int,
                                      unsigned long long, double, double*, hipStream_t);
 template void launch_mc_color<double>(const Couplings<double>&, const McArgs&, double*, const McWork*, int,

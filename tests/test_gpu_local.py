@@ -142,11 +142,52 @@ def test_langevin_after_sweeps_sees_updated_images():
     assert np.max(np.abs(s.dipoles_flat() - S_ref)) <= 1e-11
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_ewald_sequential_trials_match_oracle(precision):
+    """With the Ewald term the device tries one site at a time per replica and
+    updates the FP64 potential everywhere after each accepted move; the
+    oracle's so_local_sequential makes the same trials with the same draws."""
+    cases = [("pyro1", sh.workloads.pyrochlore_model(L=1, seed=23, precision=precision, dipolar=True)),
+             ("pyro211", sh.workloads.pyrochlore_model(dims=(2, 1, 1), seed=24, precision=precision, dipolar=True))]
+    for name, s in cases:
+        kap = np.repeat(s.kappas, s.ncells)
+        for pname in (["uniform", "mix"] if precision == 64 else ["flip"]):
+            prop = PROPOSALS[pname]
+            announce(f"local sequential ewald {name} {pname} fp{precision}")
+            S0 = s.dipoles_flat()
+            E0 = sh.energy(s)
+            key = O.replica_key(s.noise_seed, 0)
+            S_ref, acc = O.Oracle(s.spec()).local_sequential(S0, oracle_props(prop), 0.6, key, s.step_counter, 2.0)
+            sampler = sh.LocalSampler(kT=0.6, nsweeps=2.0, propose=prop)
+            sh.step_(s, sampler)
+            S_dev = s.dipoles_flat()
+            assert np.max(np.abs(S_dev - S_ref)) <= TOL[precision] * kap.max(), (name, pname)
+            assert np.sum(np.any(np.abs(S_dev - S0) > 1e-9, axis=1)) > 0, (name, pname)
+            scale = energy_scale(s)
+            assert abs(sampler.dE - acc[0]) <= 10 * TOL[precision] * scale, (name, pname, sampler.dE, acc[0])
+            assert abs(sh.energy(s) - E0 - sampler.dE) <= 10 * TOL[precision] * scale, (name, pname)
+
+
+def test_ewald_sequential_batch_matches_oracle():
+    systems = [sh.workloads.pyrochlore_model(dims=(2, 2, 1), seed=40 + r, precision=64, dipolar=True)
+               for r in range(2)]
+    announce("local sequential ewald replica batch")
+    batch = sh.ReplicaBatch(systems, precision=64)
+    kTs = np.array([0.3, 1.2])
+    refs = [s.dipoles_flat() for s in systems]
+    prop = sh.mix_proposals(0.5, sh.propose_uniform, 0.5, sh.propose_delta(0.25))
+    sampler = sh.LocalSampler(kT=0.0, nsweeps=1.5, propose=prop)
+    sh.step_(batch, sampler, kTs=kTs)
+    batch.sync_to_systems()
+    for r, s in enumerate(systems):
+        S, acc = O.Oracle(s.spec()).local_sequential(refs[r], oracle_props(prop), kTs[r],
+                                                     O.replica_key(batch.noise_seed, r), 0, 1.5)
+        assert np.max(np.abs(s.dipoles_flat() - S)) <= 1e-12 * 1.5, r
+        assert abs(sampler.dE[r] - acc[0]) <= 1e-9 * energy_scale(s), r
+
+
 def test_unsupported_and_invalid():
     announce("local sampler errors")
-    ew = sh.workloads.pyrochlore_model(L=1, seed=2, precision=64, dipolar=True)
-    with pytest.raises(ValueError, match="Ewald"):
-        sh.step_(ew, sh.LocalSampler(kT=1.0))
     selfint = sh.workloads.cubic_model(dims=(1, 2, 2), seed=3, precision=64)
     with pytest.raises(ValueError, match="self-interaction"):
         sh.step_(selfint, sh.LocalSampler(kT=1.0))

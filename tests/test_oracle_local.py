@@ -80,6 +80,25 @@ def test_accumulated_changes_match_states():
     assert 0.15 < moved < 0.35
 
 
+def test_sequential_trials_with_ewald():
+    """Sequential trials (the device's form when the Ewald term is on): the
+    accumulated changes are the changes of the Ewald-inclusive energy and of
+    the summed dipoles; kT = 0 never raises the energy; the draws pick sites
+    uniformly (every site of 16 is tried over 40 sweeps)."""
+    s = sh.workloads.pyrochlore_model(L=1, seed=21, dipolar=True)
+    o = O.Oracle(s.spec())
+    S = s.dipoles_flat()
+    props = [(FLIP, 0, 0.3), (DELTA, 0.4, 0.5), (UNIFORM, 0, 0.2)]
+    S2, acc = o.local_sequential(S, props, 0.5, 77, 3, 4.0)
+    assert np.sum(np.any(S2 != S, axis=1)) > s.nsites // 4
+    np.testing.assert_allclose(acc[0], o.energy(S2) - o.energy(S), atol=1e-10)
+    np.testing.assert_allclose(acc[1:], S2.sum(0) - S.sum(0), atol=1e-12)
+    S3, acc3 = o.local_sequential(S, props, 0.0, 5, 0, 3.0)
+    assert acc3[0] <= 0 and o.energy(S3) <= o.energy(S) + 1e-12
+    S4, _ = o.local_sequential(S, [(UNIFORM, 0, 1)], 1e9, 8, 0, 40.0)
+    assert np.all(np.any(S4 != S, axis=1))
+
+
 def chain_mean(o, S, m, col, nc, props, kT, key, nsweeps, observe, burn=200):
     vals = []
     for q in range(nsweeps):
