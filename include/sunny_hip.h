@@ -215,7 +215,8 @@ int snh_corr_add_sample(snh_system *sys, double dt, double tol, int32_t max_iter
 int snh_corr_data(snh_system *sys, double *data, int64_t *nsamples);
 
 /* step!(sys, LocalSampler(; kT, nsweeps, propose)) for every replica
- * (src/MonteCarlo/Samplers.jl:133-153), :dipole or :SUN, without Ewald.
+ * (src/MonteCarlo/Samplers.jl:133-153), :dipole or :SUN; with Ewald (:dipole
+ * only) the trials run one at a time, as described at the end.
  * Proposal p of the mixture (@mix_proposals, Samplers.jl:67-100) has
  * kinds[p] = 0 propose_uniform (randspin, src/System/System.jl:444-447),
  * 1 propose_flip (flip, System.jl:440-442; flip_ket for :SUN, Spin.jl:119-125)
@@ -226,11 +227,18 @@ int snh_corr_data(snh_system *sys, double *data, int64_t *nsamples);
  * (src/System/Interactions.jl:311-380) in FP64, and it is accepted when
  * kT = 0 and dE <= 0, or rand <= exp(-dE/kT).
  * Trial order: the reference draws ceil(nsweeps * nsites) random sites; here
- * floor(nsweeps) sweeps try every site once, colour class by colour class
- * (snh_local_coloring), and a fractional remainder f adds a sweep that tries
- * each site with probability f.  Draws are Philox4x32-10 keyed by
- * (seed, replica) and counted by (site, sweep0 + k) on a stream disjoint from
- * snh_step_langevin's.  dE_dS (may be NULL): [R][4], the sums of dE and of
+ * ceil(2 nsweeps) passes go colour class by colour class (snh_local_coloring)
+ * and try each site with probability 1/2 (the last pass: the remainder), so a
+ * call makes nsweeps * nsites trials on average; pass k is numbered
+ * P = 2 sweep0 + k.  Draws are Philox4x32-10 keyed by (seed, replica) and
+ * counted by (site, P) on a stream disjoint from snh_step_langevin's.
+ * With the Ewald term every pair of sites interacts: each replica then makes
+ * ceil(nsweeps * nsites) sequential trials at random sites, trial
+ * T = sweep0 * nsites + t drawing its site, u_acc and u_mix from counter
+ * (T lo, 0x4D430002, T hi, 0) and its normals from (T lo, 0x4D430003, T hi,
+ * 0); dE adds ewald_energy_delta (src/System/Ewald.jl:234-241) from an FP64
+ * potential that every accepted move updates at all sites (O(nsites) per
+ * move, as in the reference).  dE_dS (may be NULL): [R][4], the sums of dE and of
  * dS = S_new - S_old over the accepted trials of this call (LocalSampler's
  * fields dE, dS accumulate them). */
 int snh_step_local(snh_system *sys, const double *kT, int32_t nprop, const int32_t *kinds,
