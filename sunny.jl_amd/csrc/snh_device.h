@@ -169,8 +169,9 @@ __device__ __forceinline__ void for_each_ghost(const SiteCoord& sc, const Ghost&
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) + Box-Muller.
 //
-// Every caller's key is uniform over its workgroup (one replica), so the key
-// lives in scalar registers.  The round keys are advanced in place behind an
+// Every caller's key must be uniform over its wave (callers keep one replica
+// per workgroup, or run one lane): the key is taken from the first active
+// lane (readfirstlane) and lives in scalar registers.  The round keys are advanced in place behind an
 // empty asm barrier: otherwise the compiler precomputes all twenty of them
 // outside the caller's loops, runs out of scalar registers and reloads them
 // from vector-register lanes in every round.
