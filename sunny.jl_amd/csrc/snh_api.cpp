@@ -1448,6 +1448,9 @@ struct SystemImpl final : snh_system {
             A.cum[p] = c;
         }
         mc_prepare();
+        // the sweeps rewrite the state: the expected-dipole planes carried
+        // between SU(N) Langevin calls no longer describe it
+        dz_valid = false;
         A.nprop = nprop;
         A.bn = mc.bn;
         A.kT = mc.kT;

@@ -148,7 +148,10 @@ def test_ewald_sequential_trials_match_oracle(precision):
     updates the FP64 potential everywhere after each accepted move; the
     oracle's so_local_sequential makes the same trials with the same draws."""
     cases = [("pyro1", sh.workloads.pyrochlore_model(L=1, seed=23, precision=precision, dipolar=True)),
-             ("pyro211", sh.workloads.pyrochlore_model(dims=(2, 1, 1), seed=24, precision=precision, dipolar=True))]
+             ("pyro211", sh.workloads.pyrochlore_model(dims=(2, 1, 1), seed=24, precision=precision, dipolar=True)),
+             # an axis of length 3: offsets differ from their negatives, so the
+             # atom order and offset sign of the potential update are pinned
+             ("pyro321", sh.workloads.pyrochlore_model(dims=(3, 2, 1), seed=25, precision=precision, dipolar=True))]
     for name, s in cases:
         kap = np.repeat(s.kappas, s.ncells)
         for pname in (["uniform", "mix"] if precision == 64 else ["flip"]):
@@ -296,3 +299,20 @@ def test_sun_mean_energy_closed_form(N, kT):
     err = blocks.std(ddof=1) / np.sqrt(20)
     exact = su3_mean_energy(kT, D) if N == 3 else su5_mean_energy(kT, D)
     assert abs(vals.mean() - exact) < max(4 * err, 0.01 * abs(exact)), (vals.mean(), exact, err)
+
+
+def test_sun_langevin_after_sweeps_sees_updated_kets():
+    """A bilinear-only SU(3) model carries the expected-dipole planes from one
+    Langevin call to the next; Metropolis sweeps in between rewrite the kets,
+    so the next Langevin step must start from the swept state's dipoles."""
+    s = sh.workloads.triangular_sun_model(L1=6, L2=5, seed=31, precision=64)
+    announce("sun langevin, local sweeps, langevin with host noise")
+    integ = sh.Langevin(0.02, damping=0.1, kT=0.3)
+    sh.step_(s, integ, nsteps=3)  # device noise: the dipole planes become current
+    sh.step_(s, sh.LocalSampler(kT=0.5, nsweeps=1.0, propose=sh.propose_uniform))
+    Z = s.coherents_flat()
+    rng = np.random.default_rng(4)
+    noise = (rng.standard_normal((s.nsites, s.N)) + 1j * rng.standard_normal((s.nsites, s.N))) / np.sqrt(2)
+    Z_ref = O.Oracle(s.spec()).step_langevin(Z=Z, noise=noise, dt=integ.dt, damping=integ.damping, kT=integ.kT)
+    sh.step_(s, integ, noise=noise)
+    assert np.max(np.abs(s.coherents_flat() - Z_ref)) <= 1e-11
