@@ -58,6 +58,11 @@ def lib():
         L.so_ewald_grad_bruteforce.argtypes = [sp, _dp, _dp]
         L.so_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.so_philox_normals.argtypes = [C.c_uint64, C.c_uint64, C.c_long, C.c_int, _dp]
+        L.so_langevin_run.argtypes = [sp, _dp, _dp, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint64,
+                                      C.c_long, C.c_long, _dp]
+        L.so_ewald_precompute_sigma.argtypes = [_dp, C.c_int, _dp, C.POINTER(C.c_int), _dp, C.c_double,
+                                                C.c_double, _dp]
+        L.so_set_mutation.argtypes = [C.c_int]
         L.so_local_energy_change.argtypes = [sp, _dp, C.c_long, _dp]
         L.so_local_energy_change.restype = C.c_double
         L.so_local_sweeps.argtypes = [sp, _dp, _ip, _ip, C.c_int, C.c_int, _ip, _dp, _dp, C.c_double, C.c_uint64,
@@ -82,8 +87,9 @@ def _i(a):
     return None if a is None else a.ctypes.data_as(_ip)
 
 
-def ewald_A(spec, dims):
-    """Real-space Ewald tensor A[j][i][cell][3][3] (Ewald.jl:61-152)."""
+def ewald_A(spec, dims, sigma_div=3.0):
+    """Real-space Ewald tensor A[j][i][cell][3][3] (Ewald.jl:61-152), with the
+    splitting width sigma = cbrt(V) / sigma_div (the reference's is 3)."""
     na = spec["na"]
     nc = int(np.prod(dims))
     A = np.zeros((na, na, nc, 3, 3))
@@ -91,8 +97,29 @@ def ewald_A(spec, dims):
     lv = np.ascontiguousarray(spec["latvecs"], float)
     pos = np.ascontiguousarray(spec["positions"], float)
     dm = np.ascontiguousarray(spec["demag"], float)
-    lib().so_ewald_precompute(_d(lv), na, _d(pos), L, _d(dm), spec["mu0_muB2"], _d(A))
+    if sigma_div == 3.0:
+        lib().so_ewald_precompute(_d(lv), na, _d(pos), L, _d(dm), spec["mu0_muB2"], _d(A))
+    else:
+        lib().so_ewald_precompute_sigma(_d(lv), na, _d(pos), L, _d(dm), spec["mu0_muB2"], float(sigma_div), _d(A))
     return A
+
+
+class mutation:
+    """Context manager that plants one of the oracle's known mistakes (see
+    so_set_mutation) for a test showing that a check catches it."""
+
+    NOISE_SCALE, CORRECTOR_NOISE, DAMPING_KAPPA, EWALD_FRACTIONAL, EWALD_SWAPPED = 1, 2, 3, 4, 5
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        lib().so_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().so_set_mutation(0)
+        return False
 
 
 class Oracle:
@@ -247,6 +274,19 @@ class Oracle:
         Sd = np.zeros((self.ns, 3))
         it = lib().so_step_midpoint_sun(self._p(), _d(Zr), _d(Sd), dt, damping, tol, max_iters)
         return Zr[..., 0] + 1j * Zr[..., 1], it
+
+    def langevin_run(self, S=None, Z=None, dt=0.02, damping=0.1, kT=0.1, key=0, step0=0, nsteps=1, every=1):
+        """nsteps Langevin steps with the device's Philox noise stream; returns
+        the final state and the energies after every `every` steps."""
+        E = np.zeros(max(nsteps // every, 0) if every > 0 else 0)
+        if self.N == 0:
+            S = np.ascontiguousarray(S, float).reshape(self.ns, 3).copy()
+            lib().so_langevin_run(self._p(), _d(S), None, dt, damping, kT, key, step0, nsteps, every, _d(E))
+            return S, E
+        Zr = np.ascontiguousarray(np.stack([Z.real, Z.imag], -1), float).copy()
+        Sd = self.expected_spin(Z)
+        lib().so_langevin_run(self._p(), _d(Sd), _d(Zr), dt, damping, kT, key, step0, nsteps, every, _d(E))
+        return Zr[..., 0] + 1j * Zr[..., 1], E
 
     def ewald_energy(self, S):
         S = np.ascontiguousarray(S, float).reshape(self.ns, 3)

@@ -12,6 +12,16 @@
 
 typedef double complex cplx;
 
+/* Planted mistakes for the tests that show a check catches them (0 = none):
+   1  fill_noise! scales by sqrt(dt lambda kT), not sqrt(2 dt lambda kT)
+   2  the Heun corrector stage drops the noise term
+   3  the dipole damping term uses S/kappa in place of S
+   4  Ewald pair displacement taken in fractional coordinates
+   5  Ewald pair displacement c + p_i - p_j (atoms swapped)
+   Never set outside those tests. */
+static int g_mutation = 0;
+void so_set_mutation(int m) { g_mutation = m; }
+
 /* ------------------------------------------------------------------------ */
 /* Indexing: System.jl:204-212 (bonded_site with periodic wrap), Types.jl:131 */
 
@@ -470,6 +480,14 @@ void so_ewald_grad_bruteforce(const so_spec *sp, const double *S, double *dE)
 void so_ewald_precompute(const double lv[9], int na, const double *pos, const int L[3],
                          const double demag[9], double mu0_muB2, double *Aout)
 {
+    so_ewald_precompute_sigma(lv, na, pos, L, demag, mu0_muB2, 3.0, Aout);
+}
+
+/* Ewald.jl:61-152 with the splitting width sigma = cbrt(V) / sigma_div (the
+   reference fixes sigma_div = 3); the sum does not depend on it. */
+void so_ewald_precompute_sigma(const double lv[9], int na, const double *pos, const int L[3],
+                               const double demag[9], double mu0_muB2, double sigma_div, double *Aout)
+{
     /* superlattice vectors (columns) and reciprocal vectors (columns) */
     double SL[9], RL[9];
     for (int r = 0; r < 3; r++)
@@ -484,7 +502,7 @@ void so_ewald_precompute(const double lv[9], int na, const double *pos, const in
         for (int cc = 0; cc < 3; cc++) RL[3 * r + cc] = 2 * M_PI * inv[3 * cc + r];
     double V = det;
     double Lc = cbrt(V);
-    double sigma = Lc / 3, s2 = sigma * sigma, s3 = sigma * sigma * sigma;
+    double sigma = Lc / sigma_div, s2 = sigma * sigma, s3 = sigma * sigma * sigma;
     double rmax = 6 * sqrt(2.0) * sigma, kmax = 6 * sqrt(2.0) / sigma;
     int nmax[3], mmax[3];
     for (int ax = 0; ax < 3; ax++) {
@@ -503,10 +521,14 @@ void so_ewald_precompute(const double lv[9], int na, const double *pos, const in
             int c1 = cell % L[0], c2 = (cell / L[0]) % L[1], c3 = cell / ((long)L[0] * L[1]);
             for (int i = 0; i < na; i++) {
                 double acc[9] = {0};
-                double fr[3] = {c1 + pos[3 * j] - pos[3 * i], c2 + pos[3 * j + 1] - pos[3 * i + 1],
-                                c3 + pos[3 * j + 2] - pos[3 * i + 2]};
+                const int sw = g_mutation == 5; /* planted: atoms swapped */
+                const int a0 = sw ? j : i, a1 = sw ? i : j;
+                double fr[3] = {c1 + pos[3 * a1] - pos[3 * a0], c2 + pos[3 * a1 + 1] - pos[3 * a0 + 1],
+                                c3 + pos[3 * a1 + 2] - pos[3 * a0 + 2]};
                 double dr[3];
                 for (int r = 0; r < 3; r++) dr[r] = lv[3 * r] * fr[0] + lv[3 * r + 1] * fr[1] + lv[3 * r + 2] * fr[2];
+                if (g_mutation == 4) /* planted: fractional displacement */
+                    for (int r = 0; r < 3; r++) dr[r] = fr[r];
                 for (int n1 = -nmax[0]; n1 <= nmax[0]; n1++)
                     for (int n2 = -nmax[1]; n2 <= nmax[1]; n2++)
                         for (int n3 = -nmax[2]; n3 <= nmax[2]; n3++) {
@@ -792,9 +814,10 @@ static inline void cross(const double *a, const double *b, double *c)
 }
 
 /* rhs_dipole!, :266-275 */
-static void rhs_dipole(long ns, double *dS, const double *S, const double *xi, const double *dE,
-                       double dt, double lambda)
+static void rhs_dipole(const so_spec *sp, long ns, double *dS, const double *S, const double *xi,
+                       const double *dE, double dt, double lambda)
 {
+    const long nc = ncells(sp);
     for (long s = 0; s < ns; s++) {
         const double *Sv = S + 3 * s, *g = dE + 3 * s;
         double v[3];
@@ -803,6 +826,8 @@ static void rhs_dipole(long ns, double *dS, const double *S, const double *xi, c
         } else {
             double sxg[3];
             cross(Sv, g, sxg);
+            if (g_mutation == 3) /* planted: damping with S / kappa */
+                for (int r = 0; r < 3; r++) sxg[r] /= sp->kappa[s / nc];
             for (int r = 0; r < 3; r++) v[r] = xi[3 * s + r] + dt * g[r] - dt * lambda * sxg[r];
         }
         double o[3];
@@ -842,7 +867,7 @@ static void fill_noise(long n, double *xi, const double *raw, double dt, double 
         memset(xi, 0, sizeof(double) * n);
         return;
     }
-    double sc = sqrt(2 * dt * lambda * kT);
+    double sc = sqrt((g_mutation == 1 ? 1 : 2) * dt * lambda * kT);
     for (long k = 0; k < n; k++) xi[k] = raw[k] * sc;
 }
 
@@ -856,13 +881,14 @@ void so_step_langevin_dipole(const so_spec *sp, double *S, const double *noise, 
            *dE = malloc(sizeof(double) * 3 * ns);
     fill_noise(3 * ns, xi, noise, dt, damping, kT);
     so_grad_dipoles(sp, S, dE);
-    rhs_dipole(ns, dS1, S, xi, dE, dt, damping);
+    rhs_dipole(sp, ns, dS1, S, xi, dE, dt, damping);
     for (long s = 0; s < ns; s++) {
         for (int r = 0; r < 3; r++) Sp[3 * s + r] = S[3 * s + r] + dS1[3 * s + r];
         normalize_dipole(Sp + 3 * s, sp->kappa[s / nc]);
     }
     so_grad_dipoles(sp, Sp, dE);
-    rhs_dipole(ns, dS2, Sp, xi, dE, dt, damping);
+    if (g_mutation == 2) memset(xi, 0, sizeof(double) * 3 * ns); /* planted: corrector without noise */
+    rhs_dipole(sp, ns, dS2, Sp, xi, dE, dt, damping);
     for (long s = 0; s < ns; s++) {
         for (int r = 0; r < 3; r++) S[3 * s + r] = S[3 * s + r] + (dS1[3 * s + r] + dS2[3 * s + r]) / 2;
         normalize_dipole(S + 3 * s, sp->kappa[s / nc]);
@@ -914,6 +940,7 @@ void so_step_langevin_sun(const so_spec *sp, double *Zr, double *S, const double
         normalize_ket(N, Zp + N * s, sp->kappa[s / nc]);
     }
     so_grad_coherents(sp, (double *)Zp, (double *)HZ);
+    if (g_mutation == 2) memset(zeta, 0, sizeof(cplx) * N * ns); /* planted: corrector without noise */
     rhs_sun(N, ns, dZ2, Zp, zeta, HZ, dt, damping);
     for (long s = 0; s < ns; s++) {
         for (int a = 0; a < N; a++) Z[N * s + a] = Z[N * s + a] + (dZ1[N * s + a] + dZ2[N * s + a]) / 2;
@@ -962,7 +989,7 @@ int so_step_midpoint_dipole(const so_spec *sp, double *S, double dt, double damp
             normalize_dipole(Sh + 3 * s, sp->kappa[s / nc]);
         }
         so_grad_dipoles(sp, Sh, dE);
-        rhs_dipole(ns, dS, Sh, xi, dE, dt, damping);
+        rhs_dipole(sp, ns, dS, Sh, xi, dE, dt, damping);
         for (long k = 0; k < 3 * ns; k++) S2[k] = S[k] + dS[k];
         if (fast_isapprox(ns, S1, S2, 3, atol)) {
             for (long s = 0; s < ns; s++) {
@@ -1066,6 +1093,31 @@ void so_philox_normals(uint64_t seed, uint64_t step, long ns, int per_site, doub
                 if (k0 + 1 < per_site) out[(long)per_site * s + k0 + 1] = r * sin(th);
             }
         }
+}
+
+/* nsteps of step!(sys, Langevin(dt; damping, kT)) (Integrators.jl:313-357)
+   drawing the noise of step step0 + k from the device's Philox stream
+   (so_philox_normals under `key`: 3 standard normals per site, or 2N normals
+   / sqrt(2) as the (re, im) parts of N complex normals), recording
+   energy(sys) after every `every` steps into E[nsteps / every].  S holds the
+   dipoles (the expected spins in :SUN mode, kept in step with Z). */
+void so_langevin_run(const so_spec *sp, double *S, double *Zr, double dt, double damping, double kT, uint64_t key,
+                     uint64_t step0, long nsteps, long every, double *E)
+{
+    const long ns = nsites(sp);
+    const int per = sp->N == 0 ? 3 : 2 * sp->N;
+    double *raw = malloc(sizeof(double) * per * ns);
+    for (long k = 0; k < nsteps; k++) {
+        so_philox_normals(key, step0 + (uint64_t)k, ns, per, raw);
+        if (sp->N == 0) {
+            so_step_langevin_dipole(sp, S, raw, dt, damping, kT);
+        } else {
+            for (long q = 0; q < per * ns; q++) raw[q] *= M_SQRT1_2;
+            so_step_langevin_sun(sp, Zr, S, raw, dt, damping, kT);
+        }
+        if (every > 0 && (k + 1) % every == 0) E[(k + 1) / every - 1] = so_energy(sp, S, sp->N ? Zr : NULL);
+    }
+    free(raw);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -100,6 +100,21 @@ void so_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]
 void so_philox_normals(uint64_t seed, uint64_t step, long nsites, int per_site,
                        double *out);
 
+/* nsteps Langevin steps with the device's Philox noise stream (key, step0 + k),
+   energy after every `every` steps into E.  S: dipoles (:SUN: expected spins
+   of Zr, updated with it); Zr: NULL in :dipole mode. */
+void so_langevin_run(const so_spec *sp, double *S, double *Zr, double dt, double damping, double kT, uint64_t key,
+                     uint64_t step0, long nsteps, long every, double *E);
+/* so_ewald_precompute with the splitting width sigma = cbrt(V) / sigma_div
+   (the reference's sigma_div is 3). */
+void so_ewald_precompute_sigma(const double latvecs[9], int na, const double *positions, const int L[3],
+                               const double demag[9], double mu0_muB2, double sigma_div, double *A);
+/* Planted mistakes, for tests that show a check catches them: 0 none,
+   1 noise scaled by sqrt(dt lambda kT), 2 corrector without noise, 3 dipole
+   damping with S/kappa, 4 Ewald displacement in fractional coordinates,
+   5 Ewald displacement with the two atoms swapped. */
+void so_set_mutation(int m);
+
 /* local_energy_change (Interactions.jl:311-380, Ewald.jl:234-241), :dipole
    mode: energy change when the dipole of `site` becomes Snew.  NAN for a
    bond that maps the site onto itself. */

@@ -17,13 +17,16 @@ def single_dipole(dims, precision=64):
     return s
 
 
-@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 3, 4)])
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 3, 4), (8, 8, 8), (16, 16, 16), (32, 8, 8)])
 def test_single_dipole_cubic_box(dims):
     """E per site = -1/6 for a lattice of parallel dipoles, demag = 0
-    (test/test_ewald.jl:16-31)."""
+    (test/test_ewald.jl:16-31): the register line transforms of every length
+    the larger grids use (2 ... 32) and the device tensor at 16^3."""
     announce(f"ewald single dipole {dims}")
     s = single_dipole(dims)
-    assert abs(sh.energy_per_site(s) - (-1 / 6)) < 1e-13
+    assert abs(sh.energy_per_site(s) - (-1 / 6)) < 1e-12
+    s32 = single_dipole(dims, precision=32)
+    assert abs(sh.energy_per_site(s32) - (-1 / 6)) < 1e-6
 
 
 def random_triclinic(precision):
@@ -132,3 +135,69 @@ def test_ewald_batch_fp32(R, model):
             S = o.step_langevin(S=S, noise=O.philox_normals(key, k, s.nsites, 3), dt=integ.dt,
                                 damping=integ.damping, kT=kTs[r])
         assert rel_err(s.dipoles_flat(), S, scale=1.0) <= 1e-4, r
+
+
+def test_triclinic_demag_third_matches_direct_sum():
+    """The triclinic cell with demag = 1/3 on the device against the oracle
+    and against the independent direct dipole sum that pins the oracle."""
+    from test_oracle_ewald import direct_dipole_sum, triclinic_dims
+
+    for dims in [(1, 1, 1), (2, 3, 1), (3, 2, 2)]:
+        announce(f"ewald triclinic demag 1/3 {dims} fp64")
+        s = triclinic_dims(dims)
+        S = s.dipoles_flat()
+        o = O.Oracle(s.spec())
+        E = sh.energy(s)
+        assert abs(E - o.energy(S)) <= 1e-11 * abs(E)
+        assert rel_err(sh.energy_grad_dipoles(s), o.grad(S)) <= 1e-11
+        if s.nsites <= 18:  # where the cutoff sum converges at Rc = 20
+            E_dir, _ = direct_dipole_sum(s, S, Rc=20.0)
+            assert abs(E - E_dir) <= 2e-6 * abs(E)
+
+
+@pytest.mark.parametrize("L", [4, 8, 16])
+def test_dipolar_pyrochlore_sizes(L):
+    """Dipolar pyrochlore (16 sublattices) at L^3 cells: device tensor, line
+    transforms of length L and the per-k contraction against the oracle, FP64
+    to 1e-11 and FP32 to 3e-5; at 16^3 this is the Ewald workload's size."""
+    kw = dict(L=L, J1=0.05, J2=0, A=0, s=3.5, seed=40 + L, dipolar=True)
+    s64 = sh.workloads.pyrochlore_model(**kw, precision=64)
+    S = s64.dipoles_flat()
+    announce(f"ewald pyrochlore {L}^3 fp64 + fp32")
+    o = O.Oracle(s64.spec())
+    g_ref, E_ref = o.grad(S), o.energy(S)
+    scale = np.abs(g_ref).max() * np.abs(S).max() * s64.nsites
+    assert rel_err(sh.energy_grad_dipoles(s64), g_ref) <= 1e-11
+    assert abs(sh.energy(s64) - E_ref) <= 1e-11 * scale
+    s32 = sh.workloads.pyrochlore_model(**kw, precision=32)
+    s32.set_state_flat(dip=S)
+    assert rel_err(sh.energy_grad_dipoles(s32), g_ref) <= 3e-5
+    assert abs(sh.energy(s32) - E_ref) <= 3e-5 * scale
+
+
+def test_ewald_batch_pyrochlore8_mfma():
+    """Sixteen dipolar pyrochlore 8^3 replicas: one column tile of the matrix
+    core contraction; energies and one Langevin step of every replica, three
+    of them against their own FP64 oracle."""
+    R = 16
+    systems = [sh.workloads.pyrochlore_model(L=8, J1=0.05, J2=0, A=0, s=3.5, seed=80 + r, dipolar=True,
+                                             precision=32) for r in range(R)]
+    announce(f"ewald batch pyrochlore 8^3 R={R} fp32")
+    refs = [s.dipoles_flat() for s in systems]
+    batch = sh.ReplicaBatch(systems, precision=32)
+    E_dev = sh.energy(batch)
+    integ = sh.Langevin(0.01, damping=0.1, kT=0.2)
+    sh.step_(batch, integ, nsteps=1)
+    batch.sync_to_systems()
+    A = O.ewald_A(systems[0].spec(), systems[0].dims)
+    for r in (0, 7, 15):
+        s = systems[r]
+        o = O.Oracle(s.spec(), A=A)
+        S = refs[r]
+        g = o.grad(S)
+        scale = np.abs(g).max() * np.abs(S).max() * s.nsites
+        assert abs(E_dev[r] - o.energy(S)) <= 3e-5 * scale, r
+        key = O.replica_key(batch.noise_seed, r)
+        S = o.step_langevin(S=S, noise=O.philox_normals(key, 0, s.nsites, 3), dt=integ.dt, damping=integ.damping,
+                            kT=integ.kT)
+        assert rel_err(s.dipoles_flat(), S, scale=3.5) <= 1e-4, r

@@ -28,14 +28,18 @@ def state(obj):
 @pytest.mark.parametrize("precision", [32, 64])
 def test_paired_groups_match_one_stream(precision, monkeypatch):
     announce(f"paired replica groups fp{precision}")
+    if precision == 64:
+        # the cost model leaves FP64 batches on the ghost layout (too few
+        # resident waves); an explicit tile request is honoured
+        monkeypatch.setenv("SNH_TILE_T1", "4")
+        monkeypatch.setenv("SNH_TILE_T2", "2")
     monkeypatch.setenv("SNH_PAIR", "2")
     a = batch(precision)
     ha = a._device()
     monkeypatch.setenv("SNH_PAIR", "0")
     b = batch(precision)
     hb = b._device()
-    if ha.query("layout") != 0:
-        pytest.skip("the batch does not fit the tiled layout here")
+    assert ha.query("layout") == 0 and hb.query("layout") == 0
     assert ha.query("pair_on") == 1 and hb.query("pair_on") == 0
     kTs = np.linspace(0.2, 0.9, a.nreplicas)
     integ = sh.Langevin(0.02, damping=0.1, kT=0.5)
