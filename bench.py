@@ -87,13 +87,37 @@ def measured_traffic(kernel, plan):
     return k["fetch_bytes"] + k["write_bytes"]
 
 
+def host_cpu():
+    """Cores this process may use and the host CPU model."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    # a worker pool is sized to the lease's CPU share (OMP_NUM_THREADS where set)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return {"nproc": os.cpu_count(), "affinity": allowed, "threads": min(allowed, share) if share > 0 else allowed,
+            "model": model}
+
+
 def cpu_baseline(base_sys, seconds=15.0):
     """FP64 CPU port (the oracle) on the same 524,288-spin replica: one
-    independent replica per thread, as Threads.@threads over clones."""
+    independent replica per thread, as Threads.@threads over clones.  A
+    stand-in, not Sunny: Julia is not installed, so Sunny's own step! cannot
+    run on this host."""
     from concurrent.futures import ThreadPoolExecutor
 
     from oracle import oracle as O
-    cores = max(1, min(16, os.cpu_count() or 1))
+    host = host_cpu()
+    cores = max(1, host["threads"])
     spec = base_sys.spec()
     rng = np.random.default_rng(5)
     states = []
@@ -120,8 +144,47 @@ def cpu_baseline(base_sys, seconds=15.0):
         t1 = time.perf_counter()
     value = done * base_sys.nsites / (t1 - t0)
     return {"value": value, "unit": "spin-updates/s", "cores": cores, "kind": "port",
+            "label": "stand-in, not Sunny (FP64 C restatement of step!; no Julia on this host)",
+            "host": host,
             "sample": f"{cores} replicas x {nsteps} Langevin steps of the 524288-spin pyrochlore model, "
                       f"FP64 CPU port (oracle), one replica per thread, {t1 - t0:.1f} s"}
+
+
+def fp32_vs_fp64(make, integ, checkpoints=(1, 10, 100), kappa=None, coherent=False, kTs=None):
+    """Error of the FP32 device path against the FP64 device path (parity
+    with the FP64 oracle to 1e-11 is what the GPU tests establish) on the
+    same initial state and the same Philox noise: relative energy deviation,
+    max | |S| - kappa | (or | |Z|^2 - kappa |) of the FP32 state, and the max
+    per-site deviation, after each checkpoint's number of steps."""
+    import sunnyhip as sh
+    a, b = make(32), make(64)
+    out, done = {}, 0
+    for k in checkpoints:
+        sh.step_(a, integ, nsteps=k - done, kTs=kTs)
+        sh.step_(b, integ, nsteps=k - done, kTs=kTs)
+        done = k
+        Ea, Eb = np.atleast_1d(sh.energy(a)), np.atleast_1d(sh.energy(b))
+        if hasattr(a, "sync_to_systems"):
+            a.sync_to_systems()
+            b.sync_to_systems()
+            sa, sb = a.systems, b.systems
+        else:
+            sa, sb = [a], [b]
+        if coherent:
+            Za = np.concatenate([s.coherents_flat() for s in sa])
+            Zb = np.concatenate([s.coherents_flat() for s in sb])
+            kap = np.concatenate([np.repeat(s.kappas, s.ncells) for s in sa])
+            norm = float(np.max(np.abs((np.abs(Za) ** 2).sum(1) - kap)))
+            dev = float(np.max(np.abs(Za - Zb)))
+        else:
+            Sa = np.concatenate([s.dipoles_flat() for s in sa])
+            Sb = np.concatenate([s.dipoles_flat() for s in sb])
+            kap = np.concatenate([np.repeat(s.kappas, s.ncells) for s in sa])
+            norm = float(np.max(np.abs(np.linalg.norm(Sa, axis=1) - kap)))
+            dev = float(np.max(np.abs(Sa - Sb)))
+        out[f"step_{k}"] = {"rel_energy": float(np.max(np.abs(Ea - Eb) / np.abs(Eb))), "max_norm_error": norm,
+                            "max_site_deviation": dev}
+    return out
 
 
 def bench_sun3(args):
@@ -147,9 +210,17 @@ def bench_sun3(args):
     log(f"SU(3) midpoint: {mid['value']:.3e} spin-updates/s, {mid['iterations']} iterations")
     per = {f: ms / max(n, 1) for f, (ms, n) in fams.items()}
     log(f"SU(3) batch: {batch.nsites} sites, {dt * 1e3 / args.steps:.3f} ms/step, {ups:.3e} spin-updates/s, {per}")
+    # FP32 against FP64 on the first four replicas (uneven sizes), same noise
+    sub = systems[:4]
+
+    def sub_batch(prec):
+        return sh.ReplicaBatch([sh.clone_system(x) for x in sub], precision=prec)
+
+    err = fp32_vs_fp64(sub_batch, sh.Langevin(0.02, damping=0.1, kT=0.0), coherent=True, kTs=kTs[:4])
+    log(f"SU(3) FP32 vs FP64: {err}")
     print(json.dumps({"metric": "spin-updates/s (auxiliary: SU(3) tempering batch)", "value": ups,
                       "unit": "spin-updates/s", "ms_per_step": dt * 1e3 / args.steps, "steps": args.steps,
-                      "dtype": "f32", "kernels_ms": per, "implicit_midpoint": mid,
+                      "dtype": "f32", "kernels_ms": per, "implicit_midpoint": mid, "fp32_vs_fp64": err,
                       "config": {"workload": "64 SU(3) triangular replicas, L uniform 32..256 (seed 0), "
                                              f"{batch.nsites} sites, kT 0.1..2.0 geometric, dt 0.02, damping 0.1"}}),
           flush=True)
@@ -175,6 +246,20 @@ def bench_cubic16(args):
     h.call("snh_synchronize")
     dt_m = time.perf_counter() - t0
     o = O.Oracle(s.spec())
+    # FP64 device against the FP64 oracle from one state with the same draws
+    c = sh.workloads.cubic_model(L=16, seed=5, precision=64)
+    S_ref, done, vs_oracle = c.dipoles_flat(), 0, {}
+    key = O.replica_key(c.noise_seed, 0)
+    for k in (1, 10, 100):
+        S_ref, _ = o.langevin_run(S=S_ref, dt=0.05, damping=0.1, kT=0.5, key=key, step0=done, nsteps=k - done,
+                                  every=0)
+        sh.step_(c, lang, nsteps=k - done)
+        done = k
+        Sd = c.dipoles_flat()
+        vs_oracle[f"step_{k}"] = {"rel_energy": abs(sh.energy(c) - o.energy(S_ref)) / abs(o.energy(S_ref)),
+                                  "max_norm_error": float(np.max(np.abs(np.linalg.norm(Sd, axis=1) - 1.0))),
+                                  "max_site_deviation": float(np.max(np.abs(Sd - S_ref)))}
+    log(f"cubic 16^3 FP64 device vs oracle: {vs_oracle}")
     S = s.dipoles_flat()
     nz = np.random.default_rng(1).standard_normal((s.nsites, 3))
     ncpu = 200
@@ -184,6 +269,7 @@ def bench_cubic16(args):
     dt_c = time.perf_counter() - t0
     out = {"langevin": s.nsites * nst / dt_l, "midpoint": s.nsites * nst / dt_m,
            "midpoint_iterations_last": int(its[0]), "cpu_port_langevin_1core": s.nsites * ncpu / dt_c,
+           "fp64_vs_oracle": vs_oracle,
            "ms_per_step": {"langevin": dt_l * 1e3 / nst, "midpoint": dt_m * 1e3 / nst}}
     log(f"cubic 16^3 FP64: Langevin {out['langevin']:.3e}, midpoint {out['midpoint']:.3e} spin-updates/s "
         f"({out['ms_per_step']}); CPU port {out['cpu_port_langevin_1core']:.3e}")
@@ -350,8 +436,19 @@ def bench_ewald(args):
         ups = ns * args.steps / dt
         log(f"Ewald R={R}: {ns} sites, {dt * 1e3 / args.steps:.3f} ms/step, {ups:.3e} spin-updates/s, {per}")
         out[R] = {"ms_per_step": dt * 1e3 / args.steps, "value": ups, "kernels_ms": per}
+        del obj, systems
+
+    def ewald_model(prec):
+        return sh.workloads.pyrochlore_model(L=16, J1=0.05, J2=0.0, A=0.0, s=3.5, seed=40, precision=prec,
+                                             dipolar=True, mu0_muB2=0.6745817653324668, demag=1 / 3)
+
+    err = fp32_vs_fp64(ewald_model, sh.Langevin(0.025, damping=0.2, kT=0.3))
+    # the gradient itself at the initial state: FP32 against FP64
+    g32, g64 = sh.energy_grad_dipoles(ewald_model(32)), sh.energy_grad_dipoles(ewald_model(64))
+    err["grad_initial_rel"] = float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64)))
+    log(f"Ewald FP32 vs FP64: {err}")
     print(json.dumps({"metric": "spin-updates/s (auxiliary: Ewald pyrochlore 16^3)", "unit": "spin-updates/s",
-                      "dtype": "f32", "steps": args.steps, "by_replicas": out,
+                      "dtype": "f32", "steps": args.steps, "by_replicas": out, "fp32_vs_fp64": err,
                       "config": {"workload": "pyrochlore 16^3 cells x 16 = 65536 spins, s=7/2, J1=0.05, "
                                              "Ewald mu0muB2=0.6745817653324668 demag 1/3, Langevin dt 0.025"}}),
           flush=True)
@@ -453,6 +550,15 @@ def main():
     ups64 = plain.nsites * steps64 / dt64
     log(f"FP64 plain: {dt64 * 1e3 / steps64:.3f} ms/step, {ups64:.3e} spin-updates/s")
 
+    # FP64 batched: the same replicas in FP64 (no longer cache-resident), to
+    # separate the effect of precision from that of the working-set size
+    batch64 = sh.ReplicaBatch(reps, precision=64, layout=layout)
+    steps64b = max(10, args.steps // 8)
+    dt64b, _ = timed_steps(batch64, integ, steps64b, min(args.warmup, 5))
+    ups64b = batch64.nsites * steps64b / dt64b
+    log(f"FP64 batched: {dt64b * 1e3 / steps64b:.3f} ms/step, {ups64b:.3e} spin-updates/s")
+    del batch64
+
     # ImplicitMidpoint (deterministic, damping 0) on the same batch and plain
     # system: spin-updates/s = spins x steps / time; iterations reported
     mp = sh.ImplicitMidpoint(0.025, damping=0.0, tol=1e-5)
@@ -491,15 +597,14 @@ def main():
         f"FP64 plain {midpoint['fp64_plain']:.3e} ({midpoint['iterations_fp64']} it), "
         f"iter kernel {midpoint['iter_kernel_ms']:.4f} ms = {midpoint['iter_kernel_GBs']:.0f} GB/s")
 
-    # FP32 vs FP64 on the same start and noise: energy deviation after k steps
-    a = sh.clone_system(base)
-    b = sh.clone_system(base)
-    a.precision, b.precision = 32, 64
-    sh.step_(a, integ, nsteps=10)
-    sh.step_(b, integ, nsteps=10)
-    Ea, Eb = sh.energy(a), sh.energy(b)
-    dS = float(np.max(np.abs(a.dipoles_flat() - b.dipoles_flat())))
-    log(f"FP32 vs FP64 after 10 steps: rel dE = {abs(Ea - Eb) / abs(Eb):.2e}, max|dS| = {dS:.2e}")
+    # FP32 vs FP64 on the same start and noise (one 524,288-spin replica)
+    def one_replica(prec):
+        c = sh.clone_system(base)
+        c.precision = prec
+        return c
+
+    err32 = fp32_vs_fp64(one_replica, integ)
+    log(f"FP32 vs FP64: {err32}")
 
     # plain copy of the batch's state planes: calibrates traffic counters
     # (profiles/) and gives the achievable streaming bandwidth
@@ -511,18 +616,24 @@ def main():
     copy_gbs = nbytes.value / (cms / cn * 1e-3) / 1e9
     log(f"copy kernel: {nbytes.value / 1e6:.1f} MB per launch, {copy_gbs:.0f} GB/s")
 
-    # roofline of the dominant kernel of the batched step
+    # roofline of the batched Heun step (predictor + corrector launches):
+    # compulsory HBM bytes per spin-update are read S + write S = 24 B in
+    # FP32; the two-launch Heun moves 36 B per spin in each stage (the
+    # predictor reads S, writes S' and H; the corrector reads S' and H,
+    # writes S), reported per stage below
     ms_p, n_p = fams32["dipole_predict"]
     ms_c, n_c = fams32["dipole_correct"]
-    # both stages move 36 B per spin: predict reads S, writes S' and H;
-    # correct reads S' and H, writes S
-    dom, ms, n, bytes_per_site = ("dipole_correct", ms_c, n_c, 36) if ms_c >= ms_p else \
-        ("dipole_predict", ms_p, n_p, 36)
-    avg_ms = ms / max(n, 1)
-    alg_bytes = bytes_per_site * batch.nsites
-    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    avg_p, avg_c = ms_p / max(n_p, 1), ms_c / max(n_c, 1)
+    pair_ms = avg_p + avg_c
+    alg_bytes = 24 * batch.nsites
+    achieved = alg_bytes / (pair_ms * 1e-3) / 1e9
+    tr_p, tr_c = measured_traffic("dipole_predict", plan), measured_traffic("dipole_correct", plan)
+    traffic = tr_p + tr_c if tr_p is not None and tr_c is not None else None
     step_ms = dt32 * 1e3 / args.steps
     step_achieved = 24 * batch.nsites / (step_ms * 1e-3) / 1e9
+    stages = {k: {"avg_launch_ms": a, "bytes_per_spin": 36, "achieved": 36 * batch.nsites / (a * 1e-3) / 1e9,
+                  "traffic": t}
+              for k, a, t in (("dipole_predict", avg_p, tr_p), ("dipole_correct", avg_c, tr_c))}
 
     line = {
         "metric": "spin-updates/s, Langevin midpoint step, FP32 batched vs FP64 plain; % HBM peak",
@@ -540,16 +651,22 @@ def main():
         "config": {"workload": WORKLOAD, "replicas": R, "spins_per_replica": base.nsites,
                    "integrator": "Langevin (stochastic Heun), device Philox noise", "device_plan": plan},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom, plan), "kernel": dom,
-                     "bytes_per_spin": bytes_per_site, "avg_launch_ms": avg_ms},
-        "roofline_step": {"bytes_per_spin_update": 24, "achieved": step_achieved, "unit": "GB/s",
-                          "frac": step_achieved / HBM_PEAK_GBS},
-        "kernels_ms": {"dipole_predict": ms_p / max(n_p, 1), "dipole_correct": ms_c / max(n_c, 1)},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_over_algorithmic": traffic / alg_bytes if traffic else None,
+                     "kernel": "dipole_predict + dipole_correct (one Heun step)", "bytes_per_spin_update": 24,
+                     "algorithmic_bytes": alg_bytes, "avg_launch_ms": pair_ms},
+        "roofline_stages": stages,
+        "roofline_wallclock": {"bytes_per_spin_update": 24, "achieved": step_achieved, "unit": "GB/s",
+                               "frac": step_achieved / HBM_PEAK_GBS},
+        "kernels_ms": {"dipole_predict": avg_p, "dipole_correct": avg_c},
         "copy_kernel": {"bytes_per_launch": nbytes.value, "GB/s": copy_gbs},
         "fp64_plain": {"value": ups64, "unit": "spin-updates/s", "ms_per_step": dt64 * 1e3 / steps64,
                        "steps": steps64, "dtype": "f64"},
+        "fp64_batched": {"value": ups64b, "unit": "spin-updates/s", "ms_per_step": dt64b * 1e3 / steps64b,
+                         "steps": steps64b, "dtype": "f64", "replicas": R},
         "speedup_fp32_batched_vs_fp64_plain": ups32 / ups64,
-        "fp32_vs_fp64_after_10_steps": {"rel_energy": abs(Ea - Eb) / abs(Eb), "max_abs_dS": dS},
+        "speedup_fp32_batched_vs_fp64_batched": ups32 / ups64b,
+        "fp32_vs_fp64": err32,
         "implicit_midpoint": midpoint,
         "cpu_baseline": None,
     }
