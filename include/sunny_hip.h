@@ -134,6 +134,22 @@ int snh_step_midpoint(snh_system *sys, double dt, double damping, double tol, in
  * src/System/Ewald.jl:155-181).  E: [R]. */
 int snh_energy(snh_system *sys, double *E);
 
+/* No counterpart in the reference: the variant of snh_energy for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  E_dev: DEVICE memory for [R] FP64
+ * energies, written asynchronously on the handle's stream (synchronise, or
+ * wait on an event recorded on that stream, before reading).  The call
+ * allocates nothing, device or host, and does not synchronise: the
+ * per-workgroup partial sums live in a static device array of
+ * SNH_NOALLOC_PARTIALS doubles (2 MiB, one copy per GPU, held until the
+ * process ends), whose address snh_create resolves for its device.  A handle
+ * whose energy launch needs more partial sums than that gets
+ * SNH_ERR_UNSUPPORTED (snh_energy still serves it).  Every call on one GPU
+ * uses that GPU's array: calls that overlap on one GPU (handles on different
+ * streams) race. */
+#define SNH_NOALLOC_PARTIALS 262144
+int snh_energy_noalloc(snh_system *sys, double *E_dev);
+
 /* set_energy_grad_dipoles! (:dipole; out [sites][3]) or
  * set_energy_grad_coherents! (:SUN; out [sites][N][2]) at the current state
  * (src/System/Interactions.jl:516-669). */

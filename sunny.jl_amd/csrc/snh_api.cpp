@@ -378,6 +378,7 @@ struct snh_system {
                                uint64_t step0, int nsteps) = 0;
     virtual void step_midpoint(double dt, double damping, double tol, int max_iters, int nsteps, int* iters) = 0;
     virtual void energy(double* E) = 0;
+    virtual void energy_noalloc(double* E_dev) = 0;
     virtual void energy_grad(double* out) = 0;
     virtual void synchronize() = 0;
     virtual void enable_timing(bool on) = 0;
@@ -403,6 +404,7 @@ namespace {
 template <typename T>
 struct SystemImpl final : snh_system {
     hipStream_t st = nullptr;
+    int dev = 0;  // the device the handle was created on
     int R = 0, na = 0, N = 0, nb = 0, nblocks = 0;
     uint32_t total = 0;  // padded sites over all replicas (plane stride)
     Ghost G{};
@@ -564,7 +566,10 @@ struct SystemImpl final : snh_system {
         nb = sp.nbonds;
         ewald = sp.ewald != 0;
         planes = (N == 0) ? 3 : 2 * N;
+        SNH_HIP(hipGetDevice(&dev));
         SNH_HIP(hipStreamCreate(&st));
+        // this device's copy of the static scratch of snh_energy_noalloc
+        SNH_HIP(noalloc_scratch_init());
 
         // ghost widths: the longest bond offset along each axis
         int w[3] = {0, 0, 0};
@@ -984,14 +989,14 @@ struct SystemImpl final : snh_system {
 
     // Ewald potential phi of state planes Xs (dipoles; expected dipoles for
     // SU(N), computed here unless the caller holds them in dip).
-    void ewald_field(const T* Xs, T* phi, const T* dip = nullptr)
+    void ewald_field(const T* Xs, T* phi, const T* dip = nullptr, bool timed = true)
     {
         const T* S = dip ? dip : Xs;
         if (N > 0 && !dip) {
             launch_sun_expected_spin<T>(C, nblocks, Xs, Sd, st);
             S = Sd;
         }
-        hipEvent_t e = tbegin();
+        hipEvent_t e = timed ? tbegin() : nullptr;
         launch_ewald_field<T>(EG, R, C.reps, total, C.gmat, S, M, Fmu, Fphi, phi, st);
         tend("ewald_field", e);
     }
@@ -2070,24 +2075,47 @@ struct SystemImpl final : snh_system {
         }
     }
 
-    void energy(double* E) override
+    // Queues energy_aux of every replica on the handle's stream: per-workgroup
+    // partial sums into `part` (one entry per workgroup of the energy launch,
+    // each written by its workgroup), then their sums per replica into out[R].
+    void energy_launch(double* part, double* out, bool timed)
     {
         const T* phi = nullptr;
         if (ewald) {
-            ewald_field(X, phi1);
+            ewald_field(X, phi1, nullptr, timed);
             phi = phi1;
         }
         if (tiled)
-            launch_tile_energy<T>(C, tp.A, tp.threads, X, phi, partial, st);
+            launch_tile_energy<T>(C, tp.A, tp.threads, X, phi, part, st);
         else if (N == 0)
-            launch_dipole_energy<T>(C, nblocks, X, phi, partial, st);
+            launch_dipole_energy<T>(C, nblocks, X, phi, part, st);
         else
-            launch_sun_energy<T>(C, nblocks, X, phi, partial, st);
-        launch_reduce_partials(partial, tiled ? d_trblk : d_rblk, R, d_out, st);
+            launch_sun_energy<T>(C, nblocks, X, phi, part, st);
+        launch_reduce_partials(part, tiled ? d_trblk : d_rblk, R, out, st);
         SNH_HIP(hipGetLastError());
+    }
+
+    void energy(double* E) override
+    {
+        energy_launch(partial, d_out, true);
         SNH_HIP(hipMemcpyAsync(pinned, d_out, sizeof(double) * R, hipMemcpyDeviceToHost, st));
         SNH_HIP(hipStreamSynchronize(st));
         for (int r = 0; r < R; r++) E[r] = pinned[r];
+    }
+
+    // energy() into device memory with the partial sums in the static scratch:
+    // no allocation, no host synchronisation, no event.
+    void energy_noalloc(double* E_dev) override
+    {
+        SNH_REQUIRE(E_dev, SNH_ERR_INVALID, "null energy buffer");
+        const size_t need = tiled ? (size_t)tp.A.nwork : (size_t)nblocks;
+        SNH_REQUIRE(need <= kNoAllocPartials, SNH_ERR_UNSUPPORTED,
+                    "snh_energy_noalloc: the energy launch needs " + std::to_string(need) +
+                        " partial sums, more than the static scratch holds (" + std::to_string(kNoAllocPartials) +
+                        "); use snh_energy");
+        double* part = noalloc_scratch(dev);
+        SNH_REQUIRE(part, SNH_ERR_HIP, "static scratch not resolved on this device");
+        energy_launch(part, E_dev, false);
     }
 
     void energy_grad(double* out) override
@@ -2238,6 +2266,11 @@ extern "C" int snh_create(const snh_spec* spec, snh_system** out)
         delete sys;
         return SNH_ERR_INVALID;
     }
+}
+
+extern "C" int snh_energy_noalloc(snh_system* sys, double* E_dev)
+{
+    return guarded(sys, [&] { sys->energy_noalloc(E_dev); });
 }
 
 extern "C" int snh_destroy(snh_system* sys)

@@ -440,18 +440,6 @@ bool ewald_use_mfma(const EwaldGeom& G, int R, size_t elem)
     return elem == sizeof(float) && R >= kEwaldMfmaMinReplicas && 3 * G.na <= 16 * kMfmaMaxRowTiles;
 }
 
-// Raises a kernel's dynamic-LDS limit (once per kernel and size).
-static void ewald_allow_lds(const void* fn, size_t bytes)
-{
-    static std::mutex mu;
-    static std::map<const void*, size_t> done;
-    std::lock_guard<std::mutex> lock(mu);
-    size_t& have = done[fn];
-    if (have >= bytes) return;
-    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    have = bytes;
-}
-
 void launch_ewald_A(const EwaldAParams& P, const double* pos, double* A, hipStream_t st)
 {
     long long n = (long long)P.L1 * P.L2 * P.L3 * P.na;
@@ -463,7 +451,7 @@ template <typename T>
 void launch_ewald_fwdA(const EwaldGeom& G, const double* A, void* M, hipStream_t st)
 {
     const size_t lds = ewald_fft_lds_bytes(G, sizeof(double));
-    ewald_allow_lds(reinterpret_cast<const void*>(&k_ewald_fwdA<T>), lds);
+    allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_fwdA<T>), lds);
     hipLaunchKernelGGL(k_ewald_fwdA<T>, dim3(9 * G.na * G.na), dim3(kFftThreads), lds, st, G, A,
                        reinterpret_cast<C2<T>*>(M));
 }
@@ -473,8 +461,8 @@ void launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint
                         const T* S, const void* M, void* Fmu, void* Fphi, T* phi, hipStream_t st)
 {
     const size_t lds = ewald_fft_lds_bytes(G, sizeof(T));
-    ewald_allow_lds(reinterpret_cast<const void*>(&k_ewald_fwd<T>), lds);
-    ewald_allow_lds(reinterpret_cast<const void*>(&k_ewald_inv<T>), lds);
+    allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_fwd<T>), lds);
+    allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_inv<T>), lds);
     hipLaunchKernelGGL(k_ewald_fwd<T>, dim3(3 * G.na, R), dim3(kFftThreads), lds, st, G, R, reps, total, gmat, S,
                        reinterpret_cast<C2<T>*>(Fmu));
     if constexpr (sizeof(T) == sizeof(float)) {
@@ -494,7 +482,7 @@ void launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint
             const int nrt = (3 * G.na + 15) / 16;
 #define SNH_EWM(NRT, NC)                                                                               \
     do {                                                                                               \
-        ewald_allow_lds(reinterpret_cast<const void*>(&k_ewald_contract_mfma<NRT, NC>), lm);           \
+        allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_contract_mfma<NRT, NC>), lm);           \
         hipLaunchKernelGGL((k_ewald_contract_mfma<NRT, NC>), grid, dim3(256 * NRT), lm, st, G, R, m, x, y); \
     } while (0)
 #define SNH_EWM_NC(NRT) \
@@ -516,7 +504,7 @@ void launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint
         }
     }
     const size_t lds2 = (size_t)R * 3 * G.na * sizeof(C2<T>);
-    ewald_allow_lds(reinterpret_cast<const void*>(&k_ewald_contract<T>), lds2);
+    allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_contract<T>), lds2);
     hipLaunchKernelGGL(k_ewald_contract<T>, dim3((unsigned)G.nk), dim3(256), lds2, st, G, R,
                        reinterpret_cast<const C2<T>*>(M), reinterpret_cast<const C2<T>*>(Fmu),
                        reinterpret_cast<C2<T>*>(Fphi));

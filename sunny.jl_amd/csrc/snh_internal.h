@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "../../include/sunny_hip.h"
 #include "snh_device.h"
 
 namespace snh {
@@ -152,6 +153,20 @@ template <typename T>
 void launch_dipole_energy(const Couplings<T>& C, int nblocks, const T* S, const T* phi, double* partial,
                           hipStream_t st);
 void launch_reduce_partials(const double* partial, const int* rblk, int R, double* out, hipStream_t st);
+
+// Raises a kernel's dynamic-LDS limit to `bytes` (HIP keeps the attribute per
+// function), once per kernel and size.  The kernels seen so far sit in a
+// fixed table, so a launch never allocates host memory (the entry point that
+// allocates nothing launches through here too); past the table's capacity
+// the attribute is simply set on every launch.
+void allow_dynamic_lds(const void* fn, size_t bytes);
+
+// Static device scratch of snh_energy_noalloc (snh_noalloc.hip): capacity in
+// doubles; noalloc_scratch_init resolves the current device's copy (once per
+// device); noalloc_scratch returns it (nullptr before init).
+constexpr size_t kNoAllocPartials = SNH_NOALLOC_PARTIALS;
+hipError_t noalloc_scratch_init();
+double* noalloc_scratch(int dev);
 void launch_midpoint_check(const double* partial, const int* rblk, int R, const double* atol2, int* conv_iter,
                            int iter, hipStream_t st);
 void launch_fill_int(int* p, int n, int v, hipStream_t st);

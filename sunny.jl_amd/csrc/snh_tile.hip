@@ -582,19 +582,6 @@ __global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_ti
 
 size_t tile_lds_bytes(const TileArgs& A, size_t elem) { return (size_t)A.nslot * A.cap * 3 * elem; }
 
-// Raises a kernel's dynamic-LDS limit to what a plan needs (once per kernel
-// and size; HIP keeps the attribute per function).
-static void allow_dynamic_lds(const void* fn, size_t bytes)
-{
-    static std::mutex mu;
-    static std::map<const void*, size_t> done;
-    std::lock_guard<std::mutex> lock(mu);
-    size_t& have = done[fn];
-    if (have >= bytes) return;
-    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    have = bytes;
-}
-
 template <typename K>
 static void launch_tile(K kernel, const TileArgs&, int, size_t lds, hipStream_t)
 {

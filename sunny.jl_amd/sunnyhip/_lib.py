@@ -26,7 +26,7 @@ EXPORTS = [
     "snh_step_langevin", "snh_step_midpoint", "snh_energy", "snh_energy_grad", "snh_synchronize",
     "snh_enable_timing", "snh_kernel_time", "snh_copy_calibration", "snh_query",
     "snh_midpoint_record", "snh_midpoint_adjoint", "snh_autotune", "snh_corr_init", "snh_corr_add_sample",
-    "snh_corr_data", "snh_step_local", "snh_local_coloring",
+    "snh_corr_data", "snh_step_local", "snh_local_coloring", "snh_energy_noalloc",
 ]
 
 
@@ -73,6 +73,7 @@ def lib():
         for f in ("snh_set_dipoles", "snh_get_dipoles", "snh_set_coherents", "snh_get_coherents",
                   "snh_energy", "snh_energy_grad"):
             getattr(L, f).argtypes = [vp, dp]
+        L.snh_energy_noalloc.argtypes = [vp, vp]  # device pointer
         L.snh_step_langevin.argtypes = [vp, C.c_double, C.c_double, dp, dp, C.c_uint64, C.c_uint64, C.c_int32]
         L.snh_step_midpoint.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
                                         C.POINTER(C.c_int32)]
