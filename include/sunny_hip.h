@@ -68,6 +68,15 @@ typedef struct {
     int32_t nreplicas;          /* R >= 1 */
     const int32_t *dims;        /* [R][3] supercell dims per replica (may differ without Ewald) */
     int32_t layout;             /* SNH_LAYOUT_AUTO or SNH_LAYOUT_GHOST (below) */
+    /* :SUN only: the general part of set_pair_coupling! (the TensorDecomposition
+     * of src/System/Types.jl:23-27, PairExchange.jl:85-111): directed bond b
+     * carries terms gen_start[b] .. gen_start[b+1]-1, each a pair (A, B) of
+     * N x N Hermitian matrices, row-major (re, im), with bond energy
+     * sum_t Re<Zi|A_t|Zi> Re<Zj|B_t|Zj> (Interactions.jl:496-503); the
+     * reverse bond holds the same terms as (B, A).  gen_start NULL: none. */
+    const int32_t *gen_start;   /* [nbonds + 1] */
+    const double *gen_A;        /* [nterms][N][N][2] */
+    const double *gen_B;        /* [nterms][N][N][2] */
 } snh_spec;
 
 /* Device layouts of the :dipole gather.  AUTO picks the LDS-tiled sweep

@@ -23,7 +23,8 @@ _ip = C.POINTER(C.c_int32)
 class SoSpec(C.Structure):
     _fields_ = [("L", C.c_int * 3), ("na", C.c_int), ("N", C.c_int), ("kappa", _dp), ("g", _dp), ("field", _dp),
                 ("stevens", _dp), ("kmax", _ip), ("Lambda", _dp), ("nb", C.c_int), ("bond", _ip), ("bilin", _dp),
-                ("biquad", _dp), ("has_biquad", _ip), ("scalar", _dp), ("ewald_A", _dp)]
+                ("biquad", _dp), ("has_biquad", _ip), ("scalar", _dp), ("ewald_A", _dp), ("gen_start", _ip),
+                ("gen_A", _dp), ("gen_B", _dp)]
 
 
 _lib = None
@@ -142,6 +143,7 @@ class Oracle:
         s.bond, s.bilin, s.biquad = _i(spec["bond"]), _d(spec["bilin"]), _d(spec["biquad"])
         s.has_biquad, s.scalar = _i(spec["has_biquad"]), _d(spec["scalar"])
         s.ewald_A = _d(self.A)
+        s.gen_start, s.gen_A, s.gen_B = _i(spec.get("gen_start")), _d(spec.get("gen_A")), _d(spec.get("gen_B"))
         self.s = s
 
     def _p(self):

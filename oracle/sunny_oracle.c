@@ -286,6 +286,33 @@ static void expected_quadrupole1(const sun_ops *o, const cplx *Z, double Q[5])
     }
 }
 
+/* Re<Z|M|Z> of an N x N complex matrix M ([N][N][2]) (dot(Z, M, Z)) */
+static double expect_op(int N, const double *M, const cplx *Z)
+{
+    cplx acc = 0;
+    for (int a = 0; a < N; a++)
+        for (int b = 0; b < N; b++) acc += conj(Z[a]) * (M[2 * (a * N + b)] + I * M[2 * (a * N + b) + 1]) * Z[b];
+    return creal(acc);
+}
+
+/* out += f (M Z) */
+static void add_op_ket(int N, const double *M, const cplx *Z, double f, cplx *out)
+{
+    for (int a = 0; a < N; a++) {
+        cplx acc = 0;
+        for (int b = 0; b < N; b++) acc += (M[2 * (a * N + b)] + I * M[2 * (a * N + b) + 1]) * Z[b];
+        out[a] += f * acc;
+    }
+}
+
+static inline int gen_terms(const so_spec *sp, int b, int *t0, int *t1)
+{
+    if (!sp->gen_start || sp->N == 0) return 0;
+    *t0 = sp->gen_start[b];
+    *t1 = sp->gen_start[b + 1];
+    return *t1 > *t0;
+}
+
 void so_expected_spin(int N, long ns, const double *Zr, double *S)
 {
     sun_ops o;
@@ -690,7 +717,23 @@ void so_grad_coherents(const so_spec *sp, const double *Zr, double *HZr)
         }
         for (int b = 0; b < sp->nb; b++) {
             const int *bd = sp->bond + 6 * b;
-            if (bd[0] != i || bd[5] || !sp->has_biquad[b]) continue;
+            int t0, t1;
+            if (bd[0] != i || bd[5]) continue;
+            /* general couplings, Interactions.jl:659-666: HZi += (A Zi) <B>j, HZj += <A>i (B Zj) */
+            if (gen_terms(sp, b, &t0, &t1))
+                for (long c = 0; c < nc; c++) {
+                    const int c1 = (int)(c % L[0]), c2 = (int)((c / L[0]) % L[1]), c3 = (int)(c / ((long)L[0] * L[1]));
+                    long si = site_index(L, c1, c2, c3, i);
+                    long sj = site_index(L, wrap(c1 + bd[2], L[0]), wrap(c2 + bd[3], L[1]), wrap(c3 + bd[4], L[2]),
+                                         bd[1]);
+                    for (int t = t0; t < t1; t++) {
+                        const double *A = sp->gen_A + (long)2 * N * N * t, *Bm = sp->gen_B + (long)2 * N * N * t;
+                        const double Abar = expect_op(N, A, Z + N * si), Bbar = expect_op(N, Bm, Z + N * sj);
+                        add_op_ket(N, A, Z + N * si, Bbar, HZ + N * si);
+                        add_op_ket(N, Bm, Z + N * sj, Abar, HZ + N * sj);
+                    }
+                }
+            if (!sp->has_biquad[b]) continue;
             const double *Bq = sp->biquad + 25 * b;
             for (int c3 = 0; c3 < L[2]; c3++)
                 for (int c2 = 0; c2 < L[1]; c2++)
@@ -795,6 +838,12 @@ double so_energy(const so_spec *sp, const double *S, const double *Zr)
                             for (int p = 0; p < 5; p++)
                                 for (int q = 0; q < 5; q++) Ei += Qi[p] * Bq[5 * p + q] * Qj[q];
                         }
+                        /* general couplings, Interactions.jl:496-503 */
+                        int t0, t1;
+                        if (gen_terms(sp, b, &t0, &t1))
+                            for (int t = t0; t < t1; t++)
+                                Ei += expect_op(N, sp->gen_A + (long)2 * N * N * t, Z + N * si) *
+                                      expect_op(N, sp->gen_B + (long)2 * N * N * t, Z + N * sj);
                     }
         }
         E += Ei;
@@ -1412,6 +1461,14 @@ static double local_delta_sun(const so_spec *sp, const sun_ops *o, const cplx *Z
             for (int p = 0; p < 5; p++)
                 for (int q = 0; q < 5; q++) dE += (Qn[p] - Qo[p]) * Bq[5 * p + q] * Qj[q];
         }
+        /* general couplings, Interactions.jl:371-377 */
+        int t0, t1;
+        if (gen_terms(sp, b, &t0, &t1))
+            for (int t = t0; t < t1; t++) {
+                const double *A = sp->gen_A + (long)2 * N * N * t;
+                dE += (expect_op(N, A, Zn) - expect_op(N, A, Z0)) *
+                      expect_op(N, sp->gen_B + (long)2 * N * N * t, Z + (long)N * sj);
+            }
     }
     if (sp->ewald_A) dE += ewald_delta(sp, S, site, dS);
     return dE;

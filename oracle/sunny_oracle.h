@@ -46,6 +46,11 @@ typedef struct {
     const int *has_biquad;    /* [nb] */
     const double *scalar;     /* [nb] constant shift */
     const double *ewald_A;    /* NULL or [na(j)][na(i)][L3][L2][L1][3][3] (A[off,i,j]) */
+    /* general pair couplings (SU(N)): bond b has terms gen_start[b]..gen_start[b+1]-1,
+       each (A, B) N x N complex [N][N][2] row-major; NULL gen_start: none */
+    const int *gen_start;     /* [nb+1] */
+    const double *gen_A;      /* [nterms][N][N][2] */
+    const double *gen_B;      /* [nterms][N][N][2] */
 } so_spec;
 
 /* Stevens classical anisotropy, OnsiteCoupling.jl:316-425. c = [c0|c2|c4|c6]. */

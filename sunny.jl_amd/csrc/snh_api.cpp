@@ -684,6 +684,14 @@ struct SystemImpl final : snh_system {
         std::vector<int> bsrc(std::max(nbp, 1), -1);  // original bond index, -1 for padding
         std::vector<T> bJ(9 * (size_t)std::max(nbp, 1), (T)0), bJ0(std::max(nbp, 1), (T)0),
             bQ(25 * (size_t)std::max(nbp, 1), (T)0), bsc(std::max(nbp, 1), (T)0);
+        // general (A, B) terms of every padded slot, in slot order
+        const bool have_gen = sp.gen_start != nullptr && N > 0;
+        SNH_REQUIRE(N > 0 || !sp.gen_start || sp.gen_start[sp.nbonds] == sp.gen_start[0], SNH_ERR_INVALID,
+                    "general pair couplings require :SUN mode (in dipole mode the most general coupling is "
+                    "bilinear plus biquadratic)");
+        SNH_REQUIRE(!have_gen || (sp.gen_A && sp.gen_B), SNH_ERR_INVALID, "general coupling terms missing");
+        std::vector<int> gstart(std::max(nbp, 1) + 1, 0);
+        std::vector<T> gA, gB;
         {
             std::vector<int> fill(na, 0);
             for (int k = 0; k < nb; k++) {
@@ -701,7 +709,24 @@ struct SystemImpl final : snh_system {
                 bJ0[slot] = (T)J[0];
                 for (int p = 0; p < 25; p++) bQ[25 * (size_t)slot + p] = bq ? (T)sp.biquad[25 * b + p] : (T)0;
                 bsc[slot] = sp.scalar ? (T)sp.scalar[b] : (T)0;
+                if (have_gen) {
+                    SNH_REQUIRE(sp.gen_start[b] >= 0 && sp.gen_start[b] <= sp.gen_start[b + 1], SNH_ERR_INVALID,
+                                "gen_start must be non-decreasing");
+                    if (sp.gen_start[b + 1] > sp.gen_start[b]) bflag[slot] |= kBondGeneral;
+                }
             }
+            if (have_gen)
+                for (int slot = 0; slot < nbp; slot++) {
+                    gstart[slot] = (int)(gA.size() / (2 * (size_t)N * N));
+                    const int b = bsrc[slot];
+                    if (b < 0) continue;
+                    for (int t = sp.gen_start[b]; t < sp.gen_start[b + 1]; t++)
+                        for (size_t q = 0; q < 2 * (size_t)N * N; q++) {
+                            gA.push_back((T)sp.gen_A[2 * (size_t)N * N * t + q]);
+                            gB.push_back((T)sp.gen_B[2 * (size_t)N * N * t + q]);
+                        }
+                }
+            if (have_gen) gstart[nbp] = (int)(gA.size() / (2 * (size_t)N * N));
             for (int i = 0; i < na; i++)
                 for (int slot = bstart[i] + count[i]; slot < bstart[i + 1]; slot++) {
                     bj[slot] = i;
@@ -712,10 +737,12 @@ struct SystemImpl final : snh_system {
         nb_in = sp.nbonds;
         bond_h.assign(sp.bond, sp.bond + 6 * (size_t)sp.nbonds);
         for (int b = 0; b < sp.nbonds; b++) any_biquad |= sp.has_biquad && sp.has_biquad[b];
+        // general couplings, like biquadratic ones, need the neighbours' kets
+        for (int k = 0; k < nbp; k++) any_biquad |= (bflag[k] & kBondGeneral) != 0;
         nb = nbp;
         C.iso = 1;
         for (int k = 0; k < nbp; k++)
-            if (!(bflag[k] & kBondIso) || (bflag[k] & kBondBiquad)) C.iso = 0;
+            if (!(bflag[k] & kBondIso) || (bflag[k] & (kBondBiquad | kBondGeneral))) C.iso = 0;
         // padded-index offset of every bond, per dims class
         std::vector<int> bdelta(std::max<size_t>(1, dcls_dims.size() * nbp), 0);
         for (size_t d = 0; d < dcls_dims.size(); d++) {
@@ -778,6 +805,10 @@ struct SystemImpl final : snh_system {
         C.bstart = own(dupload(bstart));
         C.bj = own(dupload(bj));
         C.bflag = own(dupload(bflag));
+        if (gA.empty()) gA.assign(2, (T)0), gB.assign(2, (T)0);
+        C.gstart = own(dupload(gstart));
+        C.gA = own(dupload(gA));
+        C.gB = own(dupload(gB));
         C.bdelta = own(dupload(bdelta));
         C.bJ0 = own(dupload(bJ0));
         C.bJ = own(dupload(bJ));

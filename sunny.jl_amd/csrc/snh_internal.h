@@ -30,6 +30,9 @@ struct Couplings {
     const T* field;      // [na][3]
     const T* lam;        // [na][N][N][2] SU(N) onsite matrix
     const T* quad_ops;   // [5][N][N][2] Stevens quadrupoles O[2,q]
+    const int* gstart;   // [nb+1] general (A, B) terms of each padded bond slot (:SUN)
+    const T* gA;         // [nterms][N][N][2]
+    const T* gB;         // [nterms][N][N][2]
     int na, nb, N;
     int chunk;           // neighbours gathered per batch of loads (6, 9 or 18); every
                          // sublattice's bond list is padded to a multiple of it
@@ -61,7 +64,7 @@ __device__ __forceinline__ unsigned long long noise_step(const NoiseArgs<T>& nz)
     return nz.step_ptr ? ldc(nz.step_ptr, 0) : nz.step;
 }
 
-enum { kBondIso = 1, kBondBiquad = 2, kBondCulled = 4 };
+enum { kBondIso = 1, kBondBiquad = 2, kBondCulled = 4, kBondGeneral = 8 };
 
 // Largest tiled workgroup for SP sites per thread: 1024 threads (<= 128
 // VGPRs) for FP32 with one site per thread, else 512.

@@ -113,6 +113,44 @@ __device__ __forceinline__ void expected_quadrupole(const Ket<T, N>& Z, const T*
     }
 }
 
+// Re<Z|M|Z> for an N x N complex matrix M ([N][N][2], row-major, a read-only
+// table indexed uniformly over the wavefront).
+template <typename T, int N>
+__device__ __forceinline__ T expect_op(const T* M, const Ket<T, N>& Z)
+{
+    T acc = 0;
+#pragma unroll
+    for (int a = 0; a < N; a++) {
+        T pr = 0, pi = 0;
+#pragma unroll
+        for (int b = 0; b < N; b++) {
+            const T mr = ldc(M, 2 * (a * N + b)), mi = ldc(M, 2 * (a * N + b) + 1);
+            pr += mr * Z.z[b].re - mi * Z.z[b].im;
+            pi += mr * Z.z[b].im + mi * Z.z[b].re;
+        }
+        acc += Z.z[a].re * pr + Z.z[a].im * pi;
+    }
+    return acc;
+}
+
+// out += f (M Z)
+template <typename T, int N>
+__device__ __forceinline__ void add_op_ket(const T* M, const Ket<T, N>& Z, T f, Ket<T, N>& out)
+{
+#pragma unroll
+    for (int a = 0; a < N; a++) {
+        T pr = 0, pi = 0;
+#pragma unroll
+        for (int b = 0; b < N; b++) {
+            const T mr = ldc(M, 2 * (a * N + b)), mi = ldc(M, 2 * (a * N + b) + 1);
+            pr += mr * Z.z[b].re - mi * Z.z[b].im;
+            pi += mr * Z.z[b].im + mi * Z.z[b].re;
+        }
+        out.z[a].re += f * pr;
+        out.z[a].im += f * pi;
+    }
+}
+
 // Source of kets for the gather: stored planes, or Zbar = (Z + Z1)/2.
 template <typename T, int N, bool kMid>
 struct KetSrc {
@@ -189,6 +227,10 @@ __device__ __forceinline__ Ket<T, N> local_hz(const Couplings<T>& C, const SiteC
     T gx = ldc(C.zee, 3 * i), gy = ldc(C.zee, 3 * i + 1), gz = ldc(C.zee, 3 * i + 2);
     T wq[5] = {0, 0, 0, 0, 0};
     bool anyq = false;
+    // general couplings: sum over terms of <Zj|B|Zj> A Z (Interactions.jl:659-666)
+    Ket<T, N> hg;
+#pragma unroll
+    for (int a = 0; a < N; a++) hg.z[a] = {0, 0};
     const int* __restrict__ dl = C.bdelta + sc.R.dcls * C.nb;
     const int b0 = ldc(C.bstart, i), b1 = ldc(C.bstart, i + 1);
     for (int c0 = b0; c0 < b1; c0 += kSunChunk) {
@@ -219,6 +261,12 @@ __device__ __forceinline__ Ket<T, N> local_hz(const Couplings<T>& C, const SiteC
                 for (int p = 0; p < 5; p++)
                     for (int q = 0; q < 5; q++) wq[p] += Bq[5 * p + q] * Qj[q];
                 anyq = true;
+            }
+            if constexpr (Src::kKets) if (fl & kBondGeneral) {
+                for (int t = ldc(C.gstart, b); t < ldc(C.gstart, b + 1); t++) {
+                    const T Bbar = expect_op<T, N>(C.gB + 2 * N * N * t, nbr[k]);
+                    add_op_ket<T, N>(C.gA + 2 * N * N * t, Z, Bbar, hg);
+                }
             }
         }
     }
@@ -270,6 +318,11 @@ __device__ __forceinline__ Ket<T, N> local_hz(const Couplings<T>& C, const SiteC
                 H.z[a].re += v.re;
                 H.z[a].im += v.im;
             }
+    }
+#pragma unroll
+    for (int a = 0; a < N; a++) {
+        H.z[a].re += hg.z[a].re;
+        H.z[a].im += hg.z[a].im;
     }
     return H;
 }
@@ -585,6 +638,10 @@ __global__ __launch_bounds__(kBlock) void k_sun_energy(Couplings<T> C, const T* 
                 for (int p = 0; p < 5; p++)
                     for (int q = 0; q < 5; q++) v += (double)Qi[p] * (double)Bq[5 * p + q] * (double)Qj[q];
             }
+            if (fl & kBondGeneral)  // Interactions.jl:496-503
+                for (int t = C.gstart[b]; t < C.gstart[b + 1]; t++)
+                    v += (double)expect_op<T, N>(C.gA + 2 * N * N * t, z) *
+                         (double)expect_op<T, N>(C.gB + 2 * N * N * t, zj);
             e += v;
         }
         if (phi) {  // 1/2 mu.phi with mu = -g <S>
@@ -721,6 +778,12 @@ __global__ __launch_bounds__(kBlock) void k_sun_mc_color(Couplings<T> C, McArgs 
                         dE += ((double)Qn[p] - (double)Qo[p]) * w;
                     }
                 }
+                if (ldc(C.bflag, b) & kBondGeneral)  // Interactions.jl:371-377
+                    for (int t = ldc(C.gstart, b); t < ldc(C.gstart, b + 1); t++) {
+                        const T* A_ = C.gA + 2 * N * N * t;
+                        const double dA = (double)expect_op<T, N>(A_, zn) - (double)expect_op<T, N>(A_, z0);
+                        dE += dA * (double)expect_op<T, N>(C.gB + 2 * N * N * t, zj[kk]);
+                    }
             }
         }
         const double kT = ldc(A.kT, W.r);

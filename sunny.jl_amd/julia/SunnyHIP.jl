@@ -24,6 +24,7 @@ struct SnhSpec
     ewald::Int32; mu0_muB2::Float64; demag::NTuple{9,Float64}; latvecs::NTuple{9,Float64}
     positions::Ptr{Float64}; nreplicas::Int32; dims::Ptr{Int32}
     layout::Int32   # 0 = SNH_LAYOUT_AUTO
+    gen_start::Ptr{Int32}; gen_A::Ptr{Float64}; gen_B::Ptr{Float64}   # SU(N) general couplings
 end
 
 mutable struct DeviceSystem{N}
@@ -42,6 +43,8 @@ function coupling_tables(sys::System{N}) where N
     ints = sys.interactions_union
     na = length(ints)
     bonds = Int32[]; bilin = Float64[]; biquad = Float64[]; hasbq = Int32[]; scalar = Float64[]
+    gen_start = Int32[0]; gen_A = Float64[]; gen_B = Float64[]
+    rowmajor_c(M) = vec(permutedims(cat(real(permutedims(M)), imag(permutedims(M)); dims=3), (3, 1, 2)))
     for i in 1:na, pc in ints[i].pair
         (; bond, isculled) = pc
         append!(bonds, Int32[bond.i - 1, bond.j - 1, bond.n..., isculled])
@@ -51,6 +54,10 @@ function coupling_tables(sys::System{N}) where N
         append!(biquad, vec(permutedims(Matrix(B))))
         push!(hasbq, iszero(pc.biquad) ? 0 : 1)
         push!(scalar, pc.scalar)
+        for (A, B) in pc.general.data   # TensorDecomposition terms (Types.jl:23-27)
+            append!(gen_A, rowmajor_c(Matrix(A))); append!(gen_B, rowmajor_c(Matrix(B)))
+        end
+        push!(gen_start, gen_start[end] + length(pc.general.data))
     end
     kappa = Float64[sys.κs[1, 1, 1, i] for i in 1:na]
     g = reduce(vcat, [vec(permutedims(sys.gs[1, 1, 1, i])) for i in 1:na])
@@ -63,12 +70,13 @@ function coupling_tables(sys::System{N}) where N
             stevens[28(i-1)+1:28i] = vcat(os.c0, os.c2, os.c4, os.c6); kmax[i] = os.kmax
         else
             M = Matrix(os)
-            Lambda[2N^2*(i-1)+1:2N^2*i] = vec(permutedims(cat(real(permutedims(M)), imag(permutedims(M)); dims=3), (3, 1, 2)))
+            Lambda[2N^2*(i-1)+1:2N^2*i] = rowmajor_c(M)
         end
     end
     dims = Int32[sys.dims...]
     pos = reduce(vcat, sys.crystal.positions)
-    return (; na, bonds, bilin, biquad, hasbq, scalar, kappa, g, field, stevens, kmax, Lambda, dims, pos)
+    return (; na, bonds, bilin, biquad, hasbq, scalar, kappa, g, field, stevens, kmax, Lambda, dims, pos,
+            gen_start, gen_A, gen_B)
 end
 
 function to_device(sys::System{N}; precision=64, seed=rand(UInt64), autotune=false) where N
@@ -79,7 +87,8 @@ function to_device(sys::System{N}; precision=64, seed=rand(UInt64), autotune=fal
                        pointer(t.kmax), pointer(t.Lambda), length(t.scalar), pointer(t.bonds), pointer(t.bilin),
                        pointer(t.biquad), pointer(t.hasbq), pointer(t.scalar), isnothing(ew) ? 0 : 1,
                        isnothing(ew) ? 0.0 : ew.μ0_μB², isnothing(ew) ? ntuple(_ -> 0.0, 9) : Tuple(vec(permutedims(ew.demag))),
-                       Tuple(vec(permutedims(sys.crystal.latvecs))), pointer(t.pos), 1, pointer(t.dims), 0)
+                       Tuple(vec(permutedims(sys.crystal.latvecs))), pointer(t.pos), 1, pointer(t.dims), 0,
+                       isempty(t.gen_A) ? C_NULL : pointer(t.gen_start), pointer(t.gen_A), pointer(t.gen_B))
         h = Ref{Ptr{Cvoid}}(C_NULL)
         check(ccall((:snh_create, lib), Cint, (Ref{SnhSpec}, Ref{Ptr{Cvoid}}), spec, h), C_NULL)
     end
@@ -166,7 +175,7 @@ function ChainRulesCore.rrule(::typeof(trajectory), d::DeviceSystem{0}, integrat
             SnhSpec(t.na, 0, 64, pointer(t.kappa), pointer(t.g), pointer(t.field), pointer(t.stevens), pointer(t.kmax),
                     pointer(t.Lambda), length(t.scalar), pointer(t.bonds), pointer(t.bilin), pointer(t.biquad), pointer(t.hasbq),
                     pointer(t.scalar), 0, 0.0, ntuple(_ -> 0.0, 9), ntuple(_ -> 0.0, 9), pointer(t.pos), 1,
-                    pointer(t.dims), 0)
+                    pointer(t.dims), 0, C_NULL, C_NULL, C_NULL)
         end
         grad = zeros(length(labels))
         dS0 = zeros(length(dLdS))

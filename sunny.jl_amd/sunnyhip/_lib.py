@@ -39,7 +39,8 @@ class SnhSpec(C.Structure):
         ("biquad", C.POINTER(C.c_double)), ("has_biquad", C.POINTER(C.c_int32)), ("scalar", C.POINTER(C.c_double)),
         ("ewald", C.c_int32), ("mu0_muB2", C.c_double), ("demag", C.c_double * 9), ("latvecs", C.c_double * 9),
         ("positions", C.POINTER(C.c_double)), ("nreplicas", C.c_int32), ("dims", C.POINTER(C.c_int32)),
-        ("layout", C.c_int32),
+        ("layout", C.c_int32), ("gen_start", C.POINTER(C.c_int32)), ("gen_A", C.POINTER(C.c_double)),
+        ("gen_B", C.POINTER(C.c_double)),
     ]
 
 
@@ -141,6 +142,8 @@ def pack_spec(spec: dict) -> SnhSpec:
     s.nreplicas = spec["nreplicas"]
     s.dims = iptr(spec["dims"])
     s.layout = spec.get("layout", 0)
+    s.gen_start = iptr(spec.get("gen_start"))
+    s.gen_A, s.gen_B = dptr(spec.get("gen_A")), dptr(spec.get("gen_B"))
     return s
 
 

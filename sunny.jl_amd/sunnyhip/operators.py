@@ -133,3 +133,90 @@ class StevensExpansion:
 
     def packed(self) -> np.ndarray:
         return np.concatenate([self.c0, self.c2, self.c4, self.c6])
+
+
+def hermitian_basis(N: int):
+    """An orthonormal basis of the N x N Hermitian matrices under the real
+    inner product tr(A B): the diagonal units, then (E_ab + E_ba)/sqrt 2 and
+    i(E_ba - E_ab)/sqrt 2 for a < b."""
+    out = []
+    for a in range(N):
+        E = np.zeros((N, N), complex)
+        E[a, a] = 1
+        out.append(E)
+    r = 1 / np.sqrt(2)
+    for a in range(N):
+        for b in range(a + 1, N):
+            E = np.zeros((N, N), complex)
+            E[a, b] = E[b, a] = r
+            out.append(E)
+            E = np.zeros((N, N), complex)
+            E[a, b], E[b, a] = -1j * r, 1j * r
+            out.append(E)
+    return out
+
+
+def tensor_expansion(op, N1: int, N2: int, tol=1e-12):
+    """op = sum_k A_k (x) B_k with Hermitian A_k (N1 x N1) and B_k (N2 x N2):
+    the coefficients of a Hermitian op in the product of Hermitian bases are
+    real, and their singular value decomposition gives the terms (the
+    construction of svd_tensor_expansion, src/Operators/TensorOperators.jl:77-113;
+    each singular value is carried by A_k)."""
+    op = np.asarray(op, complex)
+    if np.max(np.abs(op - op.conj().T), initial=0) > 1e-12:
+        raise ValueError("Detected non-Hermitian operator")
+    H1, H2 = hermitian_basis(N1), hermitian_basis(N2)
+    # tr((H1_m (x) H2_n) op) = sum_{a,b,i,j} H1_m[b,a] H2_n[j,i] op[(a,i),(b,j)]
+    T4 = op.reshape(N1, N2, N1, N2)
+    Cm = np.einsum("mba,nji,aibj->mn", np.array(H1), np.array(H2), T4)
+    if np.max(np.abs(Cm.imag), initial=0) > 1e-10:
+        raise ValueError("expansion coefficients of a Hermitian operator must be real")
+    U, S, Vt = np.linalg.svd(Cm.real)
+    terms = []
+    for k, sig in enumerate(S):
+        if sig < tol:
+            break
+        A = sig * np.tensordot(U[:, k], np.array(H1), axes=1)
+        B = np.tensordot(Vt[k], np.array(H2), axes=1)
+        terms.append((A, B))
+    return terms
+
+
+def decompose_pair_coupling(op, N1: int, N2: int, extract_parts=True):
+    """Splits a pair operator on C^N1 (x) C^N2 into a scalar, a bilinear 3x3
+    part over the spin operators, a biquadratic 5x5 part over the Stevens
+    quadrupoles O[2,q] (q descending; None when absent or N < 3) and the
+    general remainder as Hermitian tensor terms (decompose_general_coupling,
+    src/System/PairExchange.jl:71-111)."""
+    op = np.asarray(op, complex).copy()
+    dim = N1 * N2
+    if op.shape != (dim, dim):
+        raise ValueError("pair operator has the wrong size")
+    scalar = float(np.real(np.trace(op)) / dim)
+    op = op - scalar * np.eye(dim)
+    bilin = np.zeros((3, 3))
+    biquad = None
+    if extract_parts:
+        S1, S2 = spin_matrices((N1 - 1) / 2), spin_matrices((N2 - 1) / 2)
+        for a in range(3):
+            for b in range(3):
+                v = np.kron(S1[a], S2[b])
+                J = np.trace(v.conj().T @ op) / np.trace(v.conj().T @ v)
+                if abs(J.imag) > 1e-12:
+                    raise ValueError("non-Hermitian bilinear part")
+                bilin[a, b] = J.real
+                op = op - v * J.real
+        if N1 > 2 and N2 > 2:
+            O1, O2 = stevens_matrices((N1 - 1) / 2)[2], stevens_matrices((N2 - 1) / 2)[2]
+            bq = np.zeros((5, 5))
+            for a in range(5):
+                for b in range(5):
+                    v = np.kron(O1[a], O2[b])
+                    J = np.trace(v.conj().T @ op) / np.trace(v.conj().T @ v)
+                    if abs(J.imag) > 1e-12:
+                        raise ValueError("non-Hermitian biquadratic part")
+                    bq[a, b] = J.real
+                    op = op - v * J.real
+            if np.max(np.abs(bq)) > 1e-12:
+                biquad = bq
+    return scalar, bilin, biquad, tensor_expansion(op, N1, N2)

@@ -16,7 +16,8 @@ import numpy as np
 
 from . import _lib
 from .crystal import Bond, Crystal, bond_parity
-from .operators import (SCALAR_BIQUAD_METRIC, StevensExpansion, matrix_to_stevens_coefficients, rcs_factors,
+from .operators import (SCALAR_BIQUAD_METRIC, StevensExpansion, decompose_pair_coupling,
+                        matrix_to_stevens_coefficients, rcs_factors,
                         spin_matrices)
 
 VACUUM_PERMEABILITY = 0.6745817653324668  # mu0 muB^2 in meV*A^3, src/Units.jl:91
@@ -69,7 +70,9 @@ class System:
         self.extfield = np.zeros((na, 3))
         self.onsite = [StevensExpansion.zero() if self.N == 0 else np.zeros((self.N, self.N), complex)
                        for _ in range(na)]
-        self.pairs = {}  # Bond -> [scalar, bilin 3x3, biquad 5x5 or None], accumulated from params
+        # Bond -> [scalar, bilin 3x3, biquad 5x5 or None, general [(A, B), ...] or None],
+        # accumulated from params
+        self.pairs = {}
         # label -> ModelParam: couplings are linear in the parameter values
         # (src/System/Types.jl:76-86); unlabeled couplings get "Unnamed" labels
         self.params = {}
@@ -171,13 +174,25 @@ class System:
         biquad = np.zeros((max(nb, 1), 25))
         has_bq = np.zeros(max(nb, 1), np.int32)
         scalar = np.zeros(max(nb, 1))
-        for k, (b, (sc, J, B)) in enumerate(bonds):
+        gen_start = np.zeros(max(nb, 1) + 1, np.int32)
+        gen = []
+        for k, (b, (sc, J, B, G)) in enumerate(bonds):
             bond[k] = (b.i, b.j, *b.n, int(bond_parity(b)))
             bilin[k] = np.asarray(J, float).reshape(9)
             if B is not None:
                 biquad[k] = np.asarray(B, float).reshape(25)
                 has_bq[k] = 1
             scalar[k] = sc
+            gen_start[k] = len(gen)
+            gen.extend(G or [])
+        gen_start[nb:] = len(gen)
+        gen_A = gen_B = None
+        if gen:
+            Nm = max(N, 1)
+            gen_A = np.ascontiguousarray(np.stack([np.stack([np.asarray(A).real, np.asarray(A).imag], -1)
+                                                   for A, _ in gen]).reshape(len(gen), Nm, Nm, 2))
+            gen_B = np.ascontiguousarray(np.stack([np.stack([np.asarray(B).real, np.asarray(B).imag], -1)
+                                                   for _, B in gen]).reshape(len(gen), Nm, Nm, 2))
         dims_list = [self.dims] if dims_list is None else dims_list
         ew = self.ewald
         return {
@@ -192,6 +207,7 @@ class System:
             "positions": np.ascontiguousarray(self.crystal.positions, float),
             "nreplicas": len(dims_list), "dims": np.ascontiguousarray(np.array(dims_list, np.int32).reshape(-1, 3)),
             "layout": int(getattr(self, "layout", 0)),
+            "gen_start": gen_start if gen else None, "gen_A": gen_A, "gen_B": gen_B,
         }
 
     # ---- device copy
@@ -220,7 +236,8 @@ def clone_system(sys: System, seed=None) -> System:
     c = System.__new__(System)
     c.__dict__.update({k: v for k, v in sys.__dict__.items() if not k.startswith("_")})
     c.onsite = [o.copy() if isinstance(o, np.ndarray) else o for o in sys.onsite]
-    c.pairs = {b: [sc, np.array(J), None if B is None else np.array(B)] for b, (sc, J, B) in sys.pairs.items()}
+    c.pairs = {b: [sc, np.array(J), None if B is None else np.array(B), None if G is None else list(G)]
+               for b, (sc, J, B, G) in sys.pairs.items()}
     c.params = {k: p.copy() for k, p in sys.params.items()}
     c.extfield = sys.extfield.copy()
     c.kappas = sys.kappas.copy()
@@ -337,7 +354,8 @@ def _to_float_or_mat3(J):
 class ModelParam:
     """A labeled coupling strength `val` and the unit couplings it scales
     (src/System/Types.jl:76-86).  pairs: Bond -> [scalar, J 3x3, B 5x5 or
-    None]; onsites: atom -> StevensExpansion (:dipole) or N x N matrix (:SUN)."""
+    None, general (A, B) terms or None]; onsites: atom -> StevensExpansion
+    (:dipole) or N x N matrix (:SUN)."""
 
     def __init__(self, label, val, pairs=None, onsites=None):
         self.label = label
@@ -347,8 +365,9 @@ class ModelParam:
 
     def copy(self):
         return ModelParam(self.label, self.val,
-                          {b: [sc, np.array(J), None if B is None else np.array(B)]
-                           for b, (sc, J, B) in self.pairs.items()},
+                          {b: [sc, np.array(J), None if B is None else np.array(B),
+                               None if G is None else [(np.array(A), np.array(Bm)) for A, Bm in G]]
+                           for b, (sc, J, B, G) in self.pairs.items()},
                           {i: (o.copy() if isinstance(o, np.ndarray) else o) for i, o in self.onsites.items()})
 
 
@@ -382,12 +401,14 @@ def repopulate_couplings_from_params_(sys: System):
     sys.onsite = [StevensExpansion.zero() if sys.N == 0 else np.zeros((sys.N, sys.N), complex)
                   for _ in range(sys.na)]
     for p in sys.params.values():
-        for b, (sc, J, B) in p.pairs.items():
-            e = sys.pairs.setdefault(b, [0.0, np.zeros((3, 3)), None])
+        for b, (sc, J, B, G) in p.pairs.items():
+            e = sys.pairs.setdefault(b, [0.0, np.zeros((3, 3)), None, None])
             e[0] += p.val * sc
             e[1] = e[1] + p.val * np.asarray(J)
             if B is not None:
                 e[2] = p.val * np.asarray(B) if e[2] is None else e[2] + p.val * np.asarray(B)
+            if G:  # TensorDecomposition: terms concatenate, scaled on the A side
+                e[3] = (e[3] or []) + [(p.val * np.asarray(A), np.asarray(Bm)) for A, Bm in G]
         for i, o in p.onsites.items():
             sys.onsite[i] = sys.onsite[i] + (o.scaled(p.val) if sys.N == 0 else p.val * o)
     sys._invalidate()
@@ -426,9 +447,9 @@ def param_spec(sys: System, label):
     unit.params = {p.label: ModelParam(p.label, 1.0, p.pairs, p.onsites)}
     unit.extfield = np.zeros_like(sys.extfield)
     unit.ewald = None
-    unit.pairs = {b: [0.0, np.zeros((3, 3)), None] for b in sys.pairs}
-    for b, (sc, J, B) in p.pairs.items():
-        unit.pairs[b] = [sc, np.asarray(J, float), None if B is None else np.asarray(B, float)]
+    unit.pairs = {b: [0.0, np.zeros((3, 3)), None, None] for b in sys.pairs}
+    for b, (sc, J, B, G) in p.pairs.items():
+        unit.pairs[b] = [sc, np.asarray(J, float), None if B is None else np.asarray(B, float), G]
     unit.onsite = [StevensExpansion.zero() if sys.N == 0 else np.zeros((sys.N, sys.N), complex)
                    for _ in range(sys.na)]
     for i, o in p.onsites.items():
@@ -468,8 +489,8 @@ def set_exchange_(sys: System, J, bond: Bond, biquad=0.0, propagate=None, param=
             if sys.mode == "dipole":
                 f *= rcs_factors(sys.spins[b.i])[2] * rcs_factors(sys.spins[b.j])[2]
             bq = f * np.diag(SCALAR_BIQUAD_METRIC)
-        pairs[b] = [scalar, Jm, bq]
-        pairs[b.reverse()] = [scalar, Jm.T.copy(), None if bq is None else bq.T.copy()]
+        pairs[b] = [scalar, Jm, bq, None]
+        pairs[b.reverse()] = [scalar, Jm.T.copy(), None if bq is None else bq.T.copy(), None]
     key = ("pair", min(bond, bond.reverse(), key=lambda x: (x.i, x.j, x.n)))
     label, val = _param_label(sys, param, key)
     _replace_param(sys, label, val, pairs=pairs, key=key)
@@ -482,8 +503,55 @@ def set_pair_biquad_(sys: System, B, bond: Bond, param=None):
     B = np.asarray(B, float).reshape(5, 5)
     if sys.mode == "dipole":
         B = B * rcs_factors(sys.spins[bond.i])[2] * rcs_factors(sys.spins[bond.j])[2]
-    pairs = {bond: [0.0, np.zeros((3, 3)), B.copy()], bond.reverse(): [0.0, np.zeros((3, 3)), B.T.copy()]}
+    pairs = {bond: [0.0, np.zeros((3, 3)), B.copy(), None],
+             bond.reverse(): [0.0, np.zeros((3, 3)), B.T.copy(), None]}
     key = ("biquad", min(bond, bond.reverse(), key=lambda x: (x.i, x.j, x.n)))
+    label, val = _param_label(sys, param, key)
+    _replace_param(sys, label, val, pairs=pairs, key=key)
+
+
+def to_product_space(Si, Sj):
+    """Operators of two sites lifted to the pair's product space:
+    A (x) I and I (x) B (to_product_space, src/Operators/TensorOperators.jl)."""
+    Ni, Nj = Si[0].shape[0], Sj[0].shape[0]
+    return ([np.kron(a, np.eye(Nj)) for a in Si], [np.kron(np.eye(Ni), b) for b in Sj])
+
+
+def set_pair_coupling_(sys: System, op, bond: Bond, extract_parts=True, param=None):
+    """General pair coupling on `bond` in every cell (set_pair_coupling!,
+    src/System/PairExchange.jl:274-312): `op` is a Hermitian matrix on the
+    pair's product space C^Ni (x) C^Nj, or a function of the lifted spin
+    matrices (Si, Sj).  It is split into a scalar, bilinear and biquadratic
+    part (extract_parts) and a general remainder sum_k A_k (x) B_k, whose
+    bond energy is sum_k Re<Zi|A_k|Zi> Re<Zj|B_k|Zj>; the reverse bond
+    carries the transposed parts and (B_k, A_k).  The general remainder needs
+    :SUN mode.  No symmetry propagation: call once per bond (with the
+    operator rotated as required)."""
+    if bond.i == bond.j and tuple(bond.n) == (0, 0, 0):
+        raise ValueError("Bond must connect different atoms.")
+    si, sj = sys.spins[bond.i], sys.spins[bond.j]
+    Ni, Nj = int(round(2 * si + 1)), int(round(2 * sj + 1))
+    if callable(op):
+        op = op(*to_product_space(spin_matrices(si), spin_matrices(sj)))
+    op = np.asarray(op, complex)
+    if op.shape != (Ni * Nj, Ni * Nj):
+        raise ValueError("Invalid matrix size.")
+    if not np.allclose(op, op.conj().T, rtol=0, atol=1e-12):
+        raise ValueError("Operator is not Hermitian")
+    if Ni == Nj == 2:
+        raise ValueError("Use set_exchange_ instead. Quantum spin 1/2 allows only bilinear coupling.")
+    scalar, J, B, G = decompose_pair_coupling(op, Ni, Nj, extract_parts=extract_parts)
+    if G and sys.mode == "dipole":
+        raise ValueError("Invalid pair coupling. In dipole mode, the most general allowed form is "
+                         "(Si, Sj) -> Si'*J*Sj + [(Si'*K1*Si)*(Sj'*K2*Sj) + ...] where J is any 3x3 matrix, "
+                         "while K1, K2 must be Hermitian and traceless.")
+    if B is not None and sys.mode == "dipole":
+        B = B * rcs_factors(si)[2] * rcs_factors(sj)[2]
+    G = [(np.asarray(A, complex), np.asarray(Bm, complex)) for A, Bm in G] or None
+    pairs = {bond: [scalar, J.copy(), None if B is None else B.copy(), G],
+             bond.reverse(): [scalar, J.T.copy(), None if B is None else B.T.copy(),
+                              None if G is None else [(Bm.copy(), A.copy()) for A, Bm in G]]}
+    key = ("pair", min(bond, bond.reverse(), key=lambda x: (x.i, x.j, x.n)))
     label, val = _param_label(sys, param, key)
     _replace_param(sys, label, val, pairs=pairs, key=key)
 

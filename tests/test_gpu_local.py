@@ -251,6 +251,8 @@ def test_ising_chains_with_flips():
 
 def sun_models(precision):
     yield "tri_sun3_biquad", sun_biquad_system(seed=5)
+    from test_gpu_sun import general_model
+    yield "su3_general", general_model(precision, seed=42)
     s5 = sh.workloads.triangular_sun_model(L1=3, L2=3, s=2.0, D=0.7, seed=6, precision=precision)
     sh.set_field_(s5, [0.0, 0.3, -0.1])
     sh.randomize_spins_(s5)

@@ -32,10 +32,22 @@ def spin2_model(precision):
     return s
 
 
+def general_model(precision, N=3, seed=40):
+    """A general pair operator (scalar, bilinear, biquadratic and tensor
+    terms) on one bond of an SU(N) cubic lattice, plus a field."""
+    from test_oracle_general import pair_model
+    s, _ = pair_model(N=N, seed=seed, L=(4, 3, 2), precision=precision)
+    sh.set_field_(s, [0.1, -0.2, 0.3])
+    sh.randomize_spins_(s)
+    return s
+
+
 def cases(precision):
     yield "tri6x5", tri_model(precision)
     yield "tri1x3", tri_model(precision, L1=1, L2=3, seed=5)
     yield "su5_diamond", spin2_model(precision)
+    yield "su3_general", general_model(precision)
+    yield "su4_general", general_model(precision, N=4, seed=41)
 
 
 @pytest.mark.parametrize("precision", [64, 32])
