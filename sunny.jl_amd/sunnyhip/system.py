@@ -12,6 +12,8 @@ and `sys.dipoles` is a view indexed `[c1, c2, c3, i, :]` (0-based).
 """
 from __future__ import annotations
 
+import copy
+
 import numpy as np
 
 from . import _lib
@@ -77,6 +79,8 @@ class System:
         # (src/System/Types.jl:76-86); unlabeled couplings get "Unnamed" labels
         self.params = {}
         self.ewald = None
+        # per-site interactions of an inhomogeneous system (to_inhomogeneous)
+        self.inhomog = None
         self.precision = int(precision)
         self.layout = 0  # device gather layout: 0 automatic (LDS tiles when they fit), 1 ghost-padded
         self.rng = np.random.default_rng(seed)
@@ -142,6 +146,13 @@ class System:
             self._dip[:] = np.asarray(dip).reshape(self.nsites, 3)
         self._host_dirty = True
 
+    def site_kappas(self):
+        """kappa of every site in Julia order (per site for an inhomogeneous
+        system, where vacancies have kappa = 0)."""
+        if self.inhomog is not None:
+            return self.inhomog.kappa.copy()
+        return np.repeat(self.kappas, self.ncells)
+
     def site_index(self, site):
         c1, c2, c3, i = site
         L1, L2, L3 = self.dims
@@ -156,6 +167,9 @@ class System:
     # ---- coupling tables (the C-ABI spec)
     def spec(self, precision=None, dims_list=None):
         """Arrays describing the couplings, as consumed by snh_create."""
+        if self.inhomog is not None:
+            from .inhomogeneous import inhomogeneous_spec
+            return inhomogeneous_spec(self, precision, dims_list)
         na, N = self.na, self.N
         stev = np.zeros((na, 28))
         kmax = np.zeros(na, np.int32)
@@ -241,6 +255,7 @@ def clone_system(sys: System, seed=None) -> System:
     c.params = {k: p.copy() for k, p in sys.params.items()}
     c.extfield = sys.extfield.copy()
     c.kappas = sys.kappas.copy()
+    c.inhomog = copy.deepcopy(sys.inhomog)
     sys._pull()
     c._dip = sys._dip.copy()
     c._coh = sys._coh.copy()
@@ -283,7 +298,7 @@ def _normalize_dipole(s, kappa):
 def set_dipole_(sys: System, direction, site):
     sys._pull()
     k = sys.site_index(site)
-    kappa = sys.kappas[site[3]]
+    kappa = sys.site_kappas()[k]
     if sys.N == 0:
         sys._dip[k] = _normalize_dipole(np.asarray(direction, float), kappa)
     else:
@@ -298,7 +313,7 @@ def set_coherent_(sys: System, Z, site):
         raise ValueError("Cannot set zero-length coherent state")
     sys._pull()
     k = sys.site_index(site)
-    Z = _normalize_ket(np.asarray(Z, complex), sys.kappas[site[3]])
+    Z = _normalize_ket(np.asarray(Z, complex), sys.site_kappas()[k])
     sys._coh[k] = Z
     sys._dip[k] = expected_spin_host(Z[None], sys.N)[0]
     sys._host_dirty = True
@@ -307,12 +322,13 @@ def set_coherent_(sys: System, Z, site):
 def polarize_spins_(sys: System, direction):
     sys._pull()
     d = np.asarray(direction, float)
-    kap = np.repeat(sys.kappas, sys.ncells)
+    kap = sys.site_kappas()
     if sys.N == 0:
         sys._dip[:] = kap[:, None] * (d / np.linalg.norm(d))
     else:
         Z = ket_from_dipole(d, sys.N)
-        sys._coh[:] = Z[None, :] / np.sqrt(np.vdot(Z, Z).real / kap[:, None])
+        d = np.sqrt(np.vdot(Z, Z).real / np.where(kap > 0, kap, 1.0))[:, None]
+        sys._coh[:] = np.where(kap[:, None] > 0, Z[None, :] / d, 0)
         sys._dip[:] = expected_spin_host(sys._coh, sys.N)
     sys._host_dirty = True
 
@@ -321,13 +337,14 @@ def randomize_spins_(sys: System):
     """Spins uniform on the sphere (dipoles) or normalised complex Gaussian
     kets (SU(N)), drawn from the system's own generator (System.jl:444-452)."""
     sys._pull()
-    kap = np.repeat(sys.kappas, sys.ncells)
+    kap = sys.site_kappas()
     if sys.N == 0:
         v = sys.rng.standard_normal((sys.nsites, 3))
         sys._dip[:] = kap[:, None] * v / np.linalg.norm(v, axis=1, keepdims=True)
     else:
         z = (sys.rng.standard_normal((sys.nsites, sys.N)) + 1j * sys.rng.standard_normal((sys.nsites, sys.N))) / np.sqrt(2)
-        sys._coh[:] = z / np.sqrt((np.abs(z) ** 2).sum(1, keepdims=True) / kap[:, None])
+        d = np.sqrt((np.abs(z) ** 2).sum(1, keepdims=True) / np.where(kap > 0, kap, 1.0)[:, None])
+        sys._coh[:] = np.where(kap[:, None] > 0, z / d, 0)
         sys._dip[:] = expected_spin_host(sys._coh, sys.N)
     sys._host_dirty = True
 
@@ -338,7 +355,14 @@ def randomize_spins_(sys: System):
 def set_field_(sys: System, B):
     """External field for every site (src/System/Interactions.jl:191-212)."""
     sys.extfield[:] = np.asarray(B, float)
+    if sys.inhomog is not None:
+        sys.inhomog.field[:] = np.asarray(B, float)
     sys._invalidate()
+
+
+def _require_homogeneous(sys, alt):
+    if sys.inhomog is not None:
+        raise ValueError(f"Use `{alt}` for an inhomogeneous system.")
 
 
 def _to_float_or_mat3(J):
@@ -466,6 +490,7 @@ def set_exchange_(sys: System, J, bond: Bond, biquad=0.0, propagate=None, param=
     Scalar J propagates to every bond of the same distance shell (exact for
     Heisenberg couplings).  Matrix J applies to `bond` in every cell (pass
     `propagate=[bonds...]` with pre-rotated matrices for symmetry partners)."""
+    _require_homogeneous(sys, "set_exchange_at_")
     if bond.i == bond.j and tuple(bond.n) == (0, 0, 0):
         raise ValueError("Bond must connect different atoms.")
     Jv = _to_float_or_mat3(J)
@@ -500,6 +525,7 @@ def set_pair_biquad_(sys: System, B, bond: Bond, param=None):
     """Coupling Q_i . B Q_j of Stevens quadrupoles (5x5, q descending) on one
     bond in every cell; the reverse bond carries B' (PairExchange.jl:447-468).
     In :dipole mode B is renormalised by rcs_2(s_i) rcs_2(s_j)."""
+    _require_homogeneous(sys, "set_pair_coupling_at_")
     B = np.asarray(B, float).reshape(5, 5)
     if sys.mode == "dipole":
         B = B * rcs_factors(sys.spins[bond.i])[2] * rcs_factors(sys.spins[bond.j])[2]
@@ -527,6 +553,7 @@ def set_pair_coupling_(sys: System, op, bond: Bond, extract_parts=True, param=No
     carries the transposed parts and (B_k, A_k).  The general remainder needs
     :SUN mode.  No symmetry propagation: call once per bond (with the
     operator rotated as required)."""
+    _require_homogeneous(sys, "set_pair_coupling_at_")
     if bond.i == bond.j and tuple(bond.n) == (0, 0, 0):
         raise ValueError("Bond must connect different atoms.")
     si, sj = sys.spins[bond.i], sys.spins[bond.j]
@@ -561,6 +588,7 @@ def set_onsite_coupling_(sys: System, op, i: int, param=None):
     `op` is an N x N matrix in the spin-s irrep, or a function of the spin
     matrices; `param` = (label, val) as in set_exchange_.  No symmetry
     propagation: call once per atom (with the operator rotated as required)."""
+    _require_homogeneous(sys, "set_onsite_coupling_at_")
     s = sys.spins[i]
     if callable(op):
         op = op(spin_matrices(s))
@@ -585,6 +613,7 @@ def set_onsite_coupling_(sys: System, op, i: int, param=None):
 
 def set_stevens_expansion_(sys: System, stvexp: StevensExpansion, i: int, param=None):
     """Sets already-renormalised classical Stevens coefficients (:dipole)."""
+    _require_homogeneous(sys, "set_onsite_coupling_at_")
     if sys.mode != "dipole":
         raise ValueError("Stevens expansions apply to :dipole mode")
     key = ("onsite", int(i))
@@ -605,4 +634,7 @@ def set_spin_rescaling_(sys: System, kappas):
     """Per-sublattice |S| (dipole) or |Z|^2 (SU(N)) rescaling (System.jl:573-)."""
     sys._pull()
     sys.kappas = np.asarray(kappas, float).reshape(sys.na).copy()
+    if sys.inhomog is not None:  # every non-vacant site of each sublattice
+        kap = np.repeat(sys.kappas, sys.ncells)
+        sys.inhomog.kappa = np.where(sys.inhomog.kappa == 0, 0.0, kap)
     sys._invalidate()
