@@ -71,6 +71,10 @@ class SampledCorrelations:
         self.corr_pairs = list(SSF_CORR_PAIRS if corr_pairs is None else corr_pairs)
         self.window = window
         self.tol = tol
+        if getattr(sys, "inhomog", None) is not None:
+            # the device holds an inhomogeneous system as one supercell-sized
+            # cell, which has no lattice to transform over
+            raise NotImplementedError("sampled correlations of an inhomogeneous system are not supported here")
         self.dims = tuple(int(x) for x in sys.dims)
         self.na = sys.crystal.natoms
         self.sys = sys

@@ -22,7 +22,7 @@ def suggest_timestep_aux(sys: System, integrator, *, tol):
     kT = getattr(integrator, "kT", 0.0)
     if sys.N == 0:
         g = energy_grad_dipoles(sys)
-        kap = np.repeat(np.asarray(sys.kappas, float), sys.ncells)
+        kap = sys.site_kappas()
         acc = float(np.sum((1 + (kap * lam) ** 2) * np.sum(g * g, axis=1)))
     else:
         g = energy_grad_coherents(sys)
@@ -114,7 +114,7 @@ def minimize_energy_(sys: System, maxiters=1000, jitter=1e-8, seed=0):
     if sys.N != 0:
         return _minimize_sun(sys, maxiters, jitter, seed)
     rng = np.random.default_rng(seed)
-    kap = np.repeat(np.asarray(sys.kappas, float), sys.ncells)[:, None]
+    kap = sys.site_kappas()[:, None]
     x = sys.dipoles_flat()
     if jitter:  # perturb_spins!(sys, jitter)
         x = x + jitter * kap * rng.standard_normal(x.shape)
@@ -145,7 +145,7 @@ def _minimize_sun(sys: System, maxiters, jitter, seed):
     gradient of E(x) is twice that, projected on the tangent space."""
     rng = np.random.default_rng(seed)
     N = sys.N
-    rk = np.sqrt(np.repeat(np.asarray(sys.kappas, float), sys.ncells))[:, None]
+    rk = np.sqrt(sys.site_kappas())[:, None]
     Z = sys.coherents_flat()
     if jitter:  # perturb_spins!(sys, jitter): Z + jitter sqrt(kappa) randn(ComplexF64)
         Z = Z + jitter * rk * (rng.standard_normal(Z.shape) + 1j * rng.standard_normal(Z.shape)) / math.sqrt(2)

@@ -87,7 +87,8 @@ def local_coloring(sys):
     m = np.zeros(3, np.int32)
     nc = _lib.C.c_int32()
     h.call("snh_local_coloring", _lib.iptr(m), _lib.C.byref(nc), None)
-    na = sys.crystal.natoms if hasattr(sys, "crystal") else sys.systems[0].crystal.natoms
+    # sublattices of the device cell (an inhomogeneous system has one per site)
+    na = int(h._keep["na"])
     color = np.zeros(na * int(np.prod(m)), np.int32)
     h.call("snh_local_coloring", None, None, _lib.iptr(color))
     return tuple(int(x) for x in m), nc.value, color.reshape(na, m[2], m[1], m[0])
