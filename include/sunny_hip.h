@@ -291,7 +291,7 @@ int snh_local_coloring(snh_system *sys, int32_t m[3], int32_t *ncolors, int32_t 
  * kept ranks are visible through snh_query ("tile_tuned",
  * "tile_tuned_correct").  Before any autotuning both kernels use the cost
  * model's first plan, except that FP32 handles whose tiled kernels take their
- * lean forms over isotropic bonds start the corrector on the same tile with
+ * lean forms over isotropic bonds start both kernels on the same tile with
  * one site per thread when the model ranks that plan. */
 int snh_autotune(snh_system *sys, int32_t candidates, int32_t launches, double *best_ms);
 

@@ -835,7 +835,9 @@ struct SystemImpl final : snh_system {
         Xo = own(dalloc<T>((size_t)planes * total));
         SNH_HIP(hipMemset(X, 0, sizeof(T) * planes * (size_t)total));
         if (tiled) {
-            apply_plan(tile_cands[0], 0);
+            // lean FP32 kernels over isotropic bonds run both stages one site
+            // per thread on the model's first tile where it offers that plan
+            apply_plan(tile_cands[default_correct_plan()], 0);
             apply_plan(tile_cands[default_correct_plan()], 1);
         }
         if (!partial) {
@@ -1122,10 +1124,11 @@ struct SystemImpl final : snh_system {
     // Makes P the tiled plan of the predictor side (which = 0) or of the
     // corrector side (which = 1): device work list, LDS bond offsets per
     // ring phase, per-replica workgroup ranges, and room for its partial sums.
-    // The corrector's (and midpoint iteration's) plan before any autotuning:
-    // the cost model's first, except that lean FP32 kernels over isotropic
-    // bonds take the same tile with one site per thread where the model
-    // offers it (their one-site forms fit two 1024-thread workgroups per CU).
+    // The plan of both stages (and of the midpoint iteration) before any
+    // autotuning: the cost model's first, except that lean FP32 kernels over
+    // isotropic bonds take the same tile with one site per thread where the
+    // model offers it (their one-site forms fit two 1024-thread workgroups
+    // per CU: the corrector in 64 registers, the predictor in 60).
     size_t default_correct_plan() const
     {
         if (sizeof(T) != 4 || !C.lean_ok || !C.iso || ewald || env_int("SNH_TILE_SP", 0)) return 0;

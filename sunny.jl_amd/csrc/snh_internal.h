@@ -73,10 +73,15 @@ constexpr int tile_threads_max()
 {
     return (sizeof(T) == 4 && SP == 1) ? 1024 : 512;
 }
-// Layer positions fetched per thread: 4 with one site per thread, 6 with two.
+// Layer positions fetched per thread: 3 with one site per thread, 6 with
+// two.  Every slot costs three loads per layer whether or not a position
+// falls in it (idle slots reload position 0), so the count is kept to what
+// the common plans fill: a 32 x 2 tile of 16 sublattices has 2176
+// positions for 1024 threads (4 slots: 0.727 ms per Heun step on the
+// 64-replica pyrochlore batch, 3 slots: 0.685 ms).
 constexpr int tile_load_per(int sp)
 {
-    return sp == 1 ? 4 : 6;
+    return sp == 1 ? 3 : 6;
 }
 
 // LDS-tiled gather (snh_tile.hip).  State planes use the unpadded layout

@@ -296,11 +296,26 @@ struct OwnIn {
     const uint32_t slot = 3u * (uint32_t)sizeof(T_) * (uint32_t)A.cap;                              \
     OwnIn<T_> in0{}, in1{}, nx0{}, nx1{}
 
+// Waves per SIMD requested for the predictor: the lean FP32 one-site form
+// fits eight (60 registers; some scalar values kept in register lanes), so
+// two 1024-thread workgroups share a CU (predictor 0.403 -> 0.372 ms on the
+// 64-replica pyrochlore batch); else the compiler's choice.
+#ifndef SNH_PRED_WAVES
+#define SNH_PRED_WAVES 8
+#endif
+template <typename T, int SP, bool kLean>
+constexpr int predict_waves()
+{
+    if (SNH_TILE_WAVES_PER_EU > 0) return SNH_TILE_WAVES_PER_EU;
+    return (SNH_PRED_WAVES > 0 && kLean && SP == 1 && sizeof(T) == 4) ? SNH_PRED_WAVES : 1;
+}
+
 // Heun predictor (see heun_predict): S' and H from S.  kLean: no Ewald
 // potential, device noise with the step as an argument (or none), onsite
 // terms of rank <= 2; fewer live pointers and branches in the layer loop.
 template <typename T, int CH, bool kIso, int SP, bool kLean>
-__global__ SNH_TILE_OCC __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_predict(Couplings<T> C, TileArgs A,
+__global__ __attribute__((amdgpu_waves_per_eu(predict_waves<T, SP, kLean>())))
+__launch_bounds__((tile_threads_max<T, SP>())) void k_tile_predict(Couplings<T> C, TileArgs A,
                                                                         const T* __restrict__ S, T* __restrict__ Sp,
                                                                         T* __restrict__ H, const T* __restrict__ phi,
                                                                         NoiseArgs<T> nz, T dt, T lambda)

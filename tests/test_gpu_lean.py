@@ -70,7 +70,9 @@ def test_lean_forms_match_general(kind, precision, monkeypatch):
         sh.step_(a, integ, nsteps=6)
         sh.step_(b, integ, nsteps=6)
         np.testing.assert_array_equal(state(a), state(b))
-    np.testing.assert_array_equal(sh.energy(a), sh.energy(b))
+    # the FP64 energy sums per workgroup of the predictor's plan, and a lean
+    # handle's default plan (one site per thread) groups the sites differently
+    np.testing.assert_allclose(sh.energy(a), sh.energy(b), rtol=1e-15, atol=0)
     mid = sh.ImplicitMidpoint(0.02, damping=0.1, tol=1e-6 if precision == 32 else 1e-12)
     ia = sh.step_(a, mid, nsteps=4)
     ib = sh.step_(b, mid, nsteps=4)
