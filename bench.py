@@ -468,6 +468,7 @@ def main():
     ap.add_argument("--fp32-only", action="store_true", help="time only the FP32 batch (tuning runs)")
     ap.add_argument("--fp64-only", action="store_true", help="time only the plain FP64 step (tuning runs)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the cost model's first tile plan")
+    ap.add_argument("--tune-candidates", type=int, default=8, help="tile plans the autotuner times")
     ap.add_argument("--kT", type=float, default=0.3, help="temperature of the timed runs (tuning: 0 disables noise)")
     ap.add_argument("--workload", choices=["pyrochlore", "sun3", "ewald", "cubic16", "adjoint", "corr", "local"],
                     default="pyrochlore",
@@ -522,7 +523,7 @@ def main():
     batch = sh.ReplicaBatch(reps, precision=32, layout=layout)
     hb = batch._device()
     if not args.no_autotune:
-        tune_ms = hb.autotune(candidates=8, launches=3)
+        tune_ms = hb.autotune(candidates=args.tune_candidates, launches=3)
         log(f"autotune: kept plan ranks {hb.query('tile_tuned')} (predictor), {hb.query('tile_tuned_correct')} "
             f"(corrector) of {hb.query('tile_candidates')}, {tune_ms:.4f} ms per predictor + corrector")
     plan = {k: hb.query(k) for k in ("layout", "tile_sp", "tile_T1", "tile_T2", "tile_threads", "tile_work",

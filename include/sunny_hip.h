@@ -292,7 +292,8 @@ int snh_local_coloring(snh_system *sys, int32_t m[3], int32_t *ncolors, int32_t 
  * "tile_tuned_correct").  Before any autotuning both kernels use the cost
  * model's first plan, except that FP32 handles whose tiled kernels take their
  * lean forms over isotropic bonds start both kernels on the same tile with
- * one site per thread when the model ranks that plan. */
+ * one site per thread when the model ranks that plan.  With the environment
+ * variable SNH_AUTOTUNE_VERBOSE=1 every timed plan is printed to stderr. */
 int snh_autotune(snh_system *sys, int32_t candidates, int32_t launches, double *best_ms);
 
 /* Diagnostics with no counterpart in the reference: integer facts about the

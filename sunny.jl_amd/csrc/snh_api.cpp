@@ -1212,6 +1212,10 @@ struct SystemImpl final : snh_system {
                     best_t[k] = t[k];
                     best[k] = c;
                 }
+            if (env_int("SNH_AUTOTUNE_VERBOSE", 0))
+                std::fprintf(stderr, "[snh_autotune] plan %d: T1 %d T2 %d sp %d threads %d workgroups %d lds %zu: "
+                                     "predictor %.4f ms, corrector %.4f ms\n",
+                             c, tp.A.T1, tp.A.T2, tp.A.sp, tp.threads, tp.A.nwork, tp.lds, t[0], t[1]);
         }
         apply_plan(tile_cands[best[0]], 0);
         apply_plan(tile_cands[best[1]], 1);
