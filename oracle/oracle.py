@@ -76,6 +76,8 @@ def lib():
                                             _dp]
         L.so_local_sequential.argtypes = [sp, _dp, C.c_int, _ip, _dp, _dp, C.c_double, C.c_uint64, C.c_uint64,
                                           C.c_double, _dp]
+        L.so_local_sequential_sun.argtypes = [sp, _dp, _dp, C.c_int, _ip, _dp, _dp, C.c_double, C.c_uint64,
+                                              C.c_uint64, C.c_double, _dp]
         _lib = L
     return _lib
 
@@ -246,6 +248,18 @@ class Oracle:
         if rc:
             raise ValueError("Energy delta for self-interaction not supported")
         return S, acc
+
+    def local_sequential_sun(self, Z, proposals, kT, key, sweep0, nsweeps):
+        """:SUN form of local_sequential; returns (Z, [dE, dSx, dSy, dSz])."""
+        Zr = self._ket_arrays(Z).copy()
+        S = self.expected_spin(Z)
+        kind, mag, cum = self._props(proposals)
+        acc = np.zeros(4)
+        rc = lib().so_local_sequential_sun(self._p(), _d(Zr), _d(S), len(kind), _i(kind), _d(mag), _d(cum),
+                                           float(kT), int(key) % 2**64, int(sweep0), float(nsweeps), _d(acc))
+        if rc:
+            raise ValueError("Energy delta for self-interaction not supported")
+        return Zr[..., 0] + 1j * Zr[..., 1], acc
 
     def expected_spin(self, Z):
         Zr = np.ascontiguousarray(np.stack([Z.real, Z.imag], -1), float)

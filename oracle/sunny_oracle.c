@@ -1492,21 +1492,71 @@ static void flip_ket(int N, const cplx *Z, cplx *out)
     }
 }
 
-/* LocalSampler passes for :SUN, the order and draws of so_local_sweeps.  The
-   2N normals of a site come from the Philox counters (s, 0x4D430000 + 2b, P),
-   b = 0, 1, ...; component a of the complex draw randn(CVec{N}) (variance
+/* One :SUN Metropolis trial at `site`: proposal kind picked by u_mix, its 2N
+   normals from Philox4x32-10 of counter ctr (second word advanced by 2 per
+   block of four); component a of the complex draw randn(CVec{N}) (variance
    1/2 per part) is (n[2a] + i n[2a+1]) / sqrt(2).  Proposals: randspin
    (System.jl:448-452), flip (flip_ket), perturbed_spin Z + mag sqrt(kappa)
-   randn (System.jl:461-468); S is kept equal to the expected spins of Z. */
+   randn (System.jl:461-468).  Accepted moves update Z, S (the expected spins
+   of Z) and acc.  Returns -1 on a self-interaction. */
+static int local_trial_sun(const so_spec *sp, const sun_ops *o, cplx *Z, double *S, long site, int nprop,
+                           const int *kind, const double *mag, const double *cum, double kT, double u_mix,
+                           double u_acc, const uint32_t ctr[4], const uint32_t kk[2], double acc[4])
+{
+    const int N = sp->N;
+    int kp = nprop - 1;
+    for (int p = 0; p < nprop - 1; p++)
+        if (u_mix < cum[p]) {
+            kp = p;
+            break;
+        }
+    const int i = (int)(site / ncells(sp));
+    const double kap = sp->kappa[i];
+    const cplx *Z0 = Z + (long)N * site;
+    cplx Zn[NMAX];
+    if (kind[kp] == 1) {
+        flip_ket(N, Z0, Zn);
+    } else {
+        double n[4 * ((2 * NMAX + 3) / 4)];
+        for (int b = 0; b < (2 * N + 3) / 4; b++) {
+            uint32_t cn[4] = {ctr[0], ctr[1] + 2u * (uint32_t)b, ctr[2], ctr[3]}, w[4];
+            so_philox4x32(cn, kk, w);
+            box_muller_pair(w[0], w[1], &n[4 * b], &n[4 * b + 1]);
+            box_muller_pair(w[2], w[3], &n[4 * b + 2], &n[4 * b + 3]);
+        }
+        const int unif = kind[kp] == 0 || isinf(mag[kp]);
+        for (int a = 0; a < N; a++) {
+            const cplx r = (n[2 * a] + I * n[2 * a + 1]) / sqrt(2.0);
+            Zn[a] = unif ? r : Z0[a] + mag[kp] * sqrt(kap) * r;
+        }
+        normalize_ket(N, Zn, kap);
+    }
+    double Sn[3];
+    const double dE = local_delta_sun(sp, o, Z, S, site, Zn, Sn);
+    if (isnan(dE)) return -1;
+    const int accept = kT == 0 ? dE <= 0 : u_acc <= exp(-dE / kT);
+    if (accept) {
+        acc[0] += dE;
+        for (int r = 0; r < 3; r++) {
+            acc[1 + r] += Sn[r] - S[3 * site + r];
+            S[3 * site + r] = Sn[r];
+        }
+        for (int a = 0; a < N; a++) Z[(long)N * site + a] = Zn[a];
+    }
+    return 0;
+}
+
+/* LocalSampler passes for :SUN, the order and draws of so_local_sweeps: the
+   normals of site s in pass P come from the counters (s, 0x4D430000 + 2b, P),
+   b = 0, 1, ..., the uniforms from (s, 0x4D430001, P). */
 int so_local_sweeps_sun(const so_spec *sp, double *Zr, double *S, const int m[3], const int *color, int ncolors,
                         int nprop, const int *kind, const double *mag, const double *cum, double kT, uint64_t key,
                         uint64_t sweep0, double nsweeps, double acc[4])
 {
-    const int N = sp->N;
     const int *L = sp->L;
     cplx *Z = (cplx *)Zr;
     sun_ops o;
-    sun_ops_init(&o, N);
+    sun_ops_init(&o, sp->N);
     const uint32_t kk[2] = {(uint32_t)key, (uint32_t)(key >> 32)};
     const long npass = (long)ceil(2 * nsweeps);
     const double inv = 1.0 / 4294967296.0;
@@ -1524,48 +1574,38 @@ int so_local_sweeps_sun(const so_spec *sp, double *Zr, double *S, const int m[3]
                             uint32_t ctr[4] = {(uint32_t)site, 0x4D430001u, (uint32_t)P, (uint32_t)(P >> 32)}, u[4];
                             so_philox4x32(ctr, kk, u);
                             if (!((double)u[2] * inv < prob)) continue;
-                            const double u_acc = (double)u[0] * inv, u_mix = (double)u[1] * inv;
-                            int kp = nprop - 1;
-                            for (int p = 0; p < nprop - 1; p++)
-                                if (u_mix < cum[p]) {
-                                    kp = p;
-                                    break;
-                                }
-                            double n[4 * ((2 * NMAX + 3) / 4)];
-                            for (int b = 0; b < (2 * N + 3) / 4; b++) {
-                                uint32_t cn[4] = {(uint32_t)site, 0x4D430000u + 2u * (uint32_t)b, (uint32_t)P,
-                                                  (uint32_t)(P >> 32)},
-                                         w[4];
-                                so_philox4x32(cn, kk, w);
-                                box_muller_pair(w[0], w[1], &n[4 * b], &n[4 * b + 1]);
-                                box_muller_pair(w[2], w[3], &n[4 * b + 2], &n[4 * b + 3]);
-                            }
-                            const double kap = sp->kappa[i];
-                            const cplx *Z0 = Z + (long)N * site;
-                            cplx Zn[NMAX];
-                            if (kind[kp] == 1) {
-                                flip_ket(N, Z0, Zn);
-                            } else {
-                                const int unif = kind[kp] == 0 || isinf(mag[kp]);
-                                for (int a = 0; a < N; a++) {
-                                    const cplx r = (n[2 * a] + I * n[2 * a + 1]) / sqrt(2.0);
-                                    Zn[a] = unif ? r : Z0[a] + mag[kp] * sqrt(kap) * r;
-                                }
-                                normalize_ket(N, Zn, kap);
-                            }
-                            double Sn[3];
-                            const double dE = local_delta_sun(sp, &o, Z, S, site, Zn, Sn);
-                            if (isnan(dE)) return -1;
-                            const int accept = kT == 0 ? dE <= 0 : u_acc <= exp(-dE / kT);
-                            if (accept) {
-                                acc[0] += dE;
-                                for (int r = 0; r < 3; r++) {
-                                    acc[1 + r] += Sn[r] - S[3 * site + r];
-                                    S[3 * site + r] = Sn[r];
-                                }
-                                for (int a = 0; a < N; a++) Z[(long)N * site + a] = Zn[a];
-                            }
+                            ctr[1] = 0x4D430000u;
+                            if (local_trial_sun(sp, &o, Z, S, site, nprop, kind, mag, cum, kT, (double)u[1] * inv,
+                                                (double)u[0] * inv, ctr, kk, acc))
+                                return -1;
                         }
+    }
+    return 0;
+}
+
+/* so_local_sequential for :SUN (the device's form with the Ewald term): the
+   site, u_acc and u_mix of trial T from counter (T lo, 0x4D430002, T hi, 0),
+   the normals from (T lo, 0x4D430003 + 2b, T hi, 0). */
+int so_local_sequential_sun(const so_spec *sp, double *Zr, double *S, int nprop, const int *kind, const double *mag,
+                            const double *cum, double kT, uint64_t key, uint64_t sweep0, double nsweeps,
+                            double acc[4])
+{
+    cplx *Z = (cplx *)Zr;
+    sun_ops o;
+    sun_ops_init(&o, sp->N);
+    const long ns = nsites(sp);
+    const long niters = (long)ceil(nsweeps * (double)ns);
+    const uint32_t kk[2] = {(uint32_t)key, (uint32_t)(key >> 32)};
+    const double inv = 1.0 / 4294967296.0;
+    for (long t = 0; t < niters; t++) {
+        const uint64_t T = sweep0 * (uint64_t)ns + (uint64_t)t;
+        uint32_t ctr[4] = {(uint32_t)T, 0x4D430002u, (uint32_t)(T >> 32), 0u}, u[4];
+        so_philox4x32(ctr, kk, u);
+        const long site = (long)(((uint64_t)u[0] * (uint64_t)ns) >> 32);
+        ctr[1] = 0x4D430003u;
+        if (local_trial_sun(sp, &o, Z, S, site, nprop, kind, mag, cum, kT, (double)u[2] * inv, (double)u[1] * inv, ctr,
+                            kk, acc))
+            return -1;
     }
     return 0;
 }

@@ -145,6 +145,11 @@ int so_local_sweeps_sun(const so_spec *sp, double *Z, double *S, const int m[3],
    T hi, 0) for the normals. */
 int so_local_sequential(const so_spec *sp, double *S, int nprop, const int *kind, const double *mag,
                         const double *cum, double kT, uint64_t key, uint64_t sweep0, double nsweeps, double acc[4]);
+/* :SUN form of so_local_sequential: the normals of trial T from the counters
+   (T lo, 0x4D430003 + 2b, T hi, 0), b = 0, 1, ... */
+int so_local_sequential_sun(const so_spec *sp, double *Z, double *S, int nprop, const int *kind, const double *mag,
+                            const double *cum, double kT, uint64_t key, uint64_t sweep0, double nsweeps,
+                            double acc[4]);
 /* The same trials at uniformly random sites, as Samplers.jl:133-153 orders
    them (ceil(nsweeps * nsites) trials; splitmix64 draws from seed). */
 int so_local_random_sites(const so_spec *sp, double *S, int nprop, const int *kind, const double *mag,

@@ -1378,8 +1378,6 @@ struct SystemImpl final : snh_system {
     void mc_prepare()
     {
         if (mc.ready) return;
-        SNH_REQUIRE(!ewald || N == 0, SNH_ERR_UNSUPPORTED,
-                    "the device LocalSampler supports dipole-dipole (Ewald) interactions in :dipole mode only");
         std::vector<std::array<int, 3>> dims;
         for (const auto& ri : reps) dims.push_back({ri.L1, ri.L2, ri.L3});
         // a bond that maps a site onto itself (local_energy_change raises)
@@ -1513,7 +1511,12 @@ struct SystemImpl final : snh_system {
             const long long niters = (long long)std::ceil(nsweeps * (double)ns);
             hipEvent_t e = tbegin();
             for (long long t = 0; t < niters; t++) {
-                launch_mc_ewald_trial<T>(C, A, X, mc.phi, mc.A0, sweep0 * ns + (uint64_t)t, mc.trial, mc.acc, R, st);
+                if (N == 0)
+                    launch_mc_ewald_trial<T>(C, A, X, mc.phi, mc.A0, sweep0 * ns + (uint64_t)t, mc.trial, mc.acc, R,
+                                             st);
+                else
+                    launch_sun_mc_ewald_trial<T>(C, A, X, mc.phi, mc.A0, sweep0 * ns + (uint64_t)t, mc.trial, mc.acc,
+                                                 R, st);
                 launch_mc_ewald_update(mc.Aew, C.reps, C.G, na, total, ns, mc.trial, mc.phi, R, st);
             }
             tend("local_ewald", e);

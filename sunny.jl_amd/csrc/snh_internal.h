@@ -238,6 +238,10 @@ struct McTrial {
 template <typename T>
 void launch_mc_ewald_trial(const Couplings<T>& C, const McArgs& A, T* S, const double* phi, const double* A0,
                            unsigned long long T0, McTrial* tr, double* acc, int R, hipStream_t st);
+// The same trial for :SUN kets (the Ewald term acts on expected dipoles).
+template <typename T>
+void launch_sun_mc_ewald_trial(const Couplings<T>& C, const McArgs& A, T* Z, const double* phi, const double* A0,
+                               unsigned long long T0, McTrial* tr, double* acc, int R, hipStream_t st);
 // phi += A dmu at the ns sites of every replica whose trial was accepted.
 void launch_mc_ewald_update(const double* Aew, const ReplicaInfo* reps, const Ghost& G, int na, uint32_t total,
                             uint32_t ns, const McTrial* tr, double* phi, int R, hipStream_t st);

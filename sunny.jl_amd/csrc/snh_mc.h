@@ -125,6 +125,50 @@ __device__ __forceinline__ int mc_pick(const McArgs& A, double u_mix)
     return k;
 }
 
+// Draws of sequential trial T0 (the Ewald form) of one replica of ns sites:
+// Philox4x32-10 of counter (T0 lo, kMcTagSequential, T0 hi, 0) gives the
+// site (w0 ns) >> 32, u_acc = w1 / 2^32 and u_mix = w2 / 2^32; the normals
+// come from the counters (T0 lo, kMcTagSequential + 1 + 2 b, T0 hi, 0).
+struct McSeqDraw {
+    uint32_t site;
+    double u_acc, u_mix;
+};
+
+__device__ __forceinline__ McSeqDraw mc_seq_draw(unsigned long long key, unsigned long long T0, uint32_t ns)
+{
+    uint32_t un[4] = {(uint32_t)T0, kMcTagSequential, (uint32_t)(T0 >> 32), 0u};
+    philox4x32(un, (uint32_t)key, (uint32_t)(key >> 32));
+    const double inv32 = 1.0 / 4294967296.0;
+    return {(uint32_t)(((unsigned long long)un[0] * ns) >> 32), (double)un[1] * inv32, (double)un[2] * inv32};
+}
+
+// ewald_energy_delta (Ewald.jl:234-241) of sublattice-i site g whose dipole
+// changes by dS: dS . (-g^T phi) + dmu . A(0, i, i) dmu / 2 with
+// dmu = -g dS (returned in dmu), phi the site's current FP64 potential and
+// A0 the [na][9] self blocks.
+template <typename T>
+__device__ __forceinline__ double mc_ewald_delta(const Couplings<T>& C, int i, const double* __restrict__ phi,
+                                                 uint32_t g, const double* __restrict__ A0, const double dS[3],
+                                                 double dmu[3])
+{
+    const uint32_t total = C.total;
+    const double p[3] = {phi[g], phi[g + total], phi[g + 2 * (size_t)total]};
+    double gm[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) gm[q] = (double)ldc(C.gmat + 9 * i, q);
+#pragma unroll
+    for (int a = 0; a < 3; a++) dmu[a] = -(gm[3 * a] * dS[0] + gm[3 * a + 1] * dS[1] + gm[3 * a + 2] * dS[2]);
+    const double* a0 = A0 + 9 * i;
+    double dE = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double grad = -(gm[a] * p[0] + gm[3 + a] * p[1] + gm[6 + a] * p[2]);
+        const double adm = a0[3 * a] * dmu[0] + a0[3 * a + 1] * dmu[1] + a0[3 * a + 2] * dmu[2];
+        dE += dS[a] * grad + dmu[a] * adm / 2;
+    }
+    return dE;
+}
+
 // Padded index of the neighbour across gather slot b of the site.
 __device__ __forceinline__ uint32_t mc_neighbour(const McArgs& A, const Ghost& G, const SiteCoord& sc, int b,
                                                  bool wide)

@@ -168,12 +168,10 @@ __global__ __launch_bounds__(kBlock) void k_mc_color(Couplings<T> C, McArgs A, T
 
 // Sequential trial of the Ewald form, workgroup r = replica r (its first
 // lane only: philox4x32 takes its key as uniform over the wave): trial T0
-// draws the site and u_acc, u_mix from Philox4x32-10 counter
-// (T lo, kMcTagSequential, T hi, 0) and the normals from counter
-// (T lo, kMcTagSequential + 1, T hi, 0).  dE adds ewald_energy_delta
-// (Ewald.jl:234-241): dS . (-g^T phi) + dmu . A(0, i, i) dmu / 2 with
-// dmu = -g dS, phi the site's current potential.  tr[r] records an accepted
-// move (site, dmu) for k_mc_ewald_update; acc[4 r + q] += (dE, dS).
+// draws the site, u_acc and u_mix by mc_seq_draw and the normals from
+// counter (T0 lo, kMcTagSequential + 1, T0 hi, 0).  dE adds
+// ewald_energy_delta (Ewald.jl:234-241).  tr[r] records an accepted move
+// (site, dmu) for k_mc_ewald_update; acc[4 r + q] += (dE, dS).
 template <typename T, int G>
 __global__ __launch_bounds__(64) void k_mc_ewald_trial(Couplings<T> C, McArgs A, T* __restrict__ S,
                                                        const double* __restrict__ phi,
@@ -186,37 +184,21 @@ __global__ __launch_bounds__(64) void k_mc_ewald_trial(Couplings<T> C, McArgs A,
     const uint32_t total = C.total;
     const uint32_t ns = (uint32_t)R.ncell * (uint32_t)C.na;
     const unsigned long long key = A.seed + (unsigned long long)r * 0x9E3779B97F4A7C15ull;
-    uint32_t un[4] = {(uint32_t)T0, kMcTagSequential, (uint32_t)(T0 >> 32), 0u};
-    philox4x32(un, (uint32_t)key, (uint32_t)(key >> 32));
-    const uint32_t site = (uint32_t)(((unsigned long long)un[0] * ns) >> 32);
-    const double inv32 = 1.0 / 4294967296.0;
-    const double u_acc = (double)un[1] * inv32, u_mix = (double)un[2] * inv32;
-    const int i = (int)(site / (uint32_t)R.ncell);
-    const int c = (int)(site - (uint32_t)i * (uint32_t)R.ncell);
+    const McSeqDraw dr = mc_seq_draw(key, T0, ns);
+    const int i = (int)(dr.site / (uint32_t)R.ncell);
+    const int c = (int)(dr.site - (uint32_t)i * (uint32_t)R.ncell);
     const int c1 = c % R.L1, c2 = (c / R.L1) % R.L2, c3 = c / (R.L1 * R.L2);
     SiteCoord sc = mc_site(R, C.G, i, c1, c2, c3, true);
     sc.r = r;
     const V3<T> s0 = {ldg(S, sc.g), ldg(S, sc.g + total), ldg(S, sc.g + 2 * total)};
     uint32_t cn[4] = {(uint32_t)T0, kMcTagSequential + 1u, (uint32_t)(T0 >> 32), 0u};
-    const V3<T> sn = mc_propose<T>(A, s0, ldc(C.kappa, i), u_mix, cn, key);
+    const V3<T> sn = mc_propose<T>(A, s0, ldc(C.kappa, i), dr.u_mix, cn, key);
     double dE = mc_local_delta<T, G>(C, A, S, sc, s0, sn, A.wide != 0);
     const double dS[3] = {(double)sn.x - (double)s0.x, (double)sn.y - (double)s0.y, (double)sn.z - (double)s0.z};
-    const double p[3] = {phi[sc.g], phi[sc.g + total], phi[sc.g + 2 * total]};
-    double g[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) g[q] = (double)ldc(C.gmat + 9 * i, q);
     double dmu[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) dmu[a] = -(g[3 * a] * dS[0] + g[3 * a + 1] * dS[1] + g[3 * a + 2] * dS[2]);
-    const double* a0 = A0 + 9 * i;
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        const double grad = -(g[a] * p[0] + g[3 + a] * p[1] + g[6 + a] * p[2]);
-        const double adm = a0[3 * a] * dmu[0] + a0[3 * a + 1] * dmu[1] + a0[3 * a + 2] * dmu[2];
-        dE += dS[a] * grad + dmu[a] * adm / 2;
-    }
+    dE += mc_ewald_delta(C, i, phi, sc.g, A0, dS, dmu);
     const double kT = A.kT[r];
-    const bool accept = kT == 0.0 ? dE <= 0.0 : u_acc <= exp(-dE / kT);
+    const bool accept = kT == 0.0 ? dE <= 0.0 : dr.u_acc <= exp(-dE / kT);
     McTrial out{};
     if (accept) {
         mc_store(S, C, sc, sn);

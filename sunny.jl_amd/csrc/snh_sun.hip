@@ -688,16 +688,126 @@ __device__ __forceinline__ double onsite_delta(const Couplings<T>& C, int i, con
     return e;
 }
 
+// Proposal k of LocalSampler for ket z0: flip_ket, randspin or
+// perturbed_spin Z + mag sqrt(kappa) randn(CVec{N}).  The 2N normals come
+// from Philox4x32-10 of counter c, its second word advanced by 2 per block
+// of four.
+template <typename T, int N>
+__device__ __forceinline__ Ket<T, N> sun_mc_propose(const McArgs& A, int k, const Ket<T, N>& z0, T kap,
+                                                    const uint32_t c[4], unsigned long long key)
+{
+    Ket<T, N> zn;
+    if (A.kind[k] == 1) {  // flip_ket: reverse(parity .* conj(Z))
+#pragma unroll
+        for (int a = 0; a < N; a++) {
+            const int j = N - 1 - a;
+            const T sg = (j & 1) ? (T)-1 : (T)1;
+            zn.z[a] = {sg * z0.z[j].re, -sg * z0.z[j].im};
+        }
+        return zn;
+    }
+    constexpr int NB = (2 * N + 3) / 4;
+    T n[4 * NB];
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        uint32_t cn[4] = {c[0], c[1] + 2u * (uint32_t)b, c[2], c[3]};
+        philox4x32(cn, (uint32_t)key, (uint32_t)(key >> 32));
+        box_muller(cn[0], cn[1], n[4 * b], n[4 * b + 1]);
+        box_muller(cn[2], cn[3], n[4 * b + 2], n[4 * b + 3]);
+    }
+    const double mag = A.mag[k];
+    const bool unif = A.kind[k] == 0 || __builtin_isinf(mag);
+    const T f = unif ? (T)0 : (T)(mag * sqrt((double)kap));
+    const T h = (T)0.70710678118654752440;  // randn(ComplexF64): variance 1/2 per part
+#pragma unroll
+    for (int a = 0; a < N; a++) {
+        const T nr = n[2 * a] * h, ni = n[2 * a + 1] * h;
+        zn.z[a] = unif ? Cx<T>{nr, ni} : Cx<T>{z0.z[a].re + f * nr, z0.z[a].im + f * ni};
+    }
+    return normalize_ket<T, N>(zn, kap);
+}
+
+// local_energy_change (Interactions.jl:311-380) of site sc going from z0 to
+// zn, without the Ewald term, in FP64: Zeeman and bilinear terms on expected
+// spins, Re(Z' Lambda Z) onsite, biquadratic terms on expected quadrupoles,
+// general couplings.  d: change of the expected spin.  Neighbour kets are
+// gathered G bonds at a time.
+template <typename T, int N, int G>
+__device__ __forceinline__ double sun_mc_local_delta(const Couplings<T>& C, const McArgs& A, const T* __restrict__ Z,
+                                                     const SiteCoord& sc, const Ket<T, N>& z0, const Ket<T, N>& zn,
+                                                     bool wide, double d[3])
+{
+    const int i = sc.i;
+    const uint32_t total = C.total;
+    T ox, oy, oz, nx, ny, nz;
+    expected_spin<T, N>(z0, ox, oy, oz);
+    expected_spin<T, N>(zn, nx, ny, nz);
+    const double dx = (double)nx - ox, dy = (double)ny - oy, dz = (double)nz - oz;
+    d[0] = dx;
+    d[1] = dy;
+    d[2] = dz;
+    double dE = (double)ldc(C.zee, 3 * i) * dx + (double)ldc(C.zee, 3 * i + 1) * dy + (double)ldc(C.zee, 3 * i + 2) * dz;
+    dE += onsite_delta<T, N>(C, i, zn, z0);
+    T Qn[5], Qo[5];
+    bool have_q = false;
+    const int b0 = ldc(C.bstart, i), b1 = ldc(C.bstart, i + 1);
+    for (int c0 = b0; c0 < b1; c0 += G) {
+        Ket<T, N> zj[G];
+#pragma unroll
+        for (int kk = 0; kk < G; kk++) zj[kk] = load_ket<T, N>(Z, total, mc_neighbour(A, C.G, sc, c0 + kk, wide));
+#pragma unroll
+        for (int kk = 0; kk < G; kk++) {
+            const int b = c0 + kk;
+            T sx, sy, sz;
+            expected_spin<T, N>(zj[kk], sx, sy, sz);
+            const double x = sx, y = sy, z = sz;
+            const T* J = C.bJ + 9 * b;
+            dE += dx * ((double)ldc(J, 0) * x + (double)ldc(J, 1) * y + (double)ldc(J, 2) * z) +
+                  dy * ((double)ldc(J, 3) * x + (double)ldc(J, 4) * y + (double)ldc(J, 5) * z) +
+                  dz * ((double)ldc(J, 6) * x + (double)ldc(J, 7) * y + (double)ldc(J, 8) * z);
+            if (ldc(C.bflag, b) & kBondBiquad) {
+                if (!have_q) {
+                    expected_quadrupole<T, N>(zn, C.quad_ops, Qn);
+                    expected_quadrupole<T, N>(z0, C.quad_ops, Qo);
+                    have_q = true;
+                }
+                T Qj[5];
+                expected_quadrupole<T, N>(zj[kk], C.quad_ops, Qj);
+                const T* Bq = C.bQ + 25 * b;
+                for (int p = 0; p < 5; p++) {
+                    double w = 0;
+                    for (int q = 0; q < 5; q++) w += (double)ldc(Bq, 5 * p + q) * (double)Qj[q];
+                    dE += ((double)Qn[p] - (double)Qo[p]) * w;
+                }
+            }
+            if (ldc(C.bflag, b) & kBondGeneral)  // Interactions.jl:371-377
+                for (int t = ldc(C.gstart, b); t < ldc(C.gstart, b + 1); t++) {
+                    const T* A_ = C.gA + 2 * N * N * t;
+                    const double dA = (double)expect_op<T, N>(A_, zn) - (double)expect_op<T, N>(A_, z0);
+                    dE += dA * (double)expect_op<T, N>(C.gB + 2 * N * N * t, zj[kk]);
+                }
+        }
+    }
+    return dE;
+}
+
+// Neighbour kets gathered per step of the Metropolis kernels: 3, 2 or 1 as N
+// grows (the SU(N) gather lists are padded to 6), which keeps them in
+// registers.
+template <typename T, int N>
+constexpr int sun_mc_gather()
+{
+    return (sizeof(T) == 4 ? N <= 3 : N <= 2) ? 3 : (N <= 4 ? 2 : 1);
+}
+
 template <typename T, int N>
 __global__ __launch_bounds__(kBlock) void k_sun_mc_color(Couplings<T> C, McArgs A, T* __restrict__ Z,
                                                          const McWork* __restrict__ work, unsigned long long P,
                                                          double prob, double* __restrict__ acc)
 {
-    constexpr int G = (sizeof(T) == 4 ? N <= 3 : N <= 2) ? 3 : (N <= 4 ? 2 : 1);
     const McWork W = work[blockIdx.x];
     const SiteCoord sc = mc_work_site(C, A, W, (int)threadIdx.x);
     const int i = sc.i;
-    const bool wide = A.wide != 0;
     const uint32_t total = C.total;
     double v[4] = {0, 0, 0, 0};
     const unsigned long long key = A.seed + (unsigned long long)W.r * 0x9E3779B97F4A7C15ull;
@@ -707,98 +817,67 @@ __global__ __launch_bounds__(kBlock) void k_sun_mc_color(Couplings<T> C, McArgs 
     const double inv32 = 1.0 / 4294967296.0;
     const double u_acc = (double)un[0] * inv32, u_mix = (double)un[1] * inv32, u_try = (double)un[2] * inv32;
     if (sc.valid && u_try < prob) {
-        const int k = mc_pick(A, u_mix);
         const Ket<T, N> z0 = load_ket<T, N>(Z, total, sc.g);
-        const T kap = ldc(C.kappa, i);
-        Ket<T, N> zn;
-        if (A.kind[k] == 1) {  // flip_ket: reverse(parity .* conj(Z))
-#pragma unroll
-            for (int a = 0; a < N; a++) {
-                const int j = N - 1 - a;
-                const T sg = (j & 1) ? (T)-1 : (T)1;
-                zn.z[a] = {sg * z0.z[j].re, -sg * z0.z[j].im};
-            }
-        } else {
-            constexpr int NB = (2 * N + 3) / 4;
-            T n[4 * NB];
-#pragma unroll
-            for (int b = 0; b < NB; b++) {
-                uint32_t cn[4] = {site, kMcTagNormals + 2u * (uint32_t)b, (uint32_t)P, (uint32_t)(P >> 32)};
-                philox4x32(cn, (uint32_t)key, (uint32_t)(key >> 32));
-                box_muller(cn[0], cn[1], n[4 * b], n[4 * b + 1]);
-                box_muller(cn[2], cn[3], n[4 * b + 2], n[4 * b + 3]);
-            }
-            const double mag = A.mag[k];
-            const bool unif = A.kind[k] == 0 || __builtin_isinf(mag);
-            const T f = unif ? (T)0 : (T)(mag * sqrt((double)kap));
-            const T h = (T)0.70710678118654752440;  // randn(ComplexF64): variance 1/2 per part
-#pragma unroll
-            for (int a = 0; a < N; a++) {
-                const T nr = n[2 * a] * h, ni = n[2 * a + 1] * h;
-                zn.z[a] = unif ? Cx<T>{nr, ni} : Cx<T>{z0.z[a].re + f * nr, z0.z[a].im + f * ni};
-            }
-            zn = normalize_ket<T, N>(zn, kap);
-        }
-        T ox, oy, oz, nx, ny, nz;
-        expected_spin<T, N>(z0, ox, oy, oz);
-        expected_spin<T, N>(zn, nx, ny, nz);
-        const double dx = (double)nx - ox, dy = (double)ny - oy, dz = (double)nz - oz;
-        double dE = (double)ldc(C.zee, 3 * i) * dx + (double)ldc(C.zee, 3 * i + 1) * dy +
-                    (double)ldc(C.zee, 3 * i + 2) * dz;
-        dE += onsite_delta<T, N>(C, i, zn, z0);
-        T Qn[5], Qo[5];
-        bool have_q = false;
-        const int b0 = ldc(C.bstart, i), b1 = ldc(C.bstart, i + 1);
-        for (int c0 = b0; c0 < b1; c0 += G) {
-            Ket<T, N> zj[G];
-#pragma unroll
-            for (int kk = 0; kk < G; kk++) zj[kk] = load_ket<T, N>(Z, total, mc_neighbour(A, C.G, sc, c0 + kk, wide));
-#pragma unroll
-            for (int kk = 0; kk < G; kk++) {
-                const int b = c0 + kk;
-                T sx, sy, sz;
-                expected_spin<T, N>(zj[kk], sx, sy, sz);
-                const double x = sx, y = sy, z = sz;
-                const T* J = C.bJ + 9 * b;
-                dE += dx * ((double)ldc(J, 0) * x + (double)ldc(J, 1) * y + (double)ldc(J, 2) * z) +
-                      dy * ((double)ldc(J, 3) * x + (double)ldc(J, 4) * y + (double)ldc(J, 5) * z) +
-                      dz * ((double)ldc(J, 6) * x + (double)ldc(J, 7) * y + (double)ldc(J, 8) * z);
-                if (ldc(C.bflag, b) & kBondBiquad) {
-                    if (!have_q) {
-                        expected_quadrupole<T, N>(zn, C.quad_ops, Qn);
-                        expected_quadrupole<T, N>(z0, C.quad_ops, Qo);
-                        have_q = true;
-                    }
-                    T Qj[5];
-                    expected_quadrupole<T, N>(zj[kk], C.quad_ops, Qj);
-                    const T* Bq = C.bQ + 25 * b;
-                    for (int p = 0; p < 5; p++) {
-                        double w = 0;
-                        for (int q = 0; q < 5; q++) w += (double)ldc(Bq, 5 * p + q) * (double)Qj[q];
-                        dE += ((double)Qn[p] - (double)Qo[p]) * w;
-                    }
-                }
-                if (ldc(C.bflag, b) & kBondGeneral)  // Interactions.jl:371-377
-                    for (int t = ldc(C.gstart, b); t < ldc(C.gstart, b + 1); t++) {
-                        const T* A_ = C.gA + 2 * N * N * t;
-                        const double dA = (double)expect_op<T, N>(A_, zn) - (double)expect_op<T, N>(A_, z0);
-                        dE += dA * (double)expect_op<T, N>(C.gB + 2 * N * N * t, zj[kk]);
-                    }
-            }
-        }
+        const uint32_t cn[4] = {site, kMcTagNormals, (uint32_t)P, (uint32_t)(P >> 32)};
+        const Ket<T, N> zn = sun_mc_propose<T, N>(A, mc_pick(A, u_mix), z0, ldc(C.kappa, i), cn, key);
+        double d[3];
+        const double dE = sun_mc_local_delta<T, N, sun_mc_gather<T, N>()>(C, A, Z, sc, z0, zn, A.wide != 0, d);
         const double kT = ldc(A.kT, W.r);
         const bool accept = kT == 0.0 ? dE <= 0.0 : u_acc <= exp(-dE / kT);
         if (accept) {
             store_ket_images<T, N>(Z, total, sc, C.G, zn);
             v[0] = dE;
-            v[1] = dx;
-            v[2] = dy;
-            v[3] = dz;
+            v[1] = d[0];
+            v[2] = d[1];
+            v[3] = d[2];
         }
     }
     block_sum4(v);
     if (threadIdx.x == 0)
         for (int q = 0; q < 4; q++) acc[4 * (size_t)blockIdx.x + q] += v[q];
+}
+
+// Sequential trial T0 of the Ewald form for :SUN, workgroup r = replica r
+// (its first lane only): the draws of the :dipole form (mc_seq_draw; normals
+// from the counters (T0 lo, kMcTagSequential + 1 + 2 b, T0 hi, 0)), the
+// energy change of the ket plus ewald_energy_delta of its expected dipole
+// (Ewald.jl:234-241).  tr[r] records an accepted move for
+// k_mc_ewald_update; acc[4 r + q] += (dE, dS).
+template <typename T, int N>
+__global__ __launch_bounds__(64) void k_sun_mc_ewald_trial(Couplings<T> C, McArgs A, T* __restrict__ Z,
+                                                           const double* __restrict__ phi,
+                                                           const double* __restrict__ A0, unsigned long long T0,
+                                                           McTrial* __restrict__ tr, double* __restrict__ acc,
+                                                           int nrep)
+{
+    const int r = blockIdx.x;
+    if (threadIdx.x != 0 || r >= nrep) return;
+    const ReplicaInfo R = C.reps[r];
+    const uint32_t ns = (uint32_t)R.ncell * (uint32_t)C.na;
+    const unsigned long long key = A.seed + (unsigned long long)r * 0x9E3779B97F4A7C15ull;
+    const McSeqDraw dr = mc_seq_draw(key, T0, ns);
+    const int i = (int)(dr.site / (uint32_t)R.ncell);
+    const int c = (int)(dr.site - (uint32_t)i * (uint32_t)R.ncell);
+    const int c1 = c % R.L1, c2 = (c / R.L1) % R.L2, c3 = c / (R.L1 * R.L2);
+    SiteCoord sc = mc_site(R, C.G, i, c1, c2, c3, true);
+    sc.r = r;
+    const Ket<T, N> z0 = load_ket<T, N>(Z, C.total, sc.g);
+    const uint32_t cn[4] = {(uint32_t)T0, kMcTagSequential + 1u, (uint32_t)(T0 >> 32), 0u};
+    const Ket<T, N> zn = sun_mc_propose<T, N>(A, mc_pick(A, dr.u_mix), z0, ldc(C.kappa, i), cn, key);
+    double dS[3];
+    double dE = sun_mc_local_delta<T, N, sun_mc_gather<T, N>()>(C, A, Z, sc, z0, zn, A.wide != 0, dS);
+    double dmu[3];
+    dE += mc_ewald_delta(C, i, phi, sc.g, A0, dS, dmu);
+    const double kT = A.kT[r];
+    const bool accept = kT == 0.0 ? dE <= 0.0 : dr.u_acc <= exp(-dE / kT);
+    McTrial out{};
+    if (accept) {
+        store_ket_images<T, N>(Z, C.total, sc, C.G, zn);
+        out = {1, i, c1, c2, c3, 0, {dmu[0], dmu[1], dmu[2]}};
+        acc[4 * r] += dE;
+        for (int a = 0; a < 3; a++) acc[4 * r + 1 + a] += dS[a];
+    }
+    tr[r] = out;
 }
 
 // ---------------------------------------------------------------------------
@@ -905,7 +984,17 @@ void launch_sun_mc_color(const Couplings<T>& C, const McArgs& A, T* Z, const McW
     SNH_SUN_DISPATCH(C.N, k_sun_mc_color, dim3(nwork), dim3(kBlock), 0, st, C, A, Z, work, P, prob, acc)
 }
 
+template <typename T>
+void launch_sun_mc_ewald_trial(const Couplings<T>& C, const McArgs& A, T* Z, const double* phi, const double* A0,
+                               unsigned long long T0, McTrial* tr, double* acc, int R, hipStream_t st)
+{
+    SNH_SUN_DISPATCH(C.N, k_sun_mc_ewald_trial, dim3(R), dim3(64), 0, st, C, A, Z, phi, A0, T0, tr, acc, R)
+}
+
 #define SNH_SUN_INST(T)                                                                                          \
+    template void launch_sun_mc_ewald_trial<T>(const Couplings<T>&, const McArgs&, T*, const double*,           \
+                                               const double*, unsigned long long, McTrial*, double*, int,        \
+                                               hipStream_t);                                                     \
     template void launch_sun_mc_color<T>(const Couplings<T>&, const McArgs&, T*, const McWork*, int,            \
                                          unsigned long long, double, double*, hipStream_t);                      \
     template void launch_sun_expected_spin<T>(const Couplings<T>&, int, const T*, T*, hipStream_t);              \

@@ -171,6 +171,31 @@ def test_ewald_sequential_trials_match_oracle(precision):
             assert abs(sh.energy(s) - E0 - sampler.dE) <= 10 * TOL[precision] * scale, (name, pname)
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_sun_ewald_sequential_trials_match_oracle(precision):
+    """:SUN with Ewald: the device's sequential trials (k_sun_mc_ewald_trial)
+    against the oracle's so_local_sequential_sun with the same draws."""
+    from test_oracle_local import sun_dipolar_system
+    s = sun_dipolar_system(seed=26)
+    s.precision = precision
+    kap = s.spec()["kappa"][0]
+    for pname in (["uniform", "mix"] if precision == 64 else ["flip"]):
+        prop = PROPOSALS[pname]
+        announce(f"local sequential ewald SU(3) cubic 2x2x3 {pname} fp{precision}")
+        Z0 = s.coherents_flat()
+        E0 = sh.energy(s)
+        key = O.replica_key(s.noise_seed, 0)
+        Z_ref, acc = O.Oracle(s.spec()).local_sequential_sun(Z0, oracle_props(prop), 0.6, key, s.step_counter, 2.0)
+        sampler = sh.LocalSampler(kT=0.6, nsweeps=2.0, propose=prop)
+        sh.step_(s, sampler)
+        Z_dev = s.coherents_flat()
+        assert np.max(np.abs(Z_dev - Z_ref)) <= TOL[precision] * np.sqrt(kap) * 4, pname
+        assert np.sum(np.any(np.abs(Z_dev - Z0) > 1e-9, axis=1)) > 0, pname
+        scale = energy_scale(s)
+        assert abs(sampler.dE - acc[0]) <= 10 * TOL[precision] * scale, (pname, sampler.dE, acc[0])
+        assert abs(sh.energy(s) - E0 - sampler.dE) <= 10 * TOL[precision] * scale, pname
+
+
 def test_ewald_sequential_batch_matches_oracle():
     systems = [sh.workloads.pyrochlore_model(dims=(2, 2, 1), seed=40 + r, precision=64, dipolar=True)
                for r in range(2)]

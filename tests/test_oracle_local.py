@@ -218,6 +218,46 @@ def test_sun_local_energy_change_is_energy_difference():
             assert abs(dE - (o.energy(Z=Z2) - E0)) <= 1e-11 * max(abs(E0), 1.0), site
 
 
+def sun_dipolar_system(seed=5, dims=(2, 2, 3)):
+    """SU(3) on a cubic lattice with J1, an easy axis and the Ewald term acting
+    on expected dipoles (axis of length 3: offsets differ from their
+    negatives)."""
+    s = sh.System(sh.cubic_crystal(1.0), [(0, sh.Moment(s=1, g=2))], "SUN", dims=dims, seed=seed)
+    sh.set_exchange_(s, 0.3, sh.Bond(0, 0, (1, 0, 0)))
+    sh.set_onsite_coupling_(s, lambda S: -0.2 * S[2] @ S[2], 0)
+    sh.enable_dipole_dipole_(s, 0.5)
+    sh.randomize_spins_(s)
+    return s
+
+
+def test_sun_ewald_local_energy_change_and_sequential_trials():
+    """:SUN with Ewald: local_energy_change equals differences of the pinned
+    total energy; sequential trials accumulate the Ewald-inclusive energy
+    change and the change of the summed expected dipoles; kT = 0 never raises
+    the energy; every site of 12 is tried over 40 sweeps."""
+    s = sun_dipolar_system()
+    o = O.Oracle(s.spec())
+    Z = s.coherents_flat()
+    E0 = o.energy(Z=Z)
+    kap = s.spec()["kappa"][0]
+    rng = np.random.default_rng(10)
+    for site in rng.choice(s.nsites, 5, replace=False):
+        for Zn in kets_for(Z[site], rng, kap):
+            Z2 = Z.copy()
+            Z2[site] = Zn
+            assert abs(o.local_energy_change_sun(Z, site, Zn) - (o.energy(Z=Z2) - E0)) <= 1e-11 * max(abs(E0), 1)
+    props = [(FLIP, 0, 0.3), (DELTA, 0.4, 0.5), (UNIFORM, 0, 0.2)]
+    Z2, acc = o.local_sequential_sun(Z, props, 0.5, 77, 3, 4.0)
+    assert np.sum(np.any(np.abs(Z2 - Z) > 1e-12, axis=1)) > s.nsites // 4
+    np.testing.assert_allclose(acc[0], o.energy(Z=Z2) - E0, atol=1e-10)
+    np.testing.assert_allclose(acc[1:], o.expected_spin(Z2).sum(0) - o.expected_spin(Z).sum(0), atol=1e-12)
+    np.testing.assert_allclose(np.linalg.norm(Z2, axis=1) ** 2, kap, rtol=1e-13)
+    Z3, acc3 = o.local_sequential_sun(Z, props, 0.0, 5, 0, 3.0)
+    assert acc3[0] <= 0 and o.energy(Z=Z3) <= E0 + 1e-12
+    Z4, _ = o.local_sequential_sun(Z, [(UNIFORM, 0, 1)], 1e9, 8, 0, 40.0)
+    assert np.all(np.any(np.abs(Z4 - Z) > 1e-12, axis=1))
+
+
 def test_sun_flip_reverses_expected_spin():
     """Time reversal (flip_ket) negates <S>: one pass at a huge kT accepts every
     flip it tries, so every site is either unchanged or reversed."""
