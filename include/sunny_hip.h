@@ -155,7 +155,8 @@ int snh_energy(snh_system *sys, double *E);
  * whose energy launch needs more partial sums than that gets
  * SNH_ERR_UNSUPPORTED (snh_energy still serves it).  Every call on one GPU
  * uses that GPU's array: calls that overlap on one GPU (handles on different
- * streams) race. */
+ * streams) race.  A handle whose Ewald grid is transformed by hipFFT (see
+ * snh_query "ewald_fft_lib") also gets SNH_ERR_UNSUPPORTED. */
 #define SNH_NOALLOC_PARTIALS 262144
 int snh_energy_noalloc(snh_system *sys, double *E_dev);
 
@@ -313,7 +314,11 @@ int snh_autotune(snh_system *sys, int32_t candidates, int32_t launches, double *
  * streams in anti-phase: chosen by snh_autotune when whole steps are
  * measurably faster that way, or forced by SNH_PAIR=2). "mid_reruns"
  * (midpoint step groups undone after overrunning their iteration budget),
- * "mid_fallbacks" (groups then rerun with a host check per step). */
+ * "mid_fallbacks" (groups then rerun with a host check per step),
+ * "ewald_fft_lib" (1 when the Ewald grid is transformed by batched hipFFT
+ * plans: grids whose transform does not fit one workgroup's LDS, an axis
+ * longer than 32 cells, or the environment variable SNH_EWALD_FFT=lib at
+ * snh_create; 0 for one workgroup per field). */
 int snh_query(snh_system *sys, const char *key, int64_t *value);
 
 #ifdef __cplusplus

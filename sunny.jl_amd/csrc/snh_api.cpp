@@ -515,6 +515,7 @@ struct SystemImpl final : snh_system {
     EwaldAParams ewP{};          // parameters of the real-space tensor
     std::vector<double> ew_pos;  // fractional positions [na][3]
     void* M = nullptr;
+    EwaldLibFft* ewlib = nullptr;  // hipFFT plans of the library FFT path
     void* Fmu = nullptr;
     void* Fphi = nullptr;
     T* phi1 = nullptr;
@@ -549,6 +550,7 @@ struct SystemImpl final : snh_system {
         for (void* p : corr.mem) (void)hipFree(p);
         timer.clear();
         for (T* p : ckpt) (void)hipFree(p);
+        ewald_lib_destroy(ewlib);
         for (void* p : owned) (void)hipFree(p);
         if (pinned) (void)hipHostFree(pinned);
         if (pinned_i) (void)hipHostFree(pinned_i);
@@ -920,8 +922,6 @@ struct SystemImpl final : snh_system {
                         SNH_ERR_UNSUPPORTED, "Ewald requires all replicas to share the same dims");
         SNH_REQUIRE(sp.positions, SNH_ERR_INVALID, "Ewald needs atom positions");
         const int L[3] = {reps[0].L1, reps[0].L2, reps[0].L3};
-        for (int a = 0; a < 3; a++)
-            SNH_REQUIRE(L[a] <= 32, SNH_ERR_UNSUPPORTED, "Ewald FFT supports at most 32 cells per axis");
         EG.L1 = L[0];
         EG.L2 = L[1];
         EG.L3 = L[2];
@@ -941,9 +941,13 @@ struct SystemImpl final : snh_system {
             hipDeviceProp_t prop;
             SNH_HIP(hipGetDeviceProperties(&prop, dev));
             const size_t lim = prop.sharedMemPerBlock;
-            // the FP64 transform of the tensor A is the largest FFT workgroup
-            SNH_REQUIRE(ewald_fft_lds_bytes(EG, sizeof(double)) <= lim, SNH_ERR_UNSUPPORTED,
-                        "Ewald FFT grid too large for one workgroup's LDS");
+            // one workgroup per field transforms grids whose FP64 transform of
+            // the tensor A (the largest) fits its LDS; larger grids, or any
+            // under SNH_EWALD_FFT=lib, go through hipFFT
+            const bool fits = L[0] <= kLmax && L[1] <= kLmax && L[2] <= kLmax &&
+                              ewald_fft_lds_bytes(EG, sizeof(double)) <= lim;
+            const char* mode = std::getenv("SNH_EWALD_FFT");
+            EG.plain = !fits || (mode && std::string(mode) == "lib") ? 1 : 0;
             const size_t lc = ewald_use_mfma(EG, R, sizeof(T)) ? ewald_mfma_lds_bytes(EG)
                                                                : (size_t)R * 3 * na * sizeof(T) * 2;
             SNH_REQUIRE(lc <= lim, SNH_ERR_UNSUPPORTED, "too many replicas x sublattices for the Ewald contraction");
@@ -992,11 +996,20 @@ struct SystemImpl final : snh_system {
         const int n3 = 3 * na;
         M = own(dalloc<char>((size_t)EG.nk * n3 * n3 * 2 * sizeof(T)));
         launch_ewald_A(P, d_pos, d_A, st);
-        launch_ewald_fwdA<T>(EG, d_A, M, st);
+        std::string err;
+        if (EG.plain)
+            err = launch_ewald_fwdA_plain<T>(EG, d_A, M, st);
+        else
+            launch_ewald_fwdA<T>(EG, d_A, M, st);
         SNH_HIP(hipGetLastError());
         SNH_HIP(hipStreamSynchronize(st));
         SNH_HIP(hipFree(d_A));
         SNH_HIP(hipFree(d_pos));
+        SNH_REQUIRE(err.empty(), SNH_ERR_HIP, "Ewald tensor: " + err);
+        if (EG.plain) {
+            ewlib = ewald_lib_create(EG, R, sizeof(T), err);
+            SNH_REQUIRE(ewlib, SNH_ERR_HIP, "Ewald FFT: " + err);
+        }
         Fmu = own(dalloc<char>(ewald_spectrum_bytes(EG, R, sizeof(T))));
         Fphi = own(dalloc<char>(ewald_spectrum_bytes(EG, R, sizeof(T))));
         phi1 = own(dalloc<T>(3 * (size_t)total));
@@ -1030,7 +1043,8 @@ struct SystemImpl final : snh_system {
             S = Sd;
         }
         hipEvent_t e = timed ? tbegin() : nullptr;
-        launch_ewald_field<T>(EG, R, C.reps, total, C.gmat, S, M, Fmu, Fphi, phi, st);
+        SNH_REQUIRE(launch_ewald_field<T>(EG, R, C.reps, total, C.gmat, S, M, Fmu, Fphi, phi, ewlib, st), SNH_ERR_HIP,
+                    "hipFFT execution failed");
         tend("ewald_field", e);
     }
 
@@ -2154,6 +2168,8 @@ struct SystemImpl final : snh_system {
                     "snh_energy_noalloc: the energy launch needs " + std::to_string(need) +
                         " partial sums, more than the static scratch holds (" + std::to_string(kNoAllocPartials) +
                         "); use snh_energy");
+        SNH_REQUIRE(!(ewald && EG.plain), SNH_ERR_UNSUPPORTED,
+                    "snh_energy_noalloc: the Ewald grid is transformed by hipFFT; use snh_energy");
         double* part = noalloc_scratch(dev);
         SNH_REQUIRE(part, SNH_ERR_HIP, "static scratch not resolved on this device");
         energy_launch(part, E_dev, false);
@@ -2223,6 +2239,7 @@ struct SystemImpl final : snh_system {
         if (key == "pair_on") return pair_on ? 1 : 0;
         if (key == "mid_reruns") return mid_reruns;
         if (key == "mid_fallbacks") return mid_fallbacks;
+        if (key == "ewald_fft_lib") return ewald && EG.plain ? 1 : 0;
         throw Failure{SNH_ERR_INVALID, "unknown query key: " + key};
     }
 

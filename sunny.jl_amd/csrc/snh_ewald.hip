@@ -27,7 +27,16 @@
 // [k / kSpecKB][replica][field][k % kSpecKB]: each transform workgroup reads
 // and writes whole 128-byte runs, and the contraction of a block for a group
 // of replicas reads one compact range.
+//
+// Grids whose one-field transform does not fit one workgroup's LDS (or with
+// an axis longer than kLmax) take the library path (G.plain): the fields are
+// gathered into [replica][field][cell] real arrays, transformed by batched
+// 3D hipFFT plans into plain spectra [replica][field][k], contracted by the
+// same kernels and transformed back; the reference transforms with FFTW
+// (Ewald.jl:2-22, 185-209) and has no size bound either.
 #include "snh_fft.h"
+
+#include <hipfft/hipfft.h>
 
 #include <cstdlib>
 #include <map>
@@ -105,6 +114,7 @@ __device__ __forceinline__ void inv3d(const EwaldGeom& G, const FftLds<T>& F)
 // Position of (replica r, field f, wavevector k) in the blocked spectra.
 __device__ __forceinline__ long long spec_index(const EwaldGeom& G, int R, int r, int f, long long k)
 {
+    if (G.plain) return ((long long)r * (3 * G.na) + f) * G.nk + k;
     return (((k / kSpecKB) * R + r) * (3 * G.na) + f) * kSpecKB + (k % kSpecKB);
 }
 
@@ -354,6 +364,70 @@ __global__ __launch_bounds__(kFftThreads) void k_ewald_inv(EwaldGeom G, int R, c
     }
 }
 
+// ---------------------------------------------------------------------------
+// Library path: elementwise gather and scatter around the hipFFT transforms.
+
+// fld[r][f][cell] = mu = -g S of field f = 3 j + beta.  grid = (cells / 256, 3 na, R).
+template <typename T>
+__global__ __launch_bounds__(256) void k_ewald_gather(EwaldGeom G, const ReplicaInfo* __restrict__ reps,
+                                                      uint32_t total, const T* __restrict__ gmat,
+                                                      const T* __restrict__ S, T* __restrict__ fld)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int f = blockIdx.y, r = blockIdx.z;
+    if (c >= G.ncell) return;
+    const int j = f / 3, beta = f % 3;
+    const int c1 = (int)(c % G.L1);
+    const long long l = c / G.L1;
+    const uint32_t g = ewald_site(G, reps[r].off, j, c1, (int)(l % G.L2), (int)(l / G.L2));
+    const T g0 = gmat[9 * j + 3 * beta], g1 = gmat[9 * j + 3 * beta + 1], g2 = gmat[9 * j + 3 * beta + 2];
+    fld[((long long)r * 3 * G.na + f) * G.ncell + c] =
+        -(g0 * ldg(S, g) + g1 * ldg(S + total, g) + g2 * ldg(S + 2 * (size_t)total, g));
+}
+
+// phi plane alpha of sublattice i (field 3 i + alpha) = fld / Ncell.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ewald_scatter(EwaldGeom G, const ReplicaInfo* __restrict__ reps,
+                                                       uint32_t total, const T* __restrict__ fld, T* __restrict__ phi)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int f = blockIdx.y, r = blockIdx.z;
+    if (c >= G.ncell) return;
+    const int i = f / 3, alpha = f % 3;
+    const int c1 = (int)(c % G.L1);
+    const long long l = c / G.L1;
+    const T scale = (T)(1.0 / (double)G.ncell);
+    stg(phi + (size_t)alpha * total, ewald_site(G, reps[r].off, i, c1, (int)(l % G.L2), (int)(l / G.L2)),
+        fld[((long long)r * 3 * G.na + f) * G.ncell + c] * scale);
+}
+
+// The fields of source sublattice j of A [j][i][cell][3][3]:
+// fld[(i 3 + alpha) 3 + beta][cell].  grid = (cells / 256, 9 na).
+__global__ __launch_bounds__(256) void k_ewald_A_fields(EwaldGeom G, const double* __restrict__ A, int j,
+                                                        double* __restrict__ fld)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int q = blockIdx.y;  // (i 3 + alpha) 3 + beta
+    if (c >= G.ncell) return;
+    const int i = q / 9, ab = q % 9;
+    fld[(long long)q * G.ncell + c] = A[(((long long)j * G.na + i) * G.ncell + c) * 9 + ab];
+}
+
+// M[(k n3 + col) n3 + row] = conj(spectrum of field (j, i, alpha, beta)) with
+// col = 3 j + beta, row = 3 i + alpha, from spec[(i 3 + alpha) 3 + beta][k].
+template <typename T>
+__global__ __launch_bounds__(256) void k_ewald_M_from_spectra(EwaldGeom G, const C2<double>* __restrict__ spec, int j,
+                                                              C2<T>* __restrict__ M)
+{
+    const long long k = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int q = blockIdx.y;
+    if (k >= G.nk) return;
+    const int i = q / 9, alpha = (q / 3) % 3, beta = q % 3;
+    const int n3 = 3 * G.na;
+    const C2<double> v = spec[(long long)q * G.nk + k];
+    M[(k * n3 + 3 * j + beta) * n3 + 3 * i + alpha] = {(T)v.re, (T)(-v.im)};
+}
+
 // Real-space Ewald tensor A[off, i, j] (FP64), scaled by mu0_muB2.
 // Thread per (cell offset, i); grid.y = j.  Layout A[j][i][cell][3][3].
 
@@ -456,15 +530,11 @@ void launch_ewald_fwdA(const EwaldGeom& G, const double* A, void* M, hipStream_t
                        reinterpret_cast<C2<T>*>(M));
 }
 
+// F[phi] = M F[mu]: matrix cores for FP32 batches, else one matrix-vector
+// product per wavevector.
 template <typename T>
-void launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint32_t total, const T* gmat,
-                        const T* S, const void* M, void* Fmu, void* Fphi, T* phi, hipStream_t st)
+static void ewald_contract(const EwaldGeom& G, int R, const void* M, void* Fmu, void* Fphi, hipStream_t st)
 {
-    const size_t lds = ewald_fft_lds_bytes(G, sizeof(T));
-    allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_fwd<T>), lds);
-    allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_inv<T>), lds);
-    hipLaunchKernelGGL(k_ewald_fwd<T>, dim3(3 * G.na, R), dim3(kFftThreads), lds, st, G, R, reps, total, gmat, S,
-                       reinterpret_cast<C2<T>*>(Fmu));
     if constexpr (sizeof(T) == sizeof(float)) {
         if (ewald_use_mfma(G, R, sizeof(T))) {
             const size_t lm = ewald_mfma_lds_bytes(G);
@@ -498,8 +568,6 @@ void launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint
             }
 #undef SNH_EWM_NC
 #undef SNH_EWM
-            hipLaunchKernelGGL(k_ewald_inv<T>, dim3(3 * G.na, R), dim3(kFftThreads), lds, st, G, R, reps, total,
-                               reinterpret_cast<const C2<T>*>(Fphi), phi);
             return;
         }
     }
@@ -508,15 +576,170 @@ void launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint
     hipLaunchKernelGGL(k_ewald_contract<T>, dim3((unsigned)G.nk), dim3(256), lds2, st, G, R,
                        reinterpret_cast<const C2<T>*>(M), reinterpret_cast<const C2<T>*>(Fmu),
                        reinterpret_cast<C2<T>*>(Fphi));
+}
+
+// Batched 3D plans of the library path, their work area and the real field
+// buffer (mu, then phi).
+struct EwaldLibFft {
+    hipfftHandle fwd = 0, inv = 0;
+    void* work = nullptr;
+    void* fld = nullptr;
+};
+
+// Creates a plan of `batch` 3D transforms over contiguous grids (real
+// [L3][L2][L1], half spectrum [L3][L2][H1]) with auto-allocation off; its
+// work size goes to *work.
+static hipfftResult ewald_plan(hipfftHandle* h, const EwaldGeom& G, int batch, hipfftType type, size_t* work)
+{
+    int n[3] = {G.L3, G.L2, G.L1};
+    int re[3] = {G.L3, G.L2, G.L1};
+    int cx[3] = {G.L3, G.L2, G.H1};
+    const bool r2c = type == HIPFFT_R2C || type == HIPFFT_D2Z;
+    hipfftResult e = hipfftCreate(h);
+    if (e != HIPFFT_SUCCESS) return e;
+    e = hipfftSetAutoAllocation(*h, 0);
+    if (e != HIPFFT_SUCCESS) return e;
+    return hipfftMakePlanMany(*h, 3, n, r2c ? re : cx, 1, r2c ? (int)G.ncell : (int)G.nk, r2c ? cx : re, 1,
+                              r2c ? (int)G.nk : (int)G.ncell, type, batch, work);
+}
+
+void ewald_lib_destroy(EwaldLibFft* L)
+{
+    if (!L) return;
+    if (L->fwd) (void)hipfftDestroy(L->fwd);
+    if (L->inv) (void)hipfftDestroy(L->inv);
+    if (L->work) (void)hipFree(L->work);
+    if (L->fld) (void)hipFree(L->fld);
+    delete L;
+}
+
+EwaldLibFft* ewald_lib_create(const EwaldGeom& G, int R, size_t elem, std::string& err)
+{
+    EwaldLibFft* L = new EwaldLibFft;
+    const bool f32 = elem == sizeof(float);
+    const int batch = R * 3 * G.na;
+    size_t w0 = 0, w1 = 0;
+    hipfftResult e = ewald_plan(&L->fwd, G, batch, f32 ? HIPFFT_R2C : HIPFFT_D2Z, &w0);
+    if (e == HIPFFT_SUCCESS) e = ewald_plan(&L->inv, G, batch, f32 ? HIPFFT_C2R : HIPFFT_Z2D, &w1);
+    if (e != HIPFFT_SUCCESS) {
+        err = "hipFFT plan creation failed (code " + std::to_string((int)e) + ")";
+        ewald_lib_destroy(L);
+        return nullptr;
+    }
+    const size_t w = std::max<size_t>(std::max(w0, w1), 256);
+    if (hipMalloc(&L->work, w) != hipSuccess || hipMalloc(&L->fld, (size_t)batch * G.ncell * elem) != hipSuccess) {
+        err = "out of device memory for the Ewald FFT buffers";
+        ewald_lib_destroy(L);
+        return nullptr;
+    }
+    if (hipfftSetWorkArea(L->fwd, L->work) != HIPFFT_SUCCESS || hipfftSetWorkArea(L->inv, L->work) != HIPFFT_SUCCESS) {
+        err = "hipFFT work area not accepted";
+        ewald_lib_destroy(L);
+        return nullptr;
+    }
+    return L;
+}
+
+// A -> M by the library: per source sublattice j, 9 na FP64 fields
+// transformed in one batch.  Temporary buffers are freed before returning.
+template <typename T>
+static std::string ewald_fwdA_lib(const EwaldGeom& G, const double* A, void* M, hipStream_t st)
+{
+    const int nf = 9 * G.na;
+    hipfftHandle h = 0;
+    size_t w = 0;
+    double* fld = nullptr;
+    C2<double>* spec = nullptr;
+    void* work = nullptr;
+    std::string err;
+    hipfftResult e = ewald_plan(&h, G, nf, HIPFFT_D2Z, &w);
+    if (e != HIPFFT_SUCCESS) err = "hipFFT plan creation failed (code " + std::to_string((int)e) + ")";
+    if (err.empty() && (hipMalloc(&fld, sizeof(double) * nf * G.ncell) != hipSuccess ||
+                        hipMalloc(&spec, sizeof(C2<double>) * nf * G.nk) != hipSuccess ||
+                        hipMalloc(&work, std::max<size_t>(w, 256)) != hipSuccess))
+        err = "out of device memory for the Ewald tensor transform";
+    if (err.empty() && (hipfftSetWorkArea(h, work) != HIPFFT_SUCCESS || hipfftSetStream(h, st) != HIPFFT_SUCCESS))
+        err = "hipFFT work area or stream not accepted";
+    const dim3 gc((unsigned)((G.ncell + 255) / 256), nf), gk((unsigned)((G.nk + 255) / 256), nf);
+    for (int j = 0; err.empty() && j < G.na; j++) {
+        hipLaunchKernelGGL(k_ewald_A_fields, gc, dim3(256), 0, st, G, A, j, fld);
+        if (hipfftExecD2Z(h, reinterpret_cast<hipfftDoubleReal*>(fld), reinterpret_cast<hipfftDoubleComplex*>(spec)) !=
+            HIPFFT_SUCCESS)
+            err = "hipFFT execution failed";
+        else
+            hipLaunchKernelGGL((k_ewald_M_from_spectra<T>), gk, dim3(256), 0, st, G, spec, j,
+                               reinterpret_cast<C2<T>*>(M));
+    }
+    if (hipStreamSynchronize(st) != hipSuccess && err.empty()) err = "Ewald tensor transform failed";
+    if (h) (void)hipfftDestroy(h);
+    if (fld) (void)hipFree(fld);
+    if (spec) (void)hipFree(spec);
+    if (work) (void)hipFree(work);
+    return err;
+}
+
+template <typename T>
+std::string launch_ewald_fwdA_plain(const EwaldGeom& G, const double* A, void* M, hipStream_t st)
+{
+    return ewald_fwdA_lib<T>(G, A, M, st);
+}
+
+// Forward transform of every field, contraction, inverse transform.
+template <typename T>
+static bool ewald_transforms_lib(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint32_t total, const T* gmat,
+                                 const T* S, EwaldLibFft* L, void* Fmu, void* Fphi, hipStream_t st, bool fwd)
+{
+    T* fld = reinterpret_cast<T*>(L->fld);
+    const dim3 grid((unsigned)((G.ncell + 255) / 256), 3 * G.na, R);
+    if (fwd) {
+        hipLaunchKernelGGL(k_ewald_gather<T>, grid, dim3(256), 0, st, G, reps, total, gmat, S, fld);
+        if (hipfftSetStream(L->fwd, st) != HIPFFT_SUCCESS) return false;
+        if constexpr (sizeof(T) == sizeof(float))
+            return hipfftExecR2C(L->fwd, fld, reinterpret_cast<hipfftComplex*>(Fmu)) == HIPFFT_SUCCESS;
+        else
+            return hipfftExecD2Z(L->fwd, fld, reinterpret_cast<hipfftDoubleComplex*>(Fmu)) == HIPFFT_SUCCESS;
+    }
+    if (hipfftSetStream(L->inv, st) != HIPFFT_SUCCESS) return false;
+    bool ok;
+    if constexpr (sizeof(T) == sizeof(float))
+        ok = hipfftExecC2R(L->inv, reinterpret_cast<hipfftComplex*>(Fphi), fld) == HIPFFT_SUCCESS;
+    else
+        ok = hipfftExecZ2D(L->inv, reinterpret_cast<hipfftDoubleComplex*>(Fphi), fld) == HIPFFT_SUCCESS;
+    return ok;
+}
+
+template <typename T>
+bool launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint32_t total, const T* gmat,
+                        const T* S, const void* M, void* Fmu, void* Fphi, T* phi, EwaldLibFft* lib, hipStream_t st)
+{
+    if (G.plain) {
+        if (!ewald_transforms_lib<T>(G, R, reps, total, gmat, S, lib, Fmu, Fphi, st, true)) return false;
+        ewald_contract<T>(G, R, M, Fmu, Fphi, st);
+        if (!ewald_transforms_lib<T>(G, R, reps, total, gmat, S, lib, Fmu, Fphi, st, false)) return false;
+        const dim3 grid((unsigned)((G.ncell + 255) / 256), 3 * G.na, R);
+        hipLaunchKernelGGL(k_ewald_scatter<T>, grid, dim3(256), 0, st, G, reps, total,
+                           reinterpret_cast<const T*>(lib->fld), phi);
+        return true;
+    }
+    const size_t lds = ewald_fft_lds_bytes(G, sizeof(T));
+    allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_fwd<T>), lds);
+    hipLaunchKernelGGL(k_ewald_fwd<T>, dim3(3 * G.na, R), dim3(kFftThreads), lds, st, G, R, reps, total, gmat, S,
+                       reinterpret_cast<C2<T>*>(Fmu));
+    ewald_contract<T>(G, R, M, Fmu, Fphi, st);
+    allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_inv<T>), lds);
     hipLaunchKernelGGL(k_ewald_inv<T>, dim3(3 * G.na, R), dim3(kFftThreads), lds, st, G, R, reps, total,
                        reinterpret_cast<const C2<T>*>(Fphi), phi);
+    return true;
 }
 
 template void launch_ewald_fwdA<float>(const EwaldGeom&, const double*, void*, hipStream_t);
 template void launch_ewald_fwdA<double>(const EwaldGeom&, const double*, void*, hipStream_t);
-template void launch_ewald_field<float>(const EwaldGeom&, int, const ReplicaInfo*, uint32_t, const float*,
-                                        const float*, const void*, void*, void*, float*, hipStream_t);
-template void launch_ewald_field<double>(const EwaldGeom&, int, const ReplicaInfo*, uint32_t, const double*,
-                                         const double*, const void*, void*, void*, double*, hipStream_t);
+template std::string launch_ewald_fwdA_plain<float>(const EwaldGeom&, const double*, void*, hipStream_t);
+template std::string launch_ewald_fwdA_plain<double>(const EwaldGeom&, const double*, void*, hipStream_t);
+template bool launch_ewald_field<float>(const EwaldGeom&, int, const ReplicaInfo*, uint32_t, const float*,
+                                        const float*, const void*, void*, void*, float*, EwaldLibFft*, hipStream_t);
+template bool launch_ewald_field<double>(const EwaldGeom&, int, const ReplicaInfo*, uint32_t, const double*,
+                                         const double*, const void*, void*, void*, double*, EwaldLibFft*,
+                                         hipStream_t);
 
 }  // namespace snh

@@ -6,8 +6,6 @@
 
 namespace snh {
 
-constexpr int kLmax = 32;  // longest supported axis of the FFT grid
-
 template <typename T>
 struct C2 {
     T re, im;

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "../../include/sunny_hip.h"
 #include "snh_device.h"
 
@@ -115,12 +117,15 @@ struct TileArgs {
 };
 
 // Geometry of the Ewald FFT grid (one replica; all replicas share it).
+constexpr int kLmax = 32;  // longest axis of the in-register line transforms
+
 struct EwaldGeom {
     int L1, L2, L3, H1, na;  // H1 = L1/2 + 1 (half spectrum along axis 1)
     long long ncell, nk;     // nk = H1 * L2 * L3
     int P1, P2;              // padded dims of the state layout
     uint32_t pcell;
     int w1, w2, w3;
+    int plain;  // 1: library FFT path, spectra [replica][field][k]
 };
 
 // Parameters of the real-space Ewald tensor precompute (FP64).
@@ -304,9 +309,18 @@ size_t ewald_spectrum_bytes(const EwaldGeom& G, int R, size_t elem);
 void launch_ewald_A(const EwaldAParams& P, const double* pos, double* A, hipStream_t st);
 template <typename T>
 void launch_ewald_fwdA(const EwaldGeom& G, const double* A, void* M, hipStream_t st);
+// The library FFT path (G.plain): hipFFT plans of R replicas' fields and
+// their buffers; create returns null with err set on failure.
+struct EwaldLibFft;
+EwaldLibFft* ewald_lib_create(const EwaldGeom& G, int R, size_t elem, std::string& err);
+void ewald_lib_destroy(EwaldLibFft* L);
+// A -> M through the library (returns an error text, empty on success).
 template <typename T>
-void launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint32_t total, const T* gmat,
-                        const T* S, const void* M, void* Fmu, void* Fphi, T* phi, hipStream_t st);
+std::string launch_ewald_fwdA_plain(const EwaldGeom& G, const double* A, void* M, hipStream_t st);
+// Potential phi of state S; false when a library transform fails.
+template <typename T>
+bool launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint32_t total, const T* gmat,
+                        const T* S, const void* M, void* Fmu, void* Fphi, T* phi, EwaldLibFft* lib, hipStream_t st);
 
 // :SUN mode (snh_sun.hip).  Buffers are plane sets [2N][total]; expected
 // dipoles are [3][total] planes with ghost images.

@@ -29,6 +29,19 @@ def test_single_dipole_cubic_box(dims):
     assert abs(sh.energy_per_site(s32) - (-1 / 6)) < 1e-6
 
 
+@pytest.mark.parametrize("dims", [(24, 24, 24), (40, 6, 6), (64, 4, 2)])
+def test_single_dipole_large_grids(dims):
+    """Grids whose transform does not fit one workgroup's LDS, or with an
+    axis longer than 32, take the hipFFT path: E per site = -1/6 for parallel
+    dipoles, demag = 0 (test/test_ewald.jl:16-31)."""
+    announce(f"ewald single dipole {dims} (library FFT)")
+    s = single_dipole(dims)
+    assert abs(sh.energy_per_site(s) - (-1 / 6)) < 1e-12
+    assert s._device().query("ewald_fft_lib") == 1
+    s32 = single_dipole(dims, precision=32)
+    assert abs(sh.energy_per_site(s32) - (-1 / 6)) < 1e-6
+
+
 def random_triclinic(precision):
     """Three atoms in a skewed cell with random g tensors and demag
     (test/test_ewald.jl:33-57)."""
@@ -201,3 +214,73 @@ def test_ewald_batch_pyrochlore8_mfma():
         S = o.step_langevin(S=S, noise=O.philox_normals(key, 0, s.nsites, 3), dt=integ.dt, damping=integ.damping,
                             kT=integ.kT)
         assert rel_err(s.dipoles_flat(), S, scale=3.5) <= 1e-4, r
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_library_fft_path_matches_oracle(precision, monkeypatch):
+    """SNH_EWALD_FFT=lib sends small grids through the hipFFT path (gather,
+    batched 3D transforms, plain spectra, scatter): energies, gradients and
+    Langevin steps against the oracle, a :SUN system, and an FP32 batch of 9
+    replicas on the matrix-core contraction over plain spectra."""
+    monkeypatch.setenv("SNH_EWALD_FFT", "lib")
+    tol = 1e-11 if precision == 64 else 3e-5
+    models = [("triclinic", random_triclinic(precision)),
+              ("pyro3_dipolar", sh.workloads.pyrochlore_model(dims=(3, 2, 4), J1=0.05, J2=0, A=0, s=3.5, seed=5,
+                                                              dipolar=True, precision=precision))]
+    for name, s in models:
+        announce(f"ewald library FFT {name} fp{precision}")
+        assert s._device().query("ewald_fft_lib") == 1
+        S = s.dipoles_flat()
+        o = O.Oracle(s.spec())
+        g_ref = o.grad(S)
+        assert rel_err(sh.energy_grad_dipoles(s), g_ref) <= tol, name
+        scale = max(1.0, np.abs(g_ref).max() * np.abs(S).max() * s.nsites)
+        assert abs(sh.energy(s) - o.energy(S)) <= tol * scale, name
+        rng = np.random.default_rng(3)
+        nz = rng.standard_normal((s.nsites, 3))
+        S = o.step_langevin(S=S, noise=nz, dt=0.01, damping=0.1, kT=0.2)
+        sh.step_(s, sh.Langevin(0.01, damping=0.1, kT=0.2), noise=nz)
+        assert rel_err(s.dipoles_flat(), S, scale=3.5) <= tol * 3, name
+    cryst = sh.cubic_crystal(1.0)
+    u = sh.System(cryst, [(0, sh.Moment(s=1, g=2))], "SUN", dims=(2, 2, 3), seed=3, precision=precision)
+    sh.set_exchange_(u, 0.3, sh.Bond(0, 0, (1, 0, 0)))
+    sh.enable_dipole_dipole_(u, 0.5)
+    sh.randomize_spins_(u)
+    announce(f"ewald library FFT SU(3) fp{precision}")
+    Z = u.coherents_flat()
+    o = O.Oracle(u.spec())
+    assert rel_err(sh.energy_grad_coherents(u), o.grad_coherents(Z)) <= tol
+    assert abs(sh.energy(u) - o.energy(Z=Z)) <= tol * 10
+    if precision == 32:
+        R = 9
+        systems = [sh.workloads.pyrochlore_model(L=2, J1=0.05, J2=0, A=0, s=3.5, seed=60 + r, dipolar=True,
+                                                 precision=32) for r in range(R)]
+        announce("ewald library FFT batch pyro2 R=9 fp32")
+        batch = sh.ReplicaBatch(systems, precision=32)
+        assert batch._device().query("ewald_fft_lib") == 1
+        E_dev = sh.energy(batch)
+        for r in (0, 4, 8):
+            S = systems[r].dipoles_flat()
+            o = O.Oracle(systems[r].spec())
+            scale = max(1.0, np.abs(o.grad(S)).max() * np.abs(S).max() * systems[r].nsites)
+            assert abs(E_dev[r] - o.energy(S)) <= 3e-5 * scale, r
+
+
+def test_library_fft_matches_workgroup_fft():
+    """Dipolar pyrochlore 8^3 both ways at FP64 (one handle per path):
+    energies and gradients agree to rounding."""
+    import os
+    kw = dict(L=8, J1=0.05, J2=0, A=0, s=3.5, seed=17, dipolar=True, precision=64)
+    a = sh.workloads.pyrochlore_model(**kw)
+    announce("ewald pyrochlore 8^3 library vs workgroup FFT fp64")
+    Ea, ga = sh.energy(a), sh.energy_grad_dipoles(a)
+    assert a._device().query("ewald_fft_lib") == 0
+    os.environ["SNH_EWALD_FFT"] = "lib"
+    try:
+        b = sh.workloads.pyrochlore_model(**kw)
+        Eb, gb = sh.energy(b), sh.energy_grad_dipoles(b)
+        assert b._device().query("ewald_fft_lib") == 1
+    finally:
+        del os.environ["SNH_EWALD_FFT"]
+    assert abs(Ea - Eb) <= 1e-12 * abs(Ea)
+    assert rel_err(gb, ga) <= 1e-12

@@ -75,3 +75,14 @@ def test_noalloc_energy_rejects_null_output():
     s = sh.workloads.cubic_model(dims=(2, 2, 2), seed=98, precision=64)
     with pytest.raises(ValueError, match="null energy buffer"):
         s._device().call("snh_energy_noalloc", None)
+
+
+def test_noalloc_energy_refuses_library_fft_grids():
+    """An Ewald grid transformed by hipFFT (an axis longer than 32) is refused
+    before any launch: the library's plans are not covered by the no-allocation
+    promise."""
+    s = sh.System(sh.cubic_crystal(1.0), [(0, sh.Moment(s=1, g=1))], "dipole", dims=(40, 2, 2), precision=64)
+    sh.enable_dipole_dipole_(s, 1.0, demag=0.0)
+    assert s._device().query("ewald_fft_lib") == 1
+    with pytest.raises(ValueError, match="hipFFT"):
+        noalloc_energy(s)
