@@ -223,7 +223,10 @@ int snh_copy_calibration(snh_system *sys, int32_t repeats, int64_t *bytes);
  *   n_all_omega = 2 (n_omega - 1) time slots (even, <= 4096), measperiod
  *   integration steps per snapshot, window 1 (cosine) or 0 (rectangular).
  * The caller computes dt and measperiod as adjusted_dt_and_downsampling_factor
- * (SampledCorrelations.jl:152-173).  Lattice axes of at most 32 cells.
+ * (SampledCorrelations.jl:152-173).  Lattice axes of up to 32 cells are
+ * transformed by line transforms in registers, longer axes (or any under the
+ * environment variable SNH_CORR_FFT=lib) by a batched hipFFT plan made here
+ * (snh_query "corr_fft_lib").
  *
  * snh_corr_add_sample replaces add_sample!(sc, sys)
  * (src/SampledCorrelations/CorrelationSampling.jl:1-175): a trajectory of
@@ -318,7 +321,8 @@ int snh_autotune(snh_system *sys, int32_t candidates, int32_t launches, double *
  * "ewald_fft_lib" (1 when the Ewald grid is transformed by batched hipFFT
  * plans: grids whose transform does not fit one workgroup's LDS, an axis
  * longer than 32 cells, or the environment variable SNH_EWALD_FFT=lib at
- * snh_create; 0 for one workgroup per field). */
+ * snh_create; 0 for one workgroup per field), "corr_fft_lib" (1 when the
+ * sampled correlations' lattice transform uses a hipFFT plan). */
 int snh_query(snh_system *sys, const char *key, int64_t *value);
 
 #ifdef __cplusplus

@@ -491,6 +491,7 @@ struct SystemImpl final : snh_system {
         void* data = nullptr;    // [ncorr][na][T][na][ncell] running mean
         double* post = nullptr;  // [T] window / (T n_contrib)
         long long nsamples = 0;
+        CorrLibFft* lib = nullptr;  // hipFFT plan of the lattice transform (axes beyond 32)
     } corr;
     // Metropolis sweeps (snh_mc.hip): colour classes and tables, built at first use
     struct Mc {
@@ -548,6 +549,7 @@ struct SystemImpl final : snh_system {
         if (st2) (void)hipStreamDestroy(st2);
         drop_graph();
         for (void* p : corr.mem) (void)hipFree(p);
+        corr_lib_destroy(corr.lib);
         timer.clear();
         for (T* p : ckpt) (void)hipFree(p);
         ewald_lib_destroy(ewlib);
@@ -1276,13 +1278,20 @@ struct SystemImpl final : snh_system {
         SNH_REQUIRE(measperiod >= 1, SNH_ERR_INVALID, "measperiod must be positive");
         const ReplicaInfo& ri = reps[0];
         const int L[3] = {ri.L1, ri.L2, ri.L3};
-        for (int a = 0; a < 3; a++)
-            SNH_REQUIRE(L[a] <= 32, SNH_ERR_UNSUPPORTED, "sampled correlations support at most 32 cells per axis");
         for (int c = 0; c < 2 * ncorr; c++)
             SNH_REQUIRE(pairs[c] >= 0 && pairs[c] < nobs, SNH_ERR_INVALID, "correlation pair out of range");
         if (st) SNH_HIP(hipStreamSynchronize(st));
         for (void* p : corr.mem) (void)hipFree(p);
+        corr_lib_destroy(corr.lib);
         corr = Corr{};
+        // axes up to 32 cells: line transforms in registers; longer axes, or
+        // any under SNH_CORR_FFT=lib: a batched hipFFT plan
+        const char* fmode = std::getenv("SNH_CORR_FFT");
+        if (L[0] > kLmax || L[1] > kLmax || L[2] > kLmax || (fmode && std::string(fmode) == "lib")) {
+            std::string err;
+            corr.lib = corr_lib_create((long long)(nt / 2 + 1) * nobs * na, L[0], L[1], L[2], err);
+            SNH_REQUIRE(corr.lib, SNH_ERR_HIP, "sampled correlations: " + err);
+        }
         corr.nobs = nobs;
         corr.ncorr = ncorr;
         corr.nt = nt;
@@ -1346,7 +1355,10 @@ struct SystemImpl final : snh_system {
             observe(n);
         }
         // lattice transform of the recorded slots (the padding stays zero)
-        launch_corr_space_fft(corr.samp, (long long)Tsnap * nobs * na, corr.L[0], corr.L[1], corr.L[2], st);
+        if (corr.lib)
+            SNH_REQUIRE(corr_lib_exec(corr.lib, corr.samp, st), SNH_ERR_HIP, "hipFFT execution failed");
+        else
+            launch_corr_space_fft(corr.samp, (long long)Tsnap * nobs * na, corr.L[0], corr.L[1], corr.L[2], st);
         launch_corr_tdft(corr.samp, corr.samp2, (long long)nobs * na * ncell, nt, -1, nullptr, nullptr, 0, st);
         const long long count = ++corr.nsamples;
         const long long nl = (long long)na * ncell;
@@ -2240,6 +2252,7 @@ struct SystemImpl final : snh_system {
         if (key == "mid_reruns") return mid_reruns;
         if (key == "mid_fallbacks") return mid_fallbacks;
         if (key == "ewald_fft_lib") return ewald && EG.plain ? 1 : 0;
+        if (key == "corr_fft_lib") return corr.lib ? 1 : 0;
         throw Failure{SNH_ERR_INVALID, "unknown query key: " + key};
     }
 

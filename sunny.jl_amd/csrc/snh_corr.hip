@@ -22,6 +22,8 @@
 //                k_corr_tdft     Z = fft(Y), accumulated into the running mean
 #include "snh_fft.h"
 
+#include <hipfft/hipfft.h>
+
 namespace snh {
 
 using Cd = C2<double>;
@@ -178,6 +180,54 @@ void launch_corr_product(const void* samp, void* P, int T, int nobs, int na, lon
 {
     hipLaunchKernelGGL(k_corr_product, dim3(corr_grid((long long)T * na * ncell)), dim3(kCorrThreads), 0, st,
                        reinterpret_cast<const Cd*>(samp), reinterpret_cast<Cd*>(P), T, nobs, na, ncell, a, i, b);
+}
+
+// Lattice transforms with an axis longer than kLmax (or under
+// SNH_CORR_FFT=lib): one batched in-place 3D Z2Z hipFFT plan per
+// corr_init, HIPFFT_BACKWARD (e^{+2 pi i k x / L}, unnormalised, as the
+// line transforms), its work area allocated with it.
+struct CorrLibFft {
+    hipfftHandle plan = 0;
+    void* work = nullptr;
+};
+
+void corr_lib_destroy(CorrLibFft* L)
+{
+    if (!L) return;
+    if (L->plan) (void)hipfftDestroy(L->plan);
+    if (L->work) (void)hipFree(L->work);
+    delete L;
+}
+
+CorrLibFft* corr_lib_create(long long nfield, int L1, int L2, int L3, std::string& err)
+{
+    CorrLibFft* L = new CorrLibFft;
+    int n[3] = {L3, L2, L1};
+    const int ncell = L1 * L2 * L3;
+    size_t w = 0;
+    hipfftResult e = hipfftCreate(&L->plan);
+    if (e == HIPFFT_SUCCESS) e = hipfftSetAutoAllocation(L->plan, 0);
+    if (e == HIPFFT_SUCCESS)
+        e = hipfftMakePlanMany(L->plan, 3, n, n, 1, ncell, n, 1, ncell, HIPFFT_Z2Z, (int)nfield, &w);
+    if (e != HIPFFT_SUCCESS) {
+        err = "hipFFT plan creation failed (code " + std::to_string((int)e) + ")";
+        corr_lib_destroy(L);
+        return nullptr;
+    }
+    if (hipMalloc(&L->work, std::max<size_t>(w, 256)) != hipSuccess ||
+        hipfftSetWorkArea(L->plan, L->work) != HIPFFT_SUCCESS) {
+        err = "hipFFT work area not allocated";
+        corr_lib_destroy(L);
+        return nullptr;
+    }
+    return L;
+}
+
+bool corr_lib_exec(CorrLibFft* L, void* buf, hipStream_t st)
+{
+    auto* b = reinterpret_cast<hipfftDoubleComplex*>(buf);
+    return hipfftSetStream(L->plan, st) == HIPFFT_SUCCESS &&
+           hipfftExecZ2Z(L->plan, b, b, HIPFFT_BACKWARD) == HIPFFT_SUCCESS;
 }
 
 template void launch_corr_observe<float>(const Couplings<float>&, int, const float*, int, const double*, void*, int,

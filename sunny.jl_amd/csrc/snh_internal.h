@@ -202,6 +202,11 @@ template <typename T>
 void launch_corr_observe(const Couplings<T>& C, int nblocks, const T* S, int nobs, const double* obs, void* samp,
                          int t, double scale, hipStream_t st);
 void launch_corr_space_fft(void* buf, long long nfield, int L1, int L2, int L3, hipStream_t st);
+// The same transform by a batched hipFFT plan (axes longer than kLmax).
+struct CorrLibFft;
+CorrLibFft* corr_lib_create(long long nfield, int L1, int L2, int L3, std::string& err);
+bool corr_lib_exec(CorrLibFft* L, void* buf, hipStream_t st);
+void corr_lib_destroy(CorrLibFft* L);
 void launch_corr_tdft(const void* in, void* out, long long nl, int T, int sign, const double* pre, const double* post,
                       long long mean_count, hipStream_t st);
 void launch_corr_product(const void* samp, void* P, int T, int nobs, int na, long long ncell, int a, int i, int b,

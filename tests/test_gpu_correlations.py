@@ -21,7 +21,26 @@ def model(name):
 
 @pytest.mark.parametrize("name,window", [("cubic4", "cosine"), ("pyro2", "cosine"), ("cubic4", "rectangular")])
 def test_sampled_correlations_match_oracle(name, window):
-    s = model(name)
+    check_against_ref(model(name), name, window)
+
+
+def test_sampled_correlations_library_fft(monkeypatch):
+    """SNH_CORR_FFT=lib: the lattice transform by a batched hipFFT plan."""
+    monkeypatch.setenv("SNH_CORR_FFT", "lib")
+    s = model("pyro2")
+    check_against_ref(s, "pyro2 (library FFT)", "cosine")
+    assert s._device().query("corr_fft_lib") == 1
+
+
+def test_sampled_correlations_long_axis():
+    """An axis of 40 cells (beyond the register line transforms) takes the
+    hipFFT plan."""
+    s = sh.workloads.cubic_model(dims=(40, 2, 1), seed=9, precision=64)
+    check_against_ref(s, "cubic 40x2x1", "cosine")
+    assert s._device().query("corr_fft_lib") == 1
+
+
+def check_against_ref(s, name, window):
     # n_all_omega = 10; omega_max = 20 keeps the trajectory short (measperiod 4,
     # 40 steps per two samples) so that rounding differences between two FP64
     # integrators stay far below the tolerance despite chaotic dynamics
