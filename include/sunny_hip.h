@@ -66,7 +66,8 @@ typedef struct {
     double latvecs[9];          /* lattice vectors as columns, row-major */
     const double *positions;    /* [na][3] fractional positions (Ewald only) */
     int32_t nreplicas;          /* R >= 1 */
-    const int32_t *dims;        /* [R][3] supercell dims per replica (may differ without Ewald) */
+    const int32_t *dims;        /* [R][3] supercell dims per replica (may differ; with Ewald each run
+                                   of consecutive equal dims shares one transformed tensor) */
     int32_t layout;             /* SNH_LAYOUT_AUTO or SNH_LAYOUT_GHOST (below) */
     /* :SUN only: the general part of set_pair_coupling! (the TensorDecomposition
      * of src/System/Types.jl:23-27, PairExchange.jl:85-111): directed bond b

@@ -505,20 +505,25 @@ struct SystemImpl final : snh_system {
         double* kT = nullptr;         // [R]
         double* acc = nullptr;        // [max(work, R)][4]
         double* out = nullptr;        // [R][4]
-        // sequential trials with the Ewald term
-        double* Aew = nullptr;   // real-space tensor [i_s][i_j][cell][3][3]
-        double* A0 = nullptr;    // [na][9] self blocks A(0, i, i)
+        // sequential trials with the Ewald term (tensors per Ewald group)
         double* phi = nullptr;   // FP64 potential planes [3][total]
         McTrial* trial = nullptr;  // [R]
     } mc;
-    // Ewald
-    EwaldGeom EG{};
-    EwaldAParams ewP{};          // parameters of the real-space tensor
+    // Ewald: one group per run of consecutive replicas with equal dims, each
+    // with its grid, transformed tensor, spectra and (library FFT path) plans
+    struct EwaldGroup {
+        int r0 = 0, nr = 0;  // replicas r0 .. r0 + nr - 1
+        EwaldGeom EG{};
+        EwaldAParams P{};  // parameters of the real-space tensor
+        void* M = nullptr;
+        void* Fmu = nullptr;
+        void* Fphi = nullptr;
+        EwaldLibFft* lib = nullptr;
+        double* Aew = nullptr;  // LocalSampler: the real-space tensor, kept
+        double* A0 = nullptr;   // LocalSampler: [na][9] self blocks A(0, i, i)
+    };
+    std::vector<EwaldGroup> ewg;
     std::vector<double> ew_pos;  // fractional positions [na][3]
-    void* M = nullptr;
-    EwaldLibFft* ewlib = nullptr;  // hipFFT plans of the library FFT path
-    void* Fmu = nullptr;
-    void* Fphi = nullptr;
     T* phi1 = nullptr;
     T* phi2 = nullptr;
     KernelTimer timer;
@@ -552,7 +557,7 @@ struct SystemImpl final : snh_system {
         corr_lib_destroy(corr.lib);
         timer.clear();
         for (T* p : ckpt) (void)hipFree(p);
-        ewald_lib_destroy(ewlib);
+        for (auto& g : ewg) ewald_lib_destroy(g.lib);
         for (void* p : owned) (void)hipFree(p);
         if (pinned) (void)hipHostFree(pinned);
         if (pinned_i) (void)hipHostFree(pinned_i);
@@ -917,45 +922,10 @@ struct SystemImpl final : snh_system {
         SNH_HIP(hipStreamWaitEvent(st, pev[0], 0));
     }
 
-    void build_ewald(const snh_spec& sp)
+    // Parameters of the real-space tensor of an L1 x L2 x L3 supercell
+    // (Ewald.jl:61-91).
+    static EwaldAParams ewald_params(const snh_spec& sp, int na, const int L[3])
     {
-        for (int r = 1; r < R; r++)
-            SNH_REQUIRE(reps[r].L1 == reps[0].L1 && reps[r].L2 == reps[0].L2 && reps[r].L3 == reps[0].L3,
-                        SNH_ERR_UNSUPPORTED, "Ewald requires all replicas to share the same dims");
-        SNH_REQUIRE(sp.positions, SNH_ERR_INVALID, "Ewald needs atom positions");
-        const int L[3] = {reps[0].L1, reps[0].L2, reps[0].L3};
-        EG.L1 = L[0];
-        EG.L2 = L[1];
-        EG.L3 = L[2];
-        EG.H1 = L[0] / 2 + 1;
-        EG.na = na;
-        EG.ncell = (long long)L[0] * L[1] * L[2];
-        EG.nk = (long long)EG.H1 * L[1] * L[2];
-        EG.P1 = reps[0].P1;
-        EG.P2 = reps[0].P2;
-        EG.pcell = reps[0].pcell;
-        EG.w1 = G.w1;
-        EG.w2 = G.w2;
-        EG.w3 = G.w3;
-        {
-            int dev = 0;
-            SNH_HIP(hipGetDevice(&dev));
-            hipDeviceProp_t prop;
-            SNH_HIP(hipGetDeviceProperties(&prop, dev));
-            const size_t lim = prop.sharedMemPerBlock;
-            // one workgroup per field transforms grids whose FP64 transform of
-            // the tensor A (the largest) fits its LDS; larger grids, or any
-            // under SNH_EWALD_FFT=lib, go through hipFFT
-            const bool fits = L[0] <= kLmax && L[1] <= kLmax && L[2] <= kLmax &&
-                              ewald_fft_lds_bytes(EG, sizeof(double)) <= lim;
-            const char* mode = std::getenv("SNH_EWALD_FFT");
-            EG.plain = !fits || (mode && std::string(mode) == "lib") ? 1 : 0;
-            const size_t lc = ewald_use_mfma(EG, R, sizeof(T)) ? ewald_mfma_lds_bytes(EG)
-                                                               : (size_t)R * 3 * na * sizeof(T) * 2;
-            SNH_REQUIRE(lc <= lim, SNH_ERR_UNSUPPORTED, "too many replicas x sublattices for the Ewald contraction");
-        }
-
-        // A precompute parameters, Ewald.jl:61-91
         EwaldAParams P{};
         for (int p = 0; p < 9; p++) P.lv[p] = sp.latvecs[p];
         for (int rr = 0; rr < 3; rr++)
@@ -990,33 +960,85 @@ struct SystemImpl final : snh_system {
         P.L2 = L[1];
         P.L3 = L[2];
         P.na = na;
-        std::vector<double> pos(sp.positions, sp.positions + 3 * na);
-        ewP = P;
-        ew_pos = pos;
-        double* d_pos = dupload(pos);
-        double* d_A = dalloc<double>((size_t)na * na * EG.ncell * 9);
-        const int n3 = 3 * na;
-        M = own(dalloc<char>((size_t)EG.nk * n3 * n3 * 2 * sizeof(T)));
-        launch_ewald_A(P, d_pos, d_A, st);
-        std::string err;
-        if (EG.plain)
-            err = launch_ewald_fwdA_plain<T>(EG, d_A, M, st);
-        else
-            launch_ewald_fwdA<T>(EG, d_A, M, st);
-        SNH_HIP(hipGetLastError());
-        SNH_HIP(hipStreamSynchronize(st));
-        SNH_HIP(hipFree(d_A));
-        SNH_HIP(hipFree(d_pos));
-        SNH_REQUIRE(err.empty(), SNH_ERR_HIP, "Ewald tensor: " + err);
-        if (EG.plain) {
-            ewlib = ewald_lib_create(EG, R, sizeof(T), err);
-            SNH_REQUIRE(ewlib, SNH_ERR_HIP, "Ewald FFT: " + err);
+        return P;
+    }
+
+    void build_ewald(const snh_spec& sp)
+    {
+        SNH_REQUIRE(sp.positions, SNH_ERR_INVALID, "Ewald needs atom positions");
+        ew_pos.assign(sp.positions, sp.positions + 3 * na);
+        int dev_ = 0;
+        SNH_HIP(hipGetDevice(&dev_));
+        hipDeviceProp_t prop;
+        SNH_HIP(hipGetDeviceProperties(&prop, dev_));
+        const size_t lim = prop.sharedMemPerBlock;
+        const char* mode = std::getenv("SNH_EWALD_FFT");
+        const bool force_lib = mode && std::string(mode) == "lib";
+        double* d_pos = dupload(ew_pos);
+        own(d_pos);
+        for (int r = 0; r < R;) {
+            int r1 = r + 1;
+            while (r1 < R && reps[r1].L1 == reps[r].L1 && reps[r1].L2 == reps[r].L2 && reps[r1].L3 == reps[r].L3) r1++;
+            EwaldGroup g;
+            g.r0 = r;
+            g.nr = r1 - r;
+            const int L[3] = {reps[r].L1, reps[r].L2, reps[r].L3};
+            EwaldGeom& EG = g.EG;
+            EG.L1 = L[0];
+            EG.L2 = L[1];
+            EG.L3 = L[2];
+            EG.H1 = L[0] / 2 + 1;
+            EG.na = na;
+            EG.ncell = (long long)L[0] * L[1] * L[2];
+            EG.nk = (long long)EG.H1 * L[1] * L[2];
+            EG.P1 = reps[r].P1;
+            EG.P2 = reps[r].P2;
+            EG.pcell = reps[r].pcell;
+            EG.w1 = G.w1;
+            EG.w2 = G.w2;
+            EG.w3 = G.w3;
+            // one workgroup per field transforms grids whose FP64 transform of
+            // the tensor A (the largest) fits its LDS; larger grids, or any
+            // under SNH_EWALD_FFT=lib, go through hipFFT
+            const bool fits = L[0] <= kLmax && L[1] <= kLmax && L[2] <= kLmax &&
+                              ewald_fft_lds_bytes(EG, sizeof(double)) <= lim;
+            EG.plain = !fits || force_lib ? 1 : 0;
+            const size_t lc = ewald_use_mfma(EG, g.nr, sizeof(T)) ? ewald_mfma_lds_bytes(EG)
+                                                                  : (size_t)g.nr * 3 * na * sizeof(T) * 2;
+            SNH_REQUIRE(lc <= lim, SNH_ERR_UNSUPPORTED, "too many replicas x sublattices for the Ewald contraction");
+            g.P = ewald_params(sp, na, L);
+            double* d_A = dalloc<double>((size_t)na * na * EG.ncell * 9);
+            const int n3 = 3 * na;
+            g.M = own(dalloc<char>((size_t)EG.nk * n3 * n3 * 2 * sizeof(T)));
+            launch_ewald_A(g.P, d_pos, d_A, st);
+            std::string err;
+            if (EG.plain)
+                err = launch_ewald_fwdA_plain<T>(EG, d_A, g.M, st);
+            else
+                launch_ewald_fwdA<T>(EG, d_A, g.M, st);
+            SNH_HIP(hipGetLastError());
+            SNH_HIP(hipStreamSynchronize(st));
+            SNH_HIP(hipFree(d_A));
+            SNH_REQUIRE(err.empty(), SNH_ERR_HIP, "Ewald tensor: " + err);
+            g.Fmu = own(dalloc<char>(ewald_spectrum_bytes(EG, g.nr, sizeof(T))));
+            g.Fphi = own(dalloc<char>(ewald_spectrum_bytes(EG, g.nr, sizeof(T))));
+            if (EG.plain) {
+                g.lib = ewald_lib_create(EG, g.nr, sizeof(T), err);
+                SNH_REQUIRE(g.lib, SNH_ERR_HIP, "Ewald FFT: " + err);
+            }
+            ewg.push_back(g);
+            r = r1;
         }
-        Fmu = own(dalloc<char>(ewald_spectrum_bytes(EG, R, sizeof(T))));
-        Fphi = own(dalloc<char>(ewald_spectrum_bytes(EG, R, sizeof(T))));
         phi1 = own(dalloc<T>(3 * (size_t)total));
         phi2 = own(dalloc<T>(3 * (size_t)total));
         if (N > 0) Sd = own(dalloc<T>(3 * (size_t)total));
+    }
+
+    bool ewald_lib_any() const
+    {
+        for (const auto& g : ewg)
+            if (g.EG.plain) return true;
+        return false;
     }
 
     // --- timing helpers
@@ -1045,8 +1067,10 @@ struct SystemImpl final : snh_system {
             S = Sd;
         }
         hipEvent_t e = timed ? tbegin() : nullptr;
-        SNH_REQUIRE(launch_ewald_field<T>(EG, R, C.reps, total, C.gmat, S, M, Fmu, Fphi, phi, ewlib, st), SNH_ERR_HIP,
-                    "hipFFT execution failed");
+        for (auto& g : ewg)
+            SNH_REQUIRE(launch_ewald_field<T>(g.EG, g.nr, C.reps + g.r0, total, C.gmat, S, g.M, g.Fmu, g.Fphi, phi,
+                                              g.lib, st),
+                        SNH_ERR_HIP, "hipFFT execution failed");
         tend("ewald_field", e);
     }
 
@@ -1463,20 +1487,20 @@ struct SystemImpl final : snh_system {
             // the real-space tensor (freed after the Fourier transform at
             // creation) is recomputed and kept: every accepted move updates
             // the potential with one of its slices
-            for (int r = 1; r < R; r++)
-                SNH_REQUIRE(reps[r].ncell == reps[0].ncell, SNH_ERR_INVALID, "Ewald replicas share their dims");
-            const size_t nA = (size_t)na * na * (size_t)EG.ncell * 9;
             double* d_pos = dupload(ew_pos);
-            mc.Aew = own(dalloc<double>(nA));
-            launch_ewald_A(ewP, d_pos, mc.Aew, st);
-            SNH_HIP(hipGetLastError());
-            SNH_HIP(hipStreamSynchronize(st));
-            SNH_HIP(hipFree(d_pos));
-            std::vector<double> a0(9 * (size_t)na);
-            for (int i = 0; i < na; i++)
-                SNH_HIP(hipMemcpy(a0.data() + 9 * (size_t)i, mc.Aew + ((size_t)(i * na + i) * EG.ncell) * 9,
-                                  9 * sizeof(double), hipMemcpyDeviceToHost));
-            mc.A0 = own(dupload(a0));
+            own(d_pos);
+            for (auto& g : ewg) {
+                const size_t nA = (size_t)na * na * (size_t)g.EG.ncell * 9;
+                g.Aew = own(dalloc<double>(nA));
+                launch_ewald_A(g.P, d_pos, g.Aew, st);
+                SNH_HIP(hipGetLastError());
+                SNH_HIP(hipStreamSynchronize(st));
+                std::vector<double> a0(9 * (size_t)na);
+                for (int i = 0; i < na; i++)
+                    SNH_HIP(hipMemcpy(a0.data() + 9 * (size_t)i, g.Aew + ((size_t)(i * na + i) * g.EG.ncell) * 9,
+                                      9 * sizeof(double), hipMemcpyDeviceToHost));
+                g.A0 = own(dupload(a0));
+            }
             mc.phi = own(dalloc<double>(3 * (size_t)total));
             mc.trial = own(dalloc<McTrial>(R));
         }
@@ -1533,17 +1557,18 @@ struct SystemImpl final : snh_system {
             ewald_field(X, phi1);
             launch_planes_to_double<T>(phi1, mc.phi, 3 * (long long)total, st);
             launch_fill_double(mc.acc, 4 * (long long)R, 0.0, st);
-            const uint32_t ns = (uint32_t)reps[0].ncell * (uint32_t)na;
-            const long long niters = (long long)std::ceil(nsweeps * (double)ns);
             hipEvent_t e = tbegin();
-            for (long long t = 0; t < niters; t++) {
-                if (N == 0)
-                    launch_mc_ewald_trial<T>(C, A, X, mc.phi, mc.A0, sweep0 * ns + (uint64_t)t, mc.trial, mc.acc, R,
-                                             st);
-                else
-                    launch_sun_mc_ewald_trial<T>(C, A, X, mc.phi, mc.A0, sweep0 * ns + (uint64_t)t, mc.trial, mc.acc,
-                                                 R, st);
-                launch_mc_ewald_update(mc.Aew, C.reps, C.G, na, total, ns, mc.trial, mc.phi, R, st);
+            for (const auto& g : ewg) {  // replicas of one dims at a time
+                const uint32_t ns = (uint32_t)reps[g.r0].ncell * (uint32_t)na;
+                const long long niters = (long long)std::ceil(nsweeps * (double)ns);
+                for (long long t = 0; t < niters; t++) {
+                    const uint64_t T0 = sweep0 * ns + (uint64_t)t;
+                    if (N == 0)
+                        launch_mc_ewald_trial<T>(C, A, X, mc.phi, g.A0, T0, mc.trial, mc.acc, g.r0, g.nr, st);
+                    else
+                        launch_sun_mc_ewald_trial<T>(C, A, X, mc.phi, g.A0, T0, mc.trial, mc.acc, g.r0, g.nr, st);
+                    launch_mc_ewald_update(g.Aew, C.reps, C.G, na, total, ns, mc.trial, mc.phi, g.r0, g.nr, st);
+                }
             }
             tend("local_ewald", e);
             SNH_HIP(hipGetLastError());
@@ -2180,7 +2205,7 @@ struct SystemImpl final : snh_system {
                     "snh_energy_noalloc: the energy launch needs " + std::to_string(need) +
                         " partial sums, more than the static scratch holds (" + std::to_string(kNoAllocPartials) +
                         "); use snh_energy");
-        SNH_REQUIRE(!(ewald && EG.plain), SNH_ERR_UNSUPPORTED,
+        SNH_REQUIRE(!(ewald && ewald_lib_any()), SNH_ERR_UNSUPPORTED,
                     "snh_energy_noalloc: the Ewald grid is transformed by hipFFT; use snh_energy");
         double* part = noalloc_scratch(dev);
         SNH_REQUIRE(part, SNH_ERR_HIP, "static scratch not resolved on this device");
@@ -2251,7 +2276,7 @@ struct SystemImpl final : snh_system {
         if (key == "pair_on") return pair_on ? 1 : 0;
         if (key == "mid_reruns") return mid_reruns;
         if (key == "mid_fallbacks") return mid_fallbacks;
-        if (key == "ewald_fft_lib") return ewald && EG.plain ? 1 : 0;
+        if (key == "ewald_fft_lib") return ewald && ewald_lib_any() ? 1 : 0;
         if (key == "corr_fft_lib") return corr.lib ? 1 : 0;
         throw Failure{SNH_ERR_INVALID, "unknown query key: " + key};
     }

@@ -243,18 +243,19 @@ struct McTrial {
     int flag, i, c1, c2, c3, pad;
     double dmu[3];
 };
-// Trial T0 of the Ewald form for every replica (phi: FP64 potential planes,
-// A0: [na][9] self blocks A(0, i, i)).
+// Trial T0 of the Ewald form for replicas r0 .. r0 + nr - 1, which share
+// their dims (phi: FP64 potential planes, A0: [na][9] self blocks
+// A(0, i, i) of those dims).
 template <typename T>
 void launch_mc_ewald_trial(const Couplings<T>& C, const McArgs& A, T* S, const double* phi, const double* A0,
-                           unsigned long long T0, McTrial* tr, double* acc, int R, hipStream_t st);
+                           unsigned long long T0, McTrial* tr, double* acc, int r0, int nr, hipStream_t st);
 // The same trial for :SUN kets (the Ewald term acts on expected dipoles).
 template <typename T>
 void launch_sun_mc_ewald_trial(const Couplings<T>& C, const McArgs& A, T* Z, const double* phi, const double* A0,
-                               unsigned long long T0, McTrial* tr, double* acc, int R, hipStream_t st);
-// phi += A dmu at the ns sites of every replica whose trial was accepted.
+                               unsigned long long T0, McTrial* tr, double* acc, int r0, int nr, hipStream_t st);
+// phi += A dmu at the ns sites of each of those replicas whose trial was accepted.
 void launch_mc_ewald_update(const double* Aew, const ReplicaInfo* reps, const Ghost& G, int na, uint32_t total,
-                            uint32_t ns, const McTrial* tr, double* phi, int R, hipStream_t st);
+                            uint32_t ns, const McTrial* tr, double* phi, int r0, int nr, hipStream_t st);
 template <typename T>
 void launch_planes_to_double(const T* src, double* dst, long long n, hipStream_t st);
 void launch_fill_double(double* p, long long n, double v, hipStream_t st);

@@ -166,8 +166,8 @@ __global__ __launch_bounds__(kBlock) void k_mc_color(Couplings<T> C, McArgs A, T
         for (int qv = 0; qv < 4; qv++) acc[4 * (size_t)blockIdx.x + qv] += v[qv];
 }
 
-// Sequential trial of the Ewald form, workgroup r = replica r (its first
-// lane only: philox4x32 takes its key as uniform over the wave): trial T0
+// Sequential trial of the Ewald form, workgroup b = replica r0 + b (its
+// first lane only: philox4x32 takes its key as uniform over the wave): trial T0
 // draws the site, u_acc and u_mix by mc_seq_draw and the normals from
 // counter (T0 lo, kMcTagSequential + 1, T0 hi, 0).  dE adds
 // ewald_energy_delta (Ewald.jl:234-241).  tr[r] records an accepted move
@@ -176,10 +176,11 @@ template <typename T, int G>
 __global__ __launch_bounds__(64) void k_mc_ewald_trial(Couplings<T> C, McArgs A, T* __restrict__ S,
                                                        const double* __restrict__ phi,
                                                        const double* __restrict__ A0, unsigned long long T0,
-                                                       McTrial* __restrict__ tr, double* __restrict__ acc, int nrep)
+                                                       McTrial* __restrict__ tr, double* __restrict__ acc, int r0,
+                                                       int nrep)
 {
-    const int r = blockIdx.x;
-    if (threadIdx.x != 0 || r >= nrep) return;
+    if (threadIdx.x != 0 || (int)blockIdx.x >= nrep) return;
+    const int r = r0 + (int)blockIdx.x;
     const ReplicaInfo R = C.reps[r];
     const uint32_t total = C.total;
     const uint32_t ns = (uint32_t)R.ncell * (uint32_t)C.na;
@@ -209,14 +210,16 @@ __global__ __launch_bounds__(64) void k_mc_ewald_trial(Couplings<T> C, McArgs A,
     tr[r] = out;
 }
 
-// After trial tr[r] was accepted: phi at every site j of replica r (grid.y)
-// gains A(c_s - c_j; i_s, i_j) dmu (phi = sum_k A mu_k, Ewald.jl:185-209).
-// Aew: real-space tensor [i_s][i_j][cell offset][3][3], FP64.
+// After trial tr[r] was accepted: phi at every site j of replica r = r0 +
+// grid.y gains A(c_s - c_j; i_s, i_j) dmu (phi = sum_k A mu_k,
+// Ewald.jl:185-209).  Aew: real-space tensor [i_s][i_j][cell offset][3][3]
+// of the replica's dims, FP64.
 __global__ __launch_bounds__(256) void k_mc_ewald_update(const double* __restrict__ Aew, const ReplicaInfo* reps,
                                                          Ghost G, int na, uint32_t total,
-                                                         const McTrial* __restrict__ tr, double* __restrict__ phi)
+                                                         const McTrial* __restrict__ tr, double* __restrict__ phi,
+                                                         int r0)
 {
-    const int r = blockIdx.y;
+    const int r = r0 + (int)blockIdx.y;
     const McTrial m = tr[r];
     if (!m.flag) return;  // uniform over the workgroup
     const ReplicaInfo R = reps[r];
@@ -278,20 +281,20 @@ void launch_mc_color(const Couplings<T>& C, const McArgs& A, T* S, const McWork*
 
 template <typename T>
 void launch_mc_ewald_trial(const Couplings<T>& C, const McArgs& A, T* S, const double* phi, const double* A0,
-                           unsigned long long T0, McTrial* tr, double* acc, int R, hipStream_t st)
+                           unsigned long long T0, McTrial* tr, double* acc, int r0, int nr, hipStream_t st)
 {
-    const dim3 grid(R);
+    const dim3 grid(nr);
     if (C.chunk == 6)
-        hipLaunchKernelGGL((k_mc_ewald_trial<T, 6>), grid, dim3(64), 0, st, C, A, S, phi, A0, T0, tr, acc, R);
+        hipLaunchKernelGGL((k_mc_ewald_trial<T, 6>), grid, dim3(64), 0, st, C, A, S, phi, A0, T0, tr, acc, r0, nr);
     else
-        hipLaunchKernelGGL((k_mc_ewald_trial<T, 9>), grid, dim3(64), 0, st, C, A, S, phi, A0, T0, tr, acc, R);
+        hipLaunchKernelGGL((k_mc_ewald_trial<T, 9>), grid, dim3(64), 0, st, C, A, S, phi, A0, T0, tr, acc, r0, nr);
 }
 
 void launch_mc_ewald_update(const double* Aew, const ReplicaInfo* reps, const Ghost& G, int na, uint32_t total,
-                            uint32_t ns, const McTrial* tr, double* phi, int R, hipStream_t st)
+                            uint32_t ns, const McTrial* tr, double* phi, int r0, int nr, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mc_ewald_update, dim3((ns + 255) / 256, R), dim3(256), 0, st, Aew, reps, G, na, total, tr,
-                       phi);
+    hipLaunchKernelGGL(k_mc_ewald_update, dim3((ns + 255) / 256, nr), dim3(256), 0, st, Aew, reps, G, na, total, tr,
+                       phi, r0);
 }
 
 template <typename T>
@@ -311,9 +314,11 @@ void launch_fill_double(double* p, long long n, double v, hipStream_t st)
 }
 
 template void launch_mc_ewald_trial<float>(const Couplings<float>&, const McArgs&, float*, const double*,
-                                           const double*, unsigned long long, McTrial*, double*, int, hipStream_t);
+                                           const double*, unsigned long long, McTrial*, double*, int, int,
+                                           hipStream_t);
 template void launch_mc_ewald_trial<double>(const Couplings<double>&, const McArgs&, double*, const double*,
-                                            const double*, unsigned long long, McTrial*, double*, int, hipStream_t);
+                                            const double*, unsigned long long, McTrial*, double*, int, int,
+                                            hipStream_t);
 template void launch_planes_to_double<float>(const float*, double*, long long, hipStream_t);
 template void launch_planes_to_double<double>(const double*, double*, long long, hipStream_t);
 template void launch_mc_color<float>(const Couplings<float>&, const McArgs&, float*, const McWork*, int,

@@ -837,8 +837,8 @@ __global__ __launch_bounds__(kBlock) void k_sun_mc_color(Couplings<T> C, McArgs 
         for (int q = 0; q < 4; q++) acc[4 * (size_t)blockIdx.x + q] += v[q];
 }
 
-// Sequential trial T0 of the Ewald form for :SUN, workgroup r = replica r
-// (its first lane only): the draws of the :dipole form (mc_seq_draw; normals
+// Sequential trial T0 of the Ewald form for :SUN, workgroup b = replica
+// r0 + b (its first lane only): the draws of the :dipole form (mc_seq_draw; normals
 // from the counters (T0 lo, kMcTagSequential + 1 + 2 b, T0 hi, 0)), the
 // energy change of the ket plus ewald_energy_delta of its expected dipole
 // (Ewald.jl:234-241).  tr[r] records an accepted move for
@@ -848,10 +848,10 @@ __global__ __launch_bounds__(64) void k_sun_mc_ewald_trial(Couplings<T> C, McArg
                                                            const double* __restrict__ phi,
                                                            const double* __restrict__ A0, unsigned long long T0,
                                                            McTrial* __restrict__ tr, double* __restrict__ acc,
-                                                           int nrep)
+                                                           int r0, int nrep)
 {
-    const int r = blockIdx.x;
-    if (threadIdx.x != 0 || r >= nrep) return;
+    if (threadIdx.x != 0 || (int)blockIdx.x >= nrep) return;
+    const int r = r0 + (int)blockIdx.x;
     const ReplicaInfo R = C.reps[r];
     const uint32_t ns = (uint32_t)R.ncell * (uint32_t)C.na;
     const unsigned long long key = A.seed + (unsigned long long)r * 0x9E3779B97F4A7C15ull;
@@ -986,14 +986,14 @@ void launch_sun_mc_color(const Couplings<T>& C, const McArgs& A, T* Z, const McW
 
 template <typename T>
 void launch_sun_mc_ewald_trial(const Couplings<T>& C, const McArgs& A, T* Z, const double* phi, const double* A0,
-                               unsigned long long T0, McTrial* tr, double* acc, int R, hipStream_t st)
+                               unsigned long long T0, McTrial* tr, double* acc, int r0, int nr, hipStream_t st)
 {
-    SNH_SUN_DISPATCH(C.N, k_sun_mc_ewald_trial, dim3(R), dim3(64), 0, st, C, A, Z, phi, A0, T0, tr, acc, R)
+    SNH_SUN_DISPATCH(C.N, k_sun_mc_ewald_trial, dim3(nr), dim3(64), 0, st, C, A, Z, phi, A0, T0, tr, acc, r0, nr)
 }
 
 #define SNH_SUN_INST(T)                                                                                          \
     template void launch_sun_mc_ewald_trial<T>(const Couplings<T>&, const McArgs&, T*, const double*,           \
-                                               const double*, unsigned long long, McTrial*, double*, int,        \
+                                               const double*, unsigned long long, McTrial*, double*, int, int,   \
                                                hipStream_t);                                                     \
     template void launch_sun_mc_color<T>(const Couplings<T>&, const McArgs&, T*, const McWork*, int,            \
                                          unsigned long long, double, double*, hipStream_t);                      \

@@ -284,3 +284,41 @@ def test_library_fft_matches_workgroup_fft():
         del os.environ["SNH_EWALD_FFT"]
     assert abs(Ea - Eb) <= 1e-12 * abs(Ea)
     assert rel_err(gb, ga) <= 1e-12
+
+
+def test_ewald_batch_mixed_dims():
+    """Replicas of different dims: each run of consecutive equal dims gets its
+    own grid, tensor and spectra.  FP64: energies and two Philox Langevin
+    steps of every replica against its own oracle; FP32: nine 2x2x1
+    replicas (matrix-core contraction) beside two 1x2x3 ones."""
+    dims = [(2, 2, 1), (2, 2, 1), (3, 2, 1), (2, 2, 1)]
+    kw = dict(J1=0.05, J2=0, A=0, s=3.5, dipolar=True)
+    systems = [sh.workloads.pyrochlore_model(dims=d, seed=110 + r, precision=64, **kw) for r, d in enumerate(dims)]
+    announce("ewald batch of mixed dims fp64")
+    refs = [s.dipoles_flat() for s in systems]
+    batch = sh.ReplicaBatch(systems, precision=64)
+    E_dev = sh.energy(batch)
+    integ = sh.Langevin(0.01, damping=0.1, kT=0.2)
+    sh.step_(batch, integ, nsteps=2)
+    batch.sync_to_systems()
+    for r, s in enumerate(systems):
+        o = O.Oracle(s.spec())
+        S = refs[r]
+        scale = max(1.0, np.abs(o.grad(S)).max() * np.abs(S).max() * s.nsites)
+        assert abs(E_dev[r] - o.energy(S)) <= 1e-11 * scale, r
+        key = O.replica_key(batch.noise_seed, r)
+        for k in range(2):
+            S = o.step_langevin(S=S, noise=O.philox_normals(key, k, s.nsites, 3), dt=integ.dt, damping=integ.damping,
+                                kT=integ.kT)
+        assert rel_err(s.dipoles_flat(), S, scale=3.5) <= 1e-11, r
+    dims32 = [(2, 2, 1)] * 9 + [(1, 2, 3)] * 2
+    systems = [sh.workloads.pyrochlore_model(dims=d, seed=130 + r, precision=32, **kw) for r, d in enumerate(dims32)]
+    announce("ewald batch of mixed dims fp32 (9 + 2 replicas)")
+    batch = sh.ReplicaBatch(systems, precision=32)
+    E_dev = sh.energy(batch)
+    for r in (0, 8, 9, 10):
+        s = systems[r]
+        o = O.Oracle(s.spec())
+        S = s.dipoles_flat()
+        scale = max(1.0, np.abs(o.grad(S)).max() * np.abs(S).max() * s.nsites)
+        assert abs(E_dev[r] - o.energy(S)) <= 3e-5 * scale, r

@@ -214,6 +214,26 @@ def test_ewald_sequential_batch_matches_oracle():
         assert abs(sampler.dE[r] - acc[0]) <= 1e-9 * energy_scale(s), r
 
 
+def test_ewald_sequential_batch_mixed_dims():
+    """Sequential trials with the Ewald term on replicas of different dims
+    (each run of equal dims has its own real-space tensor)."""
+    dims = [(2, 2, 1), (1, 2, 3), (1, 2, 3)]
+    systems = [sh.workloads.pyrochlore_model(dims=d, seed=140 + r, precision=64, dipolar=True)
+               for r, d in enumerate(dims)]
+    announce("local sequential ewald replica batch of mixed dims")
+    batch = sh.ReplicaBatch(systems, precision=64)
+    refs = [s.dipoles_flat() for s in systems]
+    prop = sh.mix_proposals(0.5, sh.propose_uniform, 0.5, sh.propose_delta(0.25))
+    sampler = sh.LocalSampler(kT=0.5, nsweeps=1.0, propose=prop)
+    sh.step_(batch, sampler)
+    batch.sync_to_systems()
+    for r, s in enumerate(systems):
+        S, acc = O.Oracle(s.spec()).local_sequential(refs[r], oracle_props(prop), 0.5,
+                                                     O.replica_key(batch.noise_seed, r), 0, 1.0)
+        assert np.max(np.abs(s.dipoles_flat() - S)) <= 1e-12 * 1.5, r
+        assert abs(sampler.dE[r] - acc[0]) <= 1e-9 * energy_scale(s), r
+
+
 def test_unsupported_and_invalid():
     announce("local sampler errors")
     selfint = sh.workloads.cubic_model(dims=(1, 2, 2), seed=3, precision=64)
