@@ -470,6 +470,8 @@ struct SystemImpl final : snh_system {
     int pair_split = 0;        // first replica of the second group (0: no groups)
     bool pair_on = false;      // step in groups: SNH_PAIR=2, or chosen by snh_autotune (SNH_PAIR=1)
     bool mid_sync = false;     // SNH_MID_SYNC=1: host convergence check after every midpoint step
+    bool mid_hat_ready = false;  // :dipole: the last finalize wrote the next step's first Shat (into Bh)
+    int* last_it = nullptr;    // [R] iterations of the last step of a budgeted group
     int64_t mid_reruns = 0;     // midpoint step groups undone after overrunning their budget
     int64_t mid_fallbacks = 0;  // midpoint step groups then run with per-step host checks
     hipStream_t st2 = nullptr;
@@ -1157,7 +1159,8 @@ struct SystemImpl final : snh_system {
         if (!B2) B2 = own(dalloc<T>(n));
         if ((ewald || N == 0) && !Bh) Bh = own(dalloc<T>(n));
         if (N == 0 && !Bh2) Bh2 = own(dalloc<T>(n));
-        SNH_REQUIRE(B1 && B2 && (Bh || (!ewald && N != 0)) && (Bh2 || N != 0), SNH_ERR_HIP,
+        if (!last_it) last_it = own(dalloc<int>(R));
+        SNH_REQUIRE(B1 && B2 && (Bh || (!ewald && N != 0)) && (Bh2 || N != 0) && last_it, SNH_ERR_HIP,
                     "midpoint buffers missing");
     }
 
@@ -1845,9 +1848,10 @@ struct SystemImpl final : snh_system {
         T* hat_next = nullptr;
         if (N == 0) {
             // Shat of iteration it lives in hats[it % 2]: formed from S and
-            // S1 = S at it = 1, then written by the previous iteration
+            // S1 = S at it = 1 (unless the previous step's finalize wrote
+            // it), then written by the previous iteration
             T* hats[2] = {Bh2, Bh};
-            if (it == 1) launch_dipole_midpoint_hat<T>(C, nblocks, X, S1, hats[1], conv_iter, st);
+            if (it == 1 && !mid_hat_ready) launch_dipole_midpoint_hat<T>(C, nblocks, X, S1, hats[1], conv_iter, st);
             hat = hats[it % 2];
             hat_next = hats[(it + 1) % 2];
             if (ewald) {
@@ -1872,11 +1876,12 @@ struct SystemImpl final : snh_system {
         launch_midpoint_check(partial, tiled ? d_tcblk : d_rblk, R, atol2, conv_iter, it, st);
     }
 
-    // S = normalize(S'') for every converged replica.
-    void midpoint_finalize()
+    // S = normalize(S'') for every converged replica; with next_hat (:dipole)
+    // also the first Shat of the next step.
+    void midpoint_finalize(bool next_hat = false)
     {
         if (N == 0)
-            launch_dipole_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, 0, st);
+            launch_dipole_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, 0, next_hat ? Bh : nullptr, st);
         else
             launch_sun_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, 0, st);
     }
@@ -1891,12 +1896,15 @@ struct SystemImpl final : snh_system {
         const size_t bytes = sizeof(T) * planes * (size_t)total;
         launch_copy_planes(X, Xo, bytes, st);  // checkpoint (Xo is Heun scratch)
         launch_fill_int(fail_step, R, -1, st);
+        launch_fill_int(conv_iter, R, 0, st);  // k_mid_budget clears it after every step
         for (int s = 0; s < n; s++) {
-            launch_fill_int(conv_iter, R, 0, st);
+            // after the first step, the previous finalize wrote this step's first Shat
+            mid_hat_ready = s > 0;
             for (int it = 1; it <= budget; it++) midpoint_iteration(it, tdt, tl);
-            midpoint_finalize();
-            launch_mid_budget(conv_iter, fail_step, max_iter, R, s, st);
+            midpoint_finalize(s + 1 < n);
+            launch_mid_budget(conv_iter, fail_step, max_iter, last_it, R, s, st);
         }
+        mid_hat_ready = false;
         SNH_HIP(hipGetLastError());
         SNH_HIP(hipMemcpyAsync(pinned_i, fail_step, sizeof(int) * R, hipMemcpyDeviceToHost, st));
         SNH_HIP(hipStreamSynchronize(st));
@@ -1914,7 +1922,7 @@ struct SystemImpl final : snh_system {
         int mx = 0;
         for (int r = 0; r < R; r++) mx = std::max(mx, pinned_i[r]);
         last_iters = mx;
-        SNH_HIP(hipMemcpyAsync(pinned_i, conv_iter, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+        SNH_HIP(hipMemcpyAsync(pinned_i, last_it, sizeof(int) * R, hipMemcpyDeviceToHost, st));
         SNH_HIP(hipStreamSynchronize(st));
         if (iters)
             for (int r = 0; r < R; r++) iters[r] = pinned_i[r];

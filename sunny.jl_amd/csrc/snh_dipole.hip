@@ -155,17 +155,25 @@ __global__ __launch_bounds__(kBlock) void k_dipole_mid_iter(Couplings<T> C, cons
 }
 
 // S = kappa*normalize(S'') for replicas that converged at iteration `iter`;
-// S'' of iteration m lives in Sa when m is odd, Sb when m is even.
+// S'' of iteration m lives in Sa when m is odd, Sb when m is even.  With
+// hat_next non-null the first Shat of the next step, kappa*normalize((S +
+// S)/2) as k_dipole_mid_hat forms it, is written too (with images).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dipole_mid_finalize(Couplings<T> C, T* __restrict__ S,
                                                                 const T* __restrict__ Sa, const T* __restrict__ Sb,
-                                                                const int* __restrict__ conv_iter, int iter)
+                                                                const int* __restrict__ conv_iter, int iter,
+                                                                T* __restrict__ hat_next)
 {
     const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
     const int c = sc.valid ? conv_iter[sc.r] : 0;  // iter 0: every converged replica
     if (c == 0 || (iter != 0 && c != iter)) return;
     const T* src = (c & 1) ? Sa : Sb;
-    store3_images(S, C.total, sc, C.G, normalize_dipole(planes(src, C.total)(sc.g), C.kappa[sc.i]));
+    const T kap = C.kappa[sc.i];
+    const V3<T> s = normalize_dipole(planes(src, C.total)(sc.g), kap);
+    store3_images(S, C.total, sc, C.G, s);
+    if (hat_next)
+        store3_images(hat_next, C.total, sc, C.G,
+                      normalize_dipole(V3<T>{(s.x + s.x) / 2, (s.y + s.y) / 2, (s.z + s.z) / 2}, kap));
 }
 
 // ---------------------------------------------------------------------------
@@ -385,16 +393,19 @@ __global__ void k_fill_int(int* __restrict__ p, int n, int v)
 }
 
 // After step s of a budgeted group: fail_step[r] = s for replicas that have
-// not converged within the budget (the first such step), and
-// max_iter[r] = the largest iteration count of the group so far.
-__global__ void k_mid_budget(const int* __restrict__ conv_iter, int* __restrict__ fail_step,
-                             int* __restrict__ max_iter, int R, int s)
+// not converged within the budget (the first such step), max_iter[r] = the
+// largest iteration count of the group so far, last_it[r] = this step's
+// count; conv_iter[r] is cleared for the next step.
+__global__ void k_mid_budget(int* __restrict__ conv_iter, int* __restrict__ fail_step, int* __restrict__ max_iter,
+                             int* __restrict__ last_it, int R, int s)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= R) return;
     const int c = conv_iter[r];
     if (c == 0 && fail_step[r] < 0) fail_step[r] = s;
     if (s == 0 || c > max_iter[r]) max_iter[r] = c;
+    last_it[r] = c;
+    conv_iter[r] = 0;
 }
 
 __global__ void k_step_set(unsigned long long* __restrict__ p, unsigned long long v)
@@ -466,9 +477,10 @@ void launch_dipole_midpoint_iter(const Couplings<T>& C, int nblocks, const T* S,
 
 template <typename T>
 void launch_dipole_midpoint_finalize(const Couplings<T>& C, int nblocks, T* S, const T* Sa, const T* Sb,
-                                     const int* conv_iter, int iter, hipStream_t st)
+                                     const int* conv_iter, int iter, T* hat_next, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_dipole_mid_finalize<T>, dim3(nblocks), dim3(kBlock), 0, st, C, S, Sa, Sb, conv_iter, iter);
+    hipLaunchKernelGGL(k_dipole_mid_finalize<T>, dim3(nblocks), dim3(kBlock), 0, st, C, S, Sa, Sb, conv_iter, iter,
+                       hat_next);
 }
 
 template <typename T>
@@ -500,10 +512,10 @@ void launch_fill_int(int* p, int n, int v, hipStream_t st)
     hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
 }
 
-void launch_mid_budget(const int* conv_iter, int* fail_step, int* max_iter, int R, int s, hipStream_t st)
+void launch_mid_budget(int* conv_iter, int* fail_step, int* max_iter, int* last_it, int R, int s, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mid_budget, dim3((R + 255) / 256), dim3(256), 0, st, conv_iter, fail_step, max_iter, R,
-                       s);
+    hipLaunchKernelGGL(k_mid_budget, dim3((R + 255) / 256), dim3(256), 0, st, conv_iter, fail_step, max_iter,
+                       last_it, R, s);
 }
 
 void launch_step_set(unsigned long long* p, unsigned long long v, hipStream_t st)
@@ -550,9 +562,9 @@ void launch_dipole_midpoint_persist(const Couplings<T>& C, int R, const int* rbl
     template void launch_dipole_midpoint_hat<T>(const Couplings<T>&, int, const T*, const T*, T*, const int*,    \
                                                 hipStream_t);                                                    \
     template void launch_dipole_midpoint_iter<T>(const Couplings<T>&, int, const T*, const T*, T*, const T*,     \
-                                                 const T*, T, T, double*, const int*, T*, hipStream_t);          \
+                                                 const T*, T, T, double*, const int*, T*, hipStream_t);                     \
     template void launch_dipole_midpoint_finalize<T>(const Couplings<T>&, int, T*, const T*, const T*,           \
-                                                     const int*, int, hipStream_t);                              \
+                                                     const int*, int, T*, hipStream_t);                              \
     template void launch_dipole_grad<T>(const Couplings<T>&, int, const T*, const T*, T*, hipStream_t);          \
     template void launch_dipole_langevin_persist<T>(const Couplings<T>&, int, const int*, T*, T*, T*,             \
                                                     const NoiseArgs<T>&, T, T, int, hipStream_t);                \

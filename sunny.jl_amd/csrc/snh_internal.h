@@ -157,9 +157,10 @@ void launch_dipole_midpoint_iter(const Couplings<T>& C, int nblocks, const T* S,
                                  const int* conv_iter, T* hat_out, hipStream_t st);
 // S = kappa n(S'') for replicas that converged at `iter` (0: at any
 // iteration); S'' of iteration m is in Sa for odd m, Sb for even m.
+// hat_next non-null: also the next step's first Shat (with images).
 template <typename T>
 void launch_dipole_midpoint_finalize(const Couplings<T>& C, int nblocks, T* S, const T* Sa, const T* Sb,
-                                     const int* conv_iter, int iter, hipStream_t st);
+                                     const int* conv_iter, int iter, T* hat_next, hipStream_t st);
 template <typename T>
 void launch_dipole_grad(const Couplings<T>& C, int nblocks, const T* S, const T* phi, T* dE, hipStream_t st);
 template <typename T>
@@ -184,8 +185,9 @@ void launch_midpoint_check(const double* partial, const int* rblk, int R, const 
                            int iter, hipStream_t st);
 void launch_fill_int(int* p, int n, int v, hipStream_t st);
 // fail_step[r] = s where replica r has not converged (first such step only);
-// max_iter[r]: largest iteration count of the group's steps 0..s.
-void launch_mid_budget(const int* conv_iter, int* fail_step, int* max_iter, int R, int s, hipStream_t st);
+// max_iter[r]: largest iteration count of the group's steps 0..s; last_it[r]:
+// step s's count; conv_iter cleared for the next step.
+void launch_mid_budget(int* conv_iter, int* fail_step, int* max_iter, int* last_it, int R, int s, hipStream_t st);
 // Persistent small-replica kernels (one workgroup per replica, rblk: its
 // range of logical blocks): nsteps Langevin steps, or nsteps midpoint steps
 // iterated to convergence on the device (iters_out / fail_step per replica).
