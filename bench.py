@@ -586,13 +586,14 @@ def main():
                 "unit": "spin-updates/s", "steps": msteps, "iterations_fp32": int(np.max(it32)),
                 "iterations_fp64": int(np.max(it64)), "tol_fp32": 1e-5, "tol_fp64": 1e-10,
                 "group_reruns_fp32": int(hb.query("mid_reruns")), "group_reruns_fp64": int(hp.query("mid_reruns")),
-                # one fixed-point sweep reads Shat (gathered), S, S'; writes S'', next Shat.
-                # Time per working iteration: all launches (including the
-                # budget's launches after convergence, which return at once)
+                # one fixed-point sweep reads Shat (gathered), S and the previous mu;
+                # writes the next Shat and mu (the iterate S'' = 2 mu Shat - S stays
+                # implicit).  Time per working iteration: all launches (including
+                # the budget's launches after convergence, which return at once)
                 # over iterations x steps
                 "iter_kernel_ms": mi_ms / max(int(np.max(it32)) * msteps, 1), "iter_kernel_launches": mi_n,
-                "iter_kernel_bytes_per_spin": 60}
-    midpoint["iter_kernel_GBs"] = 60 * batch.nsites / (midpoint["iter_kernel_ms"] * 1e-3) / 1e9
+                "iter_kernel_bytes_per_spin": 44}
+    midpoint["iter_kernel_GBs"] = 44 * batch.nsites / (midpoint["iter_kernel_ms"] * 1e-3) / 1e9
     midpoint["speedup"] = midpoint["fp32_batched"] / midpoint["fp64_plain"]
     log(f"midpoint: FP32 batched {midpoint['fp32_batched']:.3e} ({midpoint['iterations_fp32']} it), "
         f"FP64 plain {midpoint['fp64_plain']:.3e} ({midpoint['iterations_fp64']} it), "

@@ -446,6 +446,7 @@ struct SystemImpl final : snh_system {
     T* B2 = nullptr;
     T* Bh = nullptr;    // materialised midpoint state: Shat (:dipole) or Zbar (SU(N) + Ewald)
     T* Bh2 = nullptr;   // :dipole: the next iteration's Shat
+    T* Bmu = nullptr;   // tiled :dipole: mu = |(S + S'')/2| / kappa of odd ([0]) and even ([total]) iterations
     T* Sd = nullptr;    // expected dipoles of a coherent state (SU(N) + Ewald)
     T* Dz = nullptr;    // SU(N) Langevin without biquadratic bonds: expected dipoles
     T* Dzp = nullptr;   //   of the state and of the predictor stage, with images
@@ -1160,7 +1161,9 @@ struct SystemImpl final : snh_system {
         if ((ewald || N == 0) && !Bh) Bh = own(dalloc<T>(n));
         if (N == 0 && !Bh2) Bh2 = own(dalloc<T>(n));
         if (!last_it) last_it = own(dalloc<int>(R));
-        SNH_REQUIRE(B1 && B2 && (Bh || (!ewald && N != 0)) && (Bh2 || N != 0) && last_it, SNH_ERR_HIP,
+        if (tiled && N == 0 && !Bmu) Bmu = own(dalloc<T>(2 * (size_t)total));
+        SNH_REQUIRE(B1 && B2 && (Bh || (!ewald && N != 0)) && (Bh2 || N != 0) && last_it && (Bmu || !tiled || N != 0),
+                    SNH_ERR_HIP,
                     "midpoint buffers missing");
     }
 
@@ -1864,9 +1867,10 @@ struct SystemImpl final : snh_system {
             phi = phi1;
         }
         hipEvent_t e = tbegin();
-        if (tiled)
+        if (tiled)  // iteration m keeps mu_m in Bmu + (m odd ? 0 : total)
             launch_tile_midpoint_iter<T>(C, tc.A, tc.threads, X, S1, S2, hat, phi, tdt, tl, partial, conv_iter,
-                                         hat_next, st);
+                                         hat_next, it == 1 ? nullptr : Bmu + (it % 2 ? total : 0),
+                                         Bmu + (it % 2 ? 0 : total), st);
         else if (N == 0)
             launch_dipole_midpoint_iter<T>(C, nblocks, X, S1, S2, hat, phi, tdt, tl, partial, conv_iter, hat_next,
                                            st);
@@ -1880,7 +1884,10 @@ struct SystemImpl final : snh_system {
     // also the first Shat of the next step.
     void midpoint_finalize(bool next_hat = false)
     {
-        if (N == 0)
+        if (N == 0 && tiled)  // Shat of iteration m + 1 in hats[(m + 1) % 2] = {Bh2, Bh}
+            launch_dipole_midpoint_finalize_mu<T>(C, nblocks, X, Bh2, Bh, Bmu, Bmu + total, conv_iter,
+                                                  next_hat ? Bh : nullptr, st);
+        else if (N == 0)
             launch_dipole_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, 0, next_hat ? Bh : nullptr, st);
         else
             launch_sun_midpoint_finalize<T>(C, nblocks, X, B1, B2, conv_iter, 0, st);

@@ -176,6 +176,29 @@ __global__ __launch_bounds__(kBlock) void k_dipole_mid_finalize(Couplings<T> C, 
                       normalize_dipole(V3<T>{(s.x + s.x) / 2, (s.y + s.y) / 2, (s.z + s.z) / 2}, kap));
 }
 
+// The same for the tiled sweep's implicit iterates: S'' of iteration m is
+// 2 mu_m Shat_{m+1} - S, with Shat_{m+1} in Ho (m odd) or He (m even) and
+// mu_m in Mo or Me (every converged replica).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dipole_mid_finalize_mu(Couplings<T> C, T* __restrict__ S, const T* Ho,
+                                                                   const T* He, const T* __restrict__ Mo,
+                                                                   const T* __restrict__ Me,
+                                                                   const int* __restrict__ conv_iter, T* hat_next)
+{
+    const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
+    const int c = sc.valid ? conv_iter[sc.r] : 0;
+    if (c == 0) return;
+    const V3<T> h = planes((c & 1) ? Ho : He, C.total)(sc.g);
+    const T t = 2 * ldg((c & 1) ? Mo : Me, sc.g);
+    const V3<T> s0 = planes(S, C.total)(sc.g);
+    const T kap = C.kappa[sc.i];
+    const V3<T> s = normalize_dipole(V3<T>{t * h.x - s0.x, t * h.y - s0.y, t * h.z - s0.z}, kap);
+    store3_images(S, C.total, sc, C.G, s);
+    if (hat_next)  // may be the buffer h came from: each site reads its own entry first
+        store3_images(hat_next, C.total, sc, C.G,
+                      normalize_dipole(V3<T>{(s.x + s.x) / 2, (s.y + s.y) / 2, (s.z + s.z) / 2}, kap));
+}
+
 // ---------------------------------------------------------------------------
 // Persistent kernels for small replicas: one workgroup of kPersist threads
 // per replica runs many steps, its 256-site blocks kPersistLanes at a time,
@@ -484,6 +507,14 @@ void launch_dipole_midpoint_finalize(const Couplings<T>& C, int nblocks, T* S, c
 }
 
 template <typename T>
+void launch_dipole_midpoint_finalize_mu(const Couplings<T>& C, int nblocks, T* S, const T* Ho, const T* He,
+                                        const T* Mo, const T* Me, const int* conv_iter, T* hat_next, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dipole_mid_finalize_mu<T>, dim3(nblocks), dim3(kBlock), 0, st, C, S, Ho, He, Mo, Me,
+                       conv_iter, hat_next);
+}
+
+template <typename T>
 void launch_dipole_grad(const Couplings<T>& C, int nblocks, const T* S, const T* phi, T* dE, hipStream_t st)
 {
     hipLaunchKernelGGL((k_dipole_grad<T, 6>), dim3(nblocks), dim3(kBlock), 0, st, C, S, phi, dE);
@@ -563,6 +594,8 @@ void launch_dipole_midpoint_persist(const Couplings<T>& C, int R, const int* rbl
                                                 hipStream_t);                                                    \
     template void launch_dipole_midpoint_iter<T>(const Couplings<T>&, int, const T*, const T*, T*, const T*,     \
                                                  const T*, T, T, double*, const int*, T*, hipStream_t);                     \
+    template void launch_dipole_midpoint_finalize_mu<T>(const Couplings<T>&, int, T*, const T*, const T*,        \
+                                                        const T*, const T*, const int*, T*, hipStream_t);       \
     template void launch_dipole_midpoint_finalize<T>(const Couplings<T>&, int, T*, const T*, const T*,           \
                                                      const int*, int, T*, hipStream_t);                              \
     template void launch_dipole_grad<T>(const Couplings<T>&, int, const T*, const T*, T*, hipStream_t);          \

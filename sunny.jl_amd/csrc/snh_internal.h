@@ -161,6 +161,11 @@ void launch_dipole_midpoint_iter(const Couplings<T>& C, int nblocks, const T* S,
 template <typename T>
 void launch_dipole_midpoint_finalize(const Couplings<T>& C, int nblocks, T* S, const T* Sa, const T* Sb,
                                      const int* conv_iter, int iter, T* hat_next, hipStream_t st);
+// The same after tiled sweeps with implicit iterates: S'' = 2 mu Shat - S
+// from the converged iteration's Shat (Ho / He: odd / even iteration) and mu.
+template <typename T>
+void launch_dipole_midpoint_finalize_mu(const Couplings<T>& C, int nblocks, T* S, const T* Ho, const T* He,
+                                        const T* Mo, const T* Me, const int* conv_iter, T* hat_next, hipStream_t st);
 template <typename T>
 void launch_dipole_grad(const Couplings<T>& C, int nblocks, const T* S, const T* phi, T* dE, hipStream_t st);
 template <typename T>
@@ -276,12 +281,13 @@ template <typename T>
 void launch_tile_correct(const Couplings<T>& C, const TileArgs& A, int nthreads, const T* Sp, const T* H, T* Sout,
                          const T* phi, T dt, T lambda, hipStream_t st);
 template <typename T>
-// Shat non-null: the ring gathers these planes; null: it forms
-// kappa n((S + S1)/2) at load.  hat_out non-null: the next iteration's Shat
-// is written for every site.
+// Shat non-null: the ring gathers these planes and the sweep writes the next
+// Shat (hat_out) and mu = |(S + S'')/2| / kappa (mu_out) instead of S''
+// (mu_prev: the previous iteration's mu, null at the first); null: it forms
+// kappa n((S + S1)/2) at load and writes S'' to S2.
 void launch_tile_midpoint_iter(const Couplings<T>& C, const TileArgs& A, int nthreads, const T* S, const T* S1,
                                T* S2, const T* Shat, const T* phi, T dt, T lambda, double* partial,
-                               const int* conv_iter, T* hat_out, hipStream_t st);
+                               const int* conv_iter, T* hat_out, const T* mu_prev, T* mu_out, hipStream_t st);
 template <typename T>
 void launch_tile_grad(const Couplings<T>& C, const TileArgs& A, int nthreads, const T* S, const T* phi, T* dE,
                       hipStream_t st);

@@ -286,6 +286,7 @@ __device__ __forceinline__ V3<T> tile_field(const Couplings<T>& C, const int* __
 template <typename T>
 struct OwnIn {
     V3<T> a, b, p;
+    T m;
 };
 
 #define SNH_TILE_SITES(T_)                                                                          \
@@ -419,6 +420,11 @@ __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_correct(Couplings<T> 
 // Shat = kappa*normalize((S + S1)/2) (or a materialised Shat when a global
 // term needs it), with the FP64 residual sum |S1 - S2|^2 per workgroup.
 // kLean: isotropic bilinear bonds, no Ewald potential, onsite rank <= 2.
+// With Shat given the iterates stay implicit: the sweep writes the next
+// Shat = kappa n(v), v = (S + S'')/2, and mu = |v|/kappa (mu_out), and
+// S'' = 2 mu Shat - S is formed where needed (S1 of the residual from the
+// previous iteration's mu_prev and this Shat; null mu_prev: S1 = S, the
+// first guess): 44 bytes per site instead of 60.
 template <typename T, int CH, bool kHatGiven, int SP, bool kLean>
 __global__ __attribute__((amdgpu_waves_per_eu(correct_waves<T, SP, kLean && kHatGiven>())))
 __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_mid_iter(Couplings<T> C, TileArgs A,
@@ -429,7 +435,9 @@ __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_mid_iter(Couplings<T>
                                                                          const T* __restrict__ phi, T dt, T lambda,
                                                                          double* __restrict__ partial,
                                                                          const int* __restrict__ conv_iter,
-                                                                         T* __restrict__ hat_out)
+                                                                         T* __restrict__ hat_out,
+                                                                         const T* __restrict__ mu_prev,
+                                                                         T* __restrict__ mu_out)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int wg = logical_block();
@@ -441,7 +449,10 @@ __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_mid_iter(Couplings<T>
     auto load = [&](const TileSite& ts, int k, OwnIn<T>& in) __attribute__((always_inline)) {
         const uint32_t g = ts.gb + lay * (uint32_t)k;
         in.a = plane_v3(S, C.total, g);
-        in.b = plane_v3(S1, C.total, g);
+        if (!kHatGiven)
+            in.b = plane_v3(S1, C.total, g);
+        else if (mu_prev)
+            in.m = ldg(mu_prev, g);
         if (!kLean && phi) in.p = plane_v3(phi, C.total, g);
     };
     auto site = [&](const TileSite& ts, int k, int ph, const int* offs, const OwnIn<T>& in) __attribute__((always_inline)) {
@@ -454,14 +465,24 @@ __launch_bounds__((tile_threads_max<T, SP>())) void k_tile_mid_iter(Couplings<T>
         if (!kLean && phi) add_ewald_field(C, i, in.p, f);
         const V3<T> d = rhs_det(hat, f, dt, lambda);
         const V3<T> o = {in.a.x + d.x, in.a.y + d.y, in.a.z + d.z};
-        plane_store(S2, C.total, g, o);
-        // the next iteration's Shat = kappa n((S + S'')/2) at this site
-        if (hat_out)
+        V3<T> s1 = in.b;
+        if (kHatGiven) {
+            // the next iteration's Shat = kappa n(v), v = (S + S'')/2, and mu
+            T mu;
             plane_store(hat_out, C.total, g,
-                        normalize_dipole(V3<T>{(in.a.x + o.x) / 2, (in.a.y + o.y) / 2, (in.a.z + o.z) / 2},
-                                         ldc(C.kappa, i)));
-        const double ex = (double)in.b.x - (double)o.x, ey = (double)in.b.y - (double)o.y,
-                     ez = (double)in.b.z - (double)o.z;
+                        normalize_dipole_mu(V3<T>{(in.a.x + o.x) / 2, (in.a.y + o.y) / 2, (in.a.z + o.z) / 2},
+                                            ldc(C.kappa, i), mu));
+            stg(mu_out, g, mu);
+            // the previous iterate S'' = 2 mu_prev Shat - S (S at the first iteration)
+            s1 = in.a;
+            if (mu_prev) {
+                const T t = 2 * in.m;
+                s1 = {t * hat.x - in.a.x, t * hat.y - in.a.y, t * hat.z - in.a.z};
+            }
+        } else {
+            plane_store(S2, C.total, g, o);
+        }
+        const double ex = (double)s1.x - (double)o.x, ey = (double)s1.y - (double)o.y, ez = (double)s1.z - (double)o.z;
         d2 += ex * ex + ey * ey + ez * ez;
     };
     auto pre = [&](int k) __attribute__((always_inline)) {
@@ -661,7 +682,7 @@ void launch_tile_correct(const Couplings<T>& C, const TileArgs& A, int nthreads,
 template <typename T>
 void launch_tile_midpoint_iter(const Couplings<T>& C, const TileArgs& A, int nthreads, const T* S, const T* S1,
                                T* S2, const T* Shat, const T* phi, T dt, T lambda, double* partial,
-                               const int* conv_iter, T* hat_out, hipStream_t st)
+                               const int* conv_iter, T* hat_out, const T* mu_prev, T* mu_out, hipStream_t st)
 {
     const size_t lds = tile_lds_bytes(A, sizeof(T));
     const bool lean = C.lean_ok && !phi && C.iso;
@@ -670,11 +691,11 @@ void launch_tile_midpoint_iter(const Couplings<T>& C, const TileArgs& A, int nth
         if (lean) {                                                                                                \
             launch_tile(k_tile_mid_iter<T, 6, HAT, SP, true>, A, nthreads, lds, st);                               \
             hipLaunchKernelGGL((k_tile_mid_iter<T, 6, HAT, SP, true>), dim3(A.nwork), dim3(nthreads), lds, st, C, A, \
-                               S, S1, S2, Shat, phi, dt, lambda, partial, conv_iter, hat_out);                     \
+                               S, S1, S2, Shat, phi, dt, lambda, partial, conv_iter, hat_out, mu_prev, mu_out);    \
         } else {                                                                                                   \
             launch_tile(k_tile_mid_iter<T, 6, HAT, SP, false>, A, nthreads, lds, st);                              \
             hipLaunchKernelGGL((k_tile_mid_iter<T, 6, HAT, SP, false>), dim3(A.nwork), dim3(nthreads), lds, st, C,   \
-                               A, S, S1, S2, Shat, phi, dt, lambda, partial, conv_iter, hat_out);                  \
+                               A, S, S1, S2, Shat, phi, dt, lambda, partial, conv_iter, hat_out, mu_prev, mu_out); \
         }                                                                                                          \
     } while (0)
     if (Shat) {
@@ -719,7 +740,8 @@ void launch_tile_energy(const Couplings<T>& C, const TileArgs& A, int nthreads, 
     template void launch_tile_correct<T>(const Couplings<T>&, const TileArgs&, int, const T*, const T*, T*,       \
                                          const T*, T, T, hipStream_t);                                           \
     template void launch_tile_midpoint_iter<T>(const Couplings<T>&, const TileArgs&, int, const T*, const T*, T*, \
-                                               const T*, const T*, T, T, double*, const int*, T*, hipStream_t);  \
+                                               const T*, const T*, T, T, double*, const int*, T*, const T*, T*, \
+                                               hipStream_t);                                                     \
     template void launch_tile_grad<T>(const Couplings<T>&, const TileArgs&, int, const T*, const T*, T*,          \
                                       hipStream_t);                                                              \
     template void launch_tile_energy<T>(const Couplings<T>&, const TileArgs&, int, const T*, const T*, double*,   \

@@ -40,6 +40,32 @@ __device__ __forceinline__ V3<float> normalize_dipole(V3<float> s, float kappa)
     return {s.x * f, s.y * f, s.z * f};
 }
 
+// normalize_dipole together with mu = |s| / kappa, the factor with s = mu *
+// result (0 for kappa = 0): the midpoint sweep keeps mu beside the next
+// Shat instead of the iterate S'' = 2 mu Shat - S.
+__device__ __forceinline__ V3<double> normalize_dipole_mu(V3<double> s, double kappa, double& mu)
+{
+    if (kappa == 0.0) {
+        mu = 0;
+        return {0, 0, 0};
+    }
+    const double n = sqrt(s.x * s.x + s.y * s.y + s.z * s.z);
+    const double f = kappa / n;
+    mu = n / kappa;
+    return {s.x * f, s.y * f, s.z * f};
+}
+
+__device__ __forceinline__ V3<float> normalize_dipole_mu(V3<float> s, float kappa, float& mu)
+{
+    if (kappa == 0.0f) {
+        mu = 0;
+        return {0, 0, 0};
+    }
+    const float f = kappa * __builtin_amdgcn_rsqf(s.x * s.x + s.y * s.y + s.z * s.z);
+    mu = __builtin_amdgcn_rcpf(f);
+    return {s.x * f, s.y * f, s.z * f};
+}
+
 // Stevens quadrupoles of a classical dipole and their gradient, Spin.jl:48-69.
 template <typename T>
 __device__ __forceinline__ void quadrupole(V3<T> s, T Q[5])
