@@ -98,6 +98,13 @@ int snh_destroy(snh_system *sys);
  * snh_create when sys is NULL).  Valid until the next call on the handle. */
 const char *snh_last_error(const snh_system *sys);
 
+/* No counterpart in the reference (build provenance): the first 16 hex
+ * digits of SHA-256 over the library's sources and this header, in sorted
+ * path order, as the build computed them ("unknown" for a build without
+ * the id).  The Python loader recomputes it from the sources beside the
+ * library and refuses a library built from other sources. */
+const char *snh_build_id(void);
+
 /* Total sites over all replicas, and sites of replica r. */
 int snh_num_sites(const snh_system *sys, int64_t *total);
 int snh_replica_sites(const snh_system *sys, int32_t r, int64_t *nsites);

@@ -2395,6 +2395,12 @@ extern "C" const char* snh_last_error(const snh_system* sys)
     return sys ? sys->err.c_str() : g_create_error.c_str();
 }
 
+#ifndef SNH_BUILD_ID
+#define SNH_BUILD_ID "unknown"
+#endif
+
+extern "C" const char* snh_build_id(void) { return SNH_BUILD_ID; }
+
 extern "C" int snh_num_sites(const snh_system* sys, int64_t* total)
 {
     if (!sys || !total) return SNH_ERR_INVALID;

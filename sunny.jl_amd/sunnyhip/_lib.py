@@ -6,6 +6,8 @@ been built; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import glob
+import hashlib
 import os
 
 import numpy as np
@@ -21,7 +23,7 @@ SNH_ERR_NOCONVERGE = 3
 SNH_ERR_UNSUPPORTED = 4
 
 EXPORTS = [
-    "snh_create", "snh_destroy", "snh_last_error", "snh_num_sites", "snh_replica_sites",
+    "snh_create", "snh_destroy", "snh_last_error", "snh_build_id", "snh_num_sites", "snh_replica_sites",
     "snh_set_dipoles", "snh_get_dipoles", "snh_set_coherents", "snh_get_coherents",
     "snh_step_langevin", "snh_step_midpoint", "snh_energy", "snh_energy_grad", "snh_synchronize",
     "snh_enable_timing", "snh_kernel_time", "snh_copy_calibration", "snh_query",
@@ -53,6 +55,22 @@ class ConvergenceError(RuntimeError):
 
 
 _lib = None
+
+
+def source_build_id():
+    """The build id of the sources beside this package, as the Makefile
+    computes it (SHA-256 over csrc/*.hip, *.h, *.cpp and ../include/*.h in
+    sorted path order, first 16 hex digits); None when they are absent."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    rel = [os.path.relpath(f, root) for pat in ("csrc/*.hip", "csrc/*.h", "csrc/*.cpp", "../include/*.h")
+           for f in glob.glob(os.path.join(root, pat))]
+    if not rel:
+        return None
+    h = hashlib.sha256()
+    for r in sorted(rel):
+        with open(os.path.join(root, r), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
 
 
 def lib():
@@ -93,9 +111,18 @@ def lib():
         L.snh_local_coloring.argtypes = [vp, ip, ip, ip]
         L.snh_midpoint_record.argtypes = [vp, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32]
         L.snh_midpoint_adjoint.argtypes = [vp, C.c_int32, C.POINTER(SnhSpec), dp, dp, dp]
+        L.snh_build_id.argtypes = []
+        L.snh_build_id.restype = C.c_char_p
         for f in EXPORTS:
-            if f != "snh_last_error":
+            if f not in ("snh_last_error", "snh_build_id"):
                 getattr(L, f).restype = C.c_int
+        # a library built from other sources than those beside it is refused
+        # (SUNNYHIP_LIB, for timing variants, skips the check)
+        want = source_build_id()
+        have = L.snh_build_id().decode()
+        if want is not None and not os.environ.get("SUNNYHIP_LIB") and have != want:
+            raise ImportError(f"{LIB_PATH} was built from other sources (build id {have}, sources {want}); "
+                              "rebuild it (python -c 'import __graft_entry__ as g; g.build()')")
         _lib = L
     return _lib
 

@@ -98,3 +98,12 @@ def test_create_without_gpu_fails_cleanly():
         return
     del h
     pytest.skip("a GPU is present")
+
+
+def test_library_built_from_these_sources():
+    """The loaded library carries the build id of the sources beside it
+    (the loader refuses a stale build)."""
+    from sunnyhip import _lib
+    want = _lib.source_build_id()
+    assert want is not None and len(want) == 16
+    assert _lib.lib().snh_build_id().decode() == want
