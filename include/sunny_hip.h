@@ -195,7 +195,8 @@ int snh_synchronize(snh_system *sys);
  * The handle's state is left at the final state.
  *
  * Supported: one replica in :dipole mode at precision 64, bilinear pair
- * couplings, Zeeman, onsite anisotropy of order <= 2, no Ewald term.
+ * couplings, Zeeman, onsite anisotropy of order <= 2, the Ewald term (its
+ * strength is not a parameter).
  * Sunny has no reverse rule for step!; these entry points are what a
  * ChainRulesCore rrule would call (the pattern of src/Fitting.jl:123-152). */
 int snh_midpoint_record(snh_system *sys, double dt, double tol, int32_t max_iters, int32_t nsteps,

@@ -12,7 +12,10 @@ reference would.
 
 Couplings supported: bilinear exchange (any 3x3 J per directed bond),
 Zeeman, onsite anisotropy of order <= 2 (gradient 2 Q2 S with |S|^2 held
-constant, OnsiteCoupling.jl:316-425).  The couplings are linear in the
+constant, OnsiteCoupling.jl:316-425), and the Ewald dipole-dipole term
+(Ewald.jl:185-209), which is linear in S: its field is applied as the dense
+matrix whose columns are the oracle's pairwise Ewald field
+(so_ewald_grad_bruteforce) of unit dipoles.  The couplings are linear in the
 parameters (ModelParam, src/System/Types.jl:76-86): the tangent of the
 couplings along parameter p is that parameter's unit coupling spec.
 """
@@ -108,7 +111,7 @@ class MidpointAD:
     Array{Vec3,4}(L1,L2,L3,na), bonds gathered per target site."""
 
     def __init__(self, spec, param_specs):
-        assert spec["N"] == 0 and spec["nreplicas"] == 1 and not spec["ewald"]
+        assert spec["N"] == 0 and spec["nreplicas"] == 1
         L1, L2, L3 = (int(x) for x in spec["dims"][0])
         na = int(spec["na"])
         self.P = len(param_specs)
@@ -143,6 +146,16 @@ class MidpointAD:
             zee[sl] = g.T @ np.asarray(spec["field"][i], float)
         self.Q = Dual(Qv, Qd)
         self.zee = zee
+        self.K = None  # Ewald field operator [3 Ns, 3 Ns] (parameter-independent)
+        if spec["ewald"]:
+            from .oracle import Oracle
+            o = Oracle(spec)
+            n3 = 3 * self.Ns
+            self.K = np.zeros((n3, n3))
+            for k in range(n3):
+                e = np.zeros(n3)
+                e[k] = 1.0
+                self.K[:, k] = o.ewald_grad_bruteforce(e.reshape(self.Ns, 3)).reshape(-1)
 
     def field(self, s):
         """dE/dS (Dual [Ns, 3])."""
@@ -155,6 +168,9 @@ class MidpointAD:
             t = _matvec(Jb, sj)
             np.add.at(acc_v, src, t.v)
             np.add.at(acc_d, src, t.d)
+        if self.K is not None:
+            acc_v += (self.K @ s.v.reshape(-1)).reshape(self.Ns, 3)
+            acc_d += (self.K @ s.d.reshape(3 * self.Ns, -1)).reshape(acc_d.shape)
         return Dual(acc_v, acc_d)
 
     def step(self, S, dt, tol, max_iters=100):

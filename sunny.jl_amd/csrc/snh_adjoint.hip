@@ -66,13 +66,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_w(Couplings<T> C, const T* __res
 
 // One sweep nu' = mu + (1/2) N(m) (-dt g x nu + M w), with the FP64 sum of
 // |nu' - nu|^2 per block.  Clin is the coupling set without Zeeman term, so
-// its field of w is M w.
+// its field of w is M w; phi_w (may be null) is the Ewald potential of w,
+// whose field -g' phi_w is the Ewald part of M w (the Ewald field is linear
+// and symmetric in S).
 template <typename T, int CH>
 __global__ __launch_bounds__(kBlock) void k_adj_apply(Couplings<T> Clin, const T* __restrict__ w,
                                                       const T* __restrict__ S, const T* __restrict__ Sdd,
                                                       const T* __restrict__ g, const T* __restrict__ nu,
                                                       const T* __restrict__ mu, T dt, T* __restrict__ nu_out,
-                                                      double* __restrict__ partial)
+                                                      double* __restrict__ partial, const T* __restrict__ phi_w)
 {
     const SiteCoord sc = site_coord(Clin.blocks, Clin.reps, Clin.G);
     double r2 = 0;
@@ -80,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_apply(Couplings<T> Clin, const T
         const uint32_t tot = Clin.total;
         const Planes<T> Wv = planes(w, tot);
         const SpinSrc<T, false> src{Wv, Wv};
-        const V3<T> hw = local_field<T, CH, false>(Clin, sc, src, Wv(sc.g), nullptr);
+        const V3<T> hw = local_field<T, CH, false>(Clin, sc, src, Wv(sc.g), phi_w);
         const V3<T> gv = planes(g, tot)(sc.g), nv = planes(nu, tot)(sc.g);
         const V3<T> gx = cross3(gv, nv);
         const V3<T> t = {hw.x - dt * gx.x, hw.y - dt * gx.y, hw.z - dt * gx.z};
@@ -154,11 +156,11 @@ void launch_adj_w(const Couplings<T>& C, int nblocks, const T* nu, const T* Shat
 
 template <typename T>
 void launch_adj_apply(const Couplings<T>& Clin, int nblocks, const T* w, const T* S, const T* Sdd, const T* g,
-                      const T* nu, const T* mu, T dt, T* nu_out, double* partial, hipStream_t st)
+                      const T* nu, const T* mu, T dt, T* nu_out, double* partial, const T* phi_w, hipStream_t st)
 {
 #define SNH_AA(CH)                                                                                             \
     hipLaunchKernelGGL((k_adj_apply<T, CH>), dim3(nblocks), dim3(kBlock), 0, st, Clin, w, S, Sdd, g, nu, mu, dt, \
-                       nu_out, partial)
+                       nu_out, partial, phi_w)
     if (Clin.chunk == 18) SNH_AA(18);
     else if (Clin.chunk == 9) SNH_AA(9);
     else SNH_AA(6);
@@ -192,7 +194,7 @@ void launch_adj_accumulate(const double* partial, int n, double* out, int slot, 
     template void launch_adj_prep<T>(const Couplings<T>&, int, const T*, const T*, const T*, T*, T*, hipStream_t); \
     template void launch_adj_w<T>(const Couplings<T>&, int, const T*, const T*, T, T*, hipStream_t);              \
     template void launch_adj_apply<T>(const Couplings<T>&, int, const T*, const T*, const T*, const T*, const T*,  \
-                                      const T*, T, T*, double*, hipStream_t);                                    \
+                                      const T*, T, T*, double*, const T*, hipStream_t);                          \
     template void launch_adj_param<T>(const Couplings<T>&, int, const T*, const T*, double*, hipStream_t);        \
     template void launch_adj_lambda<T>(const Couplings<T>&, int, const T*, const T*, T*, hipStream_t);
 

@@ -1991,7 +1991,6 @@ struct SystemImpl final : snh_system {
         SNH_REQUIRE(sizeof(T) == 8, SNH_ERR_UNSUPPORTED, "the midpoint adjoint runs at precision 64");
         SNH_REQUIRE(R == 1, SNH_ERR_UNSUPPORTED, "the midpoint adjoint takes one replica");
         SNH_REQUIRE(!tiled, SNH_ERR_UNSUPPORTED, "the midpoint adjoint needs the ghost-padded layout");
-        SNH_REQUIRE(!ewald, SNH_ERR_UNSUPPORTED, "the midpoint adjoint does not support the Ewald term");
         SNH_REQUIRE(!any_biquad, SNH_ERR_UNSUPPORTED, "the midpoint adjoint supports bilinear pair couplings");
         for (int k : kmax_h)
             SNH_REQUIRE(k <= 2, SNH_ERR_UNSUPPORTED, "the midpoint adjoint supports onsite anisotropy of order <= 2");
@@ -2155,13 +2154,18 @@ struct SystemImpl final : snh_system {
                     const T* S = segS[n - n0];
                     const T* Sdd = segD[n - n0];
                     launch_adj_prep<T>(C, nblocks, S, Sdd, lam, Shat, mu, st);
-                    launch_dipole_grad<T>(C, nblocks, Shat, nullptr, g, st);
+                    // the field at Shat, with its Ewald potential (phi1)
+                    if (ewald) ewald_field(Shat, phi1);
+                    launch_dipole_grad<T>(C, nblocks, Shat, ewald ? phi1 : nullptr, g, st);
                     SNH_HIP(hipMemcpyAsync(nu, mu, bytes, hipMemcpyDeviceToDevice, st));
                     // the linear sweep contracts like the forward fixed point
                     const int sweeps = rec_iters[n] + 2;
                     for (int k = 0; k < sweeps; k++) {
                         launch_adj_w<T>(C, nblocks, nu, Shat, tdt, w, st);
-                        launch_adj_apply<T>(Clin, nblocks, w, S, Sdd, g, nu, mu, tdt, nu2, partial, st);
+                        // the Ewald part of M w: the field of w's Ewald potential (phi2)
+                        if (ewald) ewald_field(w, phi2);
+                        launch_adj_apply<T>(Clin, nblocks, w, S, Sdd, g, nu, mu, tdt, nu2, partial,
+                                            ewald ? phi2 : nullptr, st);
                         std::swap(nu, nu2);
                     }
                     launch_adj_w<T>(C, nblocks, nu, Shat, tdt, w, st);
