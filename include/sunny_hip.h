@@ -194,9 +194,10 @@ int snh_synchronize(snh_system *sys);
  *   dLdS_initial  dL/dS at the recorded initial state ([sites][3]; may be NULL).
  * The handle's state is left at the final state.
  *
- * Supported: one replica in :dipole mode at precision 64, bilinear pair
- * couplings, Zeeman, onsite anisotropy of order <= 2, the Ewald term (its
- * strength is not a parameter).
+ * Supported: one replica in :dipole mode at precision 64, with every
+ * :dipole coupling: bilinear and biquadratic pair couplings, Zeeman, Stevens
+ * anisotropy up to order 6, and the Ewald term (its strength is not a
+ * parameter).
  * Sunny has no reverse rule for step!; these entry points are what a
  * ChainRulesCore rrule would call (the pattern of src/Fitting.jl:123-152). */
 int snh_midpoint_record(snh_system *sys, double dt, double tol, int32_t max_iters, int32_t nsteps,

@@ -12,10 +12,17 @@ reference would.
 
 Couplings supported: bilinear exchange (any 3x3 J per directed bond),
 Zeeman, onsite anisotropy of order <= 2 (gradient 2 Q2 S with |S|^2 held
-constant, OnsiteCoupling.jl:316-425), and the Ewald dipole-dipole term
+constant, OnsiteCoupling.jl:316-425), the Ewald dipole-dipole term
 (Ewald.jl:185-209), which is linear in S: its field is applied as the dense
 matrix whose columns are the oracle's pairwise Ewald field
-(so_ewald_grad_bruteforce) of unit dipoles.  The couplings are linear in the
+(so_ewald_grad_bruteforce) of unit dipoles; and the nonlinear terms,
+biquadratic bonds and Stevens anisotropy of order 4 and 6: their field is
+the C oracle's gradient of a spec holding only those terms, a polynomial of
+degree <= 5 in the spins, so its directional derivative along each tangent
+is the 7-point central difference (exact for polynomials of degree <= 6, so
+only rounding remains), and its derivative along a parameter is the same
+gradient of the parameter's unit couplings (couplings are linear in the
+parameters).  The couplings are linear in the
 parameters (ModelParam, src/System/Types.jl:76-86): the tangent of the
 couplings along parameter p is that parameter's unit coupling spec.
 """
@@ -102,6 +109,24 @@ def _normalize(s, kappa):
     return s * Dual(np.repeat(f.v[..., None], 3, -1), np.repeat(f.d[..., None, :], 3, -2))
 
 
+def _nonlinear_part(spec, sites):
+    """The spec's biquadratic bonds and its Stevens terms on the atoms
+    `sites` (those the system's anisotropy makes nonlinear), nothing else."""
+    s = dict(spec)
+    s["bilin"] = np.zeros_like(np.asarray(spec["bilin"], float))
+    s["scalar"] = np.zeros_like(np.asarray(spec["scalar"], float))
+    s["field"] = np.zeros_like(np.asarray(spec["field"], float))
+    st = np.array(spec["stevens"], float)
+    st[~sites] = 0
+    s["stevens"] = st
+    s["kmax"] = np.where(sites, np.asarray(spec["kmax"]), 0).astype(np.int32)
+    s["ewald"] = 0
+    return s
+
+
+_FD7 = ((1, 45.0), (2, -9.0), (3, 1.0))  # f'(0) = sum_k w_k (f(k h) - f(-k h)) / (60 h)
+
+
 def _q2(c2):
     return np.array([[2 * c2[0], 2 * c2[4], c2[1]], [2 * c2[4], -2 * c2[0], c2[3]], [c2[1], c2[3], 6 * c2[2]]])
 
@@ -137,15 +162,29 @@ class MidpointAD:
         zee = np.zeros((self.Ns, 3))
         for i in range(na):
             sl = slice(self.ncell * i, self.ncell * (i + 1))
-            assert spec["kmax"][i] <= 2
             Qv[sl] = _q2(np.asarray(spec["stevens"][i], float)[1:6])
             for p, ps in enumerate(param_specs):
-                assert ps["kmax"][i] <= 2
                 Qd[sl, :, :, p] = _q2(np.asarray(ps["stevens"][i], float)[1:6])
             g = np.asarray(spec["g"][i], float).reshape(3, 3)
             zee[sl] = g.T @ np.asarray(spec["field"][i], float)
         self.Q = Dual(Qv, Qd)
         self.zee = zee
+        # nonlinear terms through the oracle; their sites drop out of Q
+        self.nl = self.nl_p = None
+        kmax = np.asarray(spec["kmax"])
+        nl_sites = kmax > 2
+        if np.asarray(spec["has_biquad"]).any() or nl_sites.any():
+            from .oracle import Oracle
+            self.nl = Oracle(_nonlinear_part(spec, nl_sites))
+            self.nl_p = [Oracle(_nonlinear_part(dict(spec, bilin=ps["bilin"], biquad=ps["biquad"],
+                                                     has_biquad=ps["has_biquad"], stevens=ps["stevens"],
+                                                     kmax=ps["kmax"]), nl_sites))
+                         for ps in param_specs]
+            for i in range(na):
+                if nl_sites[i]:
+                    sl = slice(self.ncell * i, self.ncell * (i + 1))
+                    self.Q.v[sl] = 0
+                    self.Q.d[sl] = 0
         self.K = None  # Ewald field operator [3 Ns, 3 Ns] (parameter-independent)
         if spec["ewald"]:
             from .oracle import Oracle
@@ -171,6 +210,17 @@ class MidpointAD:
         if self.K is not None:
             acc_v += (self.K @ s.v.reshape(-1)).reshape(self.Ns, 3)
             acc_d += (self.K @ s.d.reshape(3 * self.Ns, -1)).reshape(acc_d.shape)
+        if self.nl is not None:
+            acc_v += self.nl.grad(s.v)
+            for t in range(acc_d.shape[-1]):
+                D = s.d[..., t]
+                m = np.abs(D).max()
+                if m > 0:
+                    h = 0.3 / m
+                    fd = sum(w * (self.nl.grad(s.v + k * h * D) - self.nl.grad(s.v - k * h * D)) for k, w in _FD7)
+                    acc_d[..., t] += fd / (60 * h)
+                if t < len(self.nl_p):  # the parameter's own nonlinear couplings
+                    acc_d[..., t] += self.nl_p[t].grad(s.v)
         return Dual(acc_v, acc_d)
 
     def step(self, S, dt, tol, max_iters=100):

@@ -24,6 +24,72 @@
 
 namespace snh {
 
+// FP64 forward-mode dual number: value and one directional derivative.
+struct Dual {
+    double v, d;
+    __device__ Dual(double a = 0, double b = 0) : v(a), d(b) {}
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a) { return {-a.v, -a.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+__device__ __forceinline__ Dual& operator+=(Dual& a, Dual b)
+{
+    a = a + b;
+    return a;
+}
+
+// The directional derivative, along w, of the field terms that are not
+// linear in the state, at Shat: onsite Stevens anisotropy of order 4 and 6
+// (by dual numbers through stevens_grad) and biquadratic bonds,
+//   g_i += sum_p v_p grad Q_p(S_i),  v = B Q(S_j),
+//   d g_i = sum_p dv_p grad Q_p(S_i) + v_p grad Q_p(w_i),  dv = B dQ(S_j)[w_j]
+// (grad Q_p is linear).  w is tangent to every Shat_i (w = -dt nu x Shat), so
+// holding X = |S|^2 constant, as the Stevens gradient does, changes nothing.
+__device__ __forceinline__ V3<double> adj_nonlinear_jvp(const Couplings<double>& C, const SiteCoord& sc,
+                                                        const double* __restrict__ Shat,
+                                                        const double* __restrict__ w)
+{
+    V3<double> r = {0, 0, 0};
+    const int i = sc.i;
+    const V3<double> s = planes(Shat, C.total)(sc.g), wi = planes(w, C.total)(sc.g);
+    const int km = C.kmax[i];
+    if (km > 2) {
+        Dual gx = 0, gy = 0, gz = 0;
+        stevens_grad<Dual, double>(Dual(s.x, wi.x), Dual(s.y, wi.y), Dual(s.z, wi.z), C.stev + 28 * i, km, gx, gy,
+                                   gz, nullptr);
+        r = {gx.d, gy.d, gz.d};
+    }
+    const int b0 = C.bstart[i], b1 = C.bstart[i + 1];
+    const int* __restrict__ dl = C.bdelta + sc.R.dcls * C.nb;
+    for (int b = b0; b < b1; b++) {
+        if (!(C.bflag[b] & kBondBiquad)) continue;
+        const uint32_t gj = sc.g + (uint32_t)dl[b];
+        const V3<double> sj = planes(Shat, C.total)(gj), wj = planes(w, C.total)(gj);
+        double Qj[5];
+        quadrupole(sj, Qj);
+        const double dQj[5] = {2 * (sj.x * wj.x - sj.y * wj.y), sj.z * wj.x + sj.x * wj.z,
+                               -2 * sj.x * wj.x - 2 * sj.y * wj.y + 4 * sj.z * wj.z, sj.z * wj.y + sj.y * wj.z,
+                               2 * (sj.y * wj.x + sj.x * wj.y)};
+        const double* B = C.bQ + 25 * b;
+        double v[5], dv[5];
+        for (int p = 0; p < 5; p++) {
+            double a = 0, da = 0;
+            for (int q = 0; q < 5; q++) {
+                a += B[5 * p + q] * Qj[q];
+                da += B[5 * p + q] * dQj[q];
+            }
+            v[p] = a;
+            dv[p] = da;
+        }
+        add_grad_quadrupole(s, dv, r);
+        add_grad_quadrupole(wi, v, r);
+    }
+    return r;
+}
+
 // (kappa/|x|) (v - xh (xh . v)), xh = x/|x|: the Jacobian of kappa n(x)
 // (symmetric) applied to v, in FP64.
 template <typename T>
@@ -65,16 +131,20 @@ __global__ __launch_bounds__(kBlock) void k_adj_w(Couplings<T> C, const T* __res
 }
 
 // One sweep nu' = mu + (1/2) N(m) (-dt g x nu + M w), with the FP64 sum of
-// |nu' - nu|^2 per block.  Clin is the coupling set without Zeeman term, so
-// its field of w is M w; phi_w (may be null) is the Ewald potential of w,
-// whose field -g' phi_w is the Ewald part of M w (the Ewald field is linear
-// and symmetric in S).
+// |nu' - nu|^2 per block, where M w is the derivative of the field at Shat
+// along w.  Clin is the coupling set without the Zeeman term and without the
+// nonlinear terms (biquadratic bonds, Stevens order 4 and 6), so its field of
+// w is the linear part of M w; phi_w (may be null) is the Ewald potential of
+// w, whose field -g' phi_w is the Ewald part (the Ewald field is linear and
+// symmetric in S).  Cnl non-null (the full couplings): the nonlinear part at
+// Shat is added (adj_nonlinear_jvp).
 template <typename T, int CH>
 __global__ __launch_bounds__(kBlock) void k_adj_apply(Couplings<T> Clin, const T* __restrict__ w,
                                                       const T* __restrict__ S, const T* __restrict__ Sdd,
                                                       const T* __restrict__ g, const T* __restrict__ nu,
                                                       const T* __restrict__ mu, T dt, T* __restrict__ nu_out,
-                                                      double* __restrict__ partial, const T* __restrict__ phi_w)
+                                                      double* __restrict__ partial, const T* __restrict__ phi_w,
+                                                      Couplings<T> Cnl, const T* __restrict__ Shat)
 {
     const SiteCoord sc = site_coord(Clin.blocks, Clin.reps, Clin.G);
     double r2 = 0;
@@ -82,7 +152,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_apply(Couplings<T> Clin, const T
         const uint32_t tot = Clin.total;
         const Planes<T> Wv = planes(w, tot);
         const SpinSrc<T, false> src{Wv, Wv};
-        const V3<T> hw = local_field<T, CH, false>(Clin, sc, src, Wv(sc.g), phi_w);
+        V3<T> hw = local_field<T, CH, false>(Clin, sc, src, Wv(sc.g), phi_w);
+        if (Shat) {
+            const V3<double> n = adj_nonlinear_jvp(Cnl, sc, Shat, w);
+            hw = {hw.x + n.x, hw.y + n.y, hw.z + n.z};
+        }
         const V3<T> gv = planes(g, tot)(sc.g), nv = planes(nu, tot)(sc.g);
         const V3<T> gx = cross3(gv, nv);
         const V3<T> t = {hw.x - dt * gx.x, hw.y - dt * gx.y, hw.z - dt * gx.z};
@@ -156,11 +230,12 @@ void launch_adj_w(const Couplings<T>& C, int nblocks, const T* nu, const T* Shat
 
 template <typename T>
 void launch_adj_apply(const Couplings<T>& Clin, int nblocks, const T* w, const T* S, const T* Sdd, const T* g,
-                      const T* nu, const T* mu, T dt, T* nu_out, double* partial, const T* phi_w, hipStream_t st)
+                      const T* nu, const T* mu, T dt, T* nu_out, double* partial, const T* phi_w,
+                      const Couplings<T>& Cnl, const T* Shat, hipStream_t st)
 {
 #define SNH_AA(CH)                                                                                             \
     hipLaunchKernelGGL((k_adj_apply<T, CH>), dim3(nblocks), dim3(kBlock), 0, st, Clin, w, S, Sdd, g, nu, mu, dt, \
-                       nu_out, partial, phi_w)
+                       nu_out, partial, phi_w, Cnl, Shat)
     if (Clin.chunk == 18) SNH_AA(18);
     else if (Clin.chunk == 9) SNH_AA(9);
     else SNH_AA(6);
@@ -194,7 +269,8 @@ void launch_adj_accumulate(const double* partial, int n, double* out, int slot, 
     template void launch_adj_prep<T>(const Couplings<T>&, int, const T*, const T*, const T*, T*, T*, hipStream_t); \
     template void launch_adj_w<T>(const Couplings<T>&, int, const T*, const T*, T, T*, hipStream_t);              \
     template void launch_adj_apply<T>(const Couplings<T>&, int, const T*, const T*, const T*, const T*, const T*,  \
-                                      const T*, T, T*, double*, const T*, hipStream_t);                          \
+                                      const T*, T, T*, double*, const T*, const Couplings<T>&, const T*,         \
+                                      hipStream_t);                                                              \
     template void launch_adj_param<T>(const Couplings<T>&, int, const T*, const T*, double*, hipStream_t);        \
     template void launch_adj_lambda<T>(const Couplings<T>&, int, const T*, const T*, T*, hipStream_t);
 

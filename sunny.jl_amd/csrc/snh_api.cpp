@@ -425,6 +425,7 @@ struct SystemImpl final : snh_system {
     int* d_trblk = nullptr;
     // bond tables as given (for the adjoint's per-parameter unit couplings)
     std::vector<int> bsrc_h;      // padded slot -> original bond, -1 for padding
+    std::vector<int> bflag_h;     // padded slot flags (kBond*)
     std::vector<int32_t> bond_h;  // [nb_in][6]
     int nb_in = 0;
     std::vector<int> kmax_h;
@@ -746,6 +747,7 @@ struct SystemImpl final : snh_system {
                 }
         }
         bsrc_h = bsrc;
+        bflag_h = bflag;
         nb_in = sp.nbonds;
         bond_h.assign(sp.bond, sp.bond + 6 * (size_t)sp.nbonds);
         for (int b = 0; b < sp.nbonds; b++) any_biquad |= sp.has_biquad && sp.has_biquad[b];
@@ -1991,9 +1993,6 @@ struct SystemImpl final : snh_system {
         SNH_REQUIRE(sizeof(T) == 8, SNH_ERR_UNSUPPORTED, "the midpoint adjoint runs at precision 64");
         SNH_REQUIRE(R == 1, SNH_ERR_UNSUPPORTED, "the midpoint adjoint takes one replica");
         SNH_REQUIRE(!tiled, SNH_ERR_UNSUPPORTED, "the midpoint adjoint needs the ghost-padded layout");
-        SNH_REQUIRE(!any_biquad, SNH_ERR_UNSUPPORTED, "the midpoint adjoint supports bilinear pair couplings");
-        for (int k : kmax_h)
-            SNH_REQUIRE(k <= 2, SNH_ERR_UNSUPPORTED, "the midpoint adjoint supports onsite anisotropy of order <= 2");
     }
 
     // One midpoint step (damping 0); returns the buffer holding S'' of the
@@ -2041,35 +2040,39 @@ struct SystemImpl final : snh_system {
                             "parameter spec: bond list must match the system's");
         const size_t nbp = bsrc_h.size();
         std::vector<int> flag(nbp, kBondIso | kBondCulled);
-        std::vector<T> J(9 * nbp, (T)0), J0(nbp, (T)0), q2(9 * na, (T)0);
+        std::vector<T> J(9 * nbp, (T)0), J0(nbp, (T)0), Q(25 * nbp, (T)0), q2(9 * na, (T)0), stev(28 * na, (T)0);
         std::vector<int> km(na, 0);
         for (size_t k = 0; k < nbp; k++) {
             const int b = bsrc_h[k];
             if (b < 0) continue;
-            SNH_REQUIRE(!(ps.has_biquad && ps.has_biquad[b]), SNH_ERR_UNSUPPORTED,
-                        "parameter spec: biquadratic couplings are not supported by the adjoint");
             const double* Jb = ps.bilin + 9 * b;
             const bool iso = Jb[1] == 0 && Jb[2] == 0 && Jb[3] == 0 && Jb[5] == 0 && Jb[6] == 0 && Jb[7] == 0 &&
                              Jb[0] == Jb[4] && Jb[0] == Jb[8];
-            flag[k] = iso ? kBondIso : 0;
+            const bool bq = ps.has_biquad && ps.has_biquad[b];
+            flag[k] = (iso ? kBondIso : 0) | (bq ? kBondBiquad : 0);
             for (int p = 0; p < 9; p++) J[9 * k + p] = (T)Jb[p];
             J0[k] = (T)Jb[0];
+            if (bq)
+                for (int p = 0; p < 25; p++) Q[25 * k + p] = (T)ps.biquad[25 * b + p];
         }
         for (int i = 0; i < na; i++) {
             const int k = ps.kmax ? ps.kmax[i] : 0;
-            SNH_REQUIRE(k == 0 || k == 2, SNH_ERR_UNSUPPORTED,
-                        "parameter spec: onsite anisotropy of order <= 2 only");
+            SNH_REQUIRE(k == 0 || k == 2 || k == 4 || k == 6, SNH_ERR_INVALID, "parameter spec: kmax must be 0, 2, 4 or 6");
             km[i] = k;
+            if (k > 0)
+                for (int p = 0; p < 28; p++) stev[28 * i + p] = (T)ps.stevens[28 * i + p];
             if (k == 2) {
                 const double* c2 = ps.stevens + 28 * i + 1;
-                const double Q[9] = {2 * c2[0], 2 * c2[4], c2[1], 2 * c2[4], -2 * c2[0], c2[3], c2[1], c2[3], 6 * c2[2]};
-                for (int p = 0; p < 9; p++) q2[9 * i + p] = (T)Q[p];
+                const double Q2[9] = {2 * c2[0], 2 * c2[4], c2[1], 2 * c2[4], -2 * c2[0], c2[3], c2[1], c2[3], 6 * c2[2]};
+                for (int p = 0; p < 9; p++) q2[9 * i + p] = (T)Q2[p];
             }
         }
         Couplings<T> Cp = C;
         Cp.bflag = own(dupload(flag));
         Cp.bJ = own(dupload(J));
         Cp.bJ0 = own(dupload(J0));
+        Cp.bQ = own(dupload(Q));
+        Cp.stev = own(dupload(stev));
         Cp.stev_q2 = own(dupload(q2));
         Cp.kmax = own(dupload(km));
         Cp.zee = zeros3;
@@ -2115,6 +2118,20 @@ struct SystemImpl final : snh_system {
             Couplings<T> Clin = C;
             Clin.zee = zeros3;
             Clin.iso = 0;
+            // the nonlinear terms leave Clin: adj_nonlinear_jvp differentiates them at Shat
+            bool nonlinear = any_biquad;
+            for (int k : kmax_h) nonlinear |= k > 2;
+            if (nonlinear) {
+                std::vector<int> fl(bflag_h), km(kmax_h);
+                for (int& f : fl) f &= ~kBondBiquad;
+                for (int& k : km) k = k > 2 ? 0 : k;
+                int* dfl = dupload(fl);
+                tmp.push_back(dfl);
+                int* dkm = dupload(km);
+                tmp.push_back(dkm);
+                Clin.bflag = dfl;
+                Clin.kmax = dkm;
+            }
 
             T *Xsave = alloc(np), *lam = alloc(np), *mu = alloc(np), *nu = alloc(np), *nu2 = alloc(np),
               *Shat = alloc(np), *w = alloc(np), *g = alloc(np);
@@ -2165,7 +2182,7 @@ struct SystemImpl final : snh_system {
                         // the Ewald part of M w: the field of w's Ewald potential (phi2)
                         if (ewald) ewald_field(w, phi2);
                         launch_adj_apply<T>(Clin, nblocks, w, S, Sdd, g, nu, mu, tdt, nu2, partial,
-                                            ewald ? phi2 : nullptr, st);
+                                            ewald ? phi2 : nullptr, C, nonlinear ? Shat : nullptr, st);
                         std::swap(nu, nu2);
                     }
                     launch_adj_w<T>(C, nblocks, nu, Shat, tdt, w, st);

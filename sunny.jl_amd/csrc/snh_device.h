@@ -232,15 +232,16 @@ __device__ __forceinline__ void philox_normals4(uint64_t seed, uint64_t step, ui
 // ---------------------------------------------------------------------------
 // Classical Stevens anisotropy gradient (and energy), the closed form of
 // Sunny's OnsiteCoupling.jl:316-425.  c = c0 | c2[5] | c4[9] | c6[13].
-// As in the reference, X = |S|^2 is held constant in the gradient.
-template <typename T>
-__device__ __forceinline__ void stevens_grad(T sx, T sy, T sz, const T* __restrict__ c, int kmax, T& gx, T& gy,
+// As in the reference, X = |S|^2 is held constant in the gradient.  Tc, the
+// coefficients' type, may differ from T (forward-mode dual numbers in T).
+template <typename T, typename Tc = T>
+__device__ __forceinline__ void stevens_grad(T sx, T sy, T sz, const Tc* __restrict__ c, int kmax, T& gx, T& gy,
                                              T& gz, T* energy)
 {
     T E = c[0];
     T dz = 0, jr = 0, ji = 0;  // dE/dz, dE/dJp (re, im)
     if (kmax > 0) {
-        const T* c2 = c + 1;
+        const Tc* c2 = c + 1;
         T X = sx * sx + sy * sy + sz * sz;
         T p1r = sx, p1i = sy;
         T z1 = sz, z2 = sz * sz;
@@ -252,7 +253,7 @@ __device__ __forceinline__ void stevens_grad(T sx, T sy, T sz, const T* __restri
         jr += c2[0] * p1r + c2[4] * p1i + (T)0.5 * c2[1] * z1;
         ji += c2[0] * p1i - c2[4] * p1r - (T)0.5 * c2[3] * z1;
         if (kmax > 2) {
-            const T* c4 = c + 6;
+            const Tc* c4 = c + 6;
             T X2 = X * X;
             T z3 = z2 * z1, z4 = z2 * z2;
             T p3r = p2r * p1r - p2i * p1i, p3i = p2r * p1i + p2i * p1r;
@@ -269,7 +270,7 @@ __device__ __forceinline__ void stevens_grad(T sx, T sy, T sz, const T* __restri
             ji += 2 * (c4[0] * p3i - c4[8] * p3r) + (T)1.5 * (c4[1] * p2i - c4[7] * p2r) * B4 +
                   (c4[2] * p1i - c4[6] * p1r) * B3 - (T)0.5 * c4[5] * B2;
             if (kmax > 4) {
-                const T* c6 = c + 15;
+                const Tc* c6 = c + 15;
                 T X3 = X2 * X;
                 T z5 = z4 * z1, z6 = z3 * z3;
                 T p5r = p4r * p1r - p4i * p1i, p5i = p4r * p1i + p4i * p1r;

@@ -303,7 +303,8 @@ template <typename T>
 void launch_adj_w(const Couplings<T>& C, int nblocks, const T* nu, const T* Shat, T dt, T* w, hipStream_t st);
 template <typename T>
 void launch_adj_apply(const Couplings<T>& Clin, int nblocks, const T* w, const T* S, const T* Sdd, const T* g,
-                      const T* nu, const T* mu, T dt, T* nu_out, double* partial, const T* phi_w, hipStream_t st);
+                      const T* nu, const T* mu, T dt, T* nu_out, double* partial, const T* phi_w,
+                      const Couplings<T>& Cnl, const T* Shat, hipStream_t st);
 template <typename T>
 void launch_adj_param(const Couplings<T>& Cp, int nblocks, const T* Shat, const T* w, double* partial,
                       hipStream_t st);

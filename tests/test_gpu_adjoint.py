@@ -74,6 +74,29 @@ def test_adjoint_with_ewald_matches_forward_mode():
     np.testing.assert_allclose(s.dipoles_flat(), Sd.v, atol=1e-12)
 
 
+def test_adjoint_nonlinear_terms_match_forward_mode():
+    """Biquadratic bonds and Stevens anisotropy of order 4 and 6 in the
+    adjoint (their derivative along w by dual numbers on the device) against
+    forward-mode duals of the same trajectory, with gradients along the
+    labels J, D and B (a biquadratic coupling)."""
+    from test_oracle_adjoint import _nonlinear
+    nsteps, dt = 10, 0.05
+    s = _nonlinear(0.9, -0.08, 0.05, seed=12)
+    s.layout = 1
+    S0 = s.dipoles_flat()
+    v = np.random.default_rng(7).standard_normal(S0.shape)
+    announce(f"adjoint cubic 3x2x2 biquadratic + Stevens 4/6, {nsteps} steps, checkpoints every 4")
+    loss, grad, dS0 = sh.midpoint_gradient(s, sh.ImplicitMidpoint(dt), nsteps, ["J", "D", "B"],
+                                           lambda S: (float((S * v).sum()), v), checkpoint_every=4)
+    u = np.random.default_rng(8).standard_normal(S0.shape)
+    ad = MidpointAD(s.spec(), [sh.param_spec(s, k) for k in ("J", "D", "B")])
+    Sd = ad.trajectory(S0, dt, nsteps, dS0=u)
+    assert abs(loss - float((Sd.v * v).sum())) <= 1e-12 * max(1.0, abs(loss))
+    fwd = np.einsum("sa,sap->p", v, Sd.d)
+    np.testing.assert_allclose(grad, fwd[:3], rtol=1e-8, atol=1e-12)
+    assert abs((dS0 * u).sum() - fwd[3]) <= 1e-8 * max(1e-12, abs(fwd[3]))
+
+
 def test_adjoint_500_steps_forward_mode_8cubed():
     L, nq, nsteps, dt = 8, 4, 500, 0.05
     ref = sh.workloads.cubic_model(L=L, J=1.0, D=-0.1, seed=5, params=True)

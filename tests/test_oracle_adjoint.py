@@ -80,6 +80,44 @@ def test_dual_restatement_with_ewald():
     assert np.abs(Sd.d[..., 1] - fdD).max() <= 1e-7 * np.abs(fdD).max()
 
 
+def _nonlinear(J, D, B, seed=9):
+    """Cubic 3x2x2 with J1 (label J), a biquadratic J1 bond (label B), an easy
+    axis D Sz^2 (label D) and fixed Stevens terms of order 4 and 6."""
+    s = sh.System(sh.cubic_crystal(1.0), [(0, sh.Moment(s=3, g=2))], "dipole", dims=(3, 2, 2), seed=seed)
+    sh.set_exchange_(s, 1.0, sh.Bond(0, 0, (1, 0, 0)), param=("J", J))
+    sh.set_exchange_(s, 0.0, sh.Bond(0, 0, (0, 1, 0)), biquad=1.0, param=("B", B))
+    O4 = sh.stevens_matrices(3.0)
+    sh.set_onsite_coupling_(s, lambda S: S[2] @ S[2], 0, param=("D", D))
+    sh.set_onsite_coupling_(s, 0.01 * O4[4][0] + 0.004 * O4[6][3], 0)
+    sh.randomize_spins_(s)
+    return s
+
+
+def test_dual_restatement_nonlinear_terms():
+    """Biquadratic bonds and Stevens order 4 and 6 enter the dual restatement
+    through the oracle's gradient: values equal the oracle's midpoint steps,
+    tangents along J, D, B and the initial state equal central differences."""
+    s = _nonlinear(0.9, -0.08, 0.05)
+    S0 = s.dipoles_flat()
+    u = np.random.default_rng(2).standard_normal(S0.shape)
+    u -= (u * S0).sum(1, keepdims=True) * S0 / 9  # tangent to the spheres (|S| = 3)
+    ad = MidpointAD(s.spec(), [sh.param_spec(s, k) for k in ("J", "D", "B")])
+    Sd = ad.trajectory(S0, 0.05, 8, dS0=u)
+
+    def run(J, D, B, S=S0):
+        o = O.Oracle(_nonlinear(J, D, B).spec())
+        for _ in range(8):
+            S, _ = o.step_midpoint(S=S, dt=0.05, damping=0.0, tol=1e-13)
+        return S
+    np.testing.assert_allclose(Sd.v, run(0.9, -0.08, 0.05), atol=1e-13)
+    h = 1e-5
+    for t, (dJ, dD, dB) in enumerate([(h, 0, 0), (0, h, 0), (0, 0, h)]):
+        fd = (run(0.9 + dJ, -0.08 + dD, 0.05 + dB) - run(0.9 - dJ, -0.08 - dD, 0.05 - dB)) / (2 * h)
+        assert np.abs(Sd.d[..., t] - fd).max() <= 1e-6 * np.abs(fd).max(), t
+    fd = (run(0.9, -0.08, 0.05, S0 + 1e-6 * u) - run(0.9, -0.08, 0.05, S0 - 1e-6 * u)) / 2e-6
+    assert np.abs(Sd.d[..., 3] - fd).max() <= 1e-6 * np.abs(fd).max()
+
+
 def test_structure_factor_direct_sum():
     dims = (8, 4, 4)
     rng = np.random.default_rng(0)
