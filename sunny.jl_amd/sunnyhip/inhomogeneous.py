@@ -23,9 +23,8 @@ import copy
 import numpy as np
 
 from .crystal import Bond, bond_parity
-from .operators import (StevensExpansion, decompose_pair_coupling, matrix_to_stevens_coefficients, rcs_factors,
-                        spin_matrices)
-from .system import SCALAR_BIQUAD_METRIC, System, clone_system, to_product_space
+from .operators import StevensExpansion, decompose_pair_coupling, rcs_factors, spin_matrices
+from .system import SCALAR_BIQUAD_METRIC, System, clone_system, onsite_coupling, to_product_space
 
 
 class SiteInteractions:
@@ -130,8 +129,11 @@ def set_exchange_at_(sys: System, J, site1, site2, biquad=0.0, offset=None):
     biquad = float(biquad)
     s1, s2 = sys.spins[site1[3]], sys.spins[site2[3]]
     if biquad != 0.0:
-        Jm = Jm - (biquad / 2) * np.eye(3)
-        scalar += biquad * s1 * (s1 + 1) * s2 * (s2 + 1) / 3
+        if sys.mode == "dipole_uncorrected":
+            scalar += biquad * s1 ** 2 * s2 ** 2 / 3
+        else:
+            Jm = Jm - (biquad / 2) * np.eye(3)
+            scalar += biquad * s1 * (s1 + 1) * s2 * (s2 + 1) / 3
         f = biquad
         if sys.mode == "dipole":
             f *= rcs_factors(s1)[2] * rcs_factors(s2)[2]
@@ -143,6 +145,9 @@ def set_exchange_at_(sys: System, J, site1, site2, biquad=0.0, offset=None):
 def set_pair_coupling_at_(sys: System, op, site1, site2, offset=None, extract_parts=True):
     """General pair operator on the single bond joining two sites
     (set_pair_coupling_at!, PairExchange.jl:502-530)."""
+    if sys.mode == "dipole_uncorrected":
+        raise ValueError("General coupling not currently supported for mode :dipole_uncorrected. "
+                         "Use set_exchange! with option `biquad` for scalar biquadratic.")
     s1, s2 = sys.spins[site1[3]], sys.spins[site2[3]]
     N1, N2 = int(round(2 * s1 + 1)), int(round(2 * s2 + 1))
     if callable(op):
@@ -151,7 +156,7 @@ def set_pair_coupling_at_(sys: System, op, site1, site2, offset=None, extract_pa
     if op.shape != (N1 * N2, N1 * N2) or not np.allclose(op, op.conj().T, rtol=0, atol=1e-12):
         raise ValueError("Invalid or non-Hermitian pair operator")
     scalar, J, B, G = decompose_pair_coupling(op, N1, N2, extract_parts=extract_parts)
-    if G and sys.mode == "dipole":
+    if G and sys.mode != "SUN":
         raise ValueError("Invalid pair coupling. In dipole mode, the most general allowed form is "
                          "(Si, Sj) -> Si'*J*Sj + [(Si'*K1*Si)*(Sj'*K2*Sj) + ...]")
     if B is not None and sys.mode == "dipole":
@@ -171,23 +176,7 @@ def set_onsite_coupling_at_(sys: System, op, site):
     k = sys.site_index(site)
     if _is_vacant(sys, k):
         raise ValueError("Cannot couple vacant site")
-    s = sys.spins[site[3]]
-    if callable(op):
-        op = op(spin_matrices(s))
-    A = np.asarray(op, complex)
-    N = int(round(2 * s + 1))
-    if A.shape != (N, N):
-        raise ValueError("Invalid matrix size.")
-    if not np.allclose(A, A.conj().T):
-        raise ValueError("Operator is not Hermitian")
-    if N == 2:
-        raise ValueError("Onsite coupling is trivial for quantum spin 1/2.")
-    if sys.mode == "SUN":
-        sys.inhomog.onsite[k] = A.copy()
-    else:
-        c = matrix_to_stevens_coefficients((A + A.conj().T) / 2)
-        lam = rcs_factors(s)
-        sys.inhomog.onsite[k] = StevensExpansion.from_coefficients({q: lam[q] * c[q] for q in range(7)})
+    sys.inhomog.onsite[k] = onsite_coupling(sys, op, sys.spins[site[3]], sys.inhomog.kappa[k])
     sys._invalidate()
 
 

@@ -14,8 +14,78 @@ import numpy as np
 SCALAR_BIQUAD_METRIC = np.array([1 / 2, 2, 1 / 6, 2, 1 / 2])
 
 
+class SpinPolynomial:
+    """Polynomial in commuting classical spin components: the large-s spin
+    operators that spin_matrices(Inf) gives for :dipole_uncorrected mode
+    (Spin.jl:16-35, Symbolic.jl).  terms: {(a, b, c): coefficient} of
+    Sx^a Sy^b Sz^c; + - * ** and @ (the same as *: the components commute)."""
+
+    def __init__(self, terms):
+        self.terms = {k: float(v) for k, v in terms.items() if v != 0}
+
+    @staticmethod
+    def _lift(o):
+        if isinstance(o, SpinPolynomial):
+            return o
+        if np.ndim(o) == 0 and np.isreal(o):
+            return SpinPolynomial({(0, 0, 0): float(np.real(o))})
+        raise TypeError("spin polynomials combine with real numbers and spin polynomials only")
+
+    def __add__(self, o):
+        o = self._lift(o)
+        t = dict(self.terms)
+        for k, v in o.terms.items():
+            t[k] = t.get(k, 0.0) + v
+        return SpinPolynomial(t)
+
+    __radd__ = __add__
+
+    def __neg__(self):
+        return SpinPolynomial({k: -v for k, v in self.terms.items()})
+
+    def __sub__(self, o):
+        return self + (-self._lift(o))
+
+    def __rsub__(self, o):
+        return self._lift(o) - self
+
+    def __mul__(self, o):
+        o = self._lift(o)
+        t = {}
+        for k1, v1 in self.terms.items():
+            for k2, v2 in o.terms.items():
+                k = (k1[0] + k2[0], k1[1] + k2[1], k1[2] + k2[2])
+                t[k] = t.get(k, 0.0) + v1 * v2
+        return SpinPolynomial(t)
+
+    __rmul__ = __mul__
+    __matmul__ = __mul__
+    __rmatmul__ = __mul__
+
+    def __pow__(self, n):
+        out = SpinPolynomial({(0, 0, 0): 1.0})
+        for _ in range(int(n)):
+            out = out * self
+        return out
+
+    def degree(self):
+        return max((sum(k) for k in self.terms), default=0)
+
+    def __call__(self, x, y, z):
+        """Value at classical spin components (arrays)."""
+        out = np.zeros(np.broadcast(x, y, z).shape)
+        for (a, b, c), v in self.terms.items():
+            out = out + v * x ** a * y ** b * z ** c
+        return out
+
+
 def spin_matrices(s: float):
-    """(Sx, Sy, Sz) in the spin-s irrep, basis m = s..-s (Spin.jl:1-14)."""
+    """(Sx, Sy, Sz) in the spin-s irrep, basis m = s..-s (Spin.jl:1-14);
+    s = inf gives the large-s operators of :dipole_uncorrected mode, commuting
+    classical components (SpinPolynomial)."""
+    if np.isinf(s):
+        return (SpinPolynomial({(1, 0, 0): 1.0}), SpinPolynomial({(0, 1, 0): 1.0}),
+                SpinPolynomial({(0, 0, 1): 1.0}))
     N = int(round(2 * s + 1))
     if abs(N - (2 * s + 1)) > 1e-12:
         raise ValueError("Spin `s` must be half-integer.")
@@ -133,6 +203,55 @@ class StevensExpansion:
 
     def packed(self) -> np.ndarray:
         return np.concatenate([self.c0, self.c2, self.c4, self.c6])
+
+
+def classical_stevens_basis(S):
+    """Values [n, 28] of the classical Stevens polynomials (k = 0, 2, 4, 6; q
+    descending) at spins S [n, 3], with X = |S|^2: the polynomials whose
+    coefficients the classical anisotropy energy multiplies
+    (OnsiteCoupling.jl:316-425)."""
+    S = np.asarray(S, float)
+    x, y, z = S[:, 0], S[:, 1], S[:, 2]
+    X = x * x + y * y + z * z
+    jp = x + 1j * y
+    P = [np.ones_like(x) + 0j] + [jp ** m for m in range(1, 7)]
+
+    def row(k, A):
+        # A[m]: the z-polynomial of |q| = m; q = k..1 (Re Jp^q), 0, -1..-k (Im Jp^|q|)
+        return [(P[q] * A[q]).real for q in range(k, 0, -1)] + [A[0]] + \
+            [(P[q] * A[q]).imag for q in range(1, k + 1)]
+    b = [np.ones_like(x)]
+    b += row(2, [3 * z ** 2 - X, z, np.ones_like(x)])
+    b += row(4, [35 * z ** 4 - 30 * X * z ** 2 + 3 * X ** 2, 7 * z ** 3 - 3 * X * z, 7 * z ** 2 - X, z,
+                 np.ones_like(x)])
+    b += row(6, [231 * z ** 6 - 315 * X * z ** 4 + 105 * X ** 2 * z ** 2 - 5 * X ** 3,
+                 33 * z ** 5 - 30 * X * z ** 3 + 5 * X ** 2 * z, 33 * z ** 4 - 18 * X * z ** 2 + X ** 2,
+                 11 * z ** 3 - 3 * X * z, 11 * z ** 2 - X, z, np.ones_like(x)])
+    return np.stack(b, 1)
+
+
+def polynomial_to_stevens_coefficients(p, kappa: float):
+    """Classical Stevens coefficients {k: [2k+1]} of a spin polynomial on the
+    sphere |S| = kappa (operator_to_stevens_coefficients(p, S^2 = kappa^2),
+    Symbolic.jl:94-103, without renormalisation): an even polynomial of
+    degree <= 6 restricted to the sphere is exactly such an expansion, found
+    by least squares on fixed sample points."""
+    p = SpinPolynomial._lift(p)
+    if any(sum(k) % 2 for k in p.terms):
+        raise ValueError("Single-ion anisotropy must be time-reversal invariant.")
+    if p.degree() > 6:
+        raise ValueError("Stevens operators for k > 6 are currently unsupported.")
+    u = np.random.default_rng(0).standard_normal((400, 3))
+    S = kappa * u / np.linalg.norm(u, axis=1, keepdims=True)
+    B = classical_stevens_basis(S)
+    v = p(S[:, 0], S[:, 1], S[:, 2])
+    c, *_ = np.linalg.lstsq(B, v, rcond=None)
+    if np.abs(B @ c - v).max() > 1e-9 * max(1.0, np.abs(v).max()):
+        raise ValueError("spin polynomial is not a Stevens expansion of rank <= 6")
+    c[np.abs(c) < 1e-13 * max(1.0, np.abs(c).max())] = 0.0
+    out = {k: np.zeros(2 * k + 1) for k in range(7)}
+    out[0], out[2], out[4], out[6] = c[:1], c[1:6], c[6:15], c[15:28]
+    return out
 
 
 def hermitian_basis(N: int):

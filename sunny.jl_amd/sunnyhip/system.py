@@ -18,8 +18,8 @@ import numpy as np
 
 from . import _lib
 from .crystal import Bond, Crystal, bond_parity
-from .operators import (SCALAR_BIQUAD_METRIC, StevensExpansion, decompose_pair_coupling,
-                        matrix_to_stevens_coefficients, rcs_factors,
+from .operators import (SCALAR_BIQUAD_METRIC, SpinPolynomial, StevensExpansion, decompose_pair_coupling,
+                        matrix_to_stevens_coefficients, polynomial_to_stevens_coefficients, rcs_factors,
                         spin_matrices)
 
 VACUUM_PERMEABILITY = 0.6745817653324668  # mu0 muB^2 in meV*A^3, src/Units.jl:91
@@ -42,14 +42,17 @@ def _julia_view(flat, dims, na, width):
 class System:
     """A supercell of `crystal` with per-sublattice moments and couplings.
 
-    mode: "dipole" or "SUN" (src/System/System.jl:33-93).  precision selects
+    mode: "dipole", "dipole_uncorrected" (the large-s limit: no
+    renormalisation of onsite and biquadratic couplings, which it takes as
+    polynomials of the classical spin, spin_matrices(inf)) or "SUN"
+    (src/System/System.jl:33-93).  precision selects
     the device kernels: 64 (plain FP64) or 32 (FP32 arithmetic with FP64
     renormalisation, residual and energy sums).
     """
 
     def __init__(self, crystal: Crystal, moments, mode: str, dims=(1, 1, 1), seed=None, precision=64):
-        if mode not in ("dipole", "SUN"):
-            raise ValueError("Mode must be `:SUN`, `:dipole`")
+        if mode not in ("dipole", "dipole_uncorrected", "SUN"):
+            raise ValueError("Mode must be `:SUN`, `:dipole`, or `:dipole_uncorrected`")
         self.crystal = crystal
         self.mode = mode
         self.dims = tuple(int(d) for d in dims)
@@ -497,10 +500,15 @@ def set_exchange_(sys: System, J, bond: Bond, biquad=0.0, propagate=None, param=
     scalar = 0.0
     biquad = float(biquad)
     if biquad != 0.0:
-        # adapt_for_biquad: (Si.Sj)^2 = Qi.g Qj - Si.Sj/2 + s1(s1+1)s2(s2+1)/3
+        # adapt_for_biquad (PairExchange.jl:322-340): (Si.Sj)^2 = Qi.g Qj - Si.Sj/2
+        # + s1(s1+1)s2(s2+1)/3 for quantum spins; Qi.g Qj + s1^2 s2^2/3 in the
+        # large-s limit (:dipole_uncorrected)
         s1, s2 = sys.spins[bond.i], sys.spins[bond.j]
-        Jv = Jv - biquad / 2 if isinstance(Jv, float) else Jv - (biquad / 2) * np.eye(3)
-        scalar += biquad * s1 * (s1 + 1) * s2 * (s2 + 1) / 3
+        if sys.mode == "dipole_uncorrected":
+            scalar += biquad * s1 ** 2 * s2 ** 2 / 3
+        else:
+            Jv = Jv - biquad / 2 if isinstance(Jv, float) else Jv - (biquad / 2) * np.eye(3)
+            scalar += biquad * s1 * (s1 + 1) * s2 * (s2 + 1) / 3
     if isinstance(Jv, float):
         targets = sys.crystal.equivalent_bonds(bond) if propagate is None else list(propagate)
     else:
@@ -524,7 +532,8 @@ def set_exchange_(sys: System, J, bond: Bond, biquad=0.0, propagate=None, param=
 def set_pair_biquad_(sys: System, B, bond: Bond, param=None):
     """Coupling Q_i . B Q_j of Stevens quadrupoles (5x5, q descending) on one
     bond in every cell; the reverse bond carries B' (PairExchange.jl:447-468).
-    In :dipole mode B is renormalised by rcs_2(s_i) rcs_2(s_j)."""
+    In :dipole mode B is renormalised by rcs_2(s_i) rcs_2(s_j) (not in the
+    large-s :dipole_uncorrected mode)."""
     _require_homogeneous(sys, "set_pair_coupling_at_")
     B = np.asarray(B, float).reshape(5, 5)
     if sys.mode == "dipole":
@@ -554,6 +563,9 @@ def set_pair_coupling_(sys: System, op, bond: Bond, extract_parts=True, param=No
     :SUN mode.  No symmetry propagation: call once per bond (with the
     operator rotated as required)."""
     _require_homogeneous(sys, "set_pair_coupling_at_")
+    if sys.mode == "dipole_uncorrected":
+        raise ValueError("General coupling not currently supported for mode :dipole_uncorrected. "
+                         "Use set_exchange! with option `biquad` for scalar biquadratic.")
     if bond.i == bond.j and tuple(bond.n) == (0, 0, 0):
         raise ValueError("Bond must connect different atoms.")
     si, sj = sys.spins[bond.i], sys.spins[bond.j]
@@ -568,7 +580,7 @@ def set_pair_coupling_(sys: System, op, bond: Bond, extract_parts=True, param=No
     if Ni == Nj == 2:
         raise ValueError("Use set_exchange_ instead. Quantum spin 1/2 allows only bilinear coupling.")
     scalar, J, B, G = decompose_pair_coupling(op, Ni, Nj, extract_parts=extract_parts)
-    if G and sys.mode == "dipole":
+    if G and sys.mode != "SUN":
         raise ValueError("Invalid pair coupling. In dipole mode, the most general allowed form is "
                          "(Si, Sj) -> Si'*J*Sj + [(Si'*K1*Si)*(Sj'*K2*Sj) + ...] where J is any 3x3 matrix, "
                          "while K1, K2 must be Hermitian and traceless.")
@@ -589,9 +601,29 @@ def set_onsite_coupling_(sys: System, op, i: int, param=None):
     matrices; `param` = (label, val) as in set_exchange_.  No symmetry
     propagation: call once per atom (with the operator rotated as required)."""
     _require_homogeneous(sys, "set_onsite_coupling_at_")
-    s = sys.spins[i]
-    if callable(op):
+    unit = onsite_coupling(sys, op, sys.spins[i], sys.kappas[i])
+    key = ("onsite", int(i))
+    label, val = _param_label(sys, param, key)
+    _replace_param(sys, label, val, onsites={int(i): unit}, key=key)
+
+
+def onsite_coupling(sys: System, op, s, kappa):
+    """The onsite coupling of an operator on a spin-s site (onsite_coupling,
+    OnsiteCoupling.jl:1-44): the matrix itself in :SUN mode; in :dipole mode
+    its Stevens coefficients renormalised by rcs_k(s); in
+    :dipole_uncorrected mode a polynomial of the classical spin (a function
+    receives spin_matrices(inf)) expanded on |S| = kappa, unrenormalised."""
+    fn = callable(op) and not isinstance(op, SpinPolynomial)  # a SpinPolynomial evaluates when called
+    if sys.mode == "dipole_uncorrected":
+        if fn:
+            op = op(spin_matrices(np.inf))
+        if not isinstance(op, SpinPolynomial) and not np.ndim(op) == 0:
+            raise ValueError("System with mode :dipole_uncorrected requires a symbolic operator.")
+        return StevensExpansion.from_coefficients(polynomial_to_stevens_coefficients(op, kappa))
+    if fn:
         op = op(spin_matrices(s))
+    if isinstance(op, SpinPolynomial):
+        raise ValueError("Symbolic operator only valid for system with mode :dipole_uncorrected.")
     A = np.asarray(op, complex)
     N = int(round(2 * s + 1))
     if A.shape != (N, N):
@@ -599,22 +631,20 @@ def set_onsite_coupling_(sys: System, op, i: int, param=None):
     if not np.allclose(A, A.conj().T):
         raise ValueError("Operator is not Hermitian")
     if N == 2:
+        if sys.mode == "dipole":
+            raise ValueError("Consider mode :dipole_uncorrected. Onsite coupling is trivial for quantum spin 1/2.")
         raise ValueError("Onsite coupling is trivial for quantum spin 1/2.")
     if sys.mode == "SUN":
-        unit = A.copy()
-    else:
-        c = matrix_to_stevens_coefficients((A + A.conj().T) / 2)
-        lam = rcs_factors(s)
-        unit = StevensExpansion.from_coefficients({k: lam[k] * c[k] for k in range(7)})
-    key = ("onsite", int(i))
-    label, val = _param_label(sys, param, key)
-    _replace_param(sys, label, val, onsites={int(i): unit}, key=key)
+        return A.copy()
+    c = matrix_to_stevens_coefficients((A + A.conj().T) / 2)
+    lam = rcs_factors(s)
+    return StevensExpansion.from_coefficients({k: lam[k] * c[k] for k in range(7)})
 
 
 def set_stevens_expansion_(sys: System, stvexp: StevensExpansion, i: int, param=None):
     """Sets already-renormalised classical Stevens coefficients (:dipole)."""
     _require_homogeneous(sys, "set_onsite_coupling_at_")
-    if sys.mode != "dipole":
+    if sys.mode == "SUN":
         raise ValueError("Stevens expansions apply to :dipole mode")
     key = ("onsite", int(i))
     label, val = _param_label(sys, param, key)

@@ -29,6 +29,14 @@ def models(precision):
     sh.set_exchange_(s, -0.2, sh.Bond(0, 0, (0, 3, 4)), propagate=[sh.Bond(0, 0, (0, 3, 4))])
     sh.randomize_spins_(s)
     yield "cubic_long_bonds", s
+    # the large-s mode: polynomial anisotropy up to order 6, exact (Si.Sj)^2
+    from test_oracle_uncorrected import _poly
+    s = sh.System(sh.cubic_crystal(1.0), [(0, sh.Moment(s=2, g=2))], "dipole_uncorrected", dims=(2, 3, 2), seed=9,
+                  precision=precision)
+    sh.set_exchange_(s, 0.7, sh.Bond(0, 0, (1, 0, 0)), biquad=0.3)
+    sh.set_onsite_coupling_(s, _poly, 0)
+    sh.randomize_spins_(s)
+    yield "cubic_uncorrected", s
 
 
 def aniso_model(precision):
