@@ -398,6 +398,25 @@ def bench_local(args):
         log(f"LocalSampler {label}: {ns} sites, {nc} colours, {(t1 - t0) * 1e3 / args.local_sweeps:.3f} ms/sweep, "
             f"{rate:.3e} trials/s, colour kernel {ms / max(n, 1):.4f} ms")
         del obj, systems
+    # the Ewald form: sequential trials at random sites, chunks of 64 per launch
+    for prec in (32, 64):
+        e = sh.workloads.pyrochlore_model(L=16, J1=0.05, J2=0.0, A=0.0, s=3.5, seed=40, precision=prec,
+                                          dipolar=True, mu0_muB2=0.6745817653324668, demag=1 / 3)
+        h = e._device()
+        trials = args.local_ewald_trials
+        sampler = sh.LocalSampler(kT=0.3, nsweeps=trials / e.nsites, propose=sh.propose_delta(0.3))
+        sh.step_(e, sampler)
+        h.call("snh_synchronize")
+        dE0 = sampler.dE
+        t0 = time.perf_counter()
+        sh.step_(e, sampler)
+        h.call("snh_synchronize")
+        dt = time.perf_counter() - t0
+        out[f"ewald_fp{prec}_r1"] = {"sites": e.nsites, "trials": trials, "us_per_trial": dt * 1e6 / trials,
+                                     "value": trials / dt, "dE": sampler.dE - dE0}
+        log(f"LocalSampler Ewald fp{prec}: {e.nsites} sites, {trials} sequential trials, "
+            f"{dt * 1e6 / trials:.3f} us/trial")
+        del e, h
     small = sh.workloads.pyrochlore_model(L=8, seed=1, precision=64)
     m, nc, col = sh.local_coloring(small)
     o = O.Oracle(small.spec())
@@ -413,7 +432,9 @@ def bench_local(args):
                       "cpu_baseline": {"value": cpu, "unit": "trials/s", "cores": 1, "kind": "port",
                                        "sample": f"oracle colour passes, pyrochlore 8^3 x 16, {sweeps} sweeps"},
                       "config": {"workload": "pyrochlore 32^3 x 16 = 524288 spins/replica, dipole s=3/2, "
-                                             "J1 1.0 J2 0.1 A -0.05, LocalSampler kT 0.3 propose_delta(0.3)"}}),
+                                             "J1 1.0 J2 0.1 A -0.05, LocalSampler kT 0.3 propose_delta(0.3); "
+                                             "Ewald cases: pyrochlore 16^3 x 16 = 65536 spins, s=7/2, J1=0.05, "
+                                             "Ewald mu0muB2=0.6745817653324668 demag 1/3"}}),
           flush=True)
 
 
@@ -460,6 +481,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--replicas", type=int, default=64)
+    ap.add_argument("--local-ewald-trials", type=int, default=65536,
+                    help="sequential Metropolis trials timed on the Ewald case of --workload local")
     ap.add_argument("--L", type=int, default=32)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

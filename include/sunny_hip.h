@@ -254,8 +254,8 @@ int snh_corr_add_sample(snh_system *sys, double dt, double tol, int32_t max_iter
 int snh_corr_data(snh_system *sys, double *data, int64_t *nsamples);
 
 /* step!(sys, LocalSampler(; kT, nsweeps, propose)) for every replica
- * (src/MonteCarlo/Samplers.jl:133-153), :dipole or :SUN; with Ewald (:dipole
- * only) the trials run one at a time, as described at the end.
+ * (src/MonteCarlo/Samplers.jl:133-153), :dipole or :SUN; with Ewald the
+ * trials run one at a time, as described at the end.
  * Proposal p of the mixture (@mix_proposals, Samplers.jl:67-100) has
  * kinds[p] = 0 propose_uniform (randspin, src/System/System.jl:444-447),
  * 1 propose_flip (flip, System.jl:440-442; flip_ket for :SUN, Spin.jl:119-125)
@@ -275,9 +275,10 @@ int snh_corr_data(snh_system *sys, double *data, int64_t *nsamples);
  * ceil(nsweeps * nsites) sequential trials at random sites, trial
  * T = sweep0 * nsites + t drawing its site, u_acc and u_mix from counter
  * (T lo, 0x4D430002, T hi, 0) and its normals from (T lo, 0x4D430003, T hi,
- * 0); dE adds ewald_energy_delta (src/System/Ewald.jl:234-241) from an FP64
- * potential that every accepted move updates at all sites (O(nsites) per
- * move, as in the reference).  dE_dS (may be NULL): [R][4], the sums of dE and of
+ * 0); dE adds ewald_energy_delta (src/System/Ewald.jl:234-241; :SUN: of the
+ * expected dipole) from an FP64 potential that every accepted move updates
+ * at all sites (O(nsites) per move, as in the reference), 64 trials per
+ * launch.  dE_dS (may be NULL): [R][4], the sums of dE and of
  * dS = S_new - S_old over the accepted trials of this call (LocalSampler's
  * fields dE, dS accumulate them). */
 int snh_step_local(snh_system *sys, const double *kT, int32_t nprop, const int32_t *kinds,

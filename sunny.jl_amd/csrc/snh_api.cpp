@@ -510,8 +510,9 @@ struct SystemImpl final : snh_system {
         double* acc = nullptr;        // [max(work, R)][4]
         double* out = nullptr;        // [R][4]
         // sequential trials with the Ewald term (tensors per Ewald group)
-        double* phi = nullptr;   // FP64 potential planes [3][total]
-        McTrial* trial = nullptr;  // [R]
+        double* phi[2] = {nullptr, nullptr};    // FP64 potential planes [3][total], alternating
+        McTrial* moves[2] = {nullptr, nullptr};  // [R][kMcChunk] accepted moves of a chunk, alternating
+        int* nmoves[2] = {nullptr, nullptr};     // [R]
     } mc;
     // Ewald: one group per run of consecutive replicas with equal dims, each
     // with its grid, transformed tensor, spectra and (library FFT path) plans
@@ -1509,8 +1510,11 @@ struct SystemImpl final : snh_system {
                                       9 * sizeof(double), hipMemcpyDeviceToHost));
                 g.A0 = own(dupload(a0));
             }
-            mc.phi = own(dalloc<double>(3 * (size_t)total));
-            mc.trial = own(dalloc<McTrial>(R));
+            for (int b = 0; b < 2; b++) {
+                mc.phi[b] = own(dalloc<double>(3 * (size_t)total));
+                mc.moves[b] = own(dalloc<McTrial>((size_t)R * kMcChunk));
+                mc.nmoves[b] = own(dalloc<int>(R));
+            }
         }
         mc.ready = true;
     }
@@ -1561,21 +1565,33 @@ struct SystemImpl final : snh_system {
         SNH_HIP(hipMemcpyAsync(mc.kT, kT, sizeof(double) * R, hipMemcpyHostToDevice, st));
         if (ewald) {
             // ceil(nsweeps * ns) sequential trials per replica, numbered from
-            // sweep0 * ns; phi starts as the Ewald potential of the state
+            // sweep0 * ns, in chunks of kMcChunk per launch; the potential
+            // starts as the Ewald potential of the state, and each launch
+            // also adds the previous chunk's moves to it for the next one
             ewald_field(X, phi1);
-            launch_planes_to_double<T>(phi1, mc.phi, 3 * (long long)total, st);
+            launch_planes_to_double<T>(phi1, mc.phi[0], 3 * (long long)total, st);
             launch_fill_double(mc.acc, 4 * (long long)R, 0.0, st);
             hipEvent_t e = tbegin();
             for (const auto& g : ewg) {  // replicas of one dims at a time
                 const uint32_t ns = (uint32_t)reps[g.r0].ncell * (uint32_t)na;
                 const long long niters = (long long)std::ceil(nsweeps * (double)ns);
-                for (long long t = 0; t < niters; t++) {
+                int cur = 0;  // mc.phi[cur]: potential before the previous chunk's moves
+                for (long long t = 0, k = 0; t < niters; t += kMcChunk, k++) {
                     const uint64_t T0 = sweep0 * ns + (uint64_t)t;
+                    const int nt = (int)std::min<long long>(kMcChunk, niters - t);
+                    McChunkIO io;
+                    io.phi_in = mc.phi[cur];
+                    io.phi_out = mc.phi[cur ^ 1];
+                    io.prev = k ? mc.moves[(k - 1) & 1] : nullptr;
+                    io.nprev = k ? mc.nmoves[(k - 1) & 1] : nullptr;
+                    io.moves = mc.moves[k & 1];
+                    io.nmoves = mc.nmoves[k & 1];
+                    io.napp = k ? (int)std::min<uint32_t>((ns + 63) / 64, kMcApplyGroups) : 0;
                     if (N == 0)
-                        launch_mc_ewald_trial<T>(C, A, X, mc.phi, g.A0, T0, mc.trial, mc.acc, g.r0, g.nr, st);
+                        launch_mc_ewald_chunk<T>(C, A, X, io, g.Aew, g.A0, T0, nt, mc.acc, g.r0, g.nr, st);
                     else
-                        launch_sun_mc_ewald_trial<T>(C, A, X, mc.phi, g.A0, T0, mc.trial, mc.acc, g.r0, g.nr, st);
-                    launch_mc_ewald_update(g.Aew, C.reps, C.G, na, total, ns, mc.trial, mc.phi, g.r0, g.nr, st);
+                        launch_sun_mc_ewald_chunk<T>(C, A, X, io, g.Aew, g.A0, T0, nt, mc.acc, g.r0, g.nr, st);
+                    if (k) cur ^= 1;
                 }
             }
             tend("local_ewald", e);

@@ -245,24 +245,47 @@ void launch_sun_mc_color(const Couplings<T>& C, const McArgs& A, T* Z, const McW
                          unsigned long long P, double prob, double* acc, hipStream_t st);
 void launch_mc_reduce(const double* acc, const int* start, int ncolors, int R, double* out, hipStream_t st);
 // Accepted sequential trial of a replica: site (sublattice i, cell c1..c3)
-// and the change of its moment dmu = -g dS (flag 0: rejected).
+// and the change of its moment dmu = -g dS.
 struct McTrial {
     int flag, i, c1, c2, c3, pad;
     double dmu[3];
 };
-// Trial T0 of the Ewald form for replicas r0 .. r0 + nr - 1, which share
-// their dims (phi: FP64 potential planes, A0: [na][9] self blocks
-// A(0, i, i) of those dims).
+// Sequential trials of the Ewald form run in chunks of at most kMcChunk per
+// launch, one wavefront per replica whose lane m holds the chunk's m-th
+// accepted move.
+constexpr int kMcChunk = 64;
+static_assert(kMcChunk <= 64, "one accepted move per lane of a wavefront");
+// Buffers of one launch of sequential trials with the Ewald term.  Its
+// trial workgroups see the potential phi_in of the chunk's start without the
+// previous chunk's moves (prev, nprev; absent for a group's first chunk) and
+// add those at their own sites; its other napp workgroups per replica write
+// phi_out = phi_in + the previous chunk's moves at every site, for the next
+// launch.  The chunk's accepted moves go to moves, nmoves ([R][kMcChunk],
+// [R]).
+struct McChunkIO {
+    const double* phi_in;
+    double* phi_out;
+    const McTrial* prev;
+    const int* nprev;
+    McTrial* moves;
+    int* nmoves;
+    int napp;  // apply workgroups per replica, 0: no previous chunk
+};
+constexpr int kMcApplyGroups = 128;  // napp at most
+
+// Trials T0 .. T0 + ntrial - 1 (ntrial <= kMcChunk) of replicas r0 .. r0 +
+// nr - 1, which share their dims, with the buffers io; Aew
+// is the real-space tensor and A0 its [na][9] self blocks A(0, i, i);
+// acc[4 r + q] += (dE, dS).
 template <typename T>
-void launch_mc_ewald_trial(const Couplings<T>& C, const McArgs& A, T* S, const double* phi, const double* A0,
-                           unsigned long long T0, McTrial* tr, double* acc, int r0, int nr, hipStream_t st);
-// The same trial for :SUN kets (the Ewald term acts on expected dipoles).
+void launch_mc_ewald_chunk(const Couplings<T>& C, const McArgs& A, T* S, const McChunkIO& io, const double* Aew,
+                           const double* A0, unsigned long long T0, int ntrial, double* acc, int r0, int nr,
+                           hipStream_t st);
+// The same for :SUN kets (the Ewald term acts on expected dipoles).
 template <typename T>
-void launch_sun_mc_ewald_trial(const Couplings<T>& C, const McArgs& A, T* Z, const double* phi, const double* A0,
-                               unsigned long long T0, McTrial* tr, double* acc, int r0, int nr, hipStream_t st);
-// phi += A dmu at the ns sites of each of those replicas whose trial was accepted.
-void launch_mc_ewald_update(const double* Aew, const ReplicaInfo* reps, const Ghost& G, int na, uint32_t total,
-                            uint32_t ns, const McTrial* tr, double* phi, int r0, int nr, hipStream_t st);
+void launch_sun_mc_ewald_chunk(const Couplings<T>& C, const McArgs& A, T* Z, const McChunkIO& io, const double* Aew,
+                               const double* A0, unsigned long long T0, int ntrial, double* acc, int r0, int nr,
+                               hipStream_t st);
 template <typename T>
 void launch_planes_to_double(const T* src, double* dst, long long n, hipStream_t st);
 void launch_fill_double(double* p, long long n, double v, hipStream_t st);

@@ -17,17 +17,23 @@ __device__ __forceinline__ int mc_wrap(int v, int L)
     return m < 0 ? m + L : m;
 }
 
-// Classical Stevens energy in FP64 from the T-typed coefficients.
+// Change of the classical Stevens energy in FP64, from the T-typed
+// coefficients, as the dipole goes from (ox, oy, oz) to (nx, ny, nz); zero
+// without anisotropy (the constant term cancels), the coefficients then
+// not loaded.
 template <typename T>
-__device__ __forceinline__ double stevens_energy_d(const Couplings<T>& C, int i, double x, double y, double z)
+__device__ __forceinline__ double stevens_delta_d(const Couplings<T>& C, int i, double nx, double ny, double nz,
+                                                  double ox, double oy, double oz)
 {
     const int km = ldc(C.kmax, i);
+    if (km == 0) return 0.0;
     double c[28];
 #pragma unroll
     for (int p = 0; p < 28; p++) c[p] = (double)ldc(C.stev + 28 * i, p);
-    double gx = 0, gy = 0, gz = 0, E = 0;
-    stevens_grad<double>(x, y, z, c, km, gx, gy, gz, &E);
-    return E;
+    double gx = 0, gy = 0, gz = 0, En = 0, Eo = 0;
+    stevens_grad<double>(nx, ny, nz, c, km, gx, gy, gz, &En);
+    stevens_grad<double>(ox, oy, oz, c, km, gx, gy, gz, &Eo);
+    return En - Eo;
 }
 
 __device__ __forceinline__ void quadrupole_d(double x, double y, double z, double Q[5])
@@ -142,17 +148,60 @@ __device__ __forceinline__ McSeqDraw mc_seq_draw(unsigned long long key, unsigne
     return {(uint32_t)(((unsigned long long)un[0] * ns) >> 32), (double)un[1] * inv32, (double)un[2] * inv32};
 }
 
-// ewald_energy_delta (Ewald.jl:234-241) of sublattice-i site g whose dipole
-// changes by dS: dS . (-g^T phi) + dmu . A(0, i, i) dmu / 2 with
-// dmu = -g dS (returned in dmu), phi the site's current FP64 potential and
-// A0 the [na][9] self blocks.
-template <typename T>
-__device__ __forceinline__ double mc_ewald_delta(const Couplings<T>& C, int i, const double* __restrict__ phi,
-                                                 uint32_t g, const double* __restrict__ A0, const double dS[3],
-                                                 double dmu[3])
+// The value of v in lane l (wave-uniform l), in 32-bit pieces.
+template <typename X>
+__device__ __forceinline__ X lane_value(X v, int l)
 {
-    const uint32_t total = C.total;
-    const double p[3] = {phi[g], phi[g + total], phi[g + 2 * (size_t)total]};
+    static_assert(sizeof(X) % 4 == 0, "32-bit pieces");
+    uint32_t w[sizeof(X) / 4];
+    __builtin_memcpy(w, &v, sizeof(X));
+#pragma unroll
+    for (int q = 0; q < (int)(sizeof(X) / 4); q++) w[q] = (uint32_t)__builtin_amdgcn_readlane((int)w[q], l);
+    X out;
+    __builtin_memcpy(&out, w, sizeof(X));
+    return out;
+}
+
+// A sequential trial's site (sublattice i, cell c1..c3 of replica R),
+// acceptance uniform and mixture component k (mc_pick of u_mix).
+struct McSeqSite {
+    int i, c1, c2, c3, k;
+    double u_acc;
+};
+
+__device__ __forceinline__ McSeqSite mc_seq_site(const ReplicaInfo& R, const McSeqDraw& dr, const McArgs& A)
+{
+    McSeqSite d;
+    d.i = (int)(dr.site / (uint32_t)R.ncell);
+    const int c = (int)(dr.site - (uint32_t)d.i * (uint32_t)R.ncell);
+    d.c1 = c % R.L1;
+    d.c2 = (c / R.L1) % R.L2;
+    d.c3 = c / (R.L1 * R.L2);
+    d.k = mc_pick(A, dr.u_mix);
+    d.u_acc = dr.u_acc;
+    return d;
+}
+
+__device__ __forceinline__ McSeqSite mc_seq_site_of_lane(const McSeqSite& my, int l)
+{
+    McSeqSite d;
+    d.i = lane_value(my.i, l);
+    d.c1 = lane_value(my.c1, l);
+    d.c2 = lane_value(my.c2, l);
+    d.c3 = lane_value(my.c3, l);
+    d.k = lane_value(my.k, l);
+    d.u_acc = lane_value(my.u_acc, l);
+    return d;
+}
+
+// ewald_energy_delta (Ewald.jl:234-241) of a sublattice-i site whose dipole
+// changes by dS: dS . (-g^T p) + dmu . A(0, i, i) dmu / 2 with dmu = -g dS
+// (returned in dmu), p the site's current FP64 potential and A0 the [na][9]
+// self blocks.
+template <typename T>
+__device__ __forceinline__ double mc_ewald_delta(const Couplings<T>& C, int i, const double p[3],
+                                                 const double* __restrict__ A0, const double dS[3], double dmu[3])
+{
     double gm[9];
 #pragma unroll
     for (int q = 0; q < 9; q++) gm[q] = (double)ldc(C.gmat + 9 * i, q);
@@ -167,6 +216,131 @@ __device__ __forceinline__ double mc_ewald_delta(const Couplings<T>& C, int i, c
         dE += dS[a] * grad + dmu[a] * adm / 2;
     }
     return dE;
+}
+
+// Sum over the 64 lanes of a wavefront; every lane gets the same total (a
+// butterfly: partners add the same two values, so no lane differs).
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// s = sum over the moves mv[0..nm) of A(c_m - c; i_m, i) dmu_m at sublattice-i
+// site (c1, c2, c3) of replica R (phi = sum_k A mu_k, Ewald.jl:185-209).
+// Aew: real-space tensor [i_m][i][cell offset][3][3] of R's dims, FP64.
+__device__ __forceinline__ void mc_moves_sum(const double* __restrict__ Aew, int na, const ReplicaInfo& R,
+                                             const McTrial* __restrict__ mv, int nm, int i, int c1, int c2, int c3,
+                                             double s[3])
+{
+    double a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll 8
+    for (int m = 0; m < nm; m++) {
+        const McTrial t = mv[m];
+        const int o1 = mc_wrap_to(t.c1 - c1, R.L1, false), o2 = mc_wrap_to(t.c2 - c2, R.L2, false),
+                  o3 = mc_wrap_to(t.c3 - c3, R.L3, false);  // differences of cells in [0, L)
+        const size_t off = (size_t)o1 + (size_t)R.L1 * ((size_t)o2 + (size_t)R.L2 * (size_t)o3);
+        const double* a = Aew + (((size_t)t.i * na + i) * (size_t)R.ncell + off) * 9;
+        a0 += a[0] * t.dmu[0] + a[1] * t.dmu[1] + a[2] * t.dmu[2];
+        a1 += a[3] * t.dmu[0] + a[4] * t.dmu[1] + a[5] * t.dmu[2];
+        a2 += a[6] * t.dmu[0] + a[7] * t.dmu[1] + a[8] * t.dmu[2];
+    }
+    s[0] = a0;
+    s[1] = a1;
+    s[2] = a2;
+}
+
+// Apply workgroup b of a launch (b < nr napp): phi_out = phi_in + the
+// previous chunk's moves at sites j = 64 (b mod napp) + lane + 64 napp k of
+// replica r0 + b / napp.  napp is kept small, so the apply streams the
+// tensor over the chunk's whole duration rather than competing with the
+// serial trials for memory at its start.
+__device__ __forceinline__ void mc_apply_part(const McChunkIO& io, const double* __restrict__ Aew,
+                                              const ReplicaInfo* __restrict__ reps, const Ghost& G, int na,
+                                              uint32_t total, int r0, int b)
+{
+    const int r = r0 + b / io.napp;
+    const ReplicaInfo R = reps[r];
+    const uint32_t ncell = (uint32_t)R.ncell;
+    const McTrial* mv = io.prev + (size_t)r * kMcChunk;
+    const int nm = io.nprev[r];
+    for (uint32_t j = (uint32_t)(b % io.napp) * 64u + threadIdx.x; j < ncell * (uint32_t)na;
+         j += 64u * (uint32_t)io.napp) {
+        const int i = (int)(j / ncell);
+        const int c = (int)(j - (uint32_t)i * ncell);
+        const int c1 = c % R.L1, c2 = (c / R.L1) % R.L2, c3 = c / (R.L1 * R.L2);
+        double s[3];
+        mc_moves_sum(Aew, na, R, mv, nm, i, c1, c2, c3, s);
+        const uint32_t g = mc_site(R, G, i, c1, c2, c3, true).g;
+#pragma unroll
+        for (int q = 0; q < 3; q++) io.phi_out[g + q * (size_t)total] = io.phi_in[g + q * (size_t)total] + s[q];
+    }
+}
+
+// Ewald potential a sequential trial sees, kept per lane for the chunk's
+// trials (lane l: trial T0 + l at site (i, c1, c2, c3)): pot starts as the
+// potential of the chunk's start at the site (phi_in plus the previous
+// chunk's moves, summed as the apply workgroups sum them) and gains
+// A(c_m - c; i_m, i) dmu_m for every move m accepted at an earlier trial of
+// the chunk.  The block A of a move is loaded when it is accepted and folded
+// into pot at the next trial, so its latency overlaps that trial's other
+// loads.
+struct McChunkPot {
+    double pot[3];
+    double a[9];    // block of the pending move at the lane's site (zero: none)
+    double dmu[3];  // the pending move's dmu (wave-uniform)
+    bool pending;   // wave-uniform
+};
+
+__device__ __forceinline__ void mc_pot_start(McChunkPot& P, const McChunkIO& io, const double* __restrict__ Aew,
+                                             int na, const ReplicaInfo& R, int r, uint32_t g, uint32_t total, int i,
+                                             int c1, int c2, int c3)
+{
+    double s[3] = {0, 0, 0};
+    if (io.napp) mc_moves_sum(Aew, na, R, io.prev + (size_t)r * kMcChunk, io.nprev[r], i, c1, c2, c3, s);
+#pragma unroll
+    for (int q = 0; q < 3; q++) P.pot[q] = io.phi_in[g + q * (size_t)total] + s[q];
+    P.pending = false;
+}
+
+// Move (im; cm1, cm2, cm3; dmu) accepted at trial t: lanes t < l < ntrial
+// load their block.
+__device__ __forceinline__ void mc_pot_accept(McChunkPot& P, const double* __restrict__ Aew, int na,
+                                              const ReplicaInfo& R, int t, int ntrial, int im, int cm1, int cm2,
+                                              int cm3, const double dmu[3], int i, int c1, int c2, int c3)
+{
+    const int l = (int)threadIdx.x;
+    if (l > t && l < ntrial) {
+        const int o1 = mc_wrap_to(cm1 - c1, R.L1, false), o2 = mc_wrap_to(cm2 - c2, R.L2, false),
+                  o3 = mc_wrap_to(cm3 - c3, R.L3, false);  // differences of cells in [0, L)
+        const size_t off = (size_t)o1 + (size_t)R.L1 * ((size_t)o2 + (size_t)R.L2 * (size_t)o3);
+        const double* a = Aew + (((size_t)im * na + i) * (size_t)R.ncell + off) * 9;
+#pragma unroll
+        for (int q = 0; q < 9; q++) P.a[q] = a[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; q++) P.a[q] = 0;
+    }
+    P.dmu[0] = dmu[0];
+    P.dmu[1] = dmu[1];
+    P.dmu[2] = dmu[2];
+    P.pending = true;
+}
+
+// Folds the pending move into pot and returns trial t's potential (every
+// lane gets it).
+__device__ __forceinline__ void mc_pot_take(McChunkPot& P, int t, double p[3])
+{
+    if (P.pending) {
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+            P.pot[q] += P.a[3 * q] * P.dmu[0] + P.a[3 * q + 1] * P.dmu[1] + P.a[3 * q + 2] * P.dmu[2];
+        P.pending = false;
+    }
+    p[0] = lane_value(P.pot[0], t);
+    p[1] = lane_value(P.pot[1], t);
+    p[2] = lane_value(P.pot[2], t);
 }
 
 // Padded index of the neighbour across gather slot b of the site.

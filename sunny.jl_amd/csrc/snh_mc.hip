@@ -28,33 +28,55 @@
 // disjoint from the Langevin stream (second counter word 0).
 //
 // With the Ewald term every pair of sites interacts, so no two trials are
-// independent: trials then run one at a time per replica (k_mc_ewald_trial),
-// each followed by an update of the FP64 Ewald potential phi at every site
-// of the replica (k_mc_ewald_update, O(N) per accepted move as in
-// Ewald.jl:212-241).
+// independent: trials then run one after another per replica, in chunks of
+// up to kMcChunk per launch (k_mc_ewald_chunk), each trial seeing the FP64
+// Ewald potential of the chunk's start plus the moves accepted since.  The
+// same launch's other workgroups add the previous chunk's moves to the
+// potential at every site of the replica (mc_apply_part, O(N) per accepted
+// move as in Ewald.jl:212-241) for the next launch, so that work overlaps
+// the chunk's serial trials.
 #include "snh_mc.h"
 #include "snh_vec.h"
 
 namespace snh {
 
-// Proposal of LocalSampler for a site holding s0: propose_flip, randspin or
-// perturbed_spin (S + mag kappa randn), chosen by u_mix; normals from
-// Philox4x32-10 of counter ctr (four normals, three used).
+// The three normals of a proposal: Philox4x32-10 of counter ctr, two
+// Box-Muller pairs (four normals, three used).
 template <typename T>
-__device__ __forceinline__ V3<T> mc_propose(const McArgs& A, V3<T> s0, T kap, double u_mix, uint32_t ctr[4],
-                                            unsigned long long key)
+__device__ __forceinline__ void mc_normals(uint32_t ctr[4], unsigned long long key, T n[3])
 {
-    const int k = mc_pick(A, u_mix);
-    if (A.kind[k] == 1) return {-s0.x, -s0.y, -s0.z};  // propose_flip
     philox4x32(ctr, (uint32_t)key, (uint32_t)(key >> 32));
-    T n[4];
-    box_muller(ctr[0], ctr[1], n[0], n[1]);
-    box_muller(ctr[2], ctr[3], n[2], n[3]);
+    T m[4];
+    box_muller(ctr[0], ctr[1], m[0], m[1]);
+    box_muller(ctr[2], ctr[3], m[2], m[3]);
+    n[0] = m[0];
+    n[1] = m[1];
+    n[2] = m[2];
+}
+
+// Proposal k of the mixture for a site holding s0: propose_flip, randspin
+// or perturbed_spin (S + mag kappa randn) from the normals n.
+template <typename T>
+__device__ __forceinline__ V3<T> mc_propose_from(const McArgs& A, int k, V3<T> s0, T kap, const T n[3])
+{
+    if (A.kind[k] == 1) return {-s0.x, -s0.y, -s0.z};  // propose_flip
     const double mag = A.mag[k];
     if (A.kind[k] == 0 || __builtin_isinf(mag))  // randspin
         return normalize_dipole(V3<T>{n[0], n[1], n[2]}, kap);
     const T f = (T)(mag * (double)kap);  // perturbed_spin
     return normalize_dipole(V3<T>{s0.x + f * n[0], s0.y + f * n[1], s0.z + f * n[2]}, kap);
+}
+
+// Proposal of LocalSampler for a site holding s0, chosen by u_mix; normals
+// from counter ctr (not drawn for propose_flip).
+template <typename T>
+__device__ __forceinline__ V3<T> mc_propose(const McArgs& A, V3<T> s0, T kap, double u_mix, uint32_t ctr[4],
+                                            unsigned long long key)
+{
+    const int k = mc_pick(A, u_mix);
+    T n[3] = {0, 0, 0};
+    if (A.kind[k] != 1) mc_normals<T>(ctr, key, n);
+    return mc_propose_from<T>(A, k, s0, kap, n);
 }
 
 // local_energy_change in FP64 (Interactions.jl:311-380) of site sc going
@@ -70,7 +92,7 @@ __device__ __forceinline__ double mc_local_delta(const Couplings<T>& C, const Mc
     const double dx = nx - ox, dy = ny - oy, dz = nz - oz;
     double dE = (double)ldc(C.zee, 3 * i) * dx + (double)ldc(C.zee, 3 * i + 1) * dy +
                 (double)ldc(C.zee, 3 * i + 2) * dz;
-    dE += stevens_energy_d(C, i, nx, ny, nz) - stevens_energy_d(C, i, ox, oy, oz);
+    dE += stevens_delta_d(C, i, nx, ny, nz, ox, oy, oz);
     double Qn[5], Qo[5];
     quadrupole_d(nx, ny, nz, Qn);
     quadrupole_d(ox, oy, oz, Qo);
@@ -166,76 +188,139 @@ __global__ __launch_bounds__(kBlock) void k_mc_color(Couplings<T> C, McArgs A, T
         for (int qv = 0; qv < 4; qv++) acc[4 * (size_t)blockIdx.x + qv] += v[qv];
 }
 
-// Sequential trial of the Ewald form, workgroup b = replica r0 + b (its
-// first lane only: philox4x32 takes its key as uniform over the wave): trial T0
-// draws the site, u_acc and u_mix by mc_seq_draw and the normals from
-// counter (T0 lo, kMcTagSequential + 1, T0 hi, 0).  dE adds
-// ewald_energy_delta (Ewald.jl:234-241).  tr[r] records an accepted move
-// (site, dmu) for k_mc_ewald_update; acc[4 r + q] += (dE, dS).
-template <typename T, int G>
-__global__ __launch_bounds__(64) void k_mc_ewald_trial(Couplings<T> C, McArgs A, T* __restrict__ S,
-                                                       const double* __restrict__ phi,
-                                                       const double* __restrict__ A0, unsigned long long T0,
-                                                       McTrial* __restrict__ tr, double* __restrict__ acc, int r0,
-                                                       int nrep)
+// Bond b's term of local_energy_change (Interactions.jl:341-368) for a
+// site changing by (dx, dy, dz) and quadrupole dQ, neighbour state (x, y, z).
+template <typename T>
+__device__ __forceinline__ double mc_bond_term(const Couplings<T>& C, int b, double dx, double dy, double dz,
+                                               const double dQ[5], double x, double y, double z)
 {
-    if (threadIdx.x != 0 || (int)blockIdx.x >= nrep) return;
+    const T* J = C.bJ + 9 * b;
+    double e = dx * ((double)J[0] * x + (double)J[1] * y + (double)J[2] * z) +
+               dy * ((double)J[3] * x + (double)J[4] * y + (double)J[5] * z) +
+               dz * ((double)J[6] * x + (double)J[7] * y + (double)J[8] * z);
+    if (C.bflag[b] & kBondBiquad) {
+        double Qj[5];
+        quadrupole_d(x, y, z, Qj);
+        const T* Bq = C.bQ + 25 * b;
+        for (int p = 0; p < 5; p++) {
+            double w = 0;
+            for (int qv = 0; qv < 5; qv++) w += (double)Bq[5 * p + qv] * Qj[qv];
+            e += dQ[p] * w;
+        }
+    }
+    return e;
+}
+
+// Neighbour indices of the sites of a chunk's trials: nbr[l][k] for bond
+// b0 + k (k < kMcNbr) of trial T0 + l; bonds beyond kMcNbr are found when
+// needed.
+constexpr int kMcNbr = 32;
+
+// Sequential trials T0 .. T0 + ntrial - 1 of the Ewald form, workgroup b <
+// nrep for replica r0 + b, one wavefront; workgroups b >= nrep are the apply
+// part (mc_apply_part).  Lane l first forms the draws of trial T0 + l
+// (mc_seq_draw; normals from counter (T lo, kMcTagSequential + 1, T hi, 0)),
+// the neighbour indices and the potential at its site; then all lanes make
+// the trials in order, each taking its draws from the lane that formed them:
+// proposal, and dE = local_energy_change + ewald_energy_delta
+// (Interactions.jl:311-380, Ewald.jl:234-241) at the potential of the
+// chunk's start plus that of the moves accepted since (McChunkPot); the bond
+// gathers are split over the lanes.  Lane 0 stores an accepted spin with its
+// ghost images.  The chunk's accepted moves go to io.moves[r][.] and
+// io.nmoves[r]; acc[4 r + q] += (dE, dS).
+template <typename T>
+__global__ __launch_bounds__(64) void k_mc_ewald_chunk(Couplings<T> C, McArgs A, T* __restrict__ S, McChunkIO io,
+                                                       const double* __restrict__ Aew,
+                                                       const double* __restrict__ A0, unsigned long long T0,
+                                                       int ntrial, double* __restrict__ acc, int r0, int nrep)
+{
+    if ((int)blockIdx.x >= nrep) {  // uniform over the workgroup
+        mc_apply_part(io, Aew, C.reps, C.G, C.na, C.total, r0, (int)blockIdx.x - nrep);
+        return;
+    }
     const int r = r0 + (int)blockIdx.x;
     const ReplicaInfo R = C.reps[r];
     const uint32_t total = C.total;
-    const uint32_t ns = (uint32_t)R.ncell * (uint32_t)C.na;
     const unsigned long long key = A.seed + (unsigned long long)r * 0x9E3779B97F4A7C15ull;
-    const McSeqDraw dr = mc_seq_draw(key, T0, ns);
-    const int i = (int)(dr.site / (uint32_t)R.ncell);
-    const int c = (int)(dr.site - (uint32_t)i * (uint32_t)R.ncell);
-    const int c1 = c % R.L1, c2 = (c / R.L1) % R.L2, c3 = c / (R.L1 * R.L2);
-    SiteCoord sc = mc_site(R, C.G, i, c1, c2, c3, true);
-    sc.r = r;
-    const V3<T> s0 = {ldg(S, sc.g), ldg(S, sc.g + total), ldg(S, sc.g + 2 * total)};
-    uint32_t cn[4] = {(uint32_t)T0, kMcTagSequential + 1u, (uint32_t)(T0 >> 32), 0u};
-    const V3<T> sn = mc_propose<T>(A, s0, ldc(C.kappa, i), dr.u_mix, cn, key);
-    double dE = mc_local_delta<T, G>(C, A, S, sc, s0, sn, A.wide != 0);
-    const double dS[3] = {(double)sn.x - (double)s0.x, (double)sn.y - (double)s0.y, (double)sn.z - (double)s0.z};
-    double dmu[3];
-    dE += mc_ewald_delta(C, i, phi, sc.g, A0, dS, dmu);
     const double kT = A.kT[r];
-    const bool accept = kT == 0.0 ? dE <= 0.0 : dr.u_acc <= exp(-dE / kT);
-    McTrial out{};
-    if (accept) {
-        mc_store(S, C, sc, sn);
-        out = {1, i, c1, c2, c3, 0, {dmu[0], dmu[1], dmu[2]}};
-        acc[4 * r] += dE;
-        for (int a = 0; a < 3; a++) acc[4 * r + 1 + a] += dS[a];
+    const bool wide = A.wide != 0;
+    const int lane = (int)threadIdx.x;
+    __shared__ uint32_t nbr[kMcChunk][kMcNbr];
+    McSeqSite my;
+    T my_n[3];
+    McChunkPot P;
+    {
+        const unsigned long long Tl = T0 + (unsigned long long)lane;
+        my = mc_seq_site(R, mc_seq_draw(key, Tl, (uint32_t)R.ncell * (uint32_t)C.na), A);
+        uint32_t cn[4] = {(uint32_t)Tl, kMcTagSequential + 1u, (uint32_t)(Tl >> 32), 0u};
+        mc_normals<T>(cn, key, my_n);
+        SiteCoord sc = mc_site(R, C.G, my.i, my.c1, my.c2, my.c3, true);
+        mc_pot_start(P, io, Aew, C.na, R, r, sc.g, total, my.i, my.c1, my.c2, my.c3);
+        const int b0 = C.bstart[my.i], nb = min(C.bstart[my.i + 1] - b0, kMcNbr);
+        for (int k = 0; k < nb; k++) nbr[lane][k] = mc_neighbour(A, C.G, sc, b0 + k, wide);
     }
-    tr[r] = out;
-}
-
-// After trial tr[r] was accepted: phi at every site j of replica r = r0 +
-// grid.y gains A(c_s - c_j; i_s, i_j) dmu (phi = sum_k A mu_k,
-// Ewald.jl:185-209).  Aew: real-space tensor [i_s][i_j][cell offset][3][3]
-// of the replica's dims, FP64.
-__global__ __launch_bounds__(256) void k_mc_ewald_update(const double* __restrict__ Aew, const ReplicaInfo* reps,
-                                                         Ghost G, int na, uint32_t total,
-                                                         const McTrial* __restrict__ tr, double* __restrict__ phi,
-                                                         int r0)
-{
-    const int r = r0 + (int)blockIdx.y;
-    const McTrial m = tr[r];
-    if (!m.flag) return;  // uniform over the workgroup
-    const ReplicaInfo R = reps[r];
-    const uint32_t ncell = (uint32_t)R.ncell;
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ncell * (uint32_t)na) return;
-    const int ij = (int)(j / ncell);
-    const int c = (int)(j - (uint32_t)ij * ncell);
-    const int c1 = c % R.L1, c2 = (c / R.L1) % R.L2, c3 = c / (R.L1 * R.L2);
-    const int o1 = mc_wrap(m.c1 - c1, R.L1), o2 = mc_wrap(m.c2 - c2, R.L2), o3 = mc_wrap(m.c3 - c3, R.L3);
-    const size_t off = (size_t)o1 + (size_t)R.L1 * ((size_t)o2 + (size_t)R.L2 * (size_t)o3);
-    const double* a = Aew + (((size_t)m.i * na + ij) * ncell + off) * 9;
-    const SiteCoord sc = mc_site(R, G, ij, c1, c2, c3, true);
+    __syncthreads();
+    McTrial mv{};
+    int nm = 0;
+    double sum[4] = {0, 0, 0, 0};
+    for (int t = 0; t < ntrial; t++) {
+        const McSeqSite d = mc_seq_site_of_lane(my, t);
+        const T n[3] = {lane_value(my_n[0], t), lane_value(my_n[1], t), lane_value(my_n[2], t)};
+        const int i = d.i;
+        SiteCoord sc = mc_site(R, C.G, i, d.c1, d.c2, d.c3, true);
+        sc.r = r;
+        const V3<T> s0 = {ldg(S, sc.g), ldg(S, sc.g + total), ldg(S, sc.g + 2 * total)};
+        // the lane's first bond, gathered before anything waits
+        const int b0 = ldc(C.bstart, i), b1 = ldc(C.bstart, i + 1);
+        const int bl = b0 + lane;
+        double jx = 0, jy = 0, jz = 0;
+        if (bl < b1) {
+            const uint32_t gj = lane < kMcNbr ? nbr[t][lane] : mc_neighbour(A, C.G, sc, bl, wide);
+            jx = ldg(S, gj);
+            jy = ldg(S, gj + total);
+            jz = ldg(S, gj + 2 * total);
+        }
+        double p[3];
+        mc_pot_take(P, t, p);
+        const V3<T> sn = mc_propose_from<T>(A, d.k, s0, ldc(C.kappa, i), n);
+        const double ox = s0.x, oy = s0.y, oz = s0.z, nx = sn.x, ny = sn.y, nz = sn.z;
+        const double dS[3] = {nx - ox, ny - oy, nz - oz};
+        double Qn[5], Qo[5], dQ[5];
+        quadrupole_d(nx, ny, nz, Qn);
+        quadrupole_d(ox, oy, oz, Qo);
 #pragma unroll
-    for (int q = 0; q < 3; q++)
-        phi[sc.g + q * total] += a[3 * q] * m.dmu[0] + a[3 * q + 1] * m.dmu[1] + a[3 * q + 2] * m.dmu[2];
+        for (int q = 0; q < 5; q++) dQ[q] = Qn[q] - Qo[q];
+        double eb = bl < b1 ? mc_bond_term<T>(C, bl, dS[0], dS[1], dS[2], dQ, jx, jy, jz) : 0.0;
+        for (int b = bl + 64; b < b1; b += 64) {  // sublattices with more than 64 bonds
+            const uint32_t gj = mc_neighbour(A, C.G, sc, b, wide);
+            eb += mc_bond_term<T>(C, b, dS[0], dS[1], dS[2], dQ, ldg(S, gj), ldg(S, gj + total),
+                                  ldg(S, gj + 2 * total));
+        }
+        double dE = (double)ldc(C.zee, 3 * i) * dS[0] + (double)ldc(C.zee, 3 * i + 1) * dS[1] +
+                    (double)ldc(C.zee, 3 * i + 2) * dS[2];
+        dE += stevens_delta_d(C, i, nx, ny, nz, ox, oy, oz);
+        dE += wave_sum(eb);
+        double dmu[3];
+        dE += mc_ewald_delta(C, i, p, A0, dS, dmu);
+        const int accept = __builtin_amdgcn_readfirstlane(kT == 0.0 ? dE <= 0.0 : d.u_acc <= exp(-dE / kT));
+        if (accept) {
+            if (lane == 0) mc_store(S, C, sc, sn);
+            if (lane == nm) mv = {1, i, d.c1, d.c2, d.c3, 0, {dmu[0], dmu[1], dmu[2]}};
+            nm++;
+            mc_pot_accept(P, Aew, C.na, R, t, ntrial, i, d.c1, d.c2, d.c3, dmu, my.i, my.c1, my.c2, my.c3);
+            sum[0] += dE;
+            for (int a = 0; a < 3; a++) sum[1 + a] += dS[a];
+        }
+        // lane 0's store precedes every lane's loads of the next trial: the
+        // lanes of one wavefront see each other's writes in program order
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (lane < nm) io.moves[(size_t)r * kMcChunk + lane] = mv;
+    if (lane == 0) {
+        io.nmoves[r] = nm;
+        for (int q = 0; q < 4; q++) acc[4 * r + q] += sum[q];
+    }
 }
 
 template <typename T>
@@ -280,21 +365,12 @@ void launch_mc_color(const Couplings<T>& C, const McArgs& A, T* S, const McWork*
 }
 
 template <typename T>
-void launch_mc_ewald_trial(const Couplings<T>& C, const McArgs& A, T* S, const double* phi, const double* A0,
-                           unsigned long long T0, McTrial* tr, double* acc, int r0, int nr, hipStream_t st)
+void launch_mc_ewald_chunk(const Couplings<T>& C, const McArgs& A, T* S, const McChunkIO& io, const double* Aew,
+                           const double* A0, unsigned long long T0, int ntrial, double* acc, int r0, int nr,
+                           hipStream_t st)
 {
-    const dim3 grid(nr);
-    if (C.chunk == 6)
-        hipLaunchKernelGGL((k_mc_ewald_trial<T, 6>), grid, dim3(64), 0, st, C, A, S, phi, A0, T0, tr, acc, r0, nr);
-    else
-        hipLaunchKernelGGL((k_mc_ewald_trial<T, 9>), grid, dim3(64), 0, st, C, A, S, phi, A0, T0, tr, acc, r0, nr);
-}
-
-void launch_mc_ewald_update(const double* Aew, const ReplicaInfo* reps, const Ghost& G, int na, uint32_t total,
-                            uint32_t ns, const McTrial* tr, double* phi, int r0, int nr, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_mc_ewald_update, dim3((ns + 255) / 256, nr), dim3(256), 0, st, Aew, reps, G, na, total, tr,
-                       phi, r0);
+    hipLaunchKernelGGL(k_mc_ewald_chunk<T>, dim3(nr * (1 + io.napp)), dim3(64), 0, st, C, A, S, io, Aew, A0, T0,
+                       ntrial, acc, r0, nr);
 }
 
 template <typename T>
@@ -313,11 +389,11 @@ void launch_fill_double(double* p, long long n, double v, hipStream_t st)
     hipLaunchKernelGGL(k_fill_double, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
 }
 
-template void launch_mc_ewald_trial<float>(const Couplings<float>&, const McArgs&, float*, const double*,
-                                           const double*, unsigned long long, McTrial*, double*, int, int,
+template void launch_mc_ewald_chunk<float>(const Couplings<float>&, const McArgs&, float*, const McChunkIO&,
+                                           const double*, const double*, unsigned long long, int, double*, int, int,
                                            hipStream_t);
-template void launch_mc_ewald_trial<double>(const Couplings<double>&, const McArgs&, double*, const double*,
-                                            const double*, unsigned long long, McTrial*, double*, int, int,
+template void launch_mc_ewald_chunk<double>(const Couplings<double>&, const McArgs&, double*, const McChunkIO&,
+                                            const double*, const double*, unsigned long long, int, double*, int, int,
                                             hipStream_t);
 template void launch_planes_to_double<float>(const float*, double*, long long, hipStream_t);
 template void launch_planes_to_double<double>(const double*, double*, long long, hipStream_t);

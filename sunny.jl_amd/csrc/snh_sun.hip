@@ -688,13 +688,30 @@ __device__ __forceinline__ double onsite_delta(const Couplings<T>& C, int i, con
     return e;
 }
 
-// Proposal k of LocalSampler for ket z0: flip_ket, randspin or
-// perturbed_spin Z + mag sqrt(kappa) randn(CVec{N}).  The 2N normals come
-// from Philox4x32-10 of counter c, its second word advanced by 2 per block
-// of four.
+// The 2N normals of a ket proposal: Philox4x32-10 of counter c, its second
+// word advanced by 2 per block of four, each block two Box-Muller pairs.
 template <typename T, int N>
-__device__ __forceinline__ Ket<T, N> sun_mc_propose(const McArgs& A, int k, const Ket<T, N>& z0, T kap,
-                                                    const uint32_t c[4], unsigned long long key)
+__device__ __forceinline__ void sun_mc_normals(const uint32_t c[4], unsigned long long key, T n[2 * N])
+{
+    constexpr int NB = (2 * N + 3) / 4;
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+        uint32_t cn[4] = {c[0], c[1] + 2u * (uint32_t)b, c[2], c[3]};
+        philox4x32(cn, (uint32_t)key, (uint32_t)(key >> 32));
+        T m[4];
+        box_muller(cn[0], cn[1], m[0], m[1]);
+        box_muller(cn[2], cn[3], m[2], m[3]);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (4 * b + q < 2 * N) n[4 * b + q] = m[q];
+    }
+}
+
+// Proposal k of LocalSampler for ket z0: flip_ket, randspin or
+// perturbed_spin Z + mag sqrt(kappa) randn(CVec{N}) from the normals n.
+template <typename T, int N>
+__device__ __forceinline__ Ket<T, N> sun_mc_propose_from(const McArgs& A, int k, const Ket<T, N>& z0, T kap,
+                                                         const T n[2 * N])
 {
     Ket<T, N> zn;
     if (A.kind[k] == 1) {  // flip_ket: reverse(parity .* conj(Z))
@@ -706,15 +723,6 @@ __device__ __forceinline__ Ket<T, N> sun_mc_propose(const McArgs& A, int k, cons
         }
         return zn;
     }
-    constexpr int NB = (2 * N + 3) / 4;
-    T n[4 * NB];
-#pragma unroll
-    for (int b = 0; b < NB; b++) {
-        uint32_t cn[4] = {c[0], c[1] + 2u * (uint32_t)b, c[2], c[3]};
-        philox4x32(cn, (uint32_t)key, (uint32_t)(key >> 32));
-        box_muller(cn[0], cn[1], n[4 * b], n[4 * b + 1]);
-        box_muller(cn[2], cn[3], n[4 * b + 2], n[4 * b + 3]);
-    }
     const double mag = A.mag[k];
     const bool unif = A.kind[k] == 0 || __builtin_isinf(mag);
     const T f = unif ? (T)0 : (T)(mag * sqrt((double)kap));
@@ -725,6 +733,19 @@ __device__ __forceinline__ Ket<T, N> sun_mc_propose(const McArgs& A, int k, cons
         zn.z[a] = unif ? Cx<T>{nr, ni} : Cx<T>{z0.z[a].re + f * nr, z0.z[a].im + f * ni};
     }
     return normalize_ket<T, N>(zn, kap);
+}
+
+// Proposal k of LocalSampler for ket z0, normals from counter c (not drawn
+// for flip_ket).
+template <typename T, int N>
+__device__ __forceinline__ Ket<T, N> sun_mc_propose(const McArgs& A, int k, const Ket<T, N>& z0, T kap,
+                                                    const uint32_t c[4], unsigned long long key)
+{
+    T n[2 * N];
+#pragma unroll
+    for (int q = 0; q < 2 * N; q++) n[q] = 0;
+    if (A.kind[k] != 1) sun_mc_normals<T, N>(c, key, n);
+    return sun_mc_propose_from<T, N>(A, k, z0, kap, n);
 }
 
 // local_energy_change (Interactions.jl:311-380) of site sc going from z0 to
@@ -837,47 +858,82 @@ __global__ __launch_bounds__(kBlock) void k_sun_mc_color(Couplings<T> C, McArgs 
         for (int q = 0; q < 4; q++) acc[4 * (size_t)blockIdx.x + q] += v[q];
 }
 
-// Sequential trial T0 of the Ewald form for :SUN, workgroup b = replica
-// r0 + b (its first lane only): the draws of the :dipole form (mc_seq_draw; normals
-// from the counters (T0 lo, kMcTagSequential + 1 + 2 b, T0 hi, 0)), the
-// energy change of the ket plus ewald_energy_delta of its expected dipole
-// (Ewald.jl:234-241).  tr[r] records an accepted move for
-// k_mc_ewald_update; acc[4 r + q] += (dE, dS).
+// Sequential trials T0 .. T0 + ntrial - 1 of the Ewald form for :SUN,
+// workgroup b < nrep for replica r0 + b, one wavefront; workgroups b >= nrep
+// are the apply part (mc_apply_part).  Lane l first forms the draws of
+// trial T0 + l (those of the :dipole form: mc_seq_draw; normals from the
+// counters (T lo, kMcTagSequential + 1, T hi, 0)) and the potential at its
+// site; then all lanes make the trials in order, each taking its draws from
+// the lane that formed them: the energy change of the ket plus
+// ewald_energy_delta of its expected dipole (Ewald.jl:234-241) at the
+// potential of the chunk's start plus that of the moves accepted since
+// (McChunkPot).  Lane 0 stores an accepted ket with its ghost images.  The
+// chunk's accepted moves go to io.moves[r][.] and io.nmoves[r];
+// acc[4 r + q] += (dE, dS).
 template <typename T, int N>
-__global__ __launch_bounds__(64) void k_sun_mc_ewald_trial(Couplings<T> C, McArgs A, T* __restrict__ Z,
-                                                           const double* __restrict__ phi,
+__global__ __launch_bounds__(64) void k_sun_mc_ewald_chunk(Couplings<T> C, McArgs A, T* __restrict__ Z, McChunkIO io,
+                                                           const double* __restrict__ Aew,
                                                            const double* __restrict__ A0, unsigned long long T0,
-                                                           McTrial* __restrict__ tr, double* __restrict__ acc,
-                                                           int r0, int nrep)
+                                                           int ntrial, double* __restrict__ acc, int r0, int nrep)
 {
-    if (threadIdx.x != 0 || (int)blockIdx.x >= nrep) return;
+    if ((int)blockIdx.x >= nrep) {  // uniform over the workgroup
+        mc_apply_part(io, Aew, C.reps, C.G, C.na, C.total, r0, (int)blockIdx.x - nrep);
+        return;
+    }
     const int r = r0 + (int)blockIdx.x;
     const ReplicaInfo R = C.reps[r];
-    const uint32_t ns = (uint32_t)R.ncell * (uint32_t)C.na;
+    const uint32_t total = C.total;
     const unsigned long long key = A.seed + (unsigned long long)r * 0x9E3779B97F4A7C15ull;
-    const McSeqDraw dr = mc_seq_draw(key, T0, ns);
-    const int i = (int)(dr.site / (uint32_t)R.ncell);
-    const int c = (int)(dr.site - (uint32_t)i * (uint32_t)R.ncell);
-    const int c1 = c % R.L1, c2 = (c / R.L1) % R.L2, c3 = c / (R.L1 * R.L2);
-    SiteCoord sc = mc_site(R, C.G, i, c1, c2, c3, true);
-    sc.r = r;
-    const Ket<T, N> z0 = load_ket<T, N>(Z, C.total, sc.g);
-    const uint32_t cn[4] = {(uint32_t)T0, kMcTagSequential + 1u, (uint32_t)(T0 >> 32), 0u};
-    const Ket<T, N> zn = sun_mc_propose<T, N>(A, mc_pick(A, dr.u_mix), z0, ldc(C.kappa, i), cn, key);
-    double dS[3];
-    double dE = sun_mc_local_delta<T, N, sun_mc_gather<T, N>()>(C, A, Z, sc, z0, zn, A.wide != 0, dS);
-    double dmu[3];
-    dE += mc_ewald_delta(C, i, phi, sc.g, A0, dS, dmu);
     const double kT = A.kT[r];
-    const bool accept = kT == 0.0 ? dE <= 0.0 : dr.u_acc <= exp(-dE / kT);
-    McTrial out{};
-    if (accept) {
-        store_ket_images<T, N>(Z, C.total, sc, C.G, zn);
-        out = {1, i, c1, c2, c3, 0, {dmu[0], dmu[1], dmu[2]}};
-        acc[4 * r] += dE;
-        for (int a = 0; a < 3; a++) acc[4 * r + 1 + a] += dS[a];
+    McSeqSite my;
+    T my_n[2 * N];
+    McChunkPot P;
+    {
+        const unsigned long long Tl = T0 + threadIdx.x;
+        my = mc_seq_site(R, mc_seq_draw(key, Tl, (uint32_t)R.ncell * (uint32_t)C.na), A);
+        const uint32_t cn[4] = {(uint32_t)Tl, kMcTagSequential + 1u, (uint32_t)(Tl >> 32), 0u};
+        sun_mc_normals<T, N>(cn, key, my_n);
+        mc_pot_start(P, io, Aew, C.na, R, r, mc_site(R, C.G, my.i, my.c1, my.c2, my.c3, true).g, total, my.i,
+                     my.c1, my.c2, my.c3);
     }
-    tr[r] = out;
+    McTrial mv{};
+    int nm = 0;
+    double sum[4] = {0, 0, 0, 0};
+    for (int t = 0; t < ntrial; t++) {
+        const McSeqSite d = mc_seq_site_of_lane(my, t);
+        T n[2 * N];
+#pragma unroll
+        for (int q = 0; q < 2 * N; q++) n[q] = lane_value(my_n[q], t);
+        const int i = d.i;
+        SiteCoord sc = mc_site(R, C.G, i, d.c1, d.c2, d.c3, true);
+        sc.r = r;
+        const Ket<T, N> z0 = load_ket<T, N>(Z, total, sc.g);
+        double p[3];
+        mc_pot_take(P, t, p);
+        const Ket<T, N> zn = sun_mc_propose_from<T, N>(A, d.k, z0, ldc(C.kappa, i), n);
+        double dS[3];
+        double dE = sun_mc_local_delta<T, N, sun_mc_gather<T, N>()>(C, A, Z, sc, z0, zn, A.wide != 0, dS);
+        double dmu[3];
+        dE += mc_ewald_delta(C, i, p, A0, dS, dmu);
+        const int accept = __builtin_amdgcn_readfirstlane(kT == 0.0 ? dE <= 0.0 : d.u_acc <= exp(-dE / kT));
+        if (accept) {
+            if (threadIdx.x == 0) store_ket_images<T, N>(Z, total, sc, C.G, zn);
+            if ((int)threadIdx.x == nm) mv = {1, i, d.c1, d.c2, d.c3, 0, {dmu[0], dmu[1], dmu[2]}};
+            nm++;
+            mc_pot_accept(P, Aew, C.na, R, t, ntrial, i, d.c1, d.c2, d.c3, dmu, my.i, my.c1, my.c2, my.c3);
+            sum[0] += dE;
+            for (int a = 0; a < 3; a++) sum[1 + a] += dS[a];
+        }
+        // lane 0's store precedes every lane's loads of the next trial: the
+        // lanes of one wavefront see each other's writes in program order
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+    if ((int)threadIdx.x < nm) io.moves[(size_t)r * kMcChunk + threadIdx.x] = mv;
+    if (threadIdx.x == 0) {
+        io.nmoves[r] = nm;
+        for (int q = 0; q < 4; q++) acc[4 * r + q] += sum[q];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -985,16 +1041,18 @@ void launch_sun_mc_color(const Couplings<T>& C, const McArgs& A, T* Z, const McW
 }
 
 template <typename T>
-void launch_sun_mc_ewald_trial(const Couplings<T>& C, const McArgs& A, T* Z, const double* phi, const double* A0,
-                               unsigned long long T0, McTrial* tr, double* acc, int r0, int nr, hipStream_t st)
+void launch_sun_mc_ewald_chunk(const Couplings<T>& C, const McArgs& A, T* Z, const McChunkIO& io, const double* Aew,
+                               const double* A0, unsigned long long T0, int ntrial, double* acc, int r0, int nr,
+                               hipStream_t st)
 {
-    SNH_SUN_DISPATCH(C.N, k_sun_mc_ewald_trial, dim3(nr), dim3(64), 0, st, C, A, Z, phi, A0, T0, tr, acc, r0, nr)
+    SNH_SUN_DISPATCH(C.N, k_sun_mc_ewald_chunk, dim3(nr * (1 + io.napp)), dim3(64), 0, st, C, A, Z, io, Aew, A0, T0,
+                     ntrial, acc, r0, nr)
 }
 
 #define SNH_SUN_INST(T)                                                                                          \
-    template void launch_sun_mc_ewald_trial<T>(const Couplings<T>&, const McArgs&, T*, const double*,           \
-                                               const double*, unsigned long long, McTrial*, double*, int, int,   \
-                                               hipStream_t);                                                     \
+    template void launch_sun_mc_ewald_chunk<T>(const Couplings<T>&, const McArgs&, T*, const McChunkIO&,        \
+                                               const double*, const double*, unsigned long long, int, double*,   \
+                                               int, int, hipStream_t);                                           \
     template void launch_sun_mc_color<T>(const Couplings<T>&, const McArgs&, T*, const McWork*, int,            \
                                          unsigned long long, double, double*, hipStream_t);                      \
     template void launch_sun_expected_spin<T>(const Couplings<T>&, int, const T*, T*, hipStream_t);              \

@@ -144,9 +144,12 @@ def test_langevin_after_sweeps_sees_updated_images():
 
 @pytest.mark.parametrize("precision", [64, 32])
 def test_ewald_sequential_trials_match_oracle(precision):
-    """With the Ewald term the device tries one site at a time per replica and
-    updates the FP64 potential everywhere after each accepted move; the
-    oracle's so_local_sequential makes the same trials with the same draws."""
+    """With the Ewald term the device tries one site at a time per replica, in
+    chunks of 64 trials per launch (each trial sees the potential of the
+    chunk's start plus the moves accepted since), and adds a chunk's moves to
+    the FP64 potential everywhere after it; the oracle's so_local_sequential
+    makes the same trials with the same draws, updating after every move.
+    pyro321 (96 sites, 192 trials) spans three chunks, the last a partial one."""
     cases = [("pyro1", sh.workloads.pyrochlore_model(L=1, seed=23, precision=precision, dipolar=True)),
              ("pyro211", sh.workloads.pyrochlore_model(dims=(2, 1, 1), seed=24, precision=precision, dipolar=True)),
              # an axis of length 3: offsets differ from their negatives, so the
@@ -173,7 +176,7 @@ def test_ewald_sequential_trials_match_oracle(precision):
 
 @pytest.mark.parametrize("precision", [64, 32])
 def test_sun_ewald_sequential_trials_match_oracle(precision):
-    """:SUN with Ewald: the device's sequential trials (k_sun_mc_ewald_trial)
+    """:SUN with Ewald: the device's sequential trials (k_sun_mc_ewald_chunk)
     against the oracle's so_local_sequential_sun with the same draws."""
     from test_oracle_local import sun_dipolar_system
     s = sun_dipolar_system(seed=26)
