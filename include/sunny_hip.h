@@ -82,7 +82,10 @@ typedef struct {
 
 /* Device layouts of the :dipole gather.  AUTO picks the LDS-tiled sweep
  * (unpadded planes) whenever its tile fits the LDS, else the ghost-padded
- * global gather; GHOST forces the latter.  Results agree to rounding. */
+ * global gather; GHOST forces the latter.  Results agree to rounding.  AUTO
+ * also takes the ghost layout for one replica of 513 to 8192 sites without
+ * Ewald, whose midpoint steps then run in one cooperative launch per call
+ * (snh_query "cooperative"). */
 #define SNH_LAYOUT_AUTO 0
 #define SNH_LAYOUT_GHOST 1
 

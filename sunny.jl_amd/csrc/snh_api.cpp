@@ -464,6 +464,9 @@ struct SystemImpl final : snh_system {
     int last_iters = 4;
     // Small :dipole replicas on the ghost layout (no Ewald) step in one
     // persistent workgroup per replica: no launch per stage, no host check
+    bool coop = false;  // cooperative midpoint launch (one replica, ghost layout)
+    double* coop_partial = nullptr;  // [2 nblocks]
+    unsigned* coop_bar = nullptr;    // count, gen, err
     // per midpoint step.
     bool persist = false;
     // Tiled Langevin batches of two or more replicas (no Ewald) step as two
@@ -595,7 +598,14 @@ struct SystemImpl final : snh_system {
         }
         SNH_REQUIRE(sp.layout == SNH_LAYOUT_AUTO || sp.layout == SNH_LAYOUT_GHOST, SNH_ERR_INVALID,
                     "layout must be SNH_LAYOUT_AUTO or SNH_LAYOUT_GHOST");
-        if (N == 0 && sp.layout == SNH_LAYOUT_AUTO) {
+        // one small :dipole replica without Ewald: midpoint steps in one
+        // cooperative launch (ghost layout), above the one-workgroup size
+        {
+            const long long sites = sp.nreplicas == 1 ? (long long)sp.dims[0] * sp.dims[1] * sp.dims[2] * na : 0;
+            coop = N == 0 && !ewald && R == 1 && sites > kPersist &&
+                   sites <= env_int("SNH_COOP_MAX_SITES", kCoopMaxSites) && env_int("SNH_COOP", 1) != 0;
+        }
+        if (N == 0 && sp.layout == SNH_LAYOUT_AUTO && !coop) {
             int dev = 0;
             SNH_HIP(hipGetDevice(&dev));
             hipDeviceProp_t prop;
@@ -1811,6 +1821,42 @@ struct SystemImpl final : snh_system {
         SNH_HIP(hipMemcpyAsync(atol2, a2.data(), sizeof(double) * R, hipMemcpyHostToDevice, st));
         SNH_HIP(hipStreamSynchronize(st));
         const T tdt = (T)dt, tl = (T)damping;
+        if (coop && !tiled && nsteps > 0) {
+            // every step iterated to convergence in one cooperative launch
+            if (!fail_step) fail_step = own(dalloc<int>(R));
+            if (!coop_partial) coop_partial = own(dalloc<double>(2 * (size_t)nblocks));
+            if (!coop_bar) coop_bar = own(dalloc<unsigned>(3));
+            launch_fill_int(fail_step, R, -1, st);
+            launch_fill_int(conv_iter, R, 0, st);
+            launch_fill_int(reinterpret_cast<int*>(coop_bar), 3, 0, st);
+            const GridBar bar{coop_bar, coop_bar + 1, reinterpret_cast<int*>(coop_bar + 2), 1u << 22};
+            hipEvent_t e = tbegin();
+            SNH_HIP(launch_dipole_midpoint_coop<T>(C, nblocks, X, B1, B2, Bh, Bh2, tdt, tl, atol2, max_iters, nsteps,
+                                                   conv_iter, fail_step, coop_partial, bar, st));
+            tend("midpoint_coop", e);
+            SNH_HIP(hipGetLastError());
+            int h[3];
+            SNH_HIP(hipMemcpyAsync(pinned_i, conv_iter, sizeof(int), hipMemcpyDeviceToHost, st));
+            SNH_HIP(hipStreamSynchronize(st));
+            h[0] = pinned_i[0];
+            SNH_HIP(hipMemcpyAsync(pinned_i, fail_step, sizeof(int), hipMemcpyDeviceToHost, st));
+            SNH_HIP(hipStreamSynchronize(st));
+            h[1] = pinned_i[0];
+            SNH_HIP(hipMemcpyAsync(pinned_i, coop_bar + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+            SNH_HIP(hipStreamSynchronize(st));
+            h[2] = pinned_i[0];
+            SNH_REQUIRE(h[2] == 0, SNH_ERR_HIP, "cooperative midpoint: a grid barrier wait timed out");
+            if (iters) iters[0] = h[0];
+            last_iters = h[0];
+            if (h[1] >= 0) {
+                char msg[256];
+                snprintf(msg, sizeof msg,
+                         "Spherical midpoint method failed to converge to tolerance %g after %d iterations.", tol,
+                         max_iters);
+                throw Failure{SNH_ERR_NOCONVERGE, msg};
+            }
+            return;
+        }
         if (persist && R == 1 && nsteps > 0) {
             // every step iterated to convergence on the device; one host check per call
             if (!fail_step) fail_step = own(dalloc<int>(R));
@@ -2323,6 +2369,7 @@ struct SystemImpl final : snh_system {
         if (key == "gather_chunk") return C.chunk;
         if (key == "graph_steps") return graph_steps;
         if (key == "persistent") return persist ? 1 : 0;
+        if (key == "cooperative") return coop ? 1 : 0;
         if (key == "graph_live") return graph_exec ? 1 : 0;
         if (key == "pair_split") return pair_split;
         if (key == "pair_on") return pair_on ? 1 : 0;

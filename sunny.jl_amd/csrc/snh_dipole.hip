@@ -308,6 +308,105 @@ __global__ __launch_bounds__(kPersist) void k_dipole_midpoint_persist(Couplings<
     }
 }
 
+// ---------------------------------------------------------------------------
+// Cooperative midpoint for one small replica: one workgroup per 256-site
+// block, all resident (cooperative launch), a grid barrier per fixed-point
+// iteration instead of a sweep launch and a check launch.
+
+// Barrier over the gridDim.x workgroups of a cooperative launch: a counter
+// and a generation in device memory (zero at launch).  Thread 0 of each
+// workgroup arrives after a release fence at agent scope (its workgroup's
+// stores become visible device-wide) and waits for the generation to move
+// with acquire loads; the last to arrive resets the counter and moves it.
+// A wait longer than bar.spin polls sets *bar.err and proceeds, and once
+// *bar.err is set no wait lasts, so every wave still exits promptly.
+__device__ __forceinline__ void grid_sync(const GridBar& bar, unsigned& epoch)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const unsigned arrived = __hip_atomic_fetch_add(bar.count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (arrived == gridDim.x - 1) {
+            __hip_atomic_store(bar.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(bar.gen, epoch + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned n = 0;
+            while (__hip_atomic_load(bar.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == epoch) {
+                if (++n > bar.spin || __hip_atomic_load(bar.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    __hip_atomic_store(bar.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    ++epoch;
+    __syncthreads();
+}
+
+// nsteps spherical-midpoint steps of the handle's one replica, each iterated
+// to fast_isapprox convergence as k_dipole_midpoint_persist does, workgroup
+// b sweeping block b.  The residual of iteration m: each workgroup's block
+// sum goes to partial[(m % 2) nb + b] and, after the barrier, every
+// workgroup sums the nb partials as k_midpoint_check does, so all take the
+// same decision (the same one the launched sweeps take).
+template <typename T, int CH>
+__global__ __launch_bounds__(kBlock) void k_dipole_midpoint_coop(Couplings<T> C, T* X, T* B1, T* B2, T* Bh, T* Bh2,
+                                                                 T dt, T lambda, const double* __restrict__ atol2,
+                                                                 int max_iters, int nsteps,
+                                                                 int* __restrict__ iters_out,
+                                                                 int* __restrict__ fail_step, double* partial,
+                                                                 GridBar bar)
+{
+    __shared__ double s_tot;
+    const int nb = (int)gridDim.x;
+    unsigned epoch = 0;
+    const SiteCoord sc = site_coord(C.blocks, C.reps, C.G);
+    T* const bufs[2] = {B2, B1};
+    T* const hats[2] = {Bh2, Bh};
+    const double a2 = atol2[0];
+    for (int s = 0; s < nsteps; s++) {
+        // the first guess S' = S: Shat = kappa n((S + S)/2) = kappa n(S)
+        if (sc.valid)
+            store3_images(Bh, C.total, sc, C.G, normalize_dipole(planes(X, C.total)(sc.g), ldc(C.kappa, sc.i)));
+        grid_sync(bar, epoch);
+        int it = 0;
+        bool conv = false;
+        while (!conv && it < max_iters) {
+            ++it;
+            const T* S1 = it == 1 ? X : bufs[(it + 1) % 2];
+            double d2 = 0;
+            if (sc.valid)
+                d2 = dipole_mid_sweep_site<T, CH>(C, sc, X, S1, bufs[it % 2], hats[it % 2], nullptr, dt, lambda,
+                                                  hats[(it + 1) % 2]);
+            // slots alternate: a workgroup rewrites slot it % 2 only after the
+            // next barrier, by which every workgroup has read it
+            double* part = partial + (it % 2) * nb;
+            const double t = block_sum(d2);
+            if (threadIdx.x == 0) part[blockIdx.x] = t;
+            grid_sync(bar, epoch);
+            double v = 0;
+            for (int k = (int)threadIdx.x; k < nb; k += kBlock) v += part[k];
+            const double tot = block_sum(v);
+            if (threadIdx.x == 0) s_tot = tot;
+            __syncthreads();
+            conv = s_tot <= a2 && !isnan(s_tot);  // uniform over the grid
+        }
+        if (!conv) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                fail_step[0] = s;
+                iters_out[0] = it;
+            }
+            return;
+        }
+        if (sc.valid)
+            store3_images(X, C.total, sc, C.G,
+                          normalize_dipole(planes(bufs[it % 2], C.total)(sc.g), ldc(C.kappa, sc.i)));
+        if (blockIdx.x == 0 && threadIdx.x == 0) iters_out[0] = it;
+    }
+}
+
 template <typename T, int CH>
 __global__ __launch_bounds__(kBlock) void k_dipole_grad(Couplings<T> C, const T* __restrict__ S,
                                                         const T* __restrict__ phi, T* __restrict__ dE)
@@ -585,6 +684,21 @@ void launch_dipole_midpoint_persist(const Couplings<T>& C, int R, const int* rbl
                        Bh2, dt, lambda, atol2, max_iters, nsteps, iters_out, fail_step);
 }
 
+template <typename T>
+hipError_t launch_dipole_midpoint_coop(const Couplings<T>& C, int nblocks, T* X, T* B1, T* B2, T* Bh, T* Bh2, T dt,
+                                       T lambda, const double* atol2, int max_iters, int nsteps, int* iters_out,
+                                       int* fail_step, double* partial, const GridBar& bar, hipStream_t st)
+{
+    Couplings<T> c = C;
+    GridBar b = bar;
+    void* args[] = {&c, &X, &B1, &B2, &Bh, &Bh2, &dt, &lambda, &atol2, &max_iters, &nsteps, &iters_out, &fail_step,
+                    &partial, &b};
+    const void* fn = C.chunk == 18   ? reinterpret_cast<const void*>(&k_dipole_midpoint_coop<T, 18>)
+                     : C.chunk == 9 ? reinterpret_cast<const void*>(&k_dipole_midpoint_coop<T, 9>)
+                                    : reinterpret_cast<const void*>(&k_dipole_midpoint_coop<T, 6>);
+    return hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(kBlock), args, 0, st);
+}
+
 #define SNH_INST(T)                                                                                              \
     template void launch_dipole_predict<T>(const Couplings<T>&, int, const T*, T*, T*, const T*,                \
                                            const NoiseArgs<T>&, T, T, hipStream_t);                             \
@@ -592,6 +706,9 @@ void launch_dipole_midpoint_persist(const Couplings<T>& C, int R, const int* rbl
                                            hipStream_t);                                                         \
     template void launch_dipole_midpoint_hat<T>(const Couplings<T>&, int, const T*, const T*, T*, const int*,    \
                                                 hipStream_t);                                                    \
+    template hipError_t launch_dipole_midpoint_coop<T>(const Couplings<T>&, int, T*, T*, T*, T*, T*, T, T,       \
+                                                       const double*, int, int, int*, int*, double*,             \
+                                                       const GridBar&, hipStream_t);                             \
     template void launch_dipole_midpoint_iter<T>(const Couplings<T>&, int, const T*, const T*, T*, const T*,     \
                                                  const T*, T, T, double*, const int*, T*, hipStream_t);                     \
     template void launch_dipole_midpoint_finalize_mu<T>(const Couplings<T>&, int, T*, const T*, const T*,        \

@@ -199,6 +199,24 @@ void launch_mid_budget(int* conv_iter, int* fail_step, int* max_iter, int* last_
 template <typename T>
 void launch_dipole_langevin_persist(const Couplings<T>& C, int R, const int* rblk, T* X, T* Xp, T* Xo,
                                    const NoiseArgs<T>& nz, T dt, T lambda, int nsteps, hipStream_t st);
+// Largest one-replica handle stepped by the cooperative midpoint launch.
+constexpr int kCoopMaxSites = 8192;
+// Barrier state of a cooperative launch: count and gen zero at launch;
+// *err set when a wait exceeded spin polls.
+struct GridBar {
+    unsigned* count;
+    unsigned* gen;
+    int* err;
+    unsigned spin;
+};
+// nsteps midpoint steps of a one-replica handle (ghost layout, no Ewald) in
+// one cooperative launch of nblocks workgroups (one per block); partial:
+// [2 nblocks].  Returns the launch status (a grid that cannot be resident is
+// refused).
+template <typename T>
+hipError_t launch_dipole_midpoint_coop(const Couplings<T>& C, int nblocks, T* X, T* B1, T* B2, T* Bh, T* Bh2, T dt,
+                                       T lambda, const double* atol2, int max_iters, int nsteps, int* iters_out,
+                                       int* fail_step, double* partial, const GridBar& bar, hipStream_t st);
 template <typename T>
 void launch_dipole_midpoint_persist(const Couplings<T>& C, int R, const int* rblk, T* X, T* B1, T* B2, T* Bh, T* Bh2,
                                     T dt, T lambda, const double* atol2, int max_iters, int nsteps, int* iters_out,

@@ -166,9 +166,13 @@ def test_replica_batch_uneven(precision, layout):
         assert abs(E_dev[r] - o.energy(S)) <= (1e-10 if precision == 64 else 1e-4) * energy_scale(s)
 
 
-def test_layout_selection():
+def test_layout_selection(monkeypatch):
     """The automatic plan picks the LDS-tiled sweep for :dipole models and
-    the ghost-padded gather when asked or in :SUN mode."""
+    the ghost-padded gather when asked, in :SUN mode, and for one small
+    replica stepped by the cooperative midpoint."""
+    c = sh.workloads.pyrochlore_model(L=4, seed=1, precision=32)
+    assert c._device().query("cooperative") == 1 and c._device().query("layout") == 1
+    monkeypatch.setenv("SNH_COOP", "0")
     s = sh.workloads.pyrochlore_model(L=4, seed=1, precision=32)
     h = s._device()
     assert h.query("layout") == 0 and h.query("tile_T1") >= 1 and h.query("tile_work") >= 1
@@ -235,6 +239,54 @@ def test_autotune_keeps_results():
     a._device().call("snh_get_dipoles", sh._lib.dptr(da))
     hb.call("snh_get_dipoles", sh._lib.dptr(db))
     np.testing.assert_allclose(da, db, rtol=0, atol=1e-6)
+
+
+def test_cooperative_midpoint_matches_launches(monkeypatch):
+    """One :dipole replica of 512 < sites <= 8192 without Ewald steps the
+    midpoint in one cooperative launch, a grid barrier per fixed-point
+    iteration: iteration counts and states equal, bitwise, the launched
+    sweeps on the same ghost layout, and the oracle's midpoint steps; a step
+    that does not converge raises the reference's error and keeps the state
+    of the previous step."""
+    def make(coop, seed=71):
+        monkeypatch.setenv("SNH_COOP", "1" if coop else "0")
+        s = sh.workloads.cubic_model(L=16, J=1.0, D=-0.1, seed=seed, precision=64)
+        s.layout = 1
+        assert s._device().query("cooperative") == int(coop) and s._device().query("layout") == 1
+        return s
+
+    announce("cooperative midpoint cubic 16^3 fp64")
+    a, b = make(True), make(False)
+    S0 = a.dipoles_flat()
+    for dt, n in ((0.05, 6), (0.02, 3)):
+        mp = sh.ImplicitMidpoint(dt, damping=0.1)
+        ia = sh.step_(a, mp, nsteps=n)
+        ib = sh.step_(b, mp, nsteps=n)
+        np.testing.assert_array_equal(ia, ib)
+        np.testing.assert_array_equal(a.dipoles_flat(), b.dipoles_flat())
+    o = O.Oracle(a.spec())
+    S = S0
+    for dt, n in ((0.05, 6), (0.02, 3)):
+        for _ in range(n):
+            S, _ = o.step_midpoint(S=S, dt=dt, damping=0.1)
+    assert rel_err(a.dipoles_flat(), S, scale=1.0) <= 1e-11
+    # a pyrochlore replica of 8192 sites (32 workgroups, 18 bonds per site)
+    monkeypatch.setenv("SNH_COOP", "1")
+    p = sh.workloads.pyrochlore_model(L=8, seed=72, precision=64)
+    assert p._device().query("cooperative") == 1
+    P0 = p.dipoles_flat()
+    sh.step_(p, sh.ImplicitMidpoint(0.03, damping=0.05), nsteps=3)
+    S = P0
+    op = O.Oracle(p.spec())
+    for _ in range(3):
+        S, _ = op.step_midpoint(S=S, dt=0.03, damping=0.05)
+    assert rel_err(p.dipoles_flat(), S, scale=1.5) <= 1e-11
+    # non-convergence: the error of Integrators.jl and the state of the last good step
+    c = make(True, seed=73)
+    before = c.dipoles_flat()
+    with pytest.raises(RuntimeError, match="failed to converge"):
+        sh.step_(c, sh.ImplicitMidpoint(0.05, damping=0.1), nsteps=2, max_iters=2)
+    np.testing.assert_array_equal(c.dipoles_flat(), before)
 
 
 def test_persistent_small_replicas_match_launches(monkeypatch):

@@ -52,6 +52,7 @@ def test_lean_forms_match_general(kind, precision, monkeypatch):
     announce(f"lean vs general tiled kernels {kind} fp{precision}")
     make = (lambda: iso_batch(precision)) if kind == "iso" else (lambda: aniso_system(precision))
     monkeypatch.setenv("SNH_PAIR", "0")
+    monkeypatch.setenv("SNH_COOP", "0")  # keep the one-replica model on the tiled layout
     if precision == 64:
         # small FP64 models default to the ghost layout: request a tile plan
         # (16 pyrochlore sublattices x 64 lanes need two sites per thread to
