@@ -133,23 +133,27 @@ __global__ __launch_bounds__(kFftThreads) void k_ewald_fwd(EwaldGeom G, int R, c
     const int f = blockIdx.x, r = blockIdx.y;
     const int j = f / 3, beta = f % 3;
     const FftLds<T> F = fft_lds<T>(G);
-    make_twiddles(F.tw1, G.L1, -1);
-    make_twiddles(F.tw2, G.L2, -1);
-    make_twiddles(F.tw3, G.L3, -1);
     const uint32_t off = reps[r].off;
     const T g0 = gmat[9 * j + 3 * beta], g1 = gmat[9 * j + 3 * beta + 1], g2 = gmat[9 * j + 3 * beta + 2];
-    constexpr int kBatch = sizeof(T) == 4 ? 8 : 4;  // loads in flight per thread
+    // loads in flight per thread: a 16^3 grid in one round
+    constexpr int kBatch = sizeof(T) == 4 ? 16 : 8;
     for (int c0 = 0; c0 < G.ncell; c0 += kBatch * kFftThreads) {
         T v[kBatch];
+        // unconditional loads (a cell past the end reads cell 0 and is
+        // dropped), so that all of them are in flight before the first wait
 #pragma unroll
         for (int q = 0; q < kBatch; q++) {
             const int c = c0 + q * kFftThreads + threadIdx.x;
-            v[q] = 0;
-            if (c < G.ncell) {
-                const int c1 = c % G.L1, l = c / G.L1;
-                const uint32_t g = ewald_site(G, off, j, c1, l % G.L2, l / G.L2);
-                v[q] = -(g0 * ldg(S, g) + g1 * ldg(S + total, g) + g2 * ldg(S + 2 * (size_t)total, g));
-            }
+            const int cc = c < G.ncell ? c : 0;
+            const int c1 = cc % G.L1, l = cc / G.L1;
+            const uint32_t g = ewald_site(G, off, j, c1, l % G.L2, l / G.L2);
+            const T m = -(g0 * ldg(S, g) + g1 * ldg(S + total, g) + g2 * ldg(S + 2 * (size_t)total, g));
+            v[q] = c < G.ncell ? m : (T)0;
+        }
+        if (c0 == 0) {  // while the first loads are in flight
+            make_twiddles(F.tw1, G.L1, -1);
+            make_twiddles(F.tw2, G.L2, -1);
+            make_twiddles(F.tw3, G.L3, -1);
         }
 #pragma unroll
         for (int q = 0; q < kBatch; q++) {
@@ -336,16 +340,20 @@ __global__ __launch_bounds__(kFftThreads) void k_ewald_inv(EwaldGeom G, int R, c
     const int f = blockIdx.x, r = blockIdx.y;
     const int i = f / 3, alpha = f % 3;
     const FftLds<T> F = fft_lds<T>(G);
-    make_twiddles(F.tw1, G.L1, +1);
-    make_twiddles(F.tw2, G.L2, +1);
-    make_twiddles(F.tw3, G.L3, +1);
-    constexpr int kBatch = sizeof(T) == 4 ? 8 : 4;  // loads in flight per thread
+    // loads in flight per thread: the half spectrum of a 16^3 grid in one round
+    constexpr int kBatch = sizeof(T) == 4 ? 12 : 6;
     for (int k0 = 0; k0 < G.nk; k0 += kBatch * kFftThreads) {
         C2<T> v[kBatch];
+        // unconditional loads (past the end: wavevector 0, dropped below)
 #pragma unroll
         for (int q = 0; q < kBatch; q++) {
             const int k = k0 + q * kFftThreads + threadIdx.x;
-            if (k < G.nk) v[q] = Fphi[spec_index(G, R, r, f, k)];
+            v[q] = Fphi[spec_index(G, R, r, f, k < G.nk ? k : 0)];
+        }
+        if (k0 == 0) {  // while the first loads are in flight
+            make_twiddles(F.tw1, G.L1, +1);
+            make_twiddles(F.tw2, G.L2, +1);
+            make_twiddles(F.tw3, G.L3, +1);
         }
 #pragma unroll
         for (int q = 0; q < kBatch; q++) {
