@@ -211,24 +211,106 @@ __device__ __forceinline__ double mc_bond_term(const Couplings<T>& C, int b, dou
     return e;
 }
 
-// Neighbour indices of the sites of a chunk's trials: nbr[l][k] for bond
-// b0 + k (k < kMcNbr) of trial T0 + l; bonds beyond kMcNbr are found when
-// needed.
+// Canonical index i ncell + c of the site across bond b (wrapped
+// periodically) from cell (c1, c2, c3) of replica R.
+__device__ __forceinline__ int mc_bond_site_id(const McArgs& A, const ReplicaInfo& R, int b, int c1, int c2, int c3)
+{
+    const int* tb = A.bn + 4 * b;
+    const int a1 = mc_wrap(c1 + tb[0], R.L1), a2 = mc_wrap(c2 + tb[1], R.L2), a3 = mc_wrap(c3 + tb[2], R.L3);
+    return tb[3] * R.ncell + a1 + R.L1 * (a2 + R.L2 * a3);
+}
+
+// dmu_l . A(c_m - c_l; i_m, i_l) dmu_m: the Ewald energy between two moved
+// dipoles (Ewald.jl:234-241 with phi = sum_k A mu_k, Ewald.jl:185-209).
+__device__ __forceinline__ double mc_pair_energy(const double* __restrict__ a, const double dl[3], const double dm[3])
+{
+    double e = 0;
+#pragma unroll
+    for (int x = 0; x < 3; x++) e += dl[x] * (a[3 * x] * dm[0] + a[3 * x + 1] * dm[1] + a[3 * x + 2] * dm[2]);
+    return e;
+}
+
+__device__ __forceinline__ const double* mc_pair_block(const double* __restrict__ Aew, int na, const ReplicaInfo& R,
+                                                       int im, int cm1, int cm2, int cm3, int i, int c1, int c2,
+                                                       int c3)
+{
+    const int o1 = mc_wrap_to(cm1 - c1, R.L1, false), o2 = mc_wrap_to(cm2 - c2, R.L2, false),
+              o3 = mc_wrap_to(cm3 - c3, R.L3, false);  // differences of cells in [0, L)
+    const size_t off = (size_t)o1 + (size_t)R.L1 * ((size_t)o2 + (size_t)R.L2 * (size_t)o3);
+    return Aew + (((size_t)im * na + i) * (size_t)R.ncell + off) * 9;
+}
+
+// Neighbour ids kept per trial in LDS; bonds beyond are found when needed.
 constexpr int kMcNbr = 32;
+
+// One sequential trial made by the whole wavefront on the current state:
+// site d, normals n, Ewald potential p at the site; the bond gathers split
+// over the lanes.  Returns dE (local_energy_change + ewald_energy_delta,
+// Interactions.jl:311-380, Ewald.jl:234-241) and sets sn, dS, dmu; every
+// lane gets the same values.
+template <typename T>
+__device__ __forceinline__ double mc_trial_wave(const Couplings<T>& C, const McArgs& A, const T* __restrict__ S,
+                                                const SiteCoord& sc, const McSeqSite& d, const T n[3],
+                                                const double p[3], const double* __restrict__ A0, bool wide,
+                                                V3<T>& sn, double dS[3], double dmu[3])
+{
+    const uint32_t total = C.total;
+    const int i = d.i;
+    const V3<T> s0 = {ldg(S, sc.g), ldg(S, sc.g + total), ldg(S, sc.g + 2 * total)};
+    const int b0 = ldc(C.bstart, i), b1 = ldc(C.bstart, i + 1);
+    const int bl = b0 + (int)threadIdx.x;
+    double jx = 0, jy = 0, jz = 0;
+    if (bl < b1) {
+        const uint32_t gj = mc_neighbour(A, C.G, sc, bl, wide);
+        jx = ldg(S, gj);
+        jy = ldg(S, gj + total);
+        jz = ldg(S, gj + 2 * total);
+    }
+    sn = mc_propose_from<T>(A, d.k, s0, ldc(C.kappa, i), n);
+    const double ox = s0.x, oy = s0.y, oz = s0.z, nx = sn.x, ny = sn.y, nz = sn.z;
+    dS[0] = nx - ox;
+    dS[1] = ny - oy;
+    dS[2] = nz - oz;
+    double Qn[5], Qo[5], dQ[5];
+    quadrupole_d(nx, ny, nz, Qn);
+    quadrupole_d(ox, oy, oz, Qo);
+#pragma unroll
+    for (int q = 0; q < 5; q++) dQ[q] = Qn[q] - Qo[q];
+    double eb = bl < b1 ? mc_bond_term<T>(C, bl, dS[0], dS[1], dS[2], dQ, jx, jy, jz) : 0.0;
+    for (int b = bl + 64; b < b1; b += 64) {  // sublattices with more than 64 bonds
+        const uint32_t gj = mc_neighbour(A, C.G, sc, b, wide);
+        eb += mc_bond_term<T>(C, b, dS[0], dS[1], dS[2], dQ, ldg(S, gj), ldg(S, gj + total),
+                              ldg(S, gj + 2 * total));
+    }
+    double dE = (double)ldc(C.zee, 3 * i) * dS[0] + (double)ldc(C.zee, 3 * i + 1) * dS[1] +
+                (double)ldc(C.zee, 3 * i + 2) * dS[2];
+    dE += stevens_delta_d(C, i, nx, ny, nz, ox, oy, oz);
+    dE += wave_sum(eb);
+    dE += mc_ewald_delta(C, i, p, A0, dS, dmu);
+    return dE;
+}
 
 // Sequential trials T0 .. T0 + ntrial - 1 of the Ewald form, workgroup b <
 // nrep for replica r0 + b, one wavefront; workgroups b >= nrep are the apply
-// part (mc_apply_part).  Lane l first forms the draws of trial T0 + l
-// (mc_seq_draw; normals from counter (T lo, kMcTagSequential + 1, T hi, 0)),
-// the neighbour indices and the potential at its site; then all lanes make
-// the trials in order, each taking its draws from the lane that formed them:
-// proposal, and dE = local_energy_change + ewald_energy_delta
-// (Interactions.jl:311-380, Ewald.jl:234-241) at the potential of the
-// chunk's start plus that of the moves accepted since (McChunkPot); the bond
-// gathers are split over the lanes.  Lane 0 stores an accepted spin with its
-// ghost images.  The chunk's accepted moves go to io.moves[r][.] and
-// io.nmoves[r]; acc[4 r + q] += (dE, dS).
-template <typename T>
+// part (mc_apply_part).  The trials keep the reference's order and outcome
+// (Samplers.jl:133-153) but are evaluated in parallel:
+//  1. lane l forms the draws of trial T0 + l (mc_seq_draw; normals from
+//     counter (T lo, kMcTagSequential + 1, T hi, 0)) and evaluates it on the
+//     chunk's starting state: E_l = local_energy_change + ewald_energy_delta
+//     at the potential p0_l of the chunk's start (Interactions.jl:311-380,
+//     Ewald.jl:234-241), and the pair energies w[m][l] = dmu_l . A(c_m -
+//     c_l) dmu_m with every earlier trial m, in LDS;
+//  2. the trials are then decided in order: trial t's energy change is
+//     E_t + sum of w[m][t] over the moves m accepted before it (the exact
+//     change of ewald_energy_delta, linear in the potential), unless an
+//     accepted move changed its site or a neighbour (a mask of the earlier
+//     trials at those sites, formed from site ids in LDS): then the
+//     whole wavefront makes trial t again on the current state (mc_trial_wave)
+//     and, when it is accepted, w[t][l] is formed again with its new dmu.
+// An accepted spin is stored with its ghost images by the lane that holds
+// it.  The chunk's accepted moves go to io.moves[r][.] and io.nmoves[r];
+// acc[4 r + q] += (dE, dS).
+template <typename T, int G>
 __global__ __launch_bounds__(64) void k_mc_ewald_chunk(Couplings<T> C, McArgs A, T* __restrict__ S, McChunkIO io,
                                                        const double* __restrict__ Aew,
                                                        const double* __restrict__ A0, unsigned long long T0,
@@ -241,82 +323,153 @@ __global__ __launch_bounds__(64) void k_mc_ewald_chunk(Couplings<T> C, McArgs A,
     const int r = r0 + (int)blockIdx.x;
     const ReplicaInfo R = C.reps[r];
     const uint32_t total = C.total;
+    const int na = C.na;
     const unsigned long long key = A.seed + (unsigned long long)r * 0x9E3779B97F4A7C15ull;
     const double kT = A.kT[r];
     const bool wide = A.wide != 0;
     const int lane = (int)threadIdx.x;
-    __shared__ uint32_t nbr[kMcChunk][kMcNbr];
+    __shared__ int nid[kMcChunk][kMcNbr + 1];  // [l][0]: trial l's site; [l][1 + k]: across bond k of it
+    __shared__ double wsh[kMcChunk][kMcChunk];  // w[m][l], m < l
+
+    // ---- 1. every lane evaluates its own trial on the chunk's starting state
     McSeqSite my;
     T my_n[3];
-    McChunkPot P;
+    const unsigned long long Tl = T0 + (unsigned long long)lane;
+    my = mc_seq_site(R, mc_seq_draw(key, Tl, (uint32_t)R.ncell * (uint32_t)na), A);
     {
-        const unsigned long long Tl = T0 + (unsigned long long)lane;
-        my = mc_seq_site(R, mc_seq_draw(key, Tl, (uint32_t)R.ncell * (uint32_t)C.na), A);
         uint32_t cn[4] = {(uint32_t)Tl, kMcTagSequential + 1u, (uint32_t)(Tl >> 32), 0u};
         mc_normals<T>(cn, key, my_n);
-        SiteCoord sc = mc_site(R, C.G, my.i, my.c1, my.c2, my.c3, true);
-        mc_pot_start(P, io, Aew, C.na, R, r, sc.g, total, my.i, my.c1, my.c2, my.c3);
-        const int b0 = C.bstart[my.i], nb = min(C.bstart[my.i + 1] - b0, kMcNbr);
-        for (int k = 0; k < nb; k++) nbr[lane][k] = mc_neighbour(A, C.G, sc, b0 + k, wide);
+    }
+    SiteCoord sc = mc_site(R, C.G, my.i, my.c1, my.c2, my.c3, true);
+    sc.r = r;
+    const int b0 = C.bstart[my.i], my_nb = C.bstart[my.i + 1] - b0;
+    nid[lane][0] = my.i * R.ncell + my.c1 + R.L1 * (my.c2 + R.L2 * my.c3);
+    for (int k = 0; k < min(my_nb, kMcNbr); k++)
+        nid[lane][1 + k] = mc_bond_site_id(A, R, b0 + k, my.c1, my.c2, my.c3);
+    double p0[3];
+    {
+        McChunkPot P;
+        mc_pot_start(P, io, Aew, na, R, r, sc.g, total, my.i, my.c1, my.c2, my.c3);
+        p0[0] = P.pot[0];
+        p0[1] = P.pot[1];
+        p0[2] = P.pot[2];
+    }
+    const V3<T> s0 = {ldg(S, sc.g), ldg(S, sc.g + total), ldg(S, sc.g + 2 * total)};
+    V3<T> my_sn = mc_propose_from<T>(A, my.k, s0, ldc(C.kappa, my.i), my_n);
+    double my_dS[3] = {(double)my_sn.x - (double)s0.x, (double)my_sn.y - (double)s0.y,
+                       (double)my_sn.z - (double)s0.z};
+    double my_dmu[3];
+    double my_E = mc_local_delta<T, G>(C, A, S, sc, s0, my_sn, wide);
+    my_E += mc_ewald_delta(C, my.i, p0, A0, my_dS, my_dmu);
+    // pair energies with the earlier trials, eight blocks in flight
+    for (int m0 = 0; m0 < ntrial; m0 += 8) {
+        double blk[8][9];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int m = min(m0 + q, ntrial - 1);
+            const double* a = mc_pair_block(Aew, na, R, lane_value(my.i, m), lane_value(my.c1, m),
+                                            lane_value(my.c2, m), lane_value(my.c3, m), my.i, my.c1, my.c2, my.c3);
+#pragma unroll
+            for (int x = 0; x < 9; x++) blk[q][x] = a[x];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int m = m0 + q;
+            if (m >= ntrial) break;
+            const double dm[3] = {lane_value(my_dmu[0], m), lane_value(my_dmu[1], m), lane_value(my_dmu[2], m)};
+            if (m < lane && lane < ntrial) wsh[m][lane] = mc_pair_energy(blk[q], my_dmu, dm);
+        }
     }
     __syncthreads();
-    McTrial mv{};
+    // conflicts: the earlier trials m at this lane's site or at one of its
+    // neighbours (trial m's site taken from lane m's register)
+    unsigned long long conf = 0;
+    {
+        const int my_id = nid[lane][0];
+        int idm[kMcChunk];
+#pragma unroll
+        for (int m = 0; m < kMcChunk; m++) idm[m] = lane_value(my_id, m);
+        for (int k = 0; k <= my_nb; k++) {
+            const int id = k <= kMcNbr ? nid[lane][k] : mc_bond_site_id(A, R, b0 + k - 1, my.c1, my.c2, my.c3);
+#pragma unroll
+            for (int m = 0; m < kMcChunk; m++) conf |= (unsigned long long)(idm[m] == id) << m;
+        }
+        conf &= (1ull << lane) - 1ull;  // earlier trials only
+    }
+
+    // ---- 2. the trials decided in order
+    double accw = 0;      // lane l: sum of w[m][l] over the accepted m < l
+    unsigned long long accm = 0;  // accepted trials (wave-uniform)
     int nm = 0;
     double sum[4] = {0, 0, 0, 0};
     for (int t = 0; t < ntrial; t++) {
-        const McSeqSite d = mc_seq_site_of_lane(my, t);
-        const T n[3] = {lane_value(my_n[0], t), lane_value(my_n[1], t), lane_value(my_n[2], t)};
-        const int i = d.i;
-        SiteCoord sc = mc_site(R, C.G, i, d.c1, d.c2, d.c3, true);
-        sc.r = r;
-        const V3<T> s0 = {ldg(S, sc.g), ldg(S, sc.g + total), ldg(S, sc.g + 2 * total)};
-        // the lane's first bond, gathered before anything waits
-        const int b0 = ldc(C.bstart, i), b1 = ldc(C.bstart, i + 1);
-        const int bl = b0 + lane;
-        double jx = 0, jy = 0, jz = 0;
-        if (bl < b1) {
-            const uint32_t gj = lane < kMcNbr ? nbr[t][lane] : mc_neighbour(A, C.G, sc, bl, wide);
-            jx = ldg(S, gj);
-            jy = ldg(S, gj + total);
-            jz = ldg(S, gj + 2 * total);
-        }
-        double p[3];
-        mc_pot_take(P, t, p);
-        const V3<T> sn = mc_propose_from<T>(A, d.k, s0, ldc(C.kappa, i), n);
-        const double ox = s0.x, oy = s0.y, oz = s0.z, nx = sn.x, ny = sn.y, nz = sn.z;
-        const double dS[3] = {nx - ox, ny - oy, nz - oz};
-        double Qn[5], Qo[5], dQ[5];
-        quadrupole_d(nx, ny, nz, Qn);
-        quadrupole_d(ox, oy, oz, Qo);
+        // did an accepted move change trial t's site or a neighbour?
+        const bool redo = (lane_value(conf, t) & accm) != 0;
+        double dE, dS[3];
+        int accept;
+        if (!redo) {
+            dE = lane_value(my_E + accw, t);
+            accept = __builtin_amdgcn_readfirstlane(kT == 0.0 ? dE <= 0.0
+                                                              : lane_value(my.u_acc, t) <= exp(-dE / kT));
+            if (accept && lane == t) mc_store(S, C, sc, my_sn);
+            dS[0] = lane_value(my_dS[0], t);
+            dS[1] = lane_value(my_dS[1], t);
+            dS[2] = lane_value(my_dS[2], t);
+        } else {
+            const McSeqSite d = mc_seq_site_of_lane(my, t);
+            const T n[3] = {lane_value(my_n[0], t), lane_value(my_n[1], t), lane_value(my_n[2], t)};
+            SiteCoord st = mc_site(R, C.G, d.i, d.c1, d.c2, d.c3, true);
+            st.r = r;
+            // potential at the site: p0 + the accepted moves of this chunk
+            double c3v[3] = {0, 0, 0};
+            if (lane < t && ((accm >> lane) & 1ull)) {
+                const double* a = mc_pair_block(Aew, na, R, my.i, my.c1, my.c2, my.c3, d.i, d.c1, d.c2, d.c3);
 #pragma unroll
-        for (int q = 0; q < 5; q++) dQ[q] = Qn[q] - Qo[q];
-        double eb = bl < b1 ? mc_bond_term<T>(C, bl, dS[0], dS[1], dS[2], dQ, jx, jy, jz) : 0.0;
-        for (int b = bl + 64; b < b1; b += 64) {  // sublattices with more than 64 bonds
-            const uint32_t gj = mc_neighbour(A, C.G, sc, b, wide);
-            eb += mc_bond_term<T>(C, b, dS[0], dS[1], dS[2], dQ, ldg(S, gj), ldg(S, gj + total),
-                                  ldg(S, gj + 2 * total));
+                for (int x = 0; x < 3; x++)
+                    c3v[x] = a[3 * x] * my_dmu[0] + a[3 * x + 1] * my_dmu[1] + a[3 * x + 2] * my_dmu[2];
+            }
+            double p[3];
+#pragma unroll
+            for (int x = 0; x < 3; x++) p[x] = lane_value(p0[x], t) + wave_sum(c3v[x]);
+            V3<T> sn;
+            double dmu[3];
+            dE = mc_trial_wave<T>(C, A, S, st, d, n, p, A0, wide, sn, dS, dmu);
+            accept = __builtin_amdgcn_readfirstlane(kT == 0.0 ? dE <= 0.0 : d.u_acc <= exp(-dE / kT));
+            if (accept) {
+                if (lane == 0) mc_store(S, C, st, sn);
+                if (lane == t) {
+                    my_dmu[0] = dmu[0];
+                    my_dmu[1] = dmu[1];
+                    my_dmu[2] = dmu[2];
+                    my_dS[0] = dS[0];
+                    my_dS[1] = dS[1];
+                    my_dS[2] = dS[2];
+                }
+                // the pair energies of this move with the later trials, with its new dmu
+                if (lane > t && lane < ntrial) {
+                    const double* a =
+                        mc_pair_block(Aew, na, R, d.i, d.c1, d.c2, d.c3, my.i, my.c1, my.c2, my.c3);
+                    wsh[t][lane] = mc_pair_energy(a, my_dmu, dmu);
+                }
+            }
         }
-        double dE = (double)ldc(C.zee, 3 * i) * dS[0] + (double)ldc(C.zee, 3 * i + 1) * dS[1] +
-                    (double)ldc(C.zee, 3 * i + 2) * dS[2];
-        dE += stevens_delta_d(C, i, nx, ny, nz, ox, oy, oz);
-        dE += wave_sum(eb);
-        double dmu[3];
-        dE += mc_ewald_delta(C, i, p, A0, dS, dmu);
-        const int accept = __builtin_amdgcn_readfirstlane(kT == 0.0 ? dE <= 0.0 : d.u_acc <= exp(-dE / kT));
         if (accept) {
-            if (lane == 0) mc_store(S, C, sc, sn);
-            if (lane == nm) mv = {1, i, d.c1, d.c2, d.c3, 0, {dmu[0], dmu[1], dmu[2]}};
+            if (lane > t) accw += wsh[t][lane];
+            accm |= 1ull << t;
             nm++;
-            mc_pot_accept(P, Aew, C.na, R, t, ntrial, i, d.c1, d.c2, d.c3, dmu, my.i, my.c1, my.c2, my.c3);
             sum[0] += dE;
             for (int a = 0; a < 3; a++) sum[1 + a] += dS[a];
         }
-        // lane 0's store precedes every lane's loads of the next trial: the
-        // lanes of one wavefront see each other's writes in program order
+        // this trial's stores (LDS and the state) precede the next trial's
+        // loads: the lanes of one wavefront see each other's writes in
+        // program order, so no wait for the stores is needed
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
-    if (lane < nm) io.moves[(size_t)r * kMcChunk + lane] = mv;
+    if (lane < ntrial && ((accm >> lane) & 1ull)) {
+        const int slot = __popcll(accm & ((1ull << lane) - 1ull));
+        io.moves[(size_t)r * kMcChunk + slot] = {1, my.i, my.c1, my.c2, my.c3, 0, {my_dmu[0], my_dmu[1], my_dmu[2]}};
+    }
     if (lane == 0) {
         io.nmoves[r] = nm;
         for (int q = 0; q < 4; q++) acc[4 * r + q] += sum[q];
@@ -369,8 +522,12 @@ void launch_mc_ewald_chunk(const Couplings<T>& C, const McArgs& A, T* S, const M
                            const double* A0, unsigned long long T0, int ntrial, double* acc, int r0, int nr,
                            hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mc_ewald_chunk<T>, dim3(nr * (1 + io.napp)), dim3(64), 0, st, C, A, S, io, Aew, A0, T0,
-                       ntrial, acc, r0, nr);
+    if (C.chunk == 6)
+        hipLaunchKernelGGL((k_mc_ewald_chunk<T, 6>), dim3(nr * (1 + io.napp)), dim3(64), 0, st, C, A, S, io, Aew, A0,
+                           T0, ntrial, acc, r0, nr);
+    else  // bond lists padded to 9 or 18
+        hipLaunchKernelGGL((k_mc_ewald_chunk<T, 9>), dim3(nr * (1 + io.napp)), dim3(64), 0, st, C, A, S, io, Aew, A0,
+                           T0, ntrial, acc, r0, nr);
 }
 
 template <typename T>
