@@ -527,7 +527,7 @@ struct SystemImpl final : snh_system {
         void* Fmu = nullptr;
         void* Fphi = nullptr;
         EwaldLibFft* lib = nullptr;
-        double* Aew = nullptr;  // LocalSampler: the real-space tensor, kept
+        double* Aew = nullptr;  // LocalSampler: the real-space tensor, kept (six doubles per block)
         double* A0 = nullptr;   // LocalSampler: [na][9] self blocks A(0, i, i)
     };
     std::vector<EwaldGroup> ewg;
@@ -1504,21 +1504,37 @@ struct SystemImpl final : snh_system {
         mc.out = own(dalloc<double>(4 * (size_t)R));
         if (ewald) {
             // the real-space tensor (freed after the Fourier transform at
-            // creation) is recomputed and kept: every accepted move updates
-            // the potential with one of its slices
+            // creation) is recomputed and kept as the upper triangles of its
+            // symmetric 3x3 blocks: every accepted move updates the
+            // potential with one of its slices
+            for (const auto& g : ewg) {
+                const double* D = g.P.demag;
+                SNH_REQUIRE(D[1] == D[3] && D[2] == D[6] && D[5] == D[7], SNH_ERR_UNSUPPORTED,
+                            "LocalSampler with the Ewald term needs a symmetric demagnetization tensor");
+            }
             double* d_pos = dupload(ew_pos);
             own(d_pos);
             for (auto& g : ewg) {
-                const size_t nA = (size_t)na * na * (size_t)g.EG.ncell * 9;
-                g.Aew = own(dalloc<double>(nA));
-                launch_ewald_A(g.P, d_pos, g.Aew, st);
-                SNH_HIP(hipGetLastError());
-                SNH_HIP(hipStreamSynchronize(st));
-                std::vector<double> a0(9 * (size_t)na);
-                for (int i = 0; i < na; i++)
-                    SNH_HIP(hipMemcpy(a0.data() + 9 * (size_t)i, g.Aew + ((size_t)(i * na + i) * g.EG.ncell) * 9,
-                                      9 * sizeof(double), hipMemcpyDeviceToHost));
-                g.A0 = own(dupload(a0));
+                const size_t nblk = (size_t)na * na * (size_t)g.EG.ncell;
+                double* A9 = dalloc<double>(nblk * 9);
+                try {
+                    launch_ewald_A(g.P, d_pos, A9, st);
+                    SNH_HIP(hipGetLastError());
+                    SNH_HIP(hipStreamSynchronize(st));
+                    std::vector<double> a0(9 * (size_t)na);
+                    for (int i = 0; i < na; i++)
+                        SNH_HIP(hipMemcpy(a0.data() + 9 * (size_t)i, A9 + ((size_t)(i * na + i) * g.EG.ncell) * 9,
+                                          9 * sizeof(double), hipMemcpyDeviceToHost));
+                    g.A0 = own(dupload(a0));
+                    g.Aew = own(dalloc<double>(nblk * 6));
+                    launch_ewald_A_sym(A9, g.Aew, (long long)nblk, st);
+                    SNH_HIP(hipGetLastError());
+                    SNH_HIP(hipStreamSynchronize(st));
+                } catch (...) {
+                    (void)hipFree(A9);
+                    throw;
+                }
+                SNH_HIP(hipFree(A9));
             }
             for (int b = 0; b < 2; b++) {
                 mc.phi[b] = own(dalloc<double>(3 * (size_t)total));
@@ -1596,7 +1612,8 @@ struct SystemImpl final : snh_system {
                     io.nprev = k ? mc.nmoves[(k - 1) & 1] : nullptr;
                     io.moves = mc.moves[k & 1];
                     io.nmoves = mc.nmoves[k & 1];
-                    io.napp = k ? (int)std::min<uint32_t>((ns + 63) / 64, kMcApplyGroups) : 0;
+                    const uint32_t th = N == 0 ? kMcThreads : 64;  // threads per workgroup of the launch
+                    io.napp = k ? (int)std::min<uint32_t>((ns + th - 1) / th, kMcApplyGroups) : 0;
                     if (N == 0)
                         launch_mc_ewald_chunk<T>(C, A, X, io, g.Aew, g.A0, T0, nt, mc.acc, g.r0, g.nr, st);
                     else

@@ -272,6 +272,7 @@ struct McTrial {
 // launch, one wavefront per replica whose lane m holds the chunk's m-th
 // accepted move.
 constexpr int kMcChunk = 64;
+constexpr int kMcThreads = 256;  // per workgroup of the :dipole launch (trial workgroups: four wavefronts)
 static_assert(kMcChunk <= 64, "one accepted move per lane of a wavefront");
 // Buffers of one launch of sequential trials with the Ewald term.  Its
 // trial workgroups see the potential phi_in of the chunk's start without the
@@ -291,6 +292,10 @@ struct McChunkIO {
 };
 constexpr int kMcApplyGroups = 128;  // napp at most
 
+// The real-space tensor's nblk 3x3 blocks A9 (nine doubles each) as the six
+// upper-triangle entries xx, xy, xz, yy, yz, zz in A6 (the blocks are
+// symmetric); the form the sequential trials read.
+void launch_ewald_A_sym(const double* A9, double* A6, long long nblk, hipStream_t st);
 // Trials T0 .. T0 + ntrial - 1 (ntrial <= kMcChunk) of replicas r0 .. r0 +
 // nr - 1, which share their dims, with the buffers io; Aew
 // is the real-space tensor and A0 its [na][9] self blocks A(0, i, i);

@@ -227,9 +227,35 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Block A(c_m - c; i_m, i) of the real-space tensor kept for sequential
+// trials: [i_m][i][cell offset] blocks of six doubles xx, xy, xz, yy, yz, zz
+// (the 3x3 blocks are symmetric), 48 bytes each, read as three 16-byte
+// loads.  Cells c_m and c of replica R both lie in [0, L).
+struct ABlock {
+    double xx, xy, xz, yy, yz, zz;
+};
+
+__device__ __forceinline__ ABlock mc_ablock(const double* __restrict__ Aew, int na, const ReplicaInfo& R, int im,
+                                            int cm1, int cm2, int cm3, int i, int c1, int c2, int c3)
+{
+    const int o1 = mc_wrap_to(cm1 - c1, R.L1, false), o2 = mc_wrap_to(cm2 - c2, R.L2, false),
+              o3 = mc_wrap_to(cm3 - c3, R.L3, false);
+    const size_t off = (size_t)o1 + (size_t)R.L1 * ((size_t)o2 + (size_t)R.L2 * (size_t)o3);
+    const double2* p = reinterpret_cast<const double2*>(Aew + (((size_t)im * na + i) * (size_t)R.ncell + off) * 6);
+    const double2 a = p[0], b = p[1], c = p[2];
+    return {a.x, a.y, b.x, b.y, c.x, c.y};
+}
+
+// out = A v for a symmetric block.
+__device__ __forceinline__ void ablock_apply(const ABlock& A, const double v[3], double out[3])
+{
+    out[0] = A.xx * v[0] + A.xy * v[1] + A.xz * v[2];
+    out[1] = A.xy * v[0] + A.yy * v[1] + A.yz * v[2];
+    out[2] = A.xz * v[0] + A.yz * v[1] + A.zz * v[2];
+}
+
 // s = sum over the moves mv[0..nm) of A(c_m - c; i_m, i) dmu_m at sublattice-i
 // site (c1, c2, c3) of replica R (phi = sum_k A mu_k, Ewald.jl:185-209).
-// Aew: real-space tensor [i_m][i][cell offset][3][3] of R's dims, FP64.
 __device__ __forceinline__ void mc_moves_sum(const double* __restrict__ Aew, int na, const ReplicaInfo& R,
                                              const McTrial* __restrict__ mv, int nm, int i, int c1, int c2, int c3,
                                              double s[3])
@@ -238,13 +264,11 @@ __device__ __forceinline__ void mc_moves_sum(const double* __restrict__ Aew, int
 #pragma unroll 8
     for (int m = 0; m < nm; m++) {
         const McTrial t = mv[m];
-        const int o1 = mc_wrap_to(t.c1 - c1, R.L1, false), o2 = mc_wrap_to(t.c2 - c2, R.L2, false),
-                  o3 = mc_wrap_to(t.c3 - c3, R.L3, false);  // differences of cells in [0, L)
-        const size_t off = (size_t)o1 + (size_t)R.L1 * ((size_t)o2 + (size_t)R.L2 * (size_t)o3);
-        const double* a = Aew + (((size_t)t.i * na + i) * (size_t)R.ncell + off) * 9;
-        a0 += a[0] * t.dmu[0] + a[1] * t.dmu[1] + a[2] * t.dmu[2];
-        a1 += a[3] * t.dmu[0] + a[4] * t.dmu[1] + a[5] * t.dmu[2];
-        a2 += a[6] * t.dmu[0] + a[7] * t.dmu[1] + a[8] * t.dmu[2];
+        double o[3];
+        ablock_apply(mc_ablock(Aew, na, R, t.i, t.c1, t.c2, t.c3, i, c1, c2, c3), t.dmu, o);
+        a0 += o[0];
+        a1 += o[1];
+        a2 += o[2];
     }
     s[0] = a0;
     s[1] = a1;
@@ -252,8 +276,8 @@ __device__ __forceinline__ void mc_moves_sum(const double* __restrict__ Aew, int
 }
 
 // Apply workgroup b of a launch (b < nr napp): phi_out = phi_in + the
-// previous chunk's moves at sites j = 64 (b mod napp) + lane + 64 napp k of
-// replica r0 + b / napp.  napp is kept small, so the apply streams the
+// previous chunk's moves at sites j = W (b mod napp) + thread + W napp k
+// (W threads per workgroup) of replica r0 + b / napp.  napp is kept small, so the apply streams the
 // tensor over the chunk's whole duration rather than competing with the
 // serial trials for memory at its start.
 __device__ __forceinline__ void mc_apply_part(const McChunkIO& io, const double* __restrict__ Aew,
@@ -265,8 +289,8 @@ __device__ __forceinline__ void mc_apply_part(const McChunkIO& io, const double*
     const uint32_t ncell = (uint32_t)R.ncell;
     const McTrial* mv = io.prev + (size_t)r * kMcChunk;
     const int nm = io.nprev[r];
-    for (uint32_t j = (uint32_t)(b % io.napp) * 64u + threadIdx.x; j < ncell * (uint32_t)na;
-         j += 64u * (uint32_t)io.napp) {
+    for (uint32_t j = (uint32_t)(b % io.napp) * blockDim.x + threadIdx.x; j < ncell * (uint32_t)na;
+         j += blockDim.x * (uint32_t)io.napp) {
         const int i = (int)(j / ncell);
         const int c = (int)(j - (uint32_t)i * ncell);
         const int c1 = c % R.L1, c2 = (c / R.L1) % R.L2, c3 = c / (R.L1 * R.L2);
@@ -288,7 +312,7 @@ __device__ __forceinline__ void mc_apply_part(const McChunkIO& io, const double*
 // loads.
 struct McChunkPot {
     double pot[3];
-    double a[9];    // block of the pending move at the lane's site (zero: none)
+    ABlock a;       // block of the pending move at the lane's site (zero: none)
     double dmu[3];  // the pending move's dmu (wave-uniform)
     bool pending;   // wave-uniform
 };
@@ -311,17 +335,10 @@ __device__ __forceinline__ void mc_pot_accept(McChunkPot& P, const double* __res
                                               int cm3, const double dmu[3], int i, int c1, int c2, int c3)
 {
     const int l = (int)threadIdx.x;
-    if (l > t && l < ntrial) {
-        const int o1 = mc_wrap_to(cm1 - c1, R.L1, false), o2 = mc_wrap_to(cm2 - c2, R.L2, false),
-                  o3 = mc_wrap_to(cm3 - c3, R.L3, false);  // differences of cells in [0, L)
-        const size_t off = (size_t)o1 + (size_t)R.L1 * ((size_t)o2 + (size_t)R.L2 * (size_t)o3);
-        const double* a = Aew + (((size_t)im * na + i) * (size_t)R.ncell + off) * 9;
-#pragma unroll
-        for (int q = 0; q < 9; q++) P.a[q] = a[q];
-    } else {
-#pragma unroll
-        for (int q = 0; q < 9; q++) P.a[q] = 0;
-    }
+    if (l > t && l < ntrial)
+        P.a = mc_ablock(Aew, na, R, im, cm1, cm2, cm3, i, c1, c2, c3);
+    else
+        P.a = {0, 0, 0, 0, 0, 0};
     P.dmu[0] = dmu[0];
     P.dmu[1] = dmu[1];
     P.dmu[2] = dmu[2];
@@ -333,9 +350,10 @@ __device__ __forceinline__ void mc_pot_accept(McChunkPot& P, const double* __res
 __device__ __forceinline__ void mc_pot_take(McChunkPot& P, int t, double p[3])
 {
     if (P.pending) {
+        double o[3];
+        ablock_apply(P.a, P.dmu, o);
 #pragma unroll
-        for (int q = 0; q < 3; q++)
-            P.pot[q] += P.a[3 * q] * P.dmu[0] + P.a[3 * q + 1] * P.dmu[1] + P.a[3 * q + 2] * P.dmu[2];
+        for (int q = 0; q < 3; q++) P.pot[q] += o[q];
         P.pending = false;
     }
     p[0] = lane_value(P.pot[0], t);

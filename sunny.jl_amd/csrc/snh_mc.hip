@@ -220,24 +220,14 @@ __device__ __forceinline__ int mc_bond_site_id(const McArgs& A, const ReplicaInf
     return tb[3] * R.ncell + a1 + R.L1 * (a2 + R.L2 * a3);
 }
 
-// dmu_l . A(c_m - c_l; i_m, i_l) dmu_m: the Ewald energy between two moved
-// dipoles (Ewald.jl:234-241 with phi = sum_k A mu_k, Ewald.jl:185-209).
-__device__ __forceinline__ double mc_pair_energy(const double* __restrict__ a, const double dl[3], const double dm[3])
+// dmu_l . A dmu_m for the block A = A(c_m - c_l; i_m, i_l): the Ewald energy
+// between two moved dipoles (Ewald.jl:234-241 with phi = sum_k A mu_k,
+// Ewald.jl:185-209).
+__device__ __forceinline__ double mc_pair_energy(const ABlock& a, const double dl[3], const double dm[3])
 {
-    double e = 0;
-#pragma unroll
-    for (int x = 0; x < 3; x++) e += dl[x] * (a[3 * x] * dm[0] + a[3 * x + 1] * dm[1] + a[3 * x + 2] * dm[2]);
-    return e;
-}
-
-__device__ __forceinline__ const double* mc_pair_block(const double* __restrict__ Aew, int na, const ReplicaInfo& R,
-                                                       int im, int cm1, int cm2, int cm3, int i, int c1, int c2,
-                                                       int c3)
-{
-    const int o1 = mc_wrap_to(cm1 - c1, R.L1, false), o2 = mc_wrap_to(cm2 - c2, R.L2, false),
-              o3 = mc_wrap_to(cm3 - c3, R.L3, false);  // differences of cells in [0, L)
-    const size_t off = (size_t)o1 + (size_t)R.L1 * ((size_t)o2 + (size_t)R.L2 * (size_t)o3);
-    return Aew + (((size_t)im * na + i) * (size_t)R.ncell + off) * 9;
+    double o[3];
+    ablock_apply(a, dm, o);
+    return dl[0] * o[0] + dl[1] * o[1] + dl[2] * o[2];
 }
 
 // Neighbour ids kept per trial in LDS; bonds beyond are found when needed.
@@ -294,12 +284,14 @@ __device__ __forceinline__ double mc_trial_wave(const Couplings<T>& C, const McA
 // nrep for replica r0 + b, one wavefront; workgroups b >= nrep are the apply
 // part (mc_apply_part).  The trials keep the reference's order and outcome
 // (Samplers.jl:133-153) but are evaluated in parallel:
-//  1. lane l forms the draws of trial T0 + l (mc_seq_draw; normals from
-//     counter (T lo, kMcTagSequential + 1, T hi, 0)) and evaluates it on the
-//     chunk's starting state: E_l = local_energy_change + ewald_energy_delta
-//     at the potential p0_l of the chunk's start (Interactions.jl:311-380,
-//     Ewald.jl:234-241), and the pair energies w[m][l] = dmu_l . A(c_m -
-//     c_l) dmu_m with every earlier trial m, in LDS;
+//  1. lane l of the first wavefront forms the draws of trial T0 + l
+//     (mc_seq_draw; normals from counter (T lo, kMcTagSequential + 1, T hi,
+//     0)) and evaluates it on the chunk's starting state: E_l =
+//     local_energy_change + ewald_energy_delta at the potential p0_l of the
+//     chunk's start (Interactions.jl:311-380, Ewald.jl:234-241); all
+//     wavefronts share the tensor-block loads of p0 (the previous chunk's
+//     moves at the trial sites) and of the pair energies w[m][l] = dmu_l .
+//     A(c_m - c_l) dmu_m with every earlier trial m, kept in LDS;
 //  2. the trials are then decided in order: trial t's energy change is
 //     E_t + sum of w[m][t] over the moves m accepted before it (the exact
 //     change of ewald_energy_delta, linear in the potential), unless an
@@ -308,13 +300,14 @@ __device__ __forceinline__ double mc_trial_wave(const Couplings<T>& C, const McA
 //     whole wavefront makes trial t again on the current state (mc_trial_wave)
 //     and, when it is accepted, w[t][l] is formed again with its new dmu.
 // An accepted spin is stored with its ghost images by the lane that holds
-// it.  The chunk's accepted moves go to io.moves[r][.] and io.nmoves[r];
-// acc[4 r + q] += (dE, dS).
+// it; the decisions are the first wavefront's alone.  The chunk's accepted
+// moves go to io.moves[r][.] and io.nmoves[r]; acc[4 r + q] += (dE, dS).
 template <typename T, int G>
-__global__ __launch_bounds__(64) void k_mc_ewald_chunk(Couplings<T> C, McArgs A, T* __restrict__ S, McChunkIO io,
-                                                       const double* __restrict__ Aew,
-                                                       const double* __restrict__ A0, unsigned long long T0,
-                                                       int ntrial, double* __restrict__ acc, int r0, int nrep)
+__global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, McArgs A, T* __restrict__ S,
+                                                               McChunkIO io, const double* __restrict__ Aew,
+                                                               const double* __restrict__ A0,
+                                                               unsigned long long T0, int ntrial,
+                                                               double* __restrict__ acc, int r0, int nrep)
 {
     if ((int)blockIdx.x >= nrep) {  // uniform over the workgroup
         mc_apply_part(io, Aew, C.reps, C.G, C.na, C.total, r0, (int)blockIdx.x - nrep);
@@ -327,60 +320,103 @@ __global__ __launch_bounds__(64) void k_mc_ewald_chunk(Couplings<T> C, McArgs A,
     const unsigned long long key = A.seed + (unsigned long long)r * 0x9E3779B97F4A7C15ull;
     const double kT = A.kT[r];
     const bool wide = A.wide != 0;
-    const int lane = (int)threadIdx.x;
+    const int lane = (int)threadIdx.x % 64, wv = (int)threadIdx.x / 64;
+    constexpr int NW = kMcThreads / 64;
     __shared__ int nid[kMcChunk][kMcNbr + 1];  // [l][0]: trial l's site; [l][1 + k]: across bond k of it
     __shared__ double wsh[kMcChunk][kMcChunk];  // w[m][l], m < l
+    __shared__ McTrial rec[kMcChunk];           // trial l's site and dmu
+    __shared__ double ppart[NW][kMcChunk][3];   // previous chunk's moves at trial l's site, per wave
 
-    // ---- 1. every lane evaluates its own trial on the chunk's starting state
-    McSeqSite my;
-    T my_n[3];
-    const unsigned long long Tl = T0 + (unsigned long long)lane;
-    my = mc_seq_site(R, mc_seq_draw(key, Tl, (uint32_t)R.ncell * (uint32_t)na), A);
-    {
+    // ---- 1. every lane of wave 0 evaluates its own trial on the chunk's
+    // starting state; all waves then share the loads of the tensor blocks
+    McSeqSite my{};
+    T my_n[3] = {0, 0, 0};
+    SiteCoord sc{};
+    int b0 = 0, my_nb = 0;
+    V3<T> my_sn{};
+    double my_dS[3] = {0, 0, 0}, my_dmu[3] = {0, 0, 0}, my_E = 0;
+    if (wv == 0) {
+        const unsigned long long Tl = T0 + (unsigned long long)lane;
+        my = mc_seq_site(R, mc_seq_draw(key, Tl, (uint32_t)R.ncell * (uint32_t)na), A);
         uint32_t cn[4] = {(uint32_t)Tl, kMcTagSequential + 1u, (uint32_t)(Tl >> 32), 0u};
         mc_normals<T>(cn, key, my_n);
+        sc = mc_site(R, C.G, my.i, my.c1, my.c2, my.c3, true);
+        sc.r = r;
+        b0 = C.bstart[my.i];
+        my_nb = C.bstart[my.i + 1] - b0;
+        nid[lane][0] = my.i * R.ncell + my.c1 + R.L1 * (my.c2 + R.L2 * my.c3);
+        for (int k = 0; k < min(my_nb, kMcNbr); k++)
+            nid[lane][1 + k] = mc_bond_site_id(A, R, b0 + k, my.c1, my.c2, my.c3);
+        const V3<T> s0 = {ldg(S, sc.g), ldg(S, sc.g + total), ldg(S, sc.g + 2 * total)};
+        my_sn = mc_propose_from<T>(A, my.k, s0, ldc(C.kappa, my.i), my_n);
+        my_dS[0] = (double)my_sn.x - (double)s0.x;
+        my_dS[1] = (double)my_sn.y - (double)s0.y;
+        my_dS[2] = (double)my_sn.z - (double)s0.z;
+        const T* gm = C.gmat + 9 * my.i;  // dmu = -g dS, as mc_ewald_delta forms it
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+            my_dmu[a] = -((double)gm[3 * a] * my_dS[0] + (double)gm[3 * a + 1] * my_dS[1] +
+                          (double)gm[3 * a + 2] * my_dS[2]);
+        rec[lane] = {1, my.i, my.c1, my.c2, my.c3, 0, {my_dmu[0], my_dmu[1], my_dmu[2]}};
+        my_E = mc_local_delta<T, G>(C, A, S, sc, s0, my_sn, wide);
     }
-    SiteCoord sc = mc_site(R, C.G, my.i, my.c1, my.c2, my.c3, true);
-    sc.r = r;
-    const int b0 = C.bstart[my.i], my_nb = C.bstart[my.i + 1] - b0;
-    nid[lane][0] = my.i * R.ncell + my.c1 + R.L1 * (my.c2 + R.L2 * my.c3);
-    for (int k = 0; k < min(my_nb, kMcNbr); k++)
-        nid[lane][1 + k] = mc_bond_site_id(A, R, b0 + k, my.c1, my.c2, my.c3);
-    double p0[3];
+    __syncthreads();
     {
-        McChunkPot P;
-        mc_pot_start(P, io, Aew, na, R, r, sc.g, total, my.i, my.c1, my.c2, my.c3);
-        p0[0] = P.pot[0];
-        p0[1] = P.pot[1];
-        p0[2] = P.pot[2];
-    }
-    const V3<T> s0 = {ldg(S, sc.g), ldg(S, sc.g + total), ldg(S, sc.g + 2 * total)};
-    V3<T> my_sn = mc_propose_from<T>(A, my.k, s0, ldc(C.kappa, my.i), my_n);
-    double my_dS[3] = {(double)my_sn.x - (double)s0.x, (double)my_sn.y - (double)s0.y,
-                       (double)my_sn.z - (double)s0.z};
-    double my_dmu[3];
-    double my_E = mc_local_delta<T, G>(C, A, S, sc, s0, my_sn, wide);
-    my_E += mc_ewald_delta(C, my.i, p0, A0, my_dS, my_dmu);
-    // pair energies with the earlier trials, eight blocks in flight
-    for (int m0 = 0; m0 < ntrial; m0 += 8) {
-        double blk[8][9];
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int m = min(m0 + q, ntrial - 1);
-            const double* a = mc_pair_block(Aew, na, R, lane_value(my.i, m), lane_value(my.c1, m),
-                                            lane_value(my.c2, m), lane_value(my.c3, m), my.i, my.c1, my.c2, my.c3);
-#pragma unroll
-            for (int x = 0; x < 9; x++) blk[q][x] = a[x];
+        const McTrial me = rec[lane];
+        // the previous chunk's moves m = wv, wv + NW, ... at this lane's trial site
+        double s3[3] = {0, 0, 0};
+        if (io.napp) {
+            const McTrial* pv = io.prev + (size_t)r * kMcChunk;
+            const int np = io.nprev[r];
+#pragma unroll 4
+            for (int m = wv; m < np; m += NW) {
+                const McTrial t = pv[m];
+                double o[3];
+                ablock_apply(mc_ablock(Aew, na, R, t.i, t.c1, t.c2, t.c3, me.i, me.c1, me.c2, me.c3), t.dmu, o);
+                s3[0] += o[0];
+                s3[1] += o[1];
+                s3[2] += o[2];
+            }
         }
+        ppart[wv][lane][0] = s3[0];
+        ppart[wv][lane][1] = s3[1];
+        ppart[wv][lane][2] = s3[2];
+        // pair energies w[m][l] with the earlier trials m = wv, wv + NW, ...
+        for (int m0 = wv; m0 < ntrial; m0 += 4 * NW) {
+            ABlock blk[4];
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int m = m0 + q;
-            if (m >= ntrial) break;
-            const double dm[3] = {lane_value(my_dmu[0], m), lane_value(my_dmu[1], m), lane_value(my_dmu[2], m)};
-            if (m < lane && lane < ntrial) wsh[m][lane] = mc_pair_energy(blk[q], my_dmu, dm);
+            for (int q = 0; q < 4; q++) {
+                const McTrial t = rec[min(m0 + q * NW, ntrial - 1)];
+                blk[q] = mc_ablock(Aew, na, R, t.i, t.c1, t.c2, t.c3, me.i, me.c1, me.c2, me.c3);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int m = m0 + q * NW;
+                if (m < lane && lane < ntrial) wsh[m][lane] = mc_pair_energy(blk[q], me.dmu, rec[m].dmu);
+            }
         }
     }
     __syncthreads();
+    if (wv != 0) return;  // the rest is wave 0's (no workgroup barrier follows)
+    // the potential of the chunk's start at the lane's trial site
+    double p0[3];
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+        double t = 0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) t += ppart[w][lane][x];
+        p0[x] = io.phi_in[sc.g + x * (size_t)total] + t;
+    }
+    my_E += mc_ewald_delta(C, my.i, p0, A0, my_dS, my_dmu);
+    // the Metropolis test u <= exp(-dE / kT) as dE <= -kT log u, decided by
+    // the lane itself; within a relative band of 1e-9 around the threshold
+    // (rounding of the logarithm) the test itself is made
+    const double thr = kT == 0.0 ? 0.0 : -kT * log(my.u_acc);
+    auto metropolis = [&](double dE) __attribute__((always_inline)) {
+        const double margin = dE - thr;
+        if (fabs(margin) > 1e-9 * (fabs(dE) + fabs(thr)) + 1e-300) return margin < 0.0;
+        return kT == 0.0 ? dE <= 0.0 : my.u_acc <= exp(-dE / kT);
+    };
     // conflicts: the earlier trials m at this lane's site or at one of its
     // neighbours (trial m's site taken from lane m's register)
     unsigned long long conf = 0;
@@ -398,23 +434,22 @@ __global__ __launch_bounds__(64) void k_mc_ewald_chunk(Couplings<T> C, McArgs A,
     }
 
     // ---- 2. the trials decided in order
-    double accw = 0;      // lane l: sum of w[m][l] over the accepted m < l
+    double accw = 0;              // lane l: sum of w[m][l] over the accepted m < l
+    double my_dE = 0;             // lane l: its trial's energy change when decided
     unsigned long long accm = 0;  // accepted trials (wave-uniform)
     int nm = 0;
-    double sum[4] = {0, 0, 0, 0};
     for (int t = 0; t < ntrial; t++) {
+        const double wt = lane > t ? wsh[t][lane] : 0.0;  // read ahead of the decision
         // did an accepted move change trial t's site or a neighbour?
         const bool redo = (lane_value(conf, t) & accm) != 0;
-        double dE, dS[3];
         int accept;
         if (!redo) {
-            dE = lane_value(my_E + accw, t);
-            accept = __builtin_amdgcn_readfirstlane(kT == 0.0 ? dE <= 0.0
-                                                              : lane_value(my.u_acc, t) <= exp(-dE / kT));
-            if (accept && lane == t) mc_store(S, C, sc, my_sn);
-            dS[0] = lane_value(my_dS[0], t);
-            dS[1] = lane_value(my_dS[1], t);
-            dS[2] = lane_value(my_dS[2], t);
+            const double e = my_E + accw;
+            accept = lane_value((int)metropolis(e), t);
+            if (lane == t) {
+                my_dE = e;
+                if (accept) mc_store(S, C, sc, my_sn);
+            }
         } else {
             const McSeqSite d = mc_seq_site_of_lane(my, t);
             const T n[3] = {lane_value(my_n[0], t), lane_value(my_n[1], t), lane_value(my_n[2], t)};
@@ -422,43 +457,41 @@ __global__ __launch_bounds__(64) void k_mc_ewald_chunk(Couplings<T> C, McArgs A,
             st.r = r;
             // potential at the site: p0 + the accepted moves of this chunk
             double c3v[3] = {0, 0, 0};
-            if (lane < t && ((accm >> lane) & 1ull)) {
-                const double* a = mc_pair_block(Aew, na, R, my.i, my.c1, my.c2, my.c3, d.i, d.c1, d.c2, d.c3);
-#pragma unroll
-                for (int x = 0; x < 3; x++)
-                    c3v[x] = a[3 * x] * my_dmu[0] + a[3 * x + 1] * my_dmu[1] + a[3 * x + 2] * my_dmu[2];
-            }
+            if (lane < t && ((accm >> lane) & 1ull))
+                ablock_apply(mc_ablock(Aew, na, R, my.i, my.c1, my.c2, my.c3, d.i, d.c1, d.c2, d.c3), my_dmu, c3v);
             double p[3];
 #pragma unroll
             for (int x = 0; x < 3; x++) p[x] = lane_value(p0[x], t) + wave_sum(c3v[x]);
             V3<T> sn;
-            double dmu[3];
-            dE = mc_trial_wave<T>(C, A, S, st, d, n, p, A0, wide, sn, dS, dmu);
+            double dmu[3], dS[3];
+            const double dE = mc_trial_wave<T>(C, A, S, st, d, n, p, A0, wide, sn, dS, dmu);
             accept = __builtin_amdgcn_readfirstlane(kT == 0.0 ? dE <= 0.0 : d.u_acc <= exp(-dE / kT));
+            if (lane == t) {
+                my_dE = dE;
+                my_dS[0] = dS[0];
+                my_dS[1] = dS[1];
+                my_dS[2] = dS[2];
+            }
             if (accept) {
                 if (lane == 0) mc_store(S, C, st, sn);
                 if (lane == t) {
                     my_dmu[0] = dmu[0];
                     my_dmu[1] = dmu[1];
                     my_dmu[2] = dmu[2];
-                    my_dS[0] = dS[0];
-                    my_dS[1] = dS[1];
-                    my_dS[2] = dS[2];
                 }
                 // the pair energies of this move with the later trials, with its new dmu
                 if (lane > t && lane < ntrial) {
-                    const double* a =
-                        mc_pair_block(Aew, na, R, d.i, d.c1, d.c2, d.c3, my.i, my.c1, my.c2, my.c3);
-                    wsh[t][lane] = mc_pair_energy(a, my_dmu, dmu);
+                    wsh[t][lane] = mc_pair_energy(mc_ablock(Aew, na, R, d.i, d.c1, d.c2, d.c3, my.i, my.c1, my.c2,
+                                                            my.c3),
+                                                  my_dmu, dmu);
+                    accw += wsh[t][lane];
                 }
             }
         }
         if (accept) {
-            if (lane > t) accw += wsh[t][lane];
+            if (!redo) accw += wt;
             accm |= 1ull << t;
             nm++;
-            sum[0] += dE;
-            for (int a = 0; a < 3; a++) sum[1 + a] += dS[a];
         }
         // this trial's stores (LDS and the state) precede the next trial's
         // loads: the lanes of one wavefront see each other's writes in
@@ -466,6 +499,11 @@ __global__ __launch_bounds__(64) void k_mc_ewald_chunk(Couplings<T> C, McArgs A,
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
+    // sums of dE and dS over the accepted trials
+    const bool mine = lane < ntrial && ((accm >> lane) & 1ull);
+    double sum[4];
+    sum[0] = wave_sum(mine ? my_dE : 0.0);
+    for (int a = 0; a < 3; a++) sum[1 + a] = wave_sum(mine ? my_dS[a] : 0.0);
     if (lane < ntrial && ((accm >> lane) & 1ull)) {
         const int slot = __popcll(accm & ((1ull << lane) - 1ull));
         io.moves[(size_t)r * kMcChunk + slot] = {1, my.i, my.c1, my.c2, my.c3, 0, {my_dmu[0], my_dmu[1], my_dmu[2]}};
@@ -474,6 +512,19 @@ __global__ __launch_bounds__(64) void k_mc_ewald_chunk(Couplings<T> C, McArgs A,
         io.nmoves[r] = nm;
         for (int q = 0; q < 4; q++) acc[4 * r + q] += sum[q];
     }
+}
+
+// The real-space tensor's 3x3 blocks (row-major, nine doubles) as the six
+// entries xx, xy, xz, yy, yz, zz of their upper triangle.
+__global__ void k_ewald_A_sym(const double* __restrict__ A9, double* __restrict__ A6, long long nblk)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nblk) return;
+    const double* a = A9 + 9 * k;
+    double2* o = reinterpret_cast<double2*>(A6 + 6 * k);
+    o[0] = {a[0], a[1]};
+    o[1] = {a[2], a[4]};
+    o[2] = {a[5], a[8]};
 }
 
 template <typename T>
@@ -517,17 +568,22 @@ void launch_mc_color(const Couplings<T>& C, const McArgs& A, T* S, const McWork*
         hipLaunchKernelGGL((k_mc_color<T, 9>), dim3(nwork), dim3(kBlock), 0, st, C, A, S, work, P, prob, acc);
 }
 
+void launch_ewald_A_sym(const double* A9, double* A6, long long nblk, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ewald_A_sym, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, st, A9, A6, nblk);
+}
+
 template <typename T>
 void launch_mc_ewald_chunk(const Couplings<T>& C, const McArgs& A, T* S, const McChunkIO& io, const double* Aew,
                            const double* A0, unsigned long long T0, int ntrial, double* acc, int r0, int nr,
                            hipStream_t st)
 {
     if (C.chunk == 6)
-        hipLaunchKernelGGL((k_mc_ewald_chunk<T, 6>), dim3(nr * (1 + io.napp)), dim3(64), 0, st, C, A, S, io, Aew, A0,
-                           T0, ntrial, acc, r0, nr);
+        hipLaunchKernelGGL((k_mc_ewald_chunk<T, 6>), dim3(nr * (1 + io.napp)), dim3(kMcThreads), 0, st, C, A, S, io,
+                           Aew, A0, T0, ntrial, acc, r0, nr);
     else  // bond lists padded to 9 or 18
-        hipLaunchKernelGGL((k_mc_ewald_chunk<T, 9>), dim3(nr * (1 + io.napp)), dim3(64), 0, st, C, A, S, io, Aew, A0,
-                           T0, ntrial, acc, r0, nr);
+        hipLaunchKernelGGL((k_mc_ewald_chunk<T, 9>), dim3(nr * (1 + io.napp)), dim3(kMcThreads), 0, st, C, A, S, io,
+                           Aew, A0, T0, ntrial, acc, r0, nr);
 }
 
 template <typename T>
