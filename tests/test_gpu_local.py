@@ -154,7 +154,10 @@ def test_ewald_sequential_trials_match_oracle(precision):
              ("pyro211", sh.workloads.pyrochlore_model(dims=(2, 1, 1), seed=24, precision=precision, dipolar=True)),
              # an axis of length 3: offsets differ from their negatives, so the
              # atom order and offset sign of the potential update are pinned
-             ("pyro321", sh.workloads.pyrochlore_model(dims=(3, 2, 1), seed=25, precision=precision, dipolar=True))]
+             ("pyro321", sh.workloads.pyrochlore_model(dims=(3, 2, 1), seed=25, precision=precision, dipolar=True)),
+             # 288 sites, 576 trials in nine chunks: most trials take the
+             # parallel evaluation's exact corrections, a few are made again
+             ("pyro332", sh.workloads.pyrochlore_model(dims=(3, 3, 2), seed=27, precision=precision, dipolar=True))]
     for name, s in cases:
         kap = np.repeat(s.kappas, s.ncells)
         for pname in (["uniform", "mix"] if precision == 64 else ["flip"]):
@@ -249,6 +252,11 @@ def test_unsupported_and_invalid():
         sh.step_(s, bad)
     with pytest.raises(ValueError, match="kT"):
         sh.step_(s, sh.LocalSampler(kT=-1.0))
+    # the sequential trials keep the Ewald tensor's blocks as upper triangles
+    d = sh.workloads.pyrochlore_model(L=1, seed=5, precision=64, dipolar=True)
+    sh.enable_dipole_dipole_(d, 0.6745817653324668, np.array([[0.4, 0.1, 0.0], [0.0, 0.3, 0.0], [0.0, 0.0, 0.3]]))
+    with pytest.raises(Exception, match="symmetric demagnetization"):
+        sh.step_(d, sh.LocalSampler(kT=1.0))
 
 
 @pytest.mark.parametrize("precision", [32, 64])
