@@ -1848,9 +1848,18 @@ struct SystemImpl final : snh_system {
             launch_fill_int(reinterpret_cast<int*>(coop_bar), 3, 0, st);
             const GridBar bar{coop_bar, coop_bar + 1, reinterpret_cast<int*>(coop_bar + 2), 1u << 22};
             hipEvent_t e = tbegin();
-            SNH_HIP(launch_dipole_midpoint_coop<T>(C, nblocks, X, B1, B2, Bh, Bh2, tdt, tl, atol2, max_iters, nsteps,
-                                                   conv_iter, fail_step, coop_partial, bar, st));
+            const hipError_t le = launch_dipole_midpoint_coop<T>(C, nblocks, X, B1, B2, Bh, Bh2, tdt, tl, atol2,
+                                                                 max_iters, nsteps, conv_iter, fail_step,
+                                                                 coop_partial, bar, st);
             tend("midpoint_coop", e);
+            if (le != hipSuccess) {
+                // a grid the device cannot keep resident: launched sweeps
+                // from now on (same layout, same results)
+                (void)hipGetLastError();
+                coop = false;
+                step_midpoint(dt, damping, tol, max_iters, nsteps, iters);
+                return;
+            }
             SNH_HIP(hipGetLastError());
             int h[3];
             SNH_HIP(hipMemcpyAsync(pinned_i, conv_iter, sizeof(int), hipMemcpyDeviceToHost, st));

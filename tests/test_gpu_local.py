@@ -374,3 +374,27 @@ def test_sun_langevin_after_sweeps_sees_updated_kets():
     Z_ref = O.Oracle(s.spec()).step_langevin(Z=Z, noise=noise, dt=integ.dt, damping=integ.damping, kT=integ.kT)
     sh.step_(s, integ, noise=noise)
     assert np.max(np.abs(s.coherents_flat() - Z_ref)) <= 1e-11
+
+
+def test_ewald_sequential_many_bonds():
+    """More bonds per site than the chunk kernel keeps neighbour ids for (56:
+    exchange to the fifth shell of a simple cubic lattice, bond lists padded
+    to a multiple of the gather width) with the Ewald term: sequential trials
+    and their energy change against the oracle."""
+    s = sh.workloads.cubic_model(L=6, J=1.0, D=-0.05, seed=31, precision=64)
+    cryst = s.crystal
+    for d, J in ((2 ** 0.5, 0.3), (3 ** 0.5, -0.2), (2.0, 0.1), (5 ** 0.5, 0.05)):
+        sh.set_exchange_(s, J, sh.workloads.bond_at_distance(cryst, d))
+    sh.enable_dipole_dipole_(s, 0.5, demag=1 / 3)
+    announce("local sequential ewald cubic 6^3, 56 bonds per site")
+    prop = PROPOSALS["mix"]
+    S0 = s.dipoles_flat()
+    E0 = sh.energy(s)
+    key = O.replica_key(s.noise_seed, 0)
+    S_ref, acc = O.Oracle(s.spec()).local_sequential(S0, oracle_props(prop), 0.6, key, s.step_counter, 1.5)
+    sampler = sh.LocalSampler(kT=0.6, nsweeps=1.5, propose=prop)
+    sh.step_(s, sampler)
+    assert np.max(np.abs(s.dipoles_flat() - S_ref)) <= 1e-12
+    scale = energy_scale(s)
+    assert abs(sampler.dE - acc[0]) <= 1e-9 * scale
+    assert abs(sh.energy(s) - E0 - sampler.dE) <= 1e-9 * scale
