@@ -269,8 +269,8 @@ struct McTrial {
     double dmu[3];
 };
 // Sequential trials of the Ewald form run in chunks of at most kMcChunk per
-// launch, one wavefront per replica whose lane m holds the chunk's m-th
-// accepted move.
+// launch, lane l of a replica's first wavefront holding the chunk's l-th
+// trial (:SUN: the l-th accepted move).
 constexpr int kMcChunk = 64;
 constexpr int kMcThreads = 256;  // per workgroup of the :dipole launch (trial workgroups: four wavefronts)
 static_assert(kMcChunk <= 64, "one accepted move per lane of a wavefront");

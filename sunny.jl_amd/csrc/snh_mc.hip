@@ -28,13 +28,14 @@
 // disjoint from the Langevin stream (second counter word 0).
 //
 // With the Ewald term every pair of sites interacts, so no two trials are
-// independent: trials then run one after another per replica, in chunks of
-// up to kMcChunk per launch (k_mc_ewald_chunk), each trial seeing the FP64
-// Ewald potential of the chunk's start plus the moves accepted since.  The
-// same launch's other workgroups add the previous chunk's moves to the
+// independent: trials then keep the reference's order, in chunks of up to
+// kMcChunk per launch (k_mc_ewald_chunk): each trial sees the FP64 Ewald
+// potential of the chunk's start plus the moves accepted since, the chunk's
+// trials evaluated in parallel and decided in order with exact corrections.
+// The same launch's other workgroups add the previous chunk's moves to the
 // potential at every site of the replica (mc_apply_part, O(N) per accepted
 // move as in Ewald.jl:212-241) for the next launch, so that work overlaps
-// the chunk's serial trials.
+// the chunk's trials.
 #include "snh_mc.h"
 #include "snh_vec.h"
 
@@ -437,7 +438,17 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, M
     double accw = 0;              // lane l: sum of w[m][l] over the accepted m < l
     double my_dE = 0;             // lane l: its trial's energy change when decided
     unsigned long long accm = 0;  // accepted trials (wave-uniform)
+    unsigned long long pend = 0;  // accepted trials whose spin is not stored yet (wave-uniform)
     int nm = 0;
+    // the spins of the pending accepted trials, each stored by its own lane
+    // (no trial of this chunk reads the state between an acceptance and
+    // the next trial made again, or the chunk's end)
+    auto flush = [&]() __attribute__((always_inline)) {
+        if (pend) {
+            if ((pend >> lane) & 1ull) mc_store(S, C, sc, my_sn);
+            pend = 0;
+        }
+    };
     for (int t = 0; t < ntrial; t++) {
         const double wt = lane > t ? wsh[t][lane] : 0.0;  // read ahead of the decision
         // did an accepted move change trial t's site or a neighbour?
@@ -446,11 +457,12 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, M
         if (!redo) {
             const double e = my_E + accw;
             accept = lane_value((int)metropolis(e), t);
-            if (lane == t) {
-                my_dE = e;
-                if (accept) mc_store(S, C, sc, my_sn);
-            }
+            if (lane == t) my_dE = e;
+            if (accept) pend |= 1ull << t;
         } else {
+            flush();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
             const McSeqSite d = mc_seq_site_of_lane(my, t);
             const T n[3] = {lane_value(my_n[0], t), lane_value(my_n[1], t), lane_value(my_n[2], t)};
             SiteCoord st = mc_site(R, C.G, d.i, d.c1, d.c2, d.c3, true);
@@ -499,6 +511,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, M
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
     }
+    flush();
     // sums of dE and dS over the accepted trials
     const bool mine = lane < ntrial && ((accm >> lane) & 1ull);
     double sum[4];
