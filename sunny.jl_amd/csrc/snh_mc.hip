@@ -300,9 +300,10 @@ __device__ __forceinline__ double mc_trial_wave(const Couplings<T>& C, const McA
 //     trials at those sites, formed from site ids in LDS): then the
 //     whole wavefront makes trial t again on the current state (mc_trial_wave)
 //     and, when it is accepted, w[t][l] is formed again with its new dmu.
-// An accepted spin is stored with its ghost images by the lane that holds
-// it; the decisions are the first wavefront's alone.  The chunk's accepted
-// moves go to io.moves[r][.] and io.nmoves[r]; acc[4 r + q] += (dE, dS).
+// Accepted spins are stored with their ghost images by the lanes that hold
+// them, together, before a trial made again and at the chunk's end; the
+// decisions are the first wavefront's alone.  The chunk's accepted moves go
+// to io.moves[r][.] and io.nmoves[r]; acc[4 r + q] += (dE, dS).
 template <typename T, int G>
 __global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, McArgs A, T* __restrict__ S,
                                                                McChunkIO io, const double* __restrict__ Aew,
