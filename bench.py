@@ -417,6 +417,27 @@ def bench_local(args):
         log(f"LocalSampler Ewald fp{prec}: {e.nsites} sites, {trials} sequential trials, "
             f"{dt * 1e6 / trials:.3f} us/trial")
         del e, h
+    # a batch of replicas (parallel tempering's shape), FP32
+    Rb = args.ewald_replicas
+    systems = [sh.workloads.pyrochlore_model(L=16, J1=0.05, J2=0.0, A=0.0, s=3.5, seed=40 + r, precision=32,
+                                             dipolar=True, mu0_muB2=0.6745817653324668, demag=1 / 3)
+               for r in range(Rb)]
+    b = sh.ReplicaBatch(systems, precision=32)
+    h = b._device()
+    trials = args.local_ewald_trials // 4
+    sampler = sh.LocalSampler(kT=0.3, nsweeps=trials / systems[0].nsites, propose=sh.propose_delta(0.3))
+    kTs = np.geomspace(0.1, 2.0, Rb)
+    sh.step_(b, sampler, kTs=kTs)
+    h.call("snh_synchronize")
+    t0 = time.perf_counter()
+    sh.step_(b, sampler, kTs=kTs)
+    h.call("snh_synchronize")
+    dt = time.perf_counter() - t0
+    out[f"ewald_fp32_r{Rb}"] = {"sites": Rb * systems[0].nsites, "trials": Rb * trials,
+                                "us_per_trial": dt * 1e6 / (Rb * trials), "value": Rb * trials / dt}
+    log(f"LocalSampler Ewald fp32 batch: {Rb} x {systems[0].nsites} sites, {trials} sequential trials each, "
+        f"{dt * 1e6 / (Rb * trials):.3f} us per trial, {Rb * trials / dt:.3e} trials/s")
+    del b, h, systems
     small = sh.workloads.pyrochlore_model(L=8, seed=1, precision=64)
     m, nc, col = sh.local_coloring(small)
     o = O.Oracle(small.spec())

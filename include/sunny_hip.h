@@ -281,7 +281,8 @@ int snh_corr_data(snh_system *sys, double *data, int64_t *nsamples);
  * 0); dE adds ewald_energy_delta (src/System/Ewald.jl:234-241; :SUN: of the
  * expected dipole) from an FP64 potential that every accepted move updates
  * at all sites (O(nsites) per move, as in the reference), 64 trials per
- * launch (SNH_ERR_UNSUPPORTED for a non-symmetric demagnetization tensor:
+ * launch; a group of 4 or more replicas instead refreshes the potential by
+ * one FFT evaluation after every 64 trials (SNH_ERR_UNSUPPORTED for a non-symmetric demagnetization tensor:
  * the tensor's 3x3 blocks are kept as their upper triangles).  dE_dS (may be NULL): [R][4], the sums of dE and of
  * dS = S_new - S_old over the accepted trials of this call (LocalSampler's
  * fields dE, dS accumulate them). */

@@ -1601,6 +1601,12 @@ struct SystemImpl final : snh_system {
             for (const auto& g : ewg) {  // replicas of one dims at a time
                 const uint32_t ns = (uint32_t)reps[g.r0].ncell * (uint32_t)na;
                 const long long niters = (long long)std::ceil(nsweeps * (double)ns);
+                // several replicas: after each chunk the potential is the FFT
+                // field of the state again (one batched evaluation), cheaper
+                // than adding every accepted move at every site of every replica
+                const bool fft = g.nr >= env_int("SNH_MC_FFT_REPLICAS", kMcFftReplicas);
+                const size_t off = reps[g.r0].off;
+                const size_t len = (g.r0 + g.nr < R ? (size_t)reps[g.r0 + g.nr].off : (size_t)total) - off;
                 int cur = 0;  // mc.phi[cur]: potential before the previous chunk's moves
                 for (long long t = 0, k = 0; t < niters; t += kMcChunk, k++) {
                     const uint64_t T0 = sweep0 * ns + (uint64_t)t;
@@ -1608,17 +1614,33 @@ struct SystemImpl final : snh_system {
                     McChunkIO io;
                     io.phi_in = mc.phi[cur];
                     io.phi_out = mc.phi[cur ^ 1];
-                    io.prev = k ? mc.moves[(k - 1) & 1] : nullptr;
-                    io.nprev = k ? mc.nmoves[(k - 1) & 1] : nullptr;
+                    io.prev = k && !fft ? mc.moves[(k - 1) & 1] : nullptr;
+                    io.nprev = k && !fft ? mc.nmoves[(k - 1) & 1] : nullptr;
                     io.moves = mc.moves[k & 1];
                     io.nmoves = mc.nmoves[k & 1];
                     const uint32_t th = N == 0 ? kMcThreads : 64;  // threads per workgroup of the launch
-                    io.napp = k ? (int)std::min<uint32_t>((ns + th - 1) / th, kMcApplyGroups) : 0;
+                    io.napp = k && !fft ? (int)std::min<uint32_t>((ns + th - 1) / th, kMcApplyGroups) : 0;
                     if (N == 0)
                         launch_mc_ewald_chunk<T>(C, A, X, io, g.Aew, g.A0, T0, nt, mc.acc, g.r0, g.nr, st);
                     else
                         launch_sun_mc_ewald_chunk<T>(C, A, X, io, g.Aew, g.A0, T0, nt, mc.acc, g.r0, g.nr, st);
-                    if (k) cur ^= 1;
+                    if (fft) {
+                        if (t + kMcChunk < niters) {  // the group's potential for the next chunk
+                            const T* Sg = X;
+                            if (N > 0) {
+                                launch_sun_expected_spin<T>(C, nblocks, X, Sd, st);
+                                Sg = Sd;
+                            }
+                            SNH_REQUIRE(launch_ewald_field<T>(g.EG, g.nr, C.reps + g.r0, total, C.gmat, Sg, g.M,
+                                                              g.Fmu, g.Fphi, phi1, g.lib, st),
+                                        SNH_ERR_HIP, "hipFFT execution failed");
+                            for (int q = 0; q < 3; q++)
+                                launch_planes_to_double<T>(phi1 + q * (size_t)total + off,
+                                                           mc.phi[cur] + q * (size_t)total + off, (long long)len, st);
+                        }
+                    } else if (k) {
+                        cur ^= 1;
+                    }
                 }
             }
             tend("local_ewald", e);

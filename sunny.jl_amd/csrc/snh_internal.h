@@ -291,6 +291,9 @@ struct McChunkIO {
     int napp;  // apply workgroups per replica, 0: no previous chunk
 };
 constexpr int kMcApplyGroups = 128;  // napp at most
+// Ewald groups of at least this many replicas refresh the potential of
+// their sequential trials by an FFT evaluation after each chunk instead.
+constexpr int kMcFftReplicas = 4;
 
 // The real-space tensor's nblk 3x3 blocks A9 (nine doubles each) as the six
 // upper-triangle entries xx, xy, xz, yy, yz, zz in A6 (the blocks are

@@ -220,6 +220,28 @@ def test_ewald_sequential_batch_matches_oracle():
         assert abs(sampler.dE[r] - acc[0]) <= 1e-9 * energy_scale(s), r
 
 
+def test_ewald_sequential_batch_fft_refresh():
+    """A batch of at least four replicas refreshes the potential of its
+    sequential trials by one FFT evaluation after each chunk (instead of
+    adding every accepted move at every site): states and energy changes
+    against per-replica oracles, three chunks each."""
+    systems = [sh.workloads.pyrochlore_model(dims=(3, 2, 1), seed=150 + r, precision=64, dipolar=True)
+               for r in range(4)]
+    announce("local sequential ewald replica batch of four, FFT refresh")
+    batch = sh.ReplicaBatch(systems, precision=64)
+    refs = [s.dipoles_flat() for s in systems]
+    kTs = np.array([0.2, 0.5, 1.0, 3.0])
+    prop = sh.mix_proposals(0.5, sh.propose_uniform, 0.5, sh.propose_delta(0.25))
+    sampler = sh.LocalSampler(kT=0.0, nsweeps=2.0, propose=prop)
+    sh.step_(batch, sampler, kTs=kTs)
+    batch.sync_to_systems()
+    for r, s in enumerate(systems):
+        S, acc = O.Oracle(s.spec()).local_sequential(refs[r], oracle_props(prop), kTs[r],
+                                                     O.replica_key(batch.noise_seed, r), 0, 2.0)
+        assert np.max(np.abs(s.dipoles_flat() - S)) <= 1e-12 * 1.5, r
+        assert abs(sampler.dE[r] - acc[0]) <= 1e-9 * energy_scale(s), r
+
+
 def test_ewald_sequential_batch_mixed_dims():
     """Sequential trials with the Ewald term on replicas of different dims
     (each run of equal dims has its own real-space tensor)."""
