@@ -337,7 +337,11 @@ int snh_autotune(snh_system *sys, int32_t candidates, int32_t launches, double *
  * "ewald_fft_lib" (1 when the Ewald grid is transformed by batched hipFFT
  * plans: grids whose transform does not fit one workgroup's LDS, an axis
  * longer than 32 cells, or the environment variable SNH_EWALD_FFT=lib at
- * snh_create; 0 for one workgroup per field), "corr_fft_lib" (1 when the
+ * snh_create; 0 for one workgroup per field), "ewald_slab" (groups of
+ * replicas whose Ewald transforms are split by axis: per-plane 2D transforms
+ * and one workgroup per in-plane wavevector for the third axis and the
+ * contraction; groups of at most 2 replicas, SNH_EWALD_SLAB=0 never, =2 up
+ * to 4), "corr_fft_lib" (1 when the
  * sampled correlations' lattice transform uses a hipFFT plan). */
 int snh_query(snh_system *sys, const char *key, int64_t *value);
 

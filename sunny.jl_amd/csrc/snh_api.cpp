@@ -1019,6 +1019,13 @@ struct SystemImpl final : snh_system {
             const bool fits = L[0] <= kLmax && L[1] <= kLmax && L[2] <= kLmax &&
                               ewald_fft_lds_bytes(EG, sizeof(double)) <= lim;
             EG.plain = !fits || force_lib ? 1 : 0;
+            // few replicas: transforms split by axis (SNH_EWALD_SLAB=0 never,
+            // 2 up to kSlabMaxReplicas replicas)
+            const int slab_mode = env_int("SNH_EWALD_SLAB", 1);
+            EG.slab = !EG.plain && slab_mode != 0 &&
+                      g.nr <= (slab_mode == 2 ? kSlabMaxReplicas : kEwaldSlabReplicas) &&
+                      ewald_slab_plane_lds_bytes(EG, sizeof(T)) <= lim &&
+                      ewald_slab_column_lds_bytes(EG, g.nr, sizeof(T)) <= lim ? 1 : 0;
             const size_t lc = ewald_use_mfma(EG, g.nr, sizeof(T)) ? ewald_mfma_lds_bytes(EG)
                                                                   : (size_t)g.nr * 3 * na * sizeof(T) * 2;
             SNH_REQUIRE(lc <= lim, SNH_ERR_UNSUPPORTED, "too many replicas x sublattices for the Ewald contraction");
@@ -2424,6 +2431,11 @@ struct SystemImpl final : snh_system {
         if (key == "mid_reruns") return mid_reruns;
         if (key == "mid_fallbacks") return mid_fallbacks;
         if (key == "ewald_fft_lib") return ewald && ewald_lib_any() ? 1 : 0;
+        if (key == "ewald_slab") {
+            int n = 0;
+            for (const auto& g : ewg) n += g.EG.slab;
+            return n;  // groups on the slab path
+        }
         if (key == "corr_fft_lib") return corr.lib ? 1 : 0;
         throw Failure{SNH_ERR_INVALID, "unknown query key: " + key};
     }

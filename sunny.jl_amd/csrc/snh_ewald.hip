@@ -373,6 +373,220 @@ __global__ __launch_bounds__(kFftThreads) void k_ewald_inv(EwaldGeom G, int R, c
 }
 
 // ---------------------------------------------------------------------------
+// Slab path for few replicas.  The per-field transforms above run one
+// workgroup per field (3 na of them per replica), which leaves most of the
+// chip idle for one replica.  Here the transform is split by axis:
+//   k_ewald_fwd_slab    one workgroup per (sublattice, plane c3): mu = -g S of
+//                       the plane's cells for the three components, 2D r2c
+//                       along axes 1 and 2 -> X[k12][r][f][c3], k12 = k1 + H1 k2
+//   k_ewald_contract_z  one workgroup per wavevector k = k12 + H1 L2 k3: the
+//                       DFT along axis 3 at k3 of every field's X column,
+//                       then F[phi]_k = M_k F[mu]_k -> Y[r][f][k3][k12]
+//   k_ewald_inv_slab    one workgroup per (sublattice, plane c3): the inverse
+//                       DFT along axis 3 at c3 of Y, 2D c2r, 1/Ncell -> phi
+// The axis-3 transforms are direct sums of L3 terms evaluated where they
+// are needed; X and Y take the place of the blocked spectra (same size).
+
+// X: position of (in-plane wavevector k12, replica r, field f, plane c3),
+// so that the contraction's workgroup at k12 reads one contiguous run.
+__device__ __forceinline__ long long slab_x_index(const EwaldGeom& G, int R, int k12, int r, int f, int c3)
+{
+    return (((long long)k12 * R + r) * (3 * G.na) + f) * G.L3 + c3;
+}
+
+// Y: position of (replica r, field f, wavevector k3, k12), so that the
+// inverse's workgroups read runs of H1 L2 wavevectors.
+__device__ __forceinline__ long long slab_y_index(const EwaldGeom& G, int r, int f, int k3, int k12)
+{
+    return (((long long)r * (3 * G.na) + f) * G.L3 + k3) * ((long long)G.H1 * G.L2) + k12;
+}
+
+// LDS of the plane kernels: three real planes [L2][L1 + 1], three half
+// spectra [H1 L2], three twiddle tables.
+template <typename T>
+struct SlabLds {
+    T* fld;
+    C2<T>* spec;
+    C2<T>* tw1;
+    C2<T>* tw2;
+    C2<T>* tw3;
+    int row, plane, nkp;
+};
+
+template <typename T>
+__device__ __forceinline__ SlabLds<T> slab_lds(const EwaldGeom& G)
+{
+    extern __shared__ __align__(16) unsigned char snh_lds[];
+    SlabLds<T> F;
+    F.row = G.L1 + 1;
+    F.plane = G.L2 * F.row;
+    F.nkp = G.H1 * G.L2;
+    F.spec = reinterpret_cast<C2<T>*>(snh_lds);
+    F.tw1 = F.spec + 3 * F.nkp;
+    F.tw2 = F.tw1 + kLmax;
+    F.tw3 = F.tw2 + kLmax;
+    F.fld = reinterpret_cast<T*>(F.tw3 + kLmax);
+    return F;
+}
+
+size_t ewald_slab_plane_lds_bytes(const EwaldGeom& G, size_t elem)
+{
+    return ((size_t)3 * G.H1 * G.L2 * 2 + 3 * 2 * kLmax + (size_t)3 * G.L2 * (G.L1 + 1)) * elem;
+}
+
+size_t ewald_slab_column_lds_bytes(const EwaldGeom& G, int R, size_t elem)
+{
+    return ((size_t)R * 3 * G.na * (G.L3 + 1) * 2 + 2 * kLmax) * elem;
+}
+
+// grid = (na L3, R): sublattice j = x / L3, plane c3 = x % L3.
+template <typename T>
+__global__ __launch_bounds__(kFftThreads) void k_ewald_fwd_slab(EwaldGeom G, int R, const ReplicaInfo* __restrict__ reps,
+                                                                uint32_t total, const T* __restrict__ gmat,
+                                                                const T* __restrict__ S, C2<T>* __restrict__ X)
+{
+    const int j = blockIdx.x / G.L3, c3 = blockIdx.x % G.L3, r = blockIdx.y;
+    const SlabLds<T> F = slab_lds<T>(G);
+    const uint32_t off = reps[r].off;
+    const int npl = G.L1 * G.L2;  // <= kLmax^2 = 4 kFftThreads
+    T v[4][3];
+    // unconditional loads (a cell past the end reads cell 0 and is dropped)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int c = q * kFftThreads + (int)threadIdx.x;
+        const int cc = c < npl ? c : 0;
+        const uint32_t g = ewald_site(G, off, j, cc % G.L1, cc / G.L1, c3);
+        v[q][0] = ldg(S, g);
+        v[q][1] = ldg(S + total, g);
+        v[q][2] = ldg(S + 2 * (size_t)total, g);
+    }
+    make_twiddles(F.tw1, G.L1, -1);
+    make_twiddles(F.tw2, G.L2, -1);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int c = q * kFftThreads + (int)threadIdx.x;
+        if (c >= npl) continue;
+        const int c1 = c % G.L1, c2 = c / G.L1;
+#pragma unroll
+        for (int beta = 0; beta < 3; beta++) {
+            const T* gr = gmat + 9 * j + 3 * beta;  // mu_beta = -(g S)_beta, as k_ewald_fwd forms it
+            F.fld[beta * F.plane + c2 * F.row + c1] = -(gr[0] * v[q][0] + gr[1] * v[q][1] + gr[2] * v[q][2]);
+        }
+    }
+    __syncthreads();
+    for (int line = threadIdx.x; line < 3 * G.L2; line += blockDim.x) {
+        const int beta = line / G.L2, c2 = line % G.L2;
+        line_r2c<T>(F.fld + beta * F.plane + c2 * F.row, F.spec + beta * F.nkp + G.H1 * c2, G.L1, F.tw1);
+    }
+    __syncthreads();
+    for (int line = threadIdx.x; line < 3 * G.H1; line += blockDim.x) {
+        const int beta = line / G.H1, k1 = line % G.H1;
+        line_c2c<T>(F.spec + beta * F.nkp, k1, G.H1, G.L2, F.tw2);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 3 * F.nkp; t += blockDim.x) {
+        const int beta = t / F.nkp, k12 = t % F.nkp;
+        X[slab_x_index(G, R, k12, r, 3 * j + beta, c3)] = F.spec[t];
+    }
+}
+
+// grid = nk (one workgroup per wavevector), all R replicas.  LDS: the X
+// columns xs[R 3na][L3] at k12, the transformed vector v[R 3na], twiddles.
+template <typename T>
+__global__ __launch_bounds__(256) void k_ewald_contract_z(EwaldGeom G, int R, const C2<T>* __restrict__ M,
+                                                          const C2<T>* __restrict__ X, C2<T>* __restrict__ Y)
+{
+    extern __shared__ __align__(16) unsigned char snh_lds[];
+    const long long k = blockIdx.x;
+    const int nkp = G.H1 * G.L2, L3 = G.L3;
+    const int k12 = (int)(k % nkp), k3 = (int)(k / nkp);
+    const int n3 = 3 * G.na, nv = R * n3;
+    C2<T>* xs = reinterpret_cast<C2<T>*>(snh_lds);
+    C2<T>* v = xs + nv * L3;
+    C2<T>* tw = v + nv;
+    const C2<T>* Xk = X + slab_x_index(G, R, k12, 0, 0, 0);  // [r n3 + f][c3], contiguous
+    for (int t = threadIdx.x; t < nv * L3; t += blockDim.x) xs[t] = Xk[t];
+    make_twiddles(tw, L3, -1);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nv; t += blockDim.x) {
+        T ar = 0, ai = 0;
+        int m = 0;  // (k3 c3) mod L3
+        for (int c3 = 0; c3 < L3; c3++) {
+            const C2<T> x = xs[t * L3 + c3], w = tw[m];
+            ar += x.re * w.re - x.im * w.im;
+            ai += x.re * w.im + x.im * w.re;
+            m += k3;
+            if (m >= L3) m -= L3;
+        }
+        v[t] = {ar, ai};
+    }
+    __syncthreads();
+    const C2<T>* Mk = M + k * n3 * n3;
+    for (int t = threadIdx.x; t < nv; t += blockDim.x) {
+        const int r = t / n3, row = t % n3;
+        const C2<T>* vr = v + r * n3;
+        T ar = 0, ai = 0;
+        for (int col = 0; col < n3; col++) {
+            const C2<T> m = Mk[col * n3 + row];
+            const C2<T> x = vr[col];
+            ar += m.re * x.re - m.im * x.im;
+            ai += m.re * x.im + m.im * x.re;
+        }
+        Y[slab_y_index(G, r, row, k3, k12)] = {ar, ai};
+    }
+}
+
+// grid = (na L3, R): sublattice i = x / L3, plane c3 = x % L3.
+template <typename T>
+__global__ __launch_bounds__(kFftThreads) void k_ewald_inv_slab(EwaldGeom G, const ReplicaInfo* __restrict__ reps,
+                                                                uint32_t total, const C2<T>* __restrict__ Y,
+                                                                T* __restrict__ phi)
+{
+    const int i = blockIdx.x / G.L3, c3 = blockIdx.x % G.L3, r = blockIdx.y;
+    const SlabLds<T> F = slab_lds<T>(G);
+    make_twiddles(F.tw1, G.L1, +1);
+    make_twiddles(F.tw2, G.L2, +1);
+    make_twiddles(F.tw3, G.L3, +1);
+    __syncthreads();
+    // the inverse transform along axis 3 at plane c3
+    for (int t = threadIdx.x; t < 3 * F.nkp; t += blockDim.x) {
+        const int alpha = t / F.nkp, k12 = t % F.nkp;
+        T ar = 0, ai = 0;
+        int m = 0;  // (k3 c3) mod L3
+        const C2<T>* Yc = Y + slab_y_index(G, r, 3 * i + alpha, 0, k12);
+        const long long nkp = F.nkp;
+#pragma unroll 8
+        for (int k3 = 0; k3 < G.L3; k3++) {
+            const C2<T> y = Yc[k3 * nkp], w = F.tw3[m];
+            ar += y.re * w.re - y.im * w.im;
+            ai += y.re * w.im + y.im * w.re;
+            m += c3;
+            if (m >= G.L3) m -= G.L3;
+        }
+        F.spec[t] = {ar, ai};
+    }
+    __syncthreads();
+    for (int line = threadIdx.x; line < 3 * G.H1; line += blockDim.x) {
+        const int alpha = line / G.H1, k1 = line % G.H1;
+        line_c2c<T>(F.spec + alpha * F.nkp, k1, G.H1, G.L2, F.tw2);
+    }
+    __syncthreads();
+    for (int line = threadIdx.x; line < 3 * G.L2; line += blockDim.x) {
+        const int alpha = line / G.L2, c2 = line % G.L2;
+        line_c2r<T>(F.spec + alpha * F.nkp + G.H1 * c2, F.fld + alpha * F.plane + c2 * F.row, G.L1, F.tw1);
+    }
+    __syncthreads();
+    const T scale = (T)(1.0 / (double)G.ncell);
+    const uint32_t off = reps[r].off;
+    const int npl = G.L1 * G.L2;
+    for (int t = threadIdx.x; t < 3 * npl; t += blockDim.x) {
+        const int alpha = t / npl, c = t % npl;
+        const int c1 = c % G.L1, c2 = c / G.L1;
+        stg(phi + (size_t)alpha * total, ewald_site(G, off, i, c1, c2, c3), F.fld[alpha * F.plane + c2 * F.row + c1] * scale);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Library path: elementwise gather and scatter around the hipFFT transforms.
 
 // fld[r][f][cell] = mu = -g S of field f = 3 j + beta.  grid = (cells / 256, 3 na, R).
@@ -727,6 +941,20 @@ bool launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint
         const dim3 grid((unsigned)((G.ncell + 255) / 256), 3 * G.na, R);
         hipLaunchKernelGGL(k_ewald_scatter<T>, grid, dim3(256), 0, st, G, reps, total,
                            reinterpret_cast<const T*>(lib->fld), phi);
+        return true;
+    }
+    if (G.slab && R <= kSlabMaxReplicas) {
+        const size_t lp = ewald_slab_plane_lds_bytes(G, sizeof(T)), lc = ewald_slab_column_lds_bytes(G, R, sizeof(T));
+        allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_fwd_slab<T>), lp);
+        hipLaunchKernelGGL(k_ewald_fwd_slab<T>, dim3(G.na * G.L3, R), dim3(kFftThreads), lp, st, G, R, reps, total,
+                           gmat, S, reinterpret_cast<C2<T>*>(Fmu));
+        allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_contract_z<T>), lc);
+        hipLaunchKernelGGL(k_ewald_contract_z<T>, dim3((unsigned)G.nk), dim3(256), lc, st, G, R,
+                           reinterpret_cast<const C2<T>*>(M), reinterpret_cast<const C2<T>*>(Fmu),
+                           reinterpret_cast<C2<T>*>(Fphi));
+        allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_inv_slab<T>), lp);
+        hipLaunchKernelGGL(k_ewald_inv_slab<T>, dim3(G.na * G.L3, R), dim3(kFftThreads), lp, st, G, reps, total,
+                           reinterpret_cast<const C2<T>*>(Fphi), phi);
         return true;
     }
     const size_t lds = ewald_fft_lds_bytes(G, sizeof(T));

@@ -125,7 +125,8 @@ struct EwaldGeom {
     int P1, P2;              // padded dims of the state layout
     uint32_t pcell;
     int w1, w2, w3;
-    int plain;  // 1: library FFT path, spectra [replica][field][k]
+    int plain;     // 1: library FFT path, spectra [replica][field][k]
+    int slab = 0;  // 1: transforms split by axis (few replicas; in-LDS path only)
 };
 
 // Parameters of the real-space Ewald tensor precompute (FP64).
@@ -368,6 +369,13 @@ size_t ewald_fft_lds_bytes(const EwaldGeom& G, size_t elem);
 constexpr int kEwaldMfmaMinReplicas = 8;
 bool ewald_use_mfma(const EwaldGeom& G, int R, size_t elem);
 size_t ewald_mfma_lds_bytes(const EwaldGeom& G);
+// The slab path (per-plane transforms, per-column contraction) serves groups
+// of at most kEwaldSlabReplicas replicas by default and kSlabMaxReplicas
+// when forced (SNH_EWALD_SLAB=2); its LDS per plane and per column workgroup.
+constexpr int kEwaldSlabReplicas = 2;
+constexpr int kSlabMaxReplicas = 4;
+size_t ewald_slab_plane_lds_bytes(const EwaldGeom& G, size_t elem);
+size_t ewald_slab_column_lds_bytes(const EwaldGeom& G, int R, size_t elem);
 // Bytes of one set of spectra (all fields of R replicas, blocked wavevectors).
 size_t ewald_spectrum_bytes(const EwaldGeom& G, int R, size_t elem);
 void launch_ewald_A(const EwaldAParams& P, const double* pos, double* A, hipStream_t st);
