@@ -1638,12 +1638,18 @@ struct SystemImpl final : snh_system {
                                 launch_sun_expected_spin<T>(C, nblocks, X, Sd, st);
                                 Sg = Sd;
                             }
+                            // in-LDS transforms write the FP64 potential of the group's
+                            // interior sites directly (the trials read no other entry)
+                            const bool direct = !g.EG.plain;
                             SNH_REQUIRE(launch_ewald_field<T>(g.EG, g.nr, C.reps + g.r0, total, C.gmat, Sg, g.M,
-                                                              g.Fmu, g.Fphi, phi1, g.lib, st),
+                                                              g.Fmu, g.Fphi, phi1, g.lib, st,
+                                                              direct ? mc.phi[cur] : nullptr),
                                         SNH_ERR_HIP, "hipFFT execution failed");
-                            for (int q = 0; q < 3; q++)
-                                launch_planes_to_double<T>(phi1 + q * (size_t)total + off,
-                                                           mc.phi[cur] + q * (size_t)total + off, (long long)len, st);
+                            if (!direct)
+                                for (int q = 0; q < 3; q++)
+                                    launch_planes_to_double<T>(phi1 + q * (size_t)total + off,
+                                                               mc.phi[cur] + q * (size_t)total + off, (long long)len,
+                                                               st);
                         }
                     } else if (k) {
                         cur ^= 1;

@@ -331,11 +331,12 @@ __global__ __launch_bounds__(256 * NRT) void k_ewald_contract_mfma(EwaldGeom G, 
 }
 
 // Inverse: c2r 3D FFT of F[phi] scaled by 1/Ncell -> phi plane alpha of
-// sublattice i.  grid = (3 na, R); field = 3 i + alpha.
-template <typename T>
+// sublattice i (stored as O: T, or FP64 for the Metropolis potential).
+// grid = (3 na, R); field = 3 i + alpha.
+template <typename T, typename O>
 __global__ __launch_bounds__(kFftThreads) void k_ewald_inv(EwaldGeom G, int R, const ReplicaInfo* __restrict__ reps,
                                                            uint32_t total, const C2<T>* __restrict__ Fphi,
-                                                           T* __restrict__ phi)
+                                                           O* __restrict__ phi)
 {
     const int f = blockIdx.x, r = blockIdx.y;
     const int i = f / 3, alpha = f % 3;
@@ -364,11 +365,12 @@ __global__ __launch_bounds__(kFftThreads) void k_ewald_inv(EwaldGeom G, int R, c
     __syncthreads();
     inv3d<T>(G, F);
     const T scale = (T)(1.0 / (double)G.ncell);
-    T* const out = phi + (size_t)alpha * total;
+    O* const out = phi + (size_t)alpha * total;
     const uint32_t off = reps[r].off;
     for (int c = threadIdx.x; c < G.ncell; c += blockDim.x) {
         const int c1 = c % G.L1, l = c / G.L1;
-        stg(out, ewald_site(G, off, i, c1, l % G.L2, l / G.L2), F.fld[l * F.row + c1] * scale);
+        const T v = F.fld[l * F.row + c1] * scale;
+        stg(out, ewald_site(G, off, i, c1, l % G.L2, l / G.L2), (O)v);
     }
 }
 
@@ -537,10 +539,10 @@ __global__ __launch_bounds__(256) void k_ewald_contract_z(EwaldGeom G, int R, co
 }
 
 // grid = (na L3, R): sublattice i = x / L3, plane c3 = x % L3.
-template <typename T>
+template <typename T, typename O>
 __global__ __launch_bounds__(kFftThreads) void k_ewald_inv_slab(EwaldGeom G, const ReplicaInfo* __restrict__ reps,
                                                                 uint32_t total, const C2<T>* __restrict__ Y,
-                                                                T* __restrict__ phi)
+                                                                O* __restrict__ phi)
 {
     const int i = blockIdx.x / G.L3, c3 = blockIdx.x % G.L3, r = blockIdx.y;
     const SlabLds<T> F = slab_lds<T>(G);
@@ -582,7 +584,8 @@ __global__ __launch_bounds__(kFftThreads) void k_ewald_inv_slab(EwaldGeom G, con
     for (int t = threadIdx.x; t < 3 * npl; t += blockDim.x) {
         const int alpha = t / npl, c = t % npl;
         const int c1 = c % G.L1, c2 = c / G.L1;
-        stg(phi + (size_t)alpha * total, ewald_site(G, off, i, c1, c2, c3), F.fld[alpha * F.plane + c2 * F.row + c1] * scale);
+        const T v = F.fld[alpha * F.plane + c2 * F.row + c1] * scale;
+        stg(phi + (size_t)alpha * total, ewald_site(G, off, i, c1, c2, c3), (O)v);
     }
 }
 
@@ -932,8 +935,10 @@ static bool ewald_transforms_lib(const EwaldGeom& G, int R, const ReplicaInfo* r
 
 template <typename T>
 bool launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint32_t total, const T* gmat,
-                        const T* S, const void* M, void* Fmu, void* Fphi, T* phi, EwaldLibFft* lib, hipStream_t st)
+                        const T* S, const void* M, void* Fmu, void* Fphi, T* phi, EwaldLibFft* lib, hipStream_t st,
+                        double* phi64)
 {
+    if (phi64 && G.plain) return false;  // the library path writes T only
     if (G.plain) {
         if (!ewald_transforms_lib<T>(G, R, reps, total, gmat, S, lib, Fmu, Fphi, st, true)) return false;
         ewald_contract<T>(G, R, M, Fmu, Fphi, st);
@@ -952,9 +957,16 @@ bool launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint
         hipLaunchKernelGGL(k_ewald_contract_z<T>, dim3((unsigned)G.nk), dim3(256), lc, st, G, R,
                            reinterpret_cast<const C2<T>*>(M), reinterpret_cast<const C2<T>*>(Fmu),
                            reinterpret_cast<C2<T>*>(Fphi));
-        allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_inv_slab<T>), lp);
-        hipLaunchKernelGGL(k_ewald_inv_slab<T>, dim3(G.na * G.L3, R), dim3(kFftThreads), lp, st, G, reps, total,
-                           reinterpret_cast<const C2<T>*>(Fphi), phi);
+        const auto* y = reinterpret_cast<const C2<T>*>(Fphi);
+        if (phi64) {
+            allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_inv_slab<T, double>), lp);
+            hipLaunchKernelGGL((k_ewald_inv_slab<T, double>), dim3(G.na * G.L3, R), dim3(kFftThreads), lp, st, G,
+                               reps, total, y, phi64);
+        } else {
+            allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_inv_slab<T, T>), lp);
+            hipLaunchKernelGGL((k_ewald_inv_slab<T, T>), dim3(G.na * G.L3, R), dim3(kFftThreads), lp, st, G, reps,
+                               total, y, phi);
+        }
         return true;
     }
     const size_t lds = ewald_fft_lds_bytes(G, sizeof(T));
@@ -962,9 +974,16 @@ bool launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint
     hipLaunchKernelGGL(k_ewald_fwd<T>, dim3(3 * G.na, R), dim3(kFftThreads), lds, st, G, R, reps, total, gmat, S,
                        reinterpret_cast<C2<T>*>(Fmu));
     ewald_contract<T>(G, R, M, Fmu, Fphi, st);
-    allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_inv<T>), lds);
-    hipLaunchKernelGGL(k_ewald_inv<T>, dim3(3 * G.na, R), dim3(kFftThreads), lds, st, G, R, reps, total,
-                       reinterpret_cast<const C2<T>*>(Fphi), phi);
+    const auto* y = reinterpret_cast<const C2<T>*>(Fphi);
+    if (phi64) {
+        allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_inv<T, double>), lds);
+        hipLaunchKernelGGL((k_ewald_inv<T, double>), dim3(3 * G.na, R), dim3(kFftThreads), lds, st, G, R, reps,
+                           total, y, phi64);
+    } else {
+        allow_dynamic_lds(reinterpret_cast<const void*>(&k_ewald_inv<T, T>), lds);
+        hipLaunchKernelGGL((k_ewald_inv<T, T>), dim3(3 * G.na, R), dim3(kFftThreads), lds, st, G, R, reps, total, y,
+                           phi);
+    }
     return true;
 }
 
@@ -973,9 +992,10 @@ template void launch_ewald_fwdA<double>(const EwaldGeom&, const double*, void*, 
 template std::string launch_ewald_fwdA_plain<float>(const EwaldGeom&, const double*, void*, hipStream_t);
 template std::string launch_ewald_fwdA_plain<double>(const EwaldGeom&, const double*, void*, hipStream_t);
 template bool launch_ewald_field<float>(const EwaldGeom&, int, const ReplicaInfo*, uint32_t, const float*,
-                                        const float*, const void*, void*, void*, float*, EwaldLibFft*, hipStream_t);
+                                        const float*, const void*, void*, void*, float*, EwaldLibFft*, hipStream_t,
+                                        double*);
 template bool launch_ewald_field<double>(const EwaldGeom&, int, const ReplicaInfo*, uint32_t, const double*,
                                          const double*, const void*, void*, void*, double*, EwaldLibFft*,
-                                         hipStream_t);
+                                         hipStream_t, double*);
 
 }  // namespace snh

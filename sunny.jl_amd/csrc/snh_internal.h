@@ -389,10 +389,13 @@ void ewald_lib_destroy(EwaldLibFft* L);
 // A -> M through the library (returns an error text, empty on success).
 template <typename T>
 std::string launch_ewald_fwdA_plain(const EwaldGeom& G, const double* A, void* M, hipStream_t st);
-// Potential phi of state S; false when a library transform fails.
+// Potential phi of state S; false when a library transform fails.  With
+// phi64 non-null the interior sites' potential goes to phi64 in FP64 instead
+// (same layout; the library path, G.plain, refuses and returns false).
 template <typename T>
 bool launch_ewald_field(const EwaldGeom& G, int R, const ReplicaInfo* reps, uint32_t total, const T* gmat,
-                        const T* S, const void* M, void* Fmu, void* Fphi, T* phi, EwaldLibFft* lib, hipStream_t st);
+                        const T* S, const void* M, void* Fmu, void* Fphi, T* phi, EwaldLibFft* lib, hipStream_t st,
+                        double* phi64 = nullptr);
 
 // :SUN mode (snh_sun.hip).  Buffers are plane sets [2N][total]; expected
 // dipoles are [3][total] planes with ghost images.

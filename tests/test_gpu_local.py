@@ -220,15 +220,20 @@ def test_ewald_sequential_batch_matches_oracle():
         assert abs(sampler.dE[r] - acc[0]) <= 1e-9 * energy_scale(s), r
 
 
-def test_ewald_sequential_batch_fft_refresh():
+@pytest.mark.parametrize("fft", ["workgroup", "lib"])
+def test_ewald_sequential_batch_fft_refresh(fft, monkeypatch):
     """A batch of at least four replicas refreshes the potential of its
     sequential trials by one FFT evaluation after each chunk (instead of
-    adding every accepted move at every site): states and energy changes
-    against per-replica oracles, three chunks each."""
+    adding every accepted move at every site), written in FP64 by the
+    inverse transforms or converted from the library path's output: states
+    and energy changes against per-replica oracles, three chunks each."""
+    if fft == "lib":
+        monkeypatch.setenv("SNH_EWALD_FFT", "lib")
     systems = [sh.workloads.pyrochlore_model(dims=(3, 2, 1), seed=150 + r, precision=64, dipolar=True)
                for r in range(4)]
-    announce("local sequential ewald replica batch of four, FFT refresh")
+    announce(f"local sequential ewald replica batch of four, FFT refresh ({fft})")
     batch = sh.ReplicaBatch(systems, precision=64)
+    assert batch._device().query("ewald_fft_lib") == (fft == "lib")
     refs = [s.dipoles_flat() for s in systems]
     kTs = np.array([0.2, 0.5, 1.0, 3.0])
     prop = sh.mix_proposals(0.5, sh.propose_uniform, 0.5, sh.propose_delta(0.25))
