@@ -450,59 +450,67 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, M
             pend = 0;
         }
     };
-    for (int t = 0; t < ntrial; t++) {
-        const double wt = lane > t ? wsh[t][lane] : 0.0;  // read ahead of the decision
-        // did an accepted move change trial t's site or a neighbour?
-        const bool redo = (lane_value(conf, t) & accm) != 0;
-        int accept;
-        if (!redo) {
+    int t = 0;
+    while (t < ntrial) {
+        // a run of trials whose site and neighbours no accepted move of this
+        // chunk changed: trial t's energy change is E_t + accw, decided by
+        // lane t; the pair energies of the next trial are read one ahead
+        double wn = lane > t ? wsh[t][lane] : 0.0;
+        for (; t < ntrial; t++) {
+            if (lane_value(conf, t) & accm) break;  // made again below
+            const double wt = wn;
+            wn = (t + 1 < ntrial && lane > t + 1) ? wsh[t + 1][lane] : 0.0;
             const double e = my_E + accw;
-            accept = lane_value((int)metropolis(e), t);
+            const int accept = lane_value((int)metropolis(e), t);
             if (lane == t) my_dE = e;
-            if (accept) pend |= 1ull << t;
-        } else {
-            flush();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            const McSeqSite d = mc_seq_site_of_lane(my, t);
-            const T n[3] = {lane_value(my_n[0], t), lane_value(my_n[1], t), lane_value(my_n[2], t)};
-            SiteCoord st = mc_site(R, C.G, d.i, d.c1, d.c2, d.c3, true);
-            st.r = r;
-            // potential at the site: p0 + the accepted moves of this chunk
-            double c3v[3] = {0, 0, 0};
-            if (lane < t && ((accm >> lane) & 1ull))
-                ablock_apply(mc_ablock(Aew, na, R, my.i, my.c1, my.c2, my.c3, d.i, d.c1, d.c2, d.c3), my_dmu, c3v);
-            double p[3];
-#pragma unroll
-            for (int x = 0; x < 3; x++) p[x] = lane_value(p0[x], t) + wave_sum(c3v[x]);
-            V3<T> sn;
-            double dmu[3], dS[3];
-            const double dE = mc_trial_wave<T>(C, A, S, st, d, n, p, A0, wide, sn, dS, dmu);
-            accept = __builtin_amdgcn_readfirstlane(kT == 0.0 ? dE <= 0.0 : d.u_acc <= exp(-dE / kT));
-            if (lane == t) {
-                my_dE = dE;
-                my_dS[0] = dS[0];
-                my_dS[1] = dS[1];
-                my_dS[2] = dS[2];
-            }
             if (accept) {
-                if (lane == 0) mc_store(S, C, st, sn);
-                if (lane == t) {
-                    my_dmu[0] = dmu[0];
-                    my_dmu[1] = dmu[1];
-                    my_dmu[2] = dmu[2];
-                }
-                // the pair energies of this move with the later trials, with its new dmu
-                if (lane > t && lane < ntrial) {
-                    wsh[t][lane] = mc_pair_energy(mc_ablock(Aew, na, R, d.i, d.c1, d.c2, d.c3, my.i, my.c1, my.c2,
-                                                            my.c3),
-                                                  my_dmu, dmu);
-                    accw += wsh[t][lane];
-                }
+                accw += wt;
+                accm |= 1ull << t;
+                pend |= 1ull << t;
+                nm++;
             }
         }
+        if (t == ntrial) break;
+        // trial t: an accepted move changed its site or a neighbour, so the
+        // whole wavefront makes it again on the current state
+        flush();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const McSeqSite d = mc_seq_site_of_lane(my, t);
+        const T n[3] = {lane_value(my_n[0], t), lane_value(my_n[1], t), lane_value(my_n[2], t)};
+        SiteCoord st = mc_site(R, C.G, d.i, d.c1, d.c2, d.c3, true);
+        st.r = r;
+        // potential at the site: p0 + the accepted moves of this chunk
+        double c3v[3] = {0, 0, 0};
+        if (lane < t && ((accm >> lane) & 1ull))
+            ablock_apply(mc_ablock(Aew, na, R, my.i, my.c1, my.c2, my.c3, d.i, d.c1, d.c2, d.c3), my_dmu, c3v);
+        double p[3];
+#pragma unroll
+        for (int x = 0; x < 3; x++) p[x] = lane_value(p0[x], t) + wave_sum(c3v[x]);
+        V3<T> sn;
+        double dmu[3], dS[3];
+        const double dE = mc_trial_wave<T>(C, A, S, st, d, n, p, A0, wide, sn, dS, dmu);
+        const int accept = __builtin_amdgcn_readfirstlane(kT == 0.0 ? dE <= 0.0 : d.u_acc <= exp(-dE / kT));
+        if (lane == t) {
+            my_dE = dE;
+            my_dS[0] = dS[0];
+            my_dS[1] = dS[1];
+            my_dS[2] = dS[2];
+        }
         if (accept) {
-            if (!redo) accw += wt;
+            if (lane == 0) mc_store(S, C, st, sn);
+            if (lane == t) {
+                my_dmu[0] = dmu[0];
+                my_dmu[1] = dmu[1];
+                my_dmu[2] = dmu[2];
+            }
+            // the pair energies of this move with the later trials, with its new dmu
+            if (lane > t && lane < ntrial) {
+                wsh[t][lane] = mc_pair_energy(mc_ablock(Aew, na, R, d.i, d.c1, d.c2, d.c3, my.i, my.c1, my.c2,
+                                                        my.c3),
+                                              my_dmu, dmu);
+                accw += wsh[t][lane];
+            }
             accm |= 1ull << t;
             nm++;
         }
@@ -511,6 +519,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, M
         // program order, so no wait for the stores is needed
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
+        t++;
     }
     flush();
     // sums of dE and dS over the accepted trials
