@@ -388,7 +388,10 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, M
             ABlock blk[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const McTrial t = rec[min(m0 + q * NW, ntrial - 1)];
+                // a pair this lane does not use (m >= lane) loads the lane's own
+                // offset-0 block instead: na such blocks, shared through the cache
+                const int m = m0 + q * NW;
+                const McTrial t = m < lane ? rec[min(m, ntrial - 1)] : me;
                 blk[q] = mc_ablock(Aew, na, R, t.i, t.c1, t.c2, t.c3, me.i, me.c1, me.c2, me.c3);
             }
 #pragma unroll
