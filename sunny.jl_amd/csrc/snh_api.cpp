@@ -1608,8 +1608,8 @@ struct SystemImpl final : snh_system {
             for (const auto& g : ewg) {  // replicas of one dims at a time
                 const uint32_t ns = (uint32_t)reps[g.r0].ncell * (uint32_t)na;
                 const long long niters = (long long)std::ceil(nsweeps * (double)ns);
-                // several replicas: after each chunk the potential is the FFT
-                // field of the state again (one batched evaluation), cheaper
+                // several replicas: after every second chunk the potential is the
+                // FFT field of the state again (one batched evaluation), cheaper
                 // than adding every accepted move at every site of every replica
                 const bool fft = g.nr >= env_int("SNH_MC_FFT_REPLICAS", kMcFftReplicas);
                 const size_t off = reps[g.r0].off;
@@ -1621,8 +1621,11 @@ struct SystemImpl final : snh_system {
                     McChunkIO io;
                     io.phi_in = mc.phi[cur];
                     io.phi_out = mc.phi[cur ^ 1];
-                    io.prev = k && !fft ? mc.moves[(k - 1) & 1] : nullptr;
-                    io.nprev = k && !fft ? mc.nmoves[(k - 1) & 1] : nullptr;
+                    // FFT groups: odd chunks see the even chunk's moves on top of
+                    // the refreshed potential, at their trial sites only
+                    const bool with_prev = k && (!fft || (k & 1));
+                    io.prev = with_prev ? mc.moves[(k - 1) & 1] : nullptr;
+                    io.nprev = with_prev ? mc.nmoves[(k - 1) & 1] : nullptr;
                     io.moves = mc.moves[k & 1];
                     io.nmoves = mc.nmoves[k & 1];
                     const uint32_t th = N == 0 ? kMcThreads : 64;  // threads per workgroup of the launch
@@ -1632,7 +1635,7 @@ struct SystemImpl final : snh_system {
                     else
                         launch_sun_mc_ewald_chunk<T>(C, A, X, io, g.Aew, g.A0, T0, nt, mc.acc, g.r0, g.nr, st);
                     if (fft) {
-                        if (t + kMcChunk < niters) {  // the group's potential for the next chunk
+                        if ((k & 1) && t + kMcChunk < niters) {  // the potential for the next two chunks
                             const T* Sg = X;
                             if (N > 0) {
                                 launch_sun_expected_spin<T>(C, nblocks, X, Sd, st);

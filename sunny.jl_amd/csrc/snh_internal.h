@@ -285,15 +285,17 @@ static_assert(kMcChunk <= 64, "one accepted move per lane of a wavefront");
 struct McChunkIO {
     const double* phi_in;
     double* phi_out;
-    const McTrial* prev;
+    const McTrial* prev;  // the previous chunk's moves, added at the trial sites (null: none)
     const int* nprev;
     McTrial* moves;
     int* nmoves;
-    int napp;  // apply workgroups per replica, 0: no previous chunk
+    int napp;  // apply workgroups per replica (phi_out = phi_in + prev moves), 0: none
 };
 constexpr int kMcApplyGroups = 128;  // napp at most
 // Ewald groups of at least this many replicas refresh the potential of
-// their sequential trials by an FFT evaluation after each chunk instead.
+// their sequential trials by an FFT evaluation after every second chunk
+// instead (the chunk between two refreshes adds the previous chunk's moves
+// at its trial sites only).
 constexpr int kMcFftReplicas = 4;
 
 // The real-space tensor's nblk 3x3 blocks A9 (nine doubles each) as the six

@@ -322,7 +322,7 @@ __device__ __forceinline__ void mc_pot_start(McChunkPot& P, const McChunkIO& io,
                                              int c1, int c2, int c3)
 {
     double s[3] = {0, 0, 0};
-    if (io.napp) mc_moves_sum(Aew, na, R, io.prev + (size_t)r * kMcChunk, io.nprev[r], i, c1, c2, c3, s);
+    if (io.prev) mc_moves_sum(Aew, na, R, io.prev + (size_t)r * kMcChunk, io.nprev[r], i, c1, c2, c3, s);
 #pragma unroll
     for (int q = 0; q < 3; q++) P.pot[q] = io.phi_in[g + q * (size_t)total] + s[q];
     P.pending = false;

@@ -367,7 +367,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, M
         const McTrial me = rec[lane];
         // the previous chunk's moves m = wv, wv + NW, ... at this lane's trial site
         double s3[3] = {0, 0, 0};
-        if (io.napp) {
+        if (io.prev) {
             const McTrial* pv = io.prev + (size_t)r * kMcChunk;
             const int np = io.nprev[r];
 #pragma unroll 4

@@ -247,6 +247,29 @@ def test_ewald_sequential_batch_fft_refresh(fft, monkeypatch):
         assert abs(sampler.dE[r] - acc[0]) <= 1e-9 * energy_scale(s), r
 
 
+def test_sun_ewald_sequential_batch_fft_refresh():
+    """:SUN with Ewald in a batch of four replicas (k_sun_mc_ewald_chunk with
+    the FFT refresh after every second chunk, the chunk between adding the
+    previous chunk's moves at its trial sites): kets and energy changes
+    against per-replica oracles over three chunks each."""
+    from test_oracle_local import sun_dipolar_system
+    systems = [sun_dipolar_system(seed=160 + r, dims=(4, 4, 3)) for r in range(4)]
+    announce("local sequential ewald SU(3) replica batch of four, FFT refresh")
+    refs = [s.coherents_flat() for s in systems]
+    batch = sh.ReplicaBatch(systems, precision=64)
+    kTs = np.array([0.2, 0.5, 1.0, 2.0])
+    prop = PROPOSALS["mix"]
+    sampler = sh.LocalSampler(kT=0.0, nsweeps=3.0, propose=prop)
+    sh.step_(batch, sampler, kTs=kTs)
+    batch.sync_to_systems()
+    for r, s in enumerate(systems):
+        kap = s.spec()["kappa"][0]
+        Z_ref, acc = O.Oracle(s.spec()).local_sequential_sun(refs[r], oracle_props(prop), kTs[r],
+                                                             O.replica_key(batch.noise_seed, r), 0, 3.0)
+        assert np.max(np.abs(s.coherents_flat() - Z_ref)) <= TOL[64] * np.sqrt(kap) * 4, r
+        assert abs(sampler.dE[r] - acc[0]) <= 10 * TOL[64] * energy_scale(s), r
+
+
 def test_ewald_sequential_batch_mixed_dims():
     """Sequential trials with the Ewald term on replicas of different dims
     (each run of equal dims has its own real-space tensor)."""
