@@ -514,8 +514,11 @@ struct SystemImpl final : snh_system {
         double* out = nullptr;        // [R][4]
         // sequential trials with the Ewald term (tensors per Ewald group)
         double* phi[2] = {nullptr, nullptr};    // FP64 potential planes [3][total], alternating
-        McTrial* moves[2] = {nullptr, nullptr};  // [R][kMcChunk] accepted moves of a chunk, alternating
-        int* nmoves[2] = {nullptr, nullptr};     // [R]
+        // accepted moves of a chunk, [kMcFftPeriod][R][kMcChunk], and their
+        // counts [kMcFftPeriod][R]: slots 0 and 1 alternate for the apply
+        // path, FFT groups fill slot k mod kMcFftPeriod between refreshes
+        McTrial* moves = nullptr;
+        int* nmoves = nullptr;
     } mc;
     // Ewald: one group per run of consecutive replicas with equal dims, each
     // with its grid, transformed tensor, spectra and (library FFT path) plans
@@ -1543,11 +1546,9 @@ struct SystemImpl final : snh_system {
                 }
                 SNH_HIP(hipFree(A9));
             }
-            for (int b = 0; b < 2; b++) {
-                mc.phi[b] = own(dalloc<double>(3 * (size_t)total));
-                mc.moves[b] = own(dalloc<McTrial>((size_t)R * kMcChunk));
-                mc.nmoves[b] = own(dalloc<int>(R));
-            }
+            for (int b = 0; b < 2; b++) mc.phi[b] = own(dalloc<double>(3 * (size_t)total));
+            mc.moves = own(dalloc<McTrial>((size_t)kMcFftPeriod * R * kMcChunk));
+            mc.nmoves = own(dalloc<int>((size_t)kMcFftPeriod * R));
         }
         mc.ready = true;
     }
@@ -1608,9 +1609,10 @@ struct SystemImpl final : snh_system {
             for (const auto& g : ewg) {  // replicas of one dims at a time
                 const uint32_t ns = (uint32_t)reps[g.r0].ncell * (uint32_t)na;
                 const long long niters = (long long)std::ceil(nsweeps * (double)ns);
-                // several replicas: after every second chunk the potential is the
-                // FFT field of the state again (one batched evaluation), cheaper
-                // than adding every accepted move at every site of every replica
+                // several replicas: after every kMcFftPeriod chunks the potential
+                // is the FFT field of the state again (one batched evaluation),
+                // cheaper than adding every accepted move at every site of every
+                // replica
                 const bool fft = g.nr >= env_int("SNH_MC_FFT_REPLICAS", kMcFftReplicas);
                 const size_t off = reps[g.r0].off;
                 const size_t len = (g.r0 + g.nr < R ? (size_t)reps[g.r0 + g.nr].off : (size_t)total) - off;
@@ -1621,13 +1623,18 @@ struct SystemImpl final : snh_system {
                     McChunkIO io;
                     io.phi_in = mc.phi[cur];
                     io.phi_out = mc.phi[cur ^ 1];
-                    // FFT groups: odd chunks see the even chunk's moves on top of
-                    // the refreshed potential, at their trial sites only
-                    const bool with_prev = k && (!fft || (k & 1));
-                    io.prev = with_prev ? mc.moves[(k - 1) & 1] : nullptr;
-                    io.nprev = with_prev ? mc.nmoves[(k - 1) & 1] : nullptr;
-                    io.moves = mc.moves[k & 1];
-                    io.nmoves = mc.nmoves[k & 1];
+                    // FFT groups: chunk slot j = k mod kMcFftPeriod sees the moves of
+                    // slots 0 .. j-1 on top of the refreshed potential, at its
+                    // trial sites only; else the previous chunk's (slot k-1 mod 2)
+                    const int slot = fft ? (int)(k % kMcFftPeriod) : (int)(k & 1);
+                    const int lists = fft ? slot : (k ? 1 : 0);
+                    const int first = fft ? 0 : slot ^ 1;
+                    io.prev = lists ? mc.moves + (size_t)first * R * kMcChunk : nullptr;
+                    io.nprev = lists ? mc.nmoves + (size_t)first * R : nullptr;
+                    io.nprevl = lists;
+                    io.nrep_all = R;
+                    io.moves = mc.moves + (size_t)slot * R * kMcChunk;
+                    io.nmoves = mc.nmoves + (size_t)slot * R;
                     const uint32_t th = N == 0 ? kMcThreads : 64;  // threads per workgroup of the launch
                     io.napp = k && !fft ? (int)std::min<uint32_t>((ns + th - 1) / th, kMcApplyGroups) : 0;
                     if (N == 0)
@@ -1635,7 +1642,7 @@ struct SystemImpl final : snh_system {
                     else
                         launch_sun_mc_ewald_chunk<T>(C, A, X, io, g.Aew, g.A0, T0, nt, mc.acc, g.r0, g.nr, st);
                     if (fft) {
-                        if ((k & 1) && t + kMcChunk < niters) {  // the potential for the next two chunks
+                        if (slot == kMcFftPeriod - 1 && t + kMcChunk < niters) {  // for the next chunks
                             const T* Sg = X;
                             if (N > 0) {
                                 launch_sun_expected_spin<T>(C, nblocks, X, Sd, st);

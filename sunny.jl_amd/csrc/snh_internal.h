@@ -285,18 +285,24 @@ static_assert(kMcChunk <= 64, "one accepted move per lane of a wavefront");
 struct McChunkIO {
     const double* phi_in;
     double* phi_out;
-    const McTrial* prev;  // the previous chunk's moves, added at the trial sites (null: none)
+    // moves of nprevl earlier chunks, added at the trial sites (null: none):
+    // list s of replica r at prev[(s R + r) kMcChunk], its count nprev[s R + r]
+    // (R = nrep_all, the handle's replicas)
+    const McTrial* prev;
     const int* nprev;
+    int nprevl;
+    int nrep_all;
     McTrial* moves;
     int* nmoves;
-    int napp;  // apply workgroups per replica (phi_out = phi_in + prev moves), 0: none
+    int napp;  // apply workgroups per replica (phi_out = phi_in + the one prev list), 0: none
 };
 constexpr int kMcApplyGroups = 128;  // napp at most
 // Ewald groups of at least this many replicas refresh the potential of
-// their sequential trials by an FFT evaluation after every second chunk
-// instead (the chunk between two refreshes adds the previous chunk's moves
-// at its trial sites only).
+// their sequential trials by an FFT evaluation after every kMcFftPeriod
+// chunks instead (the chunks between add the moves since the refresh at
+// their trial sites only).
 constexpr int kMcFftReplicas = 4;
+constexpr int kMcFftPeriod = 4;
 
 // The real-space tensor's nblk 3x3 blocks A9 (nine doubles each) as the six
 // upper-triangle entries xx, xy, xz, yy, yz, zz in A6 (the blocks are

@@ -321,10 +321,16 @@ __device__ __forceinline__ void mc_pot_start(McChunkPot& P, const McChunkIO& io,
                                              int na, const ReplicaInfo& R, int r, uint32_t g, uint32_t total, int i,
                                              int c1, int c2, int c3)
 {
-    double s[3] = {0, 0, 0};
-    if (io.prev) mc_moves_sum(Aew, na, R, io.prev + (size_t)r * kMcChunk, io.nprev[r], i, c1, c2, c3, s);
+    P.pot[0] = P.pot[1] = P.pot[2] = 0;
+    for (int l = 0; io.prev && l < io.nprevl; l++) {
+        const size_t lr = (size_t)l * io.nrep_all + r;
+        double s[3];
+        mc_moves_sum(Aew, na, R, io.prev + lr * kMcChunk, io.nprev[lr], i, c1, c2, c3, s);
 #pragma unroll
-    for (int q = 0; q < 3; q++) P.pot[q] = io.phi_in[g + q * (size_t)total] + s[q];
+        for (int q = 0; q < 3; q++) P.pot[q] += s[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) P.pot[q] = io.phi_in[g + q * (size_t)total] + P.pot[q];
     P.pending = false;
 }
 

@@ -365,11 +365,12 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, M
     __syncthreads();
     {
         const McTrial me = rec[lane];
-        // the previous chunk's moves m = wv, wv + NW, ... at this lane's trial site
+        // the earlier chunks' moves m = wv, wv + NW, ... at this lane's trial site
         double s3[3] = {0, 0, 0};
-        if (io.prev) {
-            const McTrial* pv = io.prev + (size_t)r * kMcChunk;
-            const int np = io.nprev[r];
+        for (int l = 0; io.prev && l < io.nprevl; l++) {
+            const size_t lr = (size_t)l * io.nrep_all + r;
+            const McTrial* pv = io.prev + lr * kMcChunk;
+            const int np = io.nprev[lr];
 #pragma unroll 4
             for (int m = wv; m < np; m += NW) {
                 const McTrial t = pv[m];
