@@ -289,10 +289,11 @@ __device__ __forceinline__ double mc_trial_wave(const Couplings<T>& C, const McA
 //     (mc_seq_draw; normals from counter (T lo, kMcTagSequential + 1, T hi,
 //     0)) and evaluates it on the chunk's starting state: E_l =
 //     local_energy_change + ewald_energy_delta at the potential p0_l of the
-//     chunk's start (Interactions.jl:311-380, Ewald.jl:234-241); all
-//     wavefronts share the tensor-block loads of p0 (the previous chunk's
-//     moves at the trial sites) and of the pair energies w[m][l] = dmu_l .
-//     A(c_m - c_l) dmu_m with every earlier trial m, kept in LDS;
+//     chunk's start (Interactions.jl:311-380, Ewald.jl:234-241), while the
+//     other wavefronts load the tensor blocks of p0 (the earlier chunks'
+//     moves at the trial sites); then all wavefronts share the loads of the
+//     pair energies w[m][l] = dmu_l . A(c_m - c_l) dmu_m with every earlier
+//     trial m, kept in LDS;
 //  2. the trials are then decided in order: trial t's energy change is
 //     E_t + sum of w[m][t] over the moves m accepted before it (the exact
 //     change of ewald_energy_delta, linear in the potential), unless an
@@ -324,13 +325,15 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, M
     const bool wide = A.wide != 0;
     const int lane = (int)threadIdx.x % 64, wv = (int)threadIdx.x / 64;
     constexpr int NW = kMcThreads / 64;
+    static_assert(NW >= 2, "waves 1.. sum the earlier moves while wave 0 makes its trials");
     __shared__ int nid[kMcChunk][kMcNbr + 1];  // [l][0]: trial l's site; [l][1 + k]: across bond k of it
     __shared__ double wsh[kMcChunk][kMcChunk];  // w[m][l], m < l
     __shared__ McTrial rec[kMcChunk];           // trial l's site and dmu
-    __shared__ double ppart[NW][kMcChunk][3];   // previous chunk's moves at trial l's site, per wave
+    __shared__ double ppart[NW][kMcChunk][3];   // earlier chunks' moves at trial l's site, per wave (0: none)
 
     // ---- 1. every lane of wave 0 evaluates its own trial on the chunk's
-    // starting state; all waves then share the loads of the tensor blocks
+    // starting state while the other waves sum the earlier chunks' moves at
+    // the trial sites; all waves then share the loads of the pair blocks
     McSeqSite my{};
     T my_n[3] = {0, 0, 0};
     SiteCoord sc{};
@@ -361,21 +364,23 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, M
                           (double)gm[3 * a + 2] * my_dS[2]);
         rec[lane] = {1, my.i, my.c1, my.c2, my.c3, 0, {my_dmu[0], my_dmu[1], my_dmu[2]}};
         my_E = mc_local_delta<T, G>(C, A, S, sc, s0, my_sn, wide);
-    }
-    __syncthreads();
-    {
-        const McTrial me = rec[lane];
-        // the earlier chunks' moves m = wv, wv + NW, ... at this lane's trial site
+        ppart[0][lane][0] = ppart[0][lane][1] = ppart[0][lane][2] = 0.0;
+    } else {
+        // meanwhile the other waves sum the earlier chunks' moves m = wv - 1,
+        // wv - 1 + (NW - 1), ... at this lane's trial site (formed from the
+        // same draw as wave 0 forms it)
+        const McSeqSite ms =
+            mc_seq_site(R, mc_seq_draw(key, T0 + (unsigned long long)lane, (uint32_t)R.ncell * (uint32_t)na), A);
         double s3[3] = {0, 0, 0};
         for (int l = 0; io.prev && l < io.nprevl; l++) {
             const size_t lr = (size_t)l * io.nrep_all + r;
             const McTrial* pv = io.prev + lr * kMcChunk;
             const int np = io.nprev[lr];
 #pragma unroll 4
-            for (int m = wv; m < np; m += NW) {
+            for (int m = wv - 1; m < np; m += NW - 1) {
                 const McTrial t = pv[m];
                 double o[3];
-                ablock_apply(mc_ablock(Aew, na, R, t.i, t.c1, t.c2, t.c3, me.i, me.c1, me.c2, me.c3), t.dmu, o);
+                ablock_apply(mc_ablock(Aew, na, R, t.i, t.c1, t.c2, t.c3, ms.i, ms.c1, ms.c2, ms.c3), t.dmu, o);
                 s3[0] += o[0];
                 s3[1] += o[1];
                 s3[2] += o[2];
@@ -384,6 +389,10 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, M
         ppart[wv][lane][0] = s3[0];
         ppart[wv][lane][1] = s3[1];
         ppart[wv][lane][2] = s3[2];
+    }
+    __syncthreads();
+    {
+        const McTrial me = rec[lane];
         // pair energies w[m][l] with the earlier trials m = wv, wv + NW, ...
         for (int m0 = wv; m0 < ntrial; m0 += 4 * NW) {
             ABlock blk[4];
