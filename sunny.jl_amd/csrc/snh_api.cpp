@@ -519,6 +519,7 @@ struct SystemImpl final : snh_system {
         // path, FFT groups fill slot k mod kMcFftPeriod between refreshes
         McTrial* moves = nullptr;
         int* nmoves = nullptr;
+        double* pprev = nullptr;  // [R][kMcChunk][3] earlier moves at the next chunk's trial sites
     } mc;
     // Ewald: one group per run of consecutive replicas with equal dims, each
     // with its grid, transformed tensor, spectra and (library FFT path) plans
@@ -1549,6 +1550,7 @@ struct SystemImpl final : snh_system {
             for (int b = 0; b < 2; b++) mc.phi[b] = own(dalloc<double>(3 * (size_t)total));
             mc.moves = own(dalloc<McTrial>((size_t)kMcFftPeriod * R * kMcChunk));
             mc.nmoves = own(dalloc<int>((size_t)kMcFftPeriod * R));
+            mc.pprev = own(dalloc<double>((size_t)R * kMcChunk * 3));
         }
         mc.ready = true;
     }
@@ -1635,6 +1637,13 @@ struct SystemImpl final : snh_system {
                     io.nrep_all = R;
                     io.moves = mc.moves + (size_t)slot * R * kMcChunk;
                     io.nmoves = mc.nmoves + (size_t)slot * R;
+                    io.pprev = nullptr;
+                    if (fft && lists && N == 0) {
+                        // the earlier moves at the chunk's trial sites, summed over
+                        // the whole device rather than by the trial workgroups
+                        launch_mc_prev_at_trials(A, io, C.reps, g.Aew, na, T0, g.r0, g.nr, mc.pprev, st);
+                        io.pprev = mc.pprev;
+                    }
                     const uint32_t th = N == 0 ? kMcThreads : 64;  // threads per workgroup of the launch
                     io.napp = k && !fft ? (int)std::min<uint32_t>((ns + th - 1) / th, kMcApplyGroups) : 0;
                     if (N == 0)

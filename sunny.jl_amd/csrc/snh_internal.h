@@ -292,6 +292,9 @@ struct McChunkIO {
     const int* nprev;
     int nprevl;
     int nrep_all;
+    // non-null: the sums of those moves at the trial sites, formed by
+    // launch_mc_prev_at_trials ([R][kMcChunk][3], FP64), used instead
+    const double* pprev;
     McTrial* moves;
     int* nmoves;
     int napp;  // apply workgroups per replica (phi_out = phi_in + the one prev list), 0: none
@@ -303,6 +306,10 @@ constexpr int kMcApplyGroups = 128;  // napp at most
 // their trial sites only).
 constexpr int kMcFftReplicas = 4;
 constexpr int kMcFftPeriod = 4;
+// out[(r kMcChunk + l) 3 + x]: the moves of io's nprevl lists at the site of
+// trial T0 + l of replica r (r0 <= r < r0 + nr), one wavefront per trial.
+void launch_mc_prev_at_trials(const McArgs& A, const McChunkIO& io, const ReplicaInfo* reps, const double* Aew,
+                              int na, unsigned long long T0, int r0, int nr, double* out, hipStream_t st);
 
 // The real-space tensor's nblk 3x3 blocks A9 (nine doubles each) as the six
 // upper-triangle entries xx, xy, xz, yy, yz, zz in A6 (the blocks are

@@ -372,7 +372,13 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, M
         const McSeqSite ms =
             mc_seq_site(R, mc_seq_draw(key, T0 + (unsigned long long)lane, (uint32_t)R.ncell * (uint32_t)na), A);
         double s3[3] = {0, 0, 0};
-        for (int l = 0; io.prev && l < io.nprevl; l++) {
+        if (io.pprev && wv == 1) {
+            const double* q = io.pprev + ((size_t)r * kMcChunk + lane) * 3;
+            s3[0] = q[0];
+            s3[1] = q[1];
+            s3[2] = q[2];
+        }
+        for (int l = 0; io.prev && !io.pprev && l < io.nprevl; l++) {
             const size_t lr = (size_t)l * io.nrep_all + r;
             const McTrial* pv = io.prev + lr * kMcChunk;
             const int np = io.nprev[lr];
@@ -548,6 +554,54 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_ewald_chunk(Couplings<T> C, M
         io.nmoves[r] = nm;
         for (int q = 0; q < 4; q++) acc[4 * r + q] += sum[q];
     }
+}
+
+// The moves of io's nprevl lists at the site of each trial of the next
+// chunk, spread over the whole device: one wavefront per (replica, trial),
+// lane j summing moves j, j + 64, ... of each list, then a wave sum.  The
+// trial site comes from the same draw as in k_mc_ewald_chunk.
+__global__ __launch_bounds__(256) void k_mc_prev_at_trials(McArgs A, McChunkIO io,
+                                                           const ReplicaInfo* __restrict__ reps,
+                                                           const double* __restrict__ Aew, int na,
+                                                           unsigned long long T0, int r0, double* __restrict__ out)
+{
+    const int lane = (int)threadIdx.x % 64;
+    const int w = (int)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);  // r' kMcChunk + l
+    const int r = r0 + w / kMcChunk, l = w % kMcChunk;
+    const ReplicaInfo R = reps[r];
+    const unsigned long long key = A.seed + (unsigned long long)r * 0x9E3779B97F4A7C15ull;
+    const McSeqSite ms =
+        mc_seq_site(R, mc_seq_draw(key, T0 + (unsigned long long)l, (uint32_t)R.ncell * (uint32_t)na), A);
+    double s3[3] = {0, 0, 0};
+    for (int q = 0; q < io.nprevl; q++) {
+        const size_t lr = (size_t)q * io.nrep_all + r;
+        const McTrial* pv = io.prev + lr * kMcChunk;
+        const int np = io.nprev[lr];
+        for (int m = lane; m < np; m += 64) {
+            const McTrial t = pv[m];
+            double o[3];
+            ablock_apply(mc_ablock(Aew, na, R, t.i, t.c1, t.c2, t.c3, ms.i, ms.c1, ms.c2, ms.c3), t.dmu, o);
+            s3[0] += o[0];
+            s3[1] += o[1];
+            s3[2] += o[2];
+        }
+    }
+#pragma unroll
+    for (int x = 0; x < 3; x++) s3[x] = wave_sum(s3[x]);
+    if (lane == 0) {
+        double* o = out + ((size_t)r * kMcChunk + l) * 3;
+        o[0] = s3[0];
+        o[1] = s3[1];
+        o[2] = s3[2];
+    }
+}
+
+void launch_mc_prev_at_trials(const McArgs& A, const McChunkIO& io, const ReplicaInfo* reps, const double* Aew,
+                              int na, unsigned long long T0, int r0, int nr, double* out, hipStream_t st)
+{
+    // kMcChunk wavefronts per replica, four per workgroup
+    hipLaunchKernelGGL(k_mc_prev_at_trials, dim3((unsigned)(nr * kMcChunk / 4)), dim3(256), 0, st, A, io, reps, Aew,
+                       na, T0, r0, out);
 }
 
 // The real-space tensor's 3x3 blocks (row-major, nine doubles) as the six
