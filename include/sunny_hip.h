@@ -282,7 +282,7 @@ int snh_corr_data(snh_system *sys, double *data, int64_t *nsamples);
  * expected dipole) from an FP64 potential that every accepted move updates
  * at all sites (O(nsites) per move, as in the reference), 64 trials per
  * launch; a group of 4 or more replicas instead refreshes the potential by
- * one FFT evaluation after every 256 trials, each launch in between adding
+ * one FFT evaluation after every 512 trials, each launch in between adding
  * the moves since the refresh at its own 64 sites (SNH_ERR_UNSUPPORTED for a non-symmetric demagnetization tensor:
  * the tensor's 3x3 blocks are kept as their upper triangles).  dE_dS (may be NULL): [R][4], the sums of dE and of
  * dS = S_new - S_old over the accepted trials of this call (LocalSampler's

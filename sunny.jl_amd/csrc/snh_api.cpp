@@ -1638,7 +1638,7 @@ struct SystemImpl final : snh_system {
                     io.moves = mc.moves + (size_t)slot * R * kMcChunk;
                     io.nmoves = mc.nmoves + (size_t)slot * R;
                     io.pprev = nullptr;
-                    if (fft && lists && N == 0) {
+                    if (fft && lists) {
                         // the earlier moves at the chunk's trial sites, summed over
                         // the whole device rather than by the trial workgroups
                         launch_mc_prev_at_trials(A, io, C.reps, g.Aew, na, T0, g.r0, g.nr, mc.pprev, st);

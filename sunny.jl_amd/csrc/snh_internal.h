@@ -305,7 +305,7 @@ constexpr int kMcApplyGroups = 128;  // napp at most
 // chunks instead (the chunks between add the moves since the refresh at
 // their trial sites only).
 constexpr int kMcFftReplicas = 4;
-constexpr int kMcFftPeriod = 4;
+constexpr int kMcFftPeriod = 8;
 // out[(r kMcChunk + l) 3 + x]: the moves of io's nprevl lists at the site of
 // trial T0 + l of replica r (r0 <= r < r0 + nr), one wavefront per trial.
 void launch_mc_prev_at_trials(const McArgs& A, const McChunkIO& io, const ReplicaInfo* reps, const double* Aew,

@@ -319,10 +319,16 @@ struct McChunkPot {
 
 __device__ __forceinline__ void mc_pot_start(McChunkPot& P, const McChunkIO& io, const double* __restrict__ Aew,
                                              int na, const ReplicaInfo& R, int r, uint32_t g, uint32_t total, int i,
-                                             int c1, int c2, int c3)
+                                             int c1, int c2, int c3, int trial)
 {
     P.pot[0] = P.pot[1] = P.pot[2] = 0;
-    for (int l = 0; io.prev && l < io.nprevl; l++) {
+    if (io.pprev) {  // summed beforehand: trial `trial` of the chunk sits at (i, c1, c2, c3)
+        const double* q = io.pprev + ((size_t)r * kMcChunk + trial) * 3;
+        P.pot[0] = q[0];
+        P.pot[1] = q[1];
+        P.pot[2] = q[2];
+    }
+    for (int l = 0; io.prev && !io.pprev && l < io.nprevl; l++) {
         const size_t lr = (size_t)l * io.nrep_all + r;
         double s[3];
         mc_moves_sum(Aew, na, R, io.prev + lr * kMcChunk, io.nprev[lr], i, c1, c2, c3, s);

@@ -894,7 +894,7 @@ __global__ __launch_bounds__(64) void k_sun_mc_ewald_chunk(Couplings<T> C, McArg
         const uint32_t cn[4] = {(uint32_t)Tl, kMcTagSequential + 1u, (uint32_t)(Tl >> 32), 0u};
         sun_mc_normals<T, N>(cn, key, my_n);
         mc_pot_start(P, io, Aew, C.na, R, r, mc_site(R, C.G, my.i, my.c1, my.c2, my.c3, true).g, total, my.i,
-                     my.c1, my.c2, my.c3);
+                     my.c1, my.c2, my.c3, (int)threadIdx.x);
     }
     McTrial mv{};
     int nm = 0;

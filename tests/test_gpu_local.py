@@ -223,12 +223,12 @@ def test_ewald_sequential_batch_matches_oracle():
 @pytest.mark.parametrize("fft", ["workgroup", "lib"])
 def test_ewald_sequential_batch_fft_refresh(fft, monkeypatch):
     """A batch of at least four replicas refreshes the potential of its
-    sequential trials by one FFT evaluation after every fourth chunk
+    sequential trials by one FFT evaluation after every eighth chunk
     (instead of adding every accepted move at every site; the chunks between
     add the moves since the refresh at their trial sites), written in FP64
     by the inverse transforms or converted from the library path's output:
-    states and energy changes against per-replica oracles over six chunks
-    each (a refresh, then two more chunks)."""
+    states and energy changes against per-replica oracles over eleven chunks
+    each (a refresh, then three more chunks)."""
     if fft == "lib":
         monkeypatch.setenv("SNH_EWALD_FFT", "lib")
     systems = [sh.workloads.pyrochlore_model(dims=(3, 2, 1), seed=150 + r, precision=64, dipolar=True)
@@ -239,21 +239,21 @@ def test_ewald_sequential_batch_fft_refresh(fft, monkeypatch):
     refs = [s.dipoles_flat() for s in systems]
     kTs = np.array([0.2, 0.5, 1.0, 3.0])
     prop = sh.mix_proposals(0.5, sh.propose_uniform, 0.5, sh.propose_delta(0.25))
-    sampler = sh.LocalSampler(kT=0.0, nsweeps=4.0, propose=prop)
+    sampler = sh.LocalSampler(kT=0.0, nsweeps=7.0, propose=prop)
     sh.step_(batch, sampler, kTs=kTs)
     batch.sync_to_systems()
     for r, s in enumerate(systems):
         S, acc = O.Oracle(s.spec()).local_sequential(refs[r], oracle_props(prop), kTs[r],
-                                                     O.replica_key(batch.noise_seed, r), 0, 4.0)
+                                                     O.replica_key(batch.noise_seed, r), 0, 7.0)
         assert np.max(np.abs(s.dipoles_flat() - S)) <= 1e-12 * 1.5, r
         assert abs(sampler.dE[r] - acc[0]) <= 1e-9 * energy_scale(s), r
 
 
 def test_sun_ewald_sequential_batch_fft_refresh():
     """:SUN with Ewald in a batch of four replicas (k_sun_mc_ewald_chunk with
-    the FFT refresh after every fourth chunk, the chunks between adding the
+    the FFT refresh after every eighth chunk, the chunks between adding the
     moves since the refresh at their trial sites): kets and energy changes
-    against per-replica oracles over five chunks each."""
+    against per-replica oracles over ten chunks each."""
     from test_oracle_local import sun_dipolar_system
     systems = [sun_dipolar_system(seed=160 + r, dims=(4, 4, 3)) for r in range(4)]
     announce("local sequential ewald SU(3) replica batch of four, FFT refresh")
@@ -261,13 +261,13 @@ def test_sun_ewald_sequential_batch_fft_refresh():
     batch = sh.ReplicaBatch(systems, precision=64)
     kTs = np.array([0.2, 0.5, 1.0, 2.0])
     prop = PROPOSALS["mix"]
-    sampler = sh.LocalSampler(kT=0.0, nsweeps=6.0, propose=prop)
+    sampler = sh.LocalSampler(kT=0.0, nsweeps=13.0, propose=prop)
     sh.step_(batch, sampler, kTs=kTs)
     batch.sync_to_systems()
     for r, s in enumerate(systems):
         kap = s.spec()["kappa"][0]
         Z_ref, acc = O.Oracle(s.spec()).local_sequential_sun(refs[r], oracle_props(prop), kTs[r],
-                                                             O.replica_key(batch.noise_seed, r), 0, 6.0)
+                                                             O.replica_key(batch.noise_seed, r), 0, 13.0)
         assert np.max(np.abs(s.coherents_flat() - Z_ref)) <= TOL[64] * np.sqrt(kap) * 4, r
         assert abs(sampler.dE[r] - acc[0]) <= 10 * TOL[64] * energy_scale(s), r
 
